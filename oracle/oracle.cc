@@ -1,0 +1,502 @@
+// CPU oracle: clean-room FP64 restatement of the reference OSCARS SR CPU path.
+// TEST INFRASTRUCTURE ONLY (checker and timed CPU baseline); the product never
+// links it.  Compile with the reference's flags (oracle/Makefile) so that the
+// arithmetic is bit-identical to the reference engine.
+//
+// Pinning (tests/test_oracle_golden.py, tests/test_oracle_closed_form.py):
+//   constants ............ closed form (CODATA values of TOSCARSSR.h:20-69)
+//   oracle_bfield ........ closed form sine + 0.25/0.75 terminations
+//   oracle_trajectory .... bitwise vs reference digests (C0 20001 pts, EPU 40001 pts)
+//                          + uniform-field circle radius
+//   oracle_energy_grid ... bitwise vs reference spectra energies (shifted grid)
+//   oracle_rect_surface .. bitwise vs reference container coordinates
+//   oracle_spectrum ...... bitwise vs reference (C0 full 2000 E, small, off-axis list)
+//                          + first-harmonic resonance formula
+//   oracle_flux .......... bitwise vs reference (C1 full 201x201, routes, point list)
+//                          + mirror symmetry
+//   oracle_power_density . bitwise vs reference (C2 full 301x301, directional tilt)
+//                          + integral vs total power
+//   oracle_total_power ... bitwise vs reference (C0, C2)
+//   random particles ..... bitwise vs reference (seeded mt19937/normal stream)
+
+#include "oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// Physical constants, TOSCARSSR.h:20-69 / TSRS.h:20-69 (dated CODATA values).
+const double kPi = 3.14159265358979323846;
+const double kTwoPi = 2.0 * kPi;
+const double kFourPi = 4.0 * kPi;
+const double kPi2 = kPi * kPi;
+const double kSqrt2Pi = 2.5066282746310002416123552393401;
+const double kC = 2.99792458e8;
+const double kH = 6.62606876e-34;
+const double kHbar = 1.054571596e-34;
+const double kQe = 1.602176462e-19;
+const double kMe = 9.10938356e-31;
+const double kEps0 = 8.854187817E-12;
+const double kMu0 = 1.2566370614E-6;
+
+inline double ev_to_omega(double e) { return e * kQe / kHbar; }             // TOSCARSSR.h:66
+inline double kg_to_gev(double m) { return 1e-9 * m * kC * kC / kQe; }       // TOSCARSSR.h:68
+
+// 3-vector helpers in the reference's operation order (TVector3D.cc:98-207).
+struct V3 {
+  double x, y, z;
+};
+inline V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double mag2(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 divs(V3 a, double d) { return V3{a.x / d, a.y / d, a.z / d}; }
+inline V3 muls(V3 a, double s) { return V3{a.x * s, a.y * s, a.z * s}; }
+inline V3 unit(V3 a) { return divs(a, std::sqrt(mag2(a))); }
+inline V3 cross(V3 a, V3 b) { return V3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+// TVector3D::Orthogonal, TVector3D.h:120-131
+inline V3 orthogonal(V3 a) {
+  double xx = a.x < 0.0 ? -a.x : a.x;
+  double yy = a.y < 0.0 ? -a.y : a.y;
+  double zz = a.z < 0.0 ? -a.z : a.z;
+  if (xx < yy) return xx < zz ? V3{0, a.z, -a.y} : V3{a.y, -a.x, 0};
+  return yy < zz ? V3{-a.z, 0, a.x} : V3{a.y, -a.x, 0};
+}
+inline void rot_x(V3& v, double a) {  // TVector3D::RotateSelfX
+  double s = std::sin(a), c = std::cos(a), yy = v.y;
+  v.y = c * yy - s * v.z;
+  v.z = s * yy + c * v.z;
+}
+inline void rot_y(V3& v, double a) {  // TVector3D::RotateSelfY
+  double s = std::sin(a), c = std::cos(a), zz = v.z;
+  v.z = c * zz - s * v.x;
+  v.x = s * zz + c * v.x;
+}
+inline void rot_z(V3& v, double a) {  // TVector3D::RotateSelfZ
+  double s = std::sin(a), c = std::cos(a), xx = v.x;
+  v.x = c * xx - s * v.y;
+  v.y = s * xx + c * v.y;
+}
+inline void rot_xyz(V3& v, V3 r) {
+  rot_x(v, r.x);
+  rot_y(v, r.y);
+  rot_z(v, r.z);
+}
+inline V3 v3(const double* p) { return V3{p[0], p[1], p[2]}; }
+
+// Ideal undulator field, TField3D_IdealUndulator.cc:50-77 (Init) and :106-139 (GetF).
+// With pole_scales != NULL the 2N interior half-period poles are scaled by
+// (1 + delta_p), p = floor((D - (-L/2 + lambda)) / (lambda/2)) (phase and taper 0).
+V3 undulator_field(const oracle_undulator& u, const double* pole_scales, V3 X) {
+  V3 F = v3(u.field), P = v3(u.period), C = v3(u.center);
+  double const lam = std::sqrt(mag2(P));
+  V3 const pu = divs(P, std::sqrt(mag2(P)));
+  double const len = std::sqrt(mag2(P)) * (u.nperiods + 2);
+  if (pole_scales) {
+    double const D = dot(X, pu);
+    double const s = std::sin(2.0 * kPi * D / lam);
+    if (D > len / 2. || D < -len / 2.) return V3{0, 0, 0};
+    if (D < -len / 2. + lam || D > len / 2. - lam) {
+      if (D < -len / 2. + lam / 2. || D > len / 2. - lam / 2.) return V3{(0.25 * F.x) * s, (0.25 * F.y) * s, (0.25 * F.z) * s};
+      return V3{(0.75 * F.x) * s, (0.75 * F.y) * s, (0.75 * F.z) * s};
+    }
+    int p = (int)std::floor((D - (-len / 2. + lam)) / (lam / 2.));
+    if (p < 0) p = 0;
+    if (p > 2 * u.nperiods - 1) p = 2 * u.nperiods - 1;
+    double const sc = 1.0 + pole_scales[p];
+    return V3{(F.x * s) * sc, (F.y * s) * sc, (F.z * s) * sc};
+  }
+  double const D = dot(sub(X, C), pu);
+  double const T = 1 + D * u.taper;
+  double const PS = u.phase * std::sqrt(mag2(P)) / kTwoPi;
+  if (D > len / 2. + PS || D < -len / 2. + PS) return V3{0, 0, 0};
+  double const s = std::sin(kTwoPi * (D - PS) / lam);
+  if (D < -len / 2. + PS + lam || D > len / 2. + PS - lam) {
+    if (D < -len / 2. + PS + lam / 2. || D > len / 2. + PS - lam / 2.)
+      return V3{((F.x * 0.25) * s) * T, ((F.y * 0.25) * s) * T, ((F.z * 0.25) * s) * T};
+    return V3{((F.x * 0.75) * s) * T, ((F.y * 0.75) * s) * T, ((F.z * 0.75) * s) * T};
+  }
+  return V3{(F.x * s) * T, (F.y * s) * T, (F.z * s) * T};
+}
+
+// TFieldContainer::GetF, TFieldContainer.cc:107-117: sum from (0,0,0).
+V3 field_sum(const oracle_fields* f, V3 X) {
+  V3 S{0, 0, 0};
+  for (int i = 0; i < f->nund; ++i) {
+    V3 b = undulator_field(f->und[i], 0, X);
+    S.x += b.x;
+    S.y += b.y;
+    S.z += b.z;
+  }
+  if (f->pole_und) {
+    V3 b = undulator_field(*f->pole_und, f->pole_scales, X);
+    S.x += b.x;
+    S.y += b.y;
+    S.z += b.z;
+  }
+  return S;
+}
+
+// OSCARSSR::DerivativesB, OSCARSSR.cc:741-765.
+inline void derivs(const oracle_fields* f, double qmg, const double* x, double* dxdt) {
+  V3 B = field_sum(f, V3{x[0], x[2], x[4]});
+  dxdt[0] = x[1];
+  dxdt[1] = qmg * (-x[5] * B.y + x[3] * B.z);
+  dxdt[2] = x[3];
+  dxdt[3] = qmg * (x[5] * B.x - x[1] * B.z);
+  dxdt[4] = x[5];
+  dxdt[5] = qmg * (x[1] * B.y - x[3] * B.x);
+}
+
+// OSCARSSR::RK4, OSCARSSR.cc:826-874 (in place: y == yout).
+inline void rk4(const oracle_fields* f, double qmg, double* y, const double* dydx, double h) {
+  double dym[6], dyt[6], yt[6];
+  double const hh = h * 0.5;
+  double const h6 = h / 6.0;
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
+  derivs(f, qmg, yt, dyt);
+  for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
+  derivs(f, qmg, yt, dym);
+  for (int i = 0; i < 6; ++i) {
+    yt[i] = y[i] + h * dym[i];
+    dym[i] += dyt[i];
+  }
+  derivs(f, qmg, yt, dyt);
+  for (int i = 0; i < 6; ++i) y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
+}
+
+// Spectrum/flux integrand for one (observation point, omega), OSCARSSR.cc:1202-1246
+// (spectrum), :2054-2092 (flux).  Returns C2 * |C1 dt SumE|^2 * weight.
+double flux_point(const double* traj, int64_t n, double dt, V3 obs, double omega, double q, double current, double weight) {
+  const double *X = traj, *Y = traj + n, *Z = traj + 2 * n, *BX = traj + 3 * n, *BY = traj + 4 * n, *BZ = traj + 5 * n;
+  double const C0 = q / (kFourPi * kC * kEps0 * kSqrt2Pi);
+  double const C2 = kFourPi * current / (kH * std::fabs(q) * kMu0 * kC) * 1e-6 * 0.001;
+  std::complex<double> const I(0, 1);
+  std::complex<double> const One(1, 0);
+  std::complex<double> const ICoverOmega = I * kC / omega;
+  std::complex<double> const C1(0, C0 * omega);
+  std::complex<double> sx(0, 0), sy(0, 0), sz(0, 0);
+  for (int iT = 0; iT != n; ++iT) {
+    V3 const R = sub(obs, V3{X[iT], Y[iT], Z[iT]});
+    V3 const N = unit(R);
+    double const D = std::sqrt(mag2(R));
+    std::complex<double> const e = std::exp(std::complex<double>(0, omega * (dt * iT + D / kC)));
+    std::complex<double> const w = One + (ICoverOmega / D);
+    sx += (std::complex<double>(BX[iT], 0) - std::complex<double>(N.x, 0) * w) / D * e;
+    sy += (std::complex<double>(BY[iT], 0) - std::complex<double>(N.y, 0) * w) / D * e;
+    sz += (std::complex<double>(BZ[iT], 0) - std::complex<double>(N.z, 0) * w) / D * e;
+  }
+  std::complex<double> const k = C1 * dt;
+  sx *= k;
+  sy *= k;
+  sz *= k;
+  std::complex<double> const d = sx * std::conj(sx) + sy * std::conj(sy) + sz * std::conj(sz);
+  return C2 * d.real() * weight;
+}
+
+// Power-density integrand for one point, OSCARSSR.cc:1522-1573 (and :1726-1771).
+double pd_point(const double* traj, int64_t n, double dt, V3 obs, V3 nrm, bool directional, double q, double current) {
+  const double *X = traj, *Y = traj + n, *Z = traj + 2 * n, *BX = traj + 3 * n, *BY = traj + 4 * n, *BZ = traj + 5 * n;
+  const double *AX = traj + 6 * n, *AY = traj + 7 * n, *AZ = traj + 8 * n;
+  double Sum = 0;
+  for (int iT = 0; iT != n; ++iT) {
+    V3 const Xp{X[iT], Y[iT], Z[iT]};
+    V3 const B{BX[iT], BY[iT], BZ[iT]};
+    V3 const A{AX[iT], AY[iT], AZ[iT]};
+    V3 const N1 = unit(sub(obs, Xp));
+    V3 const N2 = unit(orthogonal(N1));
+    V3 const N3 = unit(cross(N1, N2));
+    double const N1DotNormal = dot(N1, nrm);
+    if (directional && N1DotNormal <= 0) continue;
+    V3 const Num = cross(N1, cross(sub(N1, B), A));
+    double const Den = std::pow(1 - dot(B, N1), 5);
+    Sum += std::pow(dot(Num, N2), 2) / Den / mag2(sub(obs, Xp)) * N1DotNormal;
+    Sum += std::pow(dot(Num, N3), 2) / Den / mag2(sub(obs, Xp)) * N1DotNormal;
+  }
+  Sum *= std::fabs(q * current) / (16 * kPi2 * kEps0 * kC) * dt;
+  Sum /= 1e6;
+  if (!directional) {
+    if (Sum < 0) Sum *= -1;
+  }
+  return Sum;
+}
+
+// The reference threading model: one std::thread per output point, NThreads
+// concurrent, joined in order (OSCARSSR.cc:1109-1132, :1823-1846, :2533-2556).
+template <class F>
+void per_point_threads(int64_t npts, int nthreads, F fn) {
+  if (nthreads <= 1) {
+    for (int64_t i = 0; i < npts; ++i) fn(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t const first = npts > nthreads ? nthreads : npts;
+  for (int64_t i = 0; i < first; ++i) th.push_back(std::thread(fn, i));
+  for (int64_t i = first; i != npts + first; ++i) {
+    int64_t const it = i % first;
+    th[it].join();
+    if (i < npts) th[it] = std::thread(fn, i);
+  }
+}
+
+// Process-global generator, TRandomA.cc:12-64.
+std::mt19937* gMT = new std::mt19937(5489u);
+std::normal_distribution<double> gNormal(0, 1);
+std::uniform_real_distribution<double> gUniform(0, 1);
+
+void finish_particle(oracle_particle* p) {
+  // TParticleA::SetGamma / SetQoverMGamma, TParticleA.cc:353-382
+  double const b2 = mag2(v3(p->beta0));
+  if (b2 == 1) return;
+  p->gamma = 1.0 / std::sqrt(1.0 - b2);
+  if (p->m == 0 || p->gamma == 0) return;
+  p->qmg = p->q / p->m / p->gamma;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_ideal_electron(double energy_gev, const double x0[3], const double d0[3], double current, oracle_particle* out) {
+  // TParticleBeam ctor (.cc:58-69), SetParticleType electron (TParticleA.cc:71-73),
+  // SetCurrent (:254-266), GetNewParticle("ideal") (TParticleBeam.cc:200-222)
+  std::memset(out, 0, sizeof(*out));
+  out->q = -kQe;
+  out->m = kMe;
+  out->current = current != 0 ? current : out->q;
+  V3 const u0 = unit(v3(d0));
+  double const gamma = energy_gev / kg_to_gev(out->m);
+  double const beta = std::sqrt(1.0 - 1.0 / (gamma * gamma));
+  V3 const b0 = muls(u0, beta);
+  out->x0[0] = x0[0];
+  out->x0[1] = x0[1];
+  out->x0[2] = x0[2];
+  out->beta0[0] = b0.x;
+  out->beta0[1] = b0.y;
+  out->beta0[2] = b0.z;
+  out->t0 = 0;
+  finish_particle(out);
+}
+
+void oracle_rng_seed(int seed, int reset_normal) {
+  // TRandomA::SetSeed recreates the engine but keeps the normal distribution's cached
+  // value (TRandomA.cc:42-48); reset_normal clears it as a fresh process would have it.
+  delete gMT;
+  gMT = new std::mt19937(seed);
+  if (reset_normal) gNormal.reset();
+}
+double oracle_rng_normal(void) { return gNormal(*gMT); }
+double oracle_rng_uniform(void) { return gUniform(*gMT); }
+
+void oracle_random_electron(double energy_gev, double sigma_e_gev, const double x0[3], const double d0[3],
+                            const double hdir[3], const double beta[2], const double emit[2],
+                            const double lattice_ref[3], double current, oracle_particle* out) {
+  // Sigmas from emittance and beta (OSCARSSR_Python.cc:1822-1829), directions from
+  // TParticleBeam::SetSigma (TParticleBeam.cc:137-158), draw order E, x, y, x', y'
+  // (TParticleBeam.cc:228-265).
+  std::memset(out, 0, sizeof(*out));
+  out->q = -kQe;
+  out->m = kMe;
+  out->current = current != 0 ? current : out->q;
+  V3 const u0 = unit(v3(d0));
+  V3 const h = unit(unit(v3(hdir)));
+  V3 const v = unit(cross(u0, h));
+  double const su0 = std::sqrt(emit[0] * beta[0]), su1 = std::sqrt(emit[1] * beta[1]);
+  double const sp0 = std::sqrt(emit[0] / beta[0]), sp1 = std::sqrt(emit[1] / beta[1]);
+  double const enew = energy_gev + sigma_e_gev * gNormal(*gMT);
+  double const gamma = enew / kg_to_gev(out->m);
+  double const b = std::sqrt(1.0 - 1.0 / (gamma * gamma));
+  double const dmid = dot(sub(v3(lattice_ref), v3(x0)), u0);
+  V3 X = v3(x0);
+  double const n1 = gNormal(*gMT);
+  V3 const dx = muls(muls(muls(h, su0), 1 + dmid), n1);
+  X.x += dx.x;
+  X.y += dx.y;
+  X.z += dx.z;
+  double const n2 = gNormal(*gMT);
+  V3 const dy = muls(muls(muls(v, su1), 1 + dmid), n2);
+  X.x += dy.x;
+  X.y += dy.y;
+  X.z += dy.z;
+  V3 B = muls(u0, b);
+  rot_y(B, sp0 * gNormal(*gMT));
+  rot_x(B, -sp1 * gNormal(*gMT));
+  out->x0[0] = X.x;
+  out->x0[1] = X.y;
+  out->x0[2] = X.z;
+  out->beta0[0] = B.x;
+  out->beta0[1] = B.y;
+  out->beta0[2] = B.z;
+  finish_particle(out);
+}
+
+int64_t oracle_trajectory_npoints(double ct_start, double ct_stop, int64_t npoints, double t0) {
+  double const dt = ((ct_stop - ct_start) / kC / (size_t)(npoints - 1));
+  size_t const nf = 1 + (ct_stop - t0) / kC / dt;
+  size_t const nb = (t0 - ct_start) / kC / dt;
+  return (int64_t)(nf + nb);
+}
+
+void oracle_bfield(const oracle_fields* f, const double x[3], double out[3]) {
+  V3 b = field_sum(f, v3(x));
+  out[0] = b.x;
+  out[1] = b.y;
+  out[2] = b.z;
+}
+
+int64_t oracle_trajectory(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                          int64_t npoints, double* traj, double* dt_out) {
+  // OSCARSSR::CalculateTrajectory(TParticleA&), OSCARSSR.cc:562-673.  The forward
+  // pass stores the state *before* each step with the derivative of the previous
+  // step (A/c lags one point); the first point's A/c, uninitialised in the reference,
+  // is stored as 0.
+  double const dt = ((ct_stop - ct_start) / kC / (size_t)(npoints - 1));
+  size_t const nf = 1 + (ct_stop - p->t0) / kC / dt;
+  size_t const nb = (p->t0 - ct_start) / kC / dt;
+  int64_t const n = (int64_t)(nf + nb);
+  *dt_out = dt;
+  double x[6], dxdt[6] = {0, 0, 0, 0, 0, 0};
+  x[0] = p->x0[0];
+  x[1] = p->beta0[0] * kC;
+  x[2] = p->x0[1];
+  x[3] = p->beta0[1] * kC;
+  x[4] = p->x0[2];
+  x[5] = p->beta0[2] * kC;
+  auto put = [&](int64_t k) {
+    traj[0 * n + k] = x[0];
+    traj[1 * n + k] = x[2];
+    traj[2 * n + k] = x[4];
+    traj[3 * n + k] = x[1] / kC;
+    traj[4 * n + k] = x[3] / kC;
+    traj[5 * n + k] = x[5] / kC;
+    traj[6 * n + k] = dxdt[1] / kC;
+    traj[7 * n + k] = dxdt[3] / kC;
+    traj[8 * n + k] = dxdt[5] / kC;
+  };
+  for (size_t i = 0; i != nf; ++i) {
+    put((int64_t)(nb + i));
+    derivs(f, p->qmg, x, dxdt);
+    rk4(f, p->qmg, x, dxdt, dt);
+  }
+  x[0] = p->x0[0];
+  x[1] = p->beta0[0] * kC;
+  x[2] = p->x0[1];
+  x[3] = p->beta0[1] * kC;
+  x[4] = p->x0[2];
+  x[5] = p->beta0[2] * kC;
+  double const mdt = -dt;
+  for (size_t i = 0; i != nb; ++i) {
+    derivs(f, p->qmg, x, dxdt);
+    rk4(f, p->qmg, x, dxdt, mdt);
+    put((int64_t)(nb - 1 - i));
+  }
+  return n;
+}
+
+void oracle_energy_grid(int64_t n, double e0, double e1, double* out) {
+  // TSpectrumContainer::Init(N, EFirst, ELast), TSpectrumContainer.cc:55-86
+  if (n == 1) {
+    out[0] = e0;
+    return;
+  }
+  for (int64_t i = 0; i < n; ++i) out[i] = e0 + (e1 - e0) / (size_t)(n - 1) * (double)(i + 1);
+}
+
+int oracle_rect_surface(const char* plane, int nx1, int nx2, double w1, double w2, const double rot[3],
+                        const double trans[3], int normal, double* xyz, double* nrm, double* x1x2) {
+  // TSurfacePoints_Rectangle::Init(plane,...) :130-210, GetXYZ :222-230, GetX1/X2 :244-259
+  std::string P(plane);
+  for (auto& ch : P) ch = (char)toupper(ch);
+  double const s1 = w1 / (nx1 - 1), s2 = w2 / (nx2 - 1);
+  V3 st, a1, a2;
+  if (P == "XY") {
+    st = V3{-w1 / 2., -w2 / 2., 0}; a1 = V3{s1, 0, 0}; a2 = V3{0, s2, 0};
+  } else if (P == "YX") {
+    st = V3{-w2 / 2., -w1 / 2., 0}; a2 = V3{s2, 0, 0}; a1 = V3{0, s1, 0};
+  } else if (P == "XZ") {
+    st = V3{-w1 / 2., 0, -w2 / 2.}; a1 = V3{s1, 0, 0}; a2 = V3{0, 0, s2};
+  } else if (P == "ZX") {
+    st = V3{-w2 / 2., 0, -w1 / 2.}; a2 = V3{s2, 0, 0}; a1 = V3{0, 0, s1};
+  } else if (P == "YZ") {
+    st = V3{0, -w1 / 2., -w2 / 2.}; a1 = V3{0, s1, 0}; a2 = V3{0, 0, s2};
+  } else if (P == "ZY") {
+    st = V3{0, -w2 / 2., -w1 / 2.}; a2 = V3{0, s2, 0}; a1 = V3{0, 0, s1};
+  } else {
+    return -1;
+  }
+  V3 const r = v3(rot);
+  rot_xyz(st, r);
+  rot_xyz(a1, r);
+  rot_xyz(a2, r);
+  st.x += trans[0];
+  st.y += trans[1];
+  st.z += trans[2];
+  V3 nv = unit(cross(a1, a2));
+  if (normal == -1) nv = muls(nv, -1);
+  else if (normal != 0 && normal != 1) return -2;
+  int64_t const np = (int64_t)nx1 * nx2;
+  for (int64_t i = 0; i < np; ++i) {
+    int const i1 = (int)(i / nx2), i2 = (int)(i % nx2);
+    V3 p = st;
+    V3 t1 = muls(a1, (double)i1);
+    p = V3{p.x + t1.x, p.y + t1.y, p.z + t1.z};
+    V3 t2 = muls(a2, (double)i2);
+    p = V3{p.x + t2.x, p.y + t2.y, p.z + t2.z};
+    xyz[3 * i] = p.x;
+    xyz[3 * i + 1] = p.y;
+    xyz[3 * i + 2] = p.z;
+    nrm[3 * i] = nv.x;
+    nrm[3 * i + 1] = nv.y;
+    nrm[3 * i + 2] = nv.z;
+    x1x2[2 * i] = s1 * i1 - std::sqrt(mag2(a1)) * (nx1 - 1) / 2.;
+    x1x2[2 * i + 1] = s2 * i2 - std::sqrt(mag2(a2)) * (nx2 - 1) / 2.;
+  }
+  return 0;
+}
+
+void oracle_spectrum(const double* traj, int64_t n, double dt, const double obs[3], const double* energies_ev,
+                     int64_t ne, double q, double current, double weight, int nthreads, double* out) {
+  V3 const o = v3(obs);
+  per_point_threads(ne, nthreads, [=](int64_t i) { out[i] = flux_point(traj, n, dt, o, ev_to_omega(energies_ev[i]), q, current, weight); });
+}
+
+void oracle_flux(const double* traj, int64_t n, double dt, const double* obs, int64_t ns, double energy_ev,
+                 double q, double current, double weight, int nthreads, double* out) {
+  double const omega = ev_to_omega(energy_ev);
+  per_point_threads(ns, nthreads, [=](int64_t i) { out[i] = flux_point(traj, n, dt, v3(obs + 3 * i), omega, q, current, weight); });
+}
+
+void oracle_power_density(const double* traj, int64_t n, double dt, const double* obs, const double* normals,
+                          int64_t ns, int directional, double q, double current, int nthreads, double* out) {
+  per_point_threads(ns, nthreads, [=](int64_t i) { out[i] = pd_point(traj, n, dt, v3(obs + 3 * i), v3(normals + 3 * i), directional != 0, q, current); });
+}
+
+double oracle_total_power(const double* traj, int64_t n, double dt, double q, double current, double gamma) {
+  // OSCARSSR::CalculateTotalPower(TParticleA&), OSCARSSR.cc:1939-1978
+  double tp = 0;
+  for (int64_t i = 0; i != n; ++i) {
+    V3 const B{traj[3 * n + i], traj[4 * n + i], traj[5 * n + i]};
+    V3 const A{traj[6 * n + i], traj[7 * n + i], traj[8 * n + i]};
+    tp += (mag2(A) - mag2(cross(B, A))) * dt;
+  }
+  tp *= std::fabs(q * current) * std::pow(gamma, 6) / (6 * kPi * kEps0 * kC);
+  return tp;
+}
+
+void oracle_kahan_add(double* sum, double* comp, const double* v, int64_t n) {
+  // T3DScalarContainer::AddToPoint, T3DScalarContainer.cc:31-47
+  for (int64_t i = 0; i < n; ++i) {
+    double const s = sum[i];
+    double const y = v[i] - comp[i];
+    double const t = s + y;
+    comp[i] = (t - s) - y;
+    sum[i] = t;
+  }
+}
+
+}  // extern "C"

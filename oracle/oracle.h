@@ -1,0 +1,96 @@
+/* CPU oracle: a clean-room FP64 restatement of the reference OSCARS SR CPU path
+ * (dhidas/SRSOriginal, OSCARS SR 1.31.05) for the radiation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so, and only as the checker / the timed CPU
+ * baseline.  The product (srsoriginal_amd) never links or calls it.
+ *
+ * Every function follows the reference operation order so that, compiled with the
+ * reference's own flags (setup.py:33: -std=c++11 -O3 -pedantic -fPIC -pthread, no
+ * FMA contraction on x86-64), results are bit-identical to the reference engine.
+ * Each is pinned in tests/test_oracle_golden.py against vectors written by the
+ * compiled reference (tests/golden/) and in tests/test_oracle_closed_form.py
+ * against closed forms; see the per-function comments in oracle.cc.
+ *
+ * Layouts: trajectories are SoA float64 [9][n] = x,y,z, bx,by,bz, ax,ay,az;
+ * point lists are AoS float64 [ns][3].
+ */
+#ifndef SRS_ORACLE_H
+#define SRS_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One ideal undulator (TField3D_IdealUndulator). */
+typedef struct oracle_undulator {
+  double field[3];
+  double period[3];
+  int nperiods;
+  double phase;
+  double taper;
+  double center[3];
+} oracle_undulator;
+
+/* A field set: ideal undulators, plus optionally one pole-scaled undulator
+ * (pole_und != NULL, pole_scales[2*nperiods] = delta_p). */
+typedef struct oracle_fields {
+  const oracle_undulator* und;
+  int nund;
+  const oracle_undulator* pole_und;
+  const double* pole_scales;
+} oracle_fields;
+
+/* Particle state needed by the trajectory (TParticleA). */
+typedef struct oracle_particle {
+  double q, m, current;
+  double x0[3], beta0[3];
+  double t0;
+  double gamma, qmg;
+} oracle_particle;
+
+/* Electron of energy E [GeV] along d0 from x0, ideal (TParticleBeam::GetNewParticle("ideal")). */
+void oracle_ideal_electron(double energy_gev, const double x0[3], const double d0[3], double current, oracle_particle* out);
+
+/* Random particles from a Gaussian beam (TParticleBeam::GetNewParticle()) using a
+ * std::mt19937 seeded as TRandomA::SetSeed and a std::normal_distribution whose state
+ * persists across calls exactly like the reference's process-global generator. */
+/* reset_normal != 0 also clears the normal distribution's cached value (a fresh process). */
+void oracle_rng_seed(int seed, int reset_normal);
+double oracle_rng_normal(void);
+double oracle_rng_uniform(void);
+void oracle_random_electron(double energy_gev, double sigma_e_gev, const double x0[3], const double d0[3],
+                            const double hdir[3], const double beta[2], const double emit[2],
+                            const double lattice_ref[3], double current, oracle_particle* out);
+
+/* Number of trajectory points CalculateTrajectory produces. */
+int64_t oracle_trajectory_npoints(double ct_start, double ct_stop, int64_t npoints, double t0);
+/* RK4 trajectory; writes traj[9][n] (n from oracle_trajectory_npoints) and dt. */
+int64_t oracle_trajectory(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                          int64_t npoints, double* traj, double* dt);
+/* Summed magnetic field at a point. */
+void oracle_bfield(const oracle_fields* f, const double x[3], double out[3]);
+
+/* Energy grid of TSpectrumContainer::Init(N, E0, E1). */
+void oracle_energy_grid(int64_t n, double e0, double e1, double* out);
+
+/* Rectangle surface (TSurfacePoints_Rectangle::Init(plane, ...)).  Returns 0 or -1 on bad plane. */
+int oracle_rect_surface(const char* plane, int nx1, int nx2, double w1, double w2, const double rot[3],
+                        const double trans[3], int normal, double* xyz, double* nrm, double* x1x2);
+
+/* Single-particle integrals; out[i] = value * weight (no container semantics). */
+void oracle_spectrum(const double* traj, int64_t n, double dt, const double obs[3], const double* energies_ev,
+                     int64_t ne, double q, double current, double weight, int nthreads, double* out);
+void oracle_flux(const double* traj, int64_t n, double dt, const double* obs, int64_t ns, double energy_ev,
+                 double q, double current, double weight, int nthreads, double* out);
+void oracle_power_density(const double* traj, int64_t n, double dt, const double* obs, const double* normals,
+                          int64_t ns, int directional, double q, double current, int nthreads, double* out);
+double oracle_total_power(const double* traj, int64_t n, double dt, double q, double current, double gamma);
+
+/* Kahan add of TSpectrumContainer::AddToFlux / T3DScalarContainer::AddToPoint. */
+void oracle_kahan_add(double* sum, double* comp, const double* v, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

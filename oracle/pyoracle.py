@@ -1,0 +1,162 @@
+"""ctypes wrapper of liboracle.so (the CPU oracle).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+D = C.c_double
+P = C.POINTER(C.c_double)
+I64 = C.c_int64
+
+
+class Undulator(C.Structure):
+    _fields_ = [("field", D * 3), ("period", D * 3), ("nperiods", C.c_int), ("phase", D), ("taper", D), ("center", D * 3)]
+
+
+class Fields(C.Structure):
+    _fields_ = [("und", C.POINTER(Undulator)), ("nund", C.c_int), ("pole_und", C.POINTER(Undulator)), ("pole_scales", P)]
+
+
+class Particle(C.Structure):
+    _fields_ = [("q", D), ("m", D), ("current", D), ("x0", D * 3), ("beta0", D * 3), ("t0", D), ("gamma", D), ("qmg", D)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = C.CDLL(path)
+        L.oracle_trajectory.restype = I64
+        L.oracle_trajectory_npoints.restype = I64
+        L.oracle_trajectory_npoints.argtypes = [D, D, I64, D]
+        L.oracle_total_power.restype = D
+        L.oracle_rng_normal.restype = D
+        L.oracle_rng_uniform.restype = D
+        L.oracle_rect_surface.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(P)
+
+
+def _v3(x):
+    return (D * 3)(*[float(v) for v in x])
+
+
+def undulator(field, period, nperiods, phase=0.0, taper=0.0, center=(0, 0, 0)):
+    return dict(field=field, period=period, nperiods=nperiods, phase=phase, taper=taper, center=center)
+
+
+class FieldSet:
+    """Keeps the ctypes structures alive for the duration of the calls."""
+
+    def __init__(self, undulators=(), pole=None, pole_scales=None):
+        self._und = (Undulator * max(1, len(undulators)))()
+        for i, u in enumerate(undulators):
+            self._und[i] = Undulator(_v3(u["field"]), _v3(u["period"]), int(u["nperiods"]), float(u.get("phase", 0)),
+                                     float(u.get("taper", 0)), _v3(u.get("center", (0, 0, 0))))
+        self.s = Fields()
+        self.s.und = self._und
+        self.s.nund = len(undulators)
+        if pole is not None:
+            self._pole = Undulator(_v3(pole["field"]), _v3(pole["period"]), int(pole["nperiods"]), 0.0, 0.0, _v3((0, 0, 0)))
+            self._scales = np.ascontiguousarray(pole_scales, dtype=np.float64)
+            self.s.pole_und = C.pointer(self._pole)
+            self.s.pole_scales = _p(self._scales)
+
+
+def ideal_electron(energy_gev, x0, d0, current):
+    p = Particle()
+    lib().oracle_ideal_electron(D(energy_gev), _v3(x0), _v3(d0), D(current), C.byref(p))
+    return p
+
+
+def rng_seed(seed, reset_normal=True):
+    lib().oracle_rng_seed(C.c_int(int(seed)), C.c_int(1 if reset_normal else 0))
+
+
+def rng_normal():
+    return lib().oracle_rng_normal()
+
+
+def random_electron(energy_gev, sigma_e_gev, x0, d0, hdir, beta, emit, lattice_ref, current):
+    p = Particle()
+    lib().oracle_random_electron(D(energy_gev), D(sigma_e_gev), _v3(x0), _v3(d0), _v3(hdir), (D * 2)(*beta), (D * 2)(*emit),
+                                 _v3(lattice_ref), D(current), C.byref(p))
+    return p
+
+
+def trajectory(fields, particle, ct_start, ct_stop, npoints):
+    n = lib().oracle_trajectory_npoints(ct_start, ct_stop, npoints, particle.t0)
+    t = np.zeros((9, n))
+    dt = D()
+    lib().oracle_trajectory(C.byref(fields.s), C.byref(particle), D(ct_start), D(ct_stop), I64(npoints), _p(t), C.byref(dt))
+    return t, dt.value
+
+
+def bfield(fields, x):
+    out = (D * 3)()
+    lib().oracle_bfield(C.byref(fields.s), _v3(x), out)
+    return np.array(out[:])
+
+
+def energy_grid(n, e0, e1):
+    out = np.zeros(n)
+    lib().oracle_energy_grid(I64(n), D(e0), D(e1), _p(out))
+    return out
+
+
+def rect_surface(plane, nx1, nx2, w1, w2, rot=(0, 0, 0), trans=(0, 0, 0), normal=0):
+    n = nx1 * nx2
+    xyz = np.zeros((n, 3)); nrm = np.zeros((n, 3)); x12 = np.zeros((n, 2))
+    rc = lib().oracle_rect_surface(plane.encode(), C.c_int(nx1), C.c_int(nx2), D(w1), D(w2), _v3(rot), _v3(trans),
+                                   C.c_int(normal), _p(xyz), _p(nrm), _p(x12))
+    if rc != 0:
+        raise ValueError("bad plane or normal")
+    return xyz, nrm, x12
+
+
+def spectrum(traj, dt, obs, energies, q, current, weight=1.0, nthreads=1):
+    traj = np.ascontiguousarray(traj); e = np.ascontiguousarray(energies, dtype=np.float64)
+    out = np.zeros(len(e))
+    lib().oracle_spectrum(_p(traj), I64(traj.shape[1]), D(dt), _v3(obs), _p(e), I64(len(e)), D(q), D(current), D(weight),
+                          C.c_int(nthreads), _p(out))
+    return out
+
+
+def flux(traj, dt, obs, energy_ev, q, current, weight=1.0, nthreads=1):
+    traj = np.ascontiguousarray(traj); obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(obs))
+    lib().oracle_flux(_p(traj), I64(traj.shape[1]), D(dt), _p(obs), I64(len(obs)), D(energy_ev), D(q), D(current), D(weight),
+                      C.c_int(nthreads), _p(out))
+    return out
+
+
+def power_density(traj, dt, obs, normals, directional, q, current, nthreads=1):
+    traj = np.ascontiguousarray(traj); obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(obs))
+    lib().oracle_power_density(_p(traj), I64(traj.shape[1]), D(dt), _p(obs), _p(nrm), I64(len(obs)), C.c_int(1 if directional else 0),
+                               D(q), D(current), C.c_int(nthreads), _p(out))
+    return out
+
+
+def total_power(traj, dt, q, current, gamma):
+    traj = np.ascontiguousarray(traj)
+    return lib().oracle_total_power(_p(traj), I64(traj.shape[1]), D(dt), D(q), D(current), D(gamma))
+
+
+def kahan_add(s, c, v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    lib().oracle_kahan_add(_p(s), _p(c), _p(v), I64(len(v)))

@@ -1,0 +1,269 @@
+// Golden-vector harness: drives the reference OSCARS SR C++ engine (compiled
+// from its own sources under /root/reference/src by oracle/Makefile, target
+// `ref`) and writes raw float64 results that tests/golden/make_golden.py turns
+// into committed fixtures.  Test infrastructure only: it is never shipped and
+// the product never links it.
+//
+// Usage:  ref_harness <mode> key=value ...
+//   modes: traj | spectrum | flux | pd | totalpower | beam | multiflux
+//   see parse() below for the keys.  Output (key out=PATH) is a flat float64
+//   little-endian file; its layout per mode is documented in write_*().
+//
+// The calls mirror what the Python entry points do:
+//   set_particle_beam        OSCARSSR_Python.cc:1641-1655 (+ sigma :1822-1829)
+//   add_bfield_undulator     OSCARSSR_Python.cc:736-818
+//   calculate_spectrum       OSCARSSR_Python.cc:2215-2351
+//   calculate_flux_rectangle OSCARSSR_Python.cc:3009-3233
+//   calculate_power_density_rectangle OSCARSSR_Python.cc:2587-2807
+//   calculate_total_power    OSCARSSR_Python.cc:2369-2398
+
+#include "OSCARSSR.h"
+#include "TField.h"
+#include "TField3D_IdealUndulator.h"
+#include "TSpectrumContainer.h"
+#include "T3DScalarContainer.h"
+#include "TSurfacePoints_Rectangle.h"
+#include "TSurfacePoints_3D.h"
+#include "TVector2D.h"
+#include "TRandomA.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+extern TRandomA* gRandomA;
+
+// Ideal planar undulator whose 2*NPeriods interior half-period poles are each
+// scaled by (1 + delta_p).  Terminations (0.25 / 0.75 half periods) are not
+// scaled.  Same envelope and sine as TField3D_IdealUndulator::GetF
+// (TField3D_IdealUndulator.cc:106-139); the pole index is
+// p = floor((D - (-L/2 + PS + lambda)) / (lambda/2)).
+class PoleScaledUndulator : public TField {
+ public:
+  PoleScaledUndulator(TVector3D const& F, TVector3D const& P, int N, std::vector<double> const& S)
+      : fF(F), fP(P), fN(N), fScale(S) {
+    fLam = fP.Mag();
+    fU = fP.UnitVector();
+    fLen = fP.Mag() * (fN + 2);
+  }
+  double GetFx(double x, double y, double z) const { return GetF(TVector3D(x, y, z)).GetX(); }
+  double GetFy(double x, double y, double z) const { return GetF(TVector3D(x, y, z)).GetY(); }
+  double GetFz(double x, double y, double z) const { return GetF(TVector3D(x, y, z)).GetZ(); }
+  TVector3D GetF(double x, double y, double z) const { return GetF(TVector3D(x, y, z)); }
+  TVector3D GetF(TVector3D const& X) const {
+    double const D = X.Dot(fU);
+    double const s = sin(2.0 * 3.14159265358979323846 * D / fLam);
+    TVector3D F(0, 0, 0);
+    if (D > fLen / 2. || D < -fLen / 2.) return F;
+    if (D < -fLen / 2. + fLam || D > fLen / 2. - fLam) {
+      if (D < -fLen / 2. + fLam / 2. || D > fLen / 2. - fLam / 2.) {
+        return TVector3D((0.25 * fF.GetX()) * s, (0.25 * fF.GetY()) * s, (0.25 * fF.GetZ()) * s);
+      }
+      return TVector3D((0.75 * fF.GetX()) * s, (0.75 * fF.GetY()) * s, (0.75 * fF.GetZ()) * s);
+    }
+    int p = (int)floor((D - (-fLen / 2. + fLam)) / (fLam / 2.));
+    if (p < 0) p = 0;
+    if (p > 2 * fN - 1) p = 2 * fN - 1;
+    double const sc = 1.0 + fScale[p];
+    return TVector3D((fF.GetX() * s) * sc, (fF.GetY() * s) * sc, (fF.GetZ() * s) * sc);
+  }
+
+ private:
+  TVector3D fF, fP, fU;
+  int fN;
+  double fLam, fLen;
+  std::vector<double> fScale;
+};
+
+static std::map<std::string, std::string> gArgs;
+
+static std::vector<double> vec(std::string const& k, std::vector<double> def = std::vector<double>()) {
+  if (!gArgs.count(k)) return def;
+  std::vector<double> v;
+  std::stringstream ss(gArgs[k]);
+  std::string tok;
+  while (std::getline(ss, tok, ',')) v.push_back(strtod(tok.c_str(), 0));
+  return v;
+}
+static double num(std::string const& k, double def) { return gArgs.count(k) ? strtod(gArgs[k].c_str(), 0) : def; }
+static std::string str(std::string const& k, std::string const& def) { return gArgs.count(k) ? gArgs[k] : def; }
+static TVector3D v3(std::string const& k, TVector3D def) {
+  std::vector<double> v = vec(k);
+  return v.size() == 3 ? TVector3D(v[0], v[1], v[2]) : def;
+}
+
+static FILE* gOut = 0;
+static void put(double v) { fwrite(&v, sizeof(double), 1, gOut); }
+
+static void setup(OSCARSSR& osr) {
+  // Undulators: und1=Bx,By,Bz,Px,Py,Pz,N[,phase[,taper[,cx,cy,cz]]], und2=...
+  for (int k = 1; k <= 4; ++k) {
+    std::vector<double> u = vec("und" + std::to_string(k));
+    if (u.size() < 7) continue;
+    double phase = u.size() > 7 ? u[7] : 0;
+    double taper = u.size() > 8 ? u[8] : 0;
+    TVector3D ctr = u.size() > 11 ? TVector3D(u[9], u[10], u[11]) : TVector3D(0, 0, 0);
+    osr.AddMagneticField((TField*)new TField3D_IdealUndulator(TVector3D(u[0], u[1], u[2]), TVector3D(u[3], u[4], u[5]), (int)u[6], ctr, phase, taper));
+  }
+  // Pole-scaled undulator: poles=Bx,By,Bz,Px,Py,Pz,N  scales=d0,d1,...
+  std::vector<double> pu = vec("poles");
+  if (pu.size() == 7) {
+    osr.AddMagneticField((TField*)new PoleScaledUndulator(TVector3D(pu[0], pu[1], pu[2]), TVector3D(pu[3], pu[4], pu[5]), (int)pu[6], vec("scales")));
+  }
+  // Beam, as set_particle_beam does.
+  TVector3D x0 = v3("x0", TVector3D(0, 0, 0));
+  TVector3D d0 = v3("d0", TVector3D(0, 0, 1));
+  osr.ClearParticleBeams();
+  osr.AddParticleBeam(str("type", "electron"), "beam", x0, d0, num("energy", 3), num("t0", 0), num("current", 0.5), 1);
+  std::vector<double> beta = vec("beta"), emit = vec("emit");
+  TVector2D su(0, 0), sup(0, 0);
+  if (beta.size() == 2 && emit.size() == 2) {
+    su[0] = sqrt(emit[0] * beta[0]);
+    su[1] = sqrt(emit[1] * beta[1]);
+    sup[0] = sqrt(emit[0] / beta[0]);
+    sup[1] = sqrt(emit[1] / beta[1]);
+  }
+  TVector3D hdir = gArgs.count("hdir") ? v3("hdir", TVector3D(1, 0, 0)) : d0.Orthogonal().UnitVector();
+  hdir = hdir.UnitVector();
+  osr.GetParticleBeam("beam").SetSigma(hdir, su, sup, v3("latref", TVector3D(0, 0, 0)), num("sigE", 0));
+  osr.SetNewParticle("", "ideal");
+  std::vector<double> ct = vec("ct", std::vector<double>{0, 3});
+  osr.SetCTStartStop(ct[0], ct[1]);
+  if (gArgs.count("npoints")) osr.SetNPointsTrajectory((size_t)num("npoints", 0));
+  if (gArgs.count("nthreads_global")) osr.SetNThreadsGlobal((int)num("nthreads_global", 2));
+  if (gArgs.count("seed")) osr.SetSeed((int)num("seed", 0));
+}
+
+static TSurfacePoints_Rectangle rect() {
+  TSurfacePoints_Rectangle S;
+  std::vector<double> n = vec("nx", std::vector<double>{11, 11});
+  std::vector<double> w = vec("width", std::vector<double>{0.01, 0.01});
+  S.Init(str("plane", "XY"), (int)n[0], (int)n[1], w[0], w[1], v3("rot", TVector3D(0, 0, 0)), v3("trans", TVector3D(0, 0, 30)), (int)num("normal", 0));
+  return S;
+}
+
+static void write_container(T3DScalarContainer const& C) {
+  // layout: N, then N x (x, y, z, v)
+  put((double)C.GetNPoints());
+  for (size_t i = 0; i != C.GetNPoints(); ++i) {
+    put(C.GetPoint(i).GetX().GetX());
+    put(C.GetPoint(i).GetX().GetY());
+    put(C.GetPoint(i).GetX().GetZ());
+    put(C.GetPoint(i).GetV());
+  }
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: ref_harness mode key=value...\n");
+    return 2;
+  }
+  std::string mode = argv[1];
+  for (int i = 2; i < argc; ++i) {
+    std::string a = argv[i];
+    size_t e = a.find('=');
+    if (e != std::string::npos) gArgs[a.substr(0, e)] = a.substr(e + 1);
+  }
+  gOut = fopen(str("out", "/dev/stdout").c_str(), "wb");
+  if (!gOut) return 3;
+
+  OSCARSSR osr;
+  setup(osr);
+  int const nthreads = (int)num("nthreads", 1);
+  int const nparticles = (int)num("nparticles", 0);
+
+  try {
+    if (mode == "traj") {
+      // layout: n, dt, then x[n] y[n] z[n] bx by bz ax ay az
+      osr.CalculateTrajectory();
+      TParticleTrajectoryPoints const& T = osr.GetTrajectory();
+      size_t n = T.GetNPoints();
+      put((double)n);
+      put(T.GetDeltaT());
+      for (int c = 0; c < 9; ++c)
+        for (size_t i = 0; i < n; ++i) {
+          TVector3D const& V = c < 3 ? T.GetX(i) : (c < 6 ? T.GetB(i) : T.GetAoverC(i));
+          put(V[c % 3]);
+        }
+    } else if (mode == "spectrum") {
+      // layout: NE, then NE x (E, flux)
+      TSpectrumContainer S;
+      std::vector<double> pts = vec("points_ev");
+      if (pts.size()) {
+        S.Init(pts);
+      } else {
+        std::vector<double> er = vec("erange", std::vector<double>{100, 3000});
+        S.Init((size_t)num("ne", 100), er[0], er[1]);
+      }
+      osr.CalculateSpectrum(v3("obs", TVector3D(0, 0, 30)), S, nparticles, nthreads, 0);
+      put((double)S.GetNPoints());
+      for (size_t i = 0; i < S.GetNPoints(); ++i) {
+        put(S.GetEnergy(i));
+        put(S.GetFlux(i));
+      }
+    } else if (mode == "flux") {
+      T3DScalarContainer C;
+      if (gArgs.count("points")) {
+        // points=x,y,z,nx,ny,nz,x,y,z,... (calculate_flux with a point list, dim=3)
+        std::vector<double> p = vec("points");
+        TSurfacePoints_3D S;
+        for (size_t i = 0; i + 5 < p.size(); i += 6) S.AddPoint(TVector3D(p[i], p[i + 1], p[i + 2]), TVector3D(p[i + 3], p[i + 4], p[i + 5]));
+        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, 3, "");
+      } else {
+        TSurfacePoints_Rectangle S = rect();
+        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, (int)num("dim", 2), "");
+      }
+      write_container(C);
+    } else if (mode == "pd") {
+      T3DScalarContainer C;
+      bool const directional = num("normal", 0) != 0;
+      if (gArgs.count("points")) {
+        std::vector<double> p = vec("points");
+        TSurfacePoints_3D S;
+        for (size_t i = 0; i + 5 < p.size(); i += 6) S.AddPoint(TVector3D(p[i], p[i + 1], p[i + 2]), TVector3D(p[i + 3], p[i + 4], p[i + 5]));
+        osr.CalculatePowerDensity(S, C, 3, directional, nparticles, "", nthreads, 0);
+      } else {
+        TSurfacePoints_Rectangle S = rect();
+        osr.CalculatePowerDensity(S, C, (int)num("dim", 2), directional, nparticles, "", nthreads, 0);
+      }
+      write_container(C);
+    } else if (mode == "totalpower") {
+      put(osr.CalculateTotalPower());
+    } else if (mode == "beam") {
+      // layout: count, then count x (x0[3], beta0[3], E0, gamma, q/m/gamma)
+      int count = (int)num("count", 8);
+      put((double)count);
+      for (int i = 0; i < count; ++i) {
+        osr.SetNewParticle();
+        TParticleA const& P = osr.GetCurrentParticle();
+        put(P.GetX0().GetX());
+        put(P.GetX0().GetY());
+        put(P.GetX0().GetZ());
+        put(P.GetB0().GetX());
+        put(P.GetB0().GetY());
+        put(P.GetB0().GetZ());
+        put(P.GetE0());
+        put(P.GetGamma());
+        put(P.GetQoverMGamma());
+      }
+    } else if (mode == "normals") {
+      // layout: count, then count standard normals straight from the global generator after set_seed
+      int count = (int)num("count", 8);
+      put((double)count);
+      for (int i = 0; i < count; ++i) put(osr.GetRandomNormal());
+    } else {
+      fprintf(stderr, "unknown mode %s\n", mode.c_str());
+      return 2;
+    }
+  } catch (std::exception const& e) {
+    fprintf(stderr, "reference threw: %s\n", e.what());
+    fclose(gOut);
+    return 4;
+  }
+  fclose(gOut);
+  return 0;
+}

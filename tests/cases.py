@@ -1,0 +1,78 @@
+"""Rebuilds the inputs of a golden case (tests/golden/manifest.json) for the oracle
+and for the product, so every parity test reads the same configuration the
+reference was run with."""
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+_MANIFEST = None
+
+
+def manifest():
+    global _MANIFEST
+    if _MANIFEST is None:
+        with open(os.path.join(GOLDEN, "manifest.json")) as f:
+            _MANIFEST = json.load(f)
+    return _MANIFEST
+
+
+def args(name):
+    return manifest()["cases"][name]["args"]
+
+
+def vec(a, key, default=None):
+    if key not in a:
+        return default
+    return [float(x) for x in a[key].split(",")]
+
+
+def undulators(a):
+    out = []
+    for k in ("und1", "und2", "und3", "und4"):
+        u = vec(a, k)
+        if not u:
+            continue
+        out.append(dict(field=u[0:3], period=u[3:6], nperiods=int(u[6]), phase=u[7] if len(u) > 7 else 0.0,
+                        taper=u[8] if len(u) > 8 else 0.0, center=u[9:12] if len(u) > 11 else [0.0, 0.0, 0.0]))
+    return out
+
+
+def pole(a):
+    u = vec(a, "poles")
+    if not u:
+        return None
+    return dict(field=u[0:3], period=u[3:6], nperiods=int(u[6]))
+
+
+def beam(a):
+    return dict(energy=float(a.get("energy", 3)), x0=vec(a, "x0", [0, 0, 0]), d0=vec(a, "d0", [0, 0, 1]),
+                current=float(a.get("current", 0.5)))
+
+
+def ct(a):
+    c = vec(a, "ct", [0, 3])
+    return c[0], c[1], int(float(a["npoints"]))
+
+
+def rect(a):
+    n = vec(a, "nx")
+    return dict(plane=a.get("plane", "XY"), nx1=int(n[0]), nx2=int(n[1]), w1=vec(a, "width")[0], w2=vec(a, "width")[1],
+                rot=vec(a, "rot", [0, 0, 0]), trans=vec(a, "trans", [0, 0, 30]), normal=int(float(a.get("normal", 0))))
+
+
+def points(a):
+    p = np.array(vec(a, "points")).reshape(-1, 6)
+    return p[:, :3].copy(), p[:, 3:].copy()
+
+
+def oracle_setup(O, name, npoints=None):
+    """Oracle fields, ideal particle, trajectory and dt for a case."""
+    a = args(name)
+    F = O.FieldSet(undulators(a), pole(a), None)
+    b = beam(a)
+    p = O.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    c0, c1, n = ct(a)
+    t, dt = O.trajectory(F, p, c0, c1, npoints or n)
+    return F, p, t, dt

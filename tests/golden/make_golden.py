@@ -1,0 +1,307 @@
+"""Writes the golden fixtures in tests/golden/ from the reference engine.
+
+The reference OSCARS SR C++ engine is compiled from its own sources under
+/root/reference/src (``make -C oracle ref``) and driven by oracle/ref_harness.cc;
+this script runs that binary on each case below and stores inputs and outputs
+as ``<case>.npz`` (plain arrays, no pickles) plus ``manifest.json``.
+
+Run only where /root/reference exists:
+
+    make -C oracle ref && python tests/golden/make_golden.py [--only case,...]
+
+Large outputs (full trajectories) are stored as a SHA-256 digest of the raw
+little-endian float64 bytes plus strided samples; everything else is stored
+whole.
+"""
+import argparse
+import hashlib
+import json
+import os
+import platform
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+# Undulators used by the configs (bfield, period, nperiods[, phase]).
+UND_C0 = "0,1,0,0,0,0.05,41"  # By = 1 T sin(2 pi z / 50 mm), 41 periods
+UND_EPU_Y = "0,0.6,0,0,0,0.057,30"
+UND_EPU_X = "0.6,0,0,0,0,0.057,30,-1.5707963267948966"  # phase -pi/2 -> +pi/2 shift
+BEAM = ["x0=0,0,-1.5", "d0=0,0,1", "energy=3", "current=0.5"]
+BEAM_C3 = BEAM + ["sigE=0.00267", "beta=1.5,0.8", "emit=0.9e-9,8e-12", "hdir=1,0,0", "latref=0,0,0"]
+
+
+def run(mode, args):
+    with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
+        path = f.name
+    cmd = [HARNESS, mode] + list(args) + ["out=" + path]
+    t0 = time.time()
+    subprocess.run(cmd, check=True)
+    dt = time.time() - t0
+    a = np.fromfile(path, dtype="<f8")
+    os.unlink(path)
+    return a, dt
+
+
+def digest(arr):
+    return hashlib.sha256(np.ascontiguousarray(arr, dtype="<f8").tobytes()).hexdigest()
+
+
+def kv(args):
+    out = {}
+    for a in args:
+        k, v = a.split("=", 1)
+        out[k] = v
+    return out
+
+
+def container(a):
+    n = int(a[0])
+    return a[1:1 + 4 * n].reshape(n, 4)
+
+
+CASES = {}
+
+
+def case(fn):
+    CASES[fn.__name__] = fn
+    return fn
+
+
+@case
+def traj_c0_2001():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001"]
+    a, dt = run("traj", args)
+    n = int(a[0])
+    return args, {"n": np.array(n), "dt": np.array(a[1]), "traj": a[2:].reshape(9, n)}, dt
+
+
+def _traj_digest(args):
+    a, dt = run("traj", args)
+    n = int(a[0])
+    t = a[2:].reshape(9, n)
+    idx = np.arange(0, n, max(1, n // 400))
+    return {"n": np.array(n), "dt": np.array(a[1]),
+            "digest_xb": np.array(digest(t[:6])), "digest_a_tail": np.array(digest(t[6:, 1:])),
+            "sample_idx": idx, "samples": t[:, idx], "a_point0": t[6:, 0]}, dt
+
+
+@case
+def traj_c0_20001():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=20001"]
+    out, dt = _traj_digest(args)
+    return args, out, dt
+
+
+@case
+def traj_epu_40001():
+    args = ["und1=" + UND_EPU_Y, "und2=" + UND_EPU_X] + BEAM + ["ct=0,3", "npoints=40001"]
+    out, dt = _traj_digest(args)
+    return args, out, dt
+
+
+@case
+def spectrum_c0_small():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "ne=200", "erange=100,3000", "obs=0,0,30", "nthreads=1"]
+    a, dt = run("spectrum", args)
+    return args, {"spectrum": a[1:].reshape(-1, 2)}, dt
+
+
+@case
+def spectrum_c0_full():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=20001", "ne=2000", "erange=100,3000", "obs=0,0,30", "nthreads=8"]
+    a, dt = run("spectrum", args)
+    return args, {"spectrum": a[1:].reshape(-1, 2)}, dt
+
+
+@case
+def spectrum_points_offaxis():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=4001", "points_ev=140,143.7,287.5,431.2,1000.5", "obs=0.001,-0.0005,30", "nthreads=1"]
+    a, dt = run("spectrum", args)
+    return args, {"spectrum": a[1:].reshape(-1, 2)}, dt
+
+
+def first_harmonic_energy():
+    """E* = C0 grid energy with the largest on-axis flux in 120..170 eV (first harmonic)."""
+    path = os.path.join(HERE, "spectrum_c0_full.npz")
+    s = np.load(path)["spectrum"]
+    m = (s[:, 0] > 120) & (s[:, 0] < 170)
+    i = np.argmax(np.where(m, s[:, 1], -1))
+    return float(s[i, 0])
+
+
+@case
+def flux_c1_small():
+    e = first_harmonic_energy()
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=20001", "plane=XY", "width=0.02,0.02", "nx=21,21", "trans=0,0,30", "energy_ev=%.17g" % e, "nthreads=8"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a), "energy_ev": np.array(e)}, dt
+
+
+@case
+def flux_c1_full():
+    e = first_harmonic_energy()
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=20001", "plane=XY", "width=0.02,0.02", "nx=201,201", "trans=0,0,30", "energy_ev=%.17g" % e, "nthreads=8"]
+    a, dt = run("flux", args)
+    c = container(a)
+    return args, {"value": c[:, 3], "energy_ev": np.array(e)}, dt
+
+
+@case
+def flux_1thread_route():
+    # nthreads=1 route: CalculateFlux(P,...) adds N more container points (OSCARSSR.cc:2352-2358)
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "plane=XY", "width=0.004,0.002", "nx=5,3", "trans=0,0,30", "energy_ev=431.2", "nthreads=1"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def flux_threads_route_dim3():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "plane=XZ", "width=0.004,0.002", "nx=5,3", "trans=0,0.001,30", "rot=0.1,0,0", "energy_ev=287.5", "nthreads=2", "dim=3"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def flux_points_list():
+    pts = "0,0,30,0,0,1,0.001,0,30,0,0,1,0,0.002,31,0,0,1,-0.003,0.001,29.5,0,0,1"
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=4001", "points=" + pts, "energy_ev=143.7", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_c2_small():
+    args = ["und1=" + UND_EPU_Y, "und2=" + UND_EPU_X] + BEAM + ["ct=0,3", "npoints=40001", "plane=XY", "width=0.06,0.06", "nx=31,31", "trans=0,0,30", "nthreads=8"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_c2_full():
+    args = ["und1=" + UND_EPU_Y, "und2=" + UND_EPU_X] + BEAM + ["ct=0,3", "npoints=40001", "plane=XY", "width=0.06,0.06", "nx=301,301", "trans=0,0,30", "nthreads=8"]
+    a, dt = run("pd", args)
+    c = container(a)
+    return args, {"value": c[:, 3]}, dt
+
+
+@case
+def pd_c0_1thread_route():
+    # 1-thread route ignores Weight and adds no duplicates beyond the pre-added N
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "plane=XY", "width=0.02,0.01", "nx=7,5", "trans=0,0,30", "nthreads=1"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_c0_threads_route():
+    # Same configuration as pd_c0_1thread_route through the threads route; the 1-thread
+    # route reads the uninitialised A/c of trajectory point 0 (OSCARSSR.cc:606) and
+    # returned NaN when the fixture was written.
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "plane=XY", "width=0.02,0.01", "nx=7,5", "trans=0,0,30", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_directional_tilted():
+    # Directional: N1.n <= 0 terms skipped, no |Sum| (OSCARSSR.cc:1549-1551, 1569-1573)
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "plane=XZ", "width=0.02,0.4", "nx=6,9", "trans=0.005,0,0.5", "rot=0,0,0", "normal=-1", "nthreads=3"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def totalpower_c0():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=20001"]
+    a, dt = run("totalpower", args)
+    return args, {"total_power": np.array(a[0])}, dt
+
+
+@case
+def totalpower_c2():
+    args = ["und1=" + UND_EPU_Y, "und2=" + UND_EPU_X] + BEAM + ["ct=0,3", "npoints=40001"]
+    a, dt = run("totalpower", args)
+    return args, {"total_power": np.array(a[0])}, dt
+
+
+@case
+def normals_seed1234():
+    args = ["und1=" + UND_C0] + BEAM + ["seed=1234", "count=64"]
+    a, dt = run("normals", args)
+    return args, {"normals": a[1:]}, dt
+
+
+@case
+def beam_c3_seed1234():
+    args = ["und1=" + UND_C0] + BEAM_C3 + ["seed=1234", "count=32"]
+    a, dt = run("beam", args)
+    return args, {"particles": a[1:].reshape(-1, 9)}, dt
+
+
+@case
+def multiflux_c3_small():
+    args = ["und1=" + UND_C0] + BEAM_C3 + ["seed=777", "ct=0,3", "npoints=3001", "plane=XY", "width=0.01,0.01", "nx=9,9", "trans=0,0,30", "energy_ev=431.2", "nparticles=4", "nthreads=1"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+def pole_deltas(seed=4, n=82, sigma=0.005):
+    return np.random.default_rng(seed).normal(0.0, sigma, n)
+
+
+@case
+def poles_flux_and_fd_gradient():
+    """Flux for a pole-scaled undulator and the reference's central-difference
+    gradient of L = sum_j (F_j(delta) - F_j(0))^2 with h = 1e-6 (config-4 loss)."""
+    d = pole_deltas()
+    base = ["poles=0,1,0,0,0,0.05,41"] + BEAM + ["ct=0,3", "npoints=2001", "plane=XY", "width=0.005,0.005", "nx=5,5", "trans=0,0,30", "energy_ev=143.7", "nthreads=1"]
+
+    def flux(delta):
+        a, _ = run("flux", base + ["scales=" + ",".join("%.17g" % x for x in delta)])
+        return container(a)[:, 3]
+
+    t0 = time.time()
+    f0 = flux(np.zeros_like(d))
+    fd = flux(d)
+    h = 1e-6
+    grad = np.zeros_like(d)
+    for p in range(len(d)):
+        dp = d.copy(); dp[p] += h
+        dm = d.copy(); dm[p] -= h
+        grad[p] = (np.sum((flux(dp) - f0) ** 2) - np.sum((flux(dm) - f0) ** 2)) / (2 * h)
+    return base, {"deltas": d, "flux_ideal": f0, "flux_scaled": fd, "loss": np.array(np.sum((fd - f0) ** 2)), "fd_grad": grad}, time.time() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default="")
+    ns = ap.parse_args()
+    if not os.path.exists(HARNESS):
+        sys.exit("build the reference harness first: make -C oracle ref")
+    only = [c for c in ns.only.split(",") if c]
+    mpath = os.path.join(HERE, "manifest.json")
+    manifest = json.load(open(mpath)) if os.path.exists(mpath) else {"cases": {}}
+    gcc = subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    manifest["generator"] = {"compiler": gcc, "flags": "-std=c++11 -O3 -pedantic -fPIC -pthread",
+                             "libc": " ".join(platform.libc_ver()), "machine": platform.machine(),
+                             "reference": "dhidas/SRSOriginal (OSCARS SR 1.31.05) src/*.cc minus the Python/CUDA files"}
+    for name, fn in CASES.items():
+        if only and name not in only:
+            continue
+        args, arrays, dt = fn()
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        manifest["cases"][name] = {"args": kv(args), "seconds": round(dt, 3)}
+        print("%-32s %7.2fs" % (name, dt), flush=True)
+    with open(mpath, "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
