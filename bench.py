@@ -1,0 +1,160 @@
+"""Benchmark of the radiation hot path on one MI355X.
+
+Workload (the reference's headline configuration on one GPU): the flux map of
+an ideal planar undulator -- 3 GeV, 0.5 A electron, By = 1 T sin(2 pi z/50 mm),
+41 periods, RK4 trajectory of 20001 points over ct = 0..3 m -- on a 201 x 201
+rectangular grid, 20 mm x 20 mm at z = 30 m, at the first-harmonic energy
+E* = 143.5218 eV.  One step = one evaluation of the whole map (40401 points x
+20001 trajectory steps = 8.08e8 terms) with inputs resident in HBM.  The mixed
+form (FP64 phase, FP32 trig/amplitudes/sums) is the headline value; the plain
+FP64 form is timed in the same run, and the mixed map is compared against it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  The roofline entry is for the dominant kernel (the mixed
+flux kernel), timed with HIP events on its launch stream; the CPU baseline is
+the FP64 CPU path (the oracle's restatement of the reference, one std::thread
+per output point as the reference does) on a bounded sample of the same map.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ENERGY_EV = 143.52176088044024   # first-harmonic argmax of the reference's C0 spectrum grid
+F_ALG_FLUX = 53                  # algorithmic FLOP per flux term
+PEAK_FP32_VALU = 157.3e12        # MI355X FP32 vector peak (published)
+PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup_problem():
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.050], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(20001)
+    osr.calculate_trajectory()
+    traj, dt = osr.trajectory_array()
+    from srsoriginal_amd.lib import _sr
+    surf = _sr.Surface.rectangle("XY", 201, 201, 0.02, 0.02, [0, 0, 0], [0, 0, 30], 0)
+    q, cur = osr._e.particle_charge_current()
+    return traj, dt, surf.xyz, q, cur
+
+
+def cpu_baseline(traj, dt, xyz, q, cur, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    # calibrate on a few points, then time a sample sized to ~budget_s
+    idx = np.linspace(0, len(xyz) - 1, 2 * threads).astype(int)
+    t0 = time.perf_counter()
+    pyoracle.flux(traj, dt, xyz[idx], ENERGY_EV, q, cur, nthreads=threads)
+    per_pt = (time.perf_counter() - t0) / len(idx)
+    npts = int(min(len(xyz), max(threads, budget_s / max(per_pt, 1e-9))))
+    idx = np.linspace(0, len(xyz) - 1, npts).astype(int)
+    t0 = time.perf_counter()
+    pyoracle.flux(traj, dt, xyz[idx], ENERGY_EV, q, cur, nthreads=threads)
+    el = time.perf_counter() - t0
+    terms = npts * traj.shape[1]
+    return {"value": terms / el, "unit": "terms/s", "cores": threads, "kind": "port",
+            "sample": "%d of the 40401 map points x %d trajectory steps, FP64 CPU path, %d threads, %.1f s"
+                      % (npts, traj.shape[1], threads, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        sys.exit("bench.py measures one GPU: this build runs on a single MI355X (multi-GPU sharding is design-only)")
+
+    import torch
+    from srsoriginal_amd import abi
+
+    traj, dt, xyz, q, cur = setup_problem()
+    n, ns = traj.shape[1], len(xyz)
+    dev = torch.device("cuda:0")
+    d_traj = torch.from_numpy(np.ascontiguousarray(traj)).to(dev)
+    d_obs = torch.from_numpy(np.ascontiguousarray(xyz.T)).to(dev)
+    d_out = {p: torch.zeros(ns, dtype=torch.float64, device=dev) for p in (abi.FP64, abi.MIXED)}
+    ptrs = [d_traj[k].data_ptr() for k in range(9)]
+    stream = torch.cuda.current_stream()
+    terms = float(n) * ns
+
+    def step(prec):
+        abi.flux_device(ptrs, n, dt, d_obs.data_ptr(), ns, [ENERGY_EV], q, cur, prec, d_out[prec].data_ptr(),
+                        stream.cuda_stream)
+
+    res = {}
+    for prec in (abi.FP64, abi.MIXED):
+        for _ in range(a.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        abi.timing_enable(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(a.steps):
+            step(prec)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.steps
+        kms, launches = abi.timing_read()
+        abi.timing_enable(False)
+        res[prec] = dict(ms=ms, kernel_ms=kms / max(launches, 1), out=d_out[prec].cpu().numpy())
+        log("%s: %.3f ms/step, kernel %.3f ms" % ("fp64" if prec == 0 else "mixed", ms, res[prec]["kernel_ms"]))
+
+    ref, mix = res[abi.FP64]["out"], res[abi.MIXED]["out"]
+    max_rel = float(np.max(np.abs(mix - ref)) / np.max(np.abs(ref)))
+    rms_rel = float(np.sqrt(np.mean((mix - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    value = terms / (res[abi.MIXED]["ms"] * 1e-3)
+    kernel_s = res[abi.MIXED]["kernel_ms"] * 1e-3
+    achieved = F_ALG_FLUX * terms / kernel_s
+    out = {
+        "metric": "obs*energy*traj-step terms/s (mixed precision, <=1e-5 rel err vs FP64)",
+        "value": value,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": res[abi.MIXED]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64 phase + fp32 trig/amplitude/sums",
+        "data": "synthetic (analytic undulator field, RK4 trajectory)",
+        "config": {"workload": "flux map 201x201, 20 mm at 30 m, E=143.52 eV, ideal undulator 41x50 mm By=1 T, "
+                               "3 GeV 0.5 A, RK4 NT=20001", "obs_points": ns, "energies": 1, "traj_points": n,
+                   "terms_per_step": terms},
+        "fp64": {"value": terms / (res[abi.FP64]["ms"] * 1e-3), "ms_per_step": res[abi.FP64]["ms"],
+                 "kernel_ms": res[abi.FP64]["kernel_ms"],
+                 "valu_frac": F_ALG_FLUX * terms / (res[abi.FP64]["kernel_ms"] * 1e-3) / PEAK_FP64_VALU},
+        "mixed_vs_fp64": {"speedup": res[abi.FP64]["ms"] / res[abi.MIXED]["ms"], "max_rel_err": max_rel,
+                          "rms_rel_err": rms_rel},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VALU / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_VALU, "traffic": None,
+                     "kernel": "flux_mixed_kernel<1>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
+                     "flop_per_term": F_ALG_FLUX},
+    }
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(traj, dt, xyz, q, cur)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,105 @@
+/* srs_hip.h - C ABI of libsrs_hip.so, the MI355X (gfx950) radiation kernels.
+ *
+ * This is the drop-in boundary that replaces the reference's CUDA backend
+ * (include/OSCARSSR_Cuda.h:25-28, src/OSCARSSR_Cuda.cu).  The reference's
+ * functions are declared extern "C" but take C++ objects; these take plain
+ * pointers and sizes only, so any host language can bind them (ctypes, cgo, JNI).
+ *
+ * Contract (same as the reference backend's, OSCARSSR.cc:892-900, :1874-1882,
+ * :2580-2588):
+ *   - the caller computes the trajectory on the host and passes it in;
+ *   - the callee returns one value per output point; the caller applies the
+ *     particle weight and accumulates into its containers;
+ *   - results follow the reference CPU integrands (OSCARSSR.cc:1202-1246,
+ *     :2054-2092, :1522-1573); SRS_PRECISION_FP64 matches them to < 1e-10
+ *     relative, SRS_PRECISION_MIXED to < 1e-5 (FP64 phase, FP32 trig/sums).
+ * Every function returns SRS_OK or an error code and sets a message readable
+ * with srs_hip_last_error(); nothing throws across the boundary.
+ * Host-pointer entry points are synchronous.  Calls from several host threads
+ * are serialised internally (one device, one stream).
+ */
+#ifndef SRS_HIP_H
+#define SRS_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRS_OK 0
+#define SRS_ERR_NO_DEVICE 1
+#define SRS_ERR_INVALID 2
+#define SRS_ERR_HIP 3
+
+#define SRS_PRECISION_FP64 0
+#define SRS_PRECISION_MIXED 1
+
+/* Trajectory, structure of arrays, n points spaced dt [s] (TParticleTrajectoryPoints,
+ * TParticleTrajectoryPoints.h:48-55): position x,y,z [m], beta = v/c, and A/c [1/s]
+ * (ax, ay, az are read by the power density only and may be NULL otherwise). */
+typedef struct srs_traj {
+  int64_t n;
+  double dt;
+  const double* x;
+  const double* y;
+  const double* z;
+  const double* bx;
+  const double* by;
+  const double* bz;
+  const double* ax;
+  const double* ay;
+  const double* az;
+} srs_traj;
+
+/* Number of visible gfx950 devices; replaces OSCARSSR_Cuda_GetDeviceCount
+ * (OSCARSSR_Cuda.cu:30-36, declared OSCARSSR_Cuda.h:25). */
+int srs_hip_device_count(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* srs_hip_last_error(void);
+
+/* Spectrum at one observation point for ne photon energies [eV]; flux_out[e] in
+ * photons/s/0.1%bw/mm^2, unweighted.  Replaces OSCARSSR_Cuda_CalculateSpectrumGPU
+ * (OSCARSSR_Cuda.h:26, OSCARSSR_Cuda.cu:1015-1204).  charge [C] and current [A]
+ * are the particle's (TParticleA::GetQ / GetCurrent). */
+int srs_hip_spectrum(const srs_traj* traj, const double obs[3], const double* energies_ev, int64_t ne, double charge,
+                     double current, int precision, double* flux_out);
+
+/* Flux on ns observation points (AoS xyz) for ne energies; flux_out[e*ns + j].
+ * Replaces OSCARSSR_Cuda_CalculateFluxGPU (OSCARSSR_Cuda.h:27, OSCARSSR_Cuda.cu:696-890). */
+int srs_hip_flux(const srs_traj* traj, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
+                 double charge, double current, int precision, double* flux_out);
+
+/* Power density [W/mm^2] on ns points with surface normals (AoS xyz).
+ * directional != 0 skips trajectory points seen from behind the surface and keeps
+ * the sign; otherwise |sum| (the reference CPU semantics, OSCARSSR.cc:1549-1573).
+ * Replaces OSCARSSR_Cuda_CalculatePowerDensityGPU (OSCARSSR_Cuda.h:28,
+ * OSCARSSR_Cuda.cu:1342-1547). */
+int srs_hip_power_density(const srs_traj* traj, const double* obs_xyz, const double* normals_xyz, int64_t ns,
+                          int directional, double charge, double current, int precision, double* pd_out);
+
+/* Device-resident variants: every array pointer is device memory, observation
+ * points and normals are SoA ([3][ns]); work is queued on `stream` (a hipStream_t,
+ * NULL = the library's stream) and not synchronised.  No counterpart in the
+ * reference (which copies host data on every call); they serve pipelines that
+ * keep data in HBM and timing of the kernels alone. */
+int srs_hip_flux_device(const srs_traj* traj, const double* obs_soa, int64_t ns, const double* energies_ev,
+                        int64_t ne, double charge, double current, int precision, double* flux_out, void* stream);
+int srs_hip_spectrum_device(const srs_traj* traj, const double obs[3], const double* energies_ev, int64_t ne,
+                            double charge, double current, int precision, double* flux_out, void* stream);
+int srs_hip_power_density_device(const srs_traj* traj, const double* obs_soa, const double* normals_soa, int64_t ns,
+                                 int directional, double charge, double current, int precision, double* pd_out,
+                                 void* stream);
+/* Kernel timing with HIP events recorded on the launch stream around each main
+ * integral kernel (not the staging or the chunk reduction).  enable resets the
+ * record; read waits for the recorded kernels, returns their summed duration and
+ * count, and resets.  No counterpart in the reference (it has no timers). */
+int srs_hip_timing_enable(int on);
+int srs_hip_timing_read(double* total_ms, int64_t* launches);
+
+/* Waits for all work queued on `stream` (NULL = the library's stream). */
+int srs_hip_synchronize(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+"""ctypes binding of the C ABI in include/srs_hip.h (libsrs_hip.so).
+
+This is the thin Python view of the drop-in boundary: the same entry points a
+cgo/JNI/N-API binding would declare.  Loading fails loudly when the library is
+missing; there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libsrs_hip.so")
+
+SRS_OK = 0
+SRS_ERR_NO_DEVICE = 1
+SRS_ERR_INVALID = 2
+SRS_ERR_HIP = 3
+FP64 = 0
+MIXED = 1
+
+_D = C.c_double
+_P = C.POINTER(C.c_double)
+_I64 = C.c_int64
+
+
+class SrsTraj(C.Structure):
+    _fields_ = [("n", _I64), ("dt", _D), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("bx", C.c_void_p),
+                ("by", C.c_void_p), ("bz", C.c_void_p), ("ax", C.c_void_p), ("ay", C.c_void_p), ("az", C.c_void_p)]
+
+
+class SrsError(ValueError):
+    """Raised for a non-zero status from libsrs_hip (maps to the reference's ValueError)."""
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libsrs_hip.so not built (%s): run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.srs_hip_last_error.restype = C.c_char_p
+        for name, args in {
+            "srs_hip_spectrum": [C.POINTER(SrsTraj), _P, _P, _I64, _D, _D, C.c_int, _P],
+            "srs_hip_flux": [C.POINTER(SrsTraj), _P, _I64, _P, _I64, _D, _D, C.c_int, _P],
+            "srs_hip_power_density": [C.POINTER(SrsTraj), _P, _P, _I64, C.c_int, _D, _D, C.c_int, _P],
+            "srs_hip_flux_device": [C.POINTER(SrsTraj), C.c_void_p, _I64, _P, _I64, _D, _D, C.c_int, C.c_void_p, C.c_void_p],
+            "srs_hip_spectrum_device": [C.POINTER(SrsTraj), _P, _P, _I64, _D, _D, C.c_int, C.c_void_p, C.c_void_p],
+            "srs_hip_power_density_device": [C.POINTER(SrsTraj), C.c_void_p, C.c_void_p, _I64, C.c_int, _D, _D, C.c_int,
+                                             C.c_void_p, C.c_void_p],
+            "srs_hip_synchronize": [C.c_void_p],
+            "srs_hip_timing_enable": [C.c_int],
+            "srs_hip_timing_read": [C.POINTER(_D), C.POINTER(_I64)],
+        }.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.srs_hip_device_count.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != SRS_OK:
+        raise SrsError(lib().srs_hip_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def traj_struct(traj, dt, keep):
+    """srs_traj over a host [9][n] (or [6][n]) float64 array; `keep` holds references."""
+    t = _f64(traj)
+    keep.append(t)
+    n = t.shape[1]
+    ptr = [t[k].ctypes.data if k < t.shape[0] else None for k in range(9)]
+    return SrsTraj(n, dt, *ptr)
+
+
+def traj_struct_device(ptrs, n, dt):
+    """srs_traj over nine device pointers (ints; ax..az may be None)."""
+    return SrsTraj(n, dt, *ptrs)
+
+
+def device_count():
+    return lib().srs_hip_device_count()
+
+
+def spectrum(traj, dt, obs, energies_ev, charge, current, precision=FP64):
+    keep = []
+    t = traj_struct(traj, dt, keep)
+    e = _f64(energies_ev)
+    o = _f64(obs)
+    out = np.zeros(len(e))
+    _check(lib().srs_hip_spectrum(C.byref(t), o.ctypes.data_as(_P), e.ctypes.data_as(_P), len(e), charge, current,
+                                  precision, out.ctypes.data_as(_P)))
+    return out
+
+
+def flux(traj, dt, obs_xyz, energies_ev, charge, current, precision=FP64):
+    keep = []
+    t = traj_struct(traj, dt, keep)
+    o = _f64(obs_xyz).reshape(-1, 3)
+    e = _f64(np.atleast_1d(energies_ev))
+    out = np.zeros(len(e) * len(o))
+    _check(lib().srs_hip_flux(C.byref(t), o.ctypes.data_as(_P), len(o), e.ctypes.data_as(_P), len(e), charge, current,
+                              precision, out.ctypes.data_as(_P)))
+    return out.reshape(len(e), len(o))
+
+
+def power_density(traj, dt, obs_xyz, normals_xyz, directional, charge, current, precision=FP64):
+    keep = []
+    t = traj_struct(traj, dt, keep)
+    o = _f64(obs_xyz).reshape(-1, 3)
+    nn = _f64(normals_xyz).reshape(-1, 3)
+    out = np.zeros(len(o))
+    _check(lib().srs_hip_power_density(C.byref(t), o.ctypes.data_as(_P), nn.ctypes.data_as(_P), len(o),
+                                       1 if directional else 0, charge, current, precision, out.ctypes.data_as(_P)))
+    return out
+
+
+def timing_enable(on=True):
+    _check(lib().srs_hip_timing_enable(1 if on else 0))
+
+
+def timing_read():
+    """(total ms, launches) of the main integral kernels since timing_enable, from HIP events."""
+    ms = _D()
+    n = _I64()
+    _check(lib().srs_hip_timing_read(C.byref(ms), C.byref(n)))
+    return ms.value, n.value
+
+
+def flux_device(traj_ptrs, n, dt, obs_soa_ptr, ns, energies_ev, charge, current, precision, out_ptr, stream):
+    """Queue a flux evaluation on device-resident data (pointers as ints)."""
+    t = SrsTraj(n, dt, *traj_ptrs)
+    e = _f64(np.atleast_1d(energies_ev))
+    _check(lib().srs_hip_flux_device(C.byref(t), obs_soa_ptr, ns, e.ctypes.data_as(_P), len(e), charge, current,
+                                     precision, out_ptr, stream))
+
+
+def power_density_device(traj_ptrs, n, dt, obs_soa_ptr, nrm_soa_ptr, ns, directional, charge, current, precision,
+                         out_ptr, stream):
+    t = SrsTraj(n, dt, *traj_ptrs)
+    _check(lib().srs_hip_power_density_device(C.byref(t), obs_soa_ptr, nrm_soa_ptr, ns, 1 if directional else 0,
+                                              charge, current, precision, out_ptr, stream))
+
+
+def spectrum_device(traj_ptrs, n, dt, obs, energies_ev, charge, current, precision, out_ptr, stream):
+    t = SrsTraj(n, dt, *traj_ptrs)
+    e = _f64(energies_ev)
+    o = _f64(obs)
+    _check(lib().srs_hip_spectrum_device(C.byref(t), o.ctypes.data_as(_P), e.ctypes.data_as(_P), len(e), charge,
+                                         current, precision, out_ptr, stream))

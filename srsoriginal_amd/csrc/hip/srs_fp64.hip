@@ -1,0 +1,348 @@
+// Plain FP64 radiation kernels for gfx950.
+//
+// These reproduce the reference CPU integrands (OSCARSSR.cc:1202-1246 spectrum,
+// :2054-2092 flux, :1522-1573 power density) with the phase path in the
+// reference's exact operation order, so results agree with the reference to
+// rounding of the amplitude terms (well inside 1e-10 relative):
+//   R = O - X;  D = sqrt((Rx*Rx + Ry*Ry) + Rz*Rz);  tau = dt*i + D/C;
+//   phi = omega*tau  (all correctly rounded, no contraction: this file is
+//   compiled with -ffp-contract=off and FMAs appear only where written).
+// The phase reaches 4.6e11 rad at 3 keV, where one ulp of D is 5e-5 rad, so any
+// reordering of that chain would break parity; amplitudes tolerate ulp changes.
+//
+// Work decomposition (VALU-bound, trig/division heavy): a block owns 256 output
+// points (observation points, or energies for the spectrum) and one chunk of the
+// trajectory; the chunk is staged through LDS in 256-point tiles that every lane
+// reads by broadcast.  Each chunk writes FP64 partial sums; a second kernel adds
+// the chunks in fixed order (deterministic) and applies the prefactors.
+#include "srs_kernels.h"
+
+namespace srs {
+namespace {
+
+constexpr int TILE = kBlock;
+
+__device__ __forceinline__ double rcp_fp64(double d) {
+  // 1/d to ~1 ulp: hardware estimate + two Newton steps (amplitudes only).
+  double y = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-d, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
+
+template <int NE>
+__global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const double* __restrict__ obs, int ns,
+                                                            EnergySet es, int chunk, double* __restrict__ part) {
+  __shared__ double sx[TILE], sy[TILE], sz[TILE], sbx[TILE], sby[TILE], sbz[TILE];
+  int const tid = threadIdx.x;
+  int const j = blockIdx.x * kBlock + tid;
+  int const jj = j < ns ? j : ns - 1;  // padding lanes compute a valid point, never store
+  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+  int64_t const i0 = (int64_t)blockIdx.y * chunk;
+  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  double are[NE][3], aim[NE][3];
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) are[e][k] = aim[e][k] = 0.0;
+
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      sx[tid] = t.x[base + tid];
+      sy[tid] = t.y[base + tid];
+      sz[tid] = t.z[base + tid];
+      sbx[tid] = t.bx[base + tid];
+      sby[tid] = t.by[base + tid];
+      sbz[tid] = t.bz[base + tid];
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+      double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
+      double const d2 = rx * rx + ry * ry + rz * rz;
+      double const D = __builtin_sqrt(d2);
+      double const tau = t.dt * (double)(base + k) + D / kC;
+      double const inv = rcp_fp64(D);
+      double const nx = rx * inv, ny = ry * inv, nz = rz * inv;
+      double const ax = (sbx[k] - nx) * inv, ay = (sby[k] - ny) * inv, az = (sbz[k] - nz) * inv;
+      double const inv2 = inv * inv;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        double s, c;
+        sincos_fp64(es.omega[e] * tau, &s, &c);
+        double const g = es.cw[e] * inv2;
+        double const gx = nx * g, gy = ny * g, gz = nz * g;
+        // term = (a - i g n) (c + i s)
+        are[e][0] = __builtin_fma(ax, c, __builtin_fma(gx, s, are[e][0]));
+        are[e][1] = __builtin_fma(ay, c, __builtin_fma(gy, s, are[e][1]));
+        are[e][2] = __builtin_fma(az, c, __builtin_fma(gz, s, are[e][2]));
+        aim[e][0] = __builtin_fma(ax, s, __builtin_fma(-gx, c, aim[e][0]));
+        aim[e][1] = __builtin_fma(ay, s, __builtin_fma(-gy, c, aim[e][1]));
+        aim[e][2] = __builtin_fma(az, s, __builtin_fma(-gz, c, aim[e][2]));
+      }
+    }
+  }
+  if (j < ns) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
+      part[b + 0 * (size_t)ns] = are[e][0];
+      part[b + 1 * (size_t)ns] = aim[e][0];
+      part[b + 2 * (size_t)ns] = are[e][1];
+      part[b + 3 * (size_t)ns] = aim[e][1];
+      part[b + 4 * (size_t)ns] = are[e][2];
+      part[b + 5 * (size_t)ns] = aim[e][2];
+    }
+  }
+}
+
+// Spectrum: one lane per photon energy; the geometry of each trajectory point is
+// shared by all energies, so it is computed once per tile into LDS:
+//   tau = dt*i + D/C (exact reference order), A_k = (beta_k - n_k)/D, G_k = n_k/D^2.
+// Per energy: term_k = (A_k - i cw G_k) e^{i omega tau}; the lane keeps
+// sum A e^{i phi} and sum G e^{i phi} (12 reals) and applies cw at the end.
+__global__ __launch_bounds__(kBlock) void spectrum_fp64_kernel(TrajView t, double ox, double oy, double oz,
+                                                               const double* __restrict__ omega, int ne, int chunk,
+                                                               double* __restrict__ part) {
+  __shared__ double stau[TILE], sa[3][TILE], sg[3][TILE];
+  int const tid = threadIdx.x;
+  int const e = blockIdx.x * kBlock + tid;
+  double const w = omega[e < ne ? e : ne - 1];
+  int64_t const i0 = (int64_t)blockIdx.y * chunk;
+  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  double acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = base + tid;
+      double const rx = ox - t.x[i], ry = oy - t.y[i], rz = oz - t.z[i];
+      double const d2 = rx * rx + ry * ry + rz * rz;
+      double const D = __builtin_sqrt(d2);
+      stau[tid] = t.dt * (double)i + D / kC;
+      double const inv = rcp_fp64(D);
+      double const nx = rx * inv, ny = ry * inv, nz = rz * inv;
+      sa[0][tid] = (t.bx[i] - nx) * inv;
+      sa[1][tid] = (t.by[i] - ny) * inv;
+      sa[2][tid] = (t.bz[i] - nz) * inv;
+      double const inv2 = inv * inv;
+      sg[0][tid] = nx * inv2;
+      sg[1][tid] = ny * inv2;
+      sg[2][tid] = nz * inv2;
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+      double s, c;
+      sincos_fp64(w * stau[k], &s, &c);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        acc[4 * q + 0] = __builtin_fma(sa[q][k], c, acc[4 * q + 0]);
+        acc[4 * q + 1] = __builtin_fma(sa[q][k], s, acc[4 * q + 1]);
+        acc[4 * q + 2] = __builtin_fma(sg[q][k], c, acc[4 * q + 2]);
+        acc[4 * q + 3] = __builtin_fma(sg[q][k], s, acc[4 * q + 3]);
+      }
+    }
+  }
+  if (e < ne) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) part[((size_t)blockIdx.y * 12 + k) * ne + e] = acc[k];
+  }
+}
+
+// Power density: the reference integrand with N1, N1.n and 1 - beta.N1 in the
+// reference's exact operation order (the denominator cancels to ~1e-8, so it must
+// round identically).  (Num.N2)^2 + (Num.N3)^2 is evaluated as |Num|^2, which it
+// equals because {N1, N2, N3} is orthonormal and Num = N1 x (...) is orthogonal to N1.
+__global__ __launch_bounds__(kBlock) void pd_fp64_kernel(TrajView t, const double* __restrict__ obs,
+                                                         const double* __restrict__ nrm, int ns, int directional,
+                                                         int chunk, double* __restrict__ part) {
+  __shared__ double sx[TILE], sy[TILE], sz[TILE], sbx[TILE], sby[TILE], sbz[TILE], sax[TILE], say[TILE], saz[TILE];
+  int const tid = threadIdx.x;
+  int const j = blockIdx.x * kBlock + tid;
+  int const jj = j < ns ? j : ns - 1;
+  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+  double const qx = nrm[jj], qy = nrm[ns + jj], qz = nrm[2 * ns + jj];
+  int64_t const i0 = (int64_t)blockIdx.y * chunk;
+  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  double sum = 0.0;
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = base + tid;
+      sx[tid] = t.x[i];
+      sy[tid] = t.y[i];
+      sz[tid] = t.z[i];
+      sbx[tid] = t.bx[i];
+      sby[tid] = t.by[i];
+      sbz[tid] = t.bz[i];
+      sax[tid] = t.ax[i];
+      say[tid] = t.ay[i];
+      saz[tid] = t.az[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+      double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
+      double const d2 = rx * rx + ry * ry + rz * rz;
+      double const D = __builtin_sqrt(d2);
+      double const n1x = rx / D, n1y = ry / D, n1z = rz / D;
+      double const ndot = n1x * qx + n1y * qy + n1z * qz;
+      if (directional && ndot <= 0) continue;
+      double const bx = sbx[k], by = sby[k], bz = sbz[k];
+      double const den = 1 - (bx * n1x + by * n1y + bz * n1z);
+      double const ux = n1x - bx, uy = n1y - by, uz = n1z - bz;
+      double const Ax = sax[k], Ay = say[k], Az = saz[k];
+      double const vx = uy * Az - Ay * uz, vy = uz * Ax - Az * ux, vz = ux * Ay - Ax * uy;
+      double const mx = n1y * vz - vy * n1z, my = n1z * vx - vz * n1x, mz = n1x * vy - vx * n1y;
+      double const num2 = __builtin_fma(mx, mx, __builtin_fma(my, my, mz * mz));
+      double const den2 = den * den;
+      double const den5 = den2 * den2 * den;
+      sum += num2 * ndot / (den5 * d2);
+    }
+  }
+  if (j < ns) part[(size_t)blockIdx.y * ns + j] = sum;
+}
+
+// Chunk reductions (fixed order) and the reference's final prefactors:
+//   flux:  SumE *= C1*dt (OSCARSSR.cc:2080) -> (re, im) * (0 + i kw) = (-im kw, re kw);
+//          F = C2 * Re(SumE . conj(SumE)) summed x, y then z (OSCARSSR.cc:2092).
+__global__ void flux_reduce_kernel(const double* __restrict__ part, int nchunk, int ns, EnergySet es, double c2,
+                                   double* __restrict__ out) {
+  int const j = blockIdx.x * blockDim.x + threadIdx.x;
+  int const e = blockIdx.y;
+  if (j >= ns) return;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < nchunk; ++c) {
+    size_t const b = ((size_t)(c * es.ne + e) * 6) * ns + j;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] += part[b + (size_t)k * ns];
+  }
+  double const kw = es.kw[e];
+  double p[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double const re = -(v[2 * q + 1] * kw);
+    double const im = v[2 * q] * kw;
+    p[q] = re * re + im * im;
+  }
+  out[(size_t)e * ns + j] = c2 * (p[0] + p[1] + p[2]);
+}
+
+// Complex-field partials part[(c*6 + q)*npts + j] with a per-point kw (mixed spectrum).
+__global__ void field_reduce_kernel(const double* __restrict__ part, int nchunk, int npts, const double* __restrict__ kw,
+                                    double c2, double* __restrict__ out) {
+  int const j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npts) return;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < nchunk; ++c)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] += part[((size_t)c * 6 + k) * npts + j];
+  double const k2 = kw[j];
+  double p[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double const re = -(v[2 * q + 1] * k2);
+    double const im = v[2 * q] * k2;
+    p[q] = re * re + im * im;
+  }
+  out[j] = c2 * (p[0] + p[1] + p[2]);
+}
+
+__global__ void spectrum_reduce_kernel(const double* __restrict__ part, int nchunk, int ne, const double* __restrict__ cw,
+                                       const double* __restrict__ kw, double c2, double* __restrict__ out) {
+  int const e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double v[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = 0.0;
+  for (int c = 0; c < nchunk; ++c)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] += part[((size_t)c * 12 + k) * ne + e];
+  double const w = cw[e], k2 = kw[e];
+  double p[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    // sum (A - i w G)(c + i s):  re = Ac + w Gs,  im = As - w Gc
+    double const sre = v[4 * q + 0] + w * v[4 * q + 3];
+    double const sim = v[4 * q + 1] - w * v[4 * q + 2];
+    double const re = -(sim * k2);
+    double const im = sre * k2;
+    p[q] = re * re + im * im;
+  }
+  out[e] = c2 * (p[0] + p[1] + p[2]);
+}
+
+__global__ void pd_reduce_kernel(const double* __restrict__ part, int nchunk, int ns, double scale, int directional,
+                                 double* __restrict__ out) {
+  int const j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * ns + j];
+  // OSCARSSR.cc:1563-1573: Sum *= |qI|/(16 pi^2 eps0 c) dt; Sum /= 1e6; |Sum| unless directional
+  s *= scale;
+  s /= 1e6;
+  if (!directional && s < 0) s = -s;
+  out[j] = s;
+}
+
+inline int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk); }
+
+}  // namespace
+
+hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,
+                            double* part, hipStream_t s) {
+  dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  switch (es.ne) {
+    case 1: flux_fp64_kernel<1><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 2: flux_fp64_kernel<2><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 3: flux_fp64_kernel<3><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 4: flux_fp64_kernel<4><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrum_fp64(const TrajView& t, const double obs[3], const double* omega, const double*, int ne,
+                                int chunk, double* part, hipStream_t s) {
+  dim3 grid((ne + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  spectrum_fp64_kernel<<<grid, kBlock, 0, s>>>(t, obs[0], obs[1], obs[2], omega, ne, chunk, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double* nrm_soa, int ns, int directional,
+                          int chunk, double* part, hipStream_t s) {
+  dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  pd_fp64_kernel<<<grid, kBlock, 0, s>>>(t, obs_soa, nrm_soa, ns, directional, chunk, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
+                              hipStream_t s) {
+  dim3 grid((ns + 255) / 256, es.ne);
+  flux_reduce_kernel<<<grid, 256, 0, s>>>(part, nchunk, ns, es, c2, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrum_reduce(const double* part, int nchunk, int ne, const double* cw, const double* kw, double c2,
+                                  double* out, hipStream_t s) {
+  spectrum_reduce_kernel<<<(ne + 255) / 256, 256, 0, s>>>(part, nchunk, ne, cw, kw, c2, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_reduce(const double* part, int nchunk, int npts, const double* kw, double c2, double* out,
+                               hipStream_t s) {
+  field_reduce_kernel<<<(npts + 255) / 256, 256, 0, s>>>(part, nchunk, npts, kw, c2, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pd_reduce(const double* part, int nchunk, int ns, double scale, int directional, double* out,
+                            hipStream_t s) {
+  pd_reduce_kernel<<<(ns + 255) / 256, 256, 0, s>>>(part, nchunk, ns, scale, directional, out);
+  return hipGetLastError();
+}
+
+}  // namespace srs

@@ -1,0 +1,57 @@
+// Host-side launch interface of the radiation kernels (internal to libsrs_hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "srs_device.h"
+
+namespace srs {
+
+constexpr int kMaxFluxEnergies = 4;   // energies handled per flux launch
+constexpr int kBlock = 256;           // threads per block (4 wave64)
+
+// Per-energy constants of the spectrum/flux integrand, computed on the host in the
+// reference's operation order: omega = (E*Qe)/Hbar, cw = C/omega, kw = (C0*omega)*dt.
+struct EnergySet {
+  int ne;
+  double omega[kMaxFluxEnergies];
+  double cw[kMaxFluxEnergies];
+  double kw[kMaxFluxEnergies];
+};
+
+// ---- plain FP64 form ------------------------------------------------------
+// Flux partial sums: part[((c*ne + e)*6 + k)*ns + j], c = trajectory chunk.
+hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,
+                            double* part, hipStream_t s);
+// Spectrum partial sums: part[(c*12 + k)*ne + e] (12 = A and G sums, re/im, xyz).
+hipError_t launch_spectrum_fp64(const TrajView& t, const double obs[3], const double* omega, const double* cw, int ne,
+                                int chunk, double* part, hipStream_t s);
+// Power density partial sums: part[c*ns + j].
+hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double* nrm_soa, int ns, int directional,
+                          int chunk, double* part, hipStream_t s);
+
+// ---- mixed form (FP64 phase, FP32 trig / amplitudes / tile sums) -----------
+hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
+                             int chunk, double* part, hipStream_t s);
+// Mixed spectrum partials: complex field sums part[(c*6 + q)*ne + e]; a lane owns
+// spectrum_mixed_run() consecutive energies.
+int spectrum_mixed_run();
+hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const double* omega, const double* cw, int ne,
+                                 int chunk, int anchor, double* part, hipStream_t s);
+hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
+                           int directional, int chunk, double* part, hipStream_t s);
+// Single-precision copies used by the mixed kernels: trajf[k*n + i] for
+// k = 0..8: x, y, z (relative), bx, by, (1 - bz), ax, ay, az.
+hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s);
+
+// ---- reductions over trajectory chunks (fixed order, deterministic) --------
+hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
+                              hipStream_t s);
+hipError_t launch_spectrum_reduce(const double* part, int nchunk, int ne, const double* cw, const double* kw, double c2,
+                                  double* out, hipStream_t s);
+hipError_t launch_field_reduce(const double* part, int nchunk, int npts, const double* kw, double c2, double* out,
+                               hipStream_t s);
+hipError_t launch_pd_reduce(const double* part, int nchunk, int ns, double scale, int directional, double* out,
+                            hipStream_t s);
+
+}  // namespace srs

@@ -1,0 +1,344 @@
+// Mixed-precision radiation kernels for gfx950.
+//
+// FP64 is kept where stability needs it: the observer distance D and the
+// retarded time tau = dt*i + D/c, the phase omega*tau (up to 5e11 rad) and its
+// reduction to [-pi, pi].  Everything after the reduction runs in FP32: sin/cos
+// of the reduced phase (v_sin_f32 / v_cos_f32), the amplitude terms and the
+// accumulation.  Cancellation-prone differences are rewritten so that FP32 sees
+// only well-conditioned quantities:
+//   beta_z - n_z = (1 - n_z) - (1 - beta_z),  1 - beta_z stored per point (FP64 -> FP32),
+//   1 - n_z = (n_x^2 + n_y^2) / (1 + n_z)          for n_z > 0.
+// Sums run in FP32 over 32-point sub-tiles; the sub-tile sums are added to FP32
+// accumulators with Kahan compensation; each trajectory chunk writes FP64 partial
+// sums and the cross-chunk reduction is FP64 (shared with the plain form).
+#include "srs_kernels.h"
+
+namespace srs {
+namespace {
+
+constexpr int TILE = kBlock;
+constexpr int SUB = 32;  // FP32 sub-tile length
+
+__device__ __forceinline__ double sqrt_fast_fp64(double d2) {
+  // sqrt via the hardware reciprocal square root plus one Newton step (<= ~1 ulp).
+  double const y = __builtin_amdgcn_rsq(d2);
+  double D = d2 * y;
+  double const e = __builtin_fma(-D, D, d2);
+  return __builtin_fma(e, 0.5 * y, D);
+}
+
+__device__ __forceinline__ void sincos_reduced_f32(double r, float* s, float* c) {
+  // r in [-pi, pi] -> revolutions for the hardware sin/cos
+  float const rev = (float)(r * 0.15915494309189535);
+  *s = __builtin_amdgcn_sinf(rev);
+  *c = __builtin_amdgcn_cosf(rev);
+}
+
+struct Kahan {
+  float s, c;
+  __device__ __forceinline__ void add(float v) {
+    float const y = v - c;
+    float const t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+// trajf layout (float): [0] bx, [1] by, [2] 1 - bz, [3] ax, [4] ay, [5] az
+__global__ void traj_to_f32_kernel(TrajView t, float* __restrict__ f) {
+  int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= t.n) return;
+  f[0 * t.n + i] = (float)t.bx[i];
+  f[1 * t.n + i] = (float)t.by[i];
+  f[2 * t.n + i] = (float)(1.0 - t.bz[i]);
+  f[3 * t.n + i] = (float)t.ax[i];
+  f[4 * t.n + i] = (float)t.ay[i];
+  f[5 * t.n + i] = (float)t.az[i];
+}
+
+template <int NE>
+__global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const float* __restrict__ tf,
+                                                             const double* __restrict__ obs, int ns, EnergySet es,
+                                                             int chunk, double* __restrict__ part) {
+  __shared__ double sx[TILE], sy[TILE], sz[TILE], st[TILE];
+  __shared__ float sbx[TILE], sby[TILE], somb[TILE];
+  int const tid = threadIdx.x;
+  int const j = blockIdx.x * kBlock + tid;
+  int const jj = j < ns ? j : ns - 1;
+  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+  float cwf[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) cwf[e] = (float)es.cw[e];
+  int64_t const i0 = (int64_t)blockIdx.y * chunk;
+  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  Kahan acc[NE][6];
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[e][k] = Kahan{0.f, 0.f};
+
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = base + tid;
+      sx[tid] = t.x[i];
+      sy[tid] = t.y[i];
+      sz[tid] = t.z[i];
+      st[tid] = t.dt * (double)i;
+      sbx[tid] = tf[i];
+      sby[tid] = tf[t.n + i];
+      somb[tid] = tf[2 * t.n + i];
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < cnt; k0 += SUB) {
+      int const k1 = (k0 + SUB < cnt) ? k0 + SUB : cnt;
+      float sub[NE][6];
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sub[e][q] = 0.f;
+      for (int k = k0; k < k1; ++k) {
+        double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
+        double const d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, rz * rz));
+        double const D = sqrt_fast_fp64(d2);
+        double const tau = __builtin_fma(D, 1.0 / kC, st[k]);
+        float const inv = __builtin_amdgcn_rcpf((float)D);
+        float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
+        float const v = __builtin_fmaf(nx, nx, ny * ny);
+        float const omnz = nz > 0.f ? v * __builtin_amdgcn_rcpf(1.f + nz) : 1.f - nz;
+        float const ax = (sbx[k] - nx) * inv, ay = (sby[k] - ny) * inv, az = (omnz - somb[k]) * inv;
+        float const g = inv * inv;
+        float const gx = nx * g, gy = ny * g, gz = nz * g;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+          float s, c;
+          sincos_reduced_f32(reduce_two_pi(es.omega[e] * tau), &s, &c);
+          float const cs = cwf[e] * s, cc = cwf[e] * c;
+          sub[e][0] = __builtin_fmaf(ax, c, __builtin_fmaf(gx, cs, sub[e][0]));
+          sub[e][1] = __builtin_fmaf(ax, s, __builtin_fmaf(-gx, cc, sub[e][1]));
+          sub[e][2] = __builtin_fmaf(ay, c, __builtin_fmaf(gy, cs, sub[e][2]));
+          sub[e][3] = __builtin_fmaf(ay, s, __builtin_fmaf(-gy, cc, sub[e][3]));
+          sub[e][4] = __builtin_fmaf(az, c, __builtin_fmaf(gz, cs, sub[e][4]));
+          sub[e][5] = __builtin_fmaf(az, s, __builtin_fmaf(-gz, cc, sub[e][5]));
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[e][q].add(sub[e][q]);
+    }
+  }
+  if (j < ns) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = (double)acc[e][q].s - (double)acc[e][q].c;
+    }
+  }
+}
+
+// Spectrum with the energy-grid recurrence: a lane owns RUN consecutive energies.
+// For each trajectory point it anchors exp(i omega_0 tau) with an FP64-reduced
+// phase and steps exp(i d_omega tau) across the run (FP64-reduced as well); the
+// run restarts from a fresh FP64 anchor, so rounding drift is bounded by RUN steps.
+// Uniform steps are exact to the grid's own rounding (the reference computes each
+// energy separately, TSpectrumContainer.cc:82); d_omega is the run's mean spacing.
+constexpr int RUN = 8;
+
+__global__ __launch_bounds__(kBlock) void spectrum_mixed_kernel(TrajView t, double ox, double oy, double oz,
+                                                                const double* __restrict__ omega, int ne, int chunk,
+                                                                double* __restrict__ part) {
+  __shared__ double stau[TILE];
+  __shared__ float sa[3][TILE], sg[3][TILE];
+  int const tid = threadIdx.x;
+  int const e0 = (blockIdx.x * kBlock + tid) * RUN;
+  int const ea = e0 < ne ? e0 : ne - 1;
+  int const eb = (e0 + RUN - 1 < ne) ? e0 + RUN - 1 : ne - 1;
+  double const w0 = omega[ea];
+  double const dw = eb > ea ? (omega[eb] - omega[ea]) / (double)(eb - ea) : 0.0;
+  int64_t const i0 = (int64_t)blockIdx.y * chunk;
+  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  float cw[RUN];
+#pragma unroll
+  for (int r = 0; r < RUN; ++r) {
+    int const e = (e0 + r < ne) ? e0 + r : ne - 1;
+    cw[r] = (float)(kC / omega[e]);
+  }
+  Kahan ex[RUN][6];
+#pragma unroll
+  for (int r = 0; r < RUN; ++r)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) ex[r][q] = Kahan{0.f, 0.f};
+
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = base + tid;
+      double const rx = ox - t.x[i], ry = oy - t.y[i], rz = oz - t.z[i];
+      double const d2 = rx * rx + ry * ry + rz * rz;
+      double const D = __builtin_sqrt(d2);
+      stau[tid] = t.dt * (double)i + D / kC;
+      double const inv = 1.0 / D;
+      double const nx = rx * inv, ny = ry * inv, nz = rz * inv;
+      sa[0][tid] = (float)((t.bx[i] - nx) * inv);
+      sa[1][tid] = (float)((t.by[i] - ny) * inv);
+      sa[2][tid] = (float)((t.bz[i] - nz) * inv);
+      double const inv2 = inv * inv;
+      sg[0][tid] = (float)(nx * inv2);
+      sg[1][tid] = (float)(ny * inv2);
+      sg[2][tid] = (float)(nz * inv2);
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < cnt; k0 += SUB) {
+      int const k1 = (k0 + SUB < cnt) ? k0 + SUB : cnt;
+      float sub[RUN][6];
+#pragma unroll
+      for (int r = 0; r < RUN; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sub[r][q] = 0.f;
+      for (int k = k0; k < k1; ++k) {
+        double const tau = stau[k];
+        float ps, pc, ws, wc;
+        sincos_reduced_f32(reduce_two_pi(w0 * tau), &ps, &pc);
+        sincos_reduced_f32(reduce_two_pi(dw * tau), &ws, &wc);
+        float const a0 = sa[0][k], a1 = sa[1][k], a2 = sa[2][k];
+        float const g0 = sg[0][k], g1 = sg[1][k], g2 = sg[2][k];
+#pragma unroll
+        for (int r = 0; r < RUN; ++r) {
+          float const cs = cw[r] * ps, cc = cw[r] * pc;
+          sub[r][0] = __builtin_fmaf(a0, pc, __builtin_fmaf(g0, cs, sub[r][0]));
+          sub[r][1] = __builtin_fmaf(a0, ps, __builtin_fmaf(-g0, cc, sub[r][1]));
+          sub[r][2] = __builtin_fmaf(a1, pc, __builtin_fmaf(g1, cs, sub[r][2]));
+          sub[r][3] = __builtin_fmaf(a1, ps, __builtin_fmaf(-g1, cc, sub[r][3]));
+          sub[r][4] = __builtin_fmaf(a2, pc, __builtin_fmaf(g2, cs, sub[r][4]));
+          sub[r][5] = __builtin_fmaf(a2, ps, __builtin_fmaf(-g2, cc, sub[r][5]));
+          // advance the phasor to the next energy of the run
+          float const npc = __builtin_fmaf(pc, wc, -ps * ws);
+          float const nps = __builtin_fmaf(ps, wc, pc * ws);
+          pc = npc;
+          ps = nps;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RUN; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ex[r][q].add(sub[r][q]);
+    }
+  }
+  // partial layout: part[(c*6 + q)*ne + e] holds the complex field sums (re/im per axis)
+#pragma unroll
+  for (int r = 0; r < RUN; ++r) {
+    int const e = e0 + r;
+    if (e < ne) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) part[((size_t)blockIdx.y * 6 + q) * ne + e] = (double)ex[r][q].s - (double)ex[r][q].c;
+    }
+  }
+}
+
+// Mixed power density: D and n in FP64 -> FP32 with 1 - n_z and 1 - beta_z
+// carried explicitly; the denominator 1 - beta.n (~1e-7) is assembled from small
+// well-conditioned pieces: 1 - b.n = omb + omnz - omb*omnz - bx nx - by ny.
+__global__ __launch_bounds__(kBlock) void pd_mixed_kernel(TrajView t, const float* __restrict__ tf,
+                                                          const double* __restrict__ obs, const double* __restrict__ nrm,
+                                                          int ns, int directional, int chunk, double* __restrict__ part) {
+  __shared__ double sx[TILE], sy[TILE], sz[TILE];
+  __shared__ float sbx[TILE], sby[TILE], somb[TILE], sax[TILE], say[TILE], saz[TILE];
+  int const tid = threadIdx.x;
+  int const j = blockIdx.x * kBlock + tid;
+  int const jj = j < ns ? j : ns - 1;
+  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+  float const qx = (float)nrm[jj], qy = (float)nrm[ns + jj], qz = (float)nrm[2 * ns + jj];
+  int64_t const i0 = (int64_t)blockIdx.y * chunk;
+  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  Kahan acc{0.f, 0.f};
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = base + tid;
+      sx[tid] = t.x[i];
+      sy[tid] = t.y[i];
+      sz[tid] = t.z[i];
+      sbx[tid] = tf[i];
+      sby[tid] = tf[t.n + i];
+      somb[tid] = tf[2 * t.n + i];
+      sax[tid] = tf[3 * t.n + i];
+      say[tid] = tf[4 * t.n + i];
+      saz[tid] = tf[5 * t.n + i];
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < cnt; k0 += SUB) {
+      int const k1 = (k0 + SUB < cnt) ? k0 + SUB : cnt;
+      float sub = 0.f;
+      for (int k = k0; k < k1; ++k) {
+        double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
+        double const d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, rz * rz));
+        double const D = sqrt_fast_fp64(d2);
+        float const inv = __builtin_amdgcn_rcpf((float)D);
+        float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
+        float const ndot = __builtin_fmaf(nx, qx, __builtin_fmaf(ny, qy, nz * qz));
+        if (directional && ndot <= 0.f) continue;
+        float const v = __builtin_fmaf(nx, nx, ny * ny);
+        float const omnz = nz > 0.f ? v * __builtin_amdgcn_rcpf(1.f + nz) : 1.f - nz;
+        float const bx = sbx[k], by = sby[k], omb = somb[k];
+        float const den = __builtin_fmaf(-omb, omnz, omb + omnz) - __builtin_fmaf(bx, nx, by * ny);
+        // u = n - beta with u_z = omb - omnz
+        float const ux = nx - bx, uy = ny - by, uz = omb - omnz;
+        float const Ax = sax[k], Ay = say[k], Az = saz[k];
+        float const vx = uy * Az - Ay * uz, vy = uz * Ax - Az * ux, vz = ux * Ay - Ax * uy;
+        float const mx = ny * vz - vy * nz, my = nz * vx - vz * nx, mz = nx * vy - vx * ny;
+        float const num2 = __builtin_fmaf(mx, mx, __builtin_fmaf(my, my, mz * mz));
+        float const den2 = den * den;
+        float const den5 = den2 * den2 * den;
+        sub += num2 * ndot * (inv * inv) / den5;
+      }
+      acc.add(sub);
+    }
+  }
+  if (j < ns) part[(size_t)blockIdx.y * ns + j] = (double)acc.s - (double)acc.c;
+}
+
+inline int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk); }
+
+}  // namespace
+
+hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s) {
+  traj_to_f32_kernel<<<(unsigned)((t.n + 255) / 256), 256, 0, s>>>(t, trajf);
+  return hipGetLastError();
+}
+
+hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
+                             int chunk, double* part, hipStream_t s) {
+  dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  switch (es.ne) {
+    case 1: flux_mixed_kernel<1><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 2: flux_mixed_kernel<2><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 3: flux_mixed_kernel<3><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 4: flux_mixed_kernel<4><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int spectrum_mixed_run() { return RUN; }
+
+hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const double* omega, const double*, int ne,
+                                 int chunk, int, double* part, hipStream_t s) {
+  int const lanes = (ne + RUN - 1) / RUN;
+  dim3 grid((lanes + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  spectrum_mixed_kernel<<<grid, kBlock, 0, s>>>(t, obs[0], obs[1], obs[2], omega, ne, chunk, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
+                           int directional, int chunk, double* part, hipStream_t s) {
+  dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  pd_mixed_kernel<<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, nrm_soa, ns, directional, chunk, part);
+  return hipGetLastError();
+}
+
+}  // namespace srs

@@ -1,0 +1,192 @@
+// pybind11 binding of the host engine: srsoriginal_amd.lib._sr
+//
+// The user-facing `sr` type (srsoriginal_amd/sr.py) mirrors the reference's
+// CPython type oscars.sr.sr (OSCARSSR_Python.cc:3670-3747) and calls into this
+// module.  Arrays cross as NumPy float64; C++ exceptions map to ValueError like
+// the reference's (std::length_error / out_of_range / invalid_argument ->
+// PyExc_ValueError, e.g. OSCARSSR_Python.cc:2329-2338).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "srs_engine.h"
+
+namespace py = pybind11;
+using srs::Vec3;
+
+namespace {
+
+Vec3 v3(std::vector<double> const& v) {
+  if (v.size() != 3) throw std::length_error("number of elements not 3");
+  return Vec3(v[0], v[1], v[2]);
+}
+
+py::array_t<double> traj_array(srs::Trajectory const& T) {
+  size_t const n = T.size();
+  py::array_t<double> a({(py::ssize_t)9, (py::ssize_t)n});
+  auto m = a.mutable_unchecked<2>();
+  std::vector<double> const* src[9] = {&T.x, &T.y, &T.z, &T.bx, &T.by, &T.bz, &T.ax, &T.ay, &T.az};
+  for (int k = 0; k < 9; ++k)
+    for (size_t i = 0; i < n; ++i) m(k, i) = (*src[k])[i];
+  return a;
+}
+
+py::tuple container_arrays(srs::ScalarContainer const& C) {
+  size_t const n = C.size();
+  py::array_t<double> x({(py::ssize_t)n, (py::ssize_t)3});
+  py::array_t<double> v((py::ssize_t)n);
+  auto mx = x.mutable_unchecked<2>();
+  auto mv = v.mutable_unchecked<1>();
+  for (size_t i = 0; i < n; ++i) {
+    mx(i, 0) = C.xyz[3 * i];
+    mx(i, 1) = C.xyz[3 * i + 1];
+    mx(i, 2) = C.xyz[3 * i + 2];
+    mv(i) = C.v[i];
+  }
+  return py::make_tuple(x, v);
+}
+
+srs::Precision prec_of(int p) {
+  if (p != 0 && p != 1) throw std::invalid_argument("'precision' must be 'double' or 'mixed'");
+  return static_cast<srs::Precision>(p);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_sr, m) {
+  m.doc() = "srsoriginal_amd host engine (OSCARSSR-compatible) over the gfx950 kernels";
+  py::register_exception_translator([](std::exception_ptr p) {
+    try {
+      if (p) std::rethrow_exception(p);
+    } catch (std::length_error const& e) {
+      PyErr_SetString(PyExc_ValueError, e.what());
+    } catch (std::out_of_range const& e) {
+      PyErr_SetString(PyExc_ValueError, e.what());
+    } catch (std::invalid_argument const& e) {
+      PyErr_SetString(PyExc_ValueError, e.what());
+    }
+  });
+
+  py::class_<srs::SurfacePoints>(m, "Surface")
+      .def_static("rectangle",
+                  [](std::string const& plane, int nx1, int nx2, double w1, double w2, std::vector<double> rot,
+                     std::vector<double> trans, int normal) {
+                    return srs::SurfacePoints::rectangle(plane, nx1, nx2, w1, w2, v3(rot), v3(trans), normal);
+                  })
+      .def_static("rectangle3",
+                  [](int nx1, int nx2, std::vector<double> X0, std::vector<double> X1, std::vector<double> X2,
+                     int normal) { return srs::SurfacePoints::rectangle3(nx1, nx2, v3(X0), v3(X1), v3(X2), normal); })
+      .def_static("points",
+                  [](py::array_t<double, py::array::c_style | py::array::forcecast> xyz,
+                     py::array_t<double, py::array::c_style | py::array::forcecast> nrm) {
+                    srs::SurfacePoints S;
+                    auto a = xyz.unchecked<2>();
+                    auto b = nrm.unchecked<2>();
+                    for (py::ssize_t i = 0; i < a.shape(0); ++i)
+                      S.add(Vec3(a(i, 0), a(i, 1), a(i, 2)), Vec3(b(i, 0), b(i, 1), b(i, 2)));
+                    return S;
+                  })
+      .def_property_readonly("npoints", &srs::SurfacePoints::size)
+      .def_property_readonly("xyz", [](srs::SurfacePoints const& S) {
+        py::array_t<double> a({(py::ssize_t)S.size(), (py::ssize_t)3});
+        std::copy(S.xyz.begin(), S.xyz.end(), a.mutable_data());
+        return a;
+      })
+      .def_property_readonly("normals", [](srs::SurfacePoints const& S) {
+        py::array_t<double> a({(py::ssize_t)S.size(), (py::ssize_t)3});
+        std::copy(S.normals.begin(), S.normals.end(), a.mutable_data());
+        return a;
+      });
+
+  py::class_<srs::OSCARSSR>(m, "Engine")
+      .def(py::init<>())
+      .def("add_undulator",
+           [](srs::OSCARSSR& e, std::vector<double> field, std::vector<double> period, int nperiods, double phase,
+              double taper, std::vector<double> center, std::vector<double> pole_deltas) {
+             srs::IdealUndulator u;
+             u.field = v3(field);
+             u.period = v3(period);
+             u.nperiods = nperiods;
+             u.phase = phase;
+             u.taper = taper;
+             u.center = v3(center);
+             if (!pole_deltas.empty()) {
+               if ((int)pole_deltas.size() != 2 * nperiods)
+                 throw std::invalid_argument("'pole_scales' must have 2*nperiods entries");
+               u.pole_delta = pole_deltas;
+             }
+             e.AddMagneticField(u);
+           })
+      .def("clear_bfields", &srs::OSCARSSR::ClearMagneticFields)
+      .def("n_bfields", [](srs::OSCARSSR const& e) { return e.Fields().size(); })
+      .def("get_bfield",
+           [](srs::OSCARSSR const& e, std::vector<double> x) {
+             Vec3 const b = e.GetB(v3(x));
+             return std::vector<double>{b.x, b.y, b.z};
+           })
+      .def("add_particle_beam",
+           [](srs::OSCARSSR& e, std::string const& type, std::string const& name, std::vector<double> x0,
+              std::vector<double> d0, double energy, double t0, double current, double weight) {
+             e.AddParticleBeam(type, name, v3(x0), v3(d0), energy, t0, current, weight);
+           })
+      .def("set_sigma",
+           [](srs::OSCARSSR& e, std::string const& name, std::vector<double> h, std::vector<double> su,
+              std::vector<double> sup, std::vector<double> at, double se) {
+             double a[2] = {su.at(0), su.at(1)}, b[2] = {sup.at(0), sup.at(1)};
+             e.GetParticleBeam(name).set_sigma(v3(h), a, b, v3(at), se);
+           })
+      .def("clear_particle_beams", &srs::OSCARSSR::ClearParticleBeams)
+      .def("n_beams", &srs::OSCARSSR::GetNParticleBeams)
+      .def("set_new_particle", [](srs::OSCARSSR& e) { e.SetNewParticle(); })
+      .def("set_new_particle_from", [](srs::OSCARSSR& e, std::string const& b, std::string const& w) { e.SetNewParticle(b, w); })
+      .def("particle_x0", [](srs::OSCARSSR const& e) { auto v = e.GetCurrentParticle().x0; return std::vector<double>{v.x, v.y, v.z}; })
+      .def("particle_beta0", [](srs::OSCARSSR const& e) { auto v = e.GetCurrentParticle().beta0; return std::vector<double>{v.x, v.y, v.z}; })
+      .def("particle_e0", [](srs::OSCARSSR const& e) { return e.GetCurrentParticle().e0_gev(); })
+      .def("particle_gamma", [](srs::OSCARSSR const& e) { return e.GetCurrentParticle().gamma; })
+      .def("particle_charge_current", [](srs::OSCARSSR const& e) {
+        return std::vector<double>{e.GetCurrentParticle().q, e.GetCurrentParticle().current};
+      })
+      .def("particle_type", [](srs::OSCARSSR const& e) { return e.GetCurrentParticle().type; })
+      .def("set_seed", &srs::OSCARSSR::SetSeed)
+      .def("rand", [](srs::OSCARSSR const&) { return srs::Random::global().uniform(); })
+      .def("norm", [](srs::OSCARSSR const&) { return srs::Random::global().normal(); })
+      .def("set_ctstartstop", &srs::OSCARSSR::SetCTStartStop)
+      .def("set_npoints_trajectory", &srs::OSCARSSR::SetNPointsTrajectory)
+      .def("set_npoints_per_meter", &srs::OSCARSSR::SetNPointsPerMeter)
+      .def("get_npoints_trajectory", &srs::OSCARSSR::GetNPointsTrajectory)
+      .def("get_ctstart", &srs::OSCARSSR::GetCTStart)
+      .def("get_ctstop", &srs::OSCARSSR::GetCTStop)
+      .def("set_nthreads_global", &srs::OSCARSSR::SetNThreadsGlobal)
+      .def("set_gpu_global", &srs::OSCARSSR::SetUseGPUGlobal)
+      .def("check_gpu", &srs::OSCARSSR::CheckGPU)
+      .def("calculate_trajectory",
+           [](srs::OSCARSSR& e) {
+             e.CalculateTrajectory();
+             return py::make_tuple(traj_array(e.GetTrajectory()), e.GetTrajectory().dt);
+           })
+      .def("get_trajectory", [](srs::OSCARSSR const& e) { return py::make_tuple(traj_array(e.GetTrajectory()), e.GetTrajectory().dt); })
+      .def("calculate_spectrum",
+           [](srs::OSCARSSR& e, std::vector<double> obs, std::vector<double> points, size_t n, double e0, double e1,
+              int nparticles, int nthreads, int gpu, int precision) {
+             srs::SpectrumContainer S;
+             if (points.empty()) S.init(n, e0, e1);
+             else S.init(points);
+             e.CalculateSpectrum(v3(obs), S, nparticles, nthreads, gpu, prec_of(precision));
+             return py::make_tuple(py::array_t<double>(S.e.size(), S.e.data()), py::array_t<double>(S.f.size(), S.f.data()));
+           })
+      .def("calculate_flux",
+           [](srs::OSCARSSR& e, srs::SurfacePoints const& S, double energy, int nparticles, int nthreads, int gpu,
+              int dim, int precision) {
+             srs::ScalarContainer C;
+             e.CalculateFlux(S, energy, C, nparticles, nthreads, gpu, dim, prec_of(precision));
+             return container_arrays(C);
+           })
+      .def("calculate_power_density",
+           [](srs::OSCARSSR& e, srs::SurfacePoints const& S, int dim, bool directional, int nparticles, int nthreads,
+              int gpu, int precision) {
+             srs::ScalarContainer C;
+             e.CalculatePowerDensity(S, C, dim, directional, nparticles, nthreads, gpu, prec_of(precision));
+             return container_arrays(C);
+           })
+      .def("calculate_total_power", &srs::OSCARSSR::CalculateTotalPower);
+}

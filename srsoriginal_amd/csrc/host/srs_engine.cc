@@ -1,0 +1,641 @@
+// Host engine (see srs_engine.h).  Arithmetic that the GPU integrals depend on
+// bit for bit (field, RK4, particle set-up, grids) keeps the reference's
+// operation order; this file is compiled without FMA contraction and without
+// -march so that it rounds exactly as the reference's x86-64 build does.
+#include "srs_engine.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+
+#include "srs_hip.h"
+
+namespace srs {
+namespace {
+
+// Reference constants, TOSCARSSR.h:20-69.
+const double kPi = 3.14159265358979323846;
+const double kTwoPi = 2.0 * kPi;
+const double kC = 2.99792458e8;
+const double kQe = 1.602176462e-19;
+const double kMe = 9.10938356e-31;
+
+inline double kg_to_gev(double m) { return 1e-9 * m * kC * kC / kQe; }
+inline double gev_to_kg(double m) { return kQe * m / (1e-9 * kC * kC); }
+
+inline Vec3 sub(Vec3 const& a, Vec3 const& b) { return Vec3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline Vec3 vadd(Vec3 const& a, Vec3 const& b) { return Vec3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline double mag2(Vec3 const& a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline double mag(Vec3 const& a) { return std::sqrt(mag2(a)); }
+inline double dot(Vec3 const& a, Vec3 const& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 scale(Vec3 const& a, double s) { return Vec3(a.x * s, a.y * s, a.z * s); }
+inline Vec3 divide(Vec3 const& a, double d) { return Vec3(a.x / d, a.y / d, a.z / d); }
+inline Vec3 unit(Vec3 const& a) { return divide(a, mag(a)); }
+inline Vec3 cross(Vec3 const& a, Vec3 const& b) {
+  return Vec3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+inline void rotate_x(Vec3& v, double a) {
+  double const s = std::sin(a), c = std::cos(a), yy = v.y;
+  v.y = c * yy - s * v.z;
+  v.z = s * yy + c * v.z;
+}
+inline void rotate_y(Vec3& v, double a) {
+  double const s = std::sin(a), c = std::cos(a), zz = v.z;
+  v.z = c * zz - s * v.x;
+  v.x = s * zz + c * v.x;
+}
+inline void rotate_z(Vec3& v, double a) {
+  double const s = std::sin(a), c = std::cos(a), xx = v.x;
+  v.x = c * xx - s * v.y;
+  v.y = s * xx + c * v.y;
+}
+inline void rotate_xyz(Vec3& v, Vec3 const& r) {
+  rotate_x(v, r.x);
+  rotate_y(v, r.y);
+  rotate_z(v, r.z);
+}
+inline std::string lower(std::string s) {
+  std::transform(s.begin(), s.end(), s.begin(), ::tolower);
+  return s;
+}
+inline std::string upper(std::string s) {
+  std::transform(s.begin(), s.end(), s.begin(), ::toupper);
+  return s;
+}
+
+void check_status(int rc) {
+  if (rc == SRS_OK) return;
+  std::string msg = srs_hip_last_error();
+  if (rc == SRS_ERR_INVALID) throw std::length_error(msg);
+  throw std::invalid_argument(msg);
+}
+
+srs_traj view(Trajectory const& t) {
+  srs_traj v;
+  v.n = (int64_t)t.size();
+  v.dt = t.dt;
+  v.x = t.x.data();
+  v.y = t.y.data();
+  v.z = t.z.data();
+  v.bx = t.bx.data();
+  v.by = t.by.data();
+  v.bz = t.bz.data();
+  v.ax = t.ax.data();
+  v.ay = t.ay.data();
+  v.az = t.az.data();
+  return v;
+}
+
+}  // namespace
+
+// ---- field ---------------------------------------------------------------------
+
+Vec3 IdealUndulator::eval(Vec3 const& X) const {
+  double const lam = mag(period);
+  Vec3 const u = divide(period, mag(period));
+  double const len = mag(period) * (nperiods + 2);
+  if (!pole_delta.empty()) {
+    // interior half-period poles scaled by (1 + delta_p); phase and taper unused
+    double const D = dot(X, u);
+    double const s = std::sin(2.0 * kPi * D / lam);
+    if (D > len / 2. || D < -len / 2.) return Vec3();
+    if (D < -len / 2. + lam || D > len / 2. - lam) {
+      if (D < -len / 2. + lam / 2. || D > len / 2. - lam / 2.)
+        return Vec3((0.25 * field.x) * s, (0.25 * field.y) * s, (0.25 * field.z) * s);
+      return Vec3((0.75 * field.x) * s, (0.75 * field.y) * s, (0.75 * field.z) * s);
+    }
+    int p = (int)std::floor((D - (-len / 2. + lam)) / (lam / 2.));
+    if (p < 0) p = 0;
+    if (p > 2 * nperiods - 1) p = 2 * nperiods - 1;
+    double const sc = 1.0 + pole_delta[(size_t)p];
+    return Vec3((field.x * s) * sc, (field.y * s) * sc, (field.z * s) * sc);
+  }
+  // TField3D_IdealUndulator::GetF, .cc:106-139
+  double const D = dot(sub(X, center), u);
+  double const T = 1 + D * taper;
+  double const PS = phase * mag(period) / kTwoPi;
+  if (D > len / 2. + PS || D < -len / 2. + PS) return Vec3();
+  double const s = std::sin(kTwoPi * (D - PS) / lam);
+  if (D < -len / 2. + PS + lam || D > len / 2. + PS - lam) {
+    double const f = (D < -len / 2. + PS + lam / 2. || D > len / 2. + PS - lam / 2.) ? 0.25 : 0.75;
+    return Vec3(((field.x * f) * s) * T, ((field.y * f) * s) * T, ((field.z * f) * s) * T);
+  }
+  return Vec3((field.x * s) * T, (field.y * s) * T, (field.z * s) * T);
+}
+
+Vec3 FieldContainer::eval(Vec3 const& X) const {
+  Vec3 S(0, 0, 0);
+  for (auto const& f : fields_) {
+    Vec3 const b = f.eval(X);
+    S.x += b.x;
+    S.y += b.y;
+    S.z += b.z;
+  }
+  return S;
+}
+
+// ---- particles -------------------------------------------------------------------
+
+void Trajectory::clear() {
+  for (auto* v : {&x, &y, &z, &bx, &by, &bz, &ax, &ay, &az}) v->clear();
+  dt = 0;
+}
+
+void Particle::set_type(std::string const& t) {
+  type = t;
+  std::string const l = lower(t);
+  if (l == "electron" || l == "anti-positron") {
+    q = -kQe; m = kMe;
+  } else if (l == "positron" || l == "anti-electron") {
+    q = kQe; m = kMe;
+  } else if (l == "muon") {
+    q = -kQe; m = gev_to_kg(0.1056583715);
+  } else if (l == "anti-muon") {
+    q = kQe; m = gev_to_kg(0.1056583715);
+  } else if (l == "proton") {
+    q = kQe; m = gev_to_kg(0.938272046);
+  } else if (l == "anti-proton") {
+    q = -kQe; m = gev_to_kg(0.938272046);
+  } else if (l == "pi+") {
+    q = kQe; m = gev_to_kg(0.13957018);
+  } else if (l == "pi-") {
+    q = -kQe; m = gev_to_kg(0.13957018);
+  } else if (l == "custom") {
+  } else {
+    throw std::invalid_argument("particle type not found: " + t);
+  }
+}
+
+void Particle::set_current(double c) { current = c != 0 ? c : q; }
+
+void Particle::set_initial(Vec3 const& X, Vec3 const& B, double T) {
+  x0 = X;
+  beta0 = B;
+  t0 = T;
+  double const b2 = mag2(beta0);
+  if (b2 == 1) return;
+  gamma = 1.0 / std::sqrt(1.0 - b2);
+  if (m == 0 || gamma == 0) return;
+  qmg = q / m / gamma;
+}
+
+double Particle::e0_gev() const { return kg_to_gev(m) * gamma; }
+
+Particle ParticleBeam::ideal() const {
+  double const g = e0 / kg_to_gev(proto.m);
+  double const b = std::sqrt(1.0 - 1.0 / (g * g));
+  Particle p = proto;
+  p.set_initial(x0, scale(u0, b), t0);
+  return p;
+}
+
+Particle ParticleBeam::random() const {
+  Random& R = Random::global();
+  double const enew = e0 + sigma_e * R.normal();
+  double const g = enew / kg_to_gev(proto.m);
+  double const b = std::sqrt(1.0 - 1.0 / (g * g));
+  double const dmid = dot(sub(sigma_at, x0), u0);
+  Vec3 X = x0;
+  X = vadd(X, scale(scale(scale(hdir, sigma_u[0]), 1 + dmid), R.normal()));
+  X = vadd(X, scale(scale(scale(vdir, sigma_u[1]), 1 + dmid), R.normal()));
+  Vec3 B = scale(u0, b);
+  rotate_y(B, sigma_up[0] * R.normal());
+  rotate_x(B, -sigma_up[1] * R.normal());
+  Particle p = proto;
+  p.traj.clear();
+  p.set_initial(X, B, t0);
+  return p;
+}
+
+void ParticleBeam::set_sigma(Vec3 const& h, const double su[2], const double sup[2], Vec3 const& at, double se) {
+  hdir = unit(h);
+  sigma_u[0] = su[0];
+  sigma_u[1] = su[1];
+  sigma_up[0] = sup[0];
+  sigma_up[1] = sup[1];
+  sigma_at = at;
+  sigma_e = se;
+  vdir = unit(cross(u0, hdir));
+  if (std::fabs(dot(hdir, u0)) > 0.00000001)
+    throw std::invalid_argument("horizontal_direction must be orthogonal to the beam direction");
+}
+
+Random::Random() {
+  std::random_device rd;
+  mt_.reset(new std::mt19937(rd()));
+}
+Random& Random::global() {
+  static Random r;
+  return r;
+}
+void Random::seed(int s) { mt_.reset(new std::mt19937(s)); }
+double Random::normal() { return normal_(*mt_); }
+double Random::uniform() { return uniform_(*mt_); }
+
+// ---- surfaces and containers ---------------------------------------------------
+
+SurfacePoints SurfacePoints::rectangle(std::string const& plane, int nx1, int nx2, double w1, double w2, Vec3 const& rot,
+                                       Vec3 const& trans, int normal) {
+  // TSurfacePoints_Rectangle::Init(plane, ...), .cc:130-210; GetXYZ :222-230; GetX1/X2 :244-259
+  std::string const P = upper(plane);
+  double const s1 = w1 / (nx1 - 1), s2 = w2 / (nx2 - 1);
+  Vec3 st, a1, a2;
+  if (P == "XY") {
+    st = Vec3(-w1 / 2., -w2 / 2., 0); a1 = Vec3(s1, 0, 0); a2 = Vec3(0, s2, 0);
+  } else if (P == "YX") {
+    st = Vec3(-w2 / 2., -w1 / 2., 0); a2 = Vec3(s2, 0, 0); a1 = Vec3(0, s1, 0);
+  } else if (P == "XZ") {
+    st = Vec3(-w1 / 2., 0, -w2 / 2.); a1 = Vec3(s1, 0, 0); a2 = Vec3(0, 0, s2);
+  } else if (P == "ZX") {
+    st = Vec3(-w2 / 2., 0, -w1 / 2.); a2 = Vec3(s2, 0, 0); a1 = Vec3(0, 0, s1);
+  } else if (P == "YZ") {
+    st = Vec3(0, -w1 / 2., -w2 / 2.); a1 = Vec3(0, s1, 0); a2 = Vec3(0, 0, s2);
+  } else if (P == "ZY") {
+    st = Vec3(0, -w2 / 2., -w1 / 2.); a2 = Vec3(0, s2, 0); a1 = Vec3(0, 0, s1);
+  } else {
+    throw std::invalid_argument("not a valid surface string: XY YX XZ ZX YZ ZY");
+  }
+  rotate_xyz(st, rot);
+  rotate_xyz(a1, rot);
+  rotate_xyz(a2, rot);
+  st = vadd(st, trans);
+  Vec3 nv = unit(cross(a1, a2));
+  if (normal == -1) nv = scale(nv, -1);
+  else if (normal != 0 && normal != 1) throw std::invalid_argument("normal must be -1, 0, or 1");
+  SurfacePoints S;
+  S.has_x12 = true;
+  size_t const n = (size_t)nx1 * (size_t)nx2;
+  S.xyz.resize(3 * n);
+  S.normals.resize(3 * n);
+  S.x1.resize(n);
+  S.x2.resize(n);
+  double const m1 = mag(a1), m2 = mag(a2);
+  for (size_t i = 0; i < n; ++i) {
+    int const i1 = (int)(i / nx2), i2 = (int)(i % nx2);
+    Vec3 const p = vadd(vadd(st, scale(a1, (double)i1)), scale(a2, (double)i2));
+    S.xyz[3 * i] = p.x;
+    S.xyz[3 * i + 1] = p.y;
+    S.xyz[3 * i + 2] = p.z;
+    S.normals[3 * i] = nv.x;
+    S.normals[3 * i + 1] = nv.y;
+    S.normals[3 * i + 2] = nv.z;
+    S.x1[i] = s1 * i1 - m1 * (nx1 - 1) / 2.;
+    S.x2[i] = s2 * i2 - m2 * (nx2 - 1) / 2.;
+  }
+  return S;
+}
+
+SurfacePoints SurfacePoints::rectangle3(int nx1, int nx2, Vec3 const& X0, Vec3 const& X1, Vec3 const& X2, int normal) {
+  // TSurfacePoints_Rectangle::Init(NX1, NX2, X0, X1, X2, Normal), .cc:76-123
+  double const w1 = mag(sub(X1, X0)), w2 = mag(sub(X2, X0));
+  double const s1 = w1 / (nx1 - 1), s2 = w2 / (nx2 - 1);
+  Vec3 const a1 = divide(sub(X1, X0), (double)(nx1 - 1));
+  Vec3 const a2 = divide(sub(X2, X0), (double)(nx2 - 1));
+  Vec3 nv = unit(cross(a1, a2));
+  if (normal == -1) nv = scale(nv, -1);
+  else if (normal != 0 && normal != 1) throw std::invalid_argument("normal must be -1, 0, or 1");
+  SurfacePoints S;
+  S.has_x12 = true;
+  size_t const n = (size_t)nx1 * (size_t)nx2;
+  double const m1 = mag(a1), m2 = mag(a2);
+  for (size_t i = 0; i < n; ++i) {
+    int const i1 = (int)(i / nx2), i2 = (int)(i % nx2);
+    Vec3 const p = vadd(vadd(X0, scale(a1, (double)i1)), scale(a2, (double)i2));
+    S.add(p, nv);
+    S.x1.push_back(s1 * i1 - m1 * (nx1 - 1) / 2.);
+    S.x2.push_back(s2 * i2 - m2 * (nx2 - 1) / 2.);
+  }
+  return S;
+}
+
+void SurfacePoints::add(Vec3 const& X, Vec3 const& N) {
+  xyz.insert(xyz.end(), {X.x, X.y, X.z});
+  normals.insert(normals.end(), {N.x, N.y, N.z});
+}
+
+void SpectrumContainer::init(size_t n, double e0, double e1) {
+  if (n < 1) throw std::length_error("'npoints' must be > 0");
+  e.assign(n, 0.0);
+  f.assign(n, 0.0);
+  comp.assign(n, 0.0);
+  if (n == 1) {
+    e[0] = e0;
+    return;
+  }
+  for (size_t i = 0; i != n; ++i) e[i] = e0 + (e1 - e0) / (n - 1) * (double)(i + 1);
+}
+
+void SpectrumContainer::init(std::vector<double> const& pts) {
+  e = pts;
+  f.assign(pts.size(), 0.0);
+  comp.assign(pts.size(), 0.0);
+}
+
+void SpectrumContainer::add_to(size_t i, double v) {
+  if (i >= f.size()) throw std::out_of_range("spectrum index out of range");
+  double const s = f[i];
+  double const y = v - comp[i];
+  double const t = s + y;
+  comp[i] = (t - s) - y;
+  f[i] = t;
+}
+
+void ScalarContainer::add_point(Vec3 const& X, double val) {
+  xyz.insert(xyz.end(), {X.x, X.y, X.z});
+  v.push_back(val);
+  comp.push_back(0);
+}
+
+void ScalarContainer::add_to(size_t i, double val) {
+  if (i >= v.size()) throw std::length_error("T3DScalarContainer::AddtoPoint index out of range");
+  double const s = v[i];
+  double const y = val - comp[i];
+  double const t = s + y;
+  comp[i] = (t - s) - y;
+  v[i] = t;
+}
+
+// ---- engine --------------------------------------------------------------------------
+
+OSCARSSR::OSCARSSR() {}
+
+void OSCARSSR::AddParticleBeam(std::string const& type, std::string const& name, Vec3 const& x0, Vec3 const& d0,
+                               double e_gev, double t0, double current, double weight) {
+  // TParticleBeamContainer::AddNewParticleBeam, .cc:35-58
+  if (beam_index_.count(name)) throw std::invalid_argument("beam with this name already exists");
+  ParticleBeam b;
+  b.proto.set_type(type);
+  b.proto.set_current(current);
+  b.x0 = x0;
+  b.u0 = unit(d0);
+  b.e0 = e_gev;
+  b.t0 = t0;
+  b.weight = weight;
+  beam_weight_sums_.push_back(beam_weight_sums_.empty() ? weight : beam_weight_sums_.back() + weight);
+  beams_.push_back(b);
+  beam_index_[name] = beams_.size() - 1;
+}
+
+ParticleBeam& OSCARSSR::GetParticleBeam(std::string const& name) {
+  auto it = beam_index_.find(name);
+  if (it == beam_index_.end()) throw std::out_of_range("beam name not in map");
+  return beams_[it->second];
+}
+
+void OSCARSSR::ClearParticleBeams() {
+  beams_.clear();
+  beam_weight_sums_.clear();
+  beam_index_.clear();
+}
+
+namespace {
+size_t beam_by_weight(std::vector<double> const& sums) {
+  // TParticleBeamContainer::GetRandomBeamIndexByWeight, .cc:109-142
+  size_t const n = sums.size();
+  if (n == 0) throw std::length_error("no beam defined");
+  if (n == 1) return 0;
+  double const r = Random::global().uniform() * sums[n - 1];
+  for (size_t i = 0; i != n; ++i)
+    if (r < sums[i]) return i;
+  throw std::out_of_range("random weight out of range");
+}
+}  // namespace
+
+void OSCARSSR::SetNewParticle() { particle_ = beams_[beam_by_weight(beam_weight_sums_)].random(); }
+
+void OSCARSSR::SetNewParticle(std::string const& beam, std::string const& ideal_or_random) {
+  ParticleBeam& b = beam.empty() ? beams_[beam_by_weight(beam_weight_sums_)] : GetParticleBeam(beam);
+  particle_ = lower(ideal_or_random) == "ideal" ? b.ideal() : b.random();
+}
+
+void OSCARSSR::SetNPointsPerMeter(size_t n) {
+  npoints_per_meter_ = n;
+  if (ct_start_ != ct_stop_) SetCTStartStop(ct_start_, ct_stop_);
+}
+
+void OSCARSSR::SetCTStartStop(double start, double stop) {
+  // OSCARSSR.cc:448-458: also resets the number of points
+  npoints_ = npoints_per_meter_ * (stop - start);
+  ct_start_ = start;
+  ct_stop_ = stop;
+}
+
+int OSCARSSR::CheckGPU() const { return srs_hip_device_count(); }
+
+void OSCARSSR::ensure_particle() {
+  if (particle_.type.empty()) {
+    if (beams_.empty()) throw std::out_of_range("no beam defined");
+    SetNewParticle();
+  }
+}
+
+void OSCARSSR::CalculateTrajectory() {
+  if (particle_.type.empty()) SetNewParticle();
+  CalculateTrajectory(particle_);
+}
+
+void OSCARSSR::CalculateTrajectory(Particle& P) const {
+  // OSCARSSR::CalculateTrajectory(TParticleA&), OSCARSSR.cc:562-673.  Forward pass
+  // stores the state before each step with the derivative of the previous step
+  // (A/c lags one point; the first point's A/c, uninitialised in the reference, is 0);
+  // the backward pass fills points before T0.
+  if (ct_start_ > P.t0) throw std::out_of_range("start time is greater than T0");
+  if (ct_start_ >= ct_stop_) throw std::out_of_range("start time is greater than stop time.  check that set_ctstartstop is set");
+  if (P.type.empty()) throw std::out_of_range("particle not initialized.  make sure you have a particle or beam defined");
+  if (npoints_ < 2) throw std::out_of_range("number of trajectory points must be >= 2");
+  Trajectory& T = P.traj;
+  T.clear();
+  double const dt = ((ct_stop_ - ct_start_) / kC / (npoints_ - 1));
+  size_t const nf = 1 + (ct_stop_ - P.t0) / kC / dt;
+  size_t const nb = (P.t0 - ct_start_) / kC / dt;
+  size_t const n = nf + nb;
+  T.dt = dt;
+  for (auto* v : {&T.x, &T.y, &T.z, &T.bx, &T.by, &T.bz, &T.ax, &T.ay, &T.az}) v->assign(n, 0.0);
+  double const qmg = P.qmg;
+  FieldContainer const& F = bfield_;
+  auto derivs = [&](const double* x, double* d) {
+    Vec3 const B = F.eval(Vec3(x[0], x[2], x[4]));
+    d[0] = x[1];
+    d[1] = qmg * (-x[5] * B.y + x[3] * B.z);
+    d[2] = x[3];
+    d[3] = qmg * (x[5] * B.x - x[1] * B.z);
+    d[4] = x[5];
+    d[5] = qmg * (x[1] * B.y - x[3] * B.x);
+  };
+  auto rk4 = [&](double* y, const double* dydx, double h) {
+    double dym[6], dyt[6], yt[6];
+    double const hh = h * 0.5, h6 = h / 6.0;
+    for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
+    derivs(yt, dyt);
+    for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
+    derivs(yt, dym);
+    for (int i = 0; i < 6; ++i) {
+      yt[i] = y[i] + h * dym[i];
+      dym[i] += dyt[i];
+    }
+    derivs(yt, dyt);
+    for (int i = 0; i < 6; ++i) y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
+  };
+  double x[6], d[6] = {0, 0, 0, 0, 0, 0};
+  auto init = [&]() {
+    x[0] = P.x0.x;
+    x[1] = P.beta0.x * kC;
+    x[2] = P.x0.y;
+    x[3] = P.beta0.y * kC;
+    x[4] = P.x0.z;
+    x[5] = P.beta0.z * kC;
+  };
+  auto store = [&](size_t k) {
+    T.x[k] = x[0];
+    T.y[k] = x[2];
+    T.z[k] = x[4];
+    T.bx[k] = x[1] / kC;
+    T.by[k] = x[3] / kC;
+    T.bz[k] = x[5] / kC;
+    T.ax[k] = d[1] / kC;
+    T.ay[k] = d[3] / kC;
+    T.az[k] = d[5] / kC;
+  };
+  init();
+  for (size_t i = 0; i != nf; ++i) {
+    store(nb + i);
+    derivs(x, d);
+    rk4(x, d, dt);
+  }
+  init();
+  double const mdt = -dt;
+  for (size_t i = 0; i != nb; ++i) {
+    derivs(x, d);
+    rk4(x, d, mdt);
+    store(nb - 1 - i);
+  }
+}
+
+// ---- radiation ---------------------------------------------------------------------------
+
+void OSCARSSR::spectrum_one(Particle& p, Vec3 const& obs, SpectrumContainer& S, double weight, Precision prec) {
+  CalculateTrajectory(p);
+  srs_traj const t = view(p.traj);
+  double const o[3] = {obs.x, obs.y, obs.z};
+  std::vector<double> out(S.e.size());
+  check_status(srs_hip_spectrum(&t, o, S.e.data(), (int64_t)S.e.size(), p.q, p.current, (int)prec, out.data()));
+  for (size_t i = 0; i < out.size(); ++i) S.add_to(i, out[i] * weight);
+}
+
+void OSCARSSR::CalculateSpectrum(Vec3 const& obs, SpectrumContainer& S, int nparticles, int nthreads, int gpu,
+                                 Precision prec) {
+  // OSCARSSR::CalculateSpectrum(Obs, S, NParticles, NThreads, GPU), OSCARSSR.cc:933-984.
+  // Every route accumulates value * weight; the integrals run on the GPU.
+  (void)nthreads;
+  (void)gpu;
+  ensure_particle();
+  if (nparticles == 0) {
+    spectrum_one(particle_, obs, S, 1, prec);
+    return;
+  }
+  double const w = 1.0 / (double)nparticles;
+  for (int i = 0; i != nparticles; ++i) {
+    SetNewParticle();
+    spectrum_one(particle_, obs, S, w, prec);
+  }
+}
+
+namespace {
+void add_points(SurfacePoints const& surf, ScalarContainer& C, int dim) {
+  size_t const n = surf.size();
+  for (size_t i = 0; i < n; ++i) {
+    if (dim == 3) {
+      C.add_point(Vec3(surf.xyz[3 * i], surf.xyz[3 * i + 1], surf.xyz[3 * i + 2]), 0);
+    } else if (dim == 2) {
+      C.add_point(surf.has_x12 ? Vec3(surf.x1[i], surf.x2[i], 0) : Vec3(0, 0, 0), 0);
+    } else {
+      throw std::out_of_range("'dim' must be 2 or 3");
+    }
+  }
+}
+}  // namespace
+
+void OSCARSSR::flux_one(Particle& p, SurfacePoints const& surf, double energy_ev, ScalarContainer& C, double weight,
+                        Precision prec) {
+  CalculateTrajectory(p);
+  srs_traj const t = view(p.traj);
+  std::vector<double> out(surf.size());
+  check_status(srs_hip_flux(&t, surf.xyz.data(), (int64_t)surf.size(), &energy_ev, 1, p.q, p.current, (int)prec,
+                            out.data()));
+  for (size_t i = 0; i < out.size(); ++i) C.add_to(i, out[i] * weight);
+}
+
+void OSCARSSR::CalculateFlux(SurfacePoints const& surf, double energy_ev, ScalarContainer& C, int nparticles,
+                             int nthreads, int gpu, int dim, Precision prec) {
+  // OSCARSSR::CalculateFlux(Surface, E, C, NP, NT, GPU, Dim), OSCARSSR.cc:2418-2489.
+  // Container layout follows the reference route (OSCARSSR.cc:2352-2358, :2512, :2570-2572):
+  //   gpu=1: N extra 3-D points per particle; gpu=0 with 1 thread: N extra points per
+  //   particle; gpu=0 with threads: none.  Values land in the first N entries.
+  int const nt = nthreads < 1 ? nthreads_global_ : nthreads;
+  if (nt <= 0) throw std::out_of_range("NThreads or NThreadsGlobal must be >= 1");
+  ensure_particle();
+  add_points(surf, C, dim);
+  int const np = nparticles == 0 ? 1 : nparticles;
+  double const w = nparticles == 0 ? 1.0 : 1.0 / (double)nparticles;
+  for (int i = 0; i != np; ++i) {
+    if (nparticles != 0) SetNewParticle();
+    if (gpu == 1) {
+      add_points(surf, C, 3);
+    } else if (nt == 1) {
+      add_points(surf, C, dim);
+    }
+    flux_one(particle_, surf, energy_ev, C, w, prec);
+  }
+}
+
+void OSCARSSR::pd_one(Particle& p, SurfacePoints const& surf, ScalarContainer& C, bool directional, double weight,
+                      Precision prec) {
+  CalculateTrajectory(p);
+  srs_traj const t = view(p.traj);
+  std::vector<double> out(surf.size());
+  check_status(srs_hip_power_density(&t, surf.xyz.data(), surf.normals.data(), (int64_t)surf.size(),
+                                     directional ? 1 : 0, p.q, p.current, (int)prec, out.data()));
+  for (size_t i = 0; i < out.size(); ++i) C.add_to(i, out[i] * weight);
+}
+
+void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer& C, int dim, bool directional,
+                                     int nparticles, int nthreads, int gpu, Precision prec) {
+  // OSCARSSR::CalculatePowerDensity(Surface, C, Dim, Dir, NP, ofile, NT, GPU), OSCARSSR.cc:1603-1677.
+  // Reference route semantics kept: gpu=0 ignores the particle weight (sum over
+  // particles, OSCARSSR.cc:1579-1585) and its threads route adds N extra points
+  // (:1804-1811); gpu=1 applies the weight and adds N extra 3-D points (:1869-1871).
+  int const nt = nthreads < 1 ? nthreads_global_ : nthreads;
+  if (nt <= 0) throw std::out_of_range("NThreads or NThreadsGlobal must be >= 1");
+  ensure_particle();
+  add_points(surf, C, dim);
+  int const np = nparticles == 0 ? 1 : nparticles;
+  double const w = nparticles == 0 ? 1.0 : 1.0 / (double)nparticles;
+  for (int i = 0; i != np; ++i) {
+    if (nparticles != 0) SetNewParticle();
+    if (gpu == 1) {
+      add_points(surf, C, 3);
+      pd_one(particle_, surf, C, directional, w, prec);
+    } else {
+      if (nt != 1) add_points(surf, C, dim);
+      pd_one(particle_, surf, C, directional, 1.0, prec);
+    }
+  }
+}
+
+double OSCARSSR::CalculateTotalPower() {
+  // OSCARSSR::CalculateTotalPower(TParticleA&), OSCARSSR.cc:1939-1978 (host, O(NT))
+  ensure_particle();
+  CalculateTrajectory(particle_);
+  Trajectory const& T = particle_.traj;
+  double tp = 0;
+  for (size_t i = 0; i != T.size(); ++i) {
+    Vec3 const B(T.bx[i], T.by[i], T.bz[i]);
+    Vec3 const A(T.ax[i], T.ay[i], T.az[i]);
+    tp += (mag2(A) - mag2(cross(B, A))) * T.dt;
+  }
+  const double kEps0 = 8.854187817E-12;
+  tp *= std::fabs(particle_.q * particle_.current) * std::pow(particle_.gamma, 6) / (6 * kPi * kEps0 * kC);
+  return tp;
+}
+
+}  // namespace srs

@@ -1,0 +1,179 @@
+"""GPU parity of the HIP kernels, called through the C ABI (include/srs_hip.h),
+against the reference's own FP64 CPU results (golden fixtures written by the
+compiled reference) and the CPU oracle.
+
+Tolerances (north star): plain FP64 form <= 1e-10 normwise relative
+(max|d|/max|ref|); mixed form <= 1e-5 normwise and RMS relative vs FP64.
+"""
+import numpy as np
+import pytest
+
+import cases
+from conftest import golden
+from gpu_util import normwise, pointwise, rms_rel, say
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+TOL_MIXED = 1e-5
+
+
+@pytest.fixture(scope="module")
+def abi():
+    from srsoriginal_amd import abi as A
+    assert A.device_count() >= 1, "no gfx950 device visible"
+    return A
+
+
+def _rect(oracle, name):
+    r = cases.rect(cases.args(name))
+    return oracle.rect_surface(r["plane"], r["nx1"], r["nx2"], r["w1"], r["w2"], r["rot"], r["trans"], r["normal"])
+
+
+def test_spectrum_c0_full_fp64_matches_reference(abi, oracle):
+    say("spectrum C0 fp64: NT=20001, NE=2000")
+    F, p, t, dt = cases.oracle_setup(oracle, "spectrum_c0_full")
+    s = golden("spectrum_c0_full")["spectrum"]
+    got = abi.spectrum(t, dt, (0, 0, 30), s[:, 0], p.q, p.current, abi.FP64)
+    err = normwise(got, s[:, 1])
+    say("  normwise %.3e  pointwise(>=1e-3 max) %.3e" % (err, pointwise(got, s[:, 1])))
+    assert err <= TOL_FP64
+    assert pointwise(got, s[:, 1]) <= 1e-8
+
+
+def test_spectrum_offaxis_and_small_fp64(abi, oracle):
+    for name in ("spectrum_points_offaxis", "spectrum_c0_small"):
+        say("spectrum fp64 " + name)
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        s = golden(name)["spectrum"]
+        obs = cases.vec(cases.args(name), "obs", [0, 0, 30])
+        got = abi.spectrum(t, dt, obs, s[:, 0], p.q, p.current, abi.FP64)
+        assert normwise(got, s[:, 1]) <= TOL_FP64
+
+
+def test_flux_c1_full_fp64_matches_reference(abi, oracle):
+    say("flux C1 fp64: NT=20001, 201x201")
+    F, p, t, dt = cases.oracle_setup(oracle, "flux_c1_full")
+    g = golden("flux_c1_full")
+    xyz, nrm, x12 = _rect(oracle, "flux_c1_full")
+    got = abi.flux(t, dt, xyz, [float(g["energy_ev"])], p.q, p.current, abi.FP64)[0]
+    err = normwise(got, g["value"])
+    say("  normwise %.3e  pointwise %.3e" % (err, pointwise(got, g["value"])))
+    assert err <= TOL_FP64
+
+
+def test_flux_point_list_and_routes_fp64(abi, oracle):
+    for name in ("flux_points_list", "flux_1thread_route", "flux_threads_route_dim3", "flux_c1_small"):
+        say("flux fp64 " + name)
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        a = cases.args(name)
+        if "points" in a:
+            xyz, nrm = cases.points(a)
+        else:
+            xyz, nrm, x12 = _rect(oracle, name)
+        e = float(golden(name)["energy_ev"]) if "energy_ev" in golden(name).files else float(a["energy_ev"])
+        ref = golden(name)["container"][:len(xyz), 3]
+        got = abi.flux(t, dt, xyz, [e], p.q, p.current, abi.FP64)[0]
+        assert normwise(got, ref) <= TOL_FP64
+
+
+def test_flux_multi_energy_groups_fp64(abi, oracle):
+    # 6 energies -> groups of 4 + 2 inside the library; ragged point count (not a multiple of 256)
+    say("flux fp64 6 energies x 77 points")
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_2001")
+    xyz, nrm, x12 = oracle.rect_surface("XY", 7, 11, 0.004, 0.006, (0, 0, 0), (0, 0, 30), 0)
+    es = [143.5, 150.0, 287.1, 431.2, 700.0, 2500.0]
+    got = abi.flux(t, dt, xyz, es, p.q, p.current, abi.FP64)
+    for k, e in enumerate(es):
+        ref = oracle.flux(t, dt, xyz, e, p.q, p.current)
+        assert normwise(got[k], ref) <= TOL_FP64
+
+
+def test_single_point_edges_fp64(abi, oracle):
+    say("edges: 1 observation point, 1 energy, short trajectories")
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_2001")
+    for n in (1, 2, 255, 257):
+        tt = np.ascontiguousarray(t[:, :n])
+        obs = np.array([[0.001, -0.002, 30.0]])
+        ref = oracle.flux(tt, dt, obs, 300.0, p.q, p.current)
+        got = abi.flux(tt, dt, obs, [300.0], p.q, p.current, abi.FP64)[0]
+        assert normwise(got, ref) <= TOL_FP64
+        sref = oracle.spectrum(tt, dt, (0, 0, 30), [300.0], p.q, p.current)
+        sgot = abi.spectrum(tt, dt, (0, 0, 30), [300.0], p.q, p.current, abi.FP64)
+        assert normwise(sgot, sref) <= TOL_FP64
+
+
+def test_power_density_c2_full_fp64_matches_reference(abi, oracle):
+    say("power density C2 fp64: NT=40001, 301x301")
+    F, p, t, dt = cases.oracle_setup(oracle, "pd_c2_full")
+    xyz, nrm, x12 = _rect(oracle, "pd_c2_full")
+    ref = golden("pd_c2_full")["value"]
+    got = abi.power_density(t, dt, xyz, nrm, False, p.q, p.current, abi.FP64)
+    err = normwise(got, ref)
+    say("  normwise %.3e  pointwise %.3e" % (err, pointwise(got, ref)))
+    assert err <= TOL_FP64
+
+
+def test_power_density_directional_and_threads_route_fp64(abi, oracle):
+    for name, directional in (("pd_directional_tilted", True), ("pd_c0_threads_route", False), ("pd_c2_small", False)):
+        say("power density fp64 " + name)
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        xyz, nrm, x12 = _rect(oracle, name)
+        ref = golden(name)["container"][:len(xyz), 3]
+        got = abi.power_density(t, dt, xyz, nrm, directional, p.q, p.current, abi.FP64)
+        assert normwise(got, ref) <= TOL_FP64
+
+
+def test_mixed_flux_c1_within_1e5(abi, oracle):
+    say("flux C1 mixed vs fp64")
+    F, p, t, dt = cases.oracle_setup(oracle, "flux_c1_full")
+    g = golden("flux_c1_full")
+    xyz, nrm, x12 = _rect(oracle, "flux_c1_full")
+    got = abi.flux(t, dt, xyz, [float(g["energy_ev"])], p.q, p.current, abi.MIXED)[0]
+    e1, e2 = normwise(got, g["value"]), rms_rel(got, g["value"])
+    say("  mixed normwise %.3e rms %.3e" % (e1, e2))
+    assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
+
+
+def test_mixed_flux_three_energies_3rd_harmonic(abi, oracle):
+    say("flux mixed 3 energies near the 3rd harmonic vs oracle")
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_20001")
+    xyz, nrm, x12 = oracle.rect_surface("XY", 21, 21, 0.01, 0.01, (0, 0, 0), (0, 0, 30), 0)
+    es = [430.0, 431.2, 432.4]
+    got = abi.flux(t, dt, xyz, es, p.q, p.current, abi.MIXED)
+    for k, e in enumerate(es):
+        ref = oracle.flux(t, dt, xyz, e, p.q, p.current, nthreads=8)
+        assert normwise(got[k], ref) <= TOL_MIXED and rms_rel(got[k], ref) <= TOL_MIXED
+
+
+def test_mixed_spectrum_c0_within_1e5(abi, oracle):
+    say("spectrum C0 mixed vs reference fp64")
+    F, p, t, dt = cases.oracle_setup(oracle, "spectrum_c0_full")
+    s = golden("spectrum_c0_full")["spectrum"]
+    got = abi.spectrum(t, dt, (0, 0, 30), s[:, 0], p.q, p.current, abi.MIXED)
+    e1, e2 = normwise(got, s[:, 1]), rms_rel(got, s[:, 1])
+    say("  mixed normwise %.3e rms %.3e" % (e1, e2))
+    assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
+
+
+def test_mixed_power_density_c2_within_1e5(abi, oracle):
+    say("power density C2 mixed vs reference fp64")
+    F, p, t, dt = cases.oracle_setup(oracle, "pd_c2_full")
+    xyz, nrm, x12 = _rect(oracle, "pd_c2_full")
+    ref = golden("pd_c2_full")["value"]
+    got = abi.power_density(t, dt, xyz, nrm, False, p.q, p.current, abi.MIXED)
+    e1, e2 = normwise(got, ref), rms_rel(got, ref)
+    # integrated power over the screen (dx dy sum, plots_mpl.py:474-489)
+    area = (0.06 / 300) ** 2 * 1e6
+    tp_ref, tp_got = ref.sum() * area, got.sum() * area
+    say("  mixed normwise %.3e rms %.3e  total %.6f W vs %.6f W" % (e1, e2, tp_got, tp_ref))
+    assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
+    assert abs(tp_got - tp_ref) / tp_ref <= TOL_MIXED
+
+
+def test_abi_rejects_bad_arguments(abi, oracle):
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_2001")
+    with pytest.raises(abi.SrsError, match="precision"):
+        abi.flux(t, dt, [[0, 0, 30]], [100.0], p.q, p.current, precision=7)
+    with pytest.raises(abi.SrsError, match="no points in trajectory"):
+        abi.flux(t[:, :0], dt, [[0, 0, 30]], [100.0], p.q, p.current)

@@ -1,0 +1,170 @@
+"""Host side of the drop-in (srsoriginal_amd.sr over lib/_sr): trajectory,
+fields, beams, surfaces, validation and error behaviour, on the CPU.  The
+product's host arithmetic must match the reference bit for bit, because the GPU
+phase integrals consume it."""
+import hashlib
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from conftest import ROOT, golden
+
+HAS_GPU = os.path.exists("/dev/kfd")
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def _engine(name):
+    from srsoriginal_amd.sr import sr
+    a = cases.args(name)
+    osr = sr()
+    for u in cases.undulators(a):
+        osr.add_bfield_undulator(bfield=u["field"], period=u["period"], nperiods=u["nperiods"], phase=u["phase"])
+    b = cases.beam(a)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=b["energy"], x0=b["x0"], d0=b["d0"],
+                          current=b["current"])
+    c0, c1, n = cases.ct(a)
+    osr.set_ctstartstop(c0, c1)
+    osr.set_npoints_trajectory(n)
+    return osr
+
+
+@pytest.mark.parametrize("name", ["traj_c0_20001", "traj_epu_40001"])
+def test_host_trajectory_matches_reference_digest(name):
+    osr = _engine(name)
+    lst = osr.calculate_trajectory()
+    t, dt = osr.trajectory_array()
+    g = golden(name)
+    assert len(lst) == int(g["n"]) and dt == float(g["dt"])
+    assert _digest(t[:6]) == str(g["digest_xb"])
+    assert _digest(t[6:, 1:]) == str(g["digest_a_tail"])
+    assert lst[5][0] == list(t[0:3, 5]) and lst[5][1] == list(t[3:6, 5])
+
+
+def test_host_fields_match_oracle(oracle):
+    osr = _engine("traj_epu_40001")
+    a = cases.args("traj_epu_40001")
+    F = oracle.FieldSet(cases.undulators(a))
+    for z in np.linspace(-1.2, 1.2, 997):
+        x = [1e-5, -2e-6, float(z)]
+        assert osr.get_bfield(x) == list(oracle.bfield(F, x))
+
+
+def test_pole_scaled_field_matches_oracle(oracle):
+    from srsoriginal_amd.sr import sr
+    d = np.random.default_rng(4).normal(0, 0.005, 82)
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41, pole_scales=list(d))
+    F = oracle.FieldSet([], dict(field=[0, 1, 0], period=[0, 0, 0.05], nperiods=41), d)
+    for z in np.linspace(-1.1, 1.1, 1001):
+        assert osr.get_bfield([0, 0, float(z)]) == list(oracle.bfield(F, [0, 0, float(z)]))
+    with pytest.raises(ValueError, match="2\\*nperiods"):
+        osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41, pole_scales=[0.1])
+
+
+def test_surfaces_match_oracle(oracle):
+    from srsoriginal_amd.lib import _sr
+    for plane, rot, trans, normal in (("XY", [0, 0, 0], [0, 0, 30], 0), ("XZ", [0.1, 0, 0], [0, 0.001, 30], 1),
+                                      ("zy", [0.02, -0.3, 0.7], [1, 2, 3], -1), ("YX", [0, 0, 0], [0, 0, 30], 0)):
+        S = _sr.Surface.rectangle(plane, 5, 7, 0.004, 0.006, rot, trans, normal)
+        xyz, nrm, x12 = oracle.rect_surface(plane, 5, 7, 0.004, 0.006, rot, trans, normal)
+        assert np.array_equal(S.xyz, xyz) and np.array_equal(S.normals, nrm)
+
+
+def test_random_particles_match_reference():
+    # fresh interpreter: the global normal distribution caches its second value
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import cases
+from srsoriginal_amd.sr import sr
+a = cases.args("beam_c3_seed1234")
+osr = sr()
+osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                      sigma_energy_GeV=float(a["sigE"]), beta=cases.vec(a, "beta"), emittance=cases.vec(a, "emit"),
+                      horizontal_direction=cases.vec(a, "hdir"), lattice_reference=cases.vec(a, "latref"))
+osr.set_seed(1234)
+rows = []
+for _ in range(32):
+    osr.set_new_particle()
+    rows.append(osr.get_particle_x0() + osr.get_particle_beta0())
+np.save(sys.argv[1], np.array(rows))
+""" % (ROOT, os.path.join(ROOT, "tests"))
+    out = os.path.join(ROOT, "_work", "beam_rows.npy") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp/beam_rows.npy"
+    subprocess.run([sys.executable, "-c", code, out], check=True)
+    rows = np.load(out)
+    ref = golden("beam_c3_seed1234")["particles"]
+    assert np.array_equal(rows, ref[:, :6])
+
+
+def test_total_power_matches_reference_and_oracle(oracle):
+    osr = _engine("totalpower_c2")
+    assert osr.calculate_total_power() == float(golden("totalpower_c2")["total_power"])
+    osr = _engine("totalpower_c0")
+    F, p, t, dt = cases.oracle_setup(oracle, "totalpower_c0")
+    assert osr.calculate_total_power() == oracle.total_power(t, dt, p.q, p.current, p.gamma)
+
+
+def test_validation_messages_follow_reference():
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    with pytest.raises(ValueError, match="No particle beam defined"):
+        osr.calculate_flux_rectangle(energy_eV=100, npoints=[3, 3], plane="XY", width=[0.01, 0.01])
+    osr = _engine("traj_c0_2001")
+    kw = dict(energy_eV=100, npoints=[3, 3], plane="XY", width=[0.01, 0.01], translation=[0, 0, 30])
+    with pytest.raises(ValueError, match="'gpu' must be 0 or 1"):
+        osr.calculate_flux_rectangle(gpu=2, **kw)
+    with pytest.raises(ValueError, match="Both are not currently allowed"):
+        osr.calculate_flux_rectangle(gpu=1, nthreads=2, **kw)
+    with pytest.raises(ValueError, match="'nthreads' must be > 0"):
+        osr.calculate_flux_rectangle(nthreads=-1, **kw)
+    with pytest.raises(ValueError, match="'dim' must be 2 or 3"):
+        osr.calculate_flux_rectangle(dim=4, **kw)
+    with pytest.raises(ValueError, match="an entry in 'npoints' is <= 0"):
+        osr.calculate_flux_rectangle(energy_eV=100, npoints=[0, 3], plane="XY", width=[0.01, 0.01])
+    with pytest.raises(ValueError, match="not a valid surface string"):
+        osr.calculate_flux_rectangle(energy_eV=100, npoints=[3, 3], plane="QQ", width=[0.01, 0.01])
+    with pytest.raises(ValueError, match="normal must be -1, 0, or 1"):
+        osr.calculate_power_density_rectangle(npoints=[3, 3], plane="XY", width=[0.01, 0.01], normal=2)
+    with pytest.raises(ValueError, match="'energy_range_eV' must be a list of length 2"):
+        osr.calculate_spectrum(obs=[0, 0, 30], energy_range_eV=[1, 2, 3], npoints=4)
+    with pytest.raises(ValueError, match="'npoints' must be > 0"):
+        osr.calculate_spectrum(obs=[0, 0, 30], energy_range_eV=[1, 2])
+    with pytest.raises(ValueError, match="Incorrect format in 'obs'"):
+        osr.calculate_spectrum(obs=[0, 30], energy_range_eV=[1, 2], npoints=3)
+    with pytest.raises(ValueError, match="'precision' must be"):
+        osr.calculate_spectrum(obs=[0, 0, 30], energy_range_eV=[1, 2], npoints=3, precision="half")
+    with pytest.raises(ValueError, match="'particle' must be 'ideal' or 'random'"):
+        osr.set_new_particle(particle="other")
+    with pytest.raises(ValueError, match="possibly 'name' already exists"):
+        osr.add_particle_beam(type="electron", name="beam", energy_GeV=3, d0=[0, 0, 1], x0=[0, 0, -1.5])
+
+
+def test_trajectory_errors():
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    with pytest.raises(ValueError, match="start time is greater than stop time"):
+        osr.calculate_trajectory()
+    osr.set_ctstartstop(1, 3)
+    with pytest.raises(ValueError, match="start time is greater than T0"):
+        osr.calculate_trajectory()
+    # set_ctstartstop resets the count to 10000 points per metre (OSCARSSR.cc:448-458)
+    osr.set_ctstartstop(0, 3)
+    assert osr.get_npoints_trajectory() == 30000
+
+
+@pytest.mark.skipif(HAS_GPU, reason="checks the no-GPU path")
+def test_integrals_without_gpu_raise():
+    osr = _engine("traj_c0_2001")
+    with pytest.raises(ValueError, match="none were found"):
+        osr.calculate_flux_rectangle(energy_eV=100, npoints=[3, 3], plane="XY", width=[0.01, 0.01],
+                                     translation=[0, 0, 30], gpu=1)
