@@ -81,6 +81,17 @@ __device__ __forceinline__ void sincos_fp64(double x, double* s_out, double* c_o
   *c_out = ((q + 1) & 2) ? -c1 : c1;
 }
 
+// Correctly rounded quotient a/b from the correctly rounded reciprocal y = RN(1/b):
+// q = RN(a*y), r = a - b*q (exact with an FMA), RN(q + r*y) = RN(a/b) (Markstein's
+// correction step).  Three operations instead of a full IEEE division; the
+// reference rounds every a/b it computes, so the quotients must match it bit for bit.
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+  double const q = a * y;
+  double const r = __builtin_fma(-q, b, a);
+  return __builtin_fma(r, y, q);
+}
+constexpr double kInvC = 0x1.ca726eb25f9dbp-29;  // RN(1/c)
+
 // Reduce an FP64 phase to r in [-pi, pi] (two FMAs, same argument as above with
 // 2*pi) for the single-precision trig of the mixed form.
 constexpr double kInvTwoPi = 0x1.45f306dc9c883p-3;

@@ -4,7 +4,7 @@
 // :2054-2092 flux, :1522-1573 power density) with the phase path in the
 // reference's exact operation order, so results agree with the reference to
 // rounding of the amplitude terms (well inside 1e-10 relative):
-//   R = O - X;  D = sqrt((Rx*Rx + Ry*Ry) + Rz*Rz);  tau = dt*i + D/C;
+//   R = O - X;  D = sqrt((Rx*Rx + Ry*Ry) + Rz*Rz);  tau = dt*i + D/C;  N = R/D
 //   phi = omega*tau  (all correctly rounded, no contraction: this file is
 //   compiled with -ffp-contract=off and FMAs appear only where written).
 // The phase reaches 4.6e11 rad at 3 keV, where one ulp of D is 5e-5 rad, so any
@@ -21,15 +21,6 @@ namespace srs {
 namespace {
 
 constexpr int TILE = kBlock;
-
-__device__ __forceinline__ double rcp_fp64(double d) {
-  // 1/d to ~1 ulp: hardware estimate + two Newton steps (amplitudes only).
-  double y = __builtin_amdgcn_rcp(d);
-  double e = __builtin_fma(-d, y, 1.0);
-  y = __builtin_fma(y, e, y);
-  e = __builtin_fma(-d, y, 1.0);
-  return __builtin_fma(y, e, y);
-}
 
 template <int NE>
 __global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const double* __restrict__ obs, int ns,
@@ -63,9 +54,10 @@ __global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const dou
       double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
       double const d2 = rx * rx + ry * ry + rz * rz;
       double const D = __builtin_sqrt(d2);
-      double const tau = t.dt * (double)(base + k) + D / kC;
-      double const inv = rcp_fp64(D);
-      double const nx = rx * inv, ny = ry * inv, nz = rz * inv;
+      double const tau = t.dt * (double)(base + k) + div_rn(D, kC, kInvC);
+      // N = R/D rounded as the reference rounds it: beta_z - n_z cancels to ~1e-8
+      double const inv = 1.0 / D;
+      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
       double const ax = (sbx[k] - nx) * inv, ay = (sby[k] - ny) * inv, az = (sbz[k] - nz) * inv;
       double const inv2 = inv * inv;
 #pragma unroll
@@ -123,9 +115,9 @@ __global__ __launch_bounds__(kBlock) void spectrum_fp64_kernel(TrajView t, doubl
       double const rx = ox - t.x[i], ry = oy - t.y[i], rz = oz - t.z[i];
       double const d2 = rx * rx + ry * ry + rz * rz;
       double const D = __builtin_sqrt(d2);
-      stau[tid] = t.dt * (double)i + D / kC;
-      double const inv = rcp_fp64(D);
-      double const nx = rx * inv, ny = ry * inv, nz = rz * inv;
+      stau[tid] = t.dt * (double)i + div_rn(D, kC, kInvC);
+      double const inv = 1.0 / D;
+      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
       sa[0][tid] = (t.bx[i] - nx) * inv;
       sa[1][tid] = (t.by[i] - ny) * inv;
       sa[2][tid] = (t.bz[i] - nz) * inv;
@@ -189,7 +181,8 @@ __global__ __launch_bounds__(kBlock) void pd_fp64_kernel(TrajView t, const doubl
       double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
       double const d2 = rx * rx + ry * ry + rz * rz;
       double const D = __builtin_sqrt(d2);
-      double const n1x = rx / D, n1y = ry / D, n1z = rz / D;
+      double const inv = 1.0 / D;
+      double const n1x = div_rn(rx, D, inv), n1y = div_rn(ry, D, inv), n1z = div_rn(rz, D, inv);
       double const ndot = n1x * qx + n1y * qy + n1z * qz;
       if (directional && ndot <= 0) continue;
       double const bx = sbx[k], by = sby[k], bz = sbz[k];

@@ -41,7 +41,7 @@ hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const d
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
                            int directional, int chunk, double* part, hipStream_t s);
 // Single-precision copies used by the mixed kernels: trajf[k*n + i] for
-// k = 0..8: x, y, z (relative), bx, by, (1 - bz), ax, ay, az.
+// k = 0..5: bx, by, (1 - bz), ax, ay, az (the last three only when t.ax..az are set).
 hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s);
 
 // ---- reductions over trajectory chunks (fixed order, deterministic) --------

@@ -19,14 +19,6 @@ namespace {
 constexpr int TILE = kBlock;
 constexpr int SUB = 32;  // FP32 sub-tile length
 
-__device__ __forceinline__ double sqrt_fast_fp64(double d2) {
-  // sqrt via the hardware reciprocal square root plus one Newton step (<= ~1 ulp).
-  double const y = __builtin_amdgcn_rsq(d2);
-  double D = d2 * y;
-  double const e = __builtin_fma(-D, D, d2);
-  return __builtin_fma(e, 0.5 * y, D);
-}
-
 __device__ __forceinline__ void sincos_reduced_f32(double r, float* s, float* c) {
   // r in [-pi, pi] -> revolutions for the hardware sin/cos
   float const rev = (float)(r * 0.15915494309189535);
@@ -44,16 +36,19 @@ struct Kahan {
   }
 };
 
-// trajf layout (float): [0] bx, [1] by, [2] 1 - bz, [3] ax, [4] ay, [5] az
-__global__ void traj_to_f32_kernel(TrajView t, float* __restrict__ f) {
+// trajf layout (float): [0] bx, [1] by, [2] 1 - bz, and with_a: [3] ax, [4] ay, [5] az
+// (the flux and spectrum paths stage no accelerations: t.ax..az are NULL there).
+__global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f) {
   int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= t.n) return;
   f[0 * t.n + i] = (float)t.bx[i];
   f[1 * t.n + i] = (float)t.by[i];
   f[2 * t.n + i] = (float)(1.0 - t.bz[i]);
-  f[3 * t.n + i] = (float)t.ax[i];
-  f[4 * t.n + i] = (float)t.ay[i];
-  f[5 * t.n + i] = (float)t.az[i];
+  if (with_a) {
+    f[3 * t.n + i] = (float)t.ax[i];
+    f[4 * t.n + i] = (float)t.ay[i];
+    f[5 * t.n + i] = (float)t.az[i];
+  }
 }
 
 template <int NE>
@@ -100,9 +95,11 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
         for (int q = 0; q < 6; ++q) sub[e][q] = 0.f;
       for (int k = k0; k < k1; ++k) {
         double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
-        double const d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, rz * rz));
-        double const D = sqrt_fast_fp64(d2);
-        double const tau = __builtin_fma(D, 1.0 / kC, st[k]);
+        // D and tau exactly as the reference rounds them: a one-Newton-step rsq sqrt is
+        // off by up to ~20 ulp of D, which is 3e-4 rad of phase at 430 eV.
+        double const d2 = rx * rx + ry * ry + rz * rz;
+        double const D = __builtin_sqrt(d2);
+        double const tau = st[k] + div_rn(D, kC, kInvC);
         float const inv = __builtin_amdgcn_rcpf((float)D);
         float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
         float const v = __builtin_fmaf(nx, nx, ny * ny);
@@ -276,8 +273,8 @@ __global__ __launch_bounds__(kBlock) void pd_mixed_kernel(TrajView t, const floa
       float sub = 0.f;
       for (int k = k0; k < k1; ++k) {
         double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
-        double const d2 = __builtin_fma(rx, rx, __builtin_fma(ry, ry, rz * rz));
-        double const D = sqrt_fast_fp64(d2);
+        double const d2 = rx * rx + ry * ry + rz * rz;
+        double const D = __builtin_sqrt(d2);
         float const inv = __builtin_amdgcn_rcpf((float)D);
         float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
         float const ndot = __builtin_fmaf(nx, qx, __builtin_fmaf(ny, qy, nz * qz));
@@ -307,7 +304,8 @@ inline int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk)
 }  // namespace
 
 hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s) {
-  traj_to_f32_kernel<<<(unsigned)((t.n + 255) / 256), 256, 0, s>>>(t, trajf);
+  int const with_a = (t.ax && t.ay && t.az) ? 1 : 0;
+  traj_to_f32_kernel<<<(unsigned)((t.n + 255) / 256), 256, 0, s>>>(t, with_a, trajf);
   return hipGetLastError();
 }
 

@@ -218,6 +218,16 @@ def pd_directional_tilted():
 
 
 @case
+def pd_directional_inside():
+    # Screen inside the undulator, tilted, normal +z: trajectory points behind it
+    # (N1.n <= 0) are skipped and the sign is kept (OSCARSSR.cc:1549-1551, 1569-1573).
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=3001", "plane=XY", "width=0.01,0.008", "nx=7,5",
+                                        "trans=0.001,0,0.4", "rot=0.3,0.1,0", "normal=1", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
 def totalpower_c0():
     args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=20001"]
     a, dt = run("totalpower", args)

@@ -10,8 +10,12 @@ def say(msg):
 
 
 def normwise(got, ref):
-    """max|got - ref| / max|ref| (the uniform parity metric)."""
-    return float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    """max|got - ref| / max|ref| (the uniform parity metric); 0 or inf for an all-zero reference."""
+    d = float(np.max(np.abs(got - ref)))
+    m = float(np.max(np.abs(ref)))
+    if m == 0.0:
+        return 0.0 if d == 0.0 else float("inf")
+    return d / m
 
 
 def rms_rel(got, ref):

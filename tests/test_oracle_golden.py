@@ -157,7 +157,8 @@ def test_power_density_single_thread_route_reads_uninitialised_acceleration(orac
 
 
 def test_power_density_directional_bitwise(oracle):
-    for name, directional in (("pd_directional_tilted", True), ("pd_c0_threads_route", False)):
+    for name, directional in (("pd_directional_tilted", True), ("pd_directional_inside", True),
+                              ("pd_c0_threads_route", False)):
         F, p, t, dt = cases.oracle_setup(oracle, name)
         xyz, nrm, x12 = _rect(oracle, name)
         c = golden(name)["container"]
