@@ -52,6 +52,49 @@ def setup_problem():
     return traj, dt, surf.xyz, q, cur
 
 
+def batched_c3_share(nparticles=512, reps=2):
+    """Config-3 shape on one GPU's share: 512 of the 4096 Gaussian-beam particles (the
+    8-GPU sharding gives each GPU 512), ragged 18k-22k point trajectories, 101x101
+    points over 10 mm at 30 m, 3 energies around the 3rd harmonic; FP64 and mixed."""
+    from srsoriginal_amd import abi, batch
+    from srsoriginal_amd.lib import _sr
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.050], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                          sigma_energy_GeV=8.9e-4 * 3, beta=[1.5, 0.8], emittance=[0.9e-9, 8e-12],
+                          horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
+    osr.set_seed(4096)
+    rows, q, cur = batch.sample_particles(osr, nparticles, 0.0, 3.0, 20001, ct_jitter=0.1, points_per_meter=6667,
+                                          rng_seed=4096)
+    xyz = _sr.Surface.rectangle("XY", 101, 101, 0.01, 0.01, [0, 0, 0], [0, 0, 30], 0).xyz
+    es = [430.0, 431.2, 432.4]
+    npts = sum(abi.trajectory_points(abi.particles([r])) for r in rows)
+    terms = float(npts) * len(xyz) * len(es)
+    res = {}
+    for prec in ("double", "mixed"):
+        out = batch.flux(osr, rows, q, cur, xyz, es, prec)  # autotune + graph capture
+        abi.timing_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = batch.flux(osr, rows, q, cur, xyz, es, prec)
+        wall = (time.perf_counter() - t0) / reps
+        kms, nl = abi.timing_read()
+        abi.timing_enable(False)
+        res[prec] = dict(out=out, wall_s=wall, graph_ms=kms / max(nl, 1), cfg=abi.batch_config())
+        log("batched %s: %.3f s/eval (graph %.1f ms), config %s" % (prec, wall, res[prec]["graph_ms"], res[prec]["cfg"]))
+    f64, mix = res["double"]["out"], res["mixed"]["out"]
+    return {"workload": "%d particles x %d traj points (ragged 18k-22k) x 101x101 x 3 energies" % (nparticles, npts),
+            "terms": terms,
+            "fp64_terms_per_s": terms / (res["double"]["graph_ms"] * 1e-3),
+            "mixed_terms_per_s": terms / (res["mixed"]["graph_ms"] * 1e-3),
+            "speedup": res["double"]["graph_ms"] / res["mixed"]["graph_ms"],
+            "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(mix - f64)) / np.max(np.abs(f64))),
+            "mixed_vs_fp64_rms_rel_err": float(np.sqrt(np.mean((mix - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2))),
+            "tile_config_fp64": res["double"]["cfg"], "tile_config_mixed": res["mixed"]["cfg"],
+            "timed": "whole hipGraph (device RK4 + persistent flux + particle reduction), HIP events"}
+
+
 def cpu_baseline(traj, dt, xyz, q, cur, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -78,6 +121,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.exit("bench.py measures one GPU: this build runs on a single MI355X (multi-GPU sharding is design-only)")
@@ -150,6 +194,8 @@ def main():
                      "kernel": "flux_mixed_kernel<1>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
                      "flop_per_term": F_ALG_FLUX},
     }
+    if not a.no_batch:
+        out["batched_c3_share"] = batched_c3_share()
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(traj, dt, xyz, q, cur)

@@ -89,6 +89,60 @@ int srs_hip_spectrum_device(const srs_traj* traj, const double obs[3], const dou
 int srs_hip_power_density_device(const srs_traj* traj, const double* obs_soa, const double* normals_soa, int64_t ns,
                                  int directional, double charge, double current, int precision, double* pd_out,
                                  void* stream);
+/* ---- batched multi-particle path (device RK4 + persistent work queue) ----------
+ * No single counterpart in the reference: it replaces the nparticles loop of
+ * OSCARSSR::CalculateFlux (OSCARSSR.cc:2465-2479: new particle, host RK4, one GPU
+ * call per particle) with one device pipeline.  Particles are sampled by the caller
+ * (host RNG, reference order) and passed as initial conditions. */
+
+/* Ideal undulator (TField3D_IdealUndulator, TField3D_IdealUndulator.cc:50-139);
+ * pole_delta != NULL scales its 2*nperiods interior half-period poles by (1 + delta_p)
+ * (phase, taper and center unused then). */
+typedef struct srs_undulator {
+  double field[3];
+  double period[3];
+  int32_t nperiods;
+  double phase;
+  double taper;
+  double center[3];
+  const double* pole_delta;
+} srs_undulator;
+
+/* One particle: initial position, beta, q/(m gamma), T0 [m], its ct range and point
+ * count (set_ctstartstop / set_npoints_trajectory; dt and the forward/backward point
+ * counts follow OSCARSSR.cc:592-598). */
+typedef struct srs_particle {
+  double x0[3];
+  double beta0[3];
+  double qmg;
+  double t0;
+  double ct_start;
+  double ct_stop;
+  int64_t npoints;
+} srs_particle;
+
+/* Points the trajectories of np particles will have (total over particles). */
+int srs_hip_trajectory_points(const srs_particle* p, int64_t np, int64_t* total);
+
+/* Device RK4 (FP64) of np particles (the GPU switch of calculate_trajectory).  traj_out
+ * is host memory [9][total] (x,y,z, beta, A/c) packed particle after particle;
+ * offsets_out[np + 1], dt_out[np]. */
+int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double* traj_out,
+                         int64_t total, int64_t* offsets_out, double* dt_out);
+
+#define SRS_BATCH_GRAPH 1    /* run the pipeline as a captured hipGraph, replayed while the shape is unchanged */
+#define SRS_BATCH_AUTOTUNE 2 /* time the candidate tile shapes on this problem once; keep the fastest */
+
+/* Multi-particle flux: flux_out[e*ns + j] = sum over particles (in order, Kahan) of
+ * weight * flux_p(e, j).  All particles share charge and current. */
+int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+                       double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
+                       double weight, int precision, int flags, double* flux_out);
+
+/* Tile shape used by the last batch call: energies per work item and trajectory
+ * points per LDS tile. */
+int srs_hip_batch_config(int* ne_per_item, int* tile);
+
 /* Kernel timing with HIP events recorded on the launch stream around each main
  * integral kernel (not the staging or the chunk reduction).  enable resets the
  * record; read waits for the recorded kernels, returns their summed duration and

@@ -155,3 +155,98 @@ def spectrum_device(traj_ptrs, n, dt, obs, energies_ev, charge, current, precisi
     o = _f64(obs)
     _check(lib().srs_hip_spectrum_device(C.byref(t), o.ctypes.data_as(_P), e.ctypes.data_as(_P), len(e), charge,
                                          current, precision, out_ptr, stream))
+
+
+# ---- batched multi-particle path -------------------------------------------------
+
+BATCH_GRAPH = 1
+BATCH_AUTOTUNE = 2
+
+
+class SrsUndulator(C.Structure):
+    _fields_ = [("field", _D * 3), ("period", _D * 3), ("nperiods", C.c_int32), ("phase", _D), ("taper", _D),
+                ("center", _D * 3), ("pole_delta", C.c_void_p)]
+
+
+class SrsParticle(C.Structure):
+    _fields_ = [("x0", _D * 3), ("beta0", _D * 3), ("qmg", _D), ("t0", _D), ("ct_start", _D), ("ct_stop", _D),
+                ("npoints", _I64)]
+
+
+def _bind_batch():
+    L = lib()
+    if getattr(L, "_batch_bound", False):
+        return L
+    L.srs_hip_trajectory_points.argtypes = [C.POINTER(SrsParticle), _I64, C.POINTER(_I64)]
+    L.srs_hip_trajectories.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _P, _I64,
+                                       C.POINTER(_I64), _P]
+    L.srs_hip_batch_flux.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P, _I64,
+                                     _P, _I64, _D, C.c_int, C.c_int, _P]
+    L.srs_hip_batch_config.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config"):
+        getattr(L, n).restype = C.c_int
+    L._batch_bound = True
+    return L
+
+
+def undulators(specs, keep):
+    """srs_undulator array from dicts (field, period, nperiods[, phase, taper, center, pole_delta])."""
+    arr = (SrsUndulator * len(specs))()
+    for i, u in enumerate(specs):
+        pd = None
+        if u.get("pole_delta") is not None:
+            d = _f64(u["pole_delta"])
+            keep.append(d)
+            pd = d.ctypes.data
+        arr[i] = SrsUndulator((_D * 3)(*u["field"]), (_D * 3)(*u["period"]), int(u["nperiods"]), float(u.get("phase", 0)),
+                              float(u.get("taper", 0)), (_D * 3)(*u.get("center", (0, 0, 0))), pd)
+    return arr
+
+
+def particles(rows):
+    """srs_particle array from dicts (x0, beta0, qmg, t0, ct_start, ct_stop, npoints)."""
+    arr = (SrsParticle * len(rows))()
+    for i, p in enumerate(rows):
+        arr[i] = SrsParticle((_D * 3)(*p["x0"]), (_D * 3)(*p["beta0"]), float(p["qmg"]), float(p.get("t0", 0.0)),
+                             float(p["ct_start"]), float(p["ct_stop"]), int(p["npoints"]))
+    return arr
+
+
+def trajectory_points(parts):
+    L = _bind_batch()
+    tot = _I64()
+    _check(L.srs_hip_trajectory_points(parts, len(parts), C.byref(tot)))
+    return tot.value
+
+
+def trajectories(und_specs, parts):
+    """Device RK4: returns (traj [9][total], offsets [np+1], dt [np])."""
+    L = _bind_batch()
+    keep = []
+    U = undulators(und_specs, keep)
+    tot = trajectory_points(parts)
+    out = np.zeros((9, tot))
+    offs = np.zeros(len(parts) + 1, dtype=np.int64)
+    dts = np.zeros(len(parts))
+    _check(L.srs_hip_trajectories(U, len(U), parts, len(parts), out.ctypes.data_as(_P), tot,
+                                  offs.ctypes.data_as(C.POINTER(_I64)), dts.ctypes.data_as(_P)))
+    return out, offs, dts
+
+
+def batch_flux(und_specs, parts, charge, current, obs_xyz, energies_ev, weight, precision=FP64, flags=0):
+    L = _bind_batch()
+    keep = []
+    U = undulators(und_specs, keep)
+    o = _f64(obs_xyz).reshape(-1, 3)
+    e = _f64(np.atleast_1d(energies_ev))
+    out = np.zeros(len(e) * len(o))
+    _check(L.srs_hip_batch_flux(U, len(U), parts, len(parts), charge, current, o.ctypes.data_as(_P), len(o),
+                                e.ctypes.data_as(_P), len(e), weight, precision, flags, out.ctypes.data_as(_P)))
+    return out.reshape(len(e), len(o))
+
+
+def batch_config():
+    L = _bind_batch()
+    a, b = C.c_int(), C.c_int()
+    _check(L.srs_hip_batch_config(C.byref(a), C.byref(b)))
+    return a.value, b.value

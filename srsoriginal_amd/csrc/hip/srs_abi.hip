@@ -15,7 +15,10 @@
 #include <vector>
 
 #include "../../../include/srs_hip.h"
+#include "srs_batch.h"
 #include "srs_kernels.h"
+
+#include <map>
 
 namespace {
 
@@ -72,6 +75,14 @@ struct Context {
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   size_t ev_used = 0;
+  // batched path
+  Buf b_part, b_und, b_poles, b_traj, b_trajf, b_energies, b_flux_pp, b_out, b_counter, b_obs;
+  hipGraph_t b_graph = nullptr;
+  hipGraphExec_t b_exec = nullptr;
+  std::string b_key;
+  std::map<std::string, srs::BatchConfig> b_tuned;
+  srs::BatchConfig b_last{1, 128};
+  int cus = 0;
 };
 
 Context& ctx() {
@@ -113,11 +124,11 @@ struct KernelTimer {
       c.ev_pool.push_back({a, b});
     }
     auto& p = c.ev_pool[c.ev_used++];
-    hipEventRecord(p.first, s);
+    (void)hipEventRecord(p.first, s);
     stop = p.second;
   }
   ~KernelTimer() {
-    if (stop) hipEventRecord(stop, s);
+    if (stop) (void)hipEventRecord(stop, s);
   }
 };
 
@@ -285,6 +296,117 @@ int check_common(int64_t nout, int precision) {
     if (_r != SRS_OK) return _r; \
   } while (0)
 
+// ---- batched multi-particle path -------------------------------------------------
+
+int particle_grid(const srs_particle& p, double* dt, int64_t* nb, int64_t* nf) {
+  // OSCARSSR.cc:574-598
+  if (p.ct_start > p.t0) return fail(SRS_ERR_INVALID, "start time is greater than T0");
+  if (p.ct_start >= p.ct_stop) return fail(SRS_ERR_INVALID, "start time is greater than stop time");
+  if (p.npoints < 2) return fail(SRS_ERR_INVALID, "number of trajectory points must be >= 2");
+  double const d = ((p.ct_stop - p.ct_start) / kC / (size_t)(p.npoints - 1));
+  size_t const f = 1 + (p.ct_stop - p.t0) / kC / d;
+  size_t const b = (p.t0 - p.ct_start) / kC / d;
+  *dt = d;
+  *nb = (int64_t)b;
+  *nf = (int64_t)f;
+  return SRS_OK;
+}
+
+int build_particles(const srs_particle* p, int64_t np, std::vector<srs::BatchParticle>& out, int64_t* total) {
+  out.resize(np);
+  int64_t off = 0;
+  for (int64_t i = 0; i < np; ++i) {
+    srs::BatchParticle& q = out[i];
+    for (int k = 0; k < 3; ++k) {
+      q.x0[k] = p[i].x0[k];
+      q.beta0[k] = p[i].beta0[k];
+    }
+    q.qmg = p[i].qmg;
+    TRY(particle_grid(p[i], &q.dt, &q.nb, &q.nf));
+    q.off = off;
+    off += q.nb + q.nf;
+  }
+  *total = off;
+  if (off > ((int64_t)1 << 40)) return fail(SRS_ERR_INVALID, "too many trajectory points");
+  return SRS_OK;
+}
+
+int stage_undulators(Context& c, const srs_undulator* und, int nund, hipStream_t s) {
+  if (nund < 1 || !und) return fail(SRS_ERR_INVALID, "no magnetic field (undulator) given");
+  std::vector<srs::DevUndulator> du(nund);
+  std::vector<double> poles;
+  for (int u = 0; u < nund; ++u) {
+    srs::DevUndulator& d = du[u];
+    for (int k = 0; k < 3; ++k) {
+      d.field[k] = und[u].field[k];
+      d.period[k] = und[u].period[k];
+      d.center[k] = und[u].center[k];
+    }
+    d.phase = und[u].phase;
+    d.taper = und[u].taper;
+    d.nperiods = und[u].nperiods;
+    d.pole_off = -1;
+    if (und[u].pole_delta) {
+      d.pole_off = (int)poles.size();
+      poles.insert(poles.end(), und[u].pole_delta, und[u].pole_delta + 2 * und[u].nperiods);
+    }
+  }
+  if (poles.empty()) poles.push_back(0.0);
+  HIPCHK(c.b_und.ensure(du.size() * sizeof(srs::DevUndulator)));
+  HIPCHK(c.b_poles.ensure(poles.size() * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_und.p, du.data(), du.size() * sizeof(srs::DevUndulator), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c.b_poles.p, poles.data(), poles.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // host staging vectors go out of scope
+  return SRS_OK;
+}
+
+int stage_batch_traj(Context& c, int64_t total, bool with_a, srs::BatchTraj* bt) {
+  HIPCHK(c.b_traj.ensure((size_t)(with_a ? 9 : 6) * total * sizeof(double)));
+  HIPCHK(c.b_trajf.ensure((size_t)3 * total * sizeof(float)));
+  double* d = c.b_traj.as<double>();
+  float* f = c.b_trajf.as<float>();
+  size_t const n = (size_t)total;
+  *bt = srs::BatchTraj{d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, with_a ? d + 6 * n : nullptr,
+                       with_a ? d + 7 * n : nullptr, with_a ? d + 8 * n : nullptr, f, f + n, f + 2 * n};
+  return SRS_OK;
+}
+
+struct BatchRun {
+  int nund;
+  srs::BatchTraj bt;
+  srs::BatchEnergies be;
+  int np, ns, ne;
+  bool mixed;
+  double weight;
+};
+
+// Queue the whole pipeline (RK4, queue reset, persistent flux, particle reduction).
+int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, int np_run, hipStream_t s) {
+  int const ntiles = (r.ns + srs::kBlock - 1) / srs::kBlock;
+  int const ngroups = (r.ne + cfg.ne_per_item - 1) / cfg.ne_per_item;
+  unsigned const nitems = (unsigned)((int64_t)np_run * ntiles * ngroups);
+  int nblocks = c.cus * srs::batch_max_blocks(cfg, r.mixed);
+  if ((unsigned)nblocks > nitems) nblocks = (int)nitems;
+  auto* P = c.b_part.as<srs::BatchParticle>();
+  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt, s));
+  HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
+  {
+    KernelTimer timer(c, s);
+    HIPCHK(srs::launch_batch_flux(cfg, r.mixed, r.bt, P, c.b_obs.as<double>(), r.ns, r.be, nitems,
+                                  c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), nblocks, s));
+  }
+  HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), np_run, r.ne * r.ns, r.weight, c.b_out.as<double>(), s));
+  return SRS_OK;
+}
+
+std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context const& c) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%d/%d/%d/%d/%d/%d/%p/%p/%p/%p/%p/%p/%.17g", r.np, r.ns, r.ne, (int)r.mixed, cfg.ne_per_item,
+           cfg.tile, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p, r.weight);
+  return buf;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -420,6 +542,174 @@ int srs_hip_timing_read(double* total_ms, int64_t* launches) {
   if (total_ms) *total_ms = sum;
   if (launches) *launches = (int64_t)c.ev_used;
   c.ev_used = 0;
+  return SRS_OK;
+}
+
+int srs_hip_trajectory_points(const srs_particle* p, int64_t np, int64_t* total) {
+  g_err.clear();
+  if (!p || np < 1 || !total) return fail(SRS_ERR_INVALID, "no particles");
+  std::vector<srs::BatchParticle> P;
+  return build_particles(p, np, P, total);
+}
+
+int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double* traj_out,
+                         int64_t total, int64_t* offsets_out, double* dt_out) {
+  g_err.clear();
+  if (!p || np < 1 || !traj_out || !offsets_out || !dt_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  std::vector<srs::BatchParticle> P;
+  int64_t tot = 0;
+  TRY(build_particles(p, np, P, &tot));
+  if (tot != total) return fail(SRS_ERR_INVALID, "traj_out size does not match the trajectory points");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  hipStream_t s = c.stream;
+  TRY(stage_undulators(c, und, nund, s));
+  HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
+  HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
+  srs::BatchTraj bt;
+  TRY(stage_batch_traj(c, tot, true, &bt));
+  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(),
+                               c.b_part.as<srs::BatchParticle>(), (int)np, bt, s));
+  double* src[9] = {bt.x, bt.y, bt.z, bt.bx, bt.by, bt.bz, bt.ax, bt.ay, bt.az};
+  for (int k = 0; k < 9; ++k)
+    HIPCHK(hipMemcpyAsync(traj_out + (size_t)k * tot, src[k], tot * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < np; ++i) {
+    offsets_out[i] = P[i].off;
+    dt_out[i] = P[i].dt;
+  }
+  offsets_out[np] = tot;
+  return SRS_OK;
+}
+
+int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+                       double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
+                       double weight, int precision, int flags, double* flux_out) {
+  g_err.clear();
+  TRY(check_common(ns, precision));
+  if (ne < 1) return fail(SRS_ERR_INVALID, "no photon energies");
+  if (!p || np < 1) return fail(SRS_ERR_INVALID, "no particles");
+  if (np > (1 << 24)) return fail(SRS_ERR_INVALID, "too many particles");
+  if (!obs_xyz || !energies_ev || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  std::vector<srs::BatchParticle> P;
+  int64_t total = 0;
+  TRY(build_particles(p, np, P, &total));
+  int64_t const ntiles = (ns + srs::kBlock - 1) / srs::kBlock;
+  if ((double)np * ntiles * ne > 4.0e9) return fail(SRS_ERR_INVALID, "too many work items");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  hipStream_t s = c.stream;
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  TRY(stage_undulators(c, und, nund, s));
+  HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
+  HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
+  TRY(stage_points(c, c.b_obs, obs_xyz, ns));
+  double const c0 = c0_of(charge);
+  std::vector<double> eh(3 * ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    double const w = omega_of(energies_ev[e]);
+    eh[e] = w;
+    eh[ne + e] = kC / w;
+    eh[2 * ne + e] = c0 * w;
+  }
+  HIPCHK(c.b_energies.ensure(3 * ne * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 3 * ne * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c.b_flux_pp.ensure((size_t)np * ne * ns * sizeof(double)));
+  HIPCHK(c.b_out.ensure((size_t)ne * ns * sizeof(double)));
+  HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
+  BatchRun r;
+  r.nund = nund;
+  TRY(stage_batch_traj(c, total, false, &r.bt));
+  double* ed = c.b_energies.as<double>();
+  r.be = srs::BatchEnergies{ed, ed + ne, ed + 2 * ne, (int)ne, c2_of(charge, current)};
+  r.np = (int)np;
+  r.ns = (int)ns;
+  r.ne = (int)ne;
+  r.mixed = precision == SRS_PRECISION_MIXED;
+  r.weight = weight;
+  HIPCHK(hipStreamSynchronize(s));  // host staging vectors
+
+  // tile shape: default, or the autotuned one for this problem signature
+  srs::BatchConfig cfg{(int)std::min<int64_t>(ne, 4) == 3 ? 3 : (ne >= 4 ? 4 : (int)ne), 128};
+  if (!srs::batch_config_valid(cfg)) cfg = srs::BatchConfig{1, 128};
+  char sig[128];
+  snprintf(sig, sizeof sig, "%d/%lld/%lld/%lld", (int)r.mixed, (long long)((ns + 255) / 256), (long long)ne,
+           (long long)(total / np));
+  if (flags & SRS_BATCH_AUTOTUNE) {
+    auto it = c.b_tuned.find(sig);
+    if (it != c.b_tuned.end()) {
+      cfg = it->second;
+    } else {
+      // time each candidate on a subset of the particles (same RK4 and inputs)
+      static const srs::BatchConfig cands[] = {{1, 64}, {1, 128}, {1, 256}, {2, 128}, {2, 256},
+                                               {3, 64}, {3, 128}, {3, 256}, {4, 128}, {4, 256}};
+      int const nsub = (int)std::min<int64_t>(np, 64);
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      float best = 1e30f;
+      bool const tsave = c.timing;
+      c.timing = false;
+      for (auto const& k : cands) {
+        if (k.ne_per_item > ne || (k.ne_per_item > 1 && ne % k.ne_per_item != 0 && k.ne_per_item != ne)) continue;
+        if (!srs::batch_config_valid(k)) continue;
+        TRY(enqueue_batch(c, r, k, nsub, s));  // warm-up
+        HIPCHK(hipEventRecord(e0, s));
+        TRY(enqueue_batch(c, r, k, nsub, s));
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          cfg = k;
+        }
+      }
+      c.timing = tsave;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      c.b_tuned[sig] = cfg;
+    }
+  }
+  c.b_last = cfg;
+
+  if (flags & SRS_BATCH_GRAPH) {
+    std::string const key = batch_key(r, cfg, c) + "/" + std::to_string(c.b_und.p ? (long long)(intptr_t)c.b_und.p : 0) +
+                            "/" + std::to_string(nund);
+    if (key != c.b_key || !c.b_exec) {
+      if (c.b_exec) (void)hipGraphExecDestroy(c.b_exec);
+      if (c.b_graph) (void)hipGraphDestroy(c.b_graph);
+      c.b_exec = nullptr;
+      c.b_graph = nullptr;
+      bool const tsave = c.timing;
+      c.timing = false;  // no timing events inside the captured graph
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int const rc = enqueue_batch(c, r, cfg, r.np, s);
+      hipGraph_t g = nullptr;
+      hipError_t const ec = hipStreamEndCapture(s, &g);
+      c.timing = tsave;
+      if (rc != SRS_OK) return rc;
+      if (ec != hipSuccess) return fail(SRS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      c.b_graph = g;
+      HIPCHK(hipGraphInstantiate(&c.b_exec, c.b_graph, nullptr, nullptr, 0));
+      c.b_key = key;
+    }
+    KernelTimer timer(c, s);
+    HIPCHK(hipGraphLaunch(c.b_exec, s));
+  } else {
+    TRY(enqueue_batch(c, r, cfg, r.np, s));
+  }
+  HIPCHK(hipMemcpyAsync(flux_out, c.b_out.p, (size_t)ne * ns * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SRS_OK;
+}
+
+int srs_hip_batch_config(int* ne_per_item, int* tile) {
+  Context& c = ctx();
+  if (ne_per_item) *ne_per_item = c.b_last.ne_per_item;
+  if (tile) *tile = c.b_last.tile;
   return SRS_OK;
 }
 

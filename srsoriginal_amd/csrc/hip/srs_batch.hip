@@ -1,0 +1,357 @@
+// Batched multi-particle radiation on gfx950 (the reference's nparticles loop,
+// OSCARSSR.cc:2465-2479, as one device pipeline):
+//
+//   rk4_batch_kernel      one lane per particle integrates its RK4 trajectory
+//                         (ragged lengths, packed SoA) in FP64 and also writes the
+//                         FP32 copies the mixed form reads;
+//   zero_counter_kernel   resets the work-queue head (a kernel, so the pipeline can
+//                         be captured in a hipGraph without memset nodes);
+//   batch_flux_kernel     persistent: each block pops (particle, observation tile,
+//                         energy group) items from the queue until it is empty and
+//                         writes that particle's flux for 256 points;
+//   particle_reduce       Kahan-sums the per-particle maps in particle order with
+//                         the weight 1/NP (T3DScalarContainer::AddToPoint order).
+#include "srs_batch.h"
+#include "srs_terms.h"
+
+namespace srs {
+namespace {
+
+// ---- device field and RK4 (reference: TField3D_IdealUndulator.cc:106-139,
+// OSCARSSR.cc:741-874; same operation order, device sine) ----------------------
+
+__device__ __forceinline__ double sin_fp64(double x) {
+  double s, c;
+  sincos_fp64(x, &s, &c);
+  return s;
+}
+
+__device__ void field_eval(const DevUndulator* __restrict__ U, int nu, const double* __restrict__ poles, double X,
+                           double Y, double Z, double* B) {
+  double Sx = 0, Sy = 0, Sz = 0;
+  for (int u = 0; u < nu; ++u) {
+    DevUndulator const& f = U[u];
+    double const pm = __builtin_sqrt(f.period[0] * f.period[0] + f.period[1] * f.period[1] + f.period[2] * f.period[2]);
+    double const ux = f.period[0] / pm, uy = f.period[1] / pm, uz = f.period[2] / pm;
+    double const lam = pm;
+    double const len = pm * (f.nperiods + 2);
+    double bx = 0, by = 0, bz = 0;
+    if (f.pole_off >= 0) {
+      double const D = X * ux + Y * uy + Z * uz;
+      double const s = sin_fp64(2.0 * kPi * D / lam);
+      if (!(D > len / 2. || D < -len / 2.)) {
+        if (D < -len / 2. + lam || D > len / 2. - lam) {
+          double const fct = (D < -len / 2. + lam / 2. || D > len / 2. - lam / 2.) ? 0.25 : 0.75;
+          bx = (fct * f.field[0]) * s;
+          by = (fct * f.field[1]) * s;
+          bz = (fct * f.field[2]) * s;
+        } else {
+          int p = (int)__builtin_floor((D - (-len / 2. + lam)) / (lam / 2.));
+          p = p < 0 ? 0 : (p > 2 * f.nperiods - 1 ? 2 * f.nperiods - 1 : p);
+          double const sc = 1.0 + poles[f.pole_off + p];
+          bx = (f.field[0] * s) * sc;
+          by = (f.field[1] * s) * sc;
+          bz = (f.field[2] * s) * sc;
+        }
+      }
+    } else {
+      double const D = (X - f.center[0]) * ux + (Y - f.center[1]) * uy + (Z - f.center[2]) * uz;
+      double const T = 1 + D * f.taper;
+      double const PS = f.phase * pm / (2.0 * kPi);
+      if (!(D > len / 2. + PS || D < -len / 2. + PS)) {
+        double const s = sin_fp64((2.0 * kPi) * (D - PS) / lam);
+        double fct = 1.0;
+        bool term = false;
+        if (D < -len / 2. + PS + lam || D > len / 2. + PS - lam) {
+          term = true;
+          fct = (D < -len / 2. + PS + lam / 2. || D > len / 2. + PS - lam / 2.) ? 0.25 : 0.75;
+        }
+        if (term) {
+          bx = ((f.field[0] * fct) * s) * T;
+          by = ((f.field[1] * fct) * s) * T;
+          bz = ((f.field[2] * fct) * s) * T;
+        } else {
+          bx = (f.field[0] * s) * T;
+          by = (f.field[1] * s) * T;
+          bz = (f.field[2] * s) * T;
+        }
+      }
+    }
+    Sx += bx;
+    Sy += by;
+    Sz += bz;
+  }
+  B[0] = Sx;
+  B[1] = Sy;
+  B[2] = Sz;
+}
+
+struct RK4Ctx {
+  const DevUndulator* U;
+  int nu;
+  const double* poles;
+  double qmg;
+  __device__ __forceinline__ void derivs(const double* x, double* d) const {
+    double B[3];
+    field_eval(U, nu, poles, x[0], x[2], x[4], B);
+    d[0] = x[1];
+    d[1] = qmg * (-x[5] * B[1] + x[3] * B[2]);
+    d[2] = x[3];
+    d[3] = qmg * (x[5] * B[0] - x[1] * B[2]);
+    d[4] = x[5];
+    d[5] = qmg * (x[1] * B[1] - x[3] * B[0]);
+  }
+  __device__ __forceinline__ void step(double* y, const double* dydx, double h) const {
+    double dym[6], dyt[6], yt[6];
+    double const hh = h * 0.5, h6 = h / 6.0;
+    for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
+    derivs(yt, dyt);
+    for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
+    derivs(yt, dym);
+    for (int i = 0; i < 6; ++i) {
+      yt[i] = y[i] + h * dym[i];
+      dym[i] += dyt[i];
+    }
+    derivs(yt, dyt);
+    for (int i = 0; i < 6; ++i) y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
+  }
+};
+
+__device__ __forceinline__ void store_point(BatchTraj const& bt, int64_t k, const double* x, const double* d) {
+  bt.x[k] = x[0];
+  bt.y[k] = x[2];
+  bt.z[k] = x[4];
+  double const bx = x[1] / kC, by = x[3] / kC, bz = x[5] / kC;
+  bt.bx[k] = bx;
+  bt.by[k] = by;
+  bt.bz[k] = bz;
+  bt.fbx[k] = (float)bx;
+  bt.fby[k] = (float)by;
+  bt.fomb[k] = (float)(1.0 - bz);
+  if (bt.ax) {
+    bt.ax[k] = d[1] / kC;
+    bt.ay[k] = d[3] / kC;
+    bt.az[k] = d[5] / kC;
+  }
+}
+
+// Same point layout as the host trajectory (OSCARSSR.cc:562-673): forward pass from
+// T0 stores the state before each step with the previous derivative (A/c lags one
+// point; 0 at the first point), the backward pass fills the nb points before T0.
+__global__ void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu, const double* __restrict__ poles,
+                                 const BatchParticle* __restrict__ P, int np, BatchTraj bt) {
+  int const p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  BatchParticle const q = P[p];
+  RK4Ctx const R{U, nu, poles, q.qmg};
+  double x[6], d[6] = {0, 0, 0, 0, 0, 0};
+  auto init = [&]() {
+    x[0] = q.x0[0];
+    x[1] = q.beta0[0] * kC;
+    x[2] = q.x0[1];
+    x[3] = q.beta0[1] * kC;
+    x[4] = q.x0[2];
+    x[5] = q.beta0[2] * kC;
+  };
+  init();
+  for (int64_t i = 0; i < q.nf; ++i) {
+    store_point(bt, q.off + q.nb + i, x, d);
+    R.derivs(x, d);
+    R.step(x, d, q.dt);
+  }
+  init();
+  for (int64_t i = 0; i < q.nb; ++i) {
+    R.derivs(x, d);
+    R.step(x, d, -q.dt);
+    store_point(bt, q.off + q.nb - 1 - i, x, d);
+  }
+}
+
+__global__ void zero_counter_kernel(unsigned* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+// Persistent flux kernel.  Item = (particle, observation tile of 256 points, group of
+// NE energies), popped from a global counter by lane 0 and broadcast through LDS.
+// Every block exits when the counter passes the item count.
+template <int NE, int TILE_, bool MIXED>
+__global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                            const double* __restrict__ obs, int ns, BatchEnergies be,
+                                                            unsigned nitems, unsigned* __restrict__ counter,
+                                                            double* __restrict__ flux_pp) {
+  __shared__ FluxTile<TILE_> T;
+  __shared__ unsigned s_item;
+  int const tid = threadIdx.x;
+  int const ntiles = (ns + kBlock - 1) / kBlock;
+  int const ngroups = (be.ne + NE - 1) / NE;
+  for (;;) {
+    __syncthreads();  // every lane has finished with the previous item and s_item
+    if (tid == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    unsigned const item = s_item;
+    if (item >= nitems) break;
+    int const g = (int)(item % (unsigned)ngroups);
+    unsigned const rest = item / (unsigned)ngroups;
+    int const tile = (int)(rest % (unsigned)ntiles);
+    int const p = (int)(rest / (unsigned)ntiles);
+    BatchParticle const q = P[p];
+    int64_t const n = q.nb + q.nf;
+    EnergySet es;
+    es.ne = NE;
+    float cwf[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      int const ee = (g * NE + e < be.ne) ? g * NE + e : be.ne - 1;
+      es.omega[e] = be.omega[ee];
+      es.cw[e] = be.cw[ee];
+      es.kw[e] = be.c0w[ee] * q.dt;  // (C0*omega)*dt, OSCARSSR.cc:2080
+      cwf[e] = (float)be.cw[ee];
+    }
+    int const j = tile * kBlock + tid;
+    int const jj = j < ns ? j : ns - 1;
+    double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+    AccFP64<NE> a64;
+    AccMixed<NE> a32;
+    if (MIXED) a32.zero(); else a64.zero();
+    for (int64_t base = 0; base < n; base += TILE_) {
+      int const cnt = (n - base) < TILE_ ? (int)(n - base) : TILE_;
+      __syncthreads();
+      for (int k = tid; k < cnt; k += kBlock) {
+        int64_t const i = q.off + base + k;
+        T.x[k] = bt.x[i];
+        T.y[k] = bt.y[i];
+        T.z[k] = bt.z[i];
+        if (MIXED) {
+          T.fbx[k] = bt.fbx[i];
+          T.fby[k] = bt.fby[i];
+          T.fomb[k] = bt.fomb[i];
+        } else {
+          T.bx[k] = bt.bx[i];
+          T.by[k] = bt.by[i];
+          T.bz[k] = bt.bz[i];
+        }
+      }
+      __syncthreads();
+      if (MIXED) flux_terms_mixed<NE, TILE_>(T, cnt, base, q.dt, ox, oy, oz, es, cwf, a32);
+      else flux_terms_fp64<NE, TILE_>(T, cnt, base, q.dt, ox, oy, oz, es, a64);
+    }
+    if (j < ns) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        int const ee = g * NE + e;
+        if (ee >= be.ne) continue;
+        double f;
+        if (MIXED) {
+          f = flux_from_sums(a32.value(e, 0), a32.value(e, 1), a32.value(e, 2), a32.value(e, 3), a32.value(e, 4),
+                             a32.value(e, 5), es.kw[e], be.c2);
+        } else {
+          f = flux_from_sums(a64.re[e][0], a64.im[e][0], a64.re[e][1], a64.im[e][1], a64.re[e][2], a64.im[e][2],
+                             es.kw[e], be.c2);
+        }
+        flux_pp[((size_t)p * be.ne + ee) * ns + j] = f;
+      }
+    }
+  }
+}
+
+__global__ void particle_reduce_kernel(const double* __restrict__ flux_pp, int np, int nout, double weight,
+                                       double* __restrict__ out) {
+  int const o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  double s = 0, c = 0;
+  for (int p = 0; p < np; ++p) {
+    double const y = flux_pp[(size_t)p * nout + o] * weight - c;
+    double const t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  out[o] = s;
+}
+
+template <int NE, int TILE_, bool MIXED>
+hipError_t launch_batch_t(BatchTraj const& bt, const BatchParticle* P, const double* obs, int ns,
+                          BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp, int nblocks,
+                          hipStream_t s) {
+  batch_flux_kernel<NE, TILE_, MIXED><<<nblocks, kBlock, 0, s>>>(bt, P, obs, ns, be, nitems, counter, flux_pp);
+  return hipGetLastError();
+}
+
+template <bool MIXED>
+hipError_t launch_batch_m(BatchConfig const& cfg, BatchTraj const& bt, const BatchParticle* P, const double* obs, int ns,
+                          BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp, int nblocks,
+                          hipStream_t s) {
+#define SRS_BATCH_CASE(NE_, TL_)                 \
+  if (cfg.ne_per_item == NE_ && cfg.tile == TL_) \
+    return launch_batch_t<NE_, TL_, MIXED>(bt, P, obs, ns, be, nitems, counter, flux_pp, nblocks, s);
+  SRS_BATCH_CASE(1, 64)
+  SRS_BATCH_CASE(1, 128)
+  SRS_BATCH_CASE(1, 256)
+  SRS_BATCH_CASE(2, 128)
+  SRS_BATCH_CASE(2, 256)
+  SRS_BATCH_CASE(3, 64)
+  SRS_BATCH_CASE(3, 128)
+  SRS_BATCH_CASE(3, 256)
+  SRS_BATCH_CASE(4, 128)
+  SRS_BATCH_CASE(4, 256)
+#undef SRS_BATCH_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+bool batch_config_valid(BatchConfig const& c) {
+  switch (c.ne_per_item) {
+    case 1: return c.tile == 64 || c.tile == 128 || c.tile == 256;
+    case 2: return c.tile == 128 || c.tile == 256;
+    case 3: return c.tile == 64 || c.tile == 128 || c.tile == 256;
+    case 4: return c.tile == 128 || c.tile == 256;
+  }
+  return false;
+}
+
+int batch_max_blocks(BatchConfig const& c, bool mixed) {
+  // resident blocks per CU for this instantiation (sizes the persistent grid)
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+#define SRS_OCC(NE_, TL_)                                                                                          \
+  if (c.ne_per_item == NE_ && c.tile == TL_)                                                                      \
+    e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_flux_kernel<NE_, TL_, true>, kBlock, 0)  \
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_flux_kernel<NE_, TL_, false>, kBlock, 0);
+  SRS_OCC(1, 64)
+  SRS_OCC(1, 128)
+  SRS_OCC(1, 256)
+  SRS_OCC(2, 128)
+  SRS_OCC(2, 256)
+  SRS_OCC(3, 64)
+  SRS_OCC(3, 128)
+  SRS_OCC(3, 256)
+  SRS_OCC(4, 128)
+  SRS_OCC(4, 256)
+#undef SRS_OCC
+  return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
+                            BatchTraj const& bt, hipStream_t s) {
+  rk4_batch_kernel<<<(np + 63) / 64, 64, 0, s>>>(U, nu, poles, P, np, bt);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_counter(unsigned* c, hipStream_t s) {
+  zero_counter_kernel<<<1, 64, 0, s>>>(c);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_flux(BatchConfig const& cfg, bool mixed, BatchTraj const& bt, const BatchParticle* P,
+                             const double* obs, int ns, BatchEnergies const& be, unsigned nitems, unsigned* counter,
+                             double* flux_pp, int nblocks, hipStream_t s) {
+  return mixed ? launch_batch_m<true>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, nblocks, s)
+               : launch_batch_m<false>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, nblocks, s);
+}
+
+hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, double weight, double* out, hipStream_t s) {
+  particle_reduce_kernel<<<(nout + 255) / 256, 256, 0, s>>>(flux_pp, np, nout, weight, out);
+  return hipGetLastError();
+}
+
+}  // namespace srs

@@ -29,79 +29,7 @@ struct TrajView {
   double dt;
 };
 
-// ---------------------------------------------------------------------------
-// FP64 sine and cosine of large arguments (|x| up to ~1e15).
-//
-// Reduction: k = rint(x * 2/pi), r = x - k*(PIO2_HI + PIO2_LO) with two fused
-// multiply-adds.  PIO2_HI = RN(pi/2) is a multiple of 2^-52 and so is every
-// x >= 1, hence x - k*PIO2_HI is an exact multiple of 2^-52 below 2 in
-// magnitude and the first FMA is exact; the second FMA rounds once.  The
-// reduced argument is within one ulp for |x| < 2^51 (Cody & Waite's additive
-// reduction, with the FMA removing the need to split k).  The quadrant comes
-// from the low bits of k + 2^52.
-//
-// Kernels: Taylor series of sin and cos on |r| <= pi/4 (+ rounding slack),
-// truncated after r^17/17! and r^16/16!: truncation error < 1e-19, so the
-// result is within about one ulp.
-// ---------------------------------------------------------------------------
-constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
-constexpr double kPio2Hi = 0x1.921fb54442d18p+0;
-constexpr double kPio2Lo = 0x1.1a62633145c07p-54;
-
-__device__ __forceinline__ void sincos_fp64(double x, double* s_out, double* c_out) {
-  double const k = __builtin_rint(x * kTwoOverPi);
-  double r = __builtin_fma(-k, kPio2Hi, x);
-  r = __builtin_fma(-k, kPio2Lo, r);
-  int const q = __double2loint(k + 0x1.0p52);
-  double const r2 = r * r;
-  // sin(r) = r + r^3 * P(r^2),  P = -1/3! + r^2/5! - ... - r^14/17!
-  double ps = -2.8114572543455206e-15;
-  ps = __builtin_fma(ps, r2, 7.647163731819816e-13);
-  ps = __builtin_fma(ps, r2, -1.6059043836821613e-10);
-  ps = __builtin_fma(ps, r2, 2.505210838544172e-08);
-  ps = __builtin_fma(ps, r2, -2.7557319223985893e-06);
-  ps = __builtin_fma(ps, r2, 0.0001984126984126984);
-  ps = __builtin_fma(ps, r2, -0.008333333333333333);
-  ps = __builtin_fma(ps, r2, 0.16666666666666666);
-  double const sn = __builtin_fma(-(r * r2), ps, r);
-  // cos(r) = 1 + r^2 * Q(r^2),  Q = -1/2! + r^2/4! - ... + r^14/16!
-  double pc = 4.779477332387385e-14;
-  pc = __builtin_fma(pc, r2, -1.1470745597729725e-11);
-  pc = __builtin_fma(pc, r2, 2.08767569878681e-09);
-  pc = __builtin_fma(pc, r2, -2.755731922398589e-07);
-  pc = __builtin_fma(pc, r2, 2.48015873015873e-05);
-  pc = __builtin_fma(pc, r2, -0.001388888888888889);
-  pc = __builtin_fma(pc, r2, 0.041666666666666664);
-  pc = __builtin_fma(pc, r2, -0.5);
-  double const cs = __builtin_fma(r2, pc, 1.0);
-  // quadrant q: (sin, cos) of x = (s, c), (c, -s), (-s, -c), (-c, s)
-  double const s1 = (q & 1) ? cs : sn;
-  double const c1 = (q & 1) ? sn : cs;
-  *s_out = (q & 2) ? -s1 : s1;
-  *c_out = ((q + 1) & 2) ? -c1 : c1;
-}
-
-// Correctly rounded quotient a/b from the correctly rounded reciprocal y = RN(1/b):
-// q = RN(a*y), r = a - b*q (exact with an FMA), RN(q + r*y) = RN(a/b) (Markstein's
-// correction step).  Three operations instead of a full IEEE division; the
-// reference rounds every a/b it computes, so the quotients must match it bit for bit.
-__device__ __forceinline__ double div_rn(double a, double b, double y) {
-  double const q = a * y;
-  double const r = __builtin_fma(-q, b, a);
-  return __builtin_fma(r, y, q);
-}
-constexpr double kInvC = 0x1.ca726eb25f9dbp-29;  // RN(1/c)
-
-// Reduce an FP64 phase to r in [-pi, pi] (two FMAs, same argument as above with
-// 2*pi) for the single-precision trig of the mixed form.
-constexpr double kInvTwoPi = 0x1.45f306dc9c883p-3;
-constexpr double kTwoPiHi = 0x1.921fb54442d18p+2;
-constexpr double kTwoPiLo = 0x1.1a62633145c07p-52;
-
-__device__ __forceinline__ double reduce_two_pi(double x) {
-  double const k = __builtin_rint(x * kInvTwoPi);
-  double r = __builtin_fma(-k, kTwoPiHi, x);
-  return __builtin_fma(-k, kTwoPiLo, r);
-}
-
 }  // namespace srs
+
+#define SRS_FN __device__ __forceinline__
+#include "srs_math.h"

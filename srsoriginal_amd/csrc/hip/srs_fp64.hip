@@ -16,76 +16,48 @@
 // reads by broadcast.  Each chunk writes FP64 partial sums; a second kernel adds
 // the chunks in fixed order (deterministic) and applies the prefactors.
 #include "srs_kernels.h"
+#include "srs_terms.h"
 
 namespace srs {
 namespace {
 
 constexpr int TILE = kBlock;
 
-template <int NE>
+template <int NE, int TILE_>
 __global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const double* __restrict__ obs, int ns,
                                                             EnergySet es, int chunk, double* __restrict__ part) {
-  __shared__ double sx[TILE], sy[TILE], sz[TILE], sbx[TILE], sby[TILE], sbz[TILE];
+  __shared__ FluxTile<TILE_> T;
   int const tid = threadIdx.x;
   int const j = blockIdx.x * kBlock + tid;
   int const jj = j < ns ? j : ns - 1;  // padding lanes compute a valid point, never store
   double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
   int64_t const i0 = (int64_t)blockIdx.y * chunk;
   int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
-  double are[NE][3], aim[NE][3];
-#pragma unroll
-  for (int e = 0; e < NE; ++e)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) are[e][k] = aim[e][k] = 0.0;
-
-  for (int64_t base = i0; base < i1; base += TILE) {
-    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+  AccFP64<NE> a;
+  a.zero();
+  for (int64_t base = i0; base < i1; base += TILE_) {
+    int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
     __syncthreads();
-    if (tid < cnt) {
-      sx[tid] = t.x[base + tid];
-      sy[tid] = t.y[base + tid];
-      sz[tid] = t.z[base + tid];
-      sbx[tid] = t.bx[base + tid];
-      sby[tid] = t.by[base + tid];
-      sbz[tid] = t.bz[base + tid];
+    for (int k = tid; k < cnt; k += kBlock) {
+      T.x[k] = t.x[base + k];
+      T.y[k] = t.y[base + k];
+      T.z[k] = t.z[base + k];
+      T.bx[k] = t.bx[base + k];
+      T.by[k] = t.by[base + k];
+      T.bz[k] = t.bz[base + k];
     }
     __syncthreads();
-    for (int k = 0; k < cnt; ++k) {
-      double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
-      double const d2 = rx * rx + ry * ry + rz * rz;
-      double const D = __builtin_sqrt(d2);
-      double const tau = t.dt * (double)(base + k) + div_rn(D, kC, kInvC);
-      // N = R/D rounded as the reference rounds it: beta_z - n_z cancels to ~1e-8
-      double const inv = 1.0 / D;
-      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
-      double const ax = (sbx[k] - nx) * inv, ay = (sby[k] - ny) * inv, az = (sbz[k] - nz) * inv;
-      double const inv2 = inv * inv;
-#pragma unroll
-      for (int e = 0; e < NE; ++e) {
-        double s, c;
-        sincos_fp64(es.omega[e] * tau, &s, &c);
-        double const g = es.cw[e] * inv2;
-        double const gx = nx * g, gy = ny * g, gz = nz * g;
-        // term = (a - i g n) (c + i s)
-        are[e][0] = __builtin_fma(ax, c, __builtin_fma(gx, s, are[e][0]));
-        are[e][1] = __builtin_fma(ay, c, __builtin_fma(gy, s, are[e][1]));
-        are[e][2] = __builtin_fma(az, c, __builtin_fma(gz, s, are[e][2]));
-        aim[e][0] = __builtin_fma(ax, s, __builtin_fma(-gx, c, aim[e][0]));
-        aim[e][1] = __builtin_fma(ay, s, __builtin_fma(-gy, c, aim[e][1]));
-        aim[e][2] = __builtin_fma(az, s, __builtin_fma(-gz, c, aim[e][2]));
-      }
-    }
+    flux_terms_fp64<NE, TILE_>(T, cnt, base, t.dt, ox, oy, oz, es, a);
   }
   if (j < ns) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
-      part[b + 0 * (size_t)ns] = are[e][0];
-      part[b + 1 * (size_t)ns] = aim[e][0];
-      part[b + 2 * (size_t)ns] = are[e][1];
-      part[b + 3 * (size_t)ns] = aim[e][1];
-      part[b + 4 * (size_t)ns] = are[e][2];
-      part[b + 5 * (size_t)ns] = aim[e][2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        part[b + (2 * k) * (size_t)ns] = a.re[e][k];
+        part[b + (2 * k + 1) * (size_t)ns] = a.im[e][k];
+      }
     }
   }
 }
@@ -290,10 +262,10 @@ hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, co
                             double* part, hipStream_t s) {
   dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
   switch (es.ne) {
-    case 1: flux_fp64_kernel<1><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
-    case 2: flux_fp64_kernel<2><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
-    case 3: flux_fp64_kernel<3><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
-    case 4: flux_fp64_kernel<4><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 1: flux_fp64_kernel<1, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 2: flux_fp64_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 3: flux_fp64_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 4: flux_fp64_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -12,29 +12,15 @@
 // accumulators with Kahan compensation; each trajectory chunk writes FP64 partial
 // sums and the cross-chunk reduction is FP64 (shared with the plain form).
 #include "srs_kernels.h"
+#include "srs_terms.h"
 
 namespace srs {
 namespace {
 
 constexpr int TILE = kBlock;
-constexpr int SUB = 32;  // FP32 sub-tile length
+constexpr int SUB = kSub;
 
-__device__ __forceinline__ void sincos_reduced_f32(double r, float* s, float* c) {
-  // r in [-pi, pi] -> revolutions for the hardware sin/cos
-  float const rev = (float)(r * 0.15915494309189535);
-  *s = __builtin_amdgcn_sinf(rev);
-  *c = __builtin_amdgcn_cosf(rev);
-}
 
-struct Kahan {
-  float s, c;
-  __device__ __forceinline__ void add(float v) {
-    float const y = v - c;
-    float const t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-};
 
 // trajf layout (float): [0] bx, [1] by, [2] 1 - bz, and with_a: [3] ax, [4] ay, [5] az
 // (the flux and spectrum paths stage no accelerations: t.ax..az are NULL there).
@@ -51,12 +37,11 @@ __global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f
   }
 }
 
-template <int NE>
+template <int NE, int TILE_>
 __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const float* __restrict__ tf,
                                                              const double* __restrict__ obs, int ns, EnergySet es,
                                                              int chunk, double* __restrict__ part) {
-  __shared__ double sx[TILE], sy[TILE], sz[TILE], st[TILE];
-  __shared__ float sbx[TILE], sby[TILE], somb[TILE];
+  __shared__ FluxTile<TILE_> T;
   int const tid = threadIdx.x;
   int const j = blockIdx.x * kBlock + tid;
   int const jj = j < ns ? j : ns - 1;
@@ -66,72 +51,29 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
   for (int e = 0; e < NE; ++e) cwf[e] = (float)es.cw[e];
   int64_t const i0 = (int64_t)blockIdx.y * chunk;
   int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
-  Kahan acc[NE][6];
-#pragma unroll
-  for (int e = 0; e < NE; ++e)
-#pragma unroll
-    for (int k = 0; k < 6; ++k) acc[e][k] = Kahan{0.f, 0.f};
-
-  for (int64_t base = i0; base < i1; base += TILE) {
-    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+  AccMixed<NE> a;
+  a.zero();
+  for (int64_t base = i0; base < i1; base += TILE_) {
+    int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
     __syncthreads();
-    if (tid < cnt) {
-      int64_t const i = base + tid;
-      sx[tid] = t.x[i];
-      sy[tid] = t.y[i];
-      sz[tid] = t.z[i];
-      st[tid] = t.dt * (double)i;
-      sbx[tid] = tf[i];
-      sby[tid] = tf[t.n + i];
-      somb[tid] = tf[2 * t.n + i];
+    for (int k = tid; k < cnt; k += kBlock) {
+      int64_t const i = base + k;
+      T.x[k] = t.x[i];
+      T.y[k] = t.y[i];
+      T.z[k] = t.z[i];
+      T.fbx[k] = tf[i];
+      T.fby[k] = tf[t.n + i];
+      T.fomb[k] = tf[2 * t.n + i];
     }
     __syncthreads();
-    for (int k0 = 0; k0 < cnt; k0 += SUB) {
-      int const k1 = (k0 + SUB < cnt) ? k0 + SUB : cnt;
-      float sub[NE][6];
-#pragma unroll
-      for (int e = 0; e < NE; ++e)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) sub[e][q] = 0.f;
-      for (int k = k0; k < k1; ++k) {
-        double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
-        // D and tau exactly as the reference rounds them: a one-Newton-step rsq sqrt is
-        // off by up to ~20 ulp of D, which is 3e-4 rad of phase at 430 eV.
-        double const d2 = rx * rx + ry * ry + rz * rz;
-        double const D = __builtin_sqrt(d2);
-        double const tau = st[k] + div_rn(D, kC, kInvC);
-        float const inv = __builtin_amdgcn_rcpf((float)D);
-        float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
-        float const v = __builtin_fmaf(nx, nx, ny * ny);
-        float const omnz = nz > 0.f ? v * __builtin_amdgcn_rcpf(1.f + nz) : 1.f - nz;
-        float const ax = (sbx[k] - nx) * inv, ay = (sby[k] - ny) * inv, az = (omnz - somb[k]) * inv;
-        float const g = inv * inv;
-        float const gx = nx * g, gy = ny * g, gz = nz * g;
-#pragma unroll
-        for (int e = 0; e < NE; ++e) {
-          float s, c;
-          sincos_reduced_f32(reduce_two_pi(es.omega[e] * tau), &s, &c);
-          float const cs = cwf[e] * s, cc = cwf[e] * c;
-          sub[e][0] = __builtin_fmaf(ax, c, __builtin_fmaf(gx, cs, sub[e][0]));
-          sub[e][1] = __builtin_fmaf(ax, s, __builtin_fmaf(-gx, cc, sub[e][1]));
-          sub[e][2] = __builtin_fmaf(ay, c, __builtin_fmaf(gy, cs, sub[e][2]));
-          sub[e][3] = __builtin_fmaf(ay, s, __builtin_fmaf(-gy, cc, sub[e][3]));
-          sub[e][4] = __builtin_fmaf(az, c, __builtin_fmaf(gz, cs, sub[e][4]));
-          sub[e][5] = __builtin_fmaf(az, s, __builtin_fmaf(-gz, cc, sub[e][5]));
-        }
-      }
-#pragma unroll
-      for (int e = 0; e < NE; ++e)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) acc[e][q].add(sub[e][q]);
-    }
+    flux_terms_mixed<NE, TILE_>(T, cnt, base, t.dt, ox, oy, oz, es, cwf, a);
   }
   if (j < ns) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = (double)acc[e][q].s - (double)acc[e][q].c;
+      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = a.value(e, q);
     }
   }
 }
@@ -163,11 +105,11 @@ __global__ __launch_bounds__(kBlock) void spectrum_mixed_kernel(TrajView t, doub
     int const e = (e0 + r < ne) ? e0 + r : ne - 1;
     cw[r] = (float)(kC / omega[e]);
   }
-  Kahan ex[RUN][6];
+  KahanF ex[RUN][6];
 #pragma unroll
   for (int r = 0; r < RUN; ++r)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) ex[r][q] = Kahan{0.f, 0.f};
+    for (int q = 0; q < 6; ++q) ex[r][q] = KahanF{0.f, 0.f};
 
   for (int64_t base = i0; base < i1; base += TILE) {
     int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
@@ -251,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void pd_mixed_kernel(TrajView t, const floa
   float const qx = (float)nrm[jj], qy = (float)nrm[ns + jj], qz = (float)nrm[2 * ns + jj];
   int64_t const i0 = (int64_t)blockIdx.y * chunk;
   int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
-  Kahan acc{0.f, 0.f};
+  KahanF acc{0.f, 0.f};
   for (int64_t base = i0; base < i1; base += TILE) {
     int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
     __syncthreads();
@@ -313,10 +255,10 @@ hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double
                              int chunk, double* part, hipStream_t s) {
   dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
   switch (es.ne) {
-    case 1: flux_mixed_kernel<1><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
-    case 2: flux_mixed_kernel<2><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
-    case 3: flux_mixed_kernel<3><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
-    case 4: flux_mixed_kernel<4><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 1: flux_mixed_kernel<1, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 2: flux_mixed_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 3: flux_mixed_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 4: flux_mixed_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

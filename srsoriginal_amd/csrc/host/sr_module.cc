@@ -147,6 +147,35 @@ PYBIND11_MODULE(_sr, m) {
         return std::vector<double>{e.GetCurrentParticle().q, e.GetCurrentParticle().current};
       })
       .def("particle_type", [](srs::OSCARSSR const& e) { return e.GetCurrentParticle().type; })
+      .def("particle_state",
+           [](srs::OSCARSSR const& e) {
+             auto const& p = e.GetCurrentParticle();
+             py::dict d;
+             d["x0"] = std::vector<double>{p.x0.x, p.x0.y, p.x0.z};
+             d["beta0"] = std::vector<double>{p.beta0.x, p.beta0.y, p.beta0.z};
+             d["qmg"] = p.qmg;
+             d["t0"] = p.t0;
+             d["gamma"] = p.gamma;
+             d["charge"] = p.q;
+             d["current"] = p.current;
+             return d;
+           })
+      .def("undulators",
+           [](srs::OSCARSSR const& e) {
+             py::list out;
+             for (auto const& u : e.Fields().fields()) {
+               py::dict d;
+               d["field"] = std::vector<double>{u.field.x, u.field.y, u.field.z};
+               d["period"] = std::vector<double>{u.period.x, u.period.y, u.period.z};
+               d["nperiods"] = u.nperiods;
+               d["phase"] = u.phase;
+               d["taper"] = u.taper;
+               d["center"] = std::vector<double>{u.center.x, u.center.y, u.center.z};
+               d["pole_delta"] = u.pole_delta.empty() ? py::object(py::none()) : py::object(py::cast(u.pole_delta));
+               out.append(d);
+             }
+             return out;
+           })
       .def("set_seed", &srs::OSCARSSR::SetSeed)
       .def("rand", [](srs::OSCARSSR const&) { return srs::Random::global().uniform(); })
       .def("norm", [](srs::OSCARSSR const&) { return srs::Random::global().normal(); })
