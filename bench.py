@@ -31,6 +31,13 @@ ENERGY_EV = 143.52176088044024   # first-harmonic argmax of the reference's C0 s
 F_ALG_FLUX = 53                  # algorithmic FLOP per flux term
 PEAK_FP32_VALU = 157.3e12        # MI355X FP32 vector peak (published)
 PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
+# Non-packed FP32 issues at the FP64 rate (64 lanes per CU per cycle); the packed FP32
+# instructions behind the 157.3 TF/s figure are not used by this build.
+PEAK_FP32_NONPACKED = 78.6e12
+# HBM bytes per launch of flux_mixed_kernel<1,256> on this workload, from rocprofv3
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
+# what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
+TRAFFIC_MIXED_BYTES = 32790.7 * 1024
 
 
 def log(msg):
@@ -122,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="one launch of each form plus a 256 MiB device copy (the HBM counter scale), for rocprofv3")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.exit("bench.py measures one GPU: this build runs on a single MI355X (multi-GPU sharding is design-only)")
@@ -142,6 +151,16 @@ def main():
     def step(prec):
         abi.flux_device(ptrs, n, dt, d_obs.data_ptr(), ns, [ENERGY_EV], q, cur, prec, d_out[prec].data_ptr(),
                         stream.cuda_stream)
+
+    if a.profile:
+        for prec in (abi.FP64, abi.MIXED):
+            step(prec)
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("profile run: one fp64 and one mixed C1 launch, one 256 MiB copy")
+        return
 
     res = {}
     for prec in (abi.FP64, abi.MIXED):
@@ -190,8 +209,9 @@ def main():
         "mixed_vs_fp64": {"speedup": res[abi.FP64]["ms"] / res[abi.MIXED]["ms"], "max_rel_err": max_rel,
                           "rms_rel_err": rms_rel},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VALU / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_VALU, "traffic": None,
-                     "kernel": "flux_mixed_kernel<1>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
+                     "frac": achieved / PEAK_FP32_VALU, "traffic": TRAFFIC_MIXED_BYTES,
+                     "frac_of_nonpacked_fp32": achieved / PEAK_FP32_NONPACKED,
+                     "kernel": "flux_mixed_kernel<1,256>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
                      "flop_per_term": F_ALG_FLUX},
     }
     if not a.no_batch:

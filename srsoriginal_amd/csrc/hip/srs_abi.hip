@@ -28,6 +28,7 @@ using srs::TrajView;
 // Reference constants, TOSCARSSR.h:20-69.
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kFourPi = 4.0 * kPi;
+constexpr double kTwoPi = 2.0 * kPi;
 constexpr double kPi2 = kPi * kPi;
 constexpr double kSqrt2Pi = 2.5066282746310002416123552393401;
 constexpr double kC = 2.99792458e8;
@@ -202,6 +203,8 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
       es.omega[e] = w;
       es.cw[e] = kC / w;
       es.kw[e] = (c0 * w) * t.dt;
+      es.fr[e] = w / kTwoPi;
+      es.fw[e] = es.fr[e] / kC;
     }
     {
       KernelTimer timer(c, s);
@@ -337,18 +340,32 @@ int stage_undulators(Context& c, const srs_undulator* und, int nund, hipStream_t
   std::vector<double> poles;
   for (int u = 0; u < nund; ++u) {
     srs::DevUndulator& d = du[u];
+    srs_undulator const& h = und[u];
+    double const pm = std::sqrt(h.period[0] * h.period[0] + h.period[1] * h.period[1] + h.period[2] * h.period[2]);
+    if (!(pm > 0)) return fail(SRS_ERR_INVALID, "undulator period must be a nonzero vector");
+    bool const poled = h.pole_delta != nullptr;
     for (int k = 0; k < 3; ++k) {
-      d.field[k] = und[u].field[k];
-      d.period[k] = und[u].period[k];
-      d.center[k] = und[u].center[k];
+      d.field[k] = h.field[k];
+      d.u[k] = h.period[k] / pm;
+      d.center[k] = poled ? 0.0 : h.center[k];
     }
-    d.phase = und[u].phase;
-    d.taper = und[u].taper;
-    d.nperiods = und[u].nperiods;
+    double const lam = pm, len = pm * (h.nperiods + 2);
+    double const ps = poled ? 0.0 : h.phase * pm / kTwoPi;
+    d.lam = lam;
+    d.rlam = 1.0 / lam;
+    d.taper = poled ? 0.0 : h.taper;
+    d.ps = ps;
+    d.lo0 = -len / 2. + ps;
+    d.hi0 = len / 2. + ps;
+    d.lo1 = -len / 2. + ps + lam;
+    d.hi1 = len / 2. + ps - lam;
+    d.lo2 = -len / 2. + ps + lam / 2.;
+    d.hi2 = len / 2. + ps - lam / 2.;
+    d.nperiods = h.nperiods;
     d.pole_off = -1;
-    if (und[u].pole_delta) {
+    if (poled) {
       d.pole_off = (int)poles.size();
-      poles.insert(poles.end(), und[u].pole_delta, und[u].pole_delta + 2 * und[u].nperiods);
+      poles.insert(poles.end(), h.pole_delta, h.pole_delta + 2 * h.nperiods);
     }
   }
   if (poles.empty()) poles.push_back(0.0);
@@ -607,15 +624,17 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
   HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
   TRY(stage_points(c, c.b_obs, obs_xyz, ns));
   double const c0 = c0_of(charge);
-  std::vector<double> eh(3 * ne);
+  std::vector<double> eh(5 * ne);
   for (int64_t e = 0; e < ne; ++e) {
     double const w = omega_of(energies_ev[e]);
     eh[e] = w;
     eh[ne + e] = kC / w;
     eh[2 * ne + e] = c0 * w;
+    eh[3 * ne + e] = w / kTwoPi;
+    eh[4 * ne + e] = eh[3 * ne + e] / kC;
   }
-  HIPCHK(c.b_energies.ensure(3 * ne * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 3 * ne * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c.b_energies.ensure(5 * ne * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 5 * ne * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(c.b_flux_pp.ensure((size_t)np * ne * ns * sizeof(double)));
   HIPCHK(c.b_out.ensure((size_t)ne * ns * sizeof(double)));
   HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
@@ -623,7 +642,7 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
   r.nund = nund;
   TRY(stage_batch_traj(c, total, false, &r.bt));
   double* ed = c.b_energies.as<double>();
-  r.be = srs::BatchEnergies{ed, ed + ne, ed + 2 * ne, (int)ne, c2_of(charge, current)};
+  r.be = srs::BatchEnergies{ed, ed + ne, ed + 2 * ne, ed + 3 * ne, ed + 4 * ne, (int)ne, c2_of(charge, current)};
   r.np = (int)np;
   r.ns = (int)ns;
   r.ne = (int)ne;

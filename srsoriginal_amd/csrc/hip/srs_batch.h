@@ -7,11 +7,19 @@
 
 namespace srs {
 
-// Ideal undulator on the device (TField3D_IdealUndulator); pole_off >= 0 selects
-// the per-pole scaled form with deltas poles[pole_off .. pole_off + 2N).
+// Ideal undulator on the device (TField3D_IdealUndulator).  The derived constants
+// are formed on the host with the reference's expressions (TField3D_IdealUndulator.cc
+// :50-139): u = period/|period|, lam = |period|, len = lam*(N+2), PS = phase*lam/2pi
+// and the region bounds, e.g. lo1 = (-len/2 + PS) + lam.  rlam = RN(1/lam) turns the
+// reference's divisions by lam into correctly rounded Markstein quotients.
+// pole_off >= 0 selects the per-pole scaled form (center, taper and phase unused:
+// center = taper = PS = 0 there) with deltas poles[pole_off .. pole_off + 2N).
 struct DevUndulator {
-  double field[3], period[3], center[3];
-  double phase, taper;
+  double field[3], u[3], center[3];
+  double lam, rlam, taper, ps;
+  double lo0, hi0;  // outside: D < lo0 or D > hi0
+  double lo1, hi1;  // termination region: D < lo1 or D > hi1
+  double lo2, hi2;  // outer half period (x0.25): D < lo2 or D > hi2, else x0.75
   int nperiods;
   int pole_off;
 };
@@ -35,6 +43,8 @@ struct BatchEnergies {
   const double* omega;  // device arrays of length ne
   const double* cw;     // c / omega
   const double* c0w;    // C0 * omega (times dt per particle gives kw)
+  const double* fr;     // omega / (2 pi)
+  const double* fw;     // omega / (2 pi C)
   int ne;
   double c2;
 };

@@ -19,6 +19,13 @@ namespace {
 
 // ---- device field and RK4 (reference: TField3D_IdealUndulator.cc:106-139,
 // OSCARSSR.cc:741-874; same operation order, device sine) ----------------------
+//
+// One lane integrates one particle; the RK4 chain is strictly sequential (20k steps
+// x 4 field evaluations), so its cost is the latency of that chain.  It is kept
+// short: the undulator records are loaded into registers once (NU <= 2 at compile
+// time), every division of the reference (2 pi D / lam, h / 6, v / c) is a
+// correctly rounded Markstein quotient from a reciprocal formed once (bitwise the
+// division), and the regions of the field are selected without branches.
 
 __device__ __forceinline__ double sin_fp64(double x) {
   double s, c;
@@ -26,74 +33,64 @@ __device__ __forceinline__ double sin_fp64(double x) {
   return s;
 }
 
-__device__ void field_eval(const DevUndulator* __restrict__ U, int nu, const double* __restrict__ poles, double X,
-                           double Y, double Z, double* B) {
-  double Sx = 0, Sy = 0, Sz = 0;
-  for (int u = 0; u < nu; ++u) {
-    DevUndulator const& f = U[u];
-    double const pm = __builtin_sqrt(f.period[0] * f.period[0] + f.period[1] * f.period[1] + f.period[2] * f.period[2]);
-    double const ux = f.period[0] / pm, uy = f.period[1] / pm, uz = f.period[2] / pm;
-    double const lam = pm;
-    double const len = pm * (f.nperiods + 2);
-    double bx = 0, by = 0, bz = 0;
-    if (f.pole_off >= 0) {
-      double const D = X * ux + Y * uy + Z * uz;
-      double const s = sin_fp64(2.0 * kPi * D / lam);
-      if (!(D > len / 2. || D < -len / 2.)) {
-        if (D < -len / 2. + lam || D > len / 2. - lam) {
-          double const fct = (D < -len / 2. + lam / 2. || D > len / 2. - lam / 2.) ? 0.25 : 0.75;
-          bx = (fct * f.field[0]) * s;
-          by = (fct * f.field[1]) * s;
-          bz = (fct * f.field[2]) * s;
-        } else {
-          int p = (int)__builtin_floor((D - (-len / 2. + lam)) / (lam / 2.));
-          p = p < 0 ? 0 : (p > 2 * f.nperiods - 1 ? 2 * f.nperiods - 1 : p);
-          double const sc = 1.0 + poles[f.pole_off + p];
-          bx = (f.field[0] * s) * sc;
-          by = (f.field[1] * s) * sc;
-          bz = (f.field[2] * s) * sc;
-        }
-      }
-    } else {
-      double const D = (X - f.center[0]) * ux + (Y - f.center[1]) * uy + (Z - f.center[2]) * uz;
-      double const T = 1 + D * f.taper;
-      double const PS = f.phase * pm / (2.0 * kPi);
-      if (!(D > len / 2. + PS || D < -len / 2. + PS)) {
-        double const s = sin_fp64((2.0 * kPi) * (D - PS) / lam);
-        double fct = 1.0;
-        bool term = false;
-        if (D < -len / 2. + PS + lam || D > len / 2. + PS - lam) {
-          term = true;
-          fct = (D < -len / 2. + PS + lam / 2. || D > len / 2. + PS - lam / 2.) ? 0.25 : 0.75;
-        }
-        if (term) {
-          bx = ((f.field[0] * fct) * s) * T;
-          by = ((f.field[1] * fct) * s) * T;
-          bz = ((f.field[2] * fct) * s) * T;
-        } else {
-          bx = (f.field[0] * s) * T;
-          by = (f.field[1] * s) * T;
-          bz = (f.field[2] * s) * T;
-        }
-      }
-    }
-    Sx += bx;
-    Sy += by;
-    Sz += bz;
+constexpr double kInvSix = 0x1.5555555555555p-3;  // RN(1/6)
+
+// Field of one undulator at (X, Y, Z), added to B (TFieldContainer::GetF sums from zero).
+__device__ __forceinline__ void field_add(DevUndulator const& f, const double* __restrict__ poles, double X, double Y,
+                                          double Z, double* B) {
+  double const D = (X - f.center[0]) * f.u[0] + (Y - f.center[1]) * f.u[1] + (Z - f.center[2]) * f.u[2];
+  double const T = 1 + D * f.taper;
+  bool const inside = !(D > f.hi0 || D < f.lo0);
+  double const s = sin_fp64(div_rn(2.0 * kPi * (D - f.ps), f.lam, f.rlam));
+  bool const term = D < f.lo1 || D > f.hi1;
+  double const fct = (D < f.lo2 || D > f.hi2) ? 0.25 : 0.75;
+  double g = T;
+  if (f.pole_off >= 0) {  // uniform across the wave
+    int p = (int)__builtin_floor(div_rn(D - f.lo1, f.lam / 2., 2.0 * f.rlam));
+    p = p < 0 ? 0 : (p > 2 * f.nperiods - 1 ? 2 * f.nperiods - 1 : p);
+    g = 1.0 + poles[f.pole_off + p];
   }
-  B[0] = Sx;
-  B[1] = Sy;
-  B[2] = Sz;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double const b = term ? ((f.field[c] * fct) * s) * T : (f.field[c] * s) * g;
+    B[c] += inside ? b : 0.0;
+  }
 }
 
-struct RK4Ctx {
+// Undulator records: NU > 0 held in registers, NU == 0 read from memory (any count).
+template <int NU>
+struct Fields {
+  DevUndulator u[NU];
+  const double* poles;
+  __device__ __forceinline__ Fields(const DevUndulator* U, int, const double* p) : poles(p) {
+#pragma unroll
+    for (int k = 0; k < NU; ++k) u[k] = U[k];
+  }
+  __device__ __forceinline__ void eval(double X, double Y, double Z, double* B) const {
+    B[0] = B[1] = B[2] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) field_add(u[k], poles, X, Y, Z, B);
+  }
+};
+template <>
+struct Fields<0> {
   const DevUndulator* U;
   int nu;
   const double* poles;
+  __device__ __forceinline__ Fields(const DevUndulator* U_, int n, const double* p) : U(U_), nu(n), poles(p) {}
+  __device__ __forceinline__ void eval(double X, double Y, double Z, double* B) const {
+    B[0] = B[1] = B[2] = 0.0;
+    for (int k = 0; k < nu; ++k) field_add(U[k], poles, X, Y, Z, B);
+  }
+};
+
+template <int NU>
+struct RK4Ctx {
+  Fields<NU> F;
   double qmg;
   __device__ __forceinline__ void derivs(const double* x, double* d) const {
     double B[3];
-    field_eval(U, nu, poles, x[0], x[2], x[4], B);
+    F.eval(x[0], x[2], x[4], B);
     d[0] = x[1];
     d[1] = qmg * (-x[5] * B[1] + x[3] * B[2]);
     d[2] = x[3];
@@ -101,18 +98,22 @@ struct RK4Ctx {
     d[4] = x[5];
     d[5] = qmg * (x[1] * B[1] - x[3] * B[0]);
   }
-  __device__ __forceinline__ void step(double* y, const double* dydx, double h) const {
+  // OSCARSSR::RK4 (OSCARSSR.cc:826-874); hh = h/2 and h6 = h/6 are formed by the caller
+  __device__ __forceinline__ void step(double* y, const double* dydx, double h, double hh, double h6) const {
     double dym[6], dyt[6], yt[6];
-    double const hh = h * 0.5, h6 = h / 6.0;
+#pragma unroll
     for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
     derivs(yt, dyt);
+#pragma unroll
     for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
     derivs(yt, dym);
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
       yt[i] = y[i] + h * dym[i];
       dym[i] += dyt[i];
     }
     derivs(yt, dyt);
+#pragma unroll
     for (int i = 0; i < 6; ++i) y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
   }
 };
@@ -121,7 +122,7 @@ __device__ __forceinline__ void store_point(BatchTraj const& bt, int64_t k, cons
   bt.x[k] = x[0];
   bt.y[k] = x[2];
   bt.z[k] = x[4];
-  double const bx = x[1] / kC, by = x[3] / kC, bz = x[5] / kC;
+  double const bx = div_rn(x[1], kC, kInvC), by = div_rn(x[3], kC, kInvC), bz = div_rn(x[5], kC, kInvC);
   bt.bx[k] = bx;
   bt.by[k] = by;
   bt.bz[k] = bz;
@@ -129,21 +130,23 @@ __device__ __forceinline__ void store_point(BatchTraj const& bt, int64_t k, cons
   bt.fby[k] = (float)by;
   bt.fomb[k] = (float)(1.0 - bz);
   if (bt.ax) {
-    bt.ax[k] = d[1] / kC;
-    bt.ay[k] = d[3] / kC;
-    bt.az[k] = d[5] / kC;
+    bt.ax[k] = div_rn(d[1], kC, kInvC);
+    bt.ay[k] = div_rn(d[3], kC, kInvC);
+    bt.az[k] = div_rn(d[5], kC, kInvC);
   }
 }
 
 // Same point layout as the host trajectory (OSCARSSR.cc:562-673): forward pass from
 // T0 stores the state before each step with the previous derivative (A/c lags one
 // point; 0 at the first point), the backward pass fills the nb points before T0.
-__global__ void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu, const double* __restrict__ poles,
-                                 const BatchParticle* __restrict__ P, int np, BatchTraj bt) {
+template <int NU>
+__global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu,
+                                                        const double* __restrict__ poles,
+                                                        const BatchParticle* __restrict__ Ps, int np, BatchTraj bt) {
   int const p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  BatchParticle const q = P[p];
-  RK4Ctx const R{U, nu, poles, q.qmg};
+  BatchParticle const q = Ps[p];
+  RK4Ctx<NU> const R{Fields<NU>(U, nu, poles), q.qmg};
   double x[6], d[6] = {0, 0, 0, 0, 0, 0};
   auto init = [&]() {
     x[0] = q.x0[0];
@@ -154,16 +157,22 @@ __global__ void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu, con
     x[5] = q.beta0[2] * kC;
   };
   init();
-  for (int64_t i = 0; i < q.nf; ++i) {
-    store_point(bt, q.off + q.nb + i, x, d);
-    R.derivs(x, d);
-    R.step(x, d, q.dt);
+  {
+    double const h = q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
+    for (int64_t i = 0; i < q.nf; ++i) {
+      store_point(bt, q.off + q.nb + i, x, d);
+      R.derivs(x, d);
+      R.step(x, d, h, hh, h6);
+    }
   }
   init();
-  for (int64_t i = 0; i < q.nb; ++i) {
-    R.derivs(x, d);
-    R.step(x, d, -q.dt);
-    store_point(bt, q.off + q.nb - 1 - i, x, d);
+  {
+    double const h = -q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
+    for (int64_t i = 0; i < q.nb; ++i) {
+      R.derivs(x, d);
+      R.step(x, d, h, hh, h6);
+      store_point(bt, q.off + q.nb - 1 - i, x, d);
+    }
   }
 }
 
@@ -171,15 +180,30 @@ __global__ void zero_counter_kernel(unsigned* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
 }
 
+// LDS of the batched kernel: the FP64 tile, or the mixed form's union.
+template <int TILE_, int NE, bool MIXED>
+struct BatchTile {
+  FluxTile<TILE_> raw;
+};
+template <int TILE_, int NE>
+struct BatchTile<TILE_, NE, true> {
+  union {
+    FluxTile<TILE_> raw;
+    ParaTile<TILE_, NE> para;
+  };
+};
+
 // Persistent flux kernel.  Item = (particle, observation tile of 256 points, group of
 // NE energies), popped from a global counter by lane 0 and broadcast through LDS.
-// Every block exits when the counter passes the item count.
+// Every block exits when the counter passes the item count.  The mixed form takes
+// the paraxial terms for every trajectory tile that passes their bounds.
 template <int NE, int TILE_, bool MIXED>
 __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                             const double* __restrict__ obs, int ns, BatchEnergies be,
                                                             unsigned nitems, unsigned* __restrict__ counter,
                                                             double* __restrict__ flux_pp) {
-  __shared__ FluxTile<TILE_> T;
+  __shared__ BatchTile<TILE_, NE, MIXED> T;
+  __shared__ double red[4 * (kBlock / 64)];
   __shared__ unsigned s_item;
   int const tid = threadIdx.x;
   int const ntiles = (ns + kBlock - 1) / kBlock;
@@ -205,35 +229,62 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
       es.omega[e] = be.omega[ee];
       es.cw[e] = be.cw[ee];
       es.kw[e] = be.c0w[ee] * q.dt;  // (C0*omega)*dt, OSCARSSR.cc:2080
+      es.fr[e] = be.fr[ee];
+      es.fw[e] = be.fw[ee];
       cwf[e] = (float)be.cw[ee];
     }
-    int const j = tile * kBlock + tid;
-    int const jj = j < ns ? j : ns - 1;
+    int const j0 = tile * kBlock;
+    int const j = j0 + tid;
+    int const jj = j < ns ? j : j0;  // idle lanes repeat the tile's first point
     double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
     AccFP64<NE> a64;
     AccMixed<NE> a32;
-    if (MIXED) a32.zero(); else a64.zero();
-    for (int64_t base = 0; base < n; base += TILE_) {
-      int const cnt = (n - base) < TILE_ ? (int)(n - base) : TILE_;
-      __syncthreads();
-      for (int k = tid; k < cnt; k += kBlock) {
-        int64_t const i = q.off + base + k;
-        T.x[k] = bt.x[i];
-        T.y[k] = bt.y[i];
-        T.z[k] = bt.z[i];
-        if (MIXED) {
-          T.fbx[k] = bt.fbx[i];
-          T.fby[k] = bt.fby[i];
-          T.fomb[k] = bt.fomb[i];
-        } else {
-          T.bx[k] = bt.bx[i];
-          T.by[k] = bt.by[i];
-          T.bz[k] = bt.bz[i];
+    bool para = false;
+    ParaBlock pb{};
+    if constexpr (MIXED) {
+      a32.zero();
+      pb = para_block(ox, oy, oz, obs[2 * ns + j0], red);
+      para = para_check(pb, bt.x + q.off, bt.y + q.off, bt.z + q.off, 0, n, es);
+    } else {
+      a64.zero();
+    }
+    if (para) {
+      if constexpr (MIXED) {
+        for (int64_t base = 0; base < n; base += TILE_) {
+          int const cnt = (n - base) < TILE_ ? (int)(n - base) : TILE_;
+          __syncthreads();
+          for (int k = tid; k < cnt; k += kBlock) {
+            int64_t const i = q.off + base + k;
+            para_stage<TILE_, NE>(T.para, k, bt.x[i], bt.y[i], bt.z[i], bt.fbx[i], bt.fby[i], bt.fomb[i], base + k,
+                                  q.dt, pb, es);
+          }
+          __syncthreads();
+          flux_terms_para<NE, TILE_>(T.para, cnt, ox, oy, es, cwf, a32);
         }
       }
-      __syncthreads();
-      if (MIXED) flux_terms_mixed<NE, TILE_>(T, cnt, base, q.dt, ox, oy, oz, es, cwf, a32);
-      else flux_terms_fp64<NE, TILE_>(T, cnt, base, q.dt, ox, oy, oz, es, a64);
+    } else {
+      for (int64_t base = 0; base < n; base += TILE_) {
+        int const cnt = (n - base) < TILE_ ? (int)(n - base) : TILE_;
+        __syncthreads();
+        for (int k = tid; k < cnt; k += kBlock) {
+          int64_t const i = q.off + base + k;
+          T.raw.x[k] = bt.x[i];
+          T.raw.y[k] = bt.y[i];
+          T.raw.z[k] = bt.z[i];
+          if (MIXED) {
+            T.raw.fbx[k] = bt.fbx[i];
+            T.raw.fby[k] = bt.fby[i];
+            T.raw.fomb[k] = bt.fomb[i];
+          } else {
+            T.raw.bx[k] = bt.bx[i];
+            T.raw.by[k] = bt.by[i];
+            T.raw.bz[k] = bt.bz[i];
+          }
+        }
+        __syncthreads();
+        if (MIXED) flux_terms_mixed<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, cwf, a32);
+        else flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, a64);
+      }
     }
     if (j < ns) {
 #pragma unroll
@@ -333,7 +384,10 @@ int batch_max_blocks(BatchConfig const& c, bool mixed) {
 
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
                             BatchTraj const& bt, hipStream_t s) {
-  rk4_batch_kernel<<<(np + 63) / 64, 64, 0, s>>>(U, nu, poles, P, np, bt);
+  dim3 const grid((np + 63) / 64);
+  if (nu == 1) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt);
+  else if (nu == 2) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt);
+  else rk4_batch_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt);
   return hipGetLastError();
 }
 

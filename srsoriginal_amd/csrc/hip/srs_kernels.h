@@ -11,12 +11,15 @@ constexpr int kMaxFluxEnergies = 4;   // energies handled per flux launch
 constexpr int kBlock = 256;           // threads per block (4 wave64)
 
 // Per-energy constants of the spectrum/flux integrand, computed on the host in the
-// reference's operation order: omega = (E*Qe)/Hbar, cw = C/omega, kw = (C0*omega)*dt.
+// reference's operation order: omega = (E*Qe)/Hbar, cw = C/omega, kw = (C0*omega)*dt;
+// fr = omega/(2 pi) [rev/s] and fw = fr/C [rev/m] for the paraxial mixed form.
 struct EnergySet {
   int ne;
   double omega[kMaxFluxEnergies];
   double cw[kMaxFluxEnergies];
   double kw[kMaxFluxEnergies];
+  double fr[kMaxFluxEnergies];
+  double fw[kMaxFluxEnergies];
 };
 
 // ---- plain FP64 form ------------------------------------------------------

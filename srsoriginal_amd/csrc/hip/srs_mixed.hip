@@ -37,36 +37,58 @@ __global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f
   }
 }
 
+// A block owns 256 observation points and one trajectory chunk.  It takes the
+// paraxial terms when its points lie on one plane z = Z and every (point,
+// trajectory point) pair of the chunk passes their bounds (a block-uniform
+// decision), and the plain mixed terms otherwise.
 template <int NE, int TILE_>
 __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const float* __restrict__ tf,
                                                              const double* __restrict__ obs, int ns, EnergySet es,
                                                              int chunk, double* __restrict__ part) {
-  __shared__ FluxTile<TILE_> T;
+  __shared__ MixedTile<TILE_, NE> T;
+  __shared__ double red[4 * (kBlock / 64)];
   int const tid = threadIdx.x;
-  int const j = blockIdx.x * kBlock + tid;
-  int const jj = j < ns ? j : ns - 1;
+  int const j0 = blockIdx.x * kBlock;
+  int const j = j0 + tid;
+  int const jj = j < ns ? j : j0;  // idle lanes repeat the block's first point
   double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
   float cwf[NE];
 #pragma unroll
   for (int e = 0; e < NE; ++e) cwf[e] = (float)es.cw[e];
   int64_t const i0 = (int64_t)blockIdx.y * chunk;
   int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  ParaBlock const pb = para_block(ox, oy, oz, obs[2 * ns + j0], red);
+  bool const para = para_check(pb, t.x, t.y, t.z, i0, i1, es);
   AccMixed<NE> a;
   a.zero();
-  for (int64_t base = i0; base < i1; base += TILE_) {
-    int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
-    __syncthreads();
-    for (int k = tid; k < cnt; k += kBlock) {
-      int64_t const i = base + k;
-      T.x[k] = t.x[i];
-      T.y[k] = t.y[i];
-      T.z[k] = t.z[i];
-      T.fbx[k] = tf[i];
-      T.fby[k] = tf[t.n + i];
-      T.fomb[k] = tf[2 * t.n + i];
+  if (para) {
+    for (int64_t base = i0; base < i1; base += TILE_) {
+      int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
+      __syncthreads();
+      for (int k = tid; k < cnt; k += kBlock) {
+        int64_t const i = base + k;
+        para_stage<TILE_, NE>(T.para, k, t.x[i], t.y[i], t.z[i], tf[i], tf[t.n + i], tf[2 * t.n + i], i, t.dt, pb,
+                              es);
+      }
+      __syncthreads();
+      flux_terms_para<NE, TILE_>(T.para, cnt, ox, oy, es, cwf, a);
     }
-    __syncthreads();
-    flux_terms_mixed<NE, TILE_>(T, cnt, base, t.dt, ox, oy, oz, es, cwf, a);
+  } else {
+    for (int64_t base = i0; base < i1; base += TILE_) {
+      int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
+      __syncthreads();
+      for (int k = tid; k < cnt; k += kBlock) {
+        int64_t const i = base + k;
+        T.raw.x[k] = t.x[i];
+        T.raw.y[k] = t.y[i];
+        T.raw.z[k] = t.z[i];
+        T.raw.fbx[k] = tf[i];
+        T.raw.fby[k] = tf[t.n + i];
+        T.raw.fomb[k] = tf[2 * t.n + i];
+      }
+      __syncthreads();
+      flux_terms_mixed<NE, TILE_>(T.raw, cnt, base, t.dt, ox, oy, oz, es, cwf, a);
+    }
   }
   if (j < ns) {
 #pragma unroll

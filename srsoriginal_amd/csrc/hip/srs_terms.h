@@ -137,6 +137,210 @@ __device__ __forceinline__ void flux_terms_mixed(FluxTile<TILE> const& T, int cn
   }
 }
 
+// ---------------------------------------------------------------------------
+// Paraxial flat-screen form of the mixed integrand.
+//
+// For a block whose observation points all lie on one plane z = Z, with
+// rx = ox - x_k, ry = oy - y_k, s = rx^2 + ry^2, rz = Z - z_k and eps = s / rz^2:
+//   D = rz + s/(2 rz) - s^2/(8 rz^3) + rz*O(eps^3)
+//   omega*tau = omega*Z/c + omega*(t_k - z_k/c) + omega*(D - rz)/c.
+// omega*Z/c is the same for every term of one observation point and cancels in
+// |E|^2.  In revolutions (f = omega/2pi, W = f/c):
+//   phase = Phi_k + W*u,  Phi_k = frac(f*(t_k - z_k/c)),  u = s/(2 rz) - s^2/(8 rz^3)
+//   (W*u up to ~1e3 rev at 431 eV; FP64).
+// Phi_k and 1/(2 rz) are formed once per staged trajectory point (compensated FP64);
+// per term only rx, ry, s, u and W*u + Phi_k are FP64 -- no sqrt, no division.
+// With 1/D = m/rz, m = 1 - eps/2 + O(eps^2):
+//   (beta_x - n_x)/D = m (beta_x/rz - rx/rz^2) (1 + O(eps))
+//   (beta_z - n_z)/D = m (s/(2 rz^2) - (1 - beta_z)) / rz (1 + O(eps))
+//   n/D^2            = (rx/rz^3, ry/rz^3, 1/rz^2) (1 + O(eps)).
+// m varies by < eps*|d rz|/rz over a 32-point sub-tile; it scales the sub-tile sums.
+// A block takes this form only when every (point, trajectory point) pair it
+// integrates has eps <= kParaEps (relative amplitude error O(eps)) and
+// W*rz*eps^3/16 <= kParaPhase (dropped phase term, revolutions); otherwise it
+// integrates with flux_terms_mixed.
+// ---------------------------------------------------------------------------
+constexpr double kParaEps = 1e-6;
+constexpr double kParaPhase = 1e-9;
+
+// One staged trajectory point of the paraxial form (one LDS record, so a term
+// reads it from a single base address).
+template <int NE>
+struct alignas(16) ParaPoint {
+  double x, y;        // trajectory position
+  double h;           // 1 / (2 rz)
+  double t3;          // 1 / (8 rz^3)
+  double phi[NE];     // Phi_k per energy, revolutions in [-0.5, 0.5]
+  float bxr, byr;     // beta_x / rz, beta_y / rz
+  float ombr;         // (1 - beta_z) / rz
+  float q;            // 1 / rz^2
+  float ir;           // 1 / rz
+  float gh;           // 1 / (2 rz^3)
+};
+
+template <int TILE, int NE>
+struct ParaTile {
+  ParaPoint<NE> p[TILE];
+};
+
+// LDS of the mixed kernels: the paraxial tile or, for tiles that fail the check,
+// the plain mixed tile.
+template <int TILE, int NE>
+union MixedTile {
+  FluxTile<TILE> raw;
+  ParaTile<TILE, NE> para;
+};
+
+// Plane and bounding box of a block's observation points.
+struct ParaBlock {
+  double z;
+  double xlo, xhi, ylo, yhi;
+  bool flat;  // every point of the block has oz == z
+};
+
+// Called by every lane of the block; lanes without a point pass a point of the
+// block.  red holds 4 * (kBlock / 64) doubles of LDS.
+__device__ __forceinline__ ParaBlock para_block(double ox, double oy, double oz, double z0, double* red) {
+  double v[4] = {ox, -ox, oy, -oy};  // minima of these give xlo, xhi, ylo, yhi
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = __builtin_fmin(v[q], __shfl_xor(v[q], o));
+  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[wave * 4 + q] = v[q];
+  bool const flat = __syncthreads_and(oz == z0) != 0;  // also orders the LDS writes above
+  ParaBlock b;
+  b.z = z0;
+  b.flat = flat;
+  double m[4] = {red[0], red[1], red[2], red[3]};
+  for (int w = 1; w < kBlock / 64; ++w)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = __builtin_fmin(m[q], red[w * 4 + q]);
+  b.xlo = m[0];
+  b.xhi = -m[1];
+  b.ylo = m[2];
+  b.yhi = -m[3];
+  return b;
+}
+
+// True when trajectory point (xk, yk, zk) satisfies the expansion bounds for every
+// observation point of the block.
+__device__ __forceinline__ bool para_point_ok(ParaBlock const& b, double xk, double yk, double zk, double fwmax) {
+  double const rz = b.z - zk;
+  double const dx = __builtin_fmax(__builtin_fabs(b.xlo - xk), __builtin_fabs(b.xhi - xk));
+  double const dy = __builtin_fmax(__builtin_fabs(b.ylo - yk), __builtin_fabs(b.yhi - yk));
+  double const eps = __builtin_fma(dx, dx, dy * dy) / (rz * rz);
+  return rz > 0.0 && eps <= kParaEps && fwmax * rz * (eps * eps * eps) <= 16.0 * kParaPhase;
+}
+
+// Block-uniform decision for trajectory points [i0, i1) of arrays x, y, z: every
+// lane checks a strided share, the block takes the paraxial form only if all pass.
+// Called by every lane of the block.
+__device__ __forceinline__ bool para_check(ParaBlock const& b, const double* __restrict__ x,
+                                           const double* __restrict__ y, const double* __restrict__ z, int64_t i0,
+                                           int64_t i1, EnergySet const& es) {
+  double fwmax = 0.0;
+  for (int e = 0; e < es.ne; ++e) fwmax = __builtin_fmax(fwmax, es.fw[e]);
+  bool ok = b.flat;
+  if (ok)
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) ok = ok && para_point_ok(b, x[i], y[i], z[i], fwmax);
+  return __syncthreads_and(ok) != 0;
+}
+
+// Stage trajectory point k (time index ti) of the paraxial tile.
+template <int TILE, int NE>
+__device__ __forceinline__ void para_stage(ParaTile<TILE, NE>& P, int k, double xk, double yk, double zk, float fbx,
+                                           float fby, float fomb, int64_t ti, double dt, ParaBlock const& b,
+                                           EnergySet const& es) {
+  double const rz = b.z - zk;
+  double const ir = 1.0 / rz;
+  float const irf = (float)ir;
+  ParaPoint<NE>& r = P.p[k];
+  r.x = xk;
+  r.y = yk;
+  r.h = 0.5 * ir;
+  r.t3 = 0.125 * ir * ir * ir;
+  r.bxr = fbx * irf;
+  r.byr = fby * irf;
+  r.ombr = fomb * irf;
+  r.q = irf * irf;
+  r.ir = irf;
+  r.gh = 0.5f * irf * irf * irf;
+  // v = t_k - z_k/c as an unevaluated sum vh + vl, with t_k = RN(dt*ti) as the
+  // reference forms it (OSCARSSR.cc:2072)
+  double const t = dt * (double)ti;
+  double const zc = zk * kInvC;
+  double const ez = __builtin_fma(-zc, kC, zk) * kInvC;  // z_k/c - zc
+  double const vh = t - zc;
+  double const tv = vh - t;
+  double const vl = ((t - (vh - tv)) + (-zc - tv)) - ez;  // two-sum error, minus the quotient's error
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    double const p = es.fr[e] * vh;
+    double const pl = __builtin_fma(es.fr[e], vh, -p) + es.fr[e] * vl;
+    r.phi[e] = (p - __builtin_rint(p)) + pl;
+  }
+}
+
+// Near-field sums (the -i (c/omega) n/D^2 part) are kept apart within a sub-tile and
+// folded in with c/omega at its end, so a term needs no per-energy scaling.
+template <int NE, int TILE>
+__device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int cnt, double ox, double oy,
+                                                EnergySet const& es, const float* cwf, AccMixed<NE>& a) {
+  for (int k0 = 0; k0 < cnt; k0 += kSub) {
+    int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
+    float sub[NE][6], gsub[NE][6];
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) sub[e][q] = gsub[e][q] = 0.f;
+    for (int k = k0; k < k1; ++k) {
+      ParaPoint<NE> const& r = P.p[k];
+      double const rx = ox - r.x, ry = oy - r.y;
+      double const s = __builtin_fma(rx, rx, ry * ry);
+      double const u = __builtin_fma(-(s * s), r.t3, s * r.h);  // D - rz
+      float const sf = (float)s, rxf = (float)rx, ryf = (float)ry;
+      float const tx = rxf * r.q, ty = ryf * r.q;
+      float const ax = r.bxr - tx, ay = r.byr - ty;
+      float const az = __builtin_fmaf(sf, r.gh, -r.ombr);
+      float const gx = tx * r.ir, gy = ty * r.ir, gz = r.q;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        double const p = __builtin_fma(es.fw[e], u, r.phi[e]);
+        float const ph = (float)(p - __builtin_rint(p));
+        float const sn = __builtin_amdgcn_sinf(ph), cn = __builtin_amdgcn_cosf(ph);
+        sub[e][0] = __builtin_fmaf(ax, cn, sub[e][0]);
+        sub[e][1] = __builtin_fmaf(ax, sn, sub[e][1]);
+        sub[e][2] = __builtin_fmaf(ay, cn, sub[e][2]);
+        sub[e][3] = __builtin_fmaf(ay, sn, sub[e][3]);
+        sub[e][4] = __builtin_fmaf(az, cn, sub[e][4]);
+        sub[e][5] = __builtin_fmaf(az, sn, sub[e][5]);
+        gsub[e][0] = __builtin_fmaf(gx, sn, gsub[e][0]);
+        gsub[e][1] = __builtin_fmaf(gx, cn, gsub[e][1]);
+        gsub[e][2] = __builtin_fmaf(gy, sn, gsub[e][2]);
+        gsub[e][3] = __builtin_fmaf(gy, cn, gsub[e][3]);
+        gsub[e][4] = __builtin_fmaf(gz, sn, gsub[e][4]);
+        gsub[e][5] = __builtin_fmaf(gz, cn, gsub[e][5]);
+      }
+    }
+    // m = 1 - eps/2 at the sub-tile's last point
+    ParaPoint<NE> const& rl = P.p[k1 - 1];
+    double const rxl = ox - rl.x, ryl = oy - rl.y;
+    float const m = __builtin_fmaf(-0.5f * (float)__builtin_fma(rxl, rxl, ryl * ryl), rl.q, 1.f);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      // re += ax c + (c/omega) gx s,  im += ax s - (c/omega) gx c
+#pragma unroll
+      for (int q = 0; q < 6; q += 2) {
+        a.q[e][q].add(__builtin_fmaf(cwf[e], gsub[e][q], sub[e][q]) * m);
+        a.q[e][q + 1].add(__builtin_fmaf(-cwf[e], gsub[e][q + 1], sub[e][q + 1]) * m);
+      }
+    }
+  }
+}
+
 // Flux of one energy from the complex field sums (OSCARSSR.cc:2080-2092):
 // SumE *= (0 + i kw) -> (-im kw, re kw); F = C2 * sum_k |.|^2 in the order x, y, z.
 __device__ __forceinline__ double flux_from_sums(double rex, double imx, double rey, double imy, double rez, double imz,

@@ -178,3 +178,27 @@ def test_abi_rejects_bad_arguments(abi, oracle):
         abi.flux(t, dt, [[0, 0, 30]], [100.0], p.q, p.current, precision=7)
     with pytest.raises(abi.SrsError, match="no points in trajectory"):
         abi.flux(t[:, :0], dt, [[0, 0, 30]], [100.0], p.q, p.current)
+
+
+def test_mixed_flux_paraxial_and_general_paths(abi, oracle):
+    """The mixed flux takes the paraxial terms for blocks of points on one plane whose
+    expansion bounds hold, and the plain mixed terms otherwise; both stay within 1e-5
+    of the FP64 form.  Cases: a tilted screen (no common plane), a close screen
+    (eps ~ 1e-4 fails the bound), a 300-point list whose first block mixes two planes
+    and whose second block is flat, and a flat screen just inside the eps bound."""
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_20001")
+    es = [143.52176088044024, 431.2]
+    tilted, _, _ = oracle.rect_surface("XY", 17, 15, 0.012, 0.01, (0.3, 0.1, 0), (0, 0, 30), 0)
+    close, _, _ = oracle.rect_surface("XY", 17, 15, 0.02, 0.02, (0, 0, 0), (0, 0, 3.0), 0)
+    rng = np.random.default_rng(11)
+    two = np.column_stack([rng.uniform(-0.005, 0.005, 300), rng.uniform(-0.005, 0.005, 300),
+                           np.where(np.arange(300) % 2 == 0, 30.0, 30.5)])
+    two[256:, 2] = 30.0
+    edge, _, _ = oracle.rect_surface("XY", 23, 23, 0.023, 0.023, (0, 0, 0), (0, 0, 20.0), 0)
+    for name, xyz in (("tilted", tilted), ("close", close), ("two planes", two), ("edge of bound", edge)):
+        f64 = abi.flux(t, dt, xyz, es, p.q, p.current, abi.FP64)
+        mix = abi.flux(t, dt, xyz, es, p.q, p.current, abi.MIXED)
+        for k, e in enumerate(es):
+            e1, e2 = normwise(mix[k], f64[k]), rms_rel(mix[k], f64[k])
+            say("  %-14s %7.2f eV: mixed vs fp64 normwise %.3e rms %.3e" % (name, e, e1, e2))
+            assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
