@@ -153,6 +153,24 @@ int srs_hip_timing_read(double* total_ms, int64_t* launches);
 /* Waits for all work queued on `stream` (NULL = the library's stream). */
 int srs_hip_synchronize(void* stream);
 
+/* ---- gradient of a flux loss with respect to per-pole field errors ----------------
+ * No counterpart in the reference (it has no derivatives); the forward model is the
+ * reference's pole-scaled ideal undulator, RK4 trajectory (OSCARSSR.cc:562-874) and
+ * flux integral (OSCARSSR.cc:2054-2092) in FP64.  Realisation r scales the
+ * 2*nperiods interior half-period poles of und[poled] by (1 + deltas[r*2N + p]);
+ * every realisation flies `particle` through the field of all undulators.
+ *   L_r = sum_j (F_rj - target_j)^2,  F_rj = flux of realisation r at obs point j.
+ * forward: loss_out[r] (and flux_out[r*ns + j] when not NULL), with RK4 checkpoints
+ * every checkpoint_every steps kept on the device; *tape_out identifies that state.
+ * backward: grad_out[r*2N + p] = dloss[r] * dL_r/d delta_p (dloss NULL = ones), by a
+ * hand-written adjoint of the flux integral and of the RK4 (each checkpoint segment
+ * recomputed).  A later forward, batch or trajectory call replaces the tape. */
+int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+                         const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
+                         double energy_ev, const double* target, int checkpoint_every, double* flux_out,
+                         double* loss_out, int64_t* tape_out);
+int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out);
+
 #ifdef __cplusplus
 }
 #endif

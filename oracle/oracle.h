@@ -87,6 +87,15 @@ void oracle_power_density(const double* traj, int64_t n, double dt, const double
                           int64_t ns, int directional, double q, double current, int nthreads, double* out);
 double oracle_total_power(const double* traj, int64_t n, double dt, double q, double current, double gamma);
 
+/* Loss L = sum_j (F_j - target_j)^2 of the flux at one energy for the field f, whose
+ * pole-scaled undulator (pole_und, pole_scales = deltas) is differentiated, and
+ * grad_out[p] = dL/d delta_p for its 2*nperiods poles, by forward-mode dual numbers
+ * through the trajectory and the flux integral (one thread per pole, nthreads
+ * concurrent).  Returns 0, or -1 without a pole-scaled undulator. */
+int oracle_pole_loss_gradient(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                              int64_t npoints, const double* obs, int64_t ns, double energy_ev, const double* target,
+                              double* loss_out, double* grad_out, int nthreads);
+
 /* Kahan add of TSpectrumContainer::AddToFlux / T3DScalarContainer::AddToPoint. */
 void oracle_kahan_add(double* sum, double* comp, const double* v, int64_t n);
 

@@ -160,3 +160,18 @@ def total_power(traj, dt, q, current, gamma):
 def kahan_add(s, c, v):
     v = np.ascontiguousarray(v, dtype=np.float64)
     lib().oracle_kahan_add(_p(s), _p(c), _p(v), I64(len(v)))
+
+
+def pole_loss_gradient(fields, particle, ct_start, ct_stop, npoints, obs, energy_ev, target, nthreads=1):
+    """(L, dL/d delta) of L = sum_j (F_j - target_j)^2 for the pole-scaled undulator of
+    `fields` (forward-mode dual numbers, FP64)."""
+    obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    grad = np.zeros(len(fields._scales))
+    loss = D()
+    rc = lib().oracle_pole_loss_gradient(C.byref(fields.s), C.byref(particle), D(ct_start), D(ct_stop), I64(npoints),
+                                         _p(obs), I64(len(obs)), D(energy_ev), _p(target), C.byref(loss), _p(grad),
+                                         C.c_int(nthreads))
+    if rc != 0:
+        raise ValueError("no pole-scaled undulator")
+    return loss.value, grad

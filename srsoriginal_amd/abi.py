@@ -183,7 +183,11 @@ def _bind_batch():
     L.srs_hip_batch_flux.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P, _I64,
                                      _P, _I64, _D, C.c_int, C.c_int, _P]
     L.srs_hip_batch_config.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
-    for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config"):
+    L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
+                                       _D, _P, _I64, _D, _P, C.c_int, _P, _P, C.POINTER(_I64)]
+    L.srs_hip_pole_backward.argtypes = [_I64, _P, _P]
+    for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config",
+              "srs_hip_pole_forward", "srs_hip_pole_backward"):
         getattr(L, n).restype = C.c_int
     L._batch_bound = True
     return L
@@ -250,3 +254,39 @@ def batch_config():
     a, b = C.c_int(), C.c_int()
     _check(L.srs_hip_batch_config(C.byref(a), C.byref(b)))
     return a.value, b.value
+
+
+def pole_forward(und_specs, poled, particle, deltas, charge, current, obs_xyz, energy_ev, target, checkpoint_every=1000,
+                 want_flux=True):
+    """Forward of the pole-error flux loss for nreal realisations (deltas [nreal, 2N]).
+    Returns (loss [nreal], flux [nreal, ns] or None, tape id)."""
+    L = _bind_batch()
+    keep = []
+    U = undulators(und_specs, keep)
+    d = _f64(np.atleast_2d(deltas))
+    o = _f64(obs_xyz).reshape(-1, 3)
+    t = _f64(target)
+    if len(t) != len(o):
+        raise SrsError("target needs one value per observation point")
+    nreal = d.shape[0]
+    loss = np.zeros(nreal)
+    flux = np.zeros((nreal, len(o))) if want_flux else None
+    tape = _I64()
+    _check(L.srs_hip_pole_forward(U, len(U), int(poled), C.byref(particle), nreal, d.ctypes.data_as(_P), charge, current,
+                                  o.ctypes.data_as(_P), len(o), float(energy_ev), t.ctypes.data_as(_P),
+                                  int(checkpoint_every), flux.ctypes.data_as(_P) if want_flux else None,
+                                  loss.ctypes.data_as(_P), C.byref(tape)))
+    return loss, flux, tape.value
+
+
+def pole_backward(tape, nreal, npoles, dloss=None):
+    """dL_r/d delta_p * dloss[r] for the realisations of forward call `tape`: [nreal, npoles]."""
+    L = _bind_batch()
+    g = np.zeros((nreal, npoles))
+    dl = None
+    if dloss is not None:
+        dl = _f64(dloss)
+        if len(dl) != nreal:
+            raise SrsError("dloss needs one value per realisation")
+    _check(L.srs_hip_pole_backward(int(tape), dl.ctypes.data_as(_P) if dl is not None else None, g.ctypes.data_as(_P)))
+    return g

@@ -84,6 +84,19 @@ struct Context {
   std::map<std::string, srs::BatchConfig> b_tuned;
   srs::BatchConfig b_last{1, 128};
   int cus = 0;
+  // pole-error gradient: the forward call leaves its tape here for the backward call
+  Buf g_ckpt, g_field, g_target, g_loss, g_dloss, g_gy, g_rec, g_grad;
+  struct Tape {
+    int64_t id = 0;
+    bool valid = false;
+    int np = 0, ns = 0, nund = 0, poled = 0, npoles = 0, every = 0;
+    int rec_steps = 0;  // longest recomputed segment: min(every, longest RK4 leg)
+    int64_t total = 0, nmax = 0;
+    srs::BatchTraj bt{};
+    srs::Checkpoints ck{};
+    double omega = 0, cw = 0, c0w = 0, c2 = 0;
+  } tape;
+  int64_t tape_counter = 0;
 };
 
 Context& ctx() {
@@ -327,6 +340,7 @@ int build_particles(const srs_particle* p, int64_t np, std::vector<srs::BatchPar
     q.qmg = p[i].qmg;
     TRY(particle_grid(p[i], &q.dt, &q.nb, &q.nf));
     q.off = off;
+    q.pole_base = 0;
     off += q.nb + q.nf;
   }
   *total = off;
@@ -405,12 +419,13 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
   int nblocks = c.cus * srs::batch_max_blocks(cfg, r.mixed);
   if ((unsigned)nblocks > nitems) nblocks = (int)nitems;
   auto* P = c.b_part.as<srs::BatchParticle>();
-  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt, s));
+  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt,
+                               srs::Checkpoints{nullptr, 1, 0}, s));
   HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
   {
     KernelTimer timer(c, s);
     HIPCHK(srs::launch_batch_flux(cfg, r.mixed, r.bt, P, c.b_obs.as<double>(), r.ns, r.be, nitems,
-                                  c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), nblocks, s));
+                                  c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), nullptr, nblocks, s));
   }
   HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), np_run, r.ne * r.ns, r.weight, c.b_out.as<double>(), s));
   return SRS_OK;
@@ -579,6 +594,7 @@ int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle*
   if (tot != total) return fail(SRS_ERR_INVALID, "traj_out size does not match the trajectory points");
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
+  c.tape.valid = false;  // shares the batch buffers with the gradient tape
   TRY(init(c));
   hipStream_t s = c.stream;
   TRY(stage_undulators(c, und, nund, s));
@@ -587,7 +603,7 @@ int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle*
   srs::BatchTraj bt;
   TRY(stage_batch_traj(c, tot, true, &bt));
   HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(),
-                               c.b_part.as<srs::BatchParticle>(), (int)np, bt, s));
+                               c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0}, s));
   double* src[9] = {bt.x, bt.y, bt.z, bt.bx, bt.by, bt.bz, bt.ax, bt.ay, bt.az};
   for (int k = 0; k < 9; ++k)
     HIPCHK(hipMemcpyAsync(traj_out + (size_t)k * tot, src[k], tot * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -617,6 +633,7 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
   TRY(init(c));
+  c.tape.valid = false;  // the batch buffers below are shared with the gradient tape
   hipStream_t s = c.stream;
   if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
   TRY(stage_undulators(c, und, nund, s));
@@ -737,6 +754,167 @@ int srs_hip_synchronize(void* stream) {
   Context& c = ctx();
   if (!stream && !c.ready) return SRS_OK;
   hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+  HIPCHK(hipStreamSynchronize(s));
+  return SRS_OK;
+}
+
+
+// ---- gradient of the flux loss with respect to per-pole field errors ---------------
+
+int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+                         const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
+                         double energy_ev, const double* target, int checkpoint_every, double* flux_out,
+                         double* loss_out, int64_t* tape_out) {
+  g_err.clear();
+  TRY(check_common(ns, SRS_PRECISION_FP64));
+  if (!und || nund < 1) return fail(SRS_ERR_INVALID, "no magnetic field (undulator) given");
+  if (poled < 0 || poled >= nund) return fail(SRS_ERR_INVALID, "poled undulator index out of range");
+  if (und[poled].nperiods < 1) return fail(SRS_ERR_INVALID, "the pole-scaled undulator needs nperiods >= 1");
+  for (int u = 0; u < nund; ++u)
+    if (u != poled && und[u].pole_delta)
+      return fail(SRS_ERR_INVALID, "only the differentiated undulator may carry pole errors");
+  if (!particle) return fail(SRS_ERR_INVALID, "no particle");
+  if (nreal < 1 || nreal > 65536) return fail(SRS_ERR_INVALID, "number of realisations must be in 1..65536");
+  if (!deltas || !target || !loss_out || !tape_out || !obs_xyz) return fail(SRS_ERR_INVALID, "NULL argument");
+  if (checkpoint_every < 1) return fail(SRS_ERR_INVALID, "checkpoint interval must be >= 1");
+  if (!(energy_ev > 0)) return fail(SRS_ERR_INVALID, "photon energy must be > 0");
+  // all undulators must share one period direction (the adjoint projects dB/dX on it)
+  double u0[3];
+  for (int u = 0; u < nund; ++u) {
+    double const* pr = und[u].period;
+    double const pm = std::sqrt(pr[0] * pr[0] + pr[1] * pr[1] + pr[2] * pr[2]);
+    if (!(pm > 0)) return fail(SRS_ERR_INVALID, "undulator period must be a nonzero vector");
+    for (int k = 0; k < 3; ++k) {
+      double const uk = pr[k] / pm;
+      if (u == 0) u0[k] = uk;
+      else if (uk != u0[k]) return fail(SRS_ERR_INVALID, "the gradient needs undulators with one period direction");
+    }
+  }
+  int const npoles = 2 * und[poled].nperiods;
+  std::vector<srs::BatchParticle> P;
+  int64_t total = 0;
+  {
+    std::vector<srs_particle> rows((size_t)nreal, *particle);
+    TRY(build_particles(rows.data(), nreal, P, &total));
+    for (int64_t r = 0; r < nreal; ++r) P[r].pole_base = r * npoles;
+  }
+  int64_t const nmax = P[0].nb + P[0].nf;
+  int const every = checkpoint_every;
+  int64_t const stride = (P[0].nf + every - 1) / every + (P[0].nb + every - 1) / every;
+
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  c.tape.valid = false;
+  hipStream_t s = c.stream;
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  {
+    std::vector<srs_undulator> uu(und, und + nund);
+    uu[poled].pole_delta = deltas;  // pole_off 0 into the realisations' delta array
+    TRY(stage_undulators(c, uu.data(), nund, s));
+  }
+  HIPCHK(c.b_poles.ensure((size_t)nreal * npoles * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_poles.p, deltas, (size_t)nreal * npoles * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c.b_part.ensure(nreal * sizeof(srs::BatchParticle)));
+  HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), nreal * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
+  TRY(stage_points(c, c.b_obs, obs_xyz, ns));
+  HIPCHK(c.g_target.ensure(ns * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.g_target.p, target, ns * sizeof(double), hipMemcpyHostToDevice, s));
+  double const w = omega_of(energy_ev), c0 = c0_of(charge);
+  double const eh[5] = {w, kC / w, c0 * w, w / kTwoPi, (w / kTwoPi) / kC};
+  HIPCHK(c.b_energies.ensure(5 * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh, 5 * sizeof(double), hipMemcpyHostToDevice, s));
+  double* ed = c.b_energies.as<double>();
+  srs::BatchEnergies const be{ed, ed + 1, ed + 2, ed + 3, ed + 4, 1, c2_of(charge, current)};
+  srs::BatchTraj bt;
+  TRY(stage_batch_traj(c, total, false, &bt));
+  HIPCHK(c.g_ckpt.ensure((size_t)nreal * stride * 6 * sizeof(double)));
+  srs::Checkpoints const ck{c.g_ckpt.as<double>(), every, stride};
+  HIPCHK(c.b_flux_pp.ensure((size_t)nreal * ns * sizeof(double)));
+  HIPCHK(c.g_field.ensure((size_t)nreal * 6 * ns * sizeof(double)));
+  HIPCHK(c.g_loss.ensure((size_t)nreal * sizeof(double)));
+  HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
+  auto* Pd = c.b_part.as<srs::BatchParticle>();
+  {
+    KernelTimer timer(c, s);
+    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(), Pd, (int)nreal, bt,
+                                 ck, s));
+  }
+  srs::BatchConfig const cfg{1, 256};
+  int64_t const ntiles = (ns + srs::kBlock - 1) / srs::kBlock;
+  unsigned const nitems = (unsigned)(nreal * ntiles);
+  int nblocks = c.cus * srs::batch_max_blocks(cfg, false);
+  if ((unsigned)nblocks > nitems) nblocks = (int)nitems;
+  HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
+  {
+    KernelTimer timer(c, s);
+    HIPCHK(srs::launch_batch_flux(cfg, false, bt, Pd, c.b_obs.as<double>(), (int)ns, be, nitems,
+                                  c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), c.g_field.as<double>(),
+                                  nblocks, s));
+  }
+  HIPCHK(srs::launch_loss(c.b_flux_pp.as<double>(), c.g_target.as<double>(), (int)ns, (int)nreal,
+                          c.g_loss.as<double>(), s));
+  HIPCHK(hipMemcpyAsync(loss_out, c.g_loss.p, nreal * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (flux_out)
+    HIPCHK(hipMemcpyAsync(flux_out, c.b_flux_pp.p, (size_t)nreal * ns * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  Context::Tape& t = c.tape;
+  t.id = ++c.tape_counter;
+  t.np = (int)nreal;
+  t.ns = (int)ns;
+  t.nund = nund;
+  t.poled = poled;
+  t.npoles = npoles;
+  t.every = every;
+  t.rec_steps = (int)std::min<int64_t>(every, std::max(P[0].nf, P[0].nb));
+  t.total = total;
+  t.nmax = nmax;
+  t.bt = bt;
+  t.ck = ck;
+  t.omega = w;
+  t.cw = kC / w;
+  t.c0w = c0 * w;
+  t.c2 = be.c2;
+  t.valid = true;
+  *tape_out = t.id;
+  return SRS_OK;
+}
+
+int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out) {
+  g_err.clear();
+  if (!grad_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  Context::Tape const& t = c.tape;
+  if (!t.valid || t.id != tape)
+    return fail(SRS_ERR_INVALID, "no forward tape for this call (another forward or batch call replaced it)");
+  hipStream_t s = c.stream;
+  HIPCHK(c.g_dloss.ensure(t.np * sizeof(double)));
+  {
+    std::vector<double> dl((size_t)t.np, 1.0);
+    if (dloss) std::copy(dloss, dloss + t.np, dl.begin());
+    HIPCHK(hipMemcpyAsync(c.g_dloss.p, dl.data(), t.np * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  HIPCHK(c.g_gy.ensure((size_t)6 * t.total * sizeof(double)));
+  HIPCHK(c.g_rec.ensure(srs::adjoint_record_doubles(t.rec_steps, t.np) * sizeof(double)));
+  HIPCHK(c.g_grad.ensure((size_t)t.np * t.npoles * sizeof(double)));
+  auto* Pd = c.b_part.as<srs::BatchParticle>();
+  {
+    KernelTimer timer(c, s);
+    HIPCHK(srs::launch_flux_adjoint(t.bt, Pd, t.np, t.nmax, c.b_obs.as<double>(), t.ns, t.omega, t.cw, t.c0w, t.c2,
+                                    c.g_field.as<double>(), c.b_flux_pp.as<double>(), c.g_target.as<double>(),
+                                    c.g_dloss.as<double>(), c.g_gy.as<double>(), t.total, s));
+  }
+  HIPCHK(srs::launch_zero(c.g_grad.as<double>(), (int64_t)t.np * t.npoles, s));
+  {
+    KernelTimer timer(c, s);
+    HIPCHK(srs::launch_rk4_adjoint(c.b_und.as<srs::DevUndulator>(), t.nund, t.poled, c.b_poles.as<double>(), Pd, t.np,
+                                   t.ck, c.g_gy.as<double>(), t.total, c.g_rec.as<double>(), c.g_grad.as<double>(),
+                                   t.npoles, s));
+  }
+  HIPCHK(hipMemcpyAsync(grad_out, c.g_grad.p, (size_t)t.np * t.npoles * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return SRS_OK;
 }

@@ -1,0 +1,364 @@
+// Adjoint of the flux loss with respect to per-pole field errors.
+//
+// Forward (srs_batch.hip kernels): realisation r is a particle whose undulator poles
+// carry their own deltas; rk4_batch_kernel integrates it (checkpointing the state
+// every C steps) and batch_flux_kernel (FP64) returns its flux map F_r and the six
+// field sums S_r of every point.  Loss L_r = sum_j (F_rj - target_j)^2.
+//
+// Backward:
+//   flux_adjoint_kernel  lanes over (r, trajectory point k), loop over observation
+//                        points staged in LDS: dL/dX_k and dL/dbeta_k of the
+//                        radiation integral (analytic derivative of every term);
+//   rk4_adjoint_kernel   one lane per realisation walks the trajectory backwards one
+//                        checkpoint segment at a time: it recomputes the segment's
+//                        steps from the checkpoint (recording each RK4 stage's
+//                        velocity, B, dB/dD and pole sine), then sweeps the steps in
+//                        reverse applying the transposed RK4 step and collecting
+//                        dL/d delta_p.
+//
+// Radiation-integral term (OSCARSSR.cc:2054-2092), R = O - X, D = |R|, n = R/D:
+//   T_c = a_c P,  a_c = (beta_c - n_c)/D - i cw n_c/D^2,  P = exp(i omega (dt k + D/c)).
+// With G_c = dL/dRe S_c + i dL/dIm S_c and H_c = conj(G_c) P:
+//   dL/dbeta_c = Re(H_c)/D
+//   dL/dR_d    = -(Re H_d + n_d S1)/D^2 + cw (Im H_d - 3 n_d S2)/D^3 - (omega/c) n_d S3,
+//   S1 = sum_c Re H_c (beta_c - 2 n_c),  S2 = sum_c Im H_c n_c,  S3 = sum_c Im(H_c a_c),
+// and dL/dX = -dL/dR.  F = C2 kw^2 sum_c |S_c|^2, so G_c = 2 (F - target) 2 C2 kw^2 S_c.
+//
+// RK4 step y' = y + h/6 ((k1 + k4) + 2 (k3 + k2)) with k_s = f(y_s), f = (v, qmg v x B(X)):
+// for a cotangent w of f at a stage, with w_a = (w1, w3, w5) and c = qmg (w_a x v),
+//   d(w.f)/dX = u (c . dB/dD),  d(w.f)/dv = (w0, w2, w4) + qmg (B x w_a),
+//   d(w.f)/d delta_p = sin(theta) (c . field)   (point inside pole p),
+// u being the undulators' common period direction.
+#include "srs_batch.h"
+#include "srs_rk4.h"
+#include "srs_terms.h"
+
+namespace srs {
+namespace {
+
+constexpr int kAdjTile = 256;
+constexpr int kStageRec = 11;          // v[3], B[3], dBdD[3], sin at pole, pole index
+constexpr int kStepRec = 4 * kStageRec;
+
+__global__ void zero_kernel(double* __restrict__ a, int64_t n) {
+  int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// L_r = sum_j (F_rj - target_j)^2, one block per realisation, fixed-order tree.
+__global__ __launch_bounds__(256) void loss_kernel(const double* __restrict__ flux_pp,
+                                                   const double* __restrict__ target, int ns,
+                                                   double* __restrict__ loss) {
+  __shared__ double red[256];
+  int const r = blockIdx.x, tid = threadIdx.x;
+  double s = 0.0;
+  for (int j = tid; j < ns; j += 256) {
+    double const d = flux_pp[(size_t)r * ns + j] - target[j];
+    s += d * d;
+  }
+  red[tid] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) loss[r] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                              const double* __restrict__ obs, int ns, double omega,
+                                                              double cw, double c0w, double c2,
+                                                              const double* __restrict__ field_pp,
+                                                              const double* __restrict__ flux_pp,
+                                                              const double* __restrict__ target,
+                                                              const double* __restrict__ dloss,
+                                                              double* __restrict__ gy, int64_t total) {
+  __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile];
+  __shared__ double sgr[3][kAdjTile], sgi[3][kAdjTile];
+  int const r = blockIdx.y, tid = threadIdx.x;
+  BatchParticle const q = P[r];
+  int64_t const n = q.nb + q.nf;
+  int64_t const k = (int64_t)blockIdx.x * kBlock + tid;
+  if ((int64_t)blockIdx.x * kBlock >= n) return;  // whole block past this realisation's points
+  bool const valid = k < n;
+  int64_t const kk = valid ? k : n - 1;
+  int64_t const i = q.off + kk;
+  double const X = bt.x[i], Y = bt.y[i], Z = bt.z[i];
+  double const b[3] = {bt.bx[i], bt.by[i], bt.bz[i]};
+  double const t0 = q.dt * (double)kk;
+  double const kw = c0w * q.dt;
+  double const gsc = dloss[r] * 4.0 * c2 * kw * kw;
+  double const woc = omega * kInvC;
+  double gR[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+  for (int j0 = 0; j0 < ns; j0 += kAdjTile) {
+    int const cnt = (ns - j0) < kAdjTile ? ns - j0 : kAdjTile;
+    __syncthreads();
+    for (int t = tid; t < cnt; t += kBlock) {
+      int const j = j0 + t;
+      sx[t] = obs[j];
+      sy[t] = obs[ns + j];
+      sz[t] = obs[2 * ns + j];
+      double const w = gsc * (flux_pp[(size_t)r * ns + j] - target[j]);
+      size_t const base = (size_t)r * 6 * ns + j;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sgr[c][t] = w * field_pp[base + (size_t)(2 * c) * ns];
+        sgi[c][t] = w * field_pp[base + (size_t)(2 * c + 1) * ns];
+      }
+    }
+    __syncthreads();
+    for (int t = 0; t < cnt; ++t) {
+      double const rr[3] = {sx[t] - X, sy[t] - Y, sz[t] - Z};
+      double const D = __builtin_sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+      double const tau = t0 + div_rn(D, kC, kInvC);
+      double const inv = 1.0 / D, inv2 = inv * inv;
+      double sn, cs;
+      sincos_fp64(omega * tau, &sn, &cs);
+      double nn[3], hr[3], hi[3], ar[3], ai[3];
+      double S1 = 0, S2 = 0, S3 = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        nn[c] = rr[c] * inv;
+        hr[c] = __builtin_fma(sgr[c][t], cs, sgi[c][t] * sn);
+        hi[c] = __builtin_fma(sgr[c][t], sn, -(sgi[c][t] * cs));
+        ar[c] = (b[c] - nn[c]) * inv;
+        ai[c] = -(cw * nn[c]) * inv2;
+        gb[c] = __builtin_fma(hr[c], inv, gb[c]);
+        S1 = __builtin_fma(hr[c], b[c] - 2.0 * nn[c], S1);
+        S2 = __builtin_fma(hi[c], nn[c], S2);
+        S3 = __builtin_fma(hr[c], ai[c], __builtin_fma(hi[c], ar[c], S3));
+      }
+      double const cw3 = cw * inv2 * inv;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double const t1 = -inv2 * __builtin_fma(nn[d], S1, hr[d]);
+        double const t2 = cw3 * __builtin_fma(-3.0 * nn[d], S2, hi[d]);
+        gR[d] += t1 + t2 - woc * nn[d] * S3;
+      }
+    }
+  }
+  if (valid) {
+    gy[0 * total + i] = -gR[0];
+    gy[1 * total + i] = gb[0] * kInvC;
+    gy[2 * total + i] = -gR[1];
+    gy[3 * total + i] = gb[1] * kInvC;
+    gy[4 * total + i] = -gR[2];
+    gy[5 * total + i] = gb[2] * kInvC;
+  }
+}
+
+// One RK4 step from y recording the four stages (same arithmetic as RK4Ctx::step).
+template <int NU>
+__device__ __forceinline__ void step_record(RK4Ctx<NU> const& R, double* y, double h, double hh, double h6,
+                                            double* __restrict__ rec, int np) {
+  double k1[6], k2[6], k3[6], k4[6], yt[6];
+  FieldJet J;
+  auto put = [&](int s, const double* ys) {
+    double v[kStageRec] = {ys[1], ys[3], ys[5], J.B[0], J.B[1], J.B[2], J.dBdD[0], J.dBdD[1], J.dBdD[2], J.sp,
+                           (double)J.p};
+#pragma unroll
+    for (int f = 0; f < kStageRec; ++f) rec[(size_t)(s * kStageRec + f) * np] = v[f];
+  };
+  R.F.jet(y[0], y[2], y[4], J);
+  R.derivs_from(y, J.B, k1);
+  put(0, y);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) yt[c] = y[c] + hh * k1[c];
+  R.F.jet(yt[0], yt[2], yt[4], J);
+  R.derivs_from(yt, J.B, k2);
+  put(1, yt);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) yt[c] = y[c] + hh * k2[c];
+  R.F.jet(yt[0], yt[2], yt[4], J);
+  R.derivs_from(yt, J.B, k3);
+  put(2, yt);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    yt[c] = y[c] + h * k3[c];
+    k3[c] += k2[c];
+  }
+  R.F.jet(yt[0], yt[2], yt[4], J);
+  R.derivs_from(yt, J.B, k4);
+  put(3, yt);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) y[c] = y[c] + h6 * (k1[c] + k4[c] + 2.0 * k3[c]);
+}
+
+struct PoleAcc {
+  double* out;  // this realisation's 2N gradient entries
+  double acc;
+  int cur;
+  __device__ __forceinline__ void add(int p, double v) {
+    if (p != cur) {
+      if (cur >= 0) out[cur] += acc;
+      cur = p;
+      acc = 0.0;
+    }
+    acc += v;
+  }
+  __device__ __forceinline__ void flush() {
+    if (cur >= 0) out[cur] += acc;
+    cur = -1;
+    acc = 0.0;
+  }
+};
+
+// Transposed RK4 step: lam = dL/dy' in, dL/dy out; dL/d delta collected in pa.
+__device__ __forceinline__ void step_adjoint(const double* __restrict__ st, double* lam, double h, double qmg,
+                                             const double* u, const double* fpole, PoleAcc& pa) {
+  double const hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
+  double kb[4][6], ybar[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    ybar[c] = lam[c];
+    kb[0][c] = h6 * lam[c];
+    kb[3][c] = h6 * lam[c];
+    kb[1][c] = 2.0 * h6 * lam[c];
+    kb[2][c] = 2.0 * h6 * lam[c];
+  }
+  double const back[4] = {0.0, hh, hh, h};  // stage s state = y + back[s] * k_{s-1}
+#pragma unroll
+  for (int s = 3; s >= 0; --s) {
+    const double* S = st + s * kStageRec;
+    double const* w = kb[s];
+    double const wa[3] = {w[1], w[3], w[5]};
+    double const v[3] = {S[0], S[1], S[2]};
+    double const B[3] = {S[3], S[4], S[5]};
+    // c = qmg (w_a x v)
+    double const c[3] = {qmg * (wa[1] * v[2] - wa[2] * v[1]), qmg * (wa[2] * v[0] - wa[0] * v[2]),
+                         qmg * (wa[0] * v[1] - wa[1] * v[0])};
+    double const sx = c[0] * S[6] + c[1] * S[7] + c[2] * S[8];
+    double ys[6];
+    ys[0] = u[0] * sx;
+    ys[2] = u[1] * sx;
+    ys[4] = u[2] * sx;
+    ys[1] = w[0] + qmg * (B[1] * wa[2] - B[2] * wa[1]);
+    ys[3] = w[2] + qmg * (B[2] * wa[0] - B[0] * wa[2]);
+    ys[5] = w[4] + qmg * (B[0] * wa[1] - B[1] * wa[0]);
+    int const p = (int)S[10];
+    if (p >= 0) pa.add(p, S[9] * (c[0] * fpole[0] + c[1] * fpole[1] + c[2] * fpole[2]));
+#pragma unroll
+    for (int q = 0; q < 6; ++q) ybar[q] += ys[q];
+    if (s > 0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) kb[s - 1][q] += back[s] * ys[q];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) lam[c] = ybar[c];
+}
+
+template <int NU>
+__global__ __launch_bounds__(64) void rk4_adjoint_kernel(const DevUndulator* __restrict__ U, int nu, int poled,
+                                                          const double* __restrict__ poles,
+                                                          const BatchParticle* __restrict__ Ps, int np,
+                                                          Checkpoints ck, const double* __restrict__ gy,
+                                                          int64_t total, double* __restrict__ rec,
+                                                          double* __restrict__ grad, int npoles) {
+  int const r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= np) return;
+  BatchParticle const q = Ps[r];
+  RK4Ctx<NU> const R{Fields<NU>(U, nu, poles + q.pole_base), q.qmg};
+  double const u[3] = {U[0].u[0], U[0].u[1], U[0].u[2]};
+  double const fpole[3] = {U[poled].field[0], U[poled].field[1], U[poled].field[2]};
+  PoleAcc pa{grad + (size_t)r * npoles, 0.0, -1};
+  double* const myrec = rec + r;
+  int64_t const C = ck.every;
+  int64_t const nf_slots = (q.nf + C - 1) / C;
+  auto load_state = [&](int64_t slot, double* y) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) y[c] = ck.state[(r * ck.stride + slot) * 6 + c];
+  };
+  auto stage_rec = [&](int64_t il) { return myrec + (size_t)il * kStepRec * np; };
+  auto load_rec = [&](int64_t il, double* st) {
+    const double* src = stage_rec(il);
+#pragma unroll
+    for (int f = 0; f < kStepRec; ++f) st[f] = src[(size_t)f * np];
+  };
+  // forward leg: points nb + i hold y_i, i = 0 .. nf-1
+  {
+    double const h = q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
+    double lam[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t s = nf_slots - 1; s >= 0; --s) {
+      int64_t const i0 = s * C, i1 = (i0 + C < q.nf) ? i0 + C : q.nf;
+      double y[6];
+      load_state(s, y);
+      for (int64_t i = i0; i < i1; ++i) step_record(R, y, h, hh, h6, stage_rec(i - i0), np);
+      double cur[kStepRec], nxt[kStepRec];
+      load_rec(i1 - 1 - i0, cur);
+      for (int64_t i = i1 - 1; i >= i0; --i) {
+        if (i > i0) load_rec(i - 1 - i0, nxt);
+        step_adjoint(cur, lam, h, q.qmg, u, fpole, pa);
+        int64_t const pt = q.off + q.nb + i;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lam[c] += gy[c * total + pt];
+#pragma unroll
+        for (int f = 0; f < kStepRec; ++f) cur[f] = nxt[f];
+      }
+    }
+  }
+  // backward leg: point nb - 1 - i holds y'_{i+1}, i = 0 .. nb-1 (y'_0 is the start state)
+  if (q.nb > 0) {
+    double const h = -q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
+    double lam[6] = {0, 0, 0, 0, 0, 0};
+    int64_t const nb_slots = (q.nb + C - 1) / C;
+    for (int64_t s = nb_slots - 1; s >= 0; --s) {
+      int64_t const i0 = s * C, i1 = (i0 + C < q.nb) ? i0 + C : q.nb;
+      double y[6];
+      load_state(nf_slots + s, y);
+      for (int64_t i = i0; i < i1; ++i) step_record(R, y, h, hh, h6, stage_rec(i - i0), np);
+      double cur[kStepRec], nxt[kStepRec];
+      load_rec(i1 - 1 - i0, cur);
+      for (int64_t i = i1 - 1; i >= i0; --i) {
+        if (i > i0) load_rec(i - 1 - i0, nxt);
+        int64_t const pt = q.off + q.nb - 1 - i;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lam[c] += gy[c * total + pt];
+        step_adjoint(cur, lam, h, q.qmg, u, fpole, pa);
+#pragma unroll
+        for (int f = 0; f < kStepRec; ++f) cur[f] = nxt[f];
+      }
+    }
+  }
+  pa.flush();
+}
+
+}  // namespace
+
+hipError_t launch_zero(double* a, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  zero_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_loss(const double* flux_pp, const double* target, int ns, int np, double* loss, hipStream_t s) {
+  loss_kernel<<<np, 256, 0, s>>>(flux_pp, target, ns, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
+                               int ns, double omega, double cw, double c0w, double c2, const double* field_pp,
+                               const double* flux_pp, const double* target, const double* dloss, double* gy,
+                               int64_t total, hipStream_t s) {
+  dim3 const grid((unsigned)((nmax + kBlock - 1) / kBlock), (unsigned)np);
+  flux_adjoint_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target, dloss,
+                                              gy, total);
+  return hipGetLastError();
+}
+
+size_t adjoint_record_doubles(int every, int np) { return (size_t)every * kStepRec * np; }
+
+hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const double* poles, const BatchParticle* P,
+                              int np, Checkpoints const& ck, const double* gy, int64_t total, double* rec,
+                              double* grad, int npoles, hipStream_t s) {
+  dim3 const grid((np + 63) / 64);
+  if (nu == 1)
+    rk4_adjoint_kernel<1><<<grid, 64, 0, s>>>(U, nu, poled, poles, P, np, ck, gy, total, rec, grad, npoles);
+  else if (nu == 2)
+    rk4_adjoint_kernel<2><<<grid, 64, 0, s>>>(U, nu, poled, poles, P, np, ck, gy, total, rec, grad, npoles);
+  else
+    rk4_adjoint_kernel<0><<<grid, 64, 0, s>>>(U, nu, poled, poles, P, np, ck, gy, total, rec, grad, npoles);
+  return hipGetLastError();
+}
+
+}  // namespace srs

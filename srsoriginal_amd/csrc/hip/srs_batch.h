@@ -25,11 +25,23 @@ struct DevUndulator {
 };
 
 // Per-particle state and trajectory grid (dt, nb, nf computed on the host with the
-// reference's formulas, OSCARSSR.cc:592-598); off = first point in the packed arrays.
+// reference's formulas, OSCARSSR.cc:592-598); off = first point in the packed arrays;
+// pole_base = offset of this particle's pole deltas in the pole array (0 unless each
+// particle carries its own field errors).
 struct BatchParticle {
   double x0[3], beta0[3];
   double qmg, dt;
   int64_t nb, nf, off;
+  int64_t pole_base;
+};
+
+// RK4 checkpoints: state [x, vx, y, vy, z, vz] every `every` steps of each leg;
+// particle p's forward-leg slot s is state[(p*stride + s)*6], its backward-leg slot s
+// is state[(p*stride + nf_slots + s)*6] with nf_slots = ceil(nf / every).
+struct Checkpoints {
+  double* state;  // NULL: none
+  int every;
+  int64_t stride;
 };
 
 // Packed trajectories of all particles (FP64 for the plain form and A/c, FP32
@@ -59,11 +71,29 @@ struct BatchConfig {
 bool batch_config_valid(BatchConfig const& c);
 int batch_max_blocks(BatchConfig const& c, bool mixed);
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
-                            BatchTraj const& bt, hipStream_t s);
+                            BatchTraj const& bt, Checkpoints const& ck, hipStream_t s);
 hipError_t launch_zero_counter(unsigned* c, hipStream_t s);
+// field_pp (FP64 form only, may be NULL): the six field sums of each (particle,
+// energy, point), field_pp[((p*ne + e)*6 + q)*ns + j], q = re/im of x, y, z.
 hipError_t launch_batch_flux(BatchConfig const& cfg, bool mixed, BatchTraj const& bt, const BatchParticle* P,
                              const double* obs, int ns, BatchEnergies const& be, unsigned nitems, unsigned* counter,
-                             double* flux_pp, int nblocks, hipStream_t s);
+                             double* flux_pp, double* field_pp, int nblocks, hipStream_t s);
 hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, double weight, double* out, hipStream_t s);
+
+// ---- adjoint of the flux loss (srs_adjoint.hip) ----------------------------
+hipError_t launch_zero(double* a, int64_t n, hipStream_t s);
+// loss[r] = sum_j (flux_pp[r*ns + j] - target[j])^2
+hipError_t launch_loss(const double* flux_pp, const double* target, int ns, int np, double* loss, hipStream_t s);
+// gy[c*total + point] = dL/dy_c of every trajectory point (y = x, vx, y, vy, z, vz), one energy
+hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
+                               int ns, double omega, double cw, double c0w, double c2, const double* field_pp,
+                               const double* flux_pp, const double* target, const double* dloss, double* gy,
+                               int64_t total, hipStream_t s);
+// doubles of scratch the RK4 adjoint needs for one checkpoint segment of np particles
+size_t adjoint_record_doubles(int every, int np);
+// grad[r*npoles + p] += dL/d delta_p (grad zeroed by the caller)
+hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const double* poles, const BatchParticle* P,
+                              int np, Checkpoints const& ck, const double* gy, int64_t total, double* rec,
+                              double* grad, int npoles, hipStream_t s);
 
 }  // namespace srs

@@ -12,129 +12,11 @@
 //   particle_reduce       Kahan-sums the per-particle maps in particle order with
 //                         the weight 1/NP (T3DScalarContainer::AddToPoint order).
 #include "srs_batch.h"
+#include "srs_rk4.h"
 #include "srs_terms.h"
 
 namespace srs {
 namespace {
-
-// ---- device field and RK4 (reference: TField3D_IdealUndulator.cc:106-139,
-// OSCARSSR.cc:741-874; same operation order, device sine) ----------------------
-//
-// One lane integrates one particle; the RK4 chain is strictly sequential (20k steps
-// x 4 field evaluations), so its cost is the latency of that chain.  It is kept
-// short: the undulator records are loaded into registers once (NU <= 2 at compile
-// time), every division of the reference (2 pi D / lam, h / 6, v / c) is a
-// correctly rounded Markstein quotient from a reciprocal formed once (bitwise the
-// division), and the regions of the field are selected without branches.
-
-__device__ __forceinline__ double sin_fp64(double x) {
-  double s, c;
-  sincos_fp64(x, &s, &c);
-  return s;
-}
-
-constexpr double kInvSix = 0x1.5555555555555p-3;  // RN(1/6)
-
-// Field of one undulator at (X, Y, Z), added to B (TFieldContainer::GetF sums from zero).
-__device__ __forceinline__ void field_add(DevUndulator const& f, const double* __restrict__ poles, double X, double Y,
-                                          double Z, double* B) {
-  double const D = (X - f.center[0]) * f.u[0] + (Y - f.center[1]) * f.u[1] + (Z - f.center[2]) * f.u[2];
-  double const T = 1 + D * f.taper;
-  bool const inside = !(D > f.hi0 || D < f.lo0);
-  double const s = sin_fp64(div_rn(2.0 * kPi * (D - f.ps), f.lam, f.rlam));
-  bool const term = D < f.lo1 || D > f.hi1;
-  double const fct = (D < f.lo2 || D > f.hi2) ? 0.25 : 0.75;
-  double g = T;
-  if (f.pole_off >= 0) {  // uniform across the wave
-    int p = (int)__builtin_floor(div_rn(D - f.lo1, f.lam / 2., 2.0 * f.rlam));
-    p = p < 0 ? 0 : (p > 2 * f.nperiods - 1 ? 2 * f.nperiods - 1 : p);
-    g = 1.0 + poles[f.pole_off + p];
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double const b = term ? ((f.field[c] * fct) * s) * T : (f.field[c] * s) * g;
-    B[c] += inside ? b : 0.0;
-  }
-}
-
-// Undulator records: NU > 0 held in registers, NU == 0 read from memory (any count).
-template <int NU>
-struct Fields {
-  DevUndulator u[NU];
-  const double* poles;
-  __device__ __forceinline__ Fields(const DevUndulator* U, int, const double* p) : poles(p) {
-#pragma unroll
-    for (int k = 0; k < NU; ++k) u[k] = U[k];
-  }
-  __device__ __forceinline__ void eval(double X, double Y, double Z, double* B) const {
-    B[0] = B[1] = B[2] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NU; ++k) field_add(u[k], poles, X, Y, Z, B);
-  }
-};
-template <>
-struct Fields<0> {
-  const DevUndulator* U;
-  int nu;
-  const double* poles;
-  __device__ __forceinline__ Fields(const DevUndulator* U_, int n, const double* p) : U(U_), nu(n), poles(p) {}
-  __device__ __forceinline__ void eval(double X, double Y, double Z, double* B) const {
-    B[0] = B[1] = B[2] = 0.0;
-    for (int k = 0; k < nu; ++k) field_add(U[k], poles, X, Y, Z, B);
-  }
-};
-
-template <int NU>
-struct RK4Ctx {
-  Fields<NU> F;
-  double qmg;
-  __device__ __forceinline__ void derivs(const double* x, double* d) const {
-    double B[3];
-    F.eval(x[0], x[2], x[4], B);
-    d[0] = x[1];
-    d[1] = qmg * (-x[5] * B[1] + x[3] * B[2]);
-    d[2] = x[3];
-    d[3] = qmg * (x[5] * B[0] - x[1] * B[2]);
-    d[4] = x[5];
-    d[5] = qmg * (x[1] * B[1] - x[3] * B[0]);
-  }
-  // OSCARSSR::RK4 (OSCARSSR.cc:826-874); hh = h/2 and h6 = h/6 are formed by the caller
-  __device__ __forceinline__ void step(double* y, const double* dydx, double h, double hh, double h6) const {
-    double dym[6], dyt[6], yt[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dydx[i];
-    derivs(yt, dyt);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) yt[i] = y[i] + hh * dyt[i];
-    derivs(yt, dym);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      yt[i] = y[i] + h * dym[i];
-      dym[i] += dyt[i];
-    }
-    derivs(yt, dyt);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
-  }
-};
-
-__device__ __forceinline__ void store_point(BatchTraj const& bt, int64_t k, const double* x, const double* d) {
-  bt.x[k] = x[0];
-  bt.y[k] = x[2];
-  bt.z[k] = x[4];
-  double const bx = div_rn(x[1], kC, kInvC), by = div_rn(x[3], kC, kInvC), bz = div_rn(x[5], kC, kInvC);
-  bt.bx[k] = bx;
-  bt.by[k] = by;
-  bt.bz[k] = bz;
-  bt.fbx[k] = (float)bx;
-  bt.fby[k] = (float)by;
-  bt.fomb[k] = (float)(1.0 - bz);
-  if (bt.ax) {
-    bt.ax[k] = div_rn(d[1], kC, kInvC);
-    bt.ay[k] = div_rn(d[3], kC, kInvC);
-    bt.az[k] = div_rn(d[5], kC, kInvC);
-  }
-}
 
 // Same point layout as the host trajectory (OSCARSSR.cc:562-673): forward pass from
 // T0 stores the state before each step with the previous derivative (A/c lags one
@@ -142,11 +24,12 @@ __device__ __forceinline__ void store_point(BatchTraj const& bt, int64_t k, cons
 template <int NU>
 __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu,
                                                         const double* __restrict__ poles,
-                                                        const BatchParticle* __restrict__ Ps, int np, BatchTraj bt) {
+                                                        const BatchParticle* __restrict__ Ps, int np, BatchTraj bt,
+                                                        Checkpoints ck) {
   int const p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   BatchParticle const q = Ps[p];
-  RK4Ctx<NU> const R{Fields<NU>(U, nu, poles), q.qmg};
+  RK4Ctx<NU> const R{Fields<NU>(U, nu, poles + q.pole_base), q.qmg};
   double x[6], d[6] = {0, 0, 0, 0, 0, 0};
   auto init = [&]() {
     x[0] = q.x0[0];
@@ -156,10 +39,16 @@ __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __res
     x[4] = q.x0[2];
     x[5] = q.beta0[2] * kC;
   };
+  auto save = [&](int64_t slot) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) ck.state[(p * ck.stride + slot) * 6 + c] = x[c];
+  };
+  int64_t const nf_slots = ck.state ? (q.nf + ck.every - 1) / ck.every : 0;
   init();
   {
     double const h = q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
     for (int64_t i = 0; i < q.nf; ++i) {
+      if (ck.state && i % ck.every == 0) save(i / ck.every);
       store_point(bt, q.off + q.nb + i, x, d);
       R.derivs(x, d);
       R.step(x, d, h, hh, h6);
@@ -169,6 +58,7 @@ __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __res
   {
     double const h = -q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
     for (int64_t i = 0; i < q.nb; ++i) {
+      if (ck.state && i % ck.every == 0) save(nf_slots + i / ck.every);
       R.derivs(x, d);
       R.step(x, d, h, hh, h6);
       store_point(bt, q.off + q.nb - 1 - i, x, d);
@@ -201,7 +91,7 @@ template <int NE, int TILE_, bool MIXED>
 __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                             const double* __restrict__ obs, int ns, BatchEnergies be,
                                                             unsigned nitems, unsigned* __restrict__ counter,
-                                                            double* __restrict__ flux_pp) {
+                                                            double* __restrict__ flux_pp, double* __restrict__ field_pp) {
   __shared__ BatchTile<TILE_, NE, MIXED> T;
   __shared__ double red[4 * (kBlock / 64)];
   __shared__ unsigned s_item;
@@ -298,6 +188,14 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
         } else {
           f = flux_from_sums(a64.re[e][0], a64.im[e][0], a64.re[e][1], a64.im[e][1], a64.re[e][2], a64.im[e][2],
                              es.kw[e], be.c2);
+          if (field_pp) {
+            size_t const b = ((size_t)p * be.ne + ee) * 6 * ns + j;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              field_pp[b + (size_t)(2 * c) * ns] = a64.re[e][c];
+              field_pp[b + (size_t)(2 * c + 1) * ns] = a64.im[e][c];
+            }
+          }
         }
         flux_pp[((size_t)p * be.ne + ee) * ns + j] = f;
       }
@@ -321,19 +219,20 @@ __global__ void particle_reduce_kernel(const double* __restrict__ flux_pp, int n
 
 template <int NE, int TILE_, bool MIXED>
 hipError_t launch_batch_t(BatchTraj const& bt, const BatchParticle* P, const double* obs, int ns,
-                          BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp, int nblocks,
-                          hipStream_t s) {
-  batch_flux_kernel<NE, TILE_, MIXED><<<nblocks, kBlock, 0, s>>>(bt, P, obs, ns, be, nitems, counter, flux_pp);
+                          BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp,
+                          double* field_pp, int nblocks, hipStream_t s) {
+  batch_flux_kernel<NE, TILE_, MIXED><<<nblocks, kBlock, 0, s>>>(bt, P, obs, ns, be, nitems, counter, flux_pp,
+                                                                 field_pp);
   return hipGetLastError();
 }
 
 template <bool MIXED>
 hipError_t launch_batch_m(BatchConfig const& cfg, BatchTraj const& bt, const BatchParticle* P, const double* obs, int ns,
-                          BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp, int nblocks,
-                          hipStream_t s) {
+                          BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp,
+                          double* field_pp, int nblocks, hipStream_t s) {
 #define SRS_BATCH_CASE(NE_, TL_)                 \
   if (cfg.ne_per_item == NE_ && cfg.tile == TL_) \
-    return launch_batch_t<NE_, TL_, MIXED>(bt, P, obs, ns, be, nitems, counter, flux_pp, nblocks, s);
+    return launch_batch_t<NE_, TL_, MIXED>(bt, P, obs, ns, be, nitems, counter, flux_pp, field_pp, nblocks, s);
   SRS_BATCH_CASE(1, 64)
   SRS_BATCH_CASE(1, 128)
   SRS_BATCH_CASE(1, 256)
@@ -383,11 +282,11 @@ int batch_max_blocks(BatchConfig const& c, bool mixed) {
 }
 
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
-                            BatchTraj const& bt, hipStream_t s) {
+                            BatchTraj const& bt, Checkpoints const& ck, hipStream_t s) {
   dim3 const grid((np + 63) / 64);
-  if (nu == 1) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt);
-  else if (nu == 2) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt);
-  else rk4_batch_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt);
+  if (nu == 1) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck);
+  else if (nu == 2) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck);
+  else rk4_batch_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck);
   return hipGetLastError();
 }
 
@@ -398,9 +297,9 @@ hipError_t launch_zero_counter(unsigned* c, hipStream_t s) {
 
 hipError_t launch_batch_flux(BatchConfig const& cfg, bool mixed, BatchTraj const& bt, const BatchParticle* P,
                              const double* obs, int ns, BatchEnergies const& be, unsigned nitems, unsigned* counter,
-                             double* flux_pp, int nblocks, hipStream_t s) {
-  return mixed ? launch_batch_m<true>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, nblocks, s)
-               : launch_batch_m<false>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, nblocks, s);
+                             double* flux_pp, double* field_pp, int nblocks, hipStream_t s) {
+  return mixed ? launch_batch_m<true>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, nullptr, nblocks, s)
+               : launch_batch_m<false>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, field_pp, nblocks, s);
 }
 
 hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, double weight, double* out, hipStream_t s) {

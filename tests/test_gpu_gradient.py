@@ -1,0 +1,125 @@
+"""GPU tests of the hand-written adjoint (flux integral + RK4 with checkpoints) for
+d(loss)/d(pole errors), through the C ABI and torch.autograd.
+
+References: the oracle's forward-mode (dual number) FP64 gradient, which shares no
+derivative formula with the adjoint, and central differences of the GPU forward.
+Central differences of this FP64 model carry ~7e-9/h relative noise (phase rounding of
+the flux), so they are taken at h = 1e-4 and compared at 1e-3."""
+import numpy as np
+import pytest
+
+import cases
+from conftest import golden
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def grad():
+    from srsoriginal_amd import abi, grad as G
+    assert abi.device_count() >= 1
+    return G
+
+
+def _fixture_problem(G, oracle, ct_start=0.0, npoints=None):
+    name = "poles_flux_and_fd_gradient"
+    a = cases.args(name)
+    g = golden(name)
+    pole = cases.pole(a)
+    b = cases.beam(a)
+    p = oracle.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    r = cases.rect(a)
+    xyz, _, _ = oracle.rect_surface(r["plane"], r["nx1"], r["nx2"], r["w1"], r["w2"], r["rot"], r["trans"], r["normal"])
+    c0, c1, n = cases.ct(a)
+    n = npoints or n
+    row = dict(x0=list(p.x0), beta0=list(p.beta0), qmg=p.qmg, t0=0.0, ct_start=ct_start, ct_stop=c1, npoints=n)
+    und = [dict(field=pole["field"], period=pole["period"], nperiods=pole["nperiods"])]
+    prob = G.PoleProblem(und, 0, row, xyz, float(a["energy_ev"]), p.q, p.current, target=g["flux_ideal"][:len(xyz)])
+    return prob, p, pole, xyz, (ct_start, c1, n), g
+
+
+def _deltas(g, nreal):
+    d = [g["deltas"]] + [np.random.default_rng(100 + k).normal(0, 0.005, len(g["deltas"])) for k in range(nreal - 1)]
+    return np.array(d)
+
+
+def test_pole_gradient_matches_oracle_dual_numbers(grad, oracle):
+    say("pole-error gradient: GPU adjoint vs oracle dual numbers (5x5, 2001 points, 3 realisations)")
+    prob, p, pole, xyz, (c0, c1, n), g = _fixture_problem(grad, oracle)
+    D = _deltas(g, 3)
+    loss, flux, tape = prob.forward(D, want_flux=True)
+    G = prob.backward(tape, len(D))
+    for r in range(len(D)):
+        F = oracle.FieldSet([], pole, D[r])
+        lo, go = oracle.pole_loss_gradient(F, p, c0, c1, n, xyz, prob.energy_ev, prob.target, nthreads=8)
+        el = abs(loss[r] - lo) / lo
+        eg = np.max(np.abs(G[r] - go)) / np.max(np.abs(go))
+        say("  realisation %d: loss rel %.3e  gradient normwise %.3e" % (r, el, eg))
+        assert el <= 1e-9 and eg <= 1e-8
+    assert abs(loss[0] - float(g["loss"])) <= 1e-9 * float(g["loss"])
+
+
+def test_pole_gradient_backward_leg_and_checkpoint_intervals(grad, oracle):
+    say("pole-error gradient with a backward RK4 leg (ct from -0.3 m); checkpoints every 1000 / 7 / whole")
+    prob, p, pole, xyz, (c0, c1, n), g = _fixture_problem(grad, oracle, ct_start=-0.3, npoints=2201)
+    D = _deltas(g, 2)
+    grads = []
+    for every in (1000, 7, 100000):
+        prob.checkpoint_every = every
+        loss, _, tape = prob.forward(D)
+        grads.append(prob.backward(tape, len(D)))
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+    for r in range(len(D)):
+        F = oracle.FieldSet([], pole, D[r])
+        lo, go = oracle.pole_loss_gradient(F, p, c0, c1, n, xyz, prob.energy_ev, prob.target, nthreads=8)
+        eg = np.max(np.abs(grads[0][r] - go)) / np.max(np.abs(go))
+        say("  realisation %d: gradient normwise %.3e" % (r, eg))
+        assert eg <= 1e-8
+
+
+def test_pole_gradient_central_differences_config4_shape(grad, oracle):
+    say("pole-error gradient vs GPU central differences, 21x21 over 5 mm at 30 m, 20001 points, 2 realisations")
+    name = "poles_flux_and_fd_gradient"
+    a = cases.args(name)
+    b = cases.beam(a)
+    p = oracle.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    xyz, _, _ = oracle.rect_surface("XY", 21, 21, 0.005, 0.005, (0, 0, 0), (0, 0, 30), 0)
+    row = dict(x0=list(p.x0), beta0=list(p.beta0), qmg=p.qmg, t0=0.0, ct_start=0.0, ct_stop=3.0, npoints=20001)
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    prob = grad.PoleProblem(und, 0, row, xyz, 143.52176088044024, p.q, p.current)
+    prob.target = prob.flux(np.zeros((1, 82)))[0]
+    rng = np.random.default_rng(7)
+    D = rng.normal(0, 0.005, (2, 82))
+    loss, _, tape = prob.forward(D)
+    G = prob.backward(tape, 2)
+    h = 1e-4
+    for k in range(3):
+        v = rng.normal(0, 1, (2, 82))
+        lp = prob.forward(D + h * v)[0]
+        lm = prob.forward(D - h * v)[0]
+        fd = (lp - lm) / (2 * h)
+        ad = np.sum(G * v, axis=1)
+        err = np.abs(fd - ad) / np.abs(ad)
+        say("  direction %d: adjoint %s  fd %s  rel %s" % (k, ad, fd, err))
+        assert np.all(err <= 1e-3)
+
+
+def test_autograd_function_and_tape(grad, oracle):
+    import torch
+    from srsoriginal_amd import abi
+    say("torch.autograd.Function over the adjoint; tape replaced by a later forward")
+    prob, p, pole, xyz, (c0, c1, n), g = _fixture_problem(grad, oracle)
+    D = _deltas(g, 3)
+    d = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor([0.5, -2.0, 1.5], dtype=torch.float64)
+    loss = grad.pole_loss(d, prob)
+    (loss * w).sum().backward()
+    ref_loss, _, tape = prob.forward(D)
+    ref = prob.backward(tape, 3, w.numpy())
+    assert np.array_equal(loss.detach().numpy(), ref_loss)
+    assert np.array_equal(d.grad.numpy(), ref)
+    stale = tape
+    prob.forward(D[:1])
+    with pytest.raises(abi.SrsError, match="tape"):
+        prob.backward(stale, 3)

@@ -256,3 +256,43 @@ def test_pole_scaled_flux_and_fd_gradient(oracle):
         lm = np.sum(np.concatenate([flux(dm) - f0, pad]) ** 2)
         gk = (lp - lm) / (2 * h)
         assert gk == g["fd_grad"][k]
+
+
+def test_pole_loss_gradient_matches_reference_central_differences(oracle):
+    """The oracle's forward-mode (dual number) gradient of the pole-error flux loss
+    against central differences.
+
+    The FP64 flux carries ~1e-7 relative noise from rounding its ~1e11 rad phase, and
+    that noise does not cancel between delta +- h, so a central difference has error
+    ~7e-9/h (relative) on top of its O(h^2) truncation: the reference's own h = 1e-6
+    gradient (fixture) is good to ~1 %, a difference of the oracle at h = 1e-4 to ~1e-4.
+    Both bounds are checked; a dropped term or a flipped sign in the derivative of any
+    O(1) part of the integrand or the field moves the gradient far outside them."""
+    name = "poles_flux_and_fd_gradient"
+    a = cases.args(name)
+    g = golden(name)
+    pole = cases.pole(a)
+    b = cases.beam(a)
+    p = oracle.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    xyz, nrm, x12 = _rect(oracle, name)
+    c0, c1, n = cases.ct(a)
+    ns = len(xyz)
+    d = g["deltas"]
+    e_ev = float(a["energy_ev"])
+    f0 = g["flux_ideal"][:ns]
+    loss, grad = oracle.pole_loss_gradient(oracle.FieldSet([], pole, d), p, c0, c1, n, xyz, e_ev, f0, nthreads=8)
+    assert abs(loss - float(g["loss"])) <= 1e-12 * float(g["loss"])
+    fd = g["fd_grad"]
+    scale = np.max(np.abs(fd))
+    assert np.max(np.abs(grad - fd)) <= 3e-2 * scale
+
+    def L(delta):
+        t, dt = oracle.trajectory(oracle.FieldSet([], pole, delta), p, c0, c1, n)
+        return np.sum((oracle.flux(t, dt, xyz, e_ev, p.q, p.current) - f0) ** 2)
+
+    h = 1e-4
+    for k in (0, 17, 40, 41, 81):
+        dp = d.copy(); dp[k] += h
+        dm = d.copy(); dm[k] -= h
+        fdk = (L(dp) - L(dm)) / (2 * h)
+        assert abs(grad[k] - fdk) <= 5e-4 * scale, (k, grad[k], fdk)
