@@ -100,8 +100,10 @@ def test_pole_gradient_central_differences_config4_shape(grad, oracle):
         lm = prob.forward(D - h * v)[0]
         fd = (lp - lm) / (2 * h)
         ad = np.sum(G * v, axis=1)
-        err = np.abs(fd - ad) / np.abs(ad)
-        say("  direction %d: adjoint %s  fd %s  rel %s" % (k, ad, fd, err))
+        # relative to |grad| |v|: a directional derivative can be small by cancellation
+        # while the difference noise keeps the scale of the whole gradient
+        err = np.abs(fd - ad) / (np.linalg.norm(G, axis=1) * np.linalg.norm(v, axis=1))
+        say("  direction %d: adjoint %s  fd %s  err %s" % (k, ad, fd, err))
         assert np.all(err <= 1e-3)
 
 
