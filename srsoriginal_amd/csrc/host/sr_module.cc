@@ -189,10 +189,12 @@ PYBIND11_MODULE(_sr, m) {
       .def("set_gpu_global", &srs::OSCARSSR::SetUseGPUGlobal)
       .def("check_gpu", &srs::OSCARSSR::CheckGPU)
       .def("calculate_trajectory",
-           [](srs::OSCARSSR& e) {
-             e.CalculateTrajectory();
+           [](srs::OSCARSSR& e, int gpu) {
+             if (gpu) e.CalculateTrajectoryGPU();
+             else e.CalculateTrajectory();
              return py::make_tuple(traj_array(e.GetTrajectory()), e.GetTrajectory().dt);
-           })
+           },
+           py::arg("gpu") = 0)
       .def("get_trajectory", [](srs::OSCARSSR const& e) { return py::make_tuple(traj_array(e.GetTrajectory()), e.GetTrajectory().dt); })
       .def("calculate_spectrum",
            [](srs::OSCARSSR& e, std::vector<double> obs, std::vector<double> points, size_t n, double e0, double e1,
@@ -205,11 +207,13 @@ PYBIND11_MODULE(_sr, m) {
            })
       .def("calculate_flux",
            [](srs::OSCARSSR& e, srs::SurfacePoints const& S, double energy, int nparticles, int nthreads, int gpu,
-              int dim, int precision) {
+              int dim, int precision, bool batched) {
              srs::ScalarContainer C;
-             e.CalculateFlux(S, energy, C, nparticles, nthreads, gpu, dim, prec_of(precision));
+             e.CalculateFlux(S, energy, C, nparticles, nthreads, gpu, dim, prec_of(precision), batched);
              return container_arrays(C);
-           })
+           },
+           py::arg("surface"), py::arg("energy"), py::arg("nparticles"), py::arg("nthreads"), py::arg("gpu"),
+           py::arg("dim"), py::arg("precision"), py::arg("batched") = false)
       .def("calculate_power_density",
            [](srs::OSCARSSR& e, srs::SurfacePoints const& S, int dim, bool directional, int nparticles, int nthreads,
               int gpu, int precision) {

@@ -555,6 +555,71 @@ void add_points(SurfacePoints const& surf, ScalarContainer& C, int dim) {
 }
 }  // namespace
 
+void OSCARSSR::device_fields(std::vector<srs_undulator>& out) const {
+  out.clear();
+  for (auto const& u : bfield_.fields()) {
+    srs_undulator d{};
+    d.field[0] = u.field.x;
+    d.field[1] = u.field.y;
+    d.field[2] = u.field.z;
+    d.period[0] = u.period.x;
+    d.period[1] = u.period.y;
+    d.period[2] = u.period.z;
+    d.nperiods = u.nperiods;
+    d.phase = u.phase;
+    d.taper = u.taper;
+    d.center[0] = u.center.x;
+    d.center[1] = u.center.y;
+    d.center[2] = u.center.z;
+    d.pole_delta = u.pole_delta.empty() ? nullptr : u.pole_delta.data();
+    out.push_back(d);
+  }
+}
+
+srs_particle OSCARSSR::device_particle(Particle const& p) const {
+  srs_particle d{};
+  d.x0[0] = p.x0.x;
+  d.x0[1] = p.x0.y;
+  d.x0[2] = p.x0.z;
+  d.beta0[0] = p.beta0.x;
+  d.beta0[1] = p.beta0.y;
+  d.beta0[2] = p.beta0.z;
+  d.qmg = p.qmg;
+  d.t0 = p.t0;
+  d.ct_start = ct_start_;
+  d.ct_stop = ct_stop_;
+  d.npoints = (int64_t)npoints_;
+  return d;
+}
+
+void OSCARSSR::CalculateTrajectoryGPU() {
+  if (particle_.type.empty()) SetNewParticle();
+  CalculateTrajectoryGPU(particle_);
+}
+
+void OSCARSSR::CalculateTrajectoryGPU(Particle& P) const {
+  // Same checks and point layout as CalculateTrajectory (OSCARSSR.cc:562-673).
+  if (ct_start_ > P.t0) throw std::out_of_range("start time is greater than T0");
+  if (ct_start_ >= ct_stop_) throw std::out_of_range("start time is greater than stop time.  check that set_ctstartstop is set");
+  if (P.type.empty()) throw std::out_of_range("particle not initialized.  make sure you have a particle or beam defined");
+  if (npoints_ < 2) throw std::out_of_range("number of trajectory points must be >= 2");
+  if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+  std::vector<srs_undulator> und;
+  device_fields(und);
+  srs_particle const dp = device_particle(P);
+  int64_t total = 0;
+  check_status(srs_hip_trajectory_points(&dp, 1, &total));
+  std::vector<double> soa((size_t)9 * total);
+  int64_t offs[2];
+  double dt = 0;
+  check_status(srs_hip_trajectories(und.data(), (int)und.size(), &dp, 1, soa.data(), total, offs, &dt));
+  Trajectory& T = P.traj;
+  T.clear();
+  T.dt = dt;
+  std::vector<double>* dst[9] = {&T.x, &T.y, &T.z, &T.bx, &T.by, &T.bz, &T.ax, &T.ay, &T.az};
+  for (int k = 0; k < 9; ++k) dst[k]->assign(soa.begin() + (size_t)k * total, soa.begin() + (size_t)(k + 1) * total);
+}
+
 void OSCARSSR::flux_one(Particle& p, SurfacePoints const& surf, double energy_ev, ScalarContainer& C, double weight,
                         Precision prec) {
   CalculateTrajectory(p);
@@ -566,7 +631,7 @@ void OSCARSSR::flux_one(Particle& p, SurfacePoints const& surf, double energy_ev
 }
 
 void OSCARSSR::CalculateFlux(SurfacePoints const& surf, double energy_ev, ScalarContainer& C, int nparticles,
-                             int nthreads, int gpu, int dim, Precision prec) {
+                             int nthreads, int gpu, int dim, Precision prec, bool batched) {
   // OSCARSSR::CalculateFlux(Surface, E, C, NP, NT, GPU, Dim), OSCARSSR.cc:2418-2489.
   // Container layout follows the reference route (OSCARSSR.cc:2352-2358, :2512, :2570-2572):
   //   gpu=1: N extra 3-D points per particle; gpu=0 with 1 thread: N extra points per
@@ -577,6 +642,33 @@ void OSCARSSR::CalculateFlux(SurfacePoints const& surf, double energy_ev, Scalar
   add_points(surf, C, dim);
   int const np = nparticles == 0 ? 1 : nparticles;
   double const w = nparticles == 0 ? 1.0 : 1.0 / (double)nparticles;
+  if (batched && gpu == 1 && nparticles > 1) {
+    // draw all particles first (same draws, same order), then one device pipeline
+    std::vector<srs_particle> parts;
+    parts.reserve(np);
+    double q = 0, cur = 0;
+    bool same = true;
+    for (int i = 0; i != np; ++i) {
+      SetNewParticle();
+      add_points(surf, C, 3);
+      parts.push_back(device_particle(particle_));
+      if (i == 0) {
+        q = particle_.q;
+        cur = particle_.current;
+      } else if (particle_.q != q || particle_.current != cur) {
+        same = false;
+      }
+    }
+    if (!same) throw std::invalid_argument("batched flux needs all beams to share charge and current (use batched=0)");
+    if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+    std::vector<srs_undulator> und;
+    device_fields(und);
+    std::vector<double> out(surf.size());
+    check_status(srs_hip_batch_flux(und.data(), (int)und.size(), parts.data(), np, q, cur, surf.xyz.data(),
+                                    (int64_t)surf.size(), &energy_ev, 1, w, (int)prec, SRS_BATCH_GRAPH, out.data()));
+    for (size_t i = 0; i < out.size(); ++i) C.add_to(i, out[i]);
+    return;
+  }
   for (int i = 0; i != np; ++i) {
     if (nparticles != 0) SetNewParticle();
     if (gpu == 1) {

@@ -1,0 +1,79 @@
+"""GPU routes of the sr type beyond the reference's: the device RK4 switch of
+calculate_trajectory and the batched multi-particle flux (one device pipeline for
+all particles instead of the per-particle loop), both against the reference.
+
+The multi-particle cases run in fresh interpreters: the reference's set_seed does not
+reset the normal distribution's cached value (TRandomA.cc:12-48), so the particle
+stream after set_seed depends on earlier draws in the same process."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from conftest import golden
+from gpu_util import normwise, say
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_device_trajectory_switch_matches_host():
+    from srsoriginal_amd.sr import sr
+    say("sr.calculate_trajectory(gpu=1) vs gpu=0, C0 undulator, 20001 points")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.050], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(20001)
+    host = np.array(osr.calculate_trajectory())
+    th, dth = osr.trajectory_array()
+    dev = np.array(osr.calculate_trajectory(gpu=1))
+    td, dtd = osr.trajectory_array()
+    assert host.shape == dev.shape and dth == dtd
+    for k, scale in ((0, 1e-5), (2, 1.0), (3, 1e-3), (5, 1.0)):
+        err = np.max(np.abs(td[k] - th[k])) / scale
+        say("  component %d max deviation / scale %.3e" % (k, err))
+        assert err < 1e-12
+
+
+_SCRIPT = r"""
+import json, sys
+sys.path.insert(0, %(root)r); sys.path.insert(0, %(tests)r)
+import numpy as np, cases
+from srsoriginal_amd.sr import sr
+a = cases.args("multiflux_c3_small")
+osr = sr()
+osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                      sigma_energy_GeV=float(a["sigE"]), beta=cases.vec(a, "beta"), emittance=cases.vec(a, "emit"),
+                      horizontal_direction=cases.vec(a, "hdir"), lattice_reference=cases.vec(a, "latref"))
+osr.set_seed(int(a["seed"]))
+osr.set_ctstartstop(0, 3)
+osr.set_npoints_trajectory(int(a["npoints"]))
+out = osr.calculate_flux_rectangle(energy_eV=float(a["energy_ev"]), npoints=[9, 9], plane="XY", width=[0.01, 0.01],
+                                   translation=[0, 0, 30], nparticles=int(a["nparticles"]), gpu=1,
+                                   batched=int(sys.argv[1]))
+print(json.dumps([v for _, v in out]))
+"""
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_multi_particle_flux_routes_match_reference(batched):
+    say("sr.calculate_flux_rectangle(nparticles=4, gpu=1, batched=%d) vs the reference's 4-particle run" % batched)
+    code = _SCRIPT % dict(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code, str(batched)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    vals = np.array(json.loads(r.stdout.strip().splitlines()[-1]))
+    ref = golden("multiflux_c3_small")["container"][:, 3]
+    n = 81
+    # gpu=1 route: N values, then N extra 3-D points per particle (OSCARSSR.cc:2570-2572)
+    assert len(vals) == n + 4 * n
+    assert np.all(vals[n:] == 0)
+    err = normwise(vals[:n], ref[:n])
+    say("  normwise vs reference %.3e" % err)
+    assert err <= 1e-9
