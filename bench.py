@@ -102,6 +102,96 @@ def batched_c3_share(nparticles=512, reps=2):
             "timed": "whole hipGraph (device RK4 + persistent flux + particle reduction), HIP events"}
 
 
+def spectrum_and_power_density(reps=5):
+    """Config 0 (FP64 spectrum, 2000 energies at (0,0,30 m)) and config 2 (EPU power
+    density on 301x301 over 60 mm at 30 m, NT=40001; mixed, checked against FP64 and
+    by the integrated power), device entry points, inputs resident in HBM."""
+    import torch
+    from srsoriginal_amd import abi
+    from srsoriginal_amd.lib import _sr
+    from srsoriginal_amd.sr import sr
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    res = {}
+    # config 0
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.050], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(20001)
+    osr.calculate_trajectory()
+    traj, dt = osr.trajectory_array()
+    q, cur = osr._e.particle_charge_current()
+    energies = 100.0 + (3000.0 - 100.0) / 1999 * (np.arange(2000) + 1)  # TSpectrumContainer.cc:82
+    d_traj = torch.from_numpy(np.ascontiguousarray(traj)).to(dev)
+    ptrs = [d_traj[k].data_ptr() for k in range(9)]
+    terms0 = float(traj.shape[1]) * len(energies)
+    spec = {}
+    for prec in (abi.FP64, abi.MIXED):
+        out = torch.zeros(len(energies), dtype=torch.float64, device=dev)
+        for _ in range(2):
+            abi.spectrum_device(ptrs, traj.shape[1], dt, (0, 0, 30), energies, q, cur, prec, out.data_ptr(),
+                                stream.cuda_stream)
+        torch.cuda.synchronize()
+        abi.timing_enable(True)
+        for _ in range(reps):
+            abi.spectrum_device(ptrs, traj.shape[1], dt, (0, 0, 30), energies, q, cur, prec, out.data_ptr(),
+                                stream.cuda_stream)
+        kms, nl = abi.timing_read()
+        abi.timing_enable(False)
+        spec[prec] = (kms / max(nl, 1), out.cpu().numpy())
+    s64, smx = spec[abi.FP64][1], spec[abi.MIXED][1]
+    res["spectrum_c0"] = {"workload": "spectrum at (0,0,30 m), 2000 energies 101.45-3001.45 eV, NT=20001",
+                          "fp64_terms_per_s": terms0 / (spec[abi.FP64][0] * 1e-3),
+                          "mixed_terms_per_s": terms0 / (spec[abi.MIXED][0] * 1e-3),
+                          "fp64_kernel_ms": spec[abi.FP64][0], "mixed_kernel_ms": spec[abi.MIXED][0],
+                          "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(smx - s64)) / np.max(np.abs(s64)))}
+    # config 2
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 0.6, 0], period=[0, 0, 0.057], nperiods=30)
+    osr.add_bfield_undulator(bfield=[0.6, 0, 0], period=[0, 0, 0.057], nperiods=30, phase=-np.pi / 2)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(40001)
+    osr.calculate_trajectory()
+    traj, dt = osr.trajectory_array()
+    q, cur = osr._e.particle_charge_current()
+    surf = _sr.Surface.rectangle("XY", 301, 301, 0.06, 0.06, [0, 0, 0], [0, 0, 30], 0)
+    xyz, nrm = surf.xyz, surf.normals
+    d_traj = torch.from_numpy(np.ascontiguousarray(traj)).to(dev)
+    ptrs = [d_traj[k].data_ptr() for k in range(9)]
+    d_obs = torch.from_numpy(np.ascontiguousarray(xyz.T)).to(dev)
+    d_nrm = torch.from_numpy(np.ascontiguousarray(nrm.T)).to(dev)
+    terms2 = float(traj.shape[1]) * len(xyz)
+    pdr = {}
+    for prec in (abi.FP64, abi.MIXED):
+        out = torch.zeros(len(xyz), dtype=torch.float64, device=dev)
+        for _ in range(2):
+            abi.power_density_device(ptrs, traj.shape[1], dt, d_obs.data_ptr(), d_nrm.data_ptr(), len(xyz), False, q,
+                                     cur, prec, out.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        abi.timing_enable(True)
+        for _ in range(reps):
+            abi.power_density_device(ptrs, traj.shape[1], dt, d_obs.data_ptr(), d_nrm.data_ptr(), len(xyz), False, q,
+                                     cur, prec, out.data_ptr(), stream.cuda_stream)
+        kms, nl = abi.timing_read()
+        abi.timing_enable(False)
+        pdr[prec] = (kms / max(nl, 1), out.cpu().numpy())
+    p64, pmx = pdr[abi.FP64][1], pdr[abi.MIXED][1]
+    area = (0.06 / 300) ** 2 * 1e6  # mm^2 per point (plots_mpl.py:474-489 sums the map this way)
+    res["power_density_c2"] = {"workload": "EPU 2x30 periods 57 mm 0.6 T, NT=40001, 301x301 over 60 mm at 30 m",
+                               "fp64_terms_per_s": terms2 / (pdr[abi.FP64][0] * 1e-3),
+                               "mixed_terms_per_s": terms2 / (pdr[abi.MIXED][0] * 1e-3),
+                               "fp64_kernel_ms": pdr[abi.FP64][0], "mixed_kernel_ms": pdr[abi.MIXED][0],
+                               "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(pmx - p64)) / np.max(np.abs(p64))),
+                               "mixed_vs_fp64_rms_rel_err": float(np.sqrt(np.mean((pmx - p64) ** 2))
+                                                                  / np.sqrt(np.mean(p64 ** 2))),
+                               "integrated_power_W": {"fp64": float(p64.sum() * area), "mixed": float(pmx.sum() * area)}}
+    log("spectrum C0: fp64 %.3f ms, mixed %.3f ms; power density C2: fp64 %.3f ms, mixed %.3f ms"
+        % (spec[abi.FP64][0], spec[abi.MIXED][0], pdr[abi.FP64][0], pdr[abi.MIXED][0]))
+    return res
+
+
 def gradient_c4(nreal=256, reps=2):
     """Config-4 gradient: 256 pole-error realisations (82 poles, delta ~ N(0, 0.005^2),
     seeded) of the 41-period undulator, loss = sum over 51x51 points (5 mm at 30 m, E*)
@@ -172,6 +262,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-other", action="store_true", help="skip the config-0 spectrum and config-2 power density")
     ap.add_argument("--profile", action="store_true",
                     help="one launch of each form plus a 256 MiB device copy (the HBM counter scale), for rocprofv3")
     a = ap.parse_args()
@@ -261,6 +352,8 @@ def main():
         out["batched_c3_share"] = batched_c3_share()
     if not a.no_grad:
         out["gradient_c4"] = gradient_c4()
+    if not a.no_other:
+        out.update(spectrum_and_power_density())
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(traj, dt, xyz, q, cur)

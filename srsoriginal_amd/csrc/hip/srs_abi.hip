@@ -85,12 +85,13 @@ struct Context {
   srs::BatchConfig b_last{1, 128};
   int cus = 0;
   // pole-error gradient: the forward call leaves its tape here for the backward call
-  Buf g_ckpt, g_field, g_target, g_loss, g_dloss, g_gy, g_rec, g_grad;
+  Buf g_ckpt, g_field, g_target, g_loss, g_dloss, g_gy, g_rec, g_grad, g_lam, g_sweep;
   struct Tape {
     int64_t id = 0;
     bool valid = false;
     int np = 0, ns = 0, nund = 0, poled = 0, npoles = 0, every = 0;
-    int rec_steps = 0;  // longest recomputed segment: min(every, longest RK4 leg)
+    int64_t nf = 0, nb = 0;
+    int64_t seg_per_pass = 1;  // checkpoint segments recomputed per adjoint pass
     int64_t total = 0, nmax = 0;
     srs::BatchTraj bt{};
     srs::Checkpoints ck{};
@@ -866,7 +867,14 @@ int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const sr
   t.poled = poled;
   t.npoles = npoles;
   t.every = every;
-  t.rec_steps = (int)std::min<int64_t>(every, std::max(P[0].nf, P[0].nb));
+  t.nf = P[0].nf;
+  t.nb = P[0].nb;
+  {
+    // as many segments per pass as a 4 GiB record budget holds (all of them at C4 sizes)
+    int64_t const nseg = (std::max(t.nf, t.nb) + every - 1) / every;
+    double const per_seg = (double)srs::adjoint_record_doubles(every, (int)nreal) * sizeof(double);
+    t.seg_per_pass = std::max<int64_t>(1, std::min<int64_t>(nseg, (int64_t)(4.0 * (1 << 30) / per_seg)));
+  }
   t.total = total;
   t.nmax = nmax;
   t.bt = bt;
@@ -898,7 +906,10 @@ int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out) {
     HIPCHK(hipStreamSynchronize(s));
   }
   HIPCHK(c.g_gy.ensure((size_t)6 * t.total * sizeof(double)));
-  HIPCHK(c.g_rec.ensure(srs::adjoint_record_doubles(t.rec_steps, t.np) * sizeof(double)));
+  int64_t const rec_steps = std::min<int64_t>(t.seg_per_pass * t.every, std::max(t.nf, t.nb));
+  HIPCHK(c.g_rec.ensure(srs::adjoint_record_doubles(rec_steps, t.np) * sizeof(double)));
+  HIPCHK(c.g_lam.ensure((size_t)t.np * 6 * sizeof(double)));
+  HIPCHK(c.g_sweep.ensure(srs::adjoint_sweep_doubles(t.seg_per_pass, t.np, t.npoles) * sizeof(double)));
   HIPCHK(c.g_grad.ensure((size_t)t.np * t.npoles * sizeof(double)));
   auto* Pd = c.b_part.as<srs::BatchParticle>();
   {
@@ -911,8 +922,9 @@ int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out) {
   {
     KernelTimer timer(c, s);
     HIPCHK(srs::launch_rk4_adjoint(c.b_und.as<srs::DevUndulator>(), t.nund, t.poled, c.b_poles.as<double>(), Pd, t.np,
-                                   t.ck, c.g_gy.as<double>(), t.total, c.g_rec.as<double>(), c.g_grad.as<double>(),
-                                   t.npoles, s));
+                                   t.nf, t.nb, t.ck, t.seg_per_pass, c.g_gy.as<double>(), t.total,
+                                   c.g_rec.as<double>(), c.g_sweep.as<double>(), c.g_lam.as<double>(),
+                                   c.g_grad.as<double>(), t.npoles, s));
   }
   HIPCHK(hipMemcpyAsync(grad_out, c.g_grad.p, (size_t)t.np * t.npoles * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

@@ -9,12 +9,14 @@
 //   flux_adjoint_kernel  lanes over (r, trajectory point k), loop over observation
 //                        points staged in LDS: dL/dX_k and dL/dbeta_k of the
 //                        radiation integral (analytic derivative of every term);
-//   rk4_adjoint_kernel   one lane per realisation walks the trajectory backwards one
-//                        checkpoint segment at a time: it recomputes the segment's
-//                        steps from the checkpoint (recording each RK4 stage's
-//                        velocity, B, dB/dD and pole sine), then sweeps the steps in
-//                        reverse applying the transposed RK4 step and collecting
-//                        dL/d delta_p.
+//   rk4_record_kernel    recomputes checkpoint segments from their checkpoints, all
+//                        (segment, realisation) pairs in parallel, recording each RK4
+//                        stage's velocity, B, dB/dD and pole sine;
+//   rk4_segment_sweep_kernel  sweeps every segment in reverse in parallel (the
+//                        transposed RK4 step is linear in the cotangent: a particular
+//                        sweep plus six unit sweeps per segment), collecting dL/d delta_p;
+//   rk4_combine_kernel   chains the segments per realisation.  Segments are processed
+//                        in passes (last first) of as many as the record budget holds.
 //
 // Radiation-integral term (OSCARSSR.cc:2054-2092), R = O - X, D = |R|, n = R/D:
 //   T_c = a_c P,  a_c = (beta_c - n_c)/D - i cw n_c/D^2,  P = exp(i omega (dt k + D/c)).
@@ -37,8 +39,9 @@ namespace srs {
 namespace {
 
 constexpr int kAdjTile = 256;
-constexpr int kStageRec = 11;          // v[3], B[3], dBdD[3], sin at pole, pole index
-constexpr int kStepRec = 4 * kStageRec;
+constexpr int kStageRec = 11;            // v[3], B[3], dBdD[3], sin at pole, pole index
+constexpr int kStepRec = 4 * kStageRec + 6;  // + the direct dL/dy of the step's point
+constexpr int kSweeps = 7;                 // per segment: particular + 6 unit end cotangents
 
 __global__ void zero_kernel(double* __restrict__ a, int64_t n) {
   int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -185,19 +188,20 @@ __device__ __forceinline__ void step_record(RK4Ctx<NU> const& R, double* y, doub
 }
 
 struct PoleAcc {
-  double* out;  // this realisation's 2N gradient entries
+  double* out;  // this lane's gradient entries, pole p at out[p * stride]
   double acc;
   int cur;
+  int stride;
   __device__ __forceinline__ void add(int p, double v) {
     if (p != cur) {
-      if (cur >= 0) out[cur] += acc;
+      if (cur >= 0) out[(size_t)cur * stride] += acc;
       cur = p;
       acc = 0.0;
     }
     acc += v;
   }
   __device__ __forceinline__ void flush() {
-    if (cur >= 0) out[cur] += acc;
+    if (cur >= 0) out[(size_t)cur * stride] += acc;
     cur = -1;
     acc = 0.0;
   }
@@ -248,79 +252,138 @@ __device__ __forceinline__ void step_adjoint(const double* __restrict__ st, doub
   for (int c = 0; c < 6; ++c) lam[c] = ybar[c];
 }
 
+// Recompute checkpoint segments [sa, sb) of one RK4 leg for every realisation in
+// parallel, one lane per (segment, realisation), writing each step's stage records to
+// rec[((i - sa*C) * kStepRec + f) * np + r] (consecutive lanes = consecutive
+// realisations, so the stores coalesce).  The recomputed steps are bitwise those of
+// the forward pass (same arithmetic from the same checkpointed state).
 template <int NU>
-__global__ __launch_bounds__(64) void rk4_adjoint_kernel(const DevUndulator* __restrict__ U, int nu, int poled,
-                                                          const double* __restrict__ poles,
-                                                          const BatchParticle* __restrict__ Ps, int np,
-                                                          Checkpoints ck, const double* __restrict__ gy,
-                                                          int64_t total, double* __restrict__ rec,
-                                                          double* __restrict__ grad, int npoles) {
-  int const r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= np) return;
+__global__ __launch_bounds__(64) void rk4_record_kernel(const DevUndulator* __restrict__ U, int nu,
+                                                         const double* __restrict__ poles,
+                                                         const BatchParticle* __restrict__ Ps, int np, Checkpoints ck,
+                                                         int backward_leg, int64_t sa, int64_t sb,
+                                                         const double* __restrict__ gy, int64_t total,
+                                                         double* __restrict__ rec) {
+  int64_t const lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= (sb - sa) * np) return;
+  int const r = (int)(lane % np);
+  int64_t const seg = sa + lane / np;
   BatchParticle const q = Ps[r];
   RK4Ctx<NU> const R{Fields<NU>(U, nu, poles + q.pole_base), q.qmg};
+  int64_t const C = ck.every;
+  int64_t const nleg = backward_leg ? q.nb : q.nf;
+  int64_t const slot = backward_leg ? (q.nf + C - 1) / C + seg : seg;
+  double const h = backward_leg ? -q.dt : q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
+  double y[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) y[c] = ck.state[(r * ck.stride + slot) * 6 + c];
+  int64_t const i0 = seg * C, i1 = (i0 + C < nleg) ? i0 + C : nleg;
+  for (int64_t i = i0; i < i1; ++i) {
+    double* const dst = rec + (size_t)(i - sa * C) * kStepRec * np + r;
+    step_record(R, y, h, hh, h6, dst, np);
+    // forward leg: step i leads from the state of point nb + i; backward leg: to that of point nb - 1 - i
+    int64_t const pt = q.off + (backward_leg ? q.nb - 1 - i : q.nb + i);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) dst[(size_t)(4 * kStageRec + c) * np] = gy[c * total + pt];
+  }
+}
+
+// The reverse sweep is linear in the cotangent it starts from, so every segment is
+// swept independently and in parallel: sweep v = 0 starts from 0 and adds the points'
+// direct gradients (particular part), sweeps v = 1..6 start from the unit cotangent
+// e_{v-1} with no direct terms (the segment's transposed transfer matrix, and its pole
+// sensitivities).  Lane (segment, v, realisation); consecutive lanes = consecutive
+// realisations, so the record loads coalesce.  Outputs per lane: the cotangent at the
+// segment start, lam_out[lane*6 + c], and dL/d delta, del_out[((lane/np)*npoles + p)*np + r].
+__global__ __launch_bounds__(64) void rk4_segment_sweep_kernel(const DevUndulator* __restrict__ U, int poled,
+                                                                const BatchParticle* __restrict__ Ps, int np, int every,
+                                                                int backward_leg, int64_t sa, int64_t sb,
+                                                                const double* __restrict__ rec,
+                                                                double* __restrict__ lam_out,
+                                                                double* __restrict__ del_out, int npoles) {
+  int64_t const lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= (sb - sa) * kSweeps * np) return;
+  int const r = (int)(lane % np);
+  int const v = (int)((lane / np) % kSweeps);
+  int64_t const seg = sa + lane / ((int64_t)np * kSweeps);
+  BatchParticle const q = Ps[r];
   double const u[3] = {U[0].u[0], U[0].u[1], U[0].u[2]};
   double const fpole[3] = {U[poled].field[0], U[poled].field[1], U[poled].field[2]};
-  PoleAcc pa{grad + (size_t)r * npoles, 0.0, -1};
-  double* const myrec = rec + r;
-  int64_t const C = ck.every;
-  int64_t const nf_slots = (q.nf + C - 1) / C;
-  auto load_state = [&](int64_t slot, double* y) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) y[c] = ck.state[(r * ck.stride + slot) * 6 + c];
-  };
-  auto stage_rec = [&](int64_t il) { return myrec + (size_t)il * kStepRec * np; };
-  auto load_rec = [&](int64_t il, double* st) {
-    const double* src = stage_rec(il);
+  PoleAcc pa{del_out + (size_t)(lane / np) * npoles * np + r, 0.0, -1, np};
+  int64_t const C = every;
+  int64_t const nleg = backward_leg ? q.nb : q.nf;
+  double const h = backward_leg ? -q.dt : q.dt;
+  int64_t const i0 = seg * C, i1 = (i0 + C < nleg) ? i0 + C : nleg;
+  double lam[6] = {0, 0, 0, 0, 0, 0};
+  if (v > 0) lam[v - 1] = 1.0;
+  double const gdir = v == 0 ? 1.0 : 0.0;
+  const double* const base = rec + (size_t)(i0 - sa * C) * kStepRec * np + r;
+  double buf[2][kStepRec];
+  auto load = [&](int64_t i, double* st) {
+    const double* src = base + (size_t)(i - i0) * kStepRec * np;
 #pragma unroll
     for (int f = 0; f < kStepRec; ++f) st[f] = src[(size_t)f * np];
   };
-  // forward leg: points nb + i hold y_i, i = 0 .. nf-1
-  {
-    double const h = q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
-    double lam[6] = {0, 0, 0, 0, 0, 0};
-    for (int64_t s = nf_slots - 1; s >= 0; --s) {
-      int64_t const i0 = s * C, i1 = (i0 + C < q.nf) ? i0 + C : q.nf;
-      double y[6];
-      load_state(s, y);
-      for (int64_t i = i0; i < i1; ++i) step_record(R, y, h, hh, h6, stage_rec(i - i0), np);
-      double cur[kStepRec], nxt[kStepRec];
-      load_rec(i1 - 1 - i0, cur);
-      for (int64_t i = i1 - 1; i >= i0; --i) {
-        if (i > i0) load_rec(i - 1 - i0, nxt);
-        step_adjoint(cur, lam, h, q.qmg, u, fpole, pa);
-        int64_t const pt = q.off + q.nb + i;
+  load(i1 - 1, buf[0]);
+  if (i1 - 2 >= i0) load(i1 - 2, buf[1]);
+  for (int64_t i = i1 - 1; i >= i0; i -= 2) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) lam[c] += gy[c * total + pt];
+    for (int d = 0; d < 2; ++d) {
+      int64_t const ii = i - d;
+      if (ii >= i0) {
+        const double* g = buf[d] + 4 * kStageRec;
+        if (backward_leg) {
 #pragma unroll
-        for (int f = 0; f < kStepRec; ++f) cur[f] = nxt[f];
-      }
-    }
-  }
-  // backward leg: point nb - 1 - i holds y'_{i+1}, i = 0 .. nb-1 (y'_0 is the start state)
-  if (q.nb > 0) {
-    double const h = -q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
-    double lam[6] = {0, 0, 0, 0, 0, 0};
-    int64_t const nb_slots = (q.nb + C - 1) / C;
-    for (int64_t s = nb_slots - 1; s >= 0; --s) {
-      int64_t const i0 = s * C, i1 = (i0 + C < q.nb) ? i0 + C : q.nb;
-      double y[6];
-      load_state(nf_slots + s, y);
-      for (int64_t i = i0; i < i1; ++i) step_record(R, y, h, hh, h6, stage_rec(i - i0), np);
-      double cur[kStepRec], nxt[kStepRec];
-      load_rec(i1 - 1 - i0, cur);
-      for (int64_t i = i1 - 1; i >= i0; --i) {
-        if (i > i0) load_rec(i - 1 - i0, nxt);
-        int64_t const pt = q.off + q.nb - 1 - i;
+          for (int c = 0; c < 6; ++c) lam[c] = __builtin_fma(gdir, g[c], lam[c]);
+          step_adjoint(buf[d], lam, h, q.qmg, u, fpole, pa);
+        } else {
+          step_adjoint(buf[d], lam, h, q.qmg, u, fpole, pa);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) lam[c] += gy[c * total + pt];
-        step_adjoint(cur, lam, h, q.qmg, u, fpole, pa);
-#pragma unroll
-        for (int f = 0; f < kStepRec; ++f) cur[f] = nxt[f];
+          for (int c = 0; c < 6; ++c) lam[c] = __builtin_fma(gdir, g[c], lam[c]);
+        }
+        if (ii - 2 >= i0) load(ii - 2, buf[d]);
       }
     }
   }
   pa.flush();
+#pragma unroll
+  for (int c = 0; c < 6; ++c) lam_out[(size_t)lane * 6 + c] = lam[c];
+}
+
+// Chains the segments of one pass for each realisation, last segment first:
+// lam_start = L0 + sum_v lam_end[v] L_{v+1},  grad += D0 + sum_v lam_end[v] D_{v+1},
+// lam_end of the previous segment = lam_start (lam_buf carries it between passes).
+__global__ __launch_bounds__(64) void rk4_combine_kernel(int np, int64_t sa, int64_t sb,
+                                                          const double* __restrict__ lam_out,
+                                                          const double* __restrict__ del_out,
+                                                          double* __restrict__ lam_buf, double* __restrict__ grad,
+                                                          int npoles) {
+  int const r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= np) return;
+  double le[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) le[c] = lam_buf[(size_t)r * 6 + c];
+  for (int64_t seg = sb - 1; seg >= sa; --seg) {
+    auto row = [&](int v) { return ((seg - sa) * kSweeps + v) * (int64_t)np + r; };
+    double ls[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) ls[c] = lam_out[(size_t)row(0) * 6 + c];
+#pragma unroll
+    for (int v = 1; v < kSweeps; ++v)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) ls[c] = __builtin_fma(le[v - 1], lam_out[(size_t)row(v) * 6 + c], ls[c]);
+    auto drow = [&](int v, int p) { return (((seg - sa) * kSweeps + v) * (int64_t)npoles + p) * np + r; };
+    for (int p = 0; p < npoles; ++p) {
+      double gp = del_out[drow(0, p)];
+#pragma unroll
+      for (int v = 1; v < kSweeps; ++v) gp = __builtin_fma(le[v - 1], del_out[drow(v, p)], gp);
+      grad[(size_t)r * npoles + p] += gp;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) le[c] = ls[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) lam_buf[(size_t)r * 6 + c] = le[c];
 }
 
 }  // namespace
@@ -346,19 +409,46 @@ hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int 
   return hipGetLastError();
 }
 
-size_t adjoint_record_doubles(int every, int np) { return (size_t)every * kStepRec * np; }
+size_t adjoint_record_doubles(int64_t steps, int np) { return (size_t)steps * kStepRec * np; }
+size_t adjoint_sweep_doubles(int64_t segments, int np, int npoles) {
+  return (size_t)segments * kSweeps * np * (6 + npoles);
+}
 
 hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const double* poles, const BatchParticle* P,
-                              int np, Checkpoints const& ck, const double* gy, int64_t total, double* rec,
+                              int np, int64_t nf, int64_t nb, Checkpoints const& ck, int64_t seg_per_pass,
+                              const double* gy, int64_t total, double* rec, double* sweep, double* lam_buf,
                               double* grad, int npoles, hipStream_t s) {
-  dim3 const grid((np + 63) / 64);
-  if (nu == 1)
-    rk4_adjoint_kernel<1><<<grid, 64, 0, s>>>(U, nu, poled, poles, P, np, ck, gy, total, rec, grad, npoles);
-  else if (nu == 2)
-    rk4_adjoint_kernel<2><<<grid, 64, 0, s>>>(U, nu, poled, poles, P, np, ck, gy, total, rec, grad, npoles);
-  else
-    rk4_adjoint_kernel<0><<<grid, 64, 0, s>>>(U, nu, poled, poles, P, np, ck, gy, total, rec, grad, npoles);
-  return hipGetLastError();
+  int64_t const C = ck.every;
+  double* const lam_out = sweep;
+  double* const del_out = sweep + (size_t)seg_per_pass * kSweeps * np * 6;
+  for (int leg = 0; leg < 2; ++leg) {
+    int64_t const nleg = leg ? nb : nf;
+    if (nleg <= 0) continue;
+    int64_t const nseg = (nleg + C - 1) / C;
+    hipError_t e = launch_zero(lam_buf, (int64_t)np * 6, s);
+    if (e != hipSuccess) return e;
+    for (int64_t sb = nseg; sb > 0; sb -= seg_per_pass) {
+      int64_t const sa = sb - seg_per_pass > 0 ? sb - seg_per_pass : 0;
+      int64_t const lanes = (sb - sa) * np;
+      dim3 const grid((unsigned)((lanes + 63) / 64));
+      if (nu == 1) rk4_record_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, total, rec);
+      else if (nu == 2) rk4_record_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, total, rec);
+      else rk4_record_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, total, rec);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      e = launch_zero(del_out, (sb - sa) * kSweeps * np * npoles, s);
+      if (e != hipSuccess) return e;
+      int64_t const slanes = lanes * kSweeps;
+      rk4_segment_sweep_kernel<<<(unsigned)((slanes + 63) / 64), 64, 0, s>>>(U, poled, P, np, ck.every, leg, sa, sb,
+                                                                             rec, lam_out, del_out, npoles);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      rk4_combine_kernel<<<(np + 63) / 64, 64, 0, s>>>(np, sa, sb, lam_out, del_out, lam_buf, grad, npoles);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
 }
 
 }  // namespace srs

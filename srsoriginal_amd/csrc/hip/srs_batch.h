@@ -89,11 +89,18 @@ hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int 
                                int ns, double omega, double cw, double c0w, double c2, const double* field_pp,
                                const double* flux_pp, const double* target, const double* dloss, double* gy,
                                int64_t total, hipStream_t s);
-// doubles of scratch the RK4 adjoint needs for one checkpoint segment of np particles
-size_t adjoint_record_doubles(int every, int np);
-// grad[r*npoles + p] += dL/d delta_p (grad zeroed by the caller)
+// doubles of record scratch for `steps` recomputed RK4 steps of np particles
+size_t adjoint_record_doubles(int64_t steps, int np);
+// doubles of per-segment sweep results for `segments` segments
+size_t adjoint_sweep_doubles(int64_t segments, int np, int npoles);
+// grad[r*npoles + p] += dL/d delta_p (grad zeroed by the caller): both RK4 legs,
+// seg_per_pass checkpoint segments recomputed per pass into rec (adjoint_record_doubles(
+// min(seg_per_pass * ck.every, longest leg), np)), swept into sweep
+// (adjoint_sweep_doubles(seg_per_pass, np, npoles)); lam_buf holds np * 6 doubles.
+// All particles share nf, nb.
 hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const double* poles, const BatchParticle* P,
-                              int np, Checkpoints const& ck, const double* gy, int64_t total, double* rec,
+                              int np, int64_t nf, int64_t nb, Checkpoints const& ck, int64_t seg_per_pass,
+                              const double* gy, int64_t total, double* rec, double* sweep, double* lam_buf,
                               double* grad, int npoles, hipStream_t s);
 
 }  // namespace srs

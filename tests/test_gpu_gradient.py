@@ -69,7 +69,9 @@ def test_pole_gradient_backward_leg_and_checkpoint_intervals(grad, oracle):
         prob.checkpoint_every = every
         loss, _, tape = prob.forward(D)
         grads.append(prob.backward(tape, len(D)))
-    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+    # segments are swept independently and chained, so intervals differ only by rounding
+    for gk in grads[1:]:
+        assert np.max(np.abs(gk - grads[0])) <= 1e-12 * np.max(np.abs(grads[0]))
     for r in range(len(D)):
         F = oracle.FieldSet([], pole, D[r])
         lo, go = oracle.pole_loss_gradient(F, p, c0, c1, n, xyz, prob.energy_ev, prob.target, nthreads=8)
