@@ -198,23 +198,35 @@ __global__ void flux_reduce_kernel(const double* __restrict__ part, int nchunk, 
 }
 
 // Complex-field partials part[(c*6 + q)*npts + j] with a per-point kw (mixed spectrum).
-__global__ void field_reduce_kernel(const double* __restrict__ part, int nchunk, int npts, const double* __restrict__ kw,
-                                    double c2, double* __restrict__ out) {
-  int const j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= npts) return;
-  double v[6] = {0, 0, 0, 0, 0, 0};
-  for (int c = 0; c < nchunk; ++c)
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] += part[((size_t)c * 6 + k) * npts + j];
-  double const k2 = kw[j];
+// One lane per (point, component) sums the chunks in order (loads coalesce across the
+// points of a block); 42 points x 6 components per 256-lane block, combined in LDS.
+constexpr int kFieldReducePts = 42;
+
+__global__ __launch_bounds__(256) void field_reduce_kernel(const double* __restrict__ part, int nchunk, int npts,
+                                                           const double* __restrict__ kw, double c2,
+                                                           double* __restrict__ out) {
+  __shared__ double v[kFieldReducePts][6];
+  int const jl = threadIdx.x / 6, k = threadIdx.x % 6;
+  int const j = blockIdx.x * kFieldReducePts + jl;
+  if (jl < kFieldReducePts && j < npts) {
+    double acc = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < nchunk; ++c) acc += part[((size_t)c * 6 + k) * npts + j];
+    v[jl][k] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x >= kFieldReducePts) return;
+  int const jj = blockIdx.x * kFieldReducePts + threadIdx.x;
+  if (jj >= npts) return;
+  double const k2 = kw[jj];
   double p[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    double const re = -(v[2 * q + 1] * k2);
-    double const im = v[2 * q] * k2;
+    double const re = -(v[threadIdx.x][2 * q + 1] * k2);
+    double const im = v[threadIdx.x][2 * q] * k2;
     p[q] = re * re + im * im;
   }
-  out[j] = c2 * (p[0] + p[1] + p[2]);
+  out[jj] = c2 * (p[0] + p[1] + p[2]);
 }
 
 __global__ void spectrum_reduce_kernel(const double* __restrict__ part, int nchunk, int ne, const double* __restrict__ cw,
@@ -300,7 +312,7 @@ hipError_t launch_spectrum_reduce(const double* part, int nchunk, int ne, const 
 
 hipError_t launch_field_reduce(const double* part, int nchunk, int npts, const double* kw, double c2, double* out,
                                hipStream_t s) {
-  field_reduce_kernel<<<(npts + 255) / 256, 256, 0, s>>>(part, nchunk, npts, kw, c2, out);
+  field_reduce_kernel<<<(npts + kFieldReducePts - 1) / kFieldReducePts, 256, 0, s>>>(part, nchunk, npts, kw, c2, out);
   return hipGetLastError();
 }
 
