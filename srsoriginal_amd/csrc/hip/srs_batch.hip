@@ -143,13 +143,17 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
         for (int64_t base = 0; base < n; base += TILE_) {
           int const cnt = (n - base) < TILE_ ? (int)(n - base) : TILE_;
           __syncthreads();
-          for (int k = tid; k < cnt; k += kBlock) {
+          int const cntp = (cnt + kSub - 1) / kSub * kSub;
+          for (int k = tid; k < cntp; k += kBlock) {
             int64_t const i = q.off + base + k;
-            para_stage<TILE_, NE>(T.para, k, bt.x[i], bt.y[i], bt.z[i], bt.fbx[i], bt.fby[i], bt.fomb[i], base + k,
-                                  q.dt, pb, es);
+            if (k < cnt)
+              para_stage<TILE_, NE>(T.para, k, bt.x[i], bt.y[i], bt.z[i], bt.fbx[i], bt.fby[i], bt.fomb[i], base + k,
+                                    q.dt, pb, es);
+            else
+              para_pad<TILE_, NE>(T.para, k);
           }
           __syncthreads();
-          flux_terms_para<NE, TILE_>(T.para, cnt, ox, oy, es, cwf, a32);
+          flux_terms_para<NE, TILE_>(T.para, cntp, ox, oy, es, cwf, a32);
         }
       }
     } else {

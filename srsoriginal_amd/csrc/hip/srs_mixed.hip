@@ -65,13 +65,17 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
     for (int64_t base = i0; base < i1; base += TILE_) {
       int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
       __syncthreads();
-      for (int k = tid; k < cnt; k += kBlock) {
+      int const cntp = (cnt + kSub - 1) / kSub * kSub;
+      for (int k = tid; k < cntp; k += kBlock) {
         int64_t const i = base + k;
-        para_stage<TILE_, NE>(T.para, k, t.x[i], t.y[i], t.z[i], tf[i], tf[t.n + i], tf[2 * t.n + i], i, t.dt, pb,
-                              es);
+        if (k < cnt)
+          para_stage<TILE_, NE>(T.para, k, t.x[i], t.y[i], t.z[i], tf[i], tf[t.n + i], tf[2 * t.n + i], i, t.dt, pb,
+                                es);
+        else
+          para_pad<TILE_, NE>(T.para, k);
       }
       __syncthreads();
-      flux_terms_para<NE, TILE_>(T.para, cnt, ox, oy, es, cwf, a);
+      flux_terms_para<NE, TILE_>(T.para, cntp, ox, oy, es, cwf, a);
     }
   } else {
     for (int64_t base = i0; base < i1; base += TILE_) {

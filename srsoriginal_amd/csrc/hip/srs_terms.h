@@ -168,8 +168,8 @@ constexpr double kParaPhase = 1e-9;
 template <int NE>
 struct alignas(16) ParaPoint {
   double x, y;        // trajectory position
-  double h;           // 1 / (2 rz)
-  double t3;          // 1 / (8 rz^3)
+  double h;           // 1 / (2 rz)      (one energy: W / (2 rz), W folded in)
+  double t3;          // 1 / (8 rz^3)    (one energy: W / (8 rz^3))
   double phi[NE];     // Phi_k per energy, revolutions in [-0.5, 0.5]
   float bxr, byr;     // beta_x / rz, beta_y / rz
   float ombr;         // (1 - beta_z) / rz
@@ -260,8 +260,9 @@ __device__ __forceinline__ void para_stage(ParaTile<TILE, NE>& P, int k, double 
   ParaPoint<NE>& r = P.p[k];
   r.x = xk;
   r.y = yk;
-  r.h = 0.5 * ir;
-  r.t3 = 0.125 * ir * ir * ir;
+  double const fold = NE == 1 ? es.fw[0] : 1.0;
+  r.h = fold * (0.5 * ir);
+  r.t3 = fold * (0.125 * ir * ir * ir);
   r.bxr = fbx * irf;
   r.byr = fby * irf;
   r.ombr = fomb * irf;
@@ -284,32 +285,56 @@ __device__ __forceinline__ void para_stage(ParaTile<TILE, NE>& P, int k, double 
   }
 }
 
+// A padding record past the tile's last point: zero amplitudes, so its terms add
+// nothing and the 32-term sub-tile loop needs no remainder.
+template <int TILE, int NE>
+__device__ __forceinline__ void para_pad(ParaTile<TILE, NE>& P, int k) {
+  ParaPoint<NE>& r = P.p[k];
+  r.x = r.y = r.h = r.t3 = 0.0;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) r.phi[e] = 0.0;
+  r.bxr = r.byr = r.ombr = r.q = r.ir = r.gh = 0.f;
+}
+
 // Near-field sums (the -i (c/omega) n/D^2 part) are kept apart within a sub-tile and
-// folded in with c/omega at its end, so a term needs no per-energy scaling.
+// folded in with c/omega at its end, so a term needs no per-energy scaling.  cnt is a
+// multiple of kSub (the tile is padded with zero-amplitude records), so the sub-tile
+// loop has a fixed trip count.
 template <int NE, int TILE>
 __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int cnt, double ox, double oy,
                                                 EnergySet const& es, const float* cwf, AccMixed<NE>& a) {
   for (int k0 = 0; k0 < cnt; k0 += kSub) {
-    int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
     float sub[NE][6], gsub[NE][6];
 #pragma unroll
     for (int e = 0; e < NE; ++e)
 #pragma unroll
       for (int q = 0; q < 6; ++q) sub[e][q] = gsub[e][q] = 0.f;
-    for (int k = k0; k < k1; ++k) {
-      ParaPoint<NE> const& r = P.p[k];
+    // kept rolled: unrolling raises the VGPR count and the lost occupancy costs more
+    // than the loop overhead it saves (measured)
+#pragma unroll 1
+    for (int kk = 0; kk < kSub; ++kk) {
+      ParaPoint<NE> const& r = P.p[k0 + kk];
       double const rx = ox - r.x, ry = oy - r.y;
       double const s = __builtin_fma(rx, rx, ry * ry);
-      double const u = __builtin_fma(-(s * s), r.t3, s * r.h);  // D - rz
+      double const s2 = -(s * s);
       float const sf = (float)s, rxf = (float)rx, ryf = (float)ry;
       float const tx = rxf * r.q, ty = ryf * r.q;
       float const ax = r.bxr - tx, ay = r.byr - ty;
       float const az = __builtin_fmaf(sf, r.gh, -r.ombr);
       float const gx = tx * r.ir, gy = ty * r.ir, gz = r.q;
+      double u = 0.0;
+      if constexpr (NE > 1) u = __builtin_fma(s2, r.t3, s * r.h);  // D - rz
 #pragma unroll
       for (int e = 0; e < NE; ++e) {
-        double const p = __builtin_fma(es.fw[e], u, r.phi[e]);
-        float const ph = (float)(p - __builtin_rint(p));
+        // revolutions W (D - rz) + Phi_k, reduced to one turn
+        double p;
+        if constexpr (NE == 1) p = __builtin_fma(s, r.h, __builtin_fma(s2, r.t3, r.phi[0]));
+        else p = __builtin_fma(es.fw[e], u, r.phi[e]);
+        // v_fract_f64 is the faster reduction with several energies, p - rint(p) with one
+        // (measured); either is exact
+        float ph;
+        if constexpr (NE == 1) ph = (float)(p - __builtin_rint(p));
+        else ph = (float)__builtin_amdgcn_fract(p);
         float const sn = __builtin_amdgcn_sinf(ph), cn = __builtin_amdgcn_cosf(ph);
         sub[e][0] = __builtin_fmaf(ax, cn, sub[e][0]);
         sub[e][1] = __builtin_fmaf(ax, sn, sub[e][1]);
@@ -325,8 +350,8 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
         gsub[e][5] = __builtin_fmaf(gz, cn, gsub[e][5]);
       }
     }
-    // m = 1 - eps/2 at the sub-tile's last point
-    ParaPoint<NE> const& rl = P.p[k1 - 1];
+    // m = 1 - eps/2 at the sub-tile's first point (always a trajectory point)
+    ParaPoint<NE> const& rl = P.p[k0];
     double const rxl = ox - rl.x, ryl = oy - rl.y;
     float const m = __builtin_fmaf(-0.5f * (float)__builtin_fma(rxl, rxl, ryl * ryl), rl.q, 1.f);
 #pragma unroll
