@@ -71,6 +71,11 @@ struct Context {
   std::mutex mu;
   bool ready = false;
   hipStream_t stream = nullptr;
+  // second stream of the batched pipeline: the RK4 segments run there while the
+  // flux integrates the previous segment on `stream` (same default priority)
+  hipStream_t stream2 = nullptr;
+  static constexpr int kMaxSegments = 8;
+  hipEvent_t seg_ev[kMaxSegments + 1] = {};
   Buf traj, trajf, obs, nrm, energies, part, out;
   // optional HIP-event timing of the integral kernels (srs_hip_timing_*)
   bool timing = false;
@@ -78,6 +83,7 @@ struct Context {
   size_t ev_used = 0;
   // batched path
   Buf b_part, b_und, b_poles, b_traj, b_trajf, b_energies, b_flux_pp, b_out, b_counter, b_obs;
+  Buf b_state, b_field;  // RK4 state between segments; field sums accumulated over segments
   hipGraph_t b_graph = nullptr;
   hipGraphExec_t b_exec = nullptr;
   std::string b_key;
@@ -122,6 +128,8 @@ int init(Context& c) {
   if (device_count_raw() < 1) return fail(SRS_ERR_NO_DEVICE, "You are requesting the GPU, but none were found");
   HIPCHK(hipSetDevice(0));
   HIPCHK(hipStreamCreate(&c.stream));
+  HIPCHK(hipStreamCreate(&c.stream2));
+  for (auto& e : c.seg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   c.ready = true;
   return SRS_OK;
 }
@@ -410,32 +418,79 @@ struct BatchRun {
   int np, ns, ne;
   bool mixed;
   double weight;
+  int64_t max_leg = 0;                         // longest RK4 leg over the particles
+  srs::Checkpoints ck{nullptr, 1, 0};          // RK4 checkpoints (gradient forward)
+  double* field_out = nullptr;                 // keep the field sums here (gradient forward)
 };
 
-// Queue the whole pipeline (RK4, queue reset, persistent flux, particle reduction).
-int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, int np_run, hipStream_t s) {
+// Time segments of the pipeline: one for short trajectories, else up to kMaxSegments
+// of >= 2048 steps, so the device RK4 (latency-bound, a few waves) of segment s + 1
+// runs on stream2 while the flux integrates segment s on the main stream.
+int64_t segment_steps(int64_t max_leg, int* nseg) {
+  int n = max_leg <= 4096 ? 1 : (int)std::min<int64_t>(Context::kMaxSegments, (max_leg + 2047) / 2048);
+  int64_t const L = (max_leg + n - 1) / n;
+  *nseg = (int)((max_leg + L - 1) / L);
+  return L;
+}
+
+// Queue the whole pipeline: RK4 segments, persistent flux per segment, particle reduction.
+int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, int np_run, hipStream_t s,
+                  bool reduce = true) {
   int const ntiles = (r.ns + srs::kBlock - 1) / srs::kBlock;
   int const ngroups = (r.ne + cfg.ne_per_item - 1) / cfg.ne_per_item;
   unsigned const nitems = (unsigned)((int64_t)np_run * ntiles * ngroups);
-  int nblocks = c.cus * srs::batch_max_blocks(cfg, r.mixed);
+  int nseg = 1;
+  int64_t const L = segment_steps(r.max_leg, &nseg);
+  // leave room for the RK4 blocks next to the persistent grid
+  int const reserve = nseg > 1 ? (np_run + 63) / 64 + 1 : 0;
+  int nblocks = std::max(1, c.cus * srs::batch_max_blocks(cfg, r.mixed) - reserve);
   if ((unsigned)nblocks > nitems) nblocks = (int)nitems;
   auto* P = c.b_part.as<srs::BatchParticle>();
-  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt,
-                               srs::Checkpoints{nullptr, 1, 0}, s));
-  HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
-  {
-    KernelTimer timer(c, s);
-    HIPCHK(srs::launch_batch_flux(cfg, r.mixed, r.bt, P, c.b_obs.as<double>(), r.ns, r.be, nitems,
-                                  c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), nullptr, nblocks, s));
+  double* field = r.field_out;
+  if (nseg > 1) {
+    HIPCHK(c.b_state.ensure((size_t)np_run * 24 * sizeof(double)));
+    if (!field) {
+      HIPCHK(c.b_field.ensure((size_t)np_run * r.ne * 6 * r.ns * sizeof(double)));
+      field = c.b_field.as<double>();
+    }
   }
-  HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), np_run, r.ne * r.ns, r.weight, c.b_out.as<double>(), s));
+  srs::RK4Seg rs{0, L, nseg > 1 ? c.b_state.as<double>() : nullptr};
+  if (nseg == 1) {
+    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt,
+                                 r.ck, rs, s));
+    HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
+    srs::FluxSeg const fs{0, L, 1, 1, field != nullptr, field};
+    HIPCHK(srs::launch_batch_flux(cfg, r.mixed, r.bt, P, c.b_obs.as<double>(), r.ns, r.be, nitems,
+                                  c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), fs, nblocks, s));
+  } else {
+    HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
+    HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
+    for (int k = 0; k < nseg; ++k) {
+      rs.lo = k * L;
+      rs.hi = (k + 1) * L;
+      HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt,
+                                   r.ck, rs, c.stream2));
+      HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
+    }
+    for (int k = 0; k < nseg; ++k) {
+      HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
+      HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
+      srs::FluxSeg const fs{k * L, (k + 1) * L, k == 0, k == nseg - 1, 1, field};
+      HIPCHK(srs::launch_batch_flux(cfg, r.mixed, r.bt, P, c.b_obs.as<double>(), r.ns, r.be, nitems,
+                                    c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), fs, nblocks, s));
+    }
+  }
+  if (reduce)
+    HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), np_run, r.ne * r.ns, r.weight, c.b_out.as<double>(),
+                                       s));
   return SRS_OK;
 }
 
 std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context const& c) {
   char buf[512];
-  snprintf(buf, sizeof buf, "%d/%d/%d/%d/%d/%d/%p/%p/%p/%p/%p/%p/%.17g", r.np, r.ns, r.ne, (int)r.mixed, cfg.ne_per_item,
-           cfg.tile, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p, r.weight);
+  snprintf(buf, sizeof buf, "%d/%d/%d/%d/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%lld/%.17g", r.np, r.ns, r.ne, (int)r.mixed,
+           cfg.ne_per_item, cfg.tile, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
+           c.b_state.p, c.b_field.p, (long long)r.max_leg, r.weight);
   return buf;
 }
 
@@ -604,7 +659,8 @@ int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle*
   srs::BatchTraj bt;
   TRY(stage_batch_traj(c, tot, true, &bt));
   HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(),
-                               c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0}, s));
+                               c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0},
+                               srs::RK4Seg{0, INT64_MAX, nullptr}, s));
   double* src[9] = {bt.x, bt.y, bt.z, bt.bx, bt.by, bt.bz, bt.ax, bt.ay, bt.az};
   for (int k = 0; k < 9; ++k)
     HIPCHK(hipMemcpyAsync(traj_out + (size_t)k * tot, src[k], tot * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -666,6 +722,11 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
   r.ne = (int)ne;
   r.mixed = precision == SRS_PRECISION_MIXED;
   r.weight = weight;
+  for (auto const& q : P) r.max_leg = std::max(r.max_leg, std::max(q.nf, q.nb));
+  // the segmented pipeline's buffers are sized here: nothing may allocate while a
+  // graph is being captured
+  HIPCHK(c.b_state.ensure((size_t)np * 24 * sizeof(double)));
+  HIPCHK(c.b_field.ensure((size_t)np * ne * 6 * ns * sizeof(double)));
   HIPCHK(hipStreamSynchronize(s));  // host staging vectors
 
   // tile shape: default, or the autotuned one for this problem signature
@@ -736,6 +797,7 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
     KernelTimer timer(c, s);
     HIPCHK(hipGraphLaunch(c.b_exec, s));
   } else {
+    KernelTimer timer(c, s);
     TRY(enqueue_batch(c, r, cfg, r.np, s));
   }
   HIPCHK(hipMemcpyAsync(flux_out, c.b_out.p, (size_t)ne * ns * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -835,23 +897,21 @@ int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const sr
   HIPCHK(c.g_field.ensure((size_t)nreal * 6 * ns * sizeof(double)));
   HIPCHK(c.g_loss.ensure((size_t)nreal * sizeof(double)));
   HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
-  auto* Pd = c.b_part.as<srs::BatchParticle>();
   {
+    BatchRun r;
+    r.nund = nund;
+    r.bt = bt;
+    r.be = be;
+    r.np = (int)nreal;
+    r.ns = (int)ns;
+    r.ne = 1;
+    r.mixed = false;
+    r.weight = 1.0;
+    r.max_leg = std::max(P[0].nf, P[0].nb);
+    r.ck = ck;
+    r.field_out = c.g_field.as<double>();
     KernelTimer timer(c, s);
-    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(), Pd, (int)nreal, bt,
-                                 ck, s));
-  }
-  srs::BatchConfig const cfg{1, 256};
-  int64_t const ntiles = (ns + srs::kBlock - 1) / srs::kBlock;
-  unsigned const nitems = (unsigned)(nreal * ntiles);
-  int nblocks = c.cus * srs::batch_max_blocks(cfg, false);
-  if ((unsigned)nblocks > nitems) nblocks = (int)nitems;
-  HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
-  {
-    KernelTimer timer(c, s);
-    HIPCHK(srs::launch_batch_flux(cfg, false, bt, Pd, c.b_obs.as<double>(), (int)ns, be, nitems,
-                                  c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), c.g_field.as<double>(),
-                                  nblocks, s));
+    TRY(enqueue_batch(c, r, srs::BatchConfig{1, 256}, (int)nreal, s, false));
   }
   HIPCHK(srs::launch_loss(c.b_flux_pp.as<double>(), c.g_target.as<double>(), (int)ns, (int)nreal,
                           c.g_loss.as<double>(), s));

@@ -70,14 +70,30 @@ struct BatchConfig {
 
 bool batch_config_valid(BatchConfig const& c);
 int batch_max_blocks(BatchConfig const& c, bool mixed);
+// Steps [lo, hi) of both RK4 legs of every particle.  A segment with lo > 0 resumes
+// from state[(p*2 + leg)*12 ..] (x[6] then the lagged derivative d[6]) and every
+// segment leaves its end state there (state may be NULL for a single segment).
+struct RK4Seg {
+  int64_t lo, hi;
+  double* state;
+};
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
-                            BatchTraj const& bt, Checkpoints const& ck, hipStream_t s);
+                            BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s);
 hipError_t launch_zero_counter(unsigned* c, hipStream_t s);
-// field_pp (FP64 form only, may be NULL): the six field sums of each (particle,
-// energy, point), field_pp[((p*ne + e)*6 + q)*ns + j], q = re/im of x, y, z.
+// One time segment of the flux: steps [lo, hi) of both RK4 legs of every particle.
+// The six field sums of each (particle, energy, point) accumulate across segments in
+// field_acc[((p*ne + e)*6 + q)*ns + j] (q = re/im of x, y, z), in segment order; the
+// first segment starts from zero, the final one writes flux_pp.  With a single
+// segment (first and final) field_acc may be NULL, or is written when keep_field.
+struct FluxSeg {
+  int64_t lo, hi;
+  int first, final;
+  int keep_field;
+  double* field_acc;
+};
 hipError_t launch_batch_flux(BatchConfig const& cfg, bool mixed, BatchTraj const& bt, const BatchParticle* P,
                              const double* obs, int ns, BatchEnergies const& be, unsigned nitems, unsigned* counter,
-                             double* flux_pp, double* field_pp, int nblocks, hipStream_t s);
+                             double* flux_pp, FluxSeg const& sg, int nblocks, hipStream_t s);
 hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, double weight, double* out, hipStream_t s);
 
 // ---- adjoint of the flux loss (srs_adjoint.hip) ----------------------------

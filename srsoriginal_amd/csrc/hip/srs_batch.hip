@@ -25,7 +25,7 @@ template <int NU>
 __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu,
                                                         const double* __restrict__ poles,
                                                         const BatchParticle* __restrict__ Ps, int np, BatchTraj bt,
-                                                        Checkpoints ck) {
+                                                        Checkpoints ck, RK4Seg sg) {
   int const p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   BatchParticle const q = Ps[p];
@@ -38,31 +38,54 @@ __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __res
     x[3] = q.beta0[1] * kC;
     x[4] = q.x0[2];
     x[5] = q.beta0[2] * kC;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) d[c] = 0.0;
+  };
+  auto load = [&](int leg) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      x[c] = sg.state[((size_t)p * 2 + leg) * 12 + c];
+      d[c] = sg.state[((size_t)p * 2 + leg) * 12 + 6 + c];
+    }
+  };
+  auto keep = [&](int leg) {
+    if (!sg.state) return;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      sg.state[((size_t)p * 2 + leg) * 12 + c] = x[c];
+      sg.state[((size_t)p * 2 + leg) * 12 + 6 + c] = d[c];
+    }
   };
   auto save = [&](int64_t slot) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) ck.state[(p * ck.stride + slot) * 6 + c] = x[c];
   };
   int64_t const nf_slots = ck.state ? (q.nf + ck.every - 1) / ck.every : 0;
-  init();
-  {
+  if (sg.lo < q.nf) {
+    if (sg.lo == 0) init();
+    else load(0);
     double const h = q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
-    for (int64_t i = 0; i < q.nf; ++i) {
+    int64_t const i1 = sg.hi < q.nf ? sg.hi : q.nf;
+    for (int64_t i = sg.lo; i < i1; ++i) {
       if (ck.state && i % ck.every == 0) save(i / ck.every);
       store_point(bt, q.off + q.nb + i, x, d);
       R.derivs(x, d);
       R.step(x, d, h, hh, h6);
     }
+    keep(0);
   }
-  init();
-  {
+  if (sg.lo < q.nb) {
+    if (sg.lo == 0) init();
+    else load(1);
     double const h = -q.dt, hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
-    for (int64_t i = 0; i < q.nb; ++i) {
+    int64_t const i1 = sg.hi < q.nb ? sg.hi : q.nb;
+    for (int64_t i = sg.lo; i < i1; ++i) {
       if (ck.state && i % ck.every == 0) save(nf_slots + i / ck.every);
       R.derivs(x, d);
       R.step(x, d, h, hh, h6);
       store_point(bt, q.off + q.nb - 1 - i, x, d);
     }
+    keep(1);
   }
 }
 
@@ -87,11 +110,26 @@ struct BatchTile<TILE_, NE, true> {
 // NE energies), popped from a global counter by lane 0 and broadcast through LDS.
 // Every block exits when the counter passes the item count.  The mixed form takes
 // the paraxial terms for every trajectory tile that passes their bounds.
+// Trajectory points of one item within steps [lo, hi) of both RK4 legs, as local
+// index ranges: backward-leg steps i hold points nb - 1 - i, forward-leg steps i hold
+// points nb + i.
+struct ItemRanges {
+  int64_t r0[2], r1[2];
+  __device__ __forceinline__ ItemRanges(BatchParticle const& q, FluxSeg const& sg) {
+    int64_t const bl = q.nb < sg.lo ? q.nb : sg.lo, bh = q.nb < sg.hi ? q.nb : sg.hi;
+    int64_t const fl = q.nf < sg.lo ? q.nf : sg.lo, fh = q.nf < sg.hi ? q.nf : sg.hi;
+    r0[0] = q.nb - bh;
+    r1[0] = q.nb - bl;
+    r0[1] = q.nb + fl;
+    r1[1] = q.nb + fh;
+  }
+};
+
 template <int NE, int TILE_, bool MIXED>
 __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                             const double* __restrict__ obs, int ns, BatchEnergies be,
                                                             unsigned nitems, unsigned* __restrict__ counter,
-                                                            double* __restrict__ flux_pp, double* __restrict__ field_pp) {
+                                                            double* __restrict__ flux_pp, FluxSeg sg) {
   __shared__ BatchTile<TILE_, NE, MIXED> T;
   __shared__ double red[4 * (kBlock / 64)];
   __shared__ unsigned s_item;
@@ -109,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
     int const tile = (int)(rest % (unsigned)ntiles);
     int const p = (int)(rest / (unsigned)ntiles);
     BatchParticle const q = P[p];
-    int64_t const n = q.nb + q.nf;
+    ItemRanges const rg(q, sg);
     EnergySet es;
     es.ne = NE;
     float cwf[NE];
@@ -134,50 +172,55 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
     if constexpr (MIXED) {
       a32.zero();
       pb = para_block(ox, oy, oz, obs[2 * ns + j0], red);
-      para = para_check(pb, bt.x + q.off, bt.y + q.off, bt.z + q.off, 0, n, es);
+      bool const p0 = para_check(pb, bt.x + q.off, bt.y + q.off, bt.z + q.off, rg.r0[0], rg.r1[0], es);
+      bool const p1 = para_check(pb, bt.x + q.off, bt.y + q.off, bt.z + q.off, rg.r0[1], rg.r1[1], es);
+      para = p0 && p1;
     } else {
       a64.zero();
     }
-    if (para) {
-      if constexpr (MIXED) {
-        for (int64_t base = 0; base < n; base += TILE_) {
-          int const cnt = (n - base) < TILE_ ? (int)(n - base) : TILE_;
+    for (int part = 0; part < 2; ++part) {
+      int64_t const lo = rg.r0[part], hi = rg.r1[part];
+      if (para) {
+        if constexpr (MIXED) {
+          for (int64_t base = lo; base < hi; base += TILE_) {
+            int const cnt = (hi - base) < TILE_ ? (int)(hi - base) : TILE_;
+            __syncthreads();
+            int const cntp = (cnt + kSub - 1) / kSub * kSub;
+            for (int k = tid; k < cntp; k += kBlock) {
+              int64_t const i = q.off + base + k;
+              if (k < cnt)
+                para_stage<TILE_, NE>(T.para, k, bt.x[i], bt.y[i], bt.z[i], bt.fbx[i], bt.fby[i], bt.fomb[i],
+                                      base + k, q.dt, pb, es);
+              else
+                para_pad<TILE_, NE>(T.para, k);
+            }
+            __syncthreads();
+            flux_terms_para<NE, TILE_>(T.para, cntp, ox, oy, es, cwf, a32);
+          }
+        }
+      } else {
+        for (int64_t base = lo; base < hi; base += TILE_) {
+          int const cnt = (hi - base) < TILE_ ? (int)(hi - base) : TILE_;
           __syncthreads();
-          int const cntp = (cnt + kSub - 1) / kSub * kSub;
-          for (int k = tid; k < cntp; k += kBlock) {
+          for (int k = tid; k < cnt; k += kBlock) {
             int64_t const i = q.off + base + k;
-            if (k < cnt)
-              para_stage<TILE_, NE>(T.para, k, bt.x[i], bt.y[i], bt.z[i], bt.fbx[i], bt.fby[i], bt.fomb[i], base + k,
-                                    q.dt, pb, es);
-            else
-              para_pad<TILE_, NE>(T.para, k);
+            T.raw.x[k] = bt.x[i];
+            T.raw.y[k] = bt.y[i];
+            T.raw.z[k] = bt.z[i];
+            if (MIXED) {
+              T.raw.fbx[k] = bt.fbx[i];
+              T.raw.fby[k] = bt.fby[i];
+              T.raw.fomb[k] = bt.fomb[i];
+            } else {
+              T.raw.bx[k] = bt.bx[i];
+              T.raw.by[k] = bt.by[i];
+              T.raw.bz[k] = bt.bz[i];
+            }
           }
           __syncthreads();
-          flux_terms_para<NE, TILE_>(T.para, cntp, ox, oy, es, cwf, a32);
+          if (MIXED) flux_terms_mixed<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, cwf, a32);
+          else flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, a64);
         }
-      }
-    } else {
-      for (int64_t base = 0; base < n; base += TILE_) {
-        int const cnt = (n - base) < TILE_ ? (int)(n - base) : TILE_;
-        __syncthreads();
-        for (int k = tid; k < cnt; k += kBlock) {
-          int64_t const i = q.off + base + k;
-          T.raw.x[k] = bt.x[i];
-          T.raw.y[k] = bt.y[i];
-          T.raw.z[k] = bt.z[i];
-          if (MIXED) {
-            T.raw.fbx[k] = bt.fbx[i];
-            T.raw.fby[k] = bt.fby[i];
-            T.raw.fomb[k] = bt.fomb[i];
-          } else {
-            T.raw.bx[k] = bt.bx[i];
-            T.raw.by[k] = bt.by[i];
-            T.raw.bz[k] = bt.bz[i];
-          }
-        }
-        __syncthreads();
-        if (MIXED) flux_terms_mixed<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, cwf, a32);
-        else flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, a64);
       }
     }
     if (j < ns) {
@@ -185,23 +228,24 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
       for (int e = 0; e < NE; ++e) {
         int const ee = g * NE + e;
         if (ee >= be.ne) continue;
-        double f;
-        if (MIXED) {
-          f = flux_from_sums(a32.value(e, 0), a32.value(e, 1), a32.value(e, 2), a32.value(e, 3), a32.value(e, 4),
-                             a32.value(e, 5), es.kw[e], be.c2);
-        } else {
-          f = flux_from_sums(a64.re[e][0], a64.im[e][0], a64.re[e][1], a64.im[e][1], a64.re[e][2], a64.im[e][2],
-                             es.kw[e], be.c2);
-          if (field_pp) {
-            size_t const b = ((size_t)p * be.ne + ee) * 6 * ns + j;
+        // this segment's six field sums (+ the earlier segments', in segment order)
+        double S[6];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              field_pp[b + (size_t)(2 * c) * ns] = a64.re[e][c];
-              field_pp[b + (size_t)(2 * c + 1) * ns] = a64.im[e][c];
-            }
-          }
+        for (int c = 0; c < 3; ++c) {
+          S[2 * c] = MIXED ? a32.value(e, 2 * c) : a64.re[e][c];
+          S[2 * c + 1] = MIXED ? a32.value(e, 2 * c + 1) : a64.im[e][c];
         }
-        flux_pp[((size_t)p * be.ne + ee) * ns + j] = f;
+        size_t const b = ((size_t)p * be.ne + ee) * 6 * ns + j;
+        if (!sg.first) {
+#pragma unroll
+          for (int q6 = 0; q6 < 6; ++q6) S[q6] = sg.field_acc[b + (size_t)q6 * ns] + S[q6];
+        }
+        if (sg.field_acc && !(sg.first && sg.final && !sg.keep_field)) {
+#pragma unroll
+          for (int q6 = 0; q6 < 6; ++q6) sg.field_acc[b + (size_t)q6 * ns] = S[q6];
+        }
+        if (sg.final) flux_pp[((size_t)p * be.ne + ee) * ns + j] = flux_from_sums(S[0], S[1], S[2], S[3], S[4], S[5],
+                                                                                    es.kw[e], be.c2);
       }
     }
   }
@@ -224,19 +268,18 @@ __global__ void particle_reduce_kernel(const double* __restrict__ flux_pp, int n
 template <int NE, int TILE_, bool MIXED>
 hipError_t launch_batch_t(BatchTraj const& bt, const BatchParticle* P, const double* obs, int ns,
                           BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp,
-                          double* field_pp, int nblocks, hipStream_t s) {
-  batch_flux_kernel<NE, TILE_, MIXED><<<nblocks, kBlock, 0, s>>>(bt, P, obs, ns, be, nitems, counter, flux_pp,
-                                                                 field_pp);
+                          FluxSeg const& sg, int nblocks, hipStream_t s) {
+  batch_flux_kernel<NE, TILE_, MIXED><<<nblocks, kBlock, 0, s>>>(bt, P, obs, ns, be, nitems, counter, flux_pp, sg);
   return hipGetLastError();
 }
 
 template <bool MIXED>
 hipError_t launch_batch_m(BatchConfig const& cfg, BatchTraj const& bt, const BatchParticle* P, const double* obs, int ns,
                           BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp,
-                          double* field_pp, int nblocks, hipStream_t s) {
+                          FluxSeg const& sg, int nblocks, hipStream_t s) {
 #define SRS_BATCH_CASE(NE_, TL_)                 \
   if (cfg.ne_per_item == NE_ && cfg.tile == TL_) \
-    return launch_batch_t<NE_, TL_, MIXED>(bt, P, obs, ns, be, nitems, counter, flux_pp, field_pp, nblocks, s);
+    return launch_batch_t<NE_, TL_, MIXED>(bt, P, obs, ns, be, nitems, counter, flux_pp, sg, nblocks, s);
   SRS_BATCH_CASE(1, 64)
   SRS_BATCH_CASE(1, 128)
   SRS_BATCH_CASE(1, 256)
@@ -286,11 +329,11 @@ int batch_max_blocks(BatchConfig const& c, bool mixed) {
 }
 
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
-                            BatchTraj const& bt, Checkpoints const& ck, hipStream_t s) {
+                            BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s) {
   dim3 const grid((np + 63) / 64);
-  if (nu == 1) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck);
-  else if (nu == 2) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck);
-  else rk4_batch_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck);
+  if (nu == 1) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  else if (nu == 2) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  else rk4_batch_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
   return hipGetLastError();
 }
 
@@ -301,9 +344,9 @@ hipError_t launch_zero_counter(unsigned* c, hipStream_t s) {
 
 hipError_t launch_batch_flux(BatchConfig const& cfg, bool mixed, BatchTraj const& bt, const BatchParticle* P,
                              const double* obs, int ns, BatchEnergies const& be, unsigned nitems, unsigned* counter,
-                             double* flux_pp, double* field_pp, int nblocks, hipStream_t s) {
-  return mixed ? launch_batch_m<true>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, nullptr, nblocks, s)
-               : launch_batch_m<false>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, field_pp, nblocks, s);
+                             double* flux_pp, FluxSeg const& sg, int nblocks, hipStream_t s) {
+  return mixed ? launch_batch_m<true>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, sg, nblocks, s)
+               : launch_batch_m<false>(cfg, bt, P, obs, ns, be, nitems, counter, flux_pp, sg, nblocks, s);
 }
 
 hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, double weight, double* out, hipStream_t s) {
