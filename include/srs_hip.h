@@ -169,6 +169,18 @@ int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const sr
                          const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
                          double energy_ev, const double* target, int checkpoint_every, double* flux_out,
                          double* loss_out, int64_t* tape_out);
+/* The same for the power density (OSCARSSR.cc:1477-1598: W/mm^2 on ns points with
+ * surface normals, AoS xyz; directional as in srs_hip_power_density):
+ *   L_r = sum_j (P_rj - target_j)^2,  P_rj = power density of realisation r at point j;
+ * pd_out[r*ns + j] = P_rj when not NULL.  The stored acceleration A/c of each
+ * trajectory point (lagged one step, as the reference stores it) is differentiated
+ * through the RK4 step that produced it.  No counterpart in the reference. */
+int srs_hip_pole_pd_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle,
+                            int64_t nreal, const double* deltas, double charge, double current,
+                            const double* obs_xyz, const double* normals_xyz, int64_t ns, int directional,
+                            const double* target, int checkpoint_every, double* pd_out, double* loss_out,
+                            int64_t* tape_out);
+/* Backward of the last srs_hip_pole_forward or srs_hip_pole_pd_forward. */
 int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out);
 
 #ifdef __cplusplus

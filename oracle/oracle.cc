@@ -20,6 +20,8 @@
 //   random particles ..... bitwise vs reference (seeded mt19937/normal stream)
 //   oracle_pole_loss_gradient  dual-number forward mode; vs the reference's own central
 //                          differences (h = 1e-6) of the pole-error flux loss
+//   oracle_pole_pd_loss_gradient  the same for the power-density loss; vs the
+//                          reference's power-density map and its central differences
 
 #include "oracle.h"
 
@@ -355,9 +357,13 @@ void rk4_dual(const oracle_fields* f, const Dual* ps, double qmg, Dual* y, const
   for (int i = 0; i < 6; ++i) y[i] = y[i] + h6 * (dydx[i] + dyt[i] + cst(2.0) * dym[i]);
 }
 
-// Loss and its derivative along pole direction m (m < 0: no direction).
-Dual pole_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
-                    int64_t npoints, const double* obs, int64_t ns, double energy_ev, const double* target, int m) {
+// Trajectory of the pole-scaled field in dual numbers, with pole direction m
+// (m < 0: none): tr[c*n + k] for c = x, y, z, beta x, y, z, A/c x, y, z.  Point layout
+// and the lagged A/c as CalculateTrajectory (OSCARSSR.cc:600-667): the forward leg
+// stores each point with the derivative of the previous step (0 at T0), the backward
+// leg stores each point with the derivative of the step that led to it.
+void pole_trajectory_dual(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                          int64_t npoints, int m, std::vector<Dual>& tr, int64_t* n_out, double* dt_out) {
   int const npoles = 2 * f->pole_und->nperiods;
   std::vector<Dual> ps((size_t)npoles);
   for (int i = 0; i < npoles; ++i) ps[i] = Dual{f->pole_scales[i], i == m ? 1.0 : 0.0};
@@ -365,7 +371,7 @@ Dual pole_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_
   size_t const nf = 1 + (ct_stop - p->t0) / kC / dt;
   size_t const nb = (p->t0 - ct_start) / kC / dt;
   int64_t const n = (int64_t)(nf + nb);
-  std::vector<Dual> tr((size_t)6 * n);  // x, y, z, bx, by, bz
+  tr.assign((size_t)9 * n, cst(0));
   Dual x[6], d[6];
   auto init = [&]() {
     x[0] = cst(p->x0[0]);
@@ -374,6 +380,7 @@ Dual pole_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_
     x[3] = cst(p->beta0[1] * kC);
     x[4] = cst(p->x0[2]);
     x[5] = cst(p->beta0[2] * kC);
+    for (int c = 0; c < 6; ++c) d[c] = cst(0);
   };
   auto put = [&](int64_t k) {
     tr[0 * n + k] = x[0];
@@ -382,6 +389,9 @@ Dual pole_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_
     tr[3 * n + k] = x[1] / cst(kC);
     tr[4 * n + k] = x[3] / cst(kC);
     tr[5 * n + k] = x[5] / cst(kC);
+    tr[6 * n + k] = d[1] / cst(kC);
+    tr[7 * n + k] = d[3] / cst(kC);
+    tr[8 * n + k] = d[5] / cst(kC);
   };
   init();
   for (size_t i = 0; i != nf; ++i) {
@@ -395,6 +405,17 @@ Dual pole_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_
     rk4_dual(f, ps.data(), p->qmg, x, d, -dt);
     put((int64_t)(nb - 1 - i));
   }
+  *n_out = n;
+  *dt_out = dt;
+}
+
+// Flux loss and its derivative along pole direction m (m < 0: no direction).
+Dual pole_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                    int64_t npoints, const double* obs, int64_t ns, double energy_ev, const double* target, int m) {
+  std::vector<Dual> tr;
+  int64_t n;
+  double dt;
+  pole_trajectory_dual(f, p, ct_start, ct_stop, npoints, m, tr, &n, &dt);
   double const omega = ev_to_omega(energy_ev);
   double const q = p->q, cur = p->current;
   double const C0 = q / (kFourPi * kC * kEps0 * kSqrt2Pi);
@@ -421,6 +442,48 @@ Dual pole_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_
     for (int a = 0; a < 3; ++a) F = F + cst(kw * kw) * (sr[a] * sr[a] + si[a] * si[a]);
     F = cst(C2) * F;
     Dual const e = F - cst(target[j]);
+    L = L + e * e;
+  }
+  return L;
+}
+
+// Power-density loss L = sum_j (P_j - target_j)^2 and its derivative along pole
+// direction m.  P_j is the integrand of OSCARSSR.cc:1522-1573 with
+// (Num.N2)^2 + (Num.N3)^2 taken as |Num|^2 (N2, N3 orthonormal and orthogonal to
+// N1, Num orthogonal to N1), the skip of points behind the surface when directional,
+// the prefactor |qI|/(16 pi^2 eps0 c) dt / 1e6 and |P| when not directional.
+Dual pole_pd_loss_dual(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                       int64_t npoints, const double* obs, const double* nrm, int64_t ns, bool directional,
+                       const double* target, int m, double* pd_out) {
+  std::vector<Dual> tr;
+  int64_t n;
+  double dt;
+  pole_trajectory_dual(f, p, ct_start, ct_stop, npoints, m, tr, &n, &dt);
+  double const scale = std::fabs(p->q * p->current) / (16 * kPi2 * kEps0 * kC) * dt;
+  Dual L = cst(0);
+  for (int64_t j = 0; j < ns; ++j) {
+    Dual sum = cst(0);
+    for (int64_t k = 0; k < n; ++k) {
+      Dual const R[3] = {cst(obs[3 * j]) - tr[k], cst(obs[3 * j + 1]) - tr[n + k], cst(obs[3 * j + 2]) - tr[2 * n + k]};
+      Dual const d2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+      Dual const D = dsqrt(d2);
+      Dual const N[3] = {R[0] / D, R[1] / D, R[2] / D};
+      Dual const nd = N[0] * cst(nrm[3 * j]) + N[1] * cst(nrm[3 * j + 1]) + N[2] * cst(nrm[3 * j + 2]);
+      if (directional && nd.v <= 0) continue;
+      Dual const B[3] = {tr[3 * n + k], tr[4 * n + k], tr[5 * n + k]};
+      Dual const A[3] = {tr[6 * n + k], tr[7 * n + k], tr[8 * n + k]};
+      Dual const u[3] = {N[0] - B[0], N[1] - B[1], N[2] - B[2]};
+      Dual const v[3] = {u[1] * A[2] - u[2] * A[1], u[2] * A[0] - u[0] * A[2], u[0] * A[1] - u[1] * A[0]};
+      Dual const w[3] = {N[1] * v[2] - N[2] * v[1], N[2] * v[0] - N[0] * v[2], N[0] * v[1] - N[1] * v[0]};
+      Dual const num2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+      Dual const b = cst(1) - (B[0] * N[0] + B[1] * N[1] + B[2] * N[2]);
+      Dual const b2 = b * b;
+      sum = sum + num2 * nd / (b2 * b2 * b * d2);
+    }
+    Dual P = cst(scale) * sum / cst(1e6);
+    if (!directional && P.v < 0) P = -P;
+    if (pd_out) pd_out[j] = P.v;
+    Dual const e = P - cst(target[j]);
     L = L + e * e;
   }
   return L;
@@ -676,6 +739,21 @@ int oracle_pole_loss_gradient(const oracle_fields* f, const oracle_particle* p, 
   *loss_out = pole_loss_dual(f, p, ct_start, ct_stop, npoints, obs, ns, energy_ev, target, -1).v;
   per_point_threads(npoles, nthreads, [&](int64_t m) {
     grad_out[m] = pole_loss_dual(f, p, ct_start, ct_stop, npoints, obs, ns, energy_ev, target, (int)m).d;
+  });
+  return 0;
+}
+
+int oracle_pole_pd_loss_gradient(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                                 int64_t npoints, const double* obs, const double* normals, int64_t ns,
+                                 int directional, const double* target, double* pd_out, double* loss_out,
+                                 double* grad_out, int nthreads) {
+  if (!f || !f->pole_und || !f->pole_scales || f->pole_und->nperiods < 1) return -1;
+  int const npoles = 2 * f->pole_und->nperiods;
+  *loss_out =
+      pole_pd_loss_dual(f, p, ct_start, ct_stop, npoints, obs, normals, ns, directional != 0, target, -1, pd_out).v;
+  per_point_threads(npoles, nthreads, [&](int64_t m) {
+    grad_out[m] = pole_pd_loss_dual(f, p, ct_start, ct_stop, npoints, obs, normals, ns, directional != 0, target,
+                                    (int)m, nullptr).d;
   });
   return 0;
 }

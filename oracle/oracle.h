@@ -96,6 +96,13 @@ int oracle_pole_loss_gradient(const oracle_fields* f, const oracle_particle* p, 
                               int64_t npoints, const double* obs, int64_t ns, double energy_ev, const double* target,
                               double* loss_out, double* grad_out, int nthreads);
 
+/* The same for L = sum_j (P_j - target_j)^2 of the power density (W/mm^2) on ns points
+ * with normals; pd_out[j] = P_j (may be NULL). */
+int oracle_pole_pd_loss_gradient(const oracle_fields* f, const oracle_particle* p, double ct_start, double ct_stop,
+                                 int64_t npoints, const double* obs, const double* normals, int64_t ns,
+                                 int directional, const double* target, double* pd_out, double* loss_out,
+                                 double* grad_out, int nthreads);
+
 /* Kahan add of TSpectrumContainer::AddToFlux / T3DScalarContainer::AddToPoint. */
 void oracle_kahan_add(double* sum, double* comp, const double* v, int64_t n);
 

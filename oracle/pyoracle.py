@@ -175,3 +175,22 @@ def pole_loss_gradient(fields, particle, ct_start, ct_stop, npoints, obs, energy
     if rc != 0:
         raise ValueError("no pole-scaled undulator")
     return loss.value, grad
+
+
+def pole_pd_loss_gradient(fields, particle, ct_start, ct_stop, npoints, obs, normals, target, directional=False,
+                          nthreads=1):
+    """(L, dL/d delta, P) of L = sum_j (P_j - target_j)^2 for the power density P on the
+    points obs with normals (forward-mode dual numbers, FP64)."""
+    obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    grad = np.zeros(len(fields._scales))
+    pd = np.zeros(len(obs))
+    loss = D()
+    rc = lib().oracle_pole_pd_loss_gradient(C.byref(fields.s), C.byref(particle), D(ct_start), D(ct_stop),
+                                            I64(npoints), _p(obs), _p(nrm), I64(len(obs)),
+                                            C.c_int(1 if directional else 0), _p(target), _p(pd), C.byref(loss),
+                                            _p(grad), C.c_int(nthreads))
+    if rc != 0:
+        raise ValueError("no pole-scaled undulator")
+    return loss.value, grad, pd

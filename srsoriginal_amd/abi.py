@@ -185,9 +185,11 @@ def _bind_batch():
     L.srs_hip_batch_config.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
                                        _D, _P, _I64, _D, _P, C.c_int, _P, _P, C.POINTER(_I64)]
+    L.srs_hip_pole_pd_forward.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P,
+                                          _D, _D, _P, _P, _I64, C.c_int, _P, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_backward.argtypes = [_I64, _P, _P]
     for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config",
-              "srs_hip_pole_forward", "srs_hip_pole_backward"):
+              "srs_hip_pole_forward", "srs_hip_pole_pd_forward", "srs_hip_pole_backward"):
         getattr(L, n).restype = C.c_int
     L._batch_bound = True
     return L
@@ -277,6 +279,31 @@ def pole_forward(und_specs, poled, particle, deltas, charge, current, obs_xyz, e
                                   int(checkpoint_every), flux.ctypes.data_as(_P) if want_flux else None,
                                   loss.ctypes.data_as(_P), C.byref(tape)))
     return loss, flux, tape.value
+
+
+def pole_pd_forward(und_specs, poled, particle, deltas, charge, current, obs_xyz, normals_xyz, target, directional=False,
+                    checkpoint_every=1000, want_map=True):
+    """Forward of the pole-error power-density loss for nreal realisations (deltas
+    [nreal, 2N]).  Returns (loss [nreal], power density [nreal, ns] or None, tape id)."""
+    L = _bind_batch()
+    keep = []
+    U = undulators(und_specs, keep)
+    d = _f64(np.atleast_2d(deltas))
+    o = _f64(obs_xyz).reshape(-1, 3)
+    nrm = _f64(normals_xyz).reshape(-1, 3)
+    t = _f64(target)
+    if len(t) != len(o) or len(nrm) != len(o):
+        raise SrsError("target and normals need one value per observation point")
+    nreal = d.shape[0]
+    loss = np.zeros(nreal)
+    pd = np.zeros((nreal, len(o))) if want_map else None
+    tape = _I64()
+    _check(L.srs_hip_pole_pd_forward(U, len(U), int(poled), C.byref(particle), nreal, d.ctypes.data_as(_P), charge,
+                                     current, o.ctypes.data_as(_P), nrm.ctypes.data_as(_P), len(o),
+                                     1 if directional else 0, t.ctypes.data_as(_P), int(checkpoint_every),
+                                     pd.ctypes.data_as(_P) if want_map else None, loss.ctypes.data_as(_P),
+                                     C.byref(tape)))
+    return loss, pd, tape.value
 
 
 def pole_backward(tape, nreal, npoles, dloss=None):

@@ -91,7 +91,7 @@ struct Context {
   srs::BatchConfig b_last{1, 128};
   int cus = 0;
   // pole-error gradient: the forward call leaves its tape here for the backward call
-  Buf g_ckpt, g_field, g_target, g_loss, g_dloss, g_gy, g_rec, g_grad, g_lam, g_sweep;
+  Buf g_ckpt, g_field, g_target, g_loss, g_dloss, g_gy, g_rec, g_grad, g_lam, g_sweep, g_nrm, g_part;
   struct Tape {
     int64_t id = 0;
     bool valid = false;
@@ -101,7 +101,10 @@ struct Context {
     int64_t total = 0, nmax = 0;
     srs::BatchTraj bt{};
     srs::Checkpoints ck{};
-    double omega = 0, cw = 0, c0w = 0, c2 = 0;
+    double omega = 0, cw = 0, c0w = 0, c2 = 0;  // flux
+    int quantity = 0;                             // 0 flux, 1 power density
+    int directional = 0;
+    double pd_scale = 0;
   } tape;
   int64_t tape_counter = 0;
 };
@@ -494,6 +497,117 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
   return buf;
 }
 
+// ---- pole-error gradient: shared parts of the forward entry points ----------------
+int pole_validate(const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+                  const double* deltas, const double* obs_xyz, int64_t ns, const double* target, int checkpoint_every,
+                  double* loss_out, int64_t* tape_out) {
+  TRY(check_common(ns, SRS_PRECISION_FP64));
+  if (!und || nund < 1) return fail(SRS_ERR_INVALID, "no magnetic field (undulator) given");
+  if (poled < 0 || poled >= nund) return fail(SRS_ERR_INVALID, "poled undulator index out of range");
+  if (und[poled].nperiods < 1) return fail(SRS_ERR_INVALID, "the pole-scaled undulator needs nperiods >= 1");
+  for (int u = 0; u < nund; ++u)
+    if (u != poled && und[u].pole_delta)
+      return fail(SRS_ERR_INVALID, "only the differentiated undulator may carry pole errors");
+  if (!particle) return fail(SRS_ERR_INVALID, "no particle");
+  if (nreal < 1 || nreal > 65536) return fail(SRS_ERR_INVALID, "number of realisations must be in 1..65536");
+  if (!deltas || !target || !loss_out || !tape_out || !obs_xyz) return fail(SRS_ERR_INVALID, "NULL argument");
+  if (checkpoint_every < 1) return fail(SRS_ERR_INVALID, "checkpoint interval must be >= 1");
+  // all undulators must share one period direction (the adjoint projects dB/dX on it)
+  double u0[3];
+  for (int u = 0; u < nund; ++u) {
+    double const* pr = und[u].period;
+    double const pm = std::sqrt(pr[0] * pr[0] + pr[1] * pr[1] + pr[2] * pr[2]);
+    if (!(pm > 0)) return fail(SRS_ERR_INVALID, "undulator period must be a nonzero vector");
+    for (int k = 0; k < 3; ++k) {
+      double const uk = pr[k] / pm;
+      if (u == 0) u0[k] = uk;
+      else if (uk != u0[k]) return fail(SRS_ERR_INVALID, "the gradient needs undulators with one period direction");
+    }
+  }
+  return SRS_OK;
+}
+
+struct PoleRun {
+  std::vector<srs::BatchParticle> P;
+  int64_t total = 0, nmax = 0, stride = 0;
+  int npoles = 0, every = 1;
+  srs::BatchTraj bt{};
+  srs::Checkpoints ck{};
+};
+
+// Realisations (copies of the particle, pole_base = r * npoles), fields, deltas,
+// observation points, target, trajectory buffers (with accelerations when with_a)
+// and checkpoints, all queued on s.  Invalidates the tape.
+int pole_stage(Context& c, const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+               const double* deltas, const double* obs_xyz, int64_t ns, const double* target, int every, bool with_a,
+               PoleRun& pr, hipStream_t s) {
+  pr.npoles = 2 * und[poled].nperiods;
+  pr.every = every;
+  {
+    std::vector<srs_particle> rows((size_t)nreal, *particle);
+    TRY(build_particles(rows.data(), nreal, pr.P, &pr.total));
+    for (int64_t r = 0; r < nreal; ++r) pr.P[r].pole_base = r * pr.npoles;
+  }
+  pr.nmax = pr.P[0].nb + pr.P[0].nf;
+  pr.stride = (pr.P[0].nf + every - 1) / every + (pr.P[0].nb + every - 1) / every;
+  c.tape.valid = false;
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  {
+    std::vector<srs_undulator> uu(und, und + nund);
+    uu[poled].pole_delta = deltas;  // pole_off 0 into the realisations' delta array
+    TRY(stage_undulators(c, uu.data(), nund, s));
+  }
+  size_t const nd = (size_t)nreal * pr.npoles;
+  HIPCHK(c.b_poles.ensure(nd * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_poles.p, deltas, nd * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c.b_part.ensure(nreal * sizeof(srs::BatchParticle)));
+  HIPCHK(hipMemcpyAsync(c.b_part.p, pr.P.data(), nreal * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
+  TRY(stage_points(c, c.b_obs, obs_xyz, ns));
+  HIPCHK(c.g_target.ensure(ns * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.g_target.p, target, ns * sizeof(double), hipMemcpyHostToDevice, s));
+  TRY(stage_batch_traj(c, pr.total, with_a, &pr.bt));
+  HIPCHK(c.g_ckpt.ensure((size_t)nreal * pr.stride * 6 * sizeof(double)));
+  pr.ck = srs::Checkpoints{c.g_ckpt.as<double>(), every, pr.stride};
+  HIPCHK(c.b_flux_pp.ensure((size_t)nreal * ns * sizeof(double)));
+  HIPCHK(c.g_loss.ensure((size_t)nreal * sizeof(double)));
+  return SRS_OK;
+}
+
+// Loss of every realisation from the per-realisation maps in b_flux_pp; copies out
+// the losses (and the maps when map_out != NULL) and waits.
+int pole_finish(Context& c, PoleRun const&, int nreal, int ns, double* map_out, double* loss_out, hipStream_t s) {
+  HIPCHK(srs::launch_loss(c.b_flux_pp.as<double>(), c.g_target.as<double>(), ns, nreal, c.g_loss.as<double>(), s));
+  HIPCHK(hipMemcpyAsync(loss_out, c.g_loss.p, nreal * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (map_out)
+    HIPCHK(hipMemcpyAsync(map_out, c.b_flux_pp.p, (size_t)nreal * ns * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SRS_OK;
+}
+
+Context::Tape& pole_tape(Context& c, PoleRun const& pr, int nreal, int ns, int nund, int poled) {
+  Context::Tape& t = c.tape;
+  t.id = ++c.tape_counter;
+  t.np = nreal;
+  t.ns = ns;
+  t.nund = nund;
+  t.poled = poled;
+  t.npoles = pr.npoles;
+  t.every = pr.every;
+  t.nf = pr.P[0].nf;
+  t.nb = pr.P[0].nb;
+  {
+    // as many segments per pass as a 4 GiB record budget holds (all of them at C4 sizes)
+    int64_t const nseg = (std::max(t.nf, t.nb) + t.every - 1) / t.every;
+    double const per_seg = (double)srs::adjoint_record_doubles(t.every, nreal) * sizeof(double);
+    t.seg_per_pass = std::max<int64_t>(1, std::min<int64_t>(nseg, (int64_t)(4.0 * (1 << 30) / per_seg)));
+  }
+  t.total = pr.total;
+  t.nmax = pr.nmax;
+  t.bt = pr.bt;
+  t.ck = pr.ck;
+  t.valid = true;
+  return t;
+}
 
 }  // namespace
 
@@ -829,121 +943,85 @@ int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const sr
                          double energy_ev, const double* target, int checkpoint_every, double* flux_out,
                          double* loss_out, int64_t* tape_out) {
   g_err.clear();
-  TRY(check_common(ns, SRS_PRECISION_FP64));
-  if (!und || nund < 1) return fail(SRS_ERR_INVALID, "no magnetic field (undulator) given");
-  if (poled < 0 || poled >= nund) return fail(SRS_ERR_INVALID, "poled undulator index out of range");
-  if (und[poled].nperiods < 1) return fail(SRS_ERR_INVALID, "the pole-scaled undulator needs nperiods >= 1");
-  for (int u = 0; u < nund; ++u)
-    if (u != poled && und[u].pole_delta)
-      return fail(SRS_ERR_INVALID, "only the differentiated undulator may carry pole errors");
-  if (!particle) return fail(SRS_ERR_INVALID, "no particle");
-  if (nreal < 1 || nreal > 65536) return fail(SRS_ERR_INVALID, "number of realisations must be in 1..65536");
-  if (!deltas || !target || !loss_out || !tape_out || !obs_xyz) return fail(SRS_ERR_INVALID, "NULL argument");
-  if (checkpoint_every < 1) return fail(SRS_ERR_INVALID, "checkpoint interval must be >= 1");
+  TRY(pole_validate(und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, loss_out,
+                    tape_out));
   if (!(energy_ev > 0)) return fail(SRS_ERR_INVALID, "photon energy must be > 0");
-  // all undulators must share one period direction (the adjoint projects dB/dX on it)
-  double u0[3];
-  for (int u = 0; u < nund; ++u) {
-    double const* pr = und[u].period;
-    double const pm = std::sqrt(pr[0] * pr[0] + pr[1] * pr[1] + pr[2] * pr[2]);
-    if (!(pm > 0)) return fail(SRS_ERR_INVALID, "undulator period must be a nonzero vector");
-    for (int k = 0; k < 3; ++k) {
-      double const uk = pr[k] / pm;
-      if (u == 0) u0[k] = uk;
-      else if (uk != u0[k]) return fail(SRS_ERR_INVALID, "the gradient needs undulators with one period direction");
-    }
-  }
-  int const npoles = 2 * und[poled].nperiods;
-  std::vector<srs::BatchParticle> P;
-  int64_t total = 0;
-  {
-    std::vector<srs_particle> rows((size_t)nreal, *particle);
-    TRY(build_particles(rows.data(), nreal, P, &total));
-    for (int64_t r = 0; r < nreal; ++r) P[r].pole_base = r * npoles;
-  }
-  int64_t const nmax = P[0].nb + P[0].nf;
-  int const every = checkpoint_every;
-  int64_t const stride = (P[0].nf + every - 1) / every + (P[0].nb + every - 1) / every;
-
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
   TRY(init(c));
-  c.tape.valid = false;
   hipStream_t s = c.stream;
-  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
-  {
-    std::vector<srs_undulator> uu(und, und + nund);
-    uu[poled].pole_delta = deltas;  // pole_off 0 into the realisations' delta array
-    TRY(stage_undulators(c, uu.data(), nund, s));
-  }
-  HIPCHK(c.b_poles.ensure((size_t)nreal * npoles * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(c.b_poles.p, deltas, (size_t)nreal * npoles * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(c.b_part.ensure(nreal * sizeof(srs::BatchParticle)));
-  HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), nreal * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
-  TRY(stage_points(c, c.b_obs, obs_xyz, ns));
-  HIPCHK(c.g_target.ensure(ns * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(c.g_target.p, target, ns * sizeof(double), hipMemcpyHostToDevice, s));
+  PoleRun pr;
+  TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, false, pr, s));
   double const w = omega_of(energy_ev), c0 = c0_of(charge);
   double const eh[5] = {w, kC / w, c0 * w, w / kTwoPi, (w / kTwoPi) / kC};
   HIPCHK(c.b_energies.ensure(5 * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c.b_energies.p, eh, 5 * sizeof(double), hipMemcpyHostToDevice, s));
   double* ed = c.b_energies.as<double>();
   srs::BatchEnergies const be{ed, ed + 1, ed + 2, ed + 3, ed + 4, 1, c2_of(charge, current)};
-  srs::BatchTraj bt;
-  TRY(stage_batch_traj(c, total, false, &bt));
-  HIPCHK(c.g_ckpt.ensure((size_t)nreal * stride * 6 * sizeof(double)));
-  srs::Checkpoints const ck{c.g_ckpt.as<double>(), every, stride};
-  HIPCHK(c.b_flux_pp.ensure((size_t)nreal * ns * sizeof(double)));
   HIPCHK(c.g_field.ensure((size_t)nreal * 6 * ns * sizeof(double)));
-  HIPCHK(c.g_loss.ensure((size_t)nreal * sizeof(double)));
   HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
   {
     BatchRun r;
     r.nund = nund;
-    r.bt = bt;
+    r.bt = pr.bt;
     r.be = be;
     r.np = (int)nreal;
     r.ns = (int)ns;
     r.ne = 1;
     r.mixed = false;
     r.weight = 1.0;
-    r.max_leg = std::max(P[0].nf, P[0].nb);
-    r.ck = ck;
+    r.max_leg = std::max(pr.P[0].nf, pr.P[0].nb);
+    r.ck = pr.ck;
     r.field_out = c.g_field.as<double>();
     KernelTimer timer(c, s);
     TRY(enqueue_batch(c, r, srs::BatchConfig{1, 256}, (int)nreal, s, false));
   }
-  HIPCHK(srs::launch_loss(c.b_flux_pp.as<double>(), c.g_target.as<double>(), (int)ns, (int)nreal,
-                          c.g_loss.as<double>(), s));
-  HIPCHK(hipMemcpyAsync(loss_out, c.g_loss.p, nreal * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (flux_out)
-    HIPCHK(hipMemcpyAsync(flux_out, c.b_flux_pp.p, (size_t)nreal * ns * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  Context::Tape& t = c.tape;
-  t.id = ++c.tape_counter;
-  t.np = (int)nreal;
-  t.ns = (int)ns;
-  t.nund = nund;
-  t.poled = poled;
-  t.npoles = npoles;
-  t.every = every;
-  t.nf = P[0].nf;
-  t.nb = P[0].nb;
-  {
-    // as many segments per pass as a 4 GiB record budget holds (all of them at C4 sizes)
-    int64_t const nseg = (std::max(t.nf, t.nb) + every - 1) / every;
-    double const per_seg = (double)srs::adjoint_record_doubles(every, (int)nreal) * sizeof(double);
-    t.seg_per_pass = std::max<int64_t>(1, std::min<int64_t>(nseg, (int64_t)(4.0 * (1 << 30) / per_seg)));
-  }
-  t.total = total;
-  t.nmax = nmax;
-  t.bt = bt;
-  t.ck = ck;
+  TRY(pole_finish(c, pr, (int)nreal, (int)ns, flux_out, loss_out, s));
+  Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);
+  t.quantity = 0;
   t.omega = w;
   t.cw = kC / w;
   t.c0w = c0 * w;
   t.c2 = be.c2;
-  t.valid = true;
+  *tape_out = t.id;
+  return SRS_OK;
+}
+
+int srs_hip_pole_pd_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle,
+                            int64_t nreal, const double* deltas, double charge, double current,
+                            const double* obs_xyz, const double* normals_xyz, int64_t ns, int directional,
+                            const double* target, int checkpoint_every, double* pd_out, double* loss_out,
+                            int64_t* tape_out) {
+  g_err.clear();
+  TRY(pole_validate(und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, loss_out,
+                    tape_out));
+  if (!normals_xyz) return fail(SRS_ERR_INVALID, "NULL argument");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  hipStream_t s = c.stream;
+  PoleRun pr;
+  TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, true, pr, s));
+  TRY(stage_points(c, c.g_nrm, normals_xyz, ns));
+  int64_t const npts = pr.P[0].nb + pr.P[0].nf;
+  int const nch = srs::pd_batch_chunks(npts, (int)ns, (int)nreal);
+  HIPCHK(c.g_part.ensure((size_t)nreal * nch * ns * sizeof(double)));
+  HIPCHK(c.g_field.ensure((size_t)nreal * ns * sizeof(double)));
+  double const scale = pd_scale_of(charge, current, pr.P[0].dt);
+  auto* Pd = c.b_part.as<srs::BatchParticle>();
+  {
+    KernelTimer timer(c, s);
+    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(), Pd, (int)nreal, pr.bt,
+                                 pr.ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
+    HIPCHK(srs::launch_pd_batch(pr.bt, Pd, (int)nreal, c.b_obs.as<double>(), c.g_nrm.as<double>(), (int)ns,
+                                directional ? 1 : 0, nch, c.g_part.as<double>(), scale, c.g_field.as<double>(),
+                                c.b_flux_pp.as<double>(), s));
+  }
+  TRY(pole_finish(c, pr, (int)nreal, (int)ns, pd_out, loss_out, s));
+  Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);
+  t.quantity = 1;
+  t.directional = directional ? 1 : 0;
+  t.pd_scale = scale;
   *tape_out = t.id;
   return SRS_OK;
 }
@@ -965,7 +1043,10 @@ int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out) {
     HIPCHK(hipMemcpyAsync(c.g_dloss.p, dl.data(), t.np * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
   }
-  HIPCHK(c.g_gy.ensure((size_t)6 * t.total * sizeof(double)));
+  bool const pd = t.quantity == 1;
+  HIPCHK(c.g_gy.ensure((size_t)(pd ? 9 : 6) * t.total * sizeof(double)));
+  double* const gy = c.g_gy.as<double>();
+  double* const ga = pd ? gy + 6 * t.total : nullptr;
   int64_t const rec_steps = std::min<int64_t>(t.seg_per_pass * t.every, std::max(t.nf, t.nb));
   HIPCHK(c.g_rec.ensure(srs::adjoint_record_doubles(rec_steps, t.np) * sizeof(double)));
   HIPCHK(c.g_lam.ensure((size_t)t.np * 6 * sizeof(double)));
@@ -974,17 +1055,21 @@ int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out) {
   auto* Pd = c.b_part.as<srs::BatchParticle>();
   {
     KernelTimer timer(c, s);
-    HIPCHK(srs::launch_flux_adjoint(t.bt, Pd, t.np, t.nmax, c.b_obs.as<double>(), t.ns, t.omega, t.cw, t.c0w, t.c2,
-                                    c.g_field.as<double>(), c.b_flux_pp.as<double>(), c.g_target.as<double>(),
-                                    c.g_dloss.as<double>(), c.g_gy.as<double>(), t.total, s));
+    if (pd)
+      HIPCHK(srs::launch_pd_adjoint(t.bt, Pd, t.np, t.nmax, c.b_obs.as<double>(), c.g_nrm.as<double>(), t.ns,
+                                    t.directional, t.pd_scale, c.g_field.as<double>(), c.b_flux_pp.as<double>(),
+                                    c.g_target.as<double>(), c.g_dloss.as<double>(), gy, ga, t.total, s));
+    else
+      HIPCHK(srs::launch_flux_adjoint(t.bt, Pd, t.np, t.nmax, c.b_obs.as<double>(), t.ns, t.omega, t.cw, t.c0w, t.c2,
+                                      c.g_field.as<double>(), c.b_flux_pp.as<double>(), c.g_target.as<double>(),
+                                      c.g_dloss.as<double>(), gy, t.total, s));
   }
   HIPCHK(srs::launch_zero(c.g_grad.as<double>(), (int64_t)t.np * t.npoles, s));
   {
     KernelTimer timer(c, s);
     HIPCHK(srs::launch_rk4_adjoint(c.b_und.as<srs::DevUndulator>(), t.nund, t.poled, c.b_poles.as<double>(), Pd, t.np,
-                                   t.nf, t.nb, t.ck, t.seg_per_pass, c.g_gy.as<double>(), t.total,
-                                   c.g_rec.as<double>(), c.g_sweep.as<double>(), c.g_lam.as<double>(),
-                                   c.g_grad.as<double>(), t.npoles, s));
+                                   t.nf, t.nb, t.ck, t.seg_per_pass, gy, ga, t.total, c.g_rec.as<double>(),
+                                   c.g_sweep.as<double>(), c.g_lam.as<double>(), c.g_grad.as<double>(), t.npoles, s));
   }
   HIPCHK(hipMemcpyAsync(grad_out, c.g_grad.p, (size_t)t.np * t.npoles * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

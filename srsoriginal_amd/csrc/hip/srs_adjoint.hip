@@ -1,14 +1,19 @@
-// Adjoint of the flux loss with respect to per-pole field errors.
+// Adjoint of the flux and power-density losses with respect to per-pole field errors.
 //
-// Forward (srs_batch.hip kernels): realisation r is a particle whose undulator poles
-// carry their own deltas; rk4_batch_kernel integrates it (checkpointing the state
-// every C steps) and batch_flux_kernel (FP64) returns its flux map F_r and the six
-// field sums S_r of every point.  Loss L_r = sum_j (F_rj - target_j)^2.
+// Forward: realisation r is a particle whose undulator poles carry their own deltas;
+// rk4_batch_kernel (srs_batch.hip) integrates it, checkpointing the state every C
+// steps.  Flux: batch_flux_kernel (FP64) returns its flux map F_r and the six field
+// sums S_r of every point.  Power density: pd_batch_kernel + pd_batch_reduce_kernel
+// return its map P_r and the signed sums before |.|.  Loss L_r = sum_j (F_rj - t_j)^2
+// (or P_rj).
 //
 // Backward:
 //   flux_adjoint_kernel  lanes over (r, trajectory point k), loop over observation
 //                        points staged in LDS: dL/dX_k and dL/dbeta_k of the
 //                        radiation integral (analytic derivative of every term);
+//   pd_adjoint_kernel    the same for the power density: dL/dX_k, dL/dbeta_k and
+//                        dL/dA_k (the stored acceleration, whose cotangent joins the
+//                        first RK4 stage of the step that produced it);
 //   rk4_record_kernel    recomputes checkpoint segments from their checkpoints, all
 //                        (segment, realisation) pairs in parallel, recording each RK4
 //                        stage's velocity, B, dB/dD and pole sine;
@@ -32,6 +37,7 @@
 //   d(w.f)/d delta_p = sin(theta) (c . field)   (point inside pole p),
 // u being the undulators' common period direction.
 #include "srs_batch.h"
+#include "srs_pd.h"
 #include "srs_rk4.h"
 #include "srs_terms.h"
 
@@ -40,7 +46,8 @@ namespace {
 
 constexpr int kAdjTile = 256;
 constexpr int kStageRec = 11;            // v[3], B[3], dBdD[3], sin at pole, pole index
-constexpr int kStepRec = 4 * kStageRec + 6;  // + the direct dL/dy of the step's point
+// + the direct dL/dy of the step's point and dL/dk1 (velocity part) of its stored acceleration
+constexpr int kStepRec = 4 * kStageRec + 9;
 constexpr int kSweeps = 7;                 // per segment: particular + 6 unit end cotangents
 
 __global__ void zero_kernel(double* __restrict__ a, int64_t n) {
@@ -150,6 +157,123 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, cons
   }
 }
 
+// Power density of every realisation: block (observation tile, realisation r, chunk
+// ch of r's trajectory), flattened into blockIdx.x with the tile fastest; one lane per
+// observation point, trajectory points staged in LDS; part[(r * nch + ch) * ns + j] =
+// the chunk's sum of pd_term.
+__global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                          const double* __restrict__ obs,
+                                                          const double* __restrict__ nrm, int ns, int directional,
+                                                          int nch, int ntiles, double* __restrict__ part) {
+  __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile], sbx[kAdjTile], sby[kAdjTile], sbz[kAdjTile];
+  __shared__ double sax[kAdjTile], say[kAdjTile], saz[kAdjTile];
+  int const tid = threadIdx.x;
+  int const tile = (int)(blockIdx.x % (unsigned)ntiles);
+  int const rc = (int)(blockIdx.x / (unsigned)ntiles);
+  int const r = rc / nch, ch = rc % nch;
+  BatchParticle const q = P[r];
+  int64_t const n = q.nb + q.nf;
+  int64_t const chunk = (n + nch - 1) / nch;
+  int64_t const i0 = ch * chunk;
+  int64_t const i1 = (i0 + chunk < n) ? i0 + chunk : n;
+  int const j = tile * kBlock + tid;
+  int const jj = j < ns ? j : ns - 1;
+  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+  double const qx = nrm[jj], qy = nrm[ns + jj], qz = nrm[2 * ns + jj];
+  double sum = 0.0;
+  for (int64_t base = i0; base < i1; base += kAdjTile) {
+    int const cnt = (i1 - base) < kAdjTile ? (int)(i1 - base) : kAdjTile;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = q.off + base + tid;
+      sx[tid] = bt.x[i];
+      sy[tid] = bt.y[i];
+      sz[tid] = bt.z[i];
+      sbx[tid] = bt.bx[i];
+      sby[tid] = bt.by[i];
+      sbz[tid] = bt.bz[i];
+      sax[tid] = bt.ax[i];
+      say[tid] = bt.ay[i];
+      saz[tid] = bt.az[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k)
+      sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
+                     directional);
+  }
+  if (j < ns) part[((size_t)r * nch + ch) * ns + j] = sum;
+}
+
+// Chunks in fixed order, then the reference's prefactor and |.| (OSCARSSR.cc:1563-1573):
+// raw[r*ns + j] = the summed integrand, pd[r*ns + j] = the power density.
+__global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch, int ns, int np, double scale,
+                                       int directional, double* __restrict__ raw, double* __restrict__ pd) {
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)np * ns) return;
+  int64_t const r = t / ns, j = t % ns;
+  double s = 0.0;
+  for (int c = 0; c < nch; ++c) s += part[((size_t)r * nch + c) * ns + j];
+  raw[t] = s;
+  s *= scale;
+  s /= 1e6;
+  if (!directional && s < 0) s = -s;
+  pd[t] = s;
+}
+
+// Power-density adjoint: lanes over (r, trajectory point k), observation points with
+// their seeds G_rj = dloss_r 2 (P_rj - t_j) scale/1e6 sign(raw_rj) staged through LDS
+// (sign 1 when directional: no |.| there).  Writes dL/dy of every point to gy (x, vx,
+// y, vy, z, vz; dL/dv = dL/dbeta / c) and dL/dA to ga.
+__global__ __launch_bounds__(kBlock) void pd_adjoint_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                            const double* __restrict__ obs,
+                                                            const double* __restrict__ nrm, int ns, int directional,
+                                                            double scale, const double* __restrict__ raw,
+                                                            const double* __restrict__ pd,
+                                                            const double* __restrict__ target,
+                                                            const double* __restrict__ dloss, double* __restrict__ gy,
+                                                            double* __restrict__ ga, int64_t total) {
+  __shared__ double so[6][kAdjTile], sg[kAdjTile];
+  int const r = blockIdx.y, tid = threadIdx.x;
+  BatchParticle const q = P[r];
+  int64_t const n = q.nb + q.nf;
+  int64_t const k = (int64_t)blockIdx.x * kBlock + tid;
+  if ((int64_t)blockIdx.x * kBlock >= n) return;  // whole block past this realisation's points
+  bool const valid = k < n;
+  int64_t const i = q.off + (valid ? k : n - 1);
+  double const X[3] = {bt.x[i], bt.y[i], bt.z[i]};
+  double const B[3] = {bt.bx[i], bt.by[i], bt.bz[i]};
+  double const A[3] = {bt.ax[i], bt.ay[i], bt.az[i]};
+  double const gsc = dloss[r] * 2.0 * scale / 1e6;
+  double gX[3] = {0, 0, 0}, gB[3] = {0, 0, 0}, gA[3] = {0, 0, 0};
+  for (int j0 = 0; j0 < ns; j0 += kAdjTile) {
+    int const cnt = (ns - j0) < kAdjTile ? ns - j0 : kAdjTile;
+    __syncthreads();
+    for (int t = tid; t < cnt; t += kBlock) {
+      int const j = j0 + t;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        so[c][t] = obs[c * ns + j];
+        so[3 + c][t] = nrm[c * ns + j];
+      }
+      size_t const rj = (size_t)r * ns + j;
+      double const sgn = (!directional && raw[rj] < 0) ? -1.0 : 1.0;
+      sg[t] = gsc * sgn * (pd[rj] - target[j]);
+    }
+    __syncthreads();
+    for (int t = 0; t < cnt; ++t)
+      pd_term_adjoint(sg[t], so[0][t], so[1][t], so[2][t], so[3][t], so[4][t], so[5][t], X, B, A, directional, gX,
+                      gB, gA);
+  }
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gy[(2 * c) * total + i] = gX[c];
+      gy[(2 * c + 1) * total + i] = gB[c] * kInvC;
+      ga[c * total + i] = gA[c];
+    }
+  }
+}
+
 // One RK4 step from y recording the four stages (same arithmetic as RK4Ctx::step).
 template <int NU>
 __device__ __forceinline__ void step_record(RK4Ctx<NU> const& R, double* y, double h, double hh, double h6,
@@ -207,9 +331,11 @@ struct PoleAcc {
   }
 };
 
-// Transposed RK4 step: lam = dL/dy' in, dL/dy out; dL/d delta collected in pa.
+// Transposed RK4 step: lam = dL/dy' in, dL/dy out; dL/d delta collected in pa.  gk1 =
+// direct cotangent of the first stage's acceleration (dL/dk1 for k1[1], k1[3], k1[5]).
 __device__ __forceinline__ void step_adjoint(const double* __restrict__ st, double* lam, double h, double qmg,
-                                             const double* u, const double* fpole, PoleAcc& pa) {
+                                             const double* u, const double* fpole, const double* gk1,
+                                             PoleAcc& pa) {
   double const hh = h * 0.5, h6 = div_rn(h, 6.0, kInvSix);
   double kb[4][6], ybar[6];
 #pragma unroll
@@ -220,6 +346,9 @@ __device__ __forceinline__ void step_adjoint(const double* __restrict__ st, doub
     kb[1][c] = 2.0 * h6 * lam[c];
     kb[2][c] = 2.0 * h6 * lam[c];
   }
+  kb[0][1] += gk1[0];
+  kb[0][3] += gk1[1];
+  kb[0][5] += gk1[2];
   double const back[4] = {0.0, hh, hh, h};  // stage s state = y + back[s] * k_{s-1}
 #pragma unroll
   for (int s = 3; s >= 0; --s) {
@@ -262,7 +391,8 @@ __global__ __launch_bounds__(64) void rk4_record_kernel(const DevUndulator* __re
                                                          const double* __restrict__ poles,
                                                          const BatchParticle* __restrict__ Ps, int np, Checkpoints ck,
                                                          int backward_leg, int64_t sa, int64_t sb,
-                                                         const double* __restrict__ gy, int64_t total,
+                                                         const double* __restrict__ gy,
+                                                         const double* __restrict__ ga, int64_t total,
                                                          double* __restrict__ rec) {
   int64_t const lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= (sb - sa) * np) return;
@@ -285,6 +415,13 @@ __global__ __launch_bounds__(64) void rk4_record_kernel(const DevUndulator* __re
     int64_t const pt = q.off + (backward_leg ? q.nb - 1 - i : q.nb + i);
 #pragma unroll
     for (int c = 0; c < 6; ++c) dst[(size_t)(4 * kStageRec + c) * np] = gy[c * total + pt];
+    // the acceleration k1/c of step i is stored with point nb + i + 1 (forward leg, none
+    // for the last step) or with the point the step leads to (backward leg)
+    bool const has_a = ga && (backward_leg || i + 1 < q.nf);
+    int64_t const pa = backward_leg ? pt : pt + 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      dst[(size_t)(4 * kStageRec + 6 + c) * np] = has_a ? ga[c * total + pa] * kInvC : 0.0;
   }
 }
 
@@ -332,12 +469,13 @@ __global__ __launch_bounds__(64) void rk4_segment_sweep_kernel(const DevUndulato
       int64_t const ii = i - d;
       if (ii >= i0) {
         const double* g = buf[d] + 4 * kStageRec;
+        double const gk1[3] = {gdir * g[6], gdir * g[7], gdir * g[8]};
         if (backward_leg) {
 #pragma unroll
           for (int c = 0; c < 6; ++c) lam[c] = __builtin_fma(gdir, g[c], lam[c]);
-          step_adjoint(buf[d], lam, h, q.qmg, u, fpole, pa);
+          step_adjoint(buf[d], lam, h, q.qmg, u, fpole, gk1, pa);
         } else {
-          step_adjoint(buf[d], lam, h, q.qmg, u, fpole, pa);
+          step_adjoint(buf[d], lam, h, q.qmg, u, fpole, gk1, pa);
 #pragma unroll
           for (int c = 0; c < 6; ++c) lam[c] = __builtin_fma(gdir, g[c], lam[c]);
         }
@@ -409,6 +547,39 @@ hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int 
   return hipGetLastError();
 }
 
+int pd_batch_chunks(int64_t npoints, int ns, int np) {
+  // enough blocks to fill the device (~2048), chunks of at least 1024 points
+  int64_t const tiles = (int64_t)((ns + kBlock - 1) / kBlock) * np;
+  int64_t want = (2048 + tiles - 1) / tiles;
+  int64_t const most = (npoints + 1023) / 1024;
+  if (want > most) want = most;
+  return (int)(want < 1 ? 1 : want);
+}
+
+hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
+                           int ns, int directional, int nch, double* part, double scale, double* raw, double* pd,
+                           hipStream_t s) {
+  int const ntiles = (ns + kBlock - 1) / kBlock;
+  int64_t const nblocks = (int64_t)ntiles * np * nch;
+  if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, nch, ntiles, part);
+  hipError_t const e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  int64_t const n = (int64_t)np * ns;
+  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nch, ns, np, scale, directional, raw, pd);
+  return hipGetLastError();
+}
+
+hipError_t launch_pd_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
+                             const double* nrm, int ns, int directional, double scale, const double* raw,
+                             const double* pd, const double* target, const double* dloss, double* gy, double* ga,
+                             int64_t total, hipStream_t s) {
+  dim3 const grid((unsigned)((nmax + kBlock - 1) / kBlock), (unsigned)np);
+  pd_adjoint_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, scale, raw, pd, target, dloss, gy, ga,
+                                            total);
+  return hipGetLastError();
+}
+
 size_t adjoint_record_doubles(int64_t steps, int np) { return (size_t)steps * kStepRec * np; }
 size_t adjoint_sweep_doubles(int64_t segments, int np, int npoles) {
   return (size_t)segments * kSweeps * np * (6 + npoles);
@@ -416,8 +587,8 @@ size_t adjoint_sweep_doubles(int64_t segments, int np, int npoles) {
 
 hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const double* poles, const BatchParticle* P,
                               int np, int64_t nf, int64_t nb, Checkpoints const& ck, int64_t seg_per_pass,
-                              const double* gy, int64_t total, double* rec, double* sweep, double* lam_buf,
-                              double* grad, int npoles, hipStream_t s) {
+                              const double* gy, const double* ga, int64_t total, double* rec, double* sweep,
+                              double* lam_buf, double* grad, int npoles, hipStream_t s) {
   int64_t const C = ck.every;
   double* const lam_out = sweep;
   double* const del_out = sweep + (size_t)seg_per_pass * kSweeps * np * 6;
@@ -431,9 +602,12 @@ hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const do
       int64_t const sa = sb - seg_per_pass > 0 ? sb - seg_per_pass : 0;
       int64_t const lanes = (sb - sa) * np;
       dim3 const grid((unsigned)((lanes + 63) / 64));
-      if (nu == 1) rk4_record_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, total, rec);
-      else if (nu == 2) rk4_record_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, total, rec);
-      else rk4_record_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, total, rec);
+      if (nu == 1)
+        rk4_record_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, ga, total, rec);
+      else if (nu == 2)
+        rk4_record_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, ga, total, rec);
+      else
+        rk4_record_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, ga, total, rec);
       e = hipGetLastError();
       if (e != hipSuccess) return e;
       e = launch_zero(del_out, (sb - sa) * kSweeps * np * npoles, s);

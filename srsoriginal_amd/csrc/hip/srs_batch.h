@@ -114,9 +114,24 @@ size_t adjoint_sweep_doubles(int64_t segments, int np, int npoles);
 // min(seg_per_pass * ck.every, longest leg), np)), swept into sweep
 // (adjoint_sweep_doubles(seg_per_pass, np, npoles)); lam_buf holds np * 6 doubles.
 // All particles share nf, nb.
+// ga (may be NULL: none) = dL/dA of every stored acceleration [3][total].
 hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const double* poles, const BatchParticle* P,
                               int np, int64_t nf, int64_t nb, Checkpoints const& ck, int64_t seg_per_pass,
-                              const double* gy, int64_t total, double* rec, double* sweep, double* lam_buf,
-                              double* grad, int npoles, hipStream_t s);
+                              const double* gy, const double* ga, int64_t total, double* rec, double* sweep,
+                              double* lam_buf, double* grad, int npoles, hipStream_t s);
+// Power density of np realisations (trajectories with accelerations in bt) on ns
+// points: chunks per realisation (pd_batch_chunks), part = np * nch * ns doubles of
+// chunk sums, raw[r*ns + j] = summed integrand, pd[r*ns + j] = scale * raw / 1e6
+// (|.| unless directional), scale = |qI|/(16 pi^2 eps0 c) dt.
+int pd_batch_chunks(int64_t npoints, int ns, int np);
+hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
+                           int ns, int directional, int nch, double* part, double scale, double* raw, double* pd,
+                           hipStream_t s);
+// dL/dy (gy [6][total]) and dL/dA (ga [3][total]) of L_r = sum_j (pd_rj - target_j)^2
+// scaled by dloss[r].
+hipError_t launch_pd_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
+                             const double* nrm, int ns, int directional, double scale, const double* raw,
+                             const double* pd, const double* target, const double* dloss, double* gy, double* ga,
+                             int64_t total, hipStream_t s);
 
 }  // namespace srs

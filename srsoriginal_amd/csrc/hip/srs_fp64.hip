@@ -16,6 +16,7 @@
 // reads by broadcast.  Each chunk writes FP64 partial sums; a second kernel adds
 // the chunks in fixed order (deterministic) and applies the prefactors.
 #include "srs_kernels.h"
+#include "srs_pd.h"
 #include "srs_terms.h"
 
 namespace srs {
@@ -117,10 +118,9 @@ __global__ __launch_bounds__(kBlock) void spectrum_fp64_kernel(TrajView t, doubl
   }
 }
 
-// Power density: the reference integrand with N1, N1.n and 1 - beta.N1 in the
-// reference's exact operation order (the denominator cancels to ~1e-8, so it must
-// round identically).  (Num.N2)^2 + (Num.N3)^2 is evaluated as |Num|^2, which it
-// equals because {N1, N2, N3} is orthonormal and Num = N1 x (...) is orthogonal to N1.
+// Power density: one lane per observation point, the chunk's trajectory points
+// staged in LDS, pd_term per pair (reference operation order for N1, N1.n and
+// 1 - beta.N1; (Num.N2)^2 + (Num.N3)^2 as |Num|^2).
 __global__ __launch_bounds__(kBlock) void pd_fp64_kernel(TrajView t, const double* __restrict__ obs,
                                                          const double* __restrict__ nrm, int ns, int directional,
                                                          int chunk, double* __restrict__ part) {
@@ -149,25 +149,9 @@ __global__ __launch_bounds__(kBlock) void pd_fp64_kernel(TrajView t, const doubl
       saz[tid] = t.az[i];
     }
     __syncthreads();
-    for (int k = 0; k < cnt; ++k) {
-      double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
-      double const d2 = rx * rx + ry * ry + rz * rz;
-      double const D = __builtin_sqrt(d2);
-      double const inv = 1.0 / D;
-      double const n1x = div_rn(rx, D, inv), n1y = div_rn(ry, D, inv), n1z = div_rn(rz, D, inv);
-      double const ndot = n1x * qx + n1y * qy + n1z * qz;
-      if (directional && ndot <= 0) continue;
-      double const bx = sbx[k], by = sby[k], bz = sbz[k];
-      double const den = 1 - (bx * n1x + by * n1y + bz * n1z);
-      double const ux = n1x - bx, uy = n1y - by, uz = n1z - bz;
-      double const Ax = sax[k], Ay = say[k], Az = saz[k];
-      double const vx = uy * Az - Ay * uz, vy = uz * Ax - Az * ux, vz = ux * Ay - Ax * uy;
-      double const mx = n1y * vz - vy * n1z, my = n1z * vx - vz * n1x, mz = n1x * vy - vx * n1y;
-      double const num2 = __builtin_fma(mx, mx, __builtin_fma(my, my, mz * mz));
-      double const den2 = den * den;
-      double const den5 = den2 * den2 * den;
-      sum += num2 * ndot / (den5 * d2);
-    }
+    for (int k = 0; k < cnt; ++k)
+      sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
+                     directional);
   }
   if (j < ns) part[(size_t)blockIdx.y * ns + j] = sum;
 }

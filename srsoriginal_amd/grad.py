@@ -1,12 +1,14 @@
-"""Gradients of the flux with respect to undulator pole errors, through torch.autograd.
+"""Gradients of the flux or the power density with respect to undulator pole errors,
+through torch.autograd.
 
 The forward model is the reference's: a pole-scaled ideal undulator (each of the 2N
 interior half-period poles scaled by 1 + delta_p), the RK4 trajectory
-(OSCARSSR.cc:562-874) and the flux integral (OSCARSSR.cc:2054-2092), in FP64 on the
-GPU for a batch of error realisations at once.  The loss of realisation r is
-L_r = sum_j (F_rj - target_j)^2 over the observation points.  The backward pass is
-the hand-written adjoint of include/srs_hip.h (srs_hip_pole_backward): the flux
-integral's adjoint kernel and the RK4's, which recomputes each checkpoint segment.
+(OSCARSSR.cc:562-874) and the flux integral (OSCARSSR.cc:2054-2092) or the power
+density (OSCARSSR.cc:1477-1598), in FP64 on the GPU for a batch of error realisations
+at once.  The loss of realisation r is L_r = sum_j (F_rj - target_j)^2 over the
+observation points (F = flux or power density).  The backward pass is the
+hand-written adjoint of include/srs_hip.h (srs_hip_pole_backward): the integral's
+adjoint kernel and the RK4's, which recomputes each checkpoint segment.
 
     prob = PoleProblem(undulators, poled=0, particle=row, obs_xyz=xyz, energy_ev=e,
                        charge=q, current=i)
@@ -14,6 +16,9 @@ integral's adjoint kernel and the RK4's, which recomputes each checkpoint segmen
     deltas = torch.tensor(d, dtype=torch.float64, requires_grad=True)
     loss = pole_loss(deltas, prob)                              # [nreal]
     loss.sum().backward()                                       # deltas.grad
+
+    # power density on a screen with normals (energy unused):
+    prob = PoleProblem(undulators, 0, row, xyz, None, q, i, quantity="power_density", normals=nrm)
 """
 import numpy as np
 import torch
@@ -26,10 +31,12 @@ class PoleProblem:
 
     undulators: list of undulator dicts (field, period, nperiods[, phase, taper, center]);
     poled: index of the undulator whose poles carry the errors; particle: dict with
-    x0, beta0, qmg, t0, ct_start, ct_stop, npoints (the srs_particle fields)."""
+    x0, beta0, qmg, t0, ct_start, ct_stop, npoints (the srs_particle fields);
+    quantity: "flux" (at energy_ev) or "power_density" (normals [ns, 3], directional
+    as calculate_power_density's normal handling)."""
 
     def __init__(self, undulators, poled, particle, obs_xyz, energy_ev, charge, current, target=None,
-                 checkpoint_every=1000):
+                 checkpoint_every=1000, quantity="flux", normals=None, directional=False):
         self.undulators = [dict(u) for u in undulators]
         for u in self.undulators:
             u.pop("pole_delta", None)
@@ -37,7 +44,19 @@ class PoleProblem:
         self.npoles = 2 * int(self.undulators[self.poled]["nperiods"])
         self.particle = dict(particle)
         self.obs_xyz = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(-1, 3)
-        self.energy_ev = float(energy_ev)
+        if quantity not in ("flux", "power_density"):
+            raise ValueError("quantity must be 'flux' or 'power_density'")
+        self.quantity = quantity
+        self.energy_ev = float(energy_ev) if energy_ev is not None else None
+        if quantity == "flux" and self.energy_ev is None:
+            raise ValueError("the flux loss needs energy_ev")
+        if quantity == "power_density":
+            if normals is None:
+                raise ValueError("the power-density loss needs normals")
+            self.normals = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+            if len(self.normals) != len(self.obs_xyz):
+                raise ValueError("normals need one row per observation point")
+        self.directional = bool(directional)
         self.charge = float(charge)
         self.current = float(current)
         self.checkpoint_every = int(checkpoint_every)
@@ -47,17 +66,22 @@ class PoleProblem:
         return abi.particles([self.particle])[0]
 
     def forward(self, deltas, want_flux=False):
-        """(loss [nreal], flux [nreal, ns] or None, tape) for deltas [nreal, 2N]."""
+        """(loss [nreal], map [nreal, ns] or None, tape) for deltas [nreal, 2N]; the map is
+        the flux or the power density (want_flux asks for it in either case)."""
         if self.target is None:
             raise ValueError("PoleProblem.target is not set")
         d = np.ascontiguousarray(np.atleast_2d(deltas), dtype=np.float64)
         if d.shape[1] != self.npoles:
             raise ValueError("deltas need %d pole values per realisation" % self.npoles)
+        if self.quantity == "power_density":
+            return abi.pole_pd_forward(self.undulators, self.poled, self._particle(), d, self.charge, self.current,
+                                       self.obs_xyz, self.normals, self.target, self.directional,
+                                       self.checkpoint_every, want_flux)
         return abi.pole_forward(self.undulators, self.poled, self._particle(), d, self.charge, self.current,
                                 self.obs_xyz, self.energy_ev, self.target, self.checkpoint_every, want_flux)
 
     def flux(self, deltas):
-        """Flux maps [nreal, ns] of the realisations."""
+        """Maps [nreal, ns] of the realisations (flux, or power density for that quantity)."""
         saved = self.target
         if saved is None:
             self.target = np.zeros(len(self.obs_xyz))

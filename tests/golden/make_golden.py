@@ -289,6 +289,30 @@ def poles_flux_and_fd_gradient():
     return base, {"deltas": d, "flux_ideal": f0, "flux_scaled": fd, "loss": np.array(np.sum((fd - f0) ** 2)), "fd_grad": grad}, time.time() - t0
 
 
+
+@case
+def poles_pd_and_fd_gradient():
+    """Power density for the pole-scaled undulator and the reference's central-difference
+    gradient of L = sum_j (P_j(delta) - P_j(0))^2 with h = 1e-6 (the power-density loss)."""
+    d = pole_deltas(seed=5)
+    # threads route: the 1-thread route reads the uninitialised A/c of point 0 (OSCARSSR.cc:606)
+    base = ["poles=0,1,0,0,0,0.05,41"] + BEAM + ["ct=0,3", "npoints=2001", "plane=XY", "width=0.02,0.02", "nx=5,5", "trans=0,0,30", "nthreads=2"]
+
+    def pd(delta):
+        a, _ = run("pd", base + ["scales=" + ",".join("%.17g" % x for x in delta)])
+        return container(a)[:25, 3]  # the route appends 25 zero-valued duplicates
+
+    t0 = time.time()
+    p0 = pd(np.zeros_like(d))
+    pdd = pd(d)
+    h = 1e-6
+    grad = np.zeros_like(d)
+    for p in range(len(d)):
+        dp = d.copy(); dp[p] += h
+        dm = d.copy(); dm[p] -= h
+        grad[p] = (np.sum((pd(dp) - p0) ** 2) - np.sum((pd(dm) - p0) ** 2)) / (2 * h)
+    return base, {"deltas": d, "pd_ideal": p0, "pd_scaled": pdd, "loss": np.array(np.sum((pdd - p0) ** 2)), "fd_grad": grad}, time.time() - t0
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", default="")

@@ -127,3 +127,113 @@ def test_autograd_function_and_tape(grad, oracle):
     prob.forward(D[:1])
     with pytest.raises(abi.SrsError, match="tape"):
         prob.backward(stale, 3)
+
+
+def _pd_problem(G, oracle, ct_start=0.0, npoints=None, screen=None, directional=False):
+    name = "poles_pd_and_fd_gradient"
+    a = cases.args(name)
+    g = golden(name)
+    pole = cases.pole(a)
+    b = cases.beam(a)
+    p = oracle.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    r = cases.rect(a)
+    if screen is None:
+        xyz, nrm, _ = oracle.rect_surface(r["plane"], r["nx1"], r["nx2"], r["w1"], r["w2"], r["rot"], r["trans"],
+                                          r["normal"])
+        target = g["pd_ideal"]
+    else:
+        xyz, nrm, _ = oracle.rect_surface(*screen)
+        target = np.zeros(len(xyz))
+    c0, c1, n = cases.ct(a)
+    n = npoints or n
+    row = dict(x0=list(p.x0), beta0=list(p.beta0), qmg=p.qmg, t0=0.0, ct_start=ct_start, ct_stop=c1, npoints=n)
+    und = [dict(field=pole["field"], period=pole["period"], nperiods=pole["nperiods"])]
+    prob = G.PoleProblem(und, 0, row, xyz, None, p.q, p.current, target=target, quantity="power_density",
+                         normals=nrm, directional=directional)
+    if screen is not None:
+        prob.target = 0.9 * prob.flux(np.zeros((1, 82)))[0]
+    return prob, p, pole, xyz, nrm, (ct_start, c1, n), g
+
+
+# The power-density gradient is conditioned by 1 - beta.N ~ 1.5e-8 (it enters as
+# 1/b^6): the oracle's own dual gradient moves by 1e-8 normwise when Num is written in
+# the algebraically equal form u (N.A) - b A instead of N x (u x A) (2e-9 at 0.3 GeV),
+# so two correct FP64 gradients agree to a few 1e-8, not to the flux's 1e-13.
+PD_GRAD_TOL = 5e-8
+
+
+def test_pd_pole_gradient_matches_oracle_dual_numbers(grad, oracle):
+    say("power-density pole-error gradient: GPU adjoint vs oracle dual numbers (5x5 over 20 mm, 2001 points)")
+    prob, p, pole, xyz, nrm, (c0, c1, n), g = _pd_problem(grad, oracle)
+    D = np.array([g["deltas"]] + [np.random.default_rng(200 + k).normal(0, 0.005, 82) for k in range(2)])
+    loss, pd, tape = prob.forward(D, want_flux=True)
+    G = prob.backward(tape, len(D))
+    for r in range(len(D)):
+        F = oracle.FieldSet([], pole, D[r])
+        lo, go, po = oracle.pole_pd_loss_gradient(F, p, c0, c1, n, xyz, nrm, prob.target, nthreads=8)
+        ep = np.max(np.abs(pd[r] - po)) / np.max(np.abs(po))
+        el = abs(loss[r] - lo) / lo
+        eg = np.max(np.abs(G[r] - go)) / np.max(np.abs(go))
+        say("  realisation %d: map %.3e  loss rel %.3e  gradient normwise %.3e" % (r, ep, el, eg))
+        assert ep <= 1e-10 and el <= 1e-9 and eg <= PD_GRAD_TOL
+    assert np.max(np.abs(pd[0] - g["pd_scaled"])) <= 1e-10 * np.max(g["pd_scaled"])
+
+
+def test_pd_pole_gradient_directional_backward_leg_and_checkpoints(grad, oracle):
+    say("power-density gradient, directional tilted screen inside the undulator, ct from -0.3 m; "
+        "checkpoints every 1000 / 7 / whole")
+    screen = ("XY", 5, 4, 0.01, 0.008, (0.3, 0.1, 0.0), (0.001, 0.0, 0.4), 1)
+    prob, p, pole, xyz, nrm, (c0, c1, n), g = _pd_problem(grad, oracle, ct_start=-0.3, npoints=2201, screen=screen,
+                                                          directional=True)
+    D = np.array([g["deltas"], np.random.default_rng(300).normal(0, 0.005, 82)])
+    grads = []
+    for every in (1000, 7, 100000):
+        prob.checkpoint_every = every
+        loss, _, tape = prob.forward(D)
+        grads.append(prob.backward(tape, len(D)))
+    for gk in grads[1:]:
+        assert np.max(np.abs(gk - grads[0])) <= 1e-12 * np.max(np.abs(grads[0]))
+    for r in range(len(D)):
+        F = oracle.FieldSet([], pole, D[r])
+        lo, go, _ = oracle.pole_pd_loss_gradient(F, p, c0, c1, n, xyz, nrm, prob.target, directional=True, nthreads=8)
+        eg = np.max(np.abs(grads[0][r] - go)) / np.max(np.abs(go))
+        say("  realisation %d: loss rel %.3e  gradient normwise %.3e" % (r, abs(loss[r] - lo) / lo, eg))
+        assert abs(loss[r] - lo) <= 1e-9 * lo and eg <= PD_GRAD_TOL
+
+
+def test_pd_pole_gradient_central_differences(grad, oracle):
+    say("power-density gradient vs GPU central differences, 21x21 over 20 mm at 30 m, 20001 points, 2 realisations")
+    b = dict(energy=3.0, x0=(0, 0, -1.5), d0=(0, 0, 1), current=0.5)
+    p = oracle.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    xyz, nrm, _ = oracle.rect_surface("XY", 21, 21, 0.02, 0.02, (0, 0, 0), (0, 0, 30), 0)
+    row = dict(x0=list(p.x0), beta0=list(p.beta0), qmg=p.qmg, t0=0.0, ct_start=0.0, ct_stop=3.0, npoints=20001)
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    prob = grad.PoleProblem(und, 0, row, xyz, None, p.q, p.current, quantity="power_density", normals=nrm,
+                            target=np.zeros(len(xyz)))
+    prob.target = prob.flux(np.zeros((1, 82)))[0]
+    rng = np.random.default_rng(11)
+    D = rng.normal(0, 0.005, (2, 82))
+    loss, _, tape = prob.forward(D)
+    G = prob.backward(tape, 2)
+    h = 1e-4
+    for k in range(3):
+        v = rng.normal(0, 1, (2, 82))
+        fd = (prob.forward(D + h * v)[0] - prob.forward(D - h * v)[0]) / (2 * h)
+        ad = np.sum(G * v, axis=1)
+        err = np.abs(fd - ad) / (np.linalg.norm(G, axis=1) * np.linalg.norm(v, axis=1))
+        say("  direction %d: adjoint %s  fd %s  err %s" % (k, ad, fd, err))
+        assert np.all(err <= 1e-3)
+
+
+def test_pd_autograd_function(grad, oracle):
+    import torch
+    say("torch.autograd.Function over the power-density adjoint")
+    prob, p, pole, xyz, nrm, _, g = _pd_problem(grad, oracle)
+    D = np.array([g["deltas"], -g["deltas"]])
+    d = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+    loss = grad.pole_loss(d, prob)
+    (loss * torch.tensor([1.0, 3.0], dtype=torch.float64)).sum().backward()
+    ref_loss, _, tape = prob.forward(D)
+    ref = prob.backward(tape, 2, np.array([1.0, 3.0]))
+    assert np.array_equal(loss.detach().numpy(), ref_loss)
+    assert np.array_equal(d.grad.numpy(), ref)

@@ -296,3 +296,43 @@ def test_pole_loss_gradient_matches_reference_central_differences(oracle):
         dm = d.copy(); dm[k] -= h
         fdk = (L(dp) - L(dm)) / (2 * h)
         assert abs(grad[k] - fdk) <= 5e-4 * scale, (k, grad[k], fdk)
+
+
+def test_pole_pd_loss_gradient_matches_reference_central_differences(oracle):
+    """The oracle's dual-number gradient of the pole-error power-density loss.
+
+    The power density's map and loss are pinned bitwise-close to the reference's
+    (fixture, threads route).  Its FP64 value carries rounding noise from the
+    denominator (1 - beta.N1)^5, 1 - beta.N1 ~ 1.5e-8, which does not cancel between
+    delta +- h: the reference's h = 1e-6 central differences (fixture, reproduced by the
+    oracle to 1e-10) lie ~1.2 % from the exact gradient, differences at h = 1e-4 ~6e-5.
+    Both bounds are checked, as for the flux loss."""
+    name = "poles_pd_and_fd_gradient"
+    a = cases.args(name)
+    g = golden(name)
+    pole = cases.pole(a)
+    b = cases.beam(a)
+    p = oracle.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    xyz, nrm, x12 = _rect(oracle, name)
+    c0, c1, n = cases.ct(a)
+    d = g["deltas"]
+    p0 = g["pd_ideal"]
+    loss, grad, pd = oracle.pole_pd_loss_gradient(oracle.FieldSet([], pole, d), p, c0, c1, n, xyz, nrm, p0,
+                                                  nthreads=8)
+    assert np.max(np.abs(pd - g["pd_scaled"])) <= 1e-14 * np.max(g["pd_scaled"])
+    assert abs(loss - float(g["loss"])) <= 1e-12 * float(g["loss"])
+    fd = g["fd_grad"]
+    scale = np.max(np.abs(fd))
+    assert np.max(np.abs(grad - fd)) <= 3e-2 * scale
+
+    def L(delta):
+        t, dt = oracle.trajectory(oracle.FieldSet([], pole, delta), p, c0, c1, n)
+        return np.sum((oracle.power_density(t, dt, xyz, nrm, False, p.q, p.current) - p0) ** 2)
+
+    assert abs(L(d) - loss) <= 1e-12 * loss
+    h = 1e-4
+    for k in (0, 17, 40, 41, 81):
+        dp = d.copy(); dp[k] += h
+        dm = d.copy(); dm[k] -= h
+        fdk = (L(dp) - L(dm)) / (2 * h)
+        assert abs(grad[k] - fdk) <= 5e-4 * scale, (k, grad[k], fdk)
