@@ -234,6 +234,51 @@ def gradient_c4(nreal=256, reps=2):
             "timed": "sum of kernel durations (HIP events): forward = RK4 + flux, backward = flux adjoint + RK4 adjoint"}
 
 
+def gradient_pd(nreal=256, reps=2):
+    """Power-density gradient on the config-4 field: 256 pole-error realisations, loss
+    = sum over 51x51 points (20 mm at 30 m, the central radiation fan) of
+    (P - ideal P)^2; forward (RK4 with checkpoints, storing A/c, + FP64 power density)
+    and backward (power-density adjoint + RK4 adjoint) timed with HIP events."""
+    from srsoriginal_amd import abi, grad
+    from srsoriginal_amd.lib import _sr
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    st = osr._e.particle_state()
+    q, cur = osr._e.particle_charge_current()
+    row = dict(x0=st["x0"], beta0=st["beta0"], qmg=st["qmg"], t0=st["t0"], ct_start=0.0, ct_stop=3.0, npoints=20001)
+    surf = _sr.Surface.rectangle("XY", 51, 51, 0.02, 0.02, [0, 0, 0], [0, 0, 30], 0)
+    xyz, nrm = surf.xyz, surf.normals
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    prob = grad.PoleProblem(und, 0, row, xyz, None, q, cur, target=np.zeros(len(xyz)), checkpoint_every=1000,
+                            quantity="power_density", normals=nrm)
+    prob.target = prob.flux(np.zeros((1, 82)))[0]
+    deltas = np.random.default_rng(2025).normal(0.0, 0.005, (nreal, 82))
+    loss, _, tape = prob.forward(deltas)  # warm-up (buffers)
+    prob.backward(tape, nreal)
+    fwd, bwd = [], []
+    for _ in range(reps):
+        abi.timing_enable(True)
+        loss, _, tape = prob.forward(deltas)
+        f_ms, _ = abi.timing_read()
+        g = prob.backward(tape, nreal)
+        b_ms, _ = abi.timing_read()
+        abi.timing_enable(False)
+        fwd.append(f_ms)
+        bwd.append(b_ms)
+    f_ms, b_ms = min(fwd), min(bwd)
+    terms = float(nreal) * len(xyz) * 20001
+    log("gradient power density: forward %.1f ms, backward %.1f ms" % (f_ms, b_ms))
+    return {"workload": "%d realisations x 82 poles, power density 51x51 over 20 mm at 30 m, NT=20001, checkpoints "
+                        "every 1000 steps" % nreal,
+            "forward_ms": f_ms, "backward_ms": b_ms, "bwd_over_fwd": b_ms / f_ms,
+            "gradient_over_forward": (f_ms + b_ms) / f_ms,
+            "forward_terms_per_s": terms / (f_ms * 1e-3), "backward_terms_per_s": terms / (b_ms * 1e-3),
+            "loss_mean": float(np.mean(loss)), "grad_rms": float(np.sqrt(np.mean(g ** 2))),
+            "timed": "sum of kernel durations (HIP events): forward = RK4 + power density, backward = power-density "
+                     "adjoint + RK4 adjoint"}
+
+
 def cpu_baseline(traj, dt, xyz, q, cur, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -352,6 +397,7 @@ def main():
         out["batched_c3_share"] = batched_c3_share()
     if not a.no_grad:
         out["gradient_c4"] = gradient_c4()
+        out["gradient_power_density"] = gradient_pd()
     if not a.no_other:
         out.update(spectrum_and_power_density())
     if not a.no_cpu:

@@ -1003,19 +1003,42 @@ int srs_hip_pole_pd_forward(const srs_undulator* und, int nund, int poled, const
   PoleRun pr;
   TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, true, pr, s));
   TRY(stage_points(c, c.g_nrm, normals_xyz, ns));
-  int64_t const npts = pr.P[0].nb + pr.P[0].nf;
-  int const nch = srs::pd_batch_chunks(npts, (int)ns, (int)nreal);
-  HIPCHK(c.g_part.ensure((size_t)nreal * nch * ns * sizeof(double)));
+  // time segments as for the flux: RK4 segment k + 1 on stream2 while the power density
+  // sums the points of segment k on s
+  int nseg = 1;
+  int64_t const L = segment_steps(std::max(pr.P[0].nf, pr.P[0].nb), &nseg);
+  int const nch = srs::pd_batch_chunks(nseg > 1 ? 2 * L : pr.P[0].nb + pr.P[0].nf, (int)ns, (int)nreal);
+  int const nslot = nseg * nch;
+  HIPCHK(c.g_part.ensure((size_t)nreal * nslot * ns * sizeof(double)));
   HIPCHK(c.g_field.ensure((size_t)nreal * ns * sizeof(double)));
+  if (nseg > 1) HIPCHK(c.b_state.ensure((size_t)nreal * 24 * sizeof(double)));
   double const scale = pd_scale_of(charge, current, pr.P[0].dt);
+  int const dir = directional ? 1 : 0;
   auto* Pd = c.b_part.as<srs::BatchParticle>();
+  auto* U = c.b_und.as<srs::DevUndulator>();
+  double* const poles = c.b_poles.as<double>();
   {
     KernelTimer timer(c, s);
-    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(), Pd, (int)nreal, pr.bt,
-                                 pr.ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
-    HIPCHK(srs::launch_pd_batch(pr.bt, Pd, (int)nreal, c.b_obs.as<double>(), c.g_nrm.as<double>(), (int)ns,
-                                directional ? 1 : 0, nch, c.g_part.as<double>(), scale, c.g_field.as<double>(),
-                                c.b_flux_pp.as<double>(), s));
+    if (nseg == 1) {
+      HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, (int)nreal, pr.bt, pr.ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
+      HIPCHK(srs::launch_pd_batch(pr.bt, Pd, (int)nreal, c.b_obs.as<double>(), c.g_nrm.as<double>(), (int)ns, dir, 0,
+                                  INT64_MAX, nch, 0, nslot, c.g_part.as<double>(), s));
+    } else {
+      HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
+      HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
+      for (int k = 0; k < nseg; ++k) {
+        HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, (int)nreal, pr.bt, pr.ck,
+                                     srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()}, c.stream2));
+        HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
+      }
+      for (int k = 0; k < nseg; ++k) {
+        HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
+        HIPCHK(srs::launch_pd_batch(pr.bt, Pd, (int)nreal, c.b_obs.as<double>(), c.g_nrm.as<double>(), (int)ns, dir,
+                                    k * L, (k + 1) * L, nch, k * nch, nslot, c.g_part.as<double>(), s));
+      }
+    }
+    HIPCHK(srs::launch_pd_batch_reduce(c.g_part.as<double>(), nslot, (int)ns, (int)nreal, scale, dir,
+                                       c.g_field.as<double>(), c.b_flux_pp.as<double>(), s));
   }
   TRY(pole_finish(c, pr, (int)nreal, (int)ns, pd_out, loss_out, s));
   Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);

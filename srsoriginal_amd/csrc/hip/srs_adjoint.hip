@@ -157,14 +157,19 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, cons
   }
 }
 
-// Power density of every realisation: block (observation tile, realisation r, chunk
-// ch of r's trajectory), flattened into blockIdx.x with the tile fastest; one lane per
-// observation point, trajectory points staged in LDS; part[(r * nch + ch) * ns + j] =
-// the chunk's sum of pd_term.
+// Power density of every realisation over the points of RK4 steps [lo, hi) of both
+// legs (backward-leg points [nb - min(hi, nb), nb - min(lo, nb)) then forward-leg
+// points [nb + min(lo, nf), nb + min(hi, nf)), taken as one index range split in nch
+// chunks): block (observation tile, realisation r, chunk ch), flattened into
+// blockIdx.x with the tile fastest; one lane per observation point, trajectory
+// points staged in LDS; part[(r * nslot + slot0 + ch) * ns + j] = the chunk's sum of
+// pd_term.  Every (segment, chunk) owns its slot, so the sums are independent of the
+// order in which segments run.
 __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                           const double* __restrict__ obs,
                                                           const double* __restrict__ nrm, int ns, int directional,
-                                                          int nch, int ntiles, double* __restrict__ part) {
+                                                          int64_t lo, int64_t hi, int nch, int ntiles, int slot0,
+                                                          int nslot, double* __restrict__ part) {
   __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile], sbx[kAdjTile], sby[kAdjTile], sbz[kAdjTile];
   __shared__ double sax[kAdjTile], say[kAdjTile], saz[kAdjTile];
   int const tid = threadIdx.x;
@@ -172,7 +177,10 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
   int const rc = (int)(blockIdx.x / (unsigned)ntiles);
   int const r = rc / nch, ch = rc % nch;
   BatchParticle const q = P[r];
-  int64_t const n = q.nb + q.nf;
+  int64_t const bl = lo < q.nb ? lo : q.nb, bh = hi < q.nb ? hi : q.nb;
+  int64_t const fl = lo < q.nf ? lo : q.nf, fh = hi < q.nf ? hi : q.nf;
+  int64_t const nbs = bh - bl, n = nbs + (fh - fl);
+  int64_t const first_b = q.off + q.nb - bh, first_f = q.off + q.nb + fl - nbs;
   int64_t const chunk = (n + nch - 1) / nch;
   int64_t const i0 = ch * chunk;
   int64_t const i1 = (i0 + chunk < n) ? i0 + chunk : n;
@@ -185,7 +193,8 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
     int const cnt = (i1 - base) < kAdjTile ? (int)(i1 - base) : kAdjTile;
     __syncthreads();
     if (tid < cnt) {
-      int64_t const i = q.off + base + tid;
+      int64_t const v = base + tid;
+      int64_t const i = v < nbs ? first_b + v : first_f + v;
       sx[tid] = bt.x[i];
       sy[tid] = bt.y[i];
       sz[tid] = bt.z[i];
@@ -201,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
       sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
                      directional);
   }
-  if (j < ns) part[((size_t)r * nch + ch) * ns + j] = sum;
+  if (j < ns) part[((size_t)r * nslot + slot0 + ch) * ns + j] = sum;
 }
 
 // Chunks in fixed order, then the reference's prefactor and |.| (OSCARSSR.cc:1563-1573):
@@ -557,16 +566,20 @@ int pd_batch_chunks(int64_t npoints, int ns, int np) {
 }
 
 hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
-                           int ns, int directional, int nch, double* part, double scale, double* raw, double* pd,
-                           hipStream_t s) {
+                           int ns, int directional, int64_t lo, int64_t hi, int nch, int slot0, int nslot,
+                           double* part, hipStream_t s) {
   int const ntiles = (ns + kBlock - 1) / kBlock;
   int64_t const nblocks = (int64_t)ntiles * np * nch;
-  if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, nch, ntiles, part);
-  hipError_t const e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (nblocks > 0x7fffffff || slot0 + nch > nslot) return hipErrorInvalidValue;
+  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch, ntiles, slot0,
+                                                       nslot, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, double scale, int directional,
+                                  double* raw, double* pd, hipStream_t s) {
   int64_t const n = (int64_t)np * ns;
-  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nch, ns, np, scale, directional, raw, pd);
+  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nslot, ns, np, scale, directional, raw, pd);
   return hipGetLastError();
 }
 

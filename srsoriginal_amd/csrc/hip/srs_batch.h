@@ -120,13 +120,17 @@ hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const do
                               const double* gy, const double* ga, int64_t total, double* rec, double* sweep,
                               double* lam_buf, double* grad, int npoles, hipStream_t s);
 // Power density of np realisations (trajectories with accelerations in bt) on ns
-// points: chunks per realisation (pd_batch_chunks), part = np * nch * ns doubles of
-// chunk sums, raw[r*ns + j] = summed integrand, pd[r*ns + j] = scale * raw / 1e6
-// (|.| unless directional), scale = |qI|/(16 pi^2 eps0 c) dt.
+// points.  launch_pd_batch sums the points of RK4 steps [lo, hi) of both legs in nch
+// chunks into slots slot0 .. slot0 + nch - 1 of part (np * nslot * ns doubles);
+// launch_pd_batch_reduce adds the nslot slots in order: raw[r*ns + j] = summed
+// integrand, pd[r*ns + j] = scale * raw / 1e6 (|.| unless directional), scale =
+// |qI|/(16 pi^2 eps0 c) dt.  pd_batch_chunks gives nch for npoints points.
 int pd_batch_chunks(int64_t npoints, int ns, int np);
 hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
-                           int ns, int directional, int nch, double* part, double scale, double* raw, double* pd,
-                           hipStream_t s);
+                           int ns, int directional, int64_t lo, int64_t hi, int nch, int slot0, int nslot,
+                           double* part, hipStream_t s);
+hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, double scale, int directional,
+                                  double* raw, double* pd, hipStream_t s);
 // dL/dy (gy [6][total]) and dL/dA (ga [3][total]) of L_r = sum_j (pd_rj - target_j)^2
 // scaled by dloss[r].
 hipError_t launch_pd_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,

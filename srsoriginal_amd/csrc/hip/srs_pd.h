@@ -62,7 +62,7 @@ __device__ __forceinline__ void pd_term_adjoint(double G, double ox, double oy, 
   double const s = num[0] * num[0] + num[1] * num[1] + num[2] * num[2];
   double const ib = 1.0 / b;
   double const ib2 = ib * ib;
-  double const w = ib2 * ib2 * ib / d2;  // 1/(b^5 d2)
+  double const w = ib2 * ib2 * ib * (inv * inv);  // 1/(b^5 d2)
   double const val = s * nd * w;
   double const g2 = 2.0 * G * nd * w;
   double m[3];

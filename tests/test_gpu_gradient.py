@@ -162,9 +162,13 @@ def _pd_problem(G, oracle, ct_start=0.0, npoints=None, screen=None, directional=
 PD_GRAD_TOL = 5e-8
 
 
-def test_pd_pole_gradient_matches_oracle_dual_numbers(grad, oracle):
-    say("power-density pole-error gradient: GPU adjoint vs oracle dual numbers (5x5 over 20 mm, 2001 points)")
-    prob, p, pole, xyz, nrm, (c0, c1, n), g = _pd_problem(grad, oracle)
+@pytest.mark.parametrize("npoints", [2001, 20001])
+def test_pd_pole_gradient_matches_oracle_dual_numbers(grad, oracle, npoints):
+    # 20001 points run the forward in RK4 time segments overlapped with the power density
+    say("power-density pole-error gradient: GPU adjoint vs oracle dual numbers (5x5 over 20 mm, %d points)" % npoints)
+    prob, p, pole, xyz, nrm, (c0, c1, n), g = _pd_problem(grad, oracle, npoints=npoints)
+    if npoints != 2001:
+        prob.target = prob.flux(np.zeros((1, 82)))[0]
     D = np.array([g["deltas"]] + [np.random.default_rng(200 + k).normal(0, 0.005, 82) for k in range(2)])
     loss, pd, tape = prob.forward(D, want_flux=True)
     G = prob.backward(tape, len(D))
@@ -176,7 +180,8 @@ def test_pd_pole_gradient_matches_oracle_dual_numbers(grad, oracle):
         eg = np.max(np.abs(G[r] - go)) / np.max(np.abs(go))
         say("  realisation %d: map %.3e  loss rel %.3e  gradient normwise %.3e" % (r, ep, el, eg))
         assert ep <= 1e-10 and el <= 1e-9 and eg <= PD_GRAD_TOL
-    assert np.max(np.abs(pd[0] - g["pd_scaled"])) <= 1e-10 * np.max(g["pd_scaled"])
+    if npoints == 2001:
+        assert np.max(np.abs(pd[0] - g["pd_scaled"])) <= 1e-10 * np.max(g["pd_scaled"])
 
 
 def test_pd_pole_gradient_directional_backward_leg_and_checkpoints(grad, oracle):
