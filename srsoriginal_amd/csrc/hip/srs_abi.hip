@@ -258,8 +258,10 @@ int run_spectrum(Context& c, const TrajView& t, const double obs[3], const doubl
   double* d = c.energies.as<double>();
   HIPCHK(hipMemcpyAsync(d, h.data(), 3 * ne * sizeof(double), hipMemcpyHostToDevice, s));
   if (precision == SRS_PRECISION_MIXED) {
-    int64_t const lanes = (ne + srs::spectrum_mixed_run() - 1) / srs::spectrum_mixed_run();
-    int const chunk = pick_chunk(t.n, lanes, 32);
+    // 8-wave blocks, energy groups x trajectory chunks ~ 512 (two per CU)
+    int64_t const groups = srs::spectrum_mixed_groups((int)ne);
+    int64_t const nch_want = std::max<int64_t>(1, std::min<int64_t>((512 + groups - 1) / groups, (t.n + 63) / 64));
+    int const chunk = (int)((t.n + nch_want - 1) / nch_want);
     int const nch = nchunks(t.n, chunk);
     HIPCHK(c.part.ensure((size_t)nch * 6 * ne * sizeof(double)));
     {

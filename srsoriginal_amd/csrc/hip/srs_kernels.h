@@ -36,9 +36,10 @@ hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double
 // ---- mixed form (FP64 phase, FP32 trig / amplitudes / tile sums) -----------
 hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
                              int chunk, double* part, hipStream_t s);
-// Mixed spectrum partials: complex field sums part[(c*6 + q)*ne + e]; a lane owns
-// spectrum_mixed_run() consecutive energies.
-int spectrum_mixed_run();
+// Mixed spectrum partials: complex field sums part[(c*6 + q)*ne + e] of trajectory
+// chunk c; the energies form spectrum_mixed_groups(ne) groups of 64 lanes x 8
+// consecutive energies, one block per (group, chunk).
+int spectrum_mixed_groups(int ne);
 hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const double* omega, const double* cw, int ne,
                                  int chunk, int anchor, double* part, hipStream_t s);
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
