@@ -165,16 +165,27 @@ constexpr double kParaPhase = 1e-9;
 
 // One staged trajectory point of the paraxial form (one LDS record, so a term
 // reads it from a single base address).
+//
+// With rx = ox - x the x amplitude splits into a part of the trajectory point and the
+// observation coordinate times a part shared by x and y:
+//   (beta_x/rz - rx/rz^2) - i cw rx/rz^3 = (ax + i cw gx) - ox (q + i cw 2 gh),
+//   ax = beta_x/rz + x/rz^2,  gx = x/rz^3,  q = 1/rz^2,  2 gh = 1/rz^3,
+// and the z near-field amplitude -i cw/rz^2 is the same q: a term accumulates
+// sum ax e, sum gx e, sum ay e, sum gy e, sum q e, sum gh e and sum az e (e = e^{i phi}),
+// and the observation coordinates and c/omega are applied per sub-tile.  s comes from
+// s = (ox^2 + oy^2) + (x^2 + y^2) - 2 (ox x + oy y) (absolute error ~1e-20 m^2 on
+// these screens, 1e-13 rev of phase).
 template <int NE>
 struct alignas(16) ParaPoint {
   double x, y;        // trajectory position
+  double bsq;         // x^2 + y^2
   double h;           // 1 / (2 rz)      (one energy: W / (2 rz), W folded in)
   double t3;          // 1 / (8 rz^3)    (one energy: W / (8 rz^3))
   double phi[NE];     // Phi_k per energy, revolutions in [-0.5, 0.5]
-  float bxr, byr;     // beta_x / rz, beta_y / rz
+  float ax, ay;       // beta_x / rz + x / rz^2, beta_y / rz + y / rz^2
+  float gx, gy;       // x / rz^3, y / rz^3
   float ombr;         // (1 - beta_z) / rz
   float q;            // 1 / rz^2
-  float ir;           // 1 / rz
   float gh;           // 1 / (2 rz^3)
 };
 
@@ -260,15 +271,18 @@ __device__ __forceinline__ void para_stage(ParaTile<TILE, NE>& P, int k, double 
   ParaPoint<NE>& r = P.p[k];
   r.x = xk;
   r.y = yk;
+  r.bsq = __builtin_fma(xk, xk, yk * yk);
   double const fold = NE == 1 ? es.fw[0] : 1.0;
   r.h = fold * (0.5 * ir);
   r.t3 = fold * (0.125 * ir * ir * ir);
-  r.bxr = fbx * irf;
-  r.byr = fby * irf;
+  double const q = ir * ir;
+  r.ax = (float)__builtin_fma(xk, q, (double)fbx * ir);
+  r.ay = (float)__builtin_fma(yk, q, (double)fby * ir);
+  r.gx = (float)(xk * q * ir);
+  r.gy = (float)(yk * q * ir);
   r.ombr = fomb * irf;
-  r.q = irf * irf;
-  r.ir = irf;
-  r.gh = 0.5f * irf * irf * irf;
+  r.q = (float)q;
+  r.gh = (float)(0.5 * q * ir);
   // v = t_k - z_k/c as an unevaluated sum vh + vl, with t_k = RN(dt*ti) as the
   // reference forms it (OSCARSSR.cc:2072)
   double const t = dt * (double)ti;
@@ -290,10 +304,10 @@ __device__ __forceinline__ void para_stage(ParaTile<TILE, NE>& P, int k, double 
 template <int TILE, int NE>
 __device__ __forceinline__ void para_pad(ParaTile<TILE, NE>& P, int k) {
   ParaPoint<NE>& r = P.p[k];
-  r.x = r.y = r.h = r.t3 = 0.0;
+  r.x = r.y = r.bsq = r.h = r.t3 = 0.0;
 #pragma unroll
   for (int e = 0; e < NE; ++e) r.phi[e] = 0.0;
-  r.bxr = r.byr = r.ombr = r.q = r.ir = r.gh = 0.f;
+  r.ax = r.ay = r.gx = r.gy = r.ombr = r.q = r.gh = 0.f;
 }
 
 // Near-field sums (the -i (c/omega) n/D^2 part) are kept apart within a sub-tile and
@@ -303,25 +317,25 @@ __device__ __forceinline__ void para_pad(ParaTile<TILE, NE>& P, int k) {
 template <int NE, int TILE>
 __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int cnt, double ox, double oy,
                                                 EnergySet const& es, const float* cwf, AccMixed<NE>& a) {
+  double const asq = __builtin_fma(ox, ox, oy * oy);
+  double const mx = -2.0 * ox, my = -2.0 * oy;
+  float const oxf = (float)ox, oyf = (float)oy;
   for (int k0 = 0; k0 < cnt; k0 += kSub) {
-    float sub[NE][6], gsub[NE][6];
+    // per energy: [0..3] sum ax c, ax s, gx s, gx c; [4..7] the same for y;
+    // [8..11] sum q c, q s, gh s, gh c; [12, 13] sum az c, az s
+    float sub[NE][14];
 #pragma unroll
     for (int e = 0; e < NE; ++e)
 #pragma unroll
-      for (int q = 0; q < 6; ++q) sub[e][q] = gsub[e][q] = 0.f;
+      for (int q = 0; q < 14; ++q) sub[e][q] = 0.f;
     // kept rolled: unrolling raises the VGPR count and the lost occupancy costs more
     // than the loop overhead it saves (measured)
 #pragma unroll 1
     for (int kk = 0; kk < kSub; ++kk) {
       ParaPoint<NE> const& r = P.p[k0 + kk];
-      double const rx = ox - r.x, ry = oy - r.y;
-      double const s = __builtin_fma(rx, rx, ry * ry);
+      double const s = __builtin_fma(mx, r.x, __builtin_fma(my, r.y, asq + r.bsq));
       double const s2 = -(s * s);
-      float const sf = (float)s, rxf = (float)rx, ryf = (float)ry;
-      float const tx = rxf * r.q, ty = ryf * r.q;
-      float const ax = r.bxr - tx, ay = r.byr - ty;
-      float const az = __builtin_fmaf(sf, r.gh, -r.ombr);
-      float const gx = tx * r.ir, gy = ty * r.ir, gz = r.q;
+      float const az = __builtin_fmaf((float)s, r.gh, -r.ombr);
       double u = 0.0;
       if constexpr (NE > 1) u = __builtin_fma(s2, r.t3, s * r.h);  // D - rz
 #pragma unroll
@@ -336,32 +350,41 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
         if constexpr (NE == 1) ph = (float)(p - __builtin_rint(p));
         else ph = (float)__builtin_amdgcn_fract(p);
         float const sn = __builtin_amdgcn_sinf(ph), cn = __builtin_amdgcn_cosf(ph);
-        sub[e][0] = __builtin_fmaf(ax, cn, sub[e][0]);
-        sub[e][1] = __builtin_fmaf(ax, sn, sub[e][1]);
-        sub[e][2] = __builtin_fmaf(ay, cn, sub[e][2]);
-        sub[e][3] = __builtin_fmaf(ay, sn, sub[e][3]);
-        sub[e][4] = __builtin_fmaf(az, cn, sub[e][4]);
-        sub[e][5] = __builtin_fmaf(az, sn, sub[e][5]);
-        gsub[e][0] = __builtin_fmaf(gx, sn, gsub[e][0]);
-        gsub[e][1] = __builtin_fmaf(gx, cn, gsub[e][1]);
-        gsub[e][2] = __builtin_fmaf(gy, sn, gsub[e][2]);
-        gsub[e][3] = __builtin_fmaf(gy, cn, gsub[e][3]);
-        gsub[e][4] = __builtin_fmaf(gz, sn, gsub[e][4]);
-        gsub[e][5] = __builtin_fmaf(gz, cn, gsub[e][5]);
+        float* const b = sub[e];
+        b[0] = __builtin_fmaf(r.ax, cn, b[0]);
+        b[1] = __builtin_fmaf(r.ax, sn, b[1]);
+        b[2] = __builtin_fmaf(r.gx, sn, b[2]);
+        b[3] = __builtin_fmaf(r.gx, cn, b[3]);
+        b[4] = __builtin_fmaf(r.ay, cn, b[4]);
+        b[5] = __builtin_fmaf(r.ay, sn, b[5]);
+        b[6] = __builtin_fmaf(r.gy, sn, b[6]);
+        b[7] = __builtin_fmaf(r.gy, cn, b[7]);
+        b[8] = __builtin_fmaf(r.q, cn, b[8]);
+        b[9] = __builtin_fmaf(r.q, sn, b[9]);
+        b[10] = __builtin_fmaf(r.gh, sn, b[10]);
+        b[11] = __builtin_fmaf(r.gh, cn, b[11]);
+        b[12] = __builtin_fmaf(az, cn, b[12]);
+        b[13] = __builtin_fmaf(az, sn, b[13]);
       }
     }
     // m = 1 - eps/2 at the sub-tile's first point (always a trajectory point)
     ParaPoint<NE> const& rl = P.p[k0];
-    double const rxl = ox - rl.x, ryl = oy - rl.y;
-    float const m = __builtin_fmaf(-0.5f * (float)__builtin_fma(rxl, rxl, ryl * ryl), rl.q, 1.f);
+    double const sl = __builtin_fma(mx, rl.x, __builtin_fma(my, rl.y, asq + rl.bsq));
+    float const m = __builtin_fmaf(-0.5f * (float)sl, rl.q, 1.f);
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
-      // re += ax c + (c/omega) gx s,  im += ax s - (c/omega) gx c
-#pragma unroll
-      for (int q = 0; q < 6; q += 2) {
-        a.q[e][q].add(__builtin_fmaf(cwf[e], gsub[e][q], sub[e][q]) * m);
-        a.q[e][q + 1].add(__builtin_fmaf(-cwf[e], gsub[e][q + 1], sub[e][q + 1]) * m);
-      }
+      float const* const b = sub[e];
+      float const cw = cwf[e], cw2 = 2.f * cwf[e];
+      // shared part (q + i cw / rz^3) e^{i phi}; field sums re/im of x, y, z:
+      //   x = sum (ax + i cw gx) e - ox T,  z = sum (az - i cw q) e
+      float const tre = __builtin_fmaf(-cw2, b[10], b[8]);
+      float const tim = __builtin_fmaf(cw2, b[11], b[9]);
+      a.q[e][0].add(__builtin_fmaf(-oxf, tre, __builtin_fmaf(-cw, b[2], b[0])) * m);
+      a.q[e][1].add(__builtin_fmaf(-oxf, tim, __builtin_fmaf(cw, b[3], b[1])) * m);
+      a.q[e][2].add(__builtin_fmaf(-oyf, tre, __builtin_fmaf(-cw, b[6], b[4])) * m);
+      a.q[e][3].add(__builtin_fmaf(-oyf, tim, __builtin_fmaf(cw, b[7], b[5])) * m);
+      a.q[e][4].add(__builtin_fmaf(cw, b[9], b[12]) * m);
+      a.q[e][5].add(__builtin_fmaf(-cw, b[8], b[13]) * m);
     }
   }
 }
