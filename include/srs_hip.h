@@ -139,6 +139,15 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
                        double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
                        double weight, int precision, int flags, double* flux_out);
 
+/* Multi-particle power density [W/mm^2] on ns points with normals (AoS xyz):
+ * pd_out[j] = sum over particles (in order, Kahan) of weight * P_p(j), each P_p as
+ * srs_hip_power_density with `directional`.  Replaces the nparticles loop of
+ * OSCARSSR::CalculatePowerDensity with the GPU switch (OSCARSSR.cc:1655-1669: new
+ * particle, host RK4, one GPU call per particle with Weight = 1/NP).  FP64. */
+int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+                                double current, const double* obs_xyz, const double* normals_xyz, int64_t ns,
+                                int directional, double weight, int precision, double* pd_out);
+
 /* Tile shape used by the last batch call: energies per work item and trajectory
  * points per LDS tile. */
 int srs_hip_batch_config(int* ne_per_item, int* tile);

@@ -182,6 +182,8 @@ def _bind_batch():
                                        C.POINTER(_I64), _P]
     L.srs_hip_batch_flux.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P, _I64,
                                      _P, _I64, _D, C.c_int, C.c_int, _P]
+    L.srs_hip_batch_power_density.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D,
+                                              _P, _P, _I64, C.c_int, _D, C.c_int, _P]
     L.srs_hip_batch_config.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
                                        _D, _P, _I64, _D, _P, C.c_int, _P, _P, C.POINTER(_I64)]
@@ -189,6 +191,7 @@ def _bind_batch():
                                           _D, _D, _P, _P, _I64, C.c_int, _P, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_backward.argtypes = [_I64, _P, _P]
     for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config",
+              "srs_hip_batch_power_density",
               "srs_hip_pole_forward", "srs_hip_pole_pd_forward", "srs_hip_pole_backward"):
         getattr(L, n).restype = C.c_int
     L._batch_bound = True
@@ -249,6 +252,21 @@ def batch_flux(und_specs, parts, charge, current, obs_xyz, energies_ev, weight, 
     _check(L.srs_hip_batch_flux(U, len(U), parts, len(parts), charge, current, o.ctypes.data_as(_P), len(o),
                                 e.ctypes.data_as(_P), len(e), weight, precision, flags, out.ctypes.data_as(_P)))
     return out.reshape(len(e), len(o))
+
+
+def batch_power_density(und_specs, parts, charge, current, obs_xyz, normals_xyz, directional, weight, precision=FP64):
+    L = _bind_batch()
+    keep = []
+    U = undulators(und_specs, keep)
+    o = _f64(obs_xyz).reshape(-1, 3)
+    nrm = _f64(normals_xyz).reshape(-1, 3)
+    if len(nrm) != len(o):
+        raise SrsError("normals need one row per observation point")
+    out = np.zeros(len(o))
+    _check(L.srs_hip_batch_power_density(U, len(U), parts, len(parts), charge, current, o.ctypes.data_as(_P),
+                                         nrm.ctypes.data_as(_P), len(o), 1 if directional else 0, weight, precision,
+                                         out.ctypes.data_as(_P)))
+    return out
 
 
 def batch_config():

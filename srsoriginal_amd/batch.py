@@ -5,7 +5,9 @@ Particles are sampled on the host by the engine's generator in the reference's
 draw order (TParticleBeam::GetNewParticle, TParticleBeam.cc:228-265), so a seeded
 run draws the same particles as the reference; trajectories (device RK4), the
 flux integrals (persistent work-queue kernel) and the 1/NP-weighted Kahan sum over
-particles run on the GPU (include/srs_hip.h, srs_hip_batch_flux).
+particles run on the GPU (include/srs_hip.h, srs_hip_batch_flux).  The power
+density has the same route (srs_hip_batch_power_density, the nparticles loop of
+OSCARSSR.cc:1655-1669, FP64).
 """
 import numpy as np
 
@@ -40,3 +42,9 @@ def flux(osr, rows, charge, current, obs_xyz, energies_ev, precision="double", g
     flags = (abi.BATCH_GRAPH if graph else 0) | (abi.BATCH_AUTOTUNE if autotune else 0)
     return abi.batch_flux(osr._e.undulators(), abi.particles(rows), charge, current, obs_xyz, energies_ev,
                           1.0 / len(rows), prec, flags)
+
+
+def power_density(osr, rows, charge, current, obs_xyz, normals_xyz, directional=False):
+    """Sum over particles of power density/NP [W/mm^2] on obs_xyz (N x 3) -> array (N,)."""
+    return abi.batch_power_density(osr._e.undulators(), abi.particles(rows), charge, current, obs_xyz, normals_xyz,
+                                   directional, 1.0 / len(rows))

@@ -163,9 +163,10 @@ double c0_of(double q) { return q / (kFourPi * kC * kEps0 * kSqrt2Pi); }        
 double c2_of(double q, double cur) {                                               // OSCARSSR.cc:1194
   return kFourPi * cur / (kH * std::fabs(q) * kMu0 * kC) * 1e-6 * 0.001;
 }
-double pd_scale_of(double q, double cur, double dt) {                              // OSCARSSR.cc:1563
-  return std::fabs(q * cur) / (16 * kPi2 * kEps0 * kC) * dt;
+double pd_qfac_of(double q, double cur) {                                           // OSCARSSR.cc:1563
+  return std::fabs(q * cur) / (16 * kPi2 * kEps0 * kC);
 }
+double pd_scale_of(double q, double cur, double dt) { return pd_qfac_of(q, cur) * dt; }
 
 int check_traj(const srs_traj* t, bool need_a) {
   if (!t) return fail(SRS_ERR_INVALID, "trajectory is NULL");
@@ -497,6 +498,44 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
            cfg.ne_per_item, cfg.tile, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
            c.b_state.p, c.b_field.p, (long long)r.max_leg, r.weight);
   return buf;
+}
+
+// Power density of np particles (or realisations) already staged in b_part / b_und /
+// b_poles with trajectory buffers bt (accelerations included), observation points in
+// b_obs and normals in g_nrm: the device RK4 in time segments on stream2 overlapped
+// with pd_batch_kernel per segment on s (adding in segment order), then the ordered reduction into
+// raw (np * ns) and b_flux_pp (np * ns, the per-particle power density).
+int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int64_t max_leg, int64_t max_pts,
+                     srs::Checkpoints const& ck, int ns, int dir, double qfac, double* raw, hipStream_t s) {
+  int nseg = 1;
+  int64_t const L = segment_steps(max_leg, &nseg);
+  int const nch = srs::pd_batch_chunks(nseg > 1 ? 2 * L : max_pts, ns, np);
+  HIPCHK(c.g_part.ensure((size_t)np * nch * ns * sizeof(double)));
+  if (nseg > 1) HIPCHK(c.b_state.ensure((size_t)np * 24 * sizeof(double)));
+  auto* Pd = c.b_part.as<srs::BatchParticle>();
+  auto* U = c.b_und.as<srs::DevUndulator>();
+  double* const poles = c.b_poles.as<double>();
+  if (nseg == 1) {
+    HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
+    HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, 0, INT64_MAX, nch, 0,
+                                c.g_part.as<double>(), s));
+  } else {
+    HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
+    HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
+    for (int k = 0; k < nseg; ++k) {
+      HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
+                                   c.stream2));
+      HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
+    }
+    for (int k = 0; k < nseg; ++k) {
+      HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
+      HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, k * L, (k + 1) * L,
+                                  nch, k > 0, c.g_part.as<double>(), s));
+    }
+  }
+  HIPCHK(srs::launch_pd_batch_reduce(c.g_part.as<double>(), nch, ns, np, Pd, qfac, dir, raw,
+                                     c.b_flux_pp.as<double>(), s));
+  return SRS_OK;
 }
 
 // ---- pole-error gradient: shared parts of the forward entry points ----------------
@@ -921,6 +960,50 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
   return SRS_OK;
 }
 
+int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+                                double current, const double* obs_xyz, const double* normals_xyz, int64_t ns,
+                                int directional, double weight, int precision, double* pd_out) {
+  g_err.clear();
+  TRY(check_common(ns, precision));
+  if (precision != SRS_PRECISION_FP64)
+    return fail(SRS_ERR_INVALID, "the batched power density is computed in FP64 (SRS_PRECISION_FP64)");
+  if (!p || np < 1) return fail(SRS_ERR_INVALID, "no particles");
+  if (np > (1 << 24)) return fail(SRS_ERR_INVALID, "too many particles");
+  if (!obs_xyz || !normals_xyz || !pd_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  std::vector<srs::BatchParticle> P;
+  int64_t total = 0;
+  TRY(build_particles(p, np, P, &total));
+  int64_t max_leg = 0, max_pts = 0;
+  for (auto const& q : P) {
+    max_leg = std::max(max_leg, std::max(q.nf, q.nb));
+    max_pts = std::max(max_pts, q.nf + q.nb);
+  }
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  c.tape.valid = false;  // the batch buffers below are shared with the gradient tape
+  hipStream_t s = c.stream;
+  TRY(stage_undulators(c, und, nund, s));
+  HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
+  HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
+  TRY(stage_points(c, c.b_obs, obs_xyz, ns));
+  TRY(stage_points(c, c.g_nrm, normals_xyz, ns));
+  srs::BatchTraj bt;
+  TRY(stage_batch_traj(c, total, true, &bt));
+  HIPCHK(c.b_flux_pp.ensure((size_t)np * ns * sizeof(double)));
+  HIPCHK(c.g_field.ensure((size_t)np * ns * sizeof(double)));
+  HIPCHK(c.b_out.ensure((size_t)ns * sizeof(double)));
+  {
+    KernelTimer timer(c, s);
+    TRY(enqueue_pd_batch(c, bt, (int)np, nund, max_leg, max_pts, srs::Checkpoints{nullptr, 1, 0}, (int)ns,
+                         directional ? 1 : 0, pd_qfac_of(charge, current), c.g_field.as<double>(), s));
+    HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), (int)np, (int)ns, weight, c.b_out.as<double>(), s));
+  }
+  HIPCHK(hipMemcpyAsync(pd_out, c.b_out.p, (size_t)ns * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SRS_OK;
+}
+
 int srs_hip_batch_config(int* ne_per_item, int* tile) {
   Context& c = ctx();
   if (ne_per_item) *ne_per_item = c.b_last.ne_per_item;
@@ -1005,42 +1088,12 @@ int srs_hip_pole_pd_forward(const srs_undulator* und, int nund, int poled, const
   PoleRun pr;
   TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, true, pr, s));
   TRY(stage_points(c, c.g_nrm, normals_xyz, ns));
-  // time segments as for the flux: RK4 segment k + 1 on stream2 while the power density
-  // sums the points of segment k on s
-  int nseg = 1;
-  int64_t const L = segment_steps(std::max(pr.P[0].nf, pr.P[0].nb), &nseg);
-  int const nch = srs::pd_batch_chunks(nseg > 1 ? 2 * L : pr.P[0].nb + pr.P[0].nf, (int)ns, (int)nreal);
-  int const nslot = nseg * nch;
-  HIPCHK(c.g_part.ensure((size_t)nreal * nslot * ns * sizeof(double)));
   HIPCHK(c.g_field.ensure((size_t)nreal * ns * sizeof(double)));
-  if (nseg > 1) HIPCHK(c.b_state.ensure((size_t)nreal * 24 * sizeof(double)));
   double const scale = pd_scale_of(charge, current, pr.P[0].dt);
-  int const dir = directional ? 1 : 0;
-  auto* Pd = c.b_part.as<srs::BatchParticle>();
-  auto* U = c.b_und.as<srs::DevUndulator>();
-  double* const poles = c.b_poles.as<double>();
   {
     KernelTimer timer(c, s);
-    if (nseg == 1) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, (int)nreal, pr.bt, pr.ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
-      HIPCHK(srs::launch_pd_batch(pr.bt, Pd, (int)nreal, c.b_obs.as<double>(), c.g_nrm.as<double>(), (int)ns, dir, 0,
-                                  INT64_MAX, nch, 0, nslot, c.g_part.as<double>(), s));
-    } else {
-      HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
-      HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
-      for (int k = 0; k < nseg; ++k) {
-        HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, (int)nreal, pr.bt, pr.ck,
-                                     srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()}, c.stream2));
-        HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
-      }
-      for (int k = 0; k < nseg; ++k) {
-        HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
-        HIPCHK(srs::launch_pd_batch(pr.bt, Pd, (int)nreal, c.b_obs.as<double>(), c.g_nrm.as<double>(), (int)ns, dir,
-                                    k * L, (k + 1) * L, nch, k * nch, nslot, c.g_part.as<double>(), s));
-      }
-    }
-    HIPCHK(srs::launch_pd_batch_reduce(c.g_part.as<double>(), nslot, (int)ns, (int)nreal, scale, dir,
-                                       c.g_field.as<double>(), c.b_flux_pp.as<double>(), s));
+    TRY(enqueue_pd_batch(c, pr.bt, (int)nreal, nund, std::max(pr.P[0].nf, pr.P[0].nb), pr.P[0].nb + pr.P[0].nf, pr.ck,
+                         (int)ns, directional ? 1 : 0, pd_qfac_of(charge, current), c.g_field.as<double>(), s));
   }
   TRY(pole_finish(c, pr, (int)nreal, (int)ns, pd_out, loss_out, s));
   Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);

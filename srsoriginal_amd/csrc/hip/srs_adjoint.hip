@@ -162,14 +162,14 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, cons
 // points [nb + min(lo, nf), nb + min(hi, nf)), taken as one index range split in nch
 // chunks): block (observation tile, realisation r, chunk ch), flattened into
 // blockIdx.x with the tile fastest; one lane per observation point, trajectory
-// points staged in LDS; part[(r * nslot + slot0 + ch) * ns + j] = the chunk's sum of
-// pd_term.  Every (segment, chunk) owns its slot, so the sums are independent of the
-// order in which segments run.
+// points staged in LDS; part[(r * nch + ch) * ns + j] = the chunk's sum of pd_term,
+// or += it when add != 0 (time segments after the first, which run in order on one
+// stream, so every slot adds its segments in segment order).
 __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                           const double* __restrict__ obs,
                                                           const double* __restrict__ nrm, int ns, int directional,
-                                                          int64_t lo, int64_t hi, int nch, int ntiles, int slot0,
-                                                          int nslot, double* __restrict__ part) {
+                                                          int64_t lo, int64_t hi, int nch, int ntiles, int add,
+                                                          double* __restrict__ part) {
   __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile], sbx[kAdjTile], sby[kAdjTile], sbz[kAdjTile];
   __shared__ double sax[kAdjTile], say[kAdjTile], saz[kAdjTile];
   int const tid = threadIdx.x;
@@ -210,20 +210,26 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
       sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
                      directional);
   }
-  if (j < ns) part[((size_t)r * nslot + slot0 + ch) * ns + j] = sum;
+  if (j < ns) {
+    double* const dst = part + ((size_t)r * nch + ch) * ns + j;
+    *dst = add ? *dst + sum : sum;
+  }
 }
 
 // Chunks in fixed order, then the reference's prefactor and |.| (OSCARSSR.cc:1563-1573):
-// raw[r*ns + j] = the summed integrand, pd[r*ns + j] = the power density.
-__global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch, int ns, int np, double scale,
-                                       int directional, double* __restrict__ raw, double* __restrict__ pd) {
+// raw[r*ns + j] = the summed integrand, pd[r*ns + j] = the power density.  The
+// prefactor is qfac * dt_r (qfac = |qI|/(16 pi^2 eps0 c), formed on the host), which
+// rounds as the reference's |qI|/(16 pi^2 eps0 c) * dt.
+__global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch, int ns, int np,
+                                       const BatchParticle* __restrict__ P, double qfac, int directional,
+                                       double* __restrict__ raw, double* __restrict__ pd) {
   int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)np * ns) return;
   int64_t const r = t / ns, j = t % ns;
   double s = 0.0;
   for (int c = 0; c < nch; ++c) s += part[((size_t)r * nch + c) * ns + j];
   raw[t] = s;
-  s *= scale;
+  s *= qfac * P[r].dt;
   s /= 1e6;
   if (!directional && s < 0) s = -s;
   pd[t] = s;
@@ -566,20 +572,21 @@ int pd_batch_chunks(int64_t npoints, int ns, int np) {
 }
 
 hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
-                           int ns, int directional, int64_t lo, int64_t hi, int nch, int slot0, int nslot,
-                           double* part, hipStream_t s) {
+                           int ns, int directional, int64_t lo, int64_t hi, int nch, int add, double* part,
+                           hipStream_t s) {
   int const ntiles = (ns + kBlock - 1) / kBlock;
   int64_t const nblocks = (int64_t)ntiles * np * nch;
-  if (nblocks > 0x7fffffff || slot0 + nch > nslot) return hipErrorInvalidValue;
-  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch, ntiles, slot0,
-                                                       nslot, part);
+  if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch, ntiles, add,
+                                                       part);
   return hipGetLastError();
 }
 
-hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, double scale, int directional,
-                                  double* raw, double* pd, hipStream_t s) {
+hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
+                                  int directional, double* raw, double* pd, hipStream_t s) {
   int64_t const n = (int64_t)np * ns;
-  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nslot, ns, np, scale, directional, raw, pd);
+  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nslot, ns, np, P, qfac, directional, raw,
+                                                                      pd);
   return hipGetLastError();
 }
 

@@ -121,16 +121,17 @@ hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const do
                               double* lam_buf, double* grad, int npoles, hipStream_t s);
 // Power density of np realisations (trajectories with accelerations in bt) on ns
 // points.  launch_pd_batch sums the points of RK4 steps [lo, hi) of both legs in nch
-// chunks into slots slot0 .. slot0 + nch - 1 of part (np * nslot * ns doubles);
-// launch_pd_batch_reduce adds the nslot slots in order: raw[r*ns + j] = summed
-// integrand, pd[r*ns + j] = scale * raw / 1e6 (|.| unless directional), scale =
-// |qI|/(16 pi^2 eps0 c) dt.  pd_batch_chunks gives nch for npoints points.
+// chunks into part (np * nch * ns doubles), writing (add = 0) or adding (add != 0:
+// later time segments, queued in order on one stream); launch_pd_batch_reduce adds
+// the nch chunks in order: raw[r*ns + j] = summed integrand, pd[r*ns + j] =
+// qfac dt_r raw / 1e6 (|.| unless directional), qfac = |qI|/(16 pi^2 eps0 c).
+// pd_batch_chunks gives nch for npoints points.
 int pd_batch_chunks(int64_t npoints, int ns, int np);
 hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
-                           int ns, int directional, int64_t lo, int64_t hi, int nch, int slot0, int nslot,
-                           double* part, hipStream_t s);
-hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, double scale, int directional,
-                                  double* raw, double* pd, hipStream_t s);
+                           int ns, int directional, int64_t lo, int64_t hi, int nch, int add, double* part,
+                           hipStream_t s);
+hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
+                                  int directional, double* raw, double* pd, hipStream_t s);
 // dL/dy (gy [6][total]) and dL/dA (ga [3][total]) of L_r = sum_j (pd_rj - target_j)^2
 // scaled by dloss[r].
 hipError_t launch_pd_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,

@@ -216,10 +216,12 @@ PYBIND11_MODULE(_sr, m) {
            py::arg("dim"), py::arg("precision"), py::arg("batched") = false)
       .def("calculate_power_density",
            [](srs::OSCARSSR& e, srs::SurfacePoints const& S, int dim, bool directional, int nparticles, int nthreads,
-              int gpu, int precision) {
+              int gpu, int precision, bool batched) {
              srs::ScalarContainer C;
-             e.CalculatePowerDensity(S, C, dim, directional, nparticles, nthreads, gpu, prec_of(precision));
+             e.CalculatePowerDensity(S, C, dim, directional, nparticles, nthreads, gpu, prec_of(precision), batched);
              return container_arrays(C);
-           })
+           },
+           py::arg("surface"), py::arg("dim"), py::arg("directional"), py::arg("nparticles"), py::arg("nthreads"),
+           py::arg("gpu"), py::arg("precision"), py::arg("batched") = false)
       .def("calculate_total_power", &srs::OSCARSSR::CalculateTotalPower);
 }

@@ -691,7 +691,7 @@ void OSCARSSR::pd_one(Particle& p, SurfacePoints const& surf, ScalarContainer& C
 }
 
 void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer& C, int dim, bool directional,
-                                     int nparticles, int nthreads, int gpu, Precision prec) {
+                                     int nparticles, int nthreads, int gpu, Precision prec, bool batched) {
   // OSCARSSR::CalculatePowerDensity(Surface, C, Dim, Dir, NP, ofile, NT, GPU), OSCARSSR.cc:1603-1677.
   // Reference route semantics kept: gpu=0 ignores the particle weight (sum over
   // particles, OSCARSSR.cc:1579-1585) and its threads route adds N extra points
@@ -702,6 +702,35 @@ void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer&
   add_points(surf, C, dim);
   int const np = nparticles == 0 ? 1 : nparticles;
   double const w = nparticles == 0 ? 1.0 : 1.0 / (double)nparticles;
+  if (batched && gpu == 1 && nparticles > 1 && prec == Precision::FP64) {
+    // the gpu=1 route's draws and container points, then one device pipeline
+    std::vector<srs_particle> parts;
+    parts.reserve(np);
+    double q = 0, cur = 0;
+    bool same = true;
+    for (int i = 0; i != np; ++i) {
+      SetNewParticle();
+      add_points(surf, C, 3);
+      parts.push_back(device_particle(particle_));
+      if (i == 0) {
+        q = particle_.q;
+        cur = particle_.current;
+      } else if (particle_.q != q || particle_.current != cur) {
+        same = false;
+      }
+    }
+    if (!same)
+      throw std::invalid_argument("batched power density needs all beams to share charge and current (use batched=0)");
+    if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+    std::vector<srs_undulator> und;
+    device_fields(und);
+    std::vector<double> out(surf.size());
+    check_status(srs_hip_batch_power_density(und.data(), (int)und.size(), parts.data(), np, q, cur, surf.xyz.data(),
+                                             surf.normals.data(), (int64_t)surf.size(), directional ? 1 : 0, w,
+                                             (int)prec, out.data()));
+    for (size_t i = 0; i < out.size(); ++i) C.add_to(i, out[i]);
+    return;
+  }
   for (int i = 0; i != np; ++i) {
     if (nparticles != 0) SetNewParticle();
     if (gpu == 1) {

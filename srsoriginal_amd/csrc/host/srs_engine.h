@@ -179,8 +179,10 @@ class OSCARSSR {
   // host trajectory + one GPU call per particle.
   void CalculateFlux(SurfacePoints const& surf, double energy_ev, ScalarContainer& C, int nparticles, int nthreads,
                      int gpu, int dim, Precision prec = Precision::FP64, bool batched = false);
+  // batched: the same for the power density (srs_hip_batch_power_density, FP64; the
+  // mixed form keeps the per-particle route).
   void CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer& C, int dim, bool directional, int nparticles,
-                             int nthreads, int gpu, Precision prec = Precision::FP64);
+                             int nthreads, int gpu, Precision prec = Precision::FP64, bool batched = false);
   double CalculateTotalPower();
 
  private:

@@ -302,7 +302,10 @@ class sr:
     # ---- power density (OSCARSSR_Python.cc:2403-2577, :2587-2807) ----------------
     def calculate_power_density_rectangle(self, npoints, plane="", width=None, x0x1x2=None, rotations=None,
                                           translation=None, ofile="", normal=0, nparticles=0, gpu=0, nthreads=0,
-                                          dim=2, precision="double"):
+                                          dim=2, precision="double", batched=1):
+        """Extension: with gpu=1, nparticles > 1 and FP64, batched=1 (default) draws all
+        particles first and integrates them in one device pipeline; batched=0 keeps the
+        reference's per-particle loop."""
         if self._e.n_beams() < 1:
             raise ValueError("No particle beam defined")
         if dim != 2 and dim != 3:
@@ -310,17 +313,17 @@ class sr:
         surf = self._rectangle(npoints, plane, width, rotations, translation, x0x1x2, normal)
         self._check_common(nparticles, gpu, nthreads)
         xyz, v = self._e.calculate_power_density(surf, int(dim), normal != 0, int(nparticles), int(nthreads), int(gpu),
-                                                 _prec(precision))
+                                                 _prec(precision), bool(batched))
         return self._out(xyz, v, ofile, dim)
 
     def calculate_power_density(self, points, normal=0, rotations=None, translation=None, nparticles=0, gpu=0,
-                                nthreads=0, ofile="", precision="double"):
+                                nthreads=0, ofile="", precision="double", batched=1):
         if self._e.n_beams() < 1:
             raise ValueError("No particle beam defined")
         surf = self._points(points, rotations, translation, normal, flip=True)
         self._check_common(nparticles, gpu, nthreads)
         xyz, v = self._e.calculate_power_density(surf, 3, normal != 0, int(nparticles), int(nthreads), int(gpu),
-                                                 _prec(precision))
+                                                 _prec(precision), bool(batched))
         return self._out(xyz, v, ofile, 3)
 
     def calculate_total_power(self):

@@ -262,6 +262,15 @@ def multiflux_c3_small():
     return args, {"container": container(a)}, dt
 
 
+@case
+def multipd_c3_small():
+    # Gaussian beam, 4 particles, threads route: the reference CPU sums the particles'
+    # power density without the 1/N weight (OSCARSSR.cc:1579-1585); the 1-thread route
+    # would read the uninitialised A/c of point 0 (OSCARSSR.cc:606)
+    args = ["und1=" + UND_C0] + BEAM_C3 + ["seed=777", "ct=0,3", "npoints=3001", "plane=XY", "width=0.04,0.02", "nx=9,5", "trans=0,0,30", "nparticles=4", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
 def pole_deltas(seed=4, n=82, sigma=0.005):
     return np.random.default_rng(seed).normal(0.0, sigma, n)
 

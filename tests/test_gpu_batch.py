@@ -98,3 +98,32 @@ def test_batch_graph_replays_match_eager_and_autotune(abi, oracle):
     tr, offs, dts = abi.trajectories(und, abi.particles(rows[:1]))
     single = abi.flux(tr, dts[0], xyz, es[:1], p.q, p.current, abi.FP64)[0]
     assert normwise(one, single) <= 1e-12
+
+
+def test_batch_power_density_matches_per_particle_calls():
+    """srs_hip_batch_power_density (device RK4 in time segments + pd_batch_kernel) against
+    the single-trajectory entry point on each particle's device trajectory, weighted."""
+    from srsoriginal_amd import abi
+    from srsoriginal_amd.lib import _sr
+    from srsoriginal_amd.sr import sr
+    say("batched power density, 6 ragged particles (one > 4096 steps: segmented) vs per-particle srs_hip_power_density")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                          sigma_energy_GeV=0.00267, beta=[1.5, 0.8], emittance=[0.9e-9, 8e-12],
+                          horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
+    osr.set_seed(99)
+    from srsoriginal_amd import batch
+    rows, q, cur = batch.sample_particles(osr, 6, -0.2, 3.0, 3001, ct_jitter=0.3, points_per_meter=3000, rng_seed=5)
+    rows[2]["npoints"] = 12001  # a long one: the pipeline runs in time segments
+    surf = _sr.Surface.rectangle("XY", 9, 7, 0.04, 0.02, [0, 0, 0], [0, 0, 30], 0)
+    parts = abi.particles(rows)
+    w = 1.0 / len(rows)
+    got = batch.power_density(osr, rows, q, cur, surf.xyz, surf.normals)
+    ref = np.zeros(len(surf.xyz))
+    for r in rows:
+        t, off, dts = abi.trajectories(osr._e.undulators(), abi.particles([r]))
+        ref += w * abi.power_density(t, dts[0], surf.xyz, surf.normals, False, q, cur, abi.FP64)
+    err = normwise(got, ref)
+    say("  normwise %.3e" % err)
+    assert err <= 1e-12

@@ -77,3 +77,43 @@ def test_multi_particle_flux_routes_match_reference(batched):
     err = normwise(vals[:n], ref[:n])
     say("  normwise vs reference %.3e" % err)
     assert err <= 1e-9
+
+
+_PD_SCRIPT = r"""
+import json, sys
+sys.path.insert(0, %(root)r); sys.path.insert(0, %(tests)r)
+import numpy as np, cases
+from srsoriginal_amd.sr import sr
+a = cases.args("multipd_c3_small")
+osr = sr()
+osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                      sigma_energy_GeV=float(a["sigE"]), beta=cases.vec(a, "beta"), emittance=cases.vec(a, "emit"),
+                      horizontal_direction=cases.vec(a, "hdir"), lattice_reference=cases.vec(a, "latref"))
+osr.set_seed(int(a["seed"]))
+osr.set_ctstartstop(0, 3)
+osr.set_npoints_trajectory(int(a["npoints"]))
+out = osr.calculate_power_density_rectangle(npoints=[9, 5], plane="XY", width=[0.04, 0.02], translation=[0, 0, 30],
+                                            nparticles=int(a["nparticles"]), gpu=1, batched=int(sys.argv[1]))
+print(json.dumps([v for _, v in out]))
+"""
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_multi_particle_power_density_routes_match_reference(batched):
+    say("sr.calculate_power_density_rectangle(nparticles=4, gpu=1, batched=%d) vs the reference's 4-particle run"
+        % batched)
+    code = _PD_SCRIPT % dict(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code, str(batched)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    vals = np.array(json.loads(r.stdout.strip().splitlines()[-1]))
+    ref = golden("multipd_c3_small")["container"][:, 3]
+    n = 45
+    # gpu=1 route: N values, then N extra 3-D points per particle (OSCARSSR.cc:1869-1871)
+    assert len(vals) == n + 4 * n
+    assert np.all(vals[n:] == 0)
+    # the GPU route applies Weight = 1/N (OSCARSSR_Cuda.cu:1488); the reference CPU run
+    # in the fixture sums the particles (OSCARSSR.cc:1579-1585)
+    err = normwise(4.0 * vals[:n], ref[:n])
+    say("  normwise vs reference %.3e" % err)
+    assert err <= 1e-9
