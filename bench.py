@@ -226,6 +226,40 @@ def batched_power_density(nparticles=256, reps=2):
             "timed": "device RK4 in time segments overlapped with pd_batch_kernel, reduction (HIP events)"}
 
 
+def batched_spectrum(nparticles=256, reps=2):
+    """Multi-particle spectrum (the C3 Gaussian beam on the C0 undulator, ragged 18k-22k
+    point trajectories) at (0, 0, 30 m) over the C0 grid of 2000 energies, FP64, one
+    device pipeline; timed with HIP events."""
+    from srsoriginal_amd import abi, batch
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.050], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                          sigma_energy_GeV=8.9e-4 * 3, beta=[1.5, 0.8], emittance=[0.9e-9, 8e-12],
+                          horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
+    osr.set_seed(4096)
+    rows, q, cur = batch.sample_particles(osr, nparticles, 0.0, 3.0, 20001, ct_jitter=0.1, points_per_meter=6667,
+                                          rng_seed=4096)
+    energies = 100.0 + (3000.0 - 100.0) / 1999.0 * (np.arange(2000) + 1)  # TSpectrumContainer grid
+    npts = sum(abi.trajectory_points(abi.particles([r])) for r in rows)
+    terms = float(npts) * len(energies)
+    out = batch.spectrum(osr, rows, q, cur, [0.0, 0.0, 30.0], energies)  # warm-up
+    ms = []
+    for _ in range(reps):
+        abi.timing_enable(True)
+        out = batch.spectrum(osr, rows, q, cur, [0.0, 0.0, 30.0], energies)
+        kms, _ = abi.timing_read()
+        abi.timing_enable(False)
+        ms.append(kms)
+    t = min(ms)
+    log("batched spectrum: %.1f ms (%d particles)" % (t, nparticles))
+    return {"workload": "%d particles x %d traj points (ragged 18k-22k) x 2000 energies at (0,0,30 m)"
+                        % (nparticles, npts),
+            "terms": terms, "fp64_ms": t, "fp64_terms_per_s": terms / (t * 1e-3),
+            "peak_flux": float(np.max(out)),
+            "timed": "device RK4 in time segments overlapped with spectrum_batch_kernel, reductions (HIP events)"}
+
+
 def gradient_c4(nreal=256, reps=2):
     """Config-4 gradient: 256 pole-error realisations (82 poles, delta ~ N(0, 0.005^2),
     seeded) of the 41-period undulator, loss = sum over 51x51 points (5 mm at 30 m, E*)
@@ -430,6 +464,7 @@ def main():
     if not a.no_batch:
         out["batched_c3_share"] = batched_c3_share()
         out["batched_power_density"] = batched_power_density()
+        out["batched_spectrum"] = batched_spectrum()
     if not a.no_grad:
         out["gradient_c4"] = gradient_c4()
         out["gradient_power_density"] = gradient_pd()

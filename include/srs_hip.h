@@ -148,6 +148,14 @@ int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_pa
                                 double current, const double* obs_xyz, const double* normals_xyz, int64_t ns,
                                 int directional, double weight, int precision, double* pd_out);
 
+/* Multi-particle spectrum at one observation point: flux_out[e] = sum over particles
+ * (in order, Kahan) of weight * spectrum_p(e).  Replaces the nparticles loop of
+ * OSCARSSR::CalculateSpectrum (OSCARSSR.cc:963-982: new particle, host RK4, one
+ * spectrum per particle with Weight = 1/NP).  FP64. */
+int srs_hip_batch_spectrum(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+                           double current, const double obs[3], const double* energies_ev, int64_t ne, double weight,
+                           int precision, double* flux_out);
+
 /* Tile shape used by the last batch call: energies per work item and trajectory
  * points per LDS tile. */
 int srs_hip_batch_config(int* ne_per_item, int* tile);

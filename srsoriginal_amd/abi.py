@@ -184,6 +184,8 @@ def _bind_batch():
                                      _P, _I64, _D, C.c_int, C.c_int, _P]
     L.srs_hip_batch_power_density.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D,
                                               _P, _P, _I64, C.c_int, _D, C.c_int, _P]
+    L.srs_hip_batch_spectrum.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P,
+                                         _P, _I64, _D, C.c_int, _P]
     L.srs_hip_batch_config.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
                                        _D, _P, _I64, _D, _P, C.c_int, _P, _P, C.POINTER(_I64)]
@@ -191,7 +193,7 @@ def _bind_batch():
                                           _D, _D, _P, _P, _I64, C.c_int, _P, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_backward.argtypes = [_I64, _P, _P]
     for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config",
-              "srs_hip_batch_power_density",
+              "srs_hip_batch_power_density", "srs_hip_batch_spectrum",
               "srs_hip_pole_forward", "srs_hip_pole_pd_forward", "srs_hip_pole_backward"):
         getattr(L, n).restype = C.c_int
     L._batch_bound = True
@@ -266,6 +268,18 @@ def batch_power_density(und_specs, parts, charge, current, obs_xyz, normals_xyz,
     _check(L.srs_hip_batch_power_density(U, len(U), parts, len(parts), charge, current, o.ctypes.data_as(_P),
                                          nrm.ctypes.data_as(_P), len(o), 1 if directional else 0, weight, precision,
                                          out.ctypes.data_as(_P)))
+    return out
+
+
+def batch_spectrum(und_specs, parts, charge, current, obs, energies_ev, weight, precision=FP64):
+    L = _bind_batch()
+    keep = []
+    U = undulators(und_specs, keep)
+    o = _f64(obs)
+    e = _f64(np.atleast_1d(energies_ev))
+    out = np.zeros(len(e))
+    _check(L.srs_hip_batch_spectrum(U, len(U), parts, len(parts), charge, current, o.ctypes.data_as(_P),
+                                    e.ctypes.data_as(_P), len(e), weight, precision, out.ctypes.data_as(_P)))
     return out
 
 

@@ -6,8 +6,9 @@ draw order (TParticleBeam::GetNewParticle, TParticleBeam.cc:228-265), so a seede
 run draws the same particles as the reference; trajectories (device RK4), the
 flux integrals (persistent work-queue kernel) and the 1/NP-weighted Kahan sum over
 particles run on the GPU (include/srs_hip.h, srs_hip_batch_flux).  The power
-density has the same route (srs_hip_batch_power_density, the nparticles loop of
-OSCARSSR.cc:1655-1669, FP64).
+density and the spectrum have the same route (srs_hip_batch_power_density,
+srs_hip_batch_spectrum: the nparticles loops of OSCARSSR.cc:1655-1669 and
+:963-982, FP64).
 """
 import numpy as np
 
@@ -42,6 +43,12 @@ def flux(osr, rows, charge, current, obs_xyz, energies_ev, precision="double", g
     flags = (abi.BATCH_GRAPH if graph else 0) | (abi.BATCH_AUTOTUNE if autotune else 0)
     return abi.batch_flux(osr._e.undulators(), abi.particles(rows), charge, current, obs_xyz, energies_ev,
                           1.0 / len(rows), prec, flags)
+
+
+def spectrum(osr, rows, charge, current, obs, energies_ev):
+    """Sum over particles of spectrum/NP at obs (3,) for each energy -> array (ne,)."""
+    return abi.batch_spectrum(osr._e.undulators(), abi.particles(rows), charge, current, obs, energies_ev,
+                              1.0 / len(rows))
 
 
 def power_density(osr, rows, charge, current, obs_xyz, normals_xyz, directional=False):

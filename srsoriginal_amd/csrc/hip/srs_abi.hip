@@ -1004,6 +1004,89 @@ int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_pa
   return SRS_OK;
 }
 
+int srs_hip_batch_spectrum(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+                           double current, const double obs[3], const double* energies_ev, int64_t ne, double weight,
+                           int precision, double* flux_out) {
+  g_err.clear();
+  TRY(check_common(ne, precision));
+  if (precision != SRS_PRECISION_FP64)
+    return fail(SRS_ERR_INVALID, "the batched spectrum is computed in FP64 (SRS_PRECISION_FP64)");
+  if (!p || np < 1) return fail(SRS_ERR_INVALID, "no particles");
+  if (np > (1 << 24)) return fail(SRS_ERR_INVALID, "too many particles");
+  if (!obs || !energies_ev || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  std::vector<srs::BatchParticle> P;
+  int64_t total = 0;
+  TRY(build_particles(p, np, P, &total));
+  int64_t max_leg = 0, max_pts = 0;
+  for (auto const& q : P) {
+    max_leg = std::max(max_leg, std::max(q.nf, q.nb));
+    max_pts = std::max(max_pts, q.nf + q.nb);
+  }
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  c.tape.valid = false;  // the batch buffers below are shared with the gradient tape
+  hipStream_t s = c.stream;
+  TRY(stage_undulators(c, und, nund, s));
+  HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
+  HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
+  std::vector<double> eh(3 * ne);
+  double const c0 = c0_of(charge);
+  for (int64_t e = 0; e < ne; ++e) {
+    double const w = omega_of(energies_ev[e]);
+    eh[e] = w;
+    eh[ne + e] = kC / w;
+    eh[2 * ne + e] = c0 * w;
+  }
+  HIPCHK(c.b_energies.ensure(3 * ne * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 3 * ne * sizeof(double), hipMemcpyHostToDevice, s));
+  double* const ed = c.b_energies.as<double>();
+  srs::BatchTraj bt;
+  TRY(stage_batch_traj(c, total, false, &bt));
+  int nseg = 1;
+  int64_t const L = segment_steps(max_leg, &nseg);
+  // chunks per segment: ~2048 blocks of 256 energies
+  int64_t const tiles = ((ne + srs::kBlock - 1) / srs::kBlock) * np;
+  int64_t const pts = nseg > 1 ? 2 * L : max_pts;
+  int const nch = (int)std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, (pts + 1023) / 1024));
+  HIPCHK(c.g_part.ensure((size_t)np * nch * 12 * ne * sizeof(double)));
+  HIPCHK(c.b_flux_pp.ensure((size_t)np * ne * sizeof(double)));
+  HIPCHK(c.b_out.ensure((size_t)ne * sizeof(double)));
+  if (nseg > 1) HIPCHK(c.b_state.ensure((size_t)np * 24 * sizeof(double)));
+  HIPCHK(hipStreamSynchronize(s));  // host staging vectors
+  auto* Pd = c.b_part.as<srs::BatchParticle>();
+  auto* U = c.b_und.as<srs::DevUndulator>();
+  srs::Checkpoints const ck{nullptr, 1, 0};
+  {
+    KernelTimer timer(c, s);
+    if (nseg == 1) {
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+                                   srs::RK4Seg{0, INT64_MAX, nullptr}, s));
+      HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, 0, INT64_MAX, nch, 0,
+                                        c.g_part.as<double>(), s));
+    } else {
+      HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
+      HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
+      for (int k = 0; k < nseg; ++k) {
+        HIPCHK(srs::launch_rk4_batch(U, nund, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+                                     srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()}, c.stream2));
+        HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
+      }
+      for (int k = 0; k < nseg; ++k) {
+        HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
+        HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, k * L, (k + 1) * L, nch, k > 0,
+                                          c.g_part.as<double>(), s));
+      }
+    }
+    HIPCHK(srs::launch_spectrum_batch_reduce(c.g_part.as<double>(), nch, (int)ne, (int)np, Pd, ed + ne, ed + 2 * ne,
+                                             c2_of(charge, current), c.b_flux_pp.as<double>(), s));
+    HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), (int)np, (int)ne, weight, c.b_out.as<double>(), s));
+  }
+  HIPCHK(hipMemcpyAsync(flux_out, c.b_out.p, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SRS_OK;
+}
+
 int srs_hip_batch_config(int* ne_per_item, int* tile) {
   Context& c = ctx();
   if (ne_per_item) *ne_per_item = c.b_last.ne_per_item;

@@ -4,7 +4,7 @@
 // rk4_batch_kernel (srs_batch.hip) integrates it, checkpointing the state every C
 // steps.  Flux: batch_flux_kernel (FP64) returns its flux map F_r and the six field
 // sums S_r of every point.  Power density: pd_batch_kernel + pd_batch_reduce_kernel
-// return its map P_r and the signed sums before |.|.  Loss L_r = sum_j (F_rj - t_j)^2
+// (srs_batch.hip) return its map P_r and the signed sums before |.|.  Loss L_r = sum_j (F_rj - t_j)^2
 // (or P_rj).
 //
 // Backward:
@@ -155,84 +155,6 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, cons
     gy[4 * total + i] = -gR[2];
     gy[5 * total + i] = gb[2] * kInvC;
   }
-}
-
-// Power density of every realisation over the points of RK4 steps [lo, hi) of both
-// legs (backward-leg points [nb - min(hi, nb), nb - min(lo, nb)) then forward-leg
-// points [nb + min(lo, nf), nb + min(hi, nf)), taken as one index range split in nch
-// chunks): block (observation tile, realisation r, chunk ch), flattened into
-// blockIdx.x with the tile fastest; one lane per observation point, trajectory
-// points staged in LDS; part[(r * nch + ch) * ns + j] = the chunk's sum of pd_term,
-// or += it when add != 0 (time segments after the first, which run in order on one
-// stream, so every slot adds its segments in segment order).
-__global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
-                                                          const double* __restrict__ obs,
-                                                          const double* __restrict__ nrm, int ns, int directional,
-                                                          int64_t lo, int64_t hi, int nch, int ntiles, int add,
-                                                          double* __restrict__ part) {
-  __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile], sbx[kAdjTile], sby[kAdjTile], sbz[kAdjTile];
-  __shared__ double sax[kAdjTile], say[kAdjTile], saz[kAdjTile];
-  int const tid = threadIdx.x;
-  int const tile = (int)(blockIdx.x % (unsigned)ntiles);
-  int const rc = (int)(blockIdx.x / (unsigned)ntiles);
-  int const r = rc / nch, ch = rc % nch;
-  BatchParticle const q = P[r];
-  int64_t const bl = lo < q.nb ? lo : q.nb, bh = hi < q.nb ? hi : q.nb;
-  int64_t const fl = lo < q.nf ? lo : q.nf, fh = hi < q.nf ? hi : q.nf;
-  int64_t const nbs = bh - bl, n = nbs + (fh - fl);
-  int64_t const first_b = q.off + q.nb - bh, first_f = q.off + q.nb + fl - nbs;
-  int64_t const chunk = (n + nch - 1) / nch;
-  int64_t const i0 = ch * chunk;
-  int64_t const i1 = (i0 + chunk < n) ? i0 + chunk : n;
-  int const j = tile * kBlock + tid;
-  int const jj = j < ns ? j : ns - 1;
-  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
-  double const qx = nrm[jj], qy = nrm[ns + jj], qz = nrm[2 * ns + jj];
-  double sum = 0.0;
-  for (int64_t base = i0; base < i1; base += kAdjTile) {
-    int const cnt = (i1 - base) < kAdjTile ? (int)(i1 - base) : kAdjTile;
-    __syncthreads();
-    if (tid < cnt) {
-      int64_t const v = base + tid;
-      int64_t const i = v < nbs ? first_b + v : first_f + v;
-      sx[tid] = bt.x[i];
-      sy[tid] = bt.y[i];
-      sz[tid] = bt.z[i];
-      sbx[tid] = bt.bx[i];
-      sby[tid] = bt.by[i];
-      sbz[tid] = bt.bz[i];
-      sax[tid] = bt.ax[i];
-      say[tid] = bt.ay[i];
-      saz[tid] = bt.az[i];
-    }
-    __syncthreads();
-    for (int k = 0; k < cnt; ++k)
-      sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
-                     directional);
-  }
-  if (j < ns) {
-    double* const dst = part + ((size_t)r * nch + ch) * ns + j;
-    *dst = add ? *dst + sum : sum;
-  }
-}
-
-// Chunks in fixed order, then the reference's prefactor and |.| (OSCARSSR.cc:1563-1573):
-// raw[r*ns + j] = the summed integrand, pd[r*ns + j] = the power density.  The
-// prefactor is qfac * dt_r (qfac = |qI|/(16 pi^2 eps0 c), formed on the host), which
-// rounds as the reference's |qI|/(16 pi^2 eps0 c) * dt.
-__global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch, int ns, int np,
-                                       const BatchParticle* __restrict__ P, double qfac, int directional,
-                                       double* __restrict__ raw, double* __restrict__ pd) {
-  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)np * ns) return;
-  int64_t const r = t / ns, j = t % ns;
-  double s = 0.0;
-  for (int c = 0; c < nch; ++c) s += part[((size_t)r * nch + c) * ns + j];
-  raw[t] = s;
-  s *= qfac * P[r].dt;
-  s /= 1e6;
-  if (!directional && s < 0) s = -s;
-  pd[t] = s;
 }
 
 // Power-density adjoint: lanes over (r, trajectory point k), observation points with
@@ -559,34 +481,6 @@ hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int 
   dim3 const grid((unsigned)((nmax + kBlock - 1) / kBlock), (unsigned)np);
   flux_adjoint_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target, dloss,
                                               gy, total);
-  return hipGetLastError();
-}
-
-int pd_batch_chunks(int64_t npoints, int ns, int np) {
-  // enough blocks to fill the device (~2048), chunks of at least 1024 points
-  int64_t const tiles = (int64_t)((ns + kBlock - 1) / kBlock) * np;
-  int64_t want = (2048 + tiles - 1) / tiles;
-  int64_t const most = (npoints + 1023) / 1024;
-  if (want > most) want = most;
-  return (int)(want < 1 ? 1 : want);
-}
-
-hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
-                           int ns, int directional, int64_t lo, int64_t hi, int nch, int add, double* part,
-                           hipStream_t s) {
-  int const ntiles = (ns + kBlock - 1) / kBlock;
-  int64_t const nblocks = (int64_t)ntiles * np * nch;
-  if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch, ntiles, add,
-                                                       part);
-  return hipGetLastError();
-}
-
-hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
-                                  int directional, double* raw, double* pd, hipStream_t s) {
-  int64_t const n = (int64_t)np * ns;
-  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nslot, ns, np, P, qfac, directional, raw,
-                                                                      pd);
   return hipGetLastError();
 }
 

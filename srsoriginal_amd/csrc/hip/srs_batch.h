@@ -132,6 +132,16 @@ hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, 
                            hipStream_t s);
 hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
                                   int directional, double* raw, double* pd, hipStream_t s);
+// Spectrum of np particles at one observation point for ne energies (omega [ne]):
+// launch_spectrum_batch sums the points of RK4 steps [lo, hi) of both legs in nch
+// chunks into part (np * nch * 12 * ne doubles; write or add as launch_pd_batch);
+// launch_spectrum_batch_reduce adds the chunks in order and applies cw = c/omega,
+// i (C0 omega) dt_r and C2: out[r*ne + e].
+hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double obs[3],
+                                 const double* omega, int ne, int64_t lo, int64_t hi, int nch, int add, double* part,
+                                 hipStream_t s);
+hipError_t launch_spectrum_batch_reduce(const double* part, int nch, int ne, int np, const BatchParticle* P,
+                                        const double* cw, const double* c0w, double c2, double* out, hipStream_t s);
 // dL/dy (gy [6][total]) and dL/dA (ga [3][total]) of L_r = sum_j (pd_rj - target_j)^2
 // scaled by dloss[r].
 hipError_t launch_pd_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,

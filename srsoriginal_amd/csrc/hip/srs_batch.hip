@@ -12,6 +12,7 @@
 //   particle_reduce       Kahan-sums the per-particle maps in particle order with
 //                         the weight 1/NP (T3DScalarContainer::AddToPoint order).
 #include "srs_batch.h"
+#include "srs_pd.h"
 #include "srs_rk4.h"
 #include "srs_terms.h"
 
@@ -294,6 +295,185 @@ hipError_t launch_batch_m(BatchConfig const& cfg, BatchTraj const& bt, const Bat
   return hipErrorInvalidValue;
 }
 
+// Power density of every realisation over the points of RK4 steps [lo, hi) of both
+// legs (backward-leg points [nb - min(hi, nb), nb - min(lo, nb)) then forward-leg
+// points [nb + min(lo, nf), nb + min(hi, nf)), taken as one index range split in nch
+// chunks): block (observation tile, realisation r, chunk ch), flattened into
+// blockIdx.x with the tile fastest; one lane per observation point, trajectory
+// points staged in LDS; part[(r * nch + ch) * ns + j] = the chunk's sum of pd_term,
+// or += it when add != 0 (time segments after the first, which run in order on one
+// stream, so every slot adds its segments in segment order).
+__global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                          const double* __restrict__ obs,
+                                                          const double* __restrict__ nrm, int ns, int directional,
+                                                          int64_t lo, int64_t hi, int nch, int ntiles, int add,
+                                                          double* __restrict__ part) {
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sbx[kBlock], sby[kBlock], sbz[kBlock];
+  __shared__ double sax[kBlock], say[kBlock], saz[kBlock];
+  int const tid = threadIdx.x;
+  int const tile = (int)(blockIdx.x % (unsigned)ntiles);
+  int const rc = (int)(blockIdx.x / (unsigned)ntiles);
+  int const r = rc / nch, ch = rc % nch;
+  BatchParticle const q = P[r];
+  int64_t const bl = lo < q.nb ? lo : q.nb, bh = hi < q.nb ? hi : q.nb;
+  int64_t const fl = lo < q.nf ? lo : q.nf, fh = hi < q.nf ? hi : q.nf;
+  int64_t const nbs = bh - bl, n = nbs + (fh - fl);
+  int64_t const first_b = q.off + q.nb - bh, first_f = q.off + q.nb + fl - nbs;
+  int64_t const chunk = (n + nch - 1) / nch;
+  int64_t const i0 = ch * chunk;
+  int64_t const i1 = (i0 + chunk < n) ? i0 + chunk : n;
+  int const j = tile * kBlock + tid;
+  int const jj = j < ns ? j : ns - 1;
+  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+  double const qx = nrm[jj], qy = nrm[ns + jj], qz = nrm[2 * ns + jj];
+  double sum = 0.0;
+  for (int64_t base = i0; base < i1; base += kBlock) {
+    int const cnt = (i1 - base) < kBlock ? (int)(i1 - base) : kBlock;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const v = base + tid;
+      int64_t const i = v < nbs ? first_b + v : first_f + v;
+      sx[tid] = bt.x[i];
+      sy[tid] = bt.y[i];
+      sz[tid] = bt.z[i];
+      sbx[tid] = bt.bx[i];
+      sby[tid] = bt.by[i];
+      sbz[tid] = bt.bz[i];
+      sax[tid] = bt.ax[i];
+      say[tid] = bt.ay[i];
+      saz[tid] = bt.az[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k)
+      sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
+                     directional);
+  }
+  if (j < ns) {
+    double* const dst = part + ((size_t)r * nch + ch) * ns + j;
+    *dst = add ? *dst + sum : sum;
+  }
+}
+
+// Chunks in fixed order, then the reference's prefactor and |.| (OSCARSSR.cc:1563-1573):
+// raw[r*ns + j] = the summed integrand, pd[r*ns + j] = the power density.  The
+// prefactor is qfac * dt_r (qfac = |qI|/(16 pi^2 eps0 c), formed on the host), which
+// rounds as the reference's |qI|/(16 pi^2 eps0 c) * dt.
+__global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch, int ns, int np,
+                                       const BatchParticle* __restrict__ P, double qfac, int directional,
+                                       double* __restrict__ raw, double* __restrict__ pd) {
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)np * ns) return;
+  int64_t const r = t / ns, j = t % ns;
+  double s = 0.0;
+  for (int c = 0; c < nch; ++c) s += part[((size_t)r * nch + c) * ns + j];
+  raw[t] = s;
+  s *= qfac * P[r].dt;
+  s /= 1e6;
+  if (!directional && s < 0) s = -s;
+  pd[t] = s;
+}
+
+// Spectrum of every particle at one observation point over the points of RK4 steps
+// [lo, hi) of both legs (the same index range and chunking as pd_batch_kernel): block
+// (energy tile, particle r, chunk ch) flattened into blockIdx.x, one lane per photon
+// energy; the geometry of each trajectory point (tau, (beta - n)/D, n/D^2, in the
+// reference's operation order as spectrum_fp64_kernel) is staged once in LDS and
+// shared by the block's energies.  part[((r * nch + ch) * 12 + k) * ne + e] = the
+// chunk's sums of A e^{i phi} and G e^{i phi} (12 reals), written or added (add).
+__global__ __launch_bounds__(kBlock) void spectrum_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                                double ox, double oy, double oz,
+                                                                const double* __restrict__ omega, int ne, int64_t lo,
+                                                                int64_t hi, int nch, int ntiles, int add,
+                                                                double* __restrict__ part) {
+  __shared__ double stau[kBlock], sa[3][kBlock], sg[3][kBlock];
+  int const tid = threadIdx.x;
+  int const tile = (int)(blockIdx.x % (unsigned)ntiles);
+  int const rc = (int)(blockIdx.x / (unsigned)ntiles);
+  int const r = rc / nch, ch = rc % nch;
+  BatchParticle const q = P[r];
+  int64_t const bl = lo < q.nb ? lo : q.nb, bh = hi < q.nb ? hi : q.nb;
+  int64_t const fl = lo < q.nf ? lo : q.nf, fh = hi < q.nf ? hi : q.nf;
+  int64_t const nbs = bh - bl, n = nbs + (fh - fl);
+  int64_t const first_b = q.nb - bh, first_f = q.nb + fl - nbs;  // point index within the particle
+  int64_t const chunk = (n + nch - 1) / nch;
+  int64_t const i0 = ch * chunk;
+  int64_t const i1 = (i0 + chunk < n) ? i0 + chunk : n;
+  int const e = tile * kBlock + tid;
+  double const w = omega[e < ne ? e : ne - 1];
+  double acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  for (int64_t base = i0; base < i1; base += kBlock) {
+    int const cnt = (i1 - base) < kBlock ? (int)(i1 - base) : kBlock;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const v = base + tid;
+      int64_t const it = v < nbs ? first_b + v : first_f + v;  // time index of the point
+      int64_t const i = q.off + it;
+      double const rx = ox - bt.x[i], ry = oy - bt.y[i], rz = oz - bt.z[i];
+      double const d2 = rx * rx + ry * ry + rz * rz;
+      double const D = __builtin_sqrt(d2);
+      stau[tid] = q.dt * (double)it + div_rn(D, kC, kInvC);
+      double const inv = 1.0 / D;
+      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
+      sa[0][tid] = (bt.bx[i] - nx) * inv;
+      sa[1][tid] = (bt.by[i] - ny) * inv;
+      sa[2][tid] = (bt.bz[i] - nz) * inv;
+      double const inv2 = inv * inv;
+      sg[0][tid] = nx * inv2;
+      sg[1][tid] = ny * inv2;
+      sg[2][tid] = nz * inv2;
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+      double s, c;
+      sincos_fp64(w * stau[k], &s, &c);
+#pragma unroll
+      for (int qq = 0; qq < 3; ++qq) {
+        acc[4 * qq + 0] = __builtin_fma(sa[qq][k], c, acc[4 * qq + 0]);
+        acc[4 * qq + 1] = __builtin_fma(sa[qq][k], s, acc[4 * qq + 1]);
+        acc[4 * qq + 2] = __builtin_fma(sg[qq][k], c, acc[4 * qq + 2]);
+        acc[4 * qq + 3] = __builtin_fma(sg[qq][k], s, acc[4 * qq + 3]);
+      }
+    }
+  }
+  if (e < ne) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      double* const dst = part + (((size_t)r * nch + ch) * 12 + k) * ne + e;
+      *dst = add ? *dst + acc[k] : acc[k];
+    }
+  }
+}
+
+// Chunks in fixed order, then the reference's prefactors (OSCARSSR.cc:1202-1246):
+// sum (A - i cw G) e^{i phi} per axis, times i (C0 omega) dt_r, F = C2 sum |.|^2;
+// out[r * ne + e].
+__global__ void spectrum_batch_reduce_kernel(const double* __restrict__ part, int nch, int ne, int np,
+                                             const BatchParticle* __restrict__ P, const double* __restrict__ cw,
+                                             const double* __restrict__ c0w, double c2, double* __restrict__ out) {
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)np * ne) return;
+  int64_t const r = t / ne, e = t % ne;
+  double v[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = 0.0;
+  for (int c = 0; c < nch; ++c)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] += part[(((size_t)r * nch + c) * 12 + k) * ne + e];
+  double const w = cw[e], k2 = c0w[e] * P[r].dt;
+  double p[3];
+#pragma unroll
+  for (int qq = 0; qq < 3; ++qq) {
+    double const sre = v[4 * qq + 0] + w * v[4 * qq + 3];
+    double const sim = v[4 * qq + 1] - w * v[4 * qq + 2];
+    double const re = -(sim * k2);
+    double const im = sre * k2;
+    p[qq] = re * re + im * im;
+  }
+  out[t] = c2 * (p[0] + p[1] + p[2]);
+}
+
 }  // namespace
 
 bool batch_config_valid(BatchConfig const& c) {
@@ -351,6 +531,52 @@ hipError_t launch_batch_flux(BatchConfig const& cfg, bool mixed, BatchTraj const
 
 hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, double weight, double* out, hipStream_t s) {
   particle_reduce_kernel<<<(nout + 255) / 256, 256, 0, s>>>(flux_pp, np, nout, weight, out);
+  return hipGetLastError();
+}
+
+int pd_batch_chunks(int64_t npoints, int ns, int np) {
+  // enough blocks to fill the device (~2048), chunks of at least 1024 points
+  int64_t const tiles = (int64_t)((ns + kBlock - 1) / kBlock) * np;
+  int64_t want = (2048 + tiles - 1) / tiles;
+  int64_t const most = (npoints + 1023) / 1024;
+  if (want > most) want = most;
+  return (int)(want < 1 ? 1 : want);
+}
+
+hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
+                           int ns, int directional, int64_t lo, int64_t hi, int nch, int add, double* part,
+                           hipStream_t s) {
+  int const ntiles = (ns + kBlock - 1) / kBlock;
+  int64_t const nblocks = (int64_t)ntiles * np * nch;
+  if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch, ntiles, add,
+                                                       part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
+                                  int directional, double* raw, double* pd, hipStream_t s) {
+  int64_t const n = (int64_t)np * ns;
+  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nslot, ns, np, P, qfac, directional, raw,
+                                                                      pd);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double obs[3],
+                                 const double* omega, int ne, int64_t lo, int64_t hi, int nch, int add, double* part,
+                                 hipStream_t s) {
+  int const ntiles = (ne + kBlock - 1) / kBlock;
+  int64_t const nblocks = (int64_t)ntiles * np * nch;
+  if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+  spectrum_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs[0], obs[1], obs[2], omega, ne, lo, hi, nch,
+                                                             ntiles, add, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrum_batch_reduce(const double* part, int nch, int ne, int np, const BatchParticle* P,
+                                        const double* cw, const double* c0w, double c2, double* out, hipStream_t s) {
+  int64_t const n = (int64_t)np * ne;
+  spectrum_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nch, ne, np, P, cw, c0w, c2, out);
   return hipGetLastError();
 }
 

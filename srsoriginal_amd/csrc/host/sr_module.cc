@@ -198,13 +198,15 @@ PYBIND11_MODULE(_sr, m) {
       .def("get_trajectory", [](srs::OSCARSSR const& e) { return py::make_tuple(traj_array(e.GetTrajectory()), e.GetTrajectory().dt); })
       .def("calculate_spectrum",
            [](srs::OSCARSSR& e, std::vector<double> obs, std::vector<double> points, size_t n, double e0, double e1,
-              int nparticles, int nthreads, int gpu, int precision) {
+              int nparticles, int nthreads, int gpu, int precision, bool batched) {
              srs::SpectrumContainer S;
              if (points.empty()) S.init(n, e0, e1);
              else S.init(points);
-             e.CalculateSpectrum(v3(obs), S, nparticles, nthreads, gpu, prec_of(precision));
+             e.CalculateSpectrum(v3(obs), S, nparticles, nthreads, gpu, prec_of(precision), batched);
              return py::make_tuple(py::array_t<double>(S.e.size(), S.e.data()), py::array_t<double>(S.f.size(), S.f.data()));
-           })
+           },
+           py::arg("obs"), py::arg("points"), py::arg("n"), py::arg("e0"), py::arg("e1"), py::arg("nparticles"),
+           py::arg("nthreads"), py::arg("gpu"), py::arg("precision"), py::arg("batched") = false)
       .def("calculate_flux",
            [](srs::OSCARSSR& e, srs::SurfacePoints const& S, double energy, int nparticles, int nthreads, int gpu,
               int dim, int precision, bool batched) {

@@ -523,17 +523,42 @@ void OSCARSSR::spectrum_one(Particle& p, Vec3 const& obs, SpectrumContainer& S, 
 }
 
 void OSCARSSR::CalculateSpectrum(Vec3 const& obs, SpectrumContainer& S, int nparticles, int nthreads, int gpu,
-                                 Precision prec) {
+                                 Precision prec, bool batched) {
   // OSCARSSR::CalculateSpectrum(Obs, S, NParticles, NThreads, GPU), OSCARSSR.cc:933-984.
   // Every route accumulates value * weight; the integrals run on the GPU.
   (void)nthreads;
-  (void)gpu;
   ensure_particle();
   if (nparticles == 0) {
     spectrum_one(particle_, obs, S, 1, prec);
     return;
   }
   double const w = 1.0 / (double)nparticles;
+  if (batched && gpu == 1 && nparticles > 1 && prec == Precision::FP64) {
+    std::vector<srs_particle> parts;
+    parts.reserve(nparticles);
+    double q = 0, cur = 0;
+    bool same = true;
+    for (int i = 0; i != nparticles; ++i) {
+      SetNewParticle();
+      parts.push_back(device_particle(particle_));
+      if (i == 0) {
+        q = particle_.q;
+        cur = particle_.current;
+      } else if (particle_.q != q || particle_.current != cur) {
+        same = false;
+      }
+    }
+    if (!same) throw std::invalid_argument("batched spectrum needs all beams to share charge and current (use batched=0)");
+    if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+    std::vector<srs_undulator> und;
+    device_fields(und);
+    double const o[3] = {obs.x, obs.y, obs.z};
+    std::vector<double> out(S.e.size());
+    check_status(srs_hip_batch_spectrum(und.data(), (int)und.size(), parts.data(), nparticles, q, cur, o, S.e.data(),
+                                        (int64_t)S.e.size(), w, (int)prec, out.data()));
+    for (size_t i = 0; i < out.size(); ++i) S.add_to(i, out[i]);
+    return;
+  }
   for (int i = 0; i != nparticles; ++i) {
     SetNewParticle();
     spectrum_one(particle_, obs, S, w, prec);

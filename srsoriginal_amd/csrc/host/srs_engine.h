@@ -172,8 +172,10 @@ class OSCARSSR {
 
   // radiation (OSCARSSR.cc:933-984, :1603-1677, :2418-2489): the dispatch keeps the
   // reference's container semantics per route (see the .cc); the integrals run on the GPU.
+  // batched: with gpu=1, nparticles > 1 and FP64, all particles in one device pipeline
+  // (srs_hip_batch_spectrum) instead of one host trajectory + one GPU call each.
   void CalculateSpectrum(Vec3 const& obs, SpectrumContainer& S, int nparticles, int nthreads, int gpu,
-                         Precision prec = Precision::FP64);
+                         Precision prec = Precision::FP64, bool batched = false);
   // batched: with gpu=1 and nparticles > 1, draw every particle first (reference order)
   // and integrate them all in one device pipeline (srs_hip_batch_flux) instead of one
   // host trajectory + one GPU call per particle.

@@ -251,7 +251,10 @@ class sr:
 
     # ---- spectrum (OSCARSSR_Python.cc:2215-2351) --------------------------------
     def calculate_spectrum(self, obs=None, npoints=0, energy_range_eV=None, points_eV=None, nparticles=0, nthreads=0,
-                           gpu=0, ofile="", bofile="", precision="double"):
+                           gpu=0, ofile="", bofile="", precision="double", batched=1):
+        """Extension: with gpu=1, nparticles > 1 and FP64, batched=1 (default) draws all
+        particles first and integrates them in one device pipeline; batched=0 keeps the
+        reference's per-particle loop."""
         if nparticles < 0:
             raise ValueError("'nparticles' must be >= 1 (sort of)")
         pts = [float(v) for v in (points_eV or [])]
@@ -265,7 +268,7 @@ class sr:
         if not pts and npoints < 1:
             raise ValueError("'npoints' must be > 0")
         e, f = self._e.calculate_spectrum(o, pts, int(npoints), e0, e1, int(nparticles), int(nthreads), int(gpu),
-                                          _prec(precision))
+                                          _prec(precision), bool(batched))
         if ofile:
             with open(ofile, "w") as fh:
                 for a, b in zip(e.tolist(), f.tolist()):

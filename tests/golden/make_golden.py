@@ -263,6 +263,14 @@ def multiflux_c3_small():
 
 
 @case
+def multispectrum_c3_small():
+    # Gaussian beam, 4 particles: every route adds each particle's spectrum x 1/N
+    # (OSCARSSR.cc:963-982)
+    args = ["und1=" + UND_C0] + BEAM_C3 + ["seed=777", "ct=0,3", "npoints=3001", "ne=64", "erange=100,1000", "obs=0.001,0,30", "nparticles=4", "nthreads=1"]
+    a, dt = run("spectrum", args)
+    return args, {"spectrum": a[1:].reshape(-1, 2)}, dt
+
+@case
 def multipd_c3_small():
     # Gaussian beam, 4 particles, threads route: the reference CPU sums the particles'
     # power density without the 1/N weight (OSCARSSR.cc:1579-1585); the 1-thread route

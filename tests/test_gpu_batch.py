@@ -127,3 +127,29 @@ def test_batch_power_density_matches_per_particle_calls():
     err = normwise(got, ref)
     say("  normwise %.3e" % err)
     assert err <= 1e-12
+
+
+def test_batch_spectrum_matches_per_particle_calls():
+    """srs_hip_batch_spectrum (device RK4 in time segments + spectrum_batch_kernel) against
+    the single-trajectory spectrum on each particle's device trajectory, weighted."""
+    from srsoriginal_amd import abi, batch
+    from srsoriginal_amd.sr import sr
+    say("batched spectrum, 5 ragged particles (segmented), 300 energies, vs per-particle srs_hip_spectrum")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                          sigma_energy_GeV=0.00267, beta=[1.5, 0.8], emittance=[0.9e-9, 8e-12],
+                          horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
+    osr.set_seed(31)
+    rows, q, cur = batch.sample_particles(osr, 5, -0.2, 3.0, 3001, ct_jitter=0.3, points_per_meter=3000, rng_seed=6)
+    energies = np.linspace(100.0, 1500.0, 300)
+    obs = [0.0005, -0.0002, 30.0]
+    got = batch.spectrum(osr, rows, q, cur, obs, energies)
+    ref = np.zeros(len(energies))
+    w = 1.0 / len(rows)
+    for r in rows:
+        t, off, dts = abi.trajectories(osr._e.undulators(), abi.particles([r]))
+        ref += w * abi.spectrum(t, dts[0], obs, energies, q, cur, abi.FP64)
+    err = normwise(got, ref)
+    say("  normwise %.3e" % err)
+    assert err <= 1e-12

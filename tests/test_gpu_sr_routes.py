@@ -117,3 +117,38 @@ def test_multi_particle_power_density_routes_match_reference(batched):
     err = normwise(4.0 * vals[:n], ref[:n])
     say("  normwise vs reference %.3e" % err)
     assert err <= 1e-9
+
+
+_SPEC_SCRIPT = r"""
+import json, sys
+sys.path.insert(0, %(root)r); sys.path.insert(0, %(tests)r)
+import numpy as np, cases
+from srsoriginal_amd.sr import sr
+a = cases.args("multispectrum_c3_small")
+osr = sr()
+osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                      sigma_energy_GeV=float(a["sigE"]), beta=cases.vec(a, "beta"), emittance=cases.vec(a, "emit"),
+                      horizontal_direction=cases.vec(a, "hdir"), lattice_reference=cases.vec(a, "latref"))
+osr.set_seed(int(a["seed"]))
+osr.set_ctstartstop(0, 3)
+osr.set_npoints_trajectory(int(a["npoints"]))
+out = osr.calculate_spectrum(obs=[0.001, 0, 30], npoints=64, energy_range_eV=[100, 1000],
+                             nparticles=int(a["nparticles"]), gpu=1, batched=int(sys.argv[1]))
+print(json.dumps(out))
+"""
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_multi_particle_spectrum_routes_match_reference(batched):
+    say("sr.calculate_spectrum(nparticles=4, gpu=1, batched=%d) vs the reference's 4-particle run" % batched)
+    code = _SPEC_SCRIPT % dict(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code, str(batched)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    vals = np.array(json.loads(r.stdout.strip().splitlines()[-1]))
+    ref = golden("multispectrum_c3_small")["spectrum"]
+    assert vals.shape == ref.shape
+    assert np.array_equal(vals[:, 0], ref[:, 0])
+    err = normwise(vals[:, 1], ref[:, 1])
+    say("  normwise vs reference %.3e" % err)
+    assert err <= 1e-9
