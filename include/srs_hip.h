@@ -95,10 +95,23 @@ int srs_hip_power_density_device(const srs_traj* traj, const double* obs_soa, co
  * call per particle) with one device pipeline.  Particles are sampled by the caller
  * (host RNG, reference order) and passed as initial conditions. */
 
-/* Ideal undulator (TField3D_IdealUndulator, TField3D_IdealUndulator.cc:50-139);
- * pole_delta != NULL scales its 2*nperiods interior half-period poles by (1 + delta_p)
- * (phase, taper and center unused then). */
-typedef struct srs_undulator {
+/* One magnetic field element; the elements of a list are summed in order
+ * (TFieldContainer::GetF, TFieldContainer.cc:107-117).
+ *   kind SRS_FIELD_UNDULATOR (0, so zero-initialised records are undulators): ideal
+ *     undulator (TField3D_IdealUndulator.cc:50-139) of field, period, nperiods, phase,
+ *     taper, center; pole_delta != NULL scales its 2*nperiods interior half-period poles
+ *     by (1 + delta_p) (phase, taper and center unused then);
+ *   kind SRS_FIELD_GAUSSIAN: field * prod_k exp(-(r_k/width_k)^2/2) over the axes with
+ *     width_k > 0 (TField3D_Gaussian.cc:88-113);
+ *   kind SRS_FIELD_UNIFORM_BOX: field inside |r_k| <= |width_k|/2 for the axes with
+ *     width_k > 0, 0 outside (TField3D_UniformBox.cc:94-117);
+ * where r = Rot(X) - center, Rot rotating the position by rotations[0], [1], [2] about
+ * X, then Y, then Z (TVector3D::RotateSelfXYZ); field is given already rotated, as the
+ * reference's constructors rotate it. */
+#define SRS_FIELD_UNDULATOR 0
+#define SRS_FIELD_GAUSSIAN 1
+#define SRS_FIELD_UNIFORM_BOX 2
+typedef struct srs_field {
   double field[3];
   double period[3];
   int32_t nperiods;
@@ -106,7 +119,10 @@ typedef struct srs_undulator {
   double taper;
   double center[3];
   const double* pole_delta;
-} srs_undulator;
+  int32_t kind;
+  double width[3];
+  double rotations[3];
+} srs_field;
 
 /* One particle: initial position, beta, q/(m gamma), T0 [m], its ct range and point
  * count (set_ctstartstop / set_npoints_trajectory; dt and the forward/backward point
@@ -127,7 +143,7 @@ int srs_hip_trajectory_points(const srs_particle* p, int64_t np, int64_t* total)
 /* Device RK4 (FP64) of np particles (the GPU switch of calculate_trajectory).  traj_out
  * is host memory [9][total] (x,y,z, beta, A/c) packed particle after particle;
  * offsets_out[np + 1], dt_out[np]. */
-int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double* traj_out,
+int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, int64_t np, double* traj_out,
                          int64_t total, int64_t* offsets_out, double* dt_out);
 
 #define SRS_BATCH_GRAPH 1    /* run the pipeline as a captured hipGraph, replayed while the shape is unchanged */
@@ -135,7 +151,7 @@ int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle*
 
 /* Multi-particle flux: flux_out[e*ns + j] = sum over particles (in order, Kahan) of
  * weight * flux_p(e, j).  All particles share charge and current. */
-int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                        double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
                        double weight, int precision, int flags, double* flux_out);
 
@@ -144,7 +160,7 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
  * srs_hip_power_density with `directional`.  Replaces the nparticles loop of
  * OSCARSSR::CalculatePowerDensity with the GPU switch (OSCARSSR.cc:1655-1669: new
  * particle, host RK4, one GPU call per particle with Weight = 1/NP).  FP64. */
-int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                                 double current, const double* obs_xyz, const double* normals_xyz, int64_t ns,
                                 int directional, double weight, int precision, double* pd_out);
 
@@ -152,7 +168,7 @@ int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_pa
  * (in order, Kahan) of weight * spectrum_p(e).  Replaces the nparticles loop of
  * OSCARSSR::CalculateSpectrum (OSCARSSR.cc:963-982: new particle, host RK4, one
  * spectrum per particle with Weight = 1/NP).  FP64. */
-int srs_hip_batch_spectrum(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                            double current, const double obs[3], const double* energies_ev, int64_t ne, double weight,
                            int precision, double* flux_out);
 
@@ -182,7 +198,7 @@ int srs_hip_synchronize(void* stream);
  * backward: grad_out[r*2N + p] = dloss[r] * dL_r/d delta_p (dloss NULL = ones), by a
  * hand-written adjoint of the flux integral and of the RK4 (each checkpoint segment
  * recomputed).  A later forward, batch or trajectory call replaces the tape. */
-int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+int srs_hip_pole_forward(const srs_field* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
                          const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
                          double energy_ev, const double* target, int checkpoint_every, double* flux_out,
                          double* loss_out, int64_t* tape_out);
@@ -192,7 +208,7 @@ int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const sr
  * pd_out[r*ns + j] = P_rj when not NULL.  The stored acceleration A/c of each
  * trajectory point (lagged one step, as the reference stores it) is differentiated
  * through the RK4 step that produced it.  No counterpart in the reference. */
-int srs_hip_pole_pd_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle,
+int srs_hip_pole_pd_forward(const srs_field* und, int nund, int poled, const srs_particle* particle,
                             int64_t nreal, const double* deltas, double charge, double current,
                             const double* obs_xyz, const double* normals_xyz, int64_t ns, int directional,
                             const double* target, int checkpoint_every, double* pd_out, double* loss_out,

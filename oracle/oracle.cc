@@ -96,7 +96,43 @@ inline V3 v3(const double* p) { return V3{p[0], p[1], p[2]}; }
 // Ideal undulator field, TField3D_IdealUndulator.cc:50-77 (Init) and :106-139 (GetF).
 // With pole_scales != NULL the 2N interior half-period poles are scaled by
 // (1 + delta_p), p = floor((D - (-L/2 + lambda)) / (lambda/2)) (phase and taper 0).
+// TField3D_Gaussian::GetF (TField3D_Gaussian.cc:88-113) and TField3D_UniformBox::GetF
+// (TField3D_UniformBox.cc:94-117): the position rotated by TVector3D::RotateSelfXYZ
+// (sin and cos per call), the center subtracted, then the Gaussian envelope or the box.
+V3 local_field(const oracle_undulator& u, V3 X) {
+  double const* r = u.rotations;
+  {
+    double const s = std::sin(r[0]), c = std::cos(r[0]), yy = X.y;
+    X.y = c * yy - s * X.z;
+    X.z = s * yy + c * X.z;
+  }
+  {
+    double const s = std::sin(r[1]), c = std::cos(r[1]), zz = X.z;
+    X.z = c * zz - s * X.x;
+    X.x = s * zz + c * X.x;
+  }
+  {
+    double const s = std::sin(r[2]), c = std::cos(r[2]), xx = X.x;
+    X.x = c * xx - s * X.y;
+    X.y = s * xx + c * X.y;
+  }
+  V3 const RX = sub(X, v3(u.center));
+  V3 const F = v3(u.field);
+  double const w[3] = {u.width[0], u.width[1], u.width[2]};
+  double const d[3] = {RX.x, RX.y, RX.z};
+  if (u.kind == 1) {
+    double frac = 1;
+    for (int k = 0; k < 3; ++k)
+      if (w[k] > 0) frac *= std::exp(-std::pow(d[k] / w[k], 2) / 2.);
+    return V3{frac * F.x, frac * F.y, frac * F.z};
+  }
+  for (int k = 0; k < 3; ++k)
+    if (w[k] > 0 && std::fabs(d[k]) > std::fabs(w[k] / 2.)) return V3{0, 0, 0};
+  return F;
+}
+
 V3 undulator_field(const oracle_undulator& u, const double* pole_scales, V3 X) {
+  if (u.kind != 0) return local_field(u, X);
   V3 F = v3(u.field), P = v3(u.period), C = v3(u.center);
   double const lam = std::sqrt(mag2(P));
   V3 const pu = divs(P, std::sqrt(mag2(P)));

@@ -23,6 +23,9 @@ extern "C" {
 #endif
 
 /* One ideal undulator (TField3D_IdealUndulator). */
+/* A field element: kind 0 ideal undulator (field, period, nperiods, phase, taper,
+ * center); kind 1 Gaussian, kind 2 uniform box (field already rotated, width = sigma or
+ * full widths, center, rotations of the position about X, Y, Z). */
 typedef struct oracle_undulator {
   double field[3];
   double period[3];
@@ -30,6 +33,9 @@ typedef struct oracle_undulator {
   double phase;
   double taper;
   double center[3];
+  int kind;
+  double width[3];
+  double rotations[3];
 } oracle_undulator;
 
 /* A field set: ideal undulators, plus optionally one pole-scaled undulator

@@ -17,7 +17,8 @@ I64 = C.c_int64
 
 
 class Undulator(C.Structure):
-    _fields_ = [("field", D * 3), ("period", D * 3), ("nperiods", C.c_int), ("phase", D), ("taper", D), ("center", D * 3)]
+    _fields_ = [("field", D * 3), ("period", D * 3), ("nperiods", C.c_int), ("phase", D), ("taper", D), ("center", D * 3),
+                ("kind", C.c_int), ("width", D * 3), ("rotations", D * 3)]
 
 
 class Fields(C.Structure):
@@ -64,13 +65,17 @@ class FieldSet:
     def __init__(self, undulators=(), pole=None, pole_scales=None):
         self._und = (Undulator * max(1, len(undulators)))()
         for i, u in enumerate(undulators):
-            self._und[i] = Undulator(_v3(u["field"]), _v3(u["period"]), int(u["nperiods"]), float(u.get("phase", 0)),
-                                     float(u.get("taper", 0)), _v3(u.get("center", (0, 0, 0))))
+            # kind 1 (Gaussian) / 2 (uniform box): field (already rotated), width, center, rotations
+            self._und[i] = Undulator(_v3(u["field"]), _v3(u.get("period", (0, 0, 0))), int(u.get("nperiods", 0)),
+                                     float(u.get("phase", 0)), float(u.get("taper", 0)),
+                                     _v3(u.get("center", (0, 0, 0))), int(u.get("kind", 0)),
+                                     _v3(u.get("width", (0, 0, 0))), _v3(u.get("rotations", (0, 0, 0))))
         self.s = Fields()
         self.s.und = self._und
         self.s.nund = len(undulators)
         if pole is not None:
-            self._pole = Undulator(_v3(pole["field"]), _v3(pole["period"]), int(pole["nperiods"]), 0.0, 0.0, _v3((0, 0, 0)))
+            self._pole = Undulator(_v3(pole["field"]), _v3(pole["period"]), int(pole["nperiods"]), 0.0, 0.0, _v3((0, 0, 0)),
+                                   0, _v3((0, 0, 0)), _v3((0, 0, 0)))
             self._scales = np.ascontiguousarray(pole_scales, dtype=np.float64)
             self.s.pole_und = C.pointer(self._pole)
             self.s.pole_scales = _p(self._scales)

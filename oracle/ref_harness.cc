@@ -12,6 +12,8 @@
 // The calls mirror what the Python entry points do:
 //   set_particle_beam        OSCARSSR_Python.cc:1641-1655 (+ sigma :1822-1829)
 //   add_bfield_undulator     OSCARSSR_Python.cc:736-818
+//   add_bfield_gaussian      OSCARSSR_Python.cc:561-636
+//   add_bfield_uniform       OSCARSSR_Python.cc:640-718
 //   calculate_spectrum       OSCARSSR_Python.cc:2215-2351
 //   calculate_flux_rectangle OSCARSSR_Python.cc:3009-3233
 //   calculate_power_density_rectangle OSCARSSR_Python.cc:2587-2807
@@ -19,7 +21,9 @@
 
 #include "OSCARSSR.h"
 #include "TField.h"
+#include "TField3D_Gaussian.h"
 #include "TField3D_IdealUndulator.h"
+#include "TField3D_UniformBox.h"
 #include "TSpectrumContainer.h"
 #include "T3DScalarContainer.h"
 #include "TSurfacePoints_Rectangle.h"
@@ -108,6 +112,24 @@ static void setup(OSCARSSR& osr) {
     double taper = u.size() > 8 ? u[8] : 0;
     TVector3D ctr = u.size() > 11 ? TVector3D(u[9], u[10], u[11]) : TVector3D(0, 0, 0);
     osr.AddMagneticField((TField*)new TField3D_IdealUndulator(TVector3D(u[0], u[1], u[2]), TVector3D(u[3], u[4], u[5]), (int)u[6], ctr, phase, taper));
+  }
+  // Gaussian fields: gauss1=Bx,By,Bz,sx,sy,sz[,cx,cy,cz[,rx,ry,rz]] (then gauss2), and
+  // uniform boxes: box1=Bx,By,Bz[,wx,wy,wz[,cx,cy,cz[,rx,ry,rz]]] (then box2), added
+  // after the undulators in this order; the field sum runs in insertion order.
+  for (int k = 1; k <= 2; ++k) {
+    std::vector<double> g = vec("gauss" + std::to_string(k));
+    if (g.size() < 6) continue;
+    TVector3D ctr = g.size() >= 9 ? TVector3D(g[6], g[7], g[8]) : TVector3D(0, 0, 0);
+    TVector3D rot = g.size() >= 12 ? TVector3D(g[9], g[10], g[11]) : TVector3D(0, 0, 0);
+    osr.AddMagneticField((TField*)new TField3D_Gaussian(TVector3D(g[0], g[1], g[2]), ctr, TVector3D(g[3], g[4], g[5]), rot));
+  }
+  for (int k = 1; k <= 2; ++k) {
+    std::vector<double> b = vec("box" + std::to_string(k));
+    if (b.size() < 3) continue;
+    TVector3D w = b.size() >= 6 ? TVector3D(b[3], b[4], b[5]) : TVector3D(0, 0, 0);
+    TVector3D ctr = b.size() >= 9 ? TVector3D(b[6], b[7], b[8]) : TVector3D(0, 0, 0);
+    TVector3D rot = b.size() >= 12 ? TVector3D(b[9], b[10], b[11]) : TVector3D(0, 0, 0);
+    osr.AddMagneticField((TField*)new TField3D_UniformBox(TVector3D(b[0], b[1], b[2]), w, ctr, rot));
   }
   // Pole-scaled undulator: poles=Bx,By,Bz,Px,Py,Pz,N  scales=d0,d1,...
   std::vector<double> pu = vec("poles");

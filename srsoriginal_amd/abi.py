@@ -163,9 +163,13 @@ BATCH_GRAPH = 1
 BATCH_AUTOTUNE = 2
 
 
-class SrsUndulator(C.Structure):
+FIELD_UNDULATOR, FIELD_GAUSSIAN, FIELD_UNIFORM_BOX = 0, 1, 2
+
+
+class SrsField(C.Structure):
     _fields_ = [("field", _D * 3), ("period", _D * 3), ("nperiods", C.c_int32), ("phase", _D), ("taper", _D),
-                ("center", _D * 3), ("pole_delta", C.c_void_p)]
+                ("center", _D * 3), ("pole_delta", C.c_void_p), ("kind", C.c_int32), ("width", _D * 3),
+                ("rotations", _D * 3)]
 
 
 class SrsParticle(C.Structure):
@@ -178,18 +182,18 @@ def _bind_batch():
     if getattr(L, "_batch_bound", False):
         return L
     L.srs_hip_trajectory_points.argtypes = [C.POINTER(SrsParticle), _I64, C.POINTER(_I64)]
-    L.srs_hip_trajectories.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _P, _I64,
+    L.srs_hip_trajectories.argtypes = [C.POINTER(SrsField), C.c_int, C.POINTER(SrsParticle), _I64, _P, _I64,
                                        C.POINTER(_I64), _P]
-    L.srs_hip_batch_flux.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P, _I64,
+    L.srs_hip_batch_flux.argtypes = [C.POINTER(SrsField), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P, _I64,
                                      _P, _I64, _D, C.c_int, C.c_int, _P]
-    L.srs_hip_batch_power_density.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D,
+    L.srs_hip_batch_power_density.argtypes = [C.POINTER(SrsField), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D,
                                               _P, _P, _I64, C.c_int, _D, C.c_int, _P]
-    L.srs_hip_batch_spectrum.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P,
+    L.srs_hip_batch_spectrum.argtypes = [C.POINTER(SrsField), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P,
                                          _P, _I64, _D, C.c_int, _P]
     L.srs_hip_batch_config.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
-    L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
+    L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsField), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
                                        _D, _P, _I64, _D, _P, C.c_int, _P, _P, C.POINTER(_I64)]
-    L.srs_hip_pole_pd_forward.argtypes = [C.POINTER(SrsUndulator), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P,
+    L.srs_hip_pole_pd_forward.argtypes = [C.POINTER(SrsField), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P,
                                           _D, _D, _P, _P, _I64, C.c_int, _P, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_backward.argtypes = [_I64, _P, _P]
     for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config",
@@ -201,16 +205,20 @@ def _bind_batch():
 
 
 def undulators(specs, keep):
-    """srs_undulator array from dicts (field, period, nperiods[, phase, taper, center, pole_delta])."""
-    arr = (SrsUndulator * len(specs))()
+    """srs_field array from dicts: undulators (field, period, nperiods[, phase, taper,
+    center, pole_delta]) or, with kind FIELD_GAUSSIAN / FIELD_UNIFORM_BOX, (field,
+    width[, center, rotations])."""
+    arr = (SrsField * len(specs))()
     for i, u in enumerate(specs):
         pd = None
         if u.get("pole_delta") is not None:
             d = _f64(u["pole_delta"])
             keep.append(d)
             pd = d.ctypes.data
-        arr[i] = SrsUndulator((_D * 3)(*u["field"]), (_D * 3)(*u["period"]), int(u["nperiods"]), float(u.get("phase", 0)),
-                              float(u.get("taper", 0)), (_D * 3)(*u.get("center", (0, 0, 0))), pd)
+        arr[i] = SrsField((_D * 3)(*u["field"]), (_D * 3)(*u.get("period", (0, 0, 0))), int(u.get("nperiods", 0)),
+                          float(u.get("phase", 0)), float(u.get("taper", 0)), (_D * 3)(*u.get("center", (0, 0, 0))),
+                          pd, int(u.get("kind", FIELD_UNDULATOR)), (_D * 3)(*u.get("width", (0, 0, 0))),
+                          (_D * 3)(*u.get("rotations", (0, 0, 0))))
     return arr
 
 

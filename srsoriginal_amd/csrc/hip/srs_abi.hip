@@ -363,13 +363,29 @@ int build_particles(const srs_particle* p, int64_t np, std::vector<srs::BatchPar
   return SRS_OK;
 }
 
-int stage_undulators(Context& c, const srs_undulator* und, int nund, hipStream_t s) {
+int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) {
   if (nund < 1 || !und) return fail(SRS_ERR_INVALID, "no magnetic field (undulator) given");
   std::vector<srs::DevUndulator> du(nund);
   std::vector<double> poles;
   for (int u = 0; u < nund; ++u) {
     srs::DevUndulator& d = du[u];
-    srs_undulator const& h = und[u];
+    srs_field const& h = und[u];
+    d = srs::DevUndulator{};
+    d.kind = h.kind;
+    d.pole_off = -1;
+    if (h.kind == SRS_FIELD_GAUSSIAN || h.kind == SRS_FIELD_UNIFORM_BOX) {
+      if (h.pole_delta) return fail(SRS_ERR_INVALID, "pole errors apply to undulators only");
+      for (int k = 0; k < 3; ++k) {
+        d.field[k] = h.field[k];
+        d.center[k] = h.center[k];
+        // Gaussian: sigma; box: half width, -1 for an unbounded axis
+        d.w[k] = h.kind == SRS_FIELD_GAUSSIAN ? h.width[k] : (h.width[k] > 0 ? std::fabs(h.width[k] / 2.) : -1.0);
+        d.rc[k] = std::cos(h.rotations[k]);  // the values TVector3D::RotateSelf* computes per call
+        d.rs[k] = std::sin(h.rotations[k]);
+      }
+      continue;
+    }
+    if (h.kind != SRS_FIELD_UNDULATOR) return fail(SRS_ERR_INVALID, "unknown field kind");
     double const pm = std::sqrt(h.period[0] * h.period[0] + h.period[1] * h.period[1] + h.period[2] * h.period[2]);
     if (!(pm > 0)) return fail(SRS_ERR_INVALID, "undulator period must be a nonzero vector");
     bool const poled = h.pole_delta != nullptr;
@@ -539,7 +555,7 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
 }
 
 // ---- pole-error gradient: shared parts of the forward entry points ----------------
-int pole_validate(const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+int pole_validate(const srs_field* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
                   const double* deltas, const double* obs_xyz, int64_t ns, const double* target, int checkpoint_every,
                   double* loss_out, int64_t* tape_out) {
   TRY(check_common(ns, SRS_PRECISION_FP64));
@@ -549,6 +565,9 @@ int pole_validate(const srs_undulator* und, int nund, int poled, const srs_parti
   for (int u = 0; u < nund; ++u)
     if (u != poled && und[u].pole_delta)
       return fail(SRS_ERR_INVALID, "only the differentiated undulator may carry pole errors");
+  for (int u = 0; u < nund; ++u)
+    if (und[u].kind != SRS_FIELD_UNDULATOR)
+      return fail(SRS_ERR_INVALID, "the pole-error gradient supports undulator fields only");
   if (!particle) return fail(SRS_ERR_INVALID, "no particle");
   if (nreal < 1 || nreal > 65536) return fail(SRS_ERR_INVALID, "number of realisations must be in 1..65536");
   if (!deltas || !target || !loss_out || !tape_out || !obs_xyz) return fail(SRS_ERR_INVALID, "NULL argument");
@@ -579,7 +598,7 @@ struct PoleRun {
 // Realisations (copies of the particle, pole_base = r * npoles), fields, deltas,
 // observation points, target, trajectory buffers (with accelerations when with_a)
 // and checkpoints, all queued on s.  Invalidates the tape.
-int pole_stage(Context& c, const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+int pole_stage(Context& c, const srs_field* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
                const double* deltas, const double* obs_xyz, int64_t ns, const double* target, int every, bool with_a,
                PoleRun& pr, hipStream_t s) {
   pr.npoles = 2 * und[poled].nperiods;
@@ -594,7 +613,7 @@ int pole_stage(Context& c, const srs_undulator* und, int nund, int poled, const 
   c.tape.valid = false;
   if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
   {
-    std::vector<srs_undulator> uu(und, und + nund);
+    std::vector<srs_field> uu(und, und + nund);
     uu[poled].pole_delta = deltas;  // pole_off 0 into the realisations' delta array
     TRY(stage_undulators(c, uu.data(), nund, s));
   }
@@ -795,7 +814,7 @@ int srs_hip_trajectory_points(const srs_particle* p, int64_t np, int64_t* total)
   return build_particles(p, np, P, total);
 }
 
-int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double* traj_out,
+int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, int64_t np, double* traj_out,
                          int64_t total, int64_t* offsets_out, double* dt_out) {
   g_err.clear();
   if (!p || np < 1 || !traj_out || !offsets_out || !dt_out) return fail(SRS_ERR_INVALID, "NULL argument");
@@ -828,7 +847,7 @@ int srs_hip_trajectories(const srs_undulator* und, int nund, const srs_particle*
   return SRS_OK;
 }
 
-int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                        double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
                        double weight, int precision, int flags, double* flux_out) {
   g_err.clear();
@@ -960,7 +979,7 @@ int srs_hip_batch_flux(const srs_undulator* und, int nund, const srs_particle* p
   return SRS_OK;
 }
 
-int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                                 double current, const double* obs_xyz, const double* normals_xyz, int64_t ns,
                                 int directional, double weight, int precision, double* pd_out) {
   g_err.clear();
@@ -1004,7 +1023,7 @@ int srs_hip_batch_power_density(const srs_undulator* und, int nund, const srs_pa
   return SRS_OK;
 }
 
-int srs_hip_batch_spectrum(const srs_undulator* und, int nund, const srs_particle* p, int64_t np, double charge,
+int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                            double current, const double obs[3], const double* energies_ev, int64_t ne, double weight,
                            int precision, double* flux_out) {
   g_err.clear();
@@ -1106,7 +1125,7 @@ int srs_hip_synchronize(void* stream) {
 
 // ---- gradient of the flux loss with respect to per-pole field errors ---------------
 
-int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+int srs_hip_pole_forward(const srs_field* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
                          const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
                          double energy_ev, const double* target, int checkpoint_every, double* flux_out,
                          double* loss_out, int64_t* tape_out) {
@@ -1155,7 +1174,7 @@ int srs_hip_pole_forward(const srs_undulator* und, int nund, int poled, const sr
   return SRS_OK;
 }
 
-int srs_hip_pole_pd_forward(const srs_undulator* und, int nund, int poled, const srs_particle* particle,
+int srs_hip_pole_pd_forward(const srs_field* und, int nund, int poled, const srs_particle* particle,
                             int64_t nreal, const double* deltas, double charge, double current,
                             const double* obs_xyz, const double* normals_xyz, int64_t ns, int directional,
                             const double* target, int checkpoint_every, double* pd_out, double* loss_out,

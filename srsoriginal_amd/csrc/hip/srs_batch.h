@@ -14,6 +14,10 @@ namespace srs {
 // reference's divisions by lam into correctly rounded Markstein quotients.
 // pole_off >= 0 selects the per-pole scaled form (center, taper and phase unused:
 // center = taper = PS = 0 there) with deltas poles[pole_off .. pole_off + 2N).
+// One field element, constants formed on the host.  kind 0: ideal undulator (the
+// members up to pole_off); kind 1: Gaussian, kind 2: uniform box, which use field
+// (already rotated), center, w (sigma, or half width |w/2|; <= 0: unbounded axis) and
+// the cosines / sines of the rotation angles applied to the position.
 struct DevUndulator {
   double field[3], u[3], center[3];
   double lam, rlam, taper, ps;
@@ -22,6 +26,9 @@ struct DevUndulator {
   double lo2, hi2;  // outer half period (x0.25): D < lo2 or D > hi2, else x0.75
   int nperiods;
   int pole_off;
+  int kind;
+  double w[3];
+  double rc[3], rs[3];
 };
 
 // Per-particle state and trajectory grid (dt, nb, nf computed on the host with the

@@ -17,10 +17,54 @@ namespace srs {
 
 constexpr double kInvSix = 0x1.5555555555555p-3;  // RN(1/6)
 
-// Field of one undulator at (X, Y, Z), added to B.  poles = this particle's pole
+// Gaussian (kind 1) or uniform box (kind 2) at (X, Y, Z), added to B:
+// TField3D_Gaussian::GetF (.cc:88-113), TField3D_UniformBox::GetF (.cc:94-117), the
+// position rotated about X, then Y, then Z as TVector3D::RotateSelfX/Y/Z.
+__device__ __forceinline__ void field_add_local(DevUndulator const& f, double X, double Y, double Z, double* B) {
+  double x = X, y = Y, z = Z;
+  {
+    double const yy = y;
+    y = f.rc[0] * yy - f.rs[0] * z;
+    z = f.rs[0] * yy + f.rc[0] * z;
+  }
+  {
+    double const zz = z;
+    z = f.rc[1] * zz - f.rs[1] * x;
+    x = f.rs[1] * zz + f.rc[1] * x;
+  }
+  {
+    double const xx = x;
+    x = f.rc[2] * xx - f.rs[2] * y;
+    y = f.rs[2] * xx + f.rc[2] * y;
+  }
+  double const r[3] = {x - f.center[0], y - f.center[1], z - f.center[2]};
+  if (f.kind == 1) {
+    double frac = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (f.w[k] > 0) {
+        double const t = r[k] / f.w[k];
+        frac *= exp(-(t * t) / 2.);
+      }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) B[k] += frac * f.field[k];
+  } else {
+    bool out = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out = out || (f.w[k] > 0 && __builtin_fabs(r[k]) > f.w[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) B[k] += out ? 0.0 : f.field[k];
+  }
+}
+
+// Field of one element at (X, Y, Z), added to B.  poles = this particle's pole
 // deltas (used when f.pole_off >= 0).
 __device__ __forceinline__ void field_add(DevUndulator const& f, const double* __restrict__ poles, double X, double Y,
                                           double Z, double* B) {
+  if (f.kind != 0) {
+    field_add_local(f, X, Y, Z, B);
+    return;
+  }
   double const D = (X - f.center[0]) * f.u[0] + (Y - f.center[1]) * f.u[1] + (Z - f.center[2]) * f.u[2];
   double const T = 1 + D * f.taper;
   bool const inside = !(D > f.hi0 || D < f.lo0);

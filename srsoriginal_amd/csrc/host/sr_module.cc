@@ -117,6 +117,19 @@ PYBIND11_MODULE(_sr, m) {
              }
              e.AddMagneticField(u);
            })
+      .def("add_field",
+           [](srs::OSCARSSR& e, int kind, std::vector<double> field, std::vector<double> width,
+              std::vector<double> rot, std::vector<double> center) {
+             // TField3D_Gaussian / TField3D_UniformBox (field given already rotated)
+             srs::IdealUndulator u;
+             if (kind != 1 && kind != 2) throw std::invalid_argument("unknown field kind");
+             u.kind = kind == 1 ? srs::FieldKind::Gaussian : srs::FieldKind::UniformBox;
+             u.field = v3(field);
+             u.width = v3(width);
+             u.rot = v3(rot);
+             u.center = v3(center);
+             e.AddMagneticField(u);
+           })
       .def("clear_bfields", &srs::OSCARSSR::ClearMagneticFields)
       .def("n_bfields", [](srs::OSCARSSR const& e) { return e.Fields().size(); })
       .def("get_bfield",
@@ -172,6 +185,9 @@ PYBIND11_MODULE(_sr, m) {
                d["taper"] = u.taper;
                d["center"] = std::vector<double>{u.center.x, u.center.y, u.center.z};
                d["pole_delta"] = u.pole_delta.empty() ? py::object(py::none()) : py::object(py::cast(u.pole_delta));
+               d["kind"] = (int)u.kind;
+               d["width"] = std::vector<double>{u.width.x, u.width.y, u.width.z};
+               d["rotations"] = std::vector<double>{u.rot.x, u.rot.y, u.rot.z};
                out.append(d);
              }
              return out;

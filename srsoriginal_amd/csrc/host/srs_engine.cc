@@ -91,6 +91,24 @@ srs_traj view(Trajectory const& t) {
 // ---- field ---------------------------------------------------------------------
 
 Vec3 IdealUndulator::eval(Vec3 const& X) const {
+  if (kind != FieldKind::Undulator) {
+    // TField3D_Gaussian::GetF (.cc:88-113) and TField3D_UniformBox::GetF (.cc:94-117)
+    Vec3 Xr = X;
+    rotate_xyz(Xr, rot);
+    Vec3 const RX = sub(Xr, center);
+    if (kind == FieldKind::Gaussian) {
+      double frac = 1;
+      if (width.x > 0) frac *= std::exp(-std::pow(RX.x / width.x, 2) / 2.);
+      if (width.y > 0) frac *= std::exp(-std::pow(RX.y / width.y, 2) / 2.);
+      if (width.z > 0) frac *= std::exp(-std::pow(RX.z / width.z, 2) / 2.);
+      return Vec3(frac * field.x, frac * field.y, frac * field.z);
+    }
+    if ((width.x > 0 && std::fabs(RX.x) > std::fabs(width.x / 2.)) ||
+        (width.y > 0 && std::fabs(RX.y) > std::fabs(width.y / 2.)) ||
+        (width.z > 0 && std::fabs(RX.z) > std::fabs(width.z / 2.)))
+      return Vec3(0, 0, 0);
+    return field;
+  }
   double const lam = mag(period);
   Vec3 const u = divide(period, mag(period));
   double const len = mag(period) * (nperiods + 2);
@@ -550,7 +568,7 @@ void OSCARSSR::CalculateSpectrum(Vec3 const& obs, SpectrumContainer& S, int npar
     }
     if (!same) throw std::invalid_argument("batched spectrum needs all beams to share charge and current (use batched=0)");
     if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
-    std::vector<srs_undulator> und;
+    std::vector<srs_field> und;
     device_fields(und);
     double const o[3] = {obs.x, obs.y, obs.z};
     std::vector<double> out(S.e.size());
@@ -580,10 +598,10 @@ void add_points(SurfacePoints const& surf, ScalarContainer& C, int dim) {
 }
 }  // namespace
 
-void OSCARSSR::device_fields(std::vector<srs_undulator>& out) const {
+void OSCARSSR::device_fields(std::vector<srs_field>& out) const {
   out.clear();
   for (auto const& u : bfield_.fields()) {
-    srs_undulator d{};
+    srs_field d{};
     d.field[0] = u.field.x;
     d.field[1] = u.field.y;
     d.field[2] = u.field.z;
@@ -597,6 +615,13 @@ void OSCARSSR::device_fields(std::vector<srs_undulator>& out) const {
     d.center[1] = u.center.y;
     d.center[2] = u.center.z;
     d.pole_delta = u.pole_delta.empty() ? nullptr : u.pole_delta.data();
+    d.kind = (int32_t)u.kind;
+    d.width[0] = u.width.x;
+    d.width[1] = u.width.y;
+    d.width[2] = u.width.z;
+    d.rotations[0] = u.rot.x;
+    d.rotations[1] = u.rot.y;
+    d.rotations[2] = u.rot.z;
     out.push_back(d);
   }
 }
@@ -629,7 +654,7 @@ void OSCARSSR::CalculateTrajectoryGPU(Particle& P) const {
   if (P.type.empty()) throw std::out_of_range("particle not initialized.  make sure you have a particle or beam defined");
   if (npoints_ < 2) throw std::out_of_range("number of trajectory points must be >= 2");
   if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
-  std::vector<srs_undulator> und;
+  std::vector<srs_field> und;
   device_fields(und);
   srs_particle const dp = device_particle(P);
   int64_t total = 0;
@@ -686,7 +711,7 @@ void OSCARSSR::CalculateFlux(SurfacePoints const& surf, double energy_ev, Scalar
     }
     if (!same) throw std::invalid_argument("batched flux needs all beams to share charge and current (use batched=0)");
     if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
-    std::vector<srs_undulator> und;
+    std::vector<srs_field> und;
     device_fields(und);
     std::vector<double> out(surf.size());
     check_status(srs_hip_batch_flux(und.data(), (int)und.size(), parts.data(), np, q, cur, surf.xyz.data(),
@@ -747,7 +772,7 @@ void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer&
     if (!same)
       throw std::invalid_argument("batched power density needs all beams to share charge and current (use batched=0)");
     if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
-    std::vector<srs_undulator> und;
+    std::vector<srs_field> und;
     device_fields(und);
     std::vector<double> out(surf.size());
     check_status(srs_hip_batch_power_density(und.data(), (int)und.size(), parts.data(), np, q, cur, surf.xyz.data(),

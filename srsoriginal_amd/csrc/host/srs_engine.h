@@ -31,11 +31,18 @@ struct Vec3 {
 // Ideal sine undulator with 0.25/0.75 end terminations
 // (TField3D_IdealUndulator.cc:50-77, :106-139).  Optional per-pole scales
 // (1 + delta_p) for the 2N interior half-period poles.
+// A magnetic field element: ideal undulator (TField3D_IdealUndulator), 3-D Gaussian
+// (TField3D_Gaussian) or uniform box (TField3D_UniformBox).
+enum class FieldKind { Undulator = 0, Gaussian = 1, UniformBox = 2 };
+
 struct IdealUndulator {
+  FieldKind kind = FieldKind::Undulator;
   Vec3 field, period, center;
   int nperiods = 0;
   double phase = 0, taper = 0;
   std::vector<double> pole_delta;  // empty = ideal
+  Vec3 width;                      // Gaussian sigma / box full width (<= 0: unbounded axis)
+  Vec3 rot;                        // Gaussian / box: rotation of the position, X then Y then Z
   Vec3 eval(Vec3 const& X) const;
 };
 
@@ -189,7 +196,7 @@ class OSCARSSR {
 
  private:
   void ensure_particle();
-  void device_fields(std::vector<srs_undulator>& out) const;
+  void device_fields(std::vector<srs_field>& out) const;
   srs_particle device_particle(Particle const& p) const;
   void flux_one(Particle& p, SurfacePoints const& surf, double energy_ev, ScalarContainer& C, double weight,
                 Precision prec);

@@ -147,6 +147,24 @@ class sr:
         self._e.add_undulator(field, per, int(nperiods), float(phase), float(taper), trans,
                               [float(x) for x in (pole_scales if pole_scales is not None else [])])
 
+    def add_bfield_gaussian(self, bfield, sigma, rotations=None, translation=None):
+        """OSCARSSR_AddMagneticFieldGaussian (OSCARSSR_Python.cc:561-636): bfield rotated by
+        rotations, centred at translation; axes with sigma <= 0 are not bounded."""
+        field = _vec3(bfield, "bfield")
+        sig = _vec3(sigma, "sigma")
+        rot = _opt3(rotations, "rotations")
+        trans = _opt3(translation, "translation")
+        self._e.add_field(1, _rotate_xyz(field, rot), sig, rot, trans)
+
+    def add_bfield_uniform(self, bfield, width=None, rotations=None, translation=None):
+        """OSCARSSR_AddMagneticFieldUniform (OSCARSSR_Python.cc:640-718): bfield inside the
+        box of the given full widths (axes with width <= 0, or no width, unbounded)."""
+        field = _vec3(bfield, "bfield")
+        w = _opt3(width, "width")
+        rot = _opt3(rotations, "rotations")
+        trans = _opt3(translation, "translation")
+        self._e.add_field(2, _rotate_xyz(field, rot), w, rot, trans)
+
     def get_bfield(self, x):
         return self._e.get_bfield(_vec3(x, "x"))
 

@@ -2,6 +2,7 @@
 and for the product, so every parity test reads the same configuration the
 reference was run with."""
 import json
+import math
 import os
 
 import numpy as np
@@ -28,7 +29,24 @@ def vec(a, key, default=None):
     return [float(x) for x in a[key].split(",")]
 
 
+def _rotate_xyz(v, r):
+    # TVector3D::RotateSelfXYZ, as the reference's field constructors apply it to the field
+    x, y, z = v
+    for axis, ang in enumerate(r):
+        s, c = math.sin(ang), math.cos(ang)
+        if axis == 0:
+            y, z = c * y - s * z, s * y + c * z
+        elif axis == 1:
+            z, x = c * z - s * x, s * z + c * x
+        else:
+            x, y = c * x - s * y, s * x + c * y
+    return [x, y, z]
+
+
 def undulators(a):
+    """Field elements of a case in the harness's order: und1..4, gauss1..2, box1..2.
+    Gaussian / box entries carry kind 1 / 2, the raw field (rot_field: rotated as the
+    constructors do), width, center and rotations."""
     out = []
     for k in ("und1", "und2", "und3", "und4"):
         u = vec(a, k)
@@ -36,6 +54,16 @@ def undulators(a):
             continue
         out.append(dict(field=u[0:3], period=u[3:6], nperiods=int(u[6]), phase=u[7] if len(u) > 7 else 0.0,
                         taper=u[8] if len(u) > 8 else 0.0, center=u[9:12] if len(u) > 11 else [0.0, 0.0, 0.0]))
+    for kind, keys, lo in ((1, ("gauss1", "gauss2"), 6), (2, ("box1", "box2"), 3)):
+        for k in keys:
+            g = vec(a, k)
+            if not g or len(g) < lo:
+                continue
+            w = g[3:6] if len(g) >= 6 else [0.0, 0.0, 0.0]
+            c = g[6:9] if len(g) >= 9 else [0.0, 0.0, 0.0]
+            r = g[9:12] if len(g) >= 12 else [0.0, 0.0, 0.0]
+            out.append(dict(kind=kind, raw_field=g[0:3], field=_rotate_xyz(g[0:3], r), width=w, center=c,
+                            rotations=r))
     return out
 
 

@@ -262,6 +262,27 @@ def multiflux_c3_small():
     return args, {"container": container(a)}, dt
 
 
+# Undulator + Gaussian kicker (sigma along z only, rotated) + rotated uniform box
+# (bounded along z only): TField3D_Gaussian.cc / TField3D_UniformBox.cc GetF, summed
+# in insertion order (TFieldContainer.cc:107-117).
+FIELDS_MIXED = ["und1=" + UND_C0, "gauss1=0,0.01,0,0,0,0.05,0,0,1.0,0,0.1,0",
+                "box1=0.005,0,0.002,0,0,0.3,0,0,-1.0,0.05,0,0.02"]
+
+
+@case
+def traj_fields_mixed():
+    args = FIELDS_MIXED + BEAM + ["ct=-0.5,3", "npoints=4001"]
+    a, dt = run("traj", args)
+    n = int(a[0])
+    return args, {"n": np.array(n), "dt": np.array(a[1]), "traj": a[2:].reshape(9, n)}, dt
+
+
+@case
+def flux_fields_mixed():
+    args = FIELDS_MIXED + BEAM + ["ct=-0.5,3", "npoints=4001", "plane=XY", "width=0.01,0.01", "nx=9,9", "trans=0.002,0,30", "energy_ev=143.52176088044024", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
 @case
 def multispectrum_c3_small():
     # Gaussian beam, 4 particles: every route adds each particle's spectrum x 1/N

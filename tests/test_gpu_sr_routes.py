@@ -152,3 +152,61 @@ def test_multi_particle_spectrum_routes_match_reference(batched):
     err = normwise(vals[:, 1], ref[:, 1])
     say("  normwise vs reference %.3e" % err)
     assert err <= 1e-9
+
+
+def _fields_engine():
+    from srsoriginal_amd.sr import sr
+    a = cases.args("flux_fields_mixed")
+    osr = sr()
+    for u in cases.undulators(a):
+        if u.get("kind") == 1:
+            osr.add_bfield_gaussian(bfield=u["raw_field"], sigma=u["width"], rotations=u["rotations"],
+                                    translation=u["center"])
+        elif u.get("kind") == 2:
+            osr.add_bfield_uniform(bfield=u["raw_field"], width=u["width"], rotations=u["rotations"],
+                                   translation=u["center"])
+        else:
+            osr.add_bfield_undulator(bfield=u["field"], period=u["period"], nperiods=u["nperiods"])
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    c0, c1, n = cases.ct(a)
+    osr.set_ctstartstop(c0, c1)
+    osr.set_npoints_trajectory(n)
+    return osr, a
+
+
+def test_device_trajectory_gaussian_and_box_fields():
+    say("device RK4 with undulator + rotated Gaussian + rotated uniform box vs the host RK4 (both legs)")
+    osr, a = _fields_engine()
+    osr.calculate_trajectory()
+    th, _ = osr.trajectory_array()
+    osr.calculate_trajectory(gpu=1)
+    td, _ = osr.trajectory_array()
+    for k, scale in ((0, 1e-4), (1, 1e-4), (2, 1.0), (3, 1e-3), (4, 1e-3), (5, 1.0)):
+        err = np.max(np.abs(td[k] - th[k])) / scale
+        say("  component %d max deviation / scale %.3e" % (k, err))
+        assert err < 1e-12
+
+
+def test_flux_gaussian_and_box_fields_routes_match_reference():
+    from srsoriginal_amd import abi
+    say("flux 9x9 with undulator + Gaussian + box fields: host-RK4 route and device-RK4 batch vs the reference")
+    osr, a = _fields_engine()
+    ref = golden("flux_fields_mixed")["container"][:81, 3]
+    out = osr.calculate_flux_rectangle(energy_eV=float(a["energy_ev"]), npoints=[9, 9], plane="XY",
+                                       width=[0.01, 0.01], translation=[0.002, 0, 30], gpu=1)
+    v = np.array([x for _, x in out])[:81]
+    err = normwise(v, ref)
+    say("  per-particle route normwise %.3e" % err)
+    assert err <= 1e-10
+    st = osr._e.particle_state()
+    row = dict(x0=st["x0"], beta0=st["beta0"], qmg=st["qmg"], t0=st["t0"], ct_start=cases.ct(a)[0],
+               ct_stop=cases.ct(a)[1], npoints=cases.ct(a)[2])
+    q, cur = osr._e.particle_charge_current()
+    from srsoriginal_amd.lib import _sr
+    xyz = _sr.Surface.rectangle("XY", 9, 9, 0.01, 0.01, [0, 0, 0], [0.002, 0, 30], 0).xyz
+    for prec, tol in ((abi.FP64, 1e-9), (abi.MIXED, 1e-5)):
+        got = abi.batch_flux(osr._e.undulators(), abi.particles([row]), q, cur, xyz, [float(a["energy_ev"])], 1.0,
+                             prec, 0)[0]
+        err = normwise(got, ref)
+        say("  device RK4 batch (%s) normwise %.3e" % ("fp64" if prec == abi.FP64 else "mixed", err))
+        assert err <= tol

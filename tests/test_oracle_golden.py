@@ -4,6 +4,7 @@ oracle follows the reference operation order and is compiled with the same
 flags, so any plausible slip (a dropped term, a swapped index, a different
 rounding order) shows up as a mismatch."""
 import hashlib
+import math
 
 import numpy as np
 
@@ -336,3 +337,20 @@ def test_pole_pd_loss_gradient_matches_reference_central_differences(oracle):
         dm = d.copy(); dm[k] -= h
         fdk = (L(dp) - L(dm)) / (2 * h)
         assert abs(grad[k] - fdk) <= 5e-4 * scale, (k, grad[k], fdk)
+
+
+def test_trajectory_gaussian_and_box_fields_bitwise(oracle):
+    """Oracle fields of every kind (TField3D_Gaussian, TField3D_UniformBox, rotated,
+    summed after the undulator) against the reference trajectory, both legs."""
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_fields_mixed")
+    g = golden("traj_fields_mixed")
+    assert dt == float(g["dt"])
+    ref = g["traj"]
+    assert t.shape == ref.shape
+    assert np.array_equal(t[:6], ref[:6])
+    nb = int(np.argmin(np.abs(t[2] - (-1.5))))
+    assert np.array_equal(np.delete(t[6:], nb, axis=1), np.delete(ref[6:], nb, axis=1))
+    # closed forms: the Gaussian integrates to B sigma sqrt(2 pi) (kick), the box to B w
+    gam = p.gamma
+    kick_x = 0.3 * 0.01 * 0.05 * math.sqrt(2 * math.pi) / 3.0 * (1 - 1 / (2 * gam ** 2))
+    assert abs(t[3, -1] - kick_x) < 0.02 * kick_x
