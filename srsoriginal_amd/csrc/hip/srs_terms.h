@@ -334,15 +334,16 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
     for (int kk = 0; kk < kSub; ++kk) {
       ParaPoint<NE> const& r = P.p[k0 + kk];
       double const s = __builtin_fma(mx, r.x, __builtin_fma(my, r.y, asq + r.bsq));
-      double const s2 = -(s * s);
+      // D - rz = s (h - t3 s)  (times W for one energy, folded into h and t3)
+      double const hs = __builtin_fma(-r.t3, s, r.h);
       float const az = __builtin_fmaf((float)s, r.gh, -r.ombr);
       double u = 0.0;
-      if constexpr (NE > 1) u = __builtin_fma(s2, r.t3, s * r.h);  // D - rz
+      if constexpr (NE > 1) u = s * hs;
 #pragma unroll
       for (int e = 0; e < NE; ++e) {
         // revolutions W (D - rz) + Phi_k, reduced to one turn
         double p;
-        if constexpr (NE == 1) p = __builtin_fma(s, r.h, __builtin_fma(s2, r.t3, r.phi[0]));
+        if constexpr (NE == 1) p = __builtin_fma(s, hs, r.phi[0]);
         else p = __builtin_fma(es.fw[e], u, r.phi[e]);
         // v_fract_f64 is the faster reduction with several energies, p - rint(p) with one
         // (measured); either is exact
