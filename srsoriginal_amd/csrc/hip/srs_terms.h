@@ -156,23 +156,29 @@ __device__ __forceinline__ void flux_terms_mixed(FluxTile<TILE> const& T, int cn
 //   n/D^2            = (rx/rz^3, ry/rz^3, 1/rz^2) (1 + O(eps)).
 // m varies by < eps*|d rz|/rz over a 32-point sub-tile; it scales the sub-tile sums.
 // A block takes this form only when every (point, trajectory point) pair it
-// integrates has eps <= kParaEps (relative amplitude error O(eps)) and
-// W*rz*eps^3/16 <= kParaPhase (dropped phase term, revolutions); otherwise it
+// integrates has eps <= kParaEps (relative amplitude error O(eps)),
+// W*rz*eps^3/16 <= kParaPhase (dropped phase term, revolutions) and
+// (c/omega)/rz <= kParaNear (x, y near field below FP32 resolution); otherwise it
 // integrates with flux_terms_mixed.
 // ---------------------------------------------------------------------------
 constexpr double kParaEps = 1e-6;
 constexpr double kParaPhase = 1e-9;
+// x, y near field: the amplitude (beta - n)/D - i (c/omega) n/D^2 of the x, y axes is
+// beta/D - (n/D)(1 + i (c/omega)/D); the block drops the factor i (c/omega)/D when it
+// is below kParaNear = 1e-9 (~2^-30, under FP32 resolution 2^-24: in FP32 it would
+// round away).  The z near field -i (c/omega) n_z/D^2 is kept (the z far field
+// (beta_z - n_z)/D is itself ~1e-8 of the x, y one).
+constexpr double kParaNear = 1e-9;
 
 // One staged trajectory point of the paraxial form (one LDS record, so a term
 // reads it from a single base address).
 //
-// With rx = ox - x the x amplitude splits into a part of the trajectory point and the
-// observation coordinate times a part shared by x and y:
-//   (beta_x/rz - rx/rz^2) - i cw rx/rz^3 = (ax + i cw gx) - ox (q + i cw 2 gh),
-//   ax = beta_x/rz + x/rz^2,  gx = x/rz^3,  q = 1/rz^2,  2 gh = 1/rz^3,
+// With rx = ox - x the x far-field amplitude splits into a part of the trajectory
+// point and the observation coordinate times a part shared by x and y:
+//   beta_x/rz - rx/rz^2 = ax - ox q,  ax = beta_x/rz + x/rz^2,  q = 1/rz^2,
 // and the z near-field amplitude -i cw/rz^2 is the same q: a term accumulates
-// sum ax e, sum gx e, sum ay e, sum gy e, sum q e, sum gh e and sum az e (e = e^{i phi}),
-// and the observation coordinates and c/omega are applied per sub-tile.  s comes from
+// sum ax e, sum ay e, sum q e and sum az e (e = e^{i phi}), and the observation
+// coordinates and c/omega are applied per sub-tile.  s comes from
 // s = (ox^2 + oy^2) + (x^2 + y^2) - 2 (ox x + oy y) (absolute error ~1e-20 m^2 on
 // these screens, 1e-13 rev of phase).
 template <int NE>
@@ -183,7 +189,6 @@ struct alignas(16) ParaPoint {
   double t3;          // 1 / (8 rz^3)    (one energy: W / (8 rz^3))
   double phi[NE];     // Phi_k per energy, revolutions in [-0.5, 0.5]
   float ax, ay;       // beta_x / rz + x / rz^2, beta_y / rz + y / rz^2
-  float gx, gy;       // x / rz^3, y / rz^3
   float ombr;         // (1 - beta_z) / rz
   float q;            // 1 / rz^2
   float gh;           // 1 / (2 rz^3)
@@ -238,12 +243,14 @@ __device__ __forceinline__ ParaBlock para_block(double ox, double oy, double oz,
 
 // True when trajectory point (xk, yk, zk) satisfies the expansion bounds for every
 // observation point of the block.
-__device__ __forceinline__ bool para_point_ok(ParaBlock const& b, double xk, double yk, double zk, double fwmax) {
+__device__ __forceinline__ bool para_point_ok(ParaBlock const& b, double xk, double yk, double zk, double fwmax,
+                                              double cwmax) {
   double const rz = b.z - zk;
   double const dx = __builtin_fmax(__builtin_fabs(b.xlo - xk), __builtin_fabs(b.xhi - xk));
   double const dy = __builtin_fmax(__builtin_fabs(b.ylo - yk), __builtin_fabs(b.yhi - yk));
   double const eps = __builtin_fma(dx, dx, dy * dy) / (rz * rz);
-  return rz > 0.0 && eps <= kParaEps && fwmax * rz * (eps * eps * eps) <= 16.0 * kParaPhase;
+  return rz > 0.0 && eps <= kParaEps && fwmax * rz * (eps * eps * eps) <= 16.0 * kParaPhase &&
+         cwmax <= kParaNear * rz;  // D >= rz
 }
 
 // Block-uniform decision for trajectory points [i0, i1) of arrays x, y, z: every
@@ -252,11 +259,15 @@ __device__ __forceinline__ bool para_point_ok(ParaBlock const& b, double xk, dou
 __device__ __forceinline__ bool para_check(ParaBlock const& b, const double* __restrict__ x,
                                            const double* __restrict__ y, const double* __restrict__ z, int64_t i0,
                                            int64_t i1, EnergySet const& es) {
-  double fwmax = 0.0;
-  for (int e = 0; e < es.ne; ++e) fwmax = __builtin_fmax(fwmax, es.fw[e]);
+  double fwmax = 0.0, cwmax = 0.0;
+  for (int e = 0; e < es.ne; ++e) {
+    fwmax = __builtin_fmax(fwmax, es.fw[e]);
+    cwmax = __builtin_fmax(cwmax, es.cw[e]);
+  }
   bool ok = b.flat;
   if (ok)
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock) ok = ok && para_point_ok(b, x[i], y[i], z[i], fwmax);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock)
+      ok = ok && para_point_ok(b, x[i], y[i], z[i], fwmax, cwmax);
   return __syncthreads_and(ok) != 0;
 }
 
@@ -278,8 +289,6 @@ __device__ __forceinline__ void para_stage(ParaTile<TILE, NE>& P, int k, double 
   double const q = ir * ir;
   r.ax = (float)__builtin_fma(xk, q, (double)fbx * ir);
   r.ay = (float)__builtin_fma(yk, q, (double)fby * ir);
-  r.gx = (float)(xk * q * ir);
-  r.gy = (float)(yk * q * ir);
   r.ombr = fomb * irf;
   r.q = (float)q;
   r.gh = (float)(0.5 * q * ir);
@@ -307,13 +316,14 @@ __device__ __forceinline__ void para_pad(ParaTile<TILE, NE>& P, int k) {
   r.x = r.y = r.bsq = r.h = r.t3 = 0.0;
 #pragma unroll
   for (int e = 0; e < NE; ++e) r.phi[e] = 0.0;
-  r.ax = r.ay = r.gx = r.gy = r.ombr = r.q = r.gh = 0.f;
+  r.ax = r.ay = r.ombr = r.q = r.gh = 0.f;
 }
 
-// Near-field sums (the -i (c/omega) n/D^2 part) are kept apart within a sub-tile and
-// folded in with c/omega at its end, so a term needs no per-energy scaling.  cnt is a
-// multiple of kSub (the tile is padded with zero-amplitude records), so the sub-tile
-// loop has a fixed trip count.
+// The z near-field sum (-i (c/omega) n_z/D^2, through sum q e) is folded in with
+// c/omega at the sub-tile's end, so a term needs no per-energy scaling; the x, y near
+// field is below kParaNear on these blocks (see above).  cnt is a multiple of kSub (the
+// tile is padded with zero-amplitude records), so the sub-tile loop has a fixed trip
+// count.
 template <int NE, int TILE>
 __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int cnt, double ox, double oy,
                                                 EnergySet const& es, const float* cwf, AccMixed<NE>& a) {
@@ -321,13 +331,12 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
   double const mx = -2.0 * ox, my = -2.0 * oy;
   float const oxf = (float)ox, oyf = (float)oy;
   for (int k0 = 0; k0 < cnt; k0 += kSub) {
-    // per energy: [0..3] sum ax c, ax s, gx s, gx c; [4..7] the same for y;
-    // [8..11] sum q c, q s, gh s, gh c; [12, 13] sum az c, az s
-    float sub[NE][14];
+    // per energy: sum ax c, ax s, ay c, ay s, q c, q s, az c, az s
+    float sub[NE][8];
 #pragma unroll
     for (int e = 0; e < NE; ++e)
 #pragma unroll
-      for (int q = 0; q < 14; ++q) sub[e][q] = 0.f;
+      for (int q = 0; q < 8; ++q) sub[e][q] = 0.f;
     // kept rolled: unrolling raises the VGPR count and the lost occupancy costs more
     // than the loop overhead it saves (measured)
 #pragma unroll 1
@@ -354,18 +363,12 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
         float* const b = sub[e];
         b[0] = __builtin_fmaf(r.ax, cn, b[0]);
         b[1] = __builtin_fmaf(r.ax, sn, b[1]);
-        b[2] = __builtin_fmaf(r.gx, sn, b[2]);
-        b[3] = __builtin_fmaf(r.gx, cn, b[3]);
-        b[4] = __builtin_fmaf(r.ay, cn, b[4]);
-        b[5] = __builtin_fmaf(r.ay, sn, b[5]);
-        b[6] = __builtin_fmaf(r.gy, sn, b[6]);
-        b[7] = __builtin_fmaf(r.gy, cn, b[7]);
-        b[8] = __builtin_fmaf(r.q, cn, b[8]);
-        b[9] = __builtin_fmaf(r.q, sn, b[9]);
-        b[10] = __builtin_fmaf(r.gh, sn, b[10]);
-        b[11] = __builtin_fmaf(r.gh, cn, b[11]);
-        b[12] = __builtin_fmaf(az, cn, b[12]);
-        b[13] = __builtin_fmaf(az, sn, b[13]);
+        b[2] = __builtin_fmaf(r.ay, cn, b[2]);
+        b[3] = __builtin_fmaf(r.ay, sn, b[3]);
+        b[4] = __builtin_fmaf(r.q, cn, b[4]);
+        b[5] = __builtin_fmaf(r.q, sn, b[5]);
+        b[6] = __builtin_fmaf(az, cn, b[6]);
+        b[7] = __builtin_fmaf(az, sn, b[7]);
       }
     }
     // m = 1 - eps/2 at the sub-tile's first point (always a trajectory point)
@@ -375,17 +378,14 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       float const* const b = sub[e];
-      float const cw = cwf[e], cw2 = 2.f * cwf[e];
-      // shared part (q + i cw / rz^3) e^{i phi}; field sums re/im of x, y, z:
-      //   x = sum (ax + i cw gx) e - ox T,  z = sum (az - i cw q) e
-      float const tre = __builtin_fmaf(-cw2, b[10], b[8]);
-      float const tim = __builtin_fmaf(cw2, b[11], b[9]);
-      a.q[e][0].add(__builtin_fmaf(-oxf, tre, __builtin_fmaf(-cw, b[2], b[0])) * m);
-      a.q[e][1].add(__builtin_fmaf(-oxf, tim, __builtin_fmaf(cw, b[3], b[1])) * m);
-      a.q[e][2].add(__builtin_fmaf(-oyf, tre, __builtin_fmaf(-cw, b[6], b[4])) * m);
-      a.q[e][3].add(__builtin_fmaf(-oyf, tim, __builtin_fmaf(cw, b[7], b[5])) * m);
-      a.q[e][4].add(__builtin_fmaf(cw, b[9], b[12]) * m);
-      a.q[e][5].add(__builtin_fmaf(-cw, b[8], b[13]) * m);
+      float const cw = cwf[e];
+      // field sums re/im of x, y, z:  x = sum ax e - ox sum q e,  z = sum (az - i cw q) e
+      a.q[e][0].add(__builtin_fmaf(-oxf, b[4], b[0]) * m);
+      a.q[e][1].add(__builtin_fmaf(-oxf, b[5], b[1]) * m);
+      a.q[e][2].add(__builtin_fmaf(-oyf, b[4], b[2]) * m);
+      a.q[e][3].add(__builtin_fmaf(-oyf, b[5], b[3]) * m);
+      a.q[e][4].add(__builtin_fmaf(cw, b[5], b[6]) * m);
+      a.q[e][5].add(__builtin_fmaf(-cw, b[4], b[7]) * m);
     }
   }
 }
