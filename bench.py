@@ -37,7 +37,7 @@ PEAK_FP32_NONPACKED = 78.6e12
 # HBM bytes per launch of flux_mixed_kernel<1,256> on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
 # what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
-TRAFFIC_MIXED_BYTES = 32747.1 * 1024
+TRAFFIC_MIXED_BYTES = 32754.9 * 1024
 
 
 def log(msg):
