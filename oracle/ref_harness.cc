@@ -222,6 +222,8 @@ int main(int argc, char** argv) {
         S.Init((size_t)num("ne", 100), er[0], er[1]);
       }
       osr.CalculateSpectrum(v3("obs", TVector3D(0, 0, 30)), S, nparticles, nthreads, 0);
+      // file output as calculate_spectrum(ofile=...) writes it (OSCARSSR_Python.cc:2340-2343)
+      if (gArgs.count("ofile")) S.WriteToFileText(str("ofile", ""));
       put((double)S.GetNPoints());
       for (size_t i = 0; i < S.GetNPoints(); ++i) {
         put(S.GetEnergy(i));
@@ -237,7 +239,7 @@ int main(int argc, char** argv) {
         osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, 3, "");
       } else {
         TSurfacePoints_Rectangle S = rect();
-        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, (int)num("dim", 2), "");
+        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, (int)num("dim", 2), str("ofile", ""));
       }
       write_container(C);
     } else if (mode == "pd") {
@@ -250,11 +252,39 @@ int main(int argc, char** argv) {
         osr.CalculatePowerDensity(S, C, 3, directional, nparticles, "", nthreads, 0);
       } else {
         TSurfacePoints_Rectangle S = rect();
-        osr.CalculatePowerDensity(S, C, (int)num("dim", 2), directional, nparticles, "", nthreads, 0);
+        osr.CalculatePowerDensity(S, C, (int)num("dim", 2), directional, nparticles, str("ofile", ""), nthreads, 0);
       }
       write_container(C);
     } else if (mode == "totalpower") {
       put(osr.CalculateTotalPower());
+    } else if (mode == "avg3d") {
+      // average_flux / average_power_density (OSCARSSR_Python.cc:3475-3563):
+      // ifiles=a,b,...  binary=0|1  dim=2|3  [ofile=, bofile=]
+      std::vector<std::string> files;
+      std::stringstream ss(str("ifiles", ""));
+      std::string tok;
+      while (std::getline(ss, tok, ',')) files.push_back(tok);
+      int const dim = (int)num("dim", 2);
+      T3DScalarContainer C;
+      if (num("binary", 0) != 0) C.AverageFromFilesBinary(files, dim);
+      else C.AverageFromFilesText(files, dim);
+      if (gArgs.count("ofile")) C.WriteToFileText(str("ofile", ""), dim);
+      if (gArgs.count("bofile")) C.WriteToFileBinary(str("bofile", ""), dim);
+      write_container(C);
+    } else if (mode == "avgspec") {
+      // average_spectra (OSCARSSR_Python.cc:3252-3318), text files: ifiles=a,b,... [ofile=]
+      std::vector<std::string> files;
+      std::stringstream ss(str("ifiles", ""));
+      std::string tok;
+      while (std::getline(ss, tok, ',')) files.push_back(tok);
+      TSpectrumContainer S;
+      S.AverageFromFilesText(files);
+      if (gArgs.count("ofile")) S.WriteToFileText(str("ofile", ""));
+      put((double)S.GetNPoints());
+      for (size_t i = 0; i < S.GetNPoints(); ++i) {
+        put(S.GetEnergy(i));
+        put(S.GetFlux(i));
+      }
     } else if (mode == "beam") {
       // layout: count, then count x (x0[3], beta0[3], E0, gamma, q/m/gamma)
       int count = (int)num("count", 8);

@@ -21,6 +21,8 @@ import sys
 
 import numpy as np
 
+from . import containers
+
 try:
     from .lib import _sr
 except ImportError as e:  # no silent fallback: the native engine is required
@@ -288,9 +290,9 @@ class sr:
         e, f = self._e.calculate_spectrum(o, pts, int(npoints), e0, e1, int(nparticles), int(nthreads), int(gpu),
                                           _prec(precision), bool(batched))
         if ofile:
-            with open(ofile, "w") as fh:
-                for a, b in zip(e.tolist(), f.tolist()):
-                    fh.write("%.17e %.17e\n" % (a, b))
+            containers.write_spectrum_text(ofile, e, f)
+        if bofile:
+            containers.write_spectrum_binary(bofile, e, f)
         return [[a, b] for a, b in zip(e.tolist(), f.tolist())]
 
     # ---- flux (OSCARSSR_Python.cc:2829-2996, :3009-3233) --------------------------
@@ -352,6 +354,46 @@ class sr:
         if self._e.n_beams() < 1:
             raise ValueError("No particle beam defined")
         return self._e.calculate_total_power()
+
+    # ---- container files (OSCARSSR_Python.cc:3252-3318, :3475-3563) -------------
+    @staticmethod
+    def _files(ifiles, bifiles):
+        ti = list(ifiles or [])
+        bi = list(bifiles or [])
+        if ti and bi:
+            raise ValueError("either text or binary files may be added, but not both.")
+        if not ti and not bi:
+            raise ValueError("No files given.  You need at least one file as input in a list.")
+        return ti, bi
+
+    def average_spectra(self, ifiles=None, bifiles=None, ofile="", bofile=""):
+        """OSCARSSR_AverageSpectra: average of text spectrum files.  As in the reference,
+        binary inputs are not read (the call returns an empty spectrum)."""
+        ti, bi = self._files(ifiles, bifiles)
+        if ti:
+            e, f = containers.average_spectrum_files(ti)
+        else:
+            e, f = np.zeros(0), np.zeros(0)
+        if ofile:
+            containers.write_spectrum_text(ofile, e, f)
+        if bofile:
+            containers.write_spectrum_binary(bofile, e, f)
+        return [[a, b] for a, b in zip(e.tolist(), f.tolist())]
+
+    def average_flux(self, ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
+        """OSCARSSR_AverageT3DScalars: average of flux (or power-density) files, text or
+        binary, written as text (ofile) and binary (bofile)."""
+        ti, bi = self._files(ifiles, bifiles)
+        xyz, v = containers.average_scalar_files(ti or bi, binary=not ti, dim=int(dim))
+        if ofile:
+            containers.write_scalar_text(ofile, xyz, v, int(dim))
+        if bofile:
+            containers.write_scalar_binary(bofile, xyz, v, int(dim))
+        return _container_list(xyz, v)
+
+    def average_power_density(self, ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
+        """OSCARSSR_AverageT3DScalars, as average_flux."""
+        return self.average_flux(ifiles, bifiles, ofile, bofile, dim)
 
     # ---- engine-level accumulators (OSCARSSR_Python.cc:3330-3470) ----------------
     def add_to_spectrum(self, spectrum, weight=1.0):
@@ -434,12 +476,7 @@ class sr:
 
     def _out(self, xyz, v, ofile, dim):
         if ofile:
-            with open(ofile, "w") as fh:
-                for (a, b, c), val in zip(xyz[:len(v)].tolist(), v.tolist()):
-                    if dim == 2:
-                        fh.write("%.17e %.17e %.17e\n" % (a, b, val))
-                    else:
-                        fh.write("%.17e %.17e %.17e %.17e\n" % (a, b, c, val))
+            containers.write_scalar_text(ofile, xyz, v, dim)
         return _container_list(xyz, v)
 
 

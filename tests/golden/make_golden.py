@@ -18,6 +18,7 @@ import hashlib
 import json
 import os
 import platform
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -299,6 +300,41 @@ def multipd_c3_small():
     args = ["und1=" + UND_C0] + BEAM_C3 + ["seed=777", "ct=0,3", "npoints=3001", "plane=XY", "width=0.04,0.02", "nx=9,5", "trans=0,0,30", "nparticles=4", "nthreads=2"]
     a, dt = run("pd", args)
     return args, {"container": container(a)}, dt
+
+def _bytes(path):
+    with open(path, "rb") as f:
+        return np.frombuffer(f.read(), dtype=np.uint8).copy()
+
+
+@case
+def files_and_averages():
+    """The reference's text / binary container files and its averaging of them:
+    calculate_flux_rectangle / calculate_power_density_rectangle / calculate_spectrum
+    with ofile (T3DScalarContainer::WriteToFileText, TSpectrumContainer::WriteToFileText),
+    average_flux over two text files and over two binary files (AverageFromFilesText /
+    AverageFromFilesBinary, Kahan sums of V/N) and average_spectra over two spectra.
+    The fixture holds the files' bytes (inputs and outputs) and the averaged containers."""
+    d = tempfile.mkdtemp()
+    P = lambda n: os.path.join(d, n)
+    base = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "plane=XY", "width=0.01,0.008", "nx=5,4", "trans=0,0,30", "nthreads=2"]
+    t0 = time.time()
+    run("flux", base + ["energy_ev=143.5", "ofile=" + P("f1.txt")])
+    run("flux", base + ["energy_ev=150.0", "ofile=" + P("f2.txt")])
+    run("pd", base + ["ofile=" + P("pd.txt")])
+    a_txt, _ = run("avg3d", ["ifiles=%s,%s" % (P("f1.txt"), P("f2.txt")), "dim=2", "ofile=" + P("avg.txt"), "bofile=" + P("avg.bin")])
+    run("avg3d", ["ifiles=" + P("f2.txt"), "dim=2", "bofile=" + P("avg2.bin")])
+    a_bin, _ = run("avg3d", ["ifiles=%s,%s" % (P("avg.bin"), P("avg2.bin")), "binary=1", "dim=2", "ofile=" + P("avgb.txt")])
+    sbase = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "ne=30", "erange=100,400", "nthreads=1"]
+    run("spectrum", sbase + ["obs=0,0,30", "ofile=" + P("s1.txt")])
+    run("spectrum", sbase + ["obs=0.001,0,30", "ofile=" + P("s2.txt")])
+    sa, _ = run("avgspec", ["ifiles=%s,%s" % (P("s1.txt"), P("s2.txt")), "ofile=" + P("savg.txt")])
+    out = {k.replace(".", "_"): _bytes(P(k)) for k in ("f1.txt", "f2.txt", "pd.txt", "avg.txt", "avg.bin", "avg2.bin",
+                                                        "avgb.txt", "s1.txt", "s2.txt", "savg.txt")}
+    out["avg_text_container"] = container(a_txt)
+    out["avg_binary_container"] = container(a_bin)
+    out["avg_spectrum"] = sa[1:].reshape(-1, 2)
+    shutil.rmtree(d)
+    return base, out, time.time() - t0
 
 def pole_deltas(seed=4, n=82, sigma=0.005):
     return np.random.default_rng(seed).normal(0.0, sigma, n)
