@@ -210,3 +210,30 @@ def test_flux_gaussian_and_box_fields_routes_match_reference():
         err = normwise(got, ref)
         say("  device RK4 batch (%s) normwise %.3e" % ("fp64" if prec == abi.FP64 else "mixed", err))
         assert err <= tol
+
+
+def test_flux_and_power_density_files_match_reference(tmp_path):
+    """calculate_flux_rectangle / calculate_power_density_rectangle with ofile (threads
+    route layout) write the reference's files: same rows, values to the 6 printed digits."""
+    from srsoriginal_amd.sr import sr
+    say("flux and power-density text files vs the files the reference wrote")
+    g = golden("files_and_averages")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(2001)
+    kw = dict(npoints=[5, 4], plane="XY", width=[0.01, 0.008], translation=[0, 0, 30], nthreads=2)
+    for name, call in (("f1_txt", lambda p: osr.calculate_flux_rectangle(energy_eV=143.5, ofile=p, **kw)),
+                       ("pd_txt", lambda p: osr.calculate_power_density_rectangle(ofile=p, **kw))):
+        p = os.path.join(str(tmp_path), name)
+        call(p)
+        got = open(p, "rb").read().split(b"\n")
+        ref = bytes(g[name]).split(b"\n")
+        assert len(got) == len(ref)
+        same = sum(a == b for a, b in zip(got, ref))
+        vg = np.array([float(x) for x in b" ".join(got).split()])
+        vr = np.array([float(x) for x in b" ".join(ref).split()])
+        err = np.max(np.abs(vg - vr) / np.maximum(np.abs(vr), 1e-300))
+        say("  %s: %d of %d lines identical, max relative difference %.2e" % (name, same, len(ref), err))
+        assert err <= 1e-6 and same >= len(ref) - 2
