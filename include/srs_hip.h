@@ -77,6 +77,13 @@ int srs_hip_flux(const srs_traj* traj, const double* obs_xyz, int64_t ns, const 
 int srs_hip_power_density(const srs_traj* traj, const double* obs_xyz, const double* normals_xyz, int64_t ns,
                           int directional, double charge, double current, int precision, double* pd_out);
 
+/* Electric field in the time domain at obs: efield_out[3*i + k] = E_k [V/m] radiated
+ * from trajectory point i, time_out[i] = its arrival time i*dt + |R|/c [s] (unevenly
+ * spaced).  Computes OSCARSSR::CalculateElectricFieldTimeDomain (OSCARSSR.cc:2646-2690),
+ * which the reference runs on the CPU only; charge [C] is the particle's. */
+int srs_hip_efield_time(const srs_traj* traj, const double obs[3], double charge, double* efield_out,
+                        double* time_out);
+
 /* Device-resident variants: every array pointer is device memory, observation
  * points and normals are SoA ([3][ns]); work is queued on `stream` (a hipStream_t,
  * NULL = the library's stream) and not synchronised.  No counterpart in the

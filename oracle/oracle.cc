@@ -743,6 +743,31 @@ void oracle_power_density(const double* traj, int64_t n, double dt, const double
   per_point_threads(ns, nthreads, [=](int64_t i) { out[i] = pd_point(traj, n, dt, v3(obs + 3 * i), v3(normals + 3 * i), directional != 0, q, current); });
 }
 
+void oracle_efield_time(const double* traj, int64_t n, double dt, const double obs[3], double q, double* e_out,
+                        double* t_out) {
+  // OSCARSSR.cc:2646-2690 (Duke, Synchrotron Radiation, p74), TVector3D operation order
+  const double *X = traj, *Y = traj + n, *Z = traj + 2 * n, *BX = traj + 3 * n, *BY = traj + 4 * n, *BZ = traj + 5 * n;
+  const double *AX = traj + 6 * n, *AY = traj + 7 * n, *AZ = traj + 8 * n;
+  double const C0 = q / (kFourPi * kEps0);
+  V3 const O = v3(obs);
+  for (int64_t i = 0; i < n; ++i) {
+    V3 const B{BX[i], BY[i], BZ[i]};
+    V3 const A{AX[i], AY[i], AZ[i]};
+    V3 const R = sub(O, V3{X[i], Y[i], Z[i]});
+    V3 const N = unit(R);
+    double const D = std::sqrt(mag2(R));
+    double const Mult = C0 * std::pow(1.0 / (1.0 - dot(N, B)), 3);
+    V3 const NB = sub(N, B);
+    V3 const Near = divs(muls(NB, 1.0 - mag2(B)), mag2(R));
+    V3 const Far = divs(muls(cross(N, cross(NB, A)), 1.0 / kC), D);
+    V3 const E = muls(V3{Near.x + Far.x, Near.y + Far.y, Near.z + Far.z}, Mult);
+    e_out[3 * i] = E.x;
+    e_out[3 * i + 1] = E.y;
+    e_out[3 * i + 2] = E.z;
+    t_out[i] = (double)i * dt + D / kC;
+  }
+}
+
 double oracle_total_power(const double* traj, int64_t n, double dt, double q, double current, double gamma) {
   // OSCARSSR::CalculateTotalPower(TParticleA&), OSCARSSR.cc:1939-1978
   double tp = 0;

@@ -109,6 +109,12 @@ int oracle_pole_pd_loss_gradient(const oracle_fields* f, const oracle_particle* 
                                  int directional, const double* target, double* pd_out, double* loss_out,
                                  double* grad_out, int nthreads);
 
+/* Electric field in the time domain at obs (OSCARSSR::CalculateElectricFieldTimeDomain,
+ * OSCARSSR.cc:2646-2690): e_out[3*i + k] = E_k of trajectory point i, t_out[i] = its
+ * arrival time i*dt + D/c. */
+void oracle_efield_time(const double* traj, int64_t n, double dt, const double obs[3], double q, double* e_out,
+                        double* t_out);
+
 /* Kahan add of TSpectrumContainer::AddToFlux / T3DScalarContainer::AddToPoint. */
 void oracle_kahan_add(double* sum, double* comp, const double* v, int64_t n);
 

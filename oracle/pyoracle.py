@@ -199,3 +199,14 @@ def pole_pd_loss_gradient(fields, particle, ct_start, ct_stop, npoints, obs, nor
     if rc != 0:
         raise ValueError("no pole-scaled undulator")
     return loss.value, grad, pd
+
+
+def efield_time(traj, dt, obs, q):
+    """(E [n, 3], arrival time [n]) of the time-domain electric field at obs."""
+    traj = np.ascontiguousarray(traj)
+    n = traj.shape[1]
+    e = np.zeros((n, 3))
+    t = np.zeros(n)
+    lib().oracle_efield_time(_p(traj), I64(n), D(dt), _p(np.ascontiguousarray(obs, dtype=np.float64)), D(q), _p(e),
+                             _p(t))
+    return e, t

@@ -257,6 +257,12 @@ int main(int argc, char** argv) {
       write_container(C);
     } else if (mode == "totalpower") {
       put(osr.CalculateTotalPower());
+    } else if (mode == "efield") {
+      // calculate_efield_vs_time (OSCARSSR_Python.cc:3573-3630): container points = E,
+      // values = arrival time
+      T3DScalarContainer XYZT;
+      osr.CalculateElectricFieldTimeDomain(v3("obs", TVector3D(0, 0, 30)), XYZT);
+      write_container(XYZT);
     } else if (mode == "avg3d") {
       // average_flux / average_power_density (OSCARSSR_Python.cc:3475-3563):
       // ifiles=a,b,...  binary=0|1  dim=2|3  [ofile=, bofile=]

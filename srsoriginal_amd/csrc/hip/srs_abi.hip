@@ -739,6 +739,26 @@ int srs_hip_power_density(const srs_traj* traj, const double* obs_xyz, const dou
   return SRS_OK;
 }
 
+int srs_hip_efield_time(const srs_traj* traj, const double obs[3], double charge, double* efield_out,
+                        double* time_out) {
+  g_err.clear();
+  TRY(check_traj(traj, true));
+  if (!obs || !efield_out || !time_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  TrajView t;
+  TRY(stage_traj(c, traj, true, &t));
+  size_t const n = (size_t)traj->n;
+  HIPCHK(c.out.ensure(4 * n * sizeof(double)));
+  double* const d = c.out.as<double>();
+  HIPCHK(srs::launch_efield_time(t, obs, charge / (kFourPi * kEps0), d, d + 3 * n, c.stream));  // OSCARSSR.cc:2661
+  HIPCHK(hipMemcpyAsync(efield_out, d, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(time_out, d + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return SRS_OK;
+}
+
 int srs_hip_flux_device(const srs_traj* traj, const double* obs_soa, int64_t ns, const double* energies_ev,
                         int64_t ne, double charge, double current, int precision, double* flux_out, void* stream) {
   g_err.clear();

@@ -156,6 +156,34 @@ __global__ __launch_bounds__(kBlock) void pd_fp64_kernel(TrajView t, const doubl
   if (j < ns) part[(size_t)blockIdx.y * ns + j] = sum;
 }
 
+// Time-domain electric field at one observer (OSCARSSR.cc:2646-2690), one lane per
+// trajectory point, the reference's TVector3D operation order:
+//   E = C0 (1/(1 - n.beta))^3 [ (1 - |beta|^2)(n - beta)/|R|^2 + (1/c) n x ((n - beta) x A/c)/|R| ],
+//   t = i dt + |R|/c,  C0 = q/(4 pi eps0).
+__global__ __launch_bounds__(kBlock) void efield_time_kernel(TrajView t, double ox, double oy, double oz, double c0,
+                                                             double* __restrict__ e, double* __restrict__ tm) {
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= t.n) return;
+  double const bx = t.bx[i], by = t.by[i], bz = t.bz[i];
+  double const ax = t.ax[i], ay = t.ay[i], az = t.az[i];
+  double const rx = ox - t.x[i], ry = oy - t.y[i], rz = oz - t.z[i];
+  double const r2 = rx * rx + ry * ry + rz * rz;
+  double const D = __builtin_sqrt(r2);
+  double const nx = rx / D, ny = ry / D, nz = rz / D;
+  double const w = 1.0 / (1.0 - (nx * bx + ny * by + nz * bz));
+  double const mult = c0 * pow(w, 3.0);
+  double const ux = nx - bx, uy = ny - by, uz = nz - bz;
+  double const sb = 1.0 - (bx * bx + by * by + bz * bz);
+  double const vx = uy * az - ay * uz, vy = uz * ax - az * ux, vz = ux * ay - ax * uy;
+  double const fx = ny * vz - vy * nz, fy = nz * vx - vz * nx, fz = nx * vy - vx * ny;
+  double const ic = 1.0 / kC;
+  double const E[3] = {((ux * sb) / r2 + (fx * ic) / D) * mult, ((uy * sb) / r2 + (fy * ic) / D) * mult,
+                       ((uz * sb) / r2 + (fz * ic) / D) * mult};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) e[3 * i + k] = E[k];
+  tm[i] = (double)i * t.dt + D / kC;
+}
+
 // Chunk reductions (fixed order) and the reference's final prefactors:
 //   flux:  SumE *= C1*dt (OSCARSSR.cc:2080) -> (re, im) * (0 + i kw) = (-im kw, re kw);
 //          F = C2 * Re(SumE . conj(SumE)) summed x, y then z (OSCARSSR.cc:2092).
@@ -278,6 +306,11 @@ hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double
                           int chunk, double* part, hipStream_t s) {
   dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
   pd_fp64_kernel<<<grid, kBlock, 0, s>>>(t, obs_soa, nrm_soa, ns, directional, chunk, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_efield_time(const TrajView& t, const double obs[3], double c0, double* e, double* tm, hipStream_t s) {
+  efield_time_kernel<<<(unsigned)((t.n + kBlock - 1) / kBlock), kBlock, 0, s>>>(t, obs[0], obs[1], obs[2], c0, e, tm);
   return hipGetLastError();
 }
 

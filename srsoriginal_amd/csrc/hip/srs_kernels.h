@@ -51,6 +51,8 @@ hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s);
 // ---- reductions over trajectory chunks (fixed order, deterministic) --------
 hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
                               hipStream_t s);
+// Time-domain electric field: e[3*i + k], arrival time tm[i] for every trajectory point.
+hipError_t launch_efield_time(const TrajView& t, const double obs[3], double c0, double* e, double* tm, hipStream_t s);
 hipError_t launch_spectrum_reduce(const double* part, int nchunk, int ne, const double* cw, const double* kw, double c2,
                                   double* out, hipStream_t s);
 hipError_t launch_field_reduce(const double* part, int nchunk, int npts, const double* kw, double c2, double* out,

@@ -241,5 +241,12 @@ PYBIND11_MODULE(_sr, m) {
            },
            py::arg("surface"), py::arg("dim"), py::arg("directional"), py::arg("nparticles"), py::arg("nthreads"),
            py::arg("gpu"), py::arg("precision"), py::arg("batched") = false)
-      .def("calculate_total_power", &srs::OSCARSSR::CalculateTotalPower);
+      .def("calculate_total_power", &srs::OSCARSSR::CalculateTotalPower)
+      .def("calculate_efield_vs_time", [](srs::OSCARSSR& e, std::vector<double> obs) {
+        std::vector<double> ef, t;
+        e.CalculateElectricFieldTimeDomain(v3(obs), ef, t);
+        py::array_t<double> E({(py::ssize_t)t.size(), (py::ssize_t)3});
+        std::copy(ef.begin(), ef.end(), E.mutable_data());
+        return py::make_tuple(E, py::array_t<double>(t.size(), t.data()));
+      });
 }

@@ -793,6 +793,16 @@ void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer&
   }
 }
 
+void OSCARSSR::CalculateElectricFieldTimeDomain(Vec3 const& obs, std::vector<double>& e, std::vector<double>& t) {
+  ensure_particle();
+  CalculateTrajectory(particle_);
+  srs_traj const tr = view(particle_.traj);
+  double const o[3] = {obs.x, obs.y, obs.z};
+  e.assign(3 * particle_.traj.size(), 0.0);
+  t.assign(particle_.traj.size(), 0.0);
+  check_status(srs_hip_efield_time(&tr, o, particle_.q, e.data(), t.data()));
+}
+
 double OSCARSSR::CalculateTotalPower() {
   // OSCARSSR::CalculateTotalPower(TParticleA&), OSCARSSR.cc:1939-1978 (host, O(NT))
   ensure_particle();

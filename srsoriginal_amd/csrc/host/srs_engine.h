@@ -193,6 +193,9 @@ class OSCARSSR {
   void CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer& C, int dim, bool directional, int nparticles,
                              int nthreads, int gpu, Precision prec = Precision::FP64, bool batched = false);
   double CalculateTotalPower();
+  // OSCARSSR::CalculateElectricFieldTimeDomain (OSCARSSR.cc:2627-2690): host RK4, the
+  // field on the device; e [3 n], t [n].
+  void CalculateElectricFieldTimeDomain(Vec3 const& obs, std::vector<double>& e, std::vector<double>& t);
 
  private:
   void ensure_particle();

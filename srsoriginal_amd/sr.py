@@ -349,6 +349,18 @@ class sr:
                                                  _prec(precision), bool(batched))
         return self._out(xyz, v, ofile, 3)
 
+    def calculate_efield_vs_time(self, obs, ofile=""):
+        """OSCARSSR_CalculateElectricFieldTimeDomain (OSCARSSR_Python.cc:3573-3630):
+        [[t, [Ex, Ey, Ez]], ...] at unevenly spaced arrival times t (trajectory order);
+        ofile writes rows "Ex Ey Ez t" (T3DScalarContainer::WriteToFileText, dim 3)."""
+        if self._e.n_beams() < 1:
+            raise ValueError("No particle beam defined")
+        o = _vec3(obs, "obs")
+        E, t = self._e.calculate_efield_vs_time(o)
+        if ofile:
+            containers.write_scalar_text(ofile, E, t, 3)
+        return [[float(a), [float(x), float(y), float(z)]] for a, (x, y, z) in zip(t.tolist(), E.tolist())]
+
     def calculate_total_power(self):
         """OSCARSSR_Python.cc:2369-2398 (host FP64, O(NT))."""
         if self._e.n_beams() < 1:

@@ -307,6 +307,13 @@ def _bytes(path):
 
 
 @case
+def efield_time_c0():
+    # the time-domain electric field of the C0 trajectory at an off-axis point
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=2001", "obs=0.002,-0.001,30"]
+    a, dt = run("efield", args)
+    return args, {"container": container(a)}, dt
+
+@case
 def files_and_averages():
     """The reference's text / binary container files and its averaging of them:
     calculate_flux_rectangle / calculate_power_density_rectangle / calculate_spectrum

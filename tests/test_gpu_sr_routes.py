@@ -237,3 +237,24 @@ def test_flux_and_power_density_files_match_reference(tmp_path):
         err = np.max(np.abs(vg - vr) / np.maximum(np.abs(vr), 1e-300))
         say("  %s: %d of %d lines identical, max relative difference %.2e" % (name, same, len(ref), err))
         assert err <= 1e-6 and same >= len(ref) - 2
+
+
+def test_efield_vs_time_matches_reference(tmp_path):
+    from srsoriginal_amd.sr import sr
+    say("calculate_efield_vs_time at (2 mm, -1 mm, 30 m) vs the reference (C0 trajectory, 2001 points)")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(2001)
+    p = os.path.join(str(tmp_path), "e.txt")
+    out = osr.calculate_efield_vs_time(obs=[0.002, -0.001, 30], ofile=p)
+    ref = golden("efield_time_c0")["container"]
+    t = np.array([r[0] for r in out])
+    E = np.array([r[1] for r in out])
+    et = np.max(np.abs(t - ref[:, 3]) / ref[:, 3])
+    ee = normwise(E.ravel(), ref[:, :3].ravel())
+    say("  arrival times %.2e, field normwise %.2e" % (et, ee))
+    assert et <= 1e-15 and ee <= 1e-12
+    rows = np.loadtxt(p)
+    assert rows.shape == (len(ref), 4) and np.allclose(rows, ref, rtol=1e-6, atol=0)

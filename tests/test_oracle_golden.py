@@ -354,3 +354,15 @@ def test_trajectory_gaussian_and_box_fields_bitwise(oracle):
     gam = p.gamma
     kick_x = 0.3 * 0.01 * 0.05 * math.sqrt(2 * math.pi) / 3.0 * (1 - 1 / (2 * gam ** 2))
     assert abs(t[3, -1] - kick_x) < 0.02 * kick_x
+
+
+def test_efield_time_domain_bitwise(oracle):
+    """OSCARSSR::CalculateElectricFieldTimeDomain (near + far field, arrival times)."""
+    F, p, t, dt = cases.oracle_setup(oracle, "efield_time_c0")
+    a = cases.args("efield_time_c0")
+    e, tm = oracle.efield_time(t, dt, cases.vec(a, "obs"), p.q)
+    ref = golden("efield_time_c0")["container"]
+    assert np.array_equal(e, ref[:, :3]) and np.array_equal(tm, ref[:, 3])
+    # arrival time is monotonic (the observer sees the trajectory in order) and the far
+    # field dominates on the undulator axis region
+    assert np.all(np.diff(tm) > 0)
