@@ -291,7 +291,7 @@ int run_pd(Context& c, const TrajView& t, const double* obs_soa, const double* n
   int const nch = nchunks(t.n, chunk);
   HIPCHK(c.part.ensure((size_t)nch * ns * sizeof(double)));
   if (precision == SRS_PRECISION_MIXED) {
-    HIPCHK(c.trajf.ensure((size_t)6 * t.n * sizeof(float)));
+    HIPCHK(c.trajf.ensure((size_t)7 * t.n * sizeof(float)));
     HIPCHK(srs::launch_traj_to_f32(t, c.trajf.as<float>(), s));
     KernelTimer timer(c, s);
     HIPCHK(srs::launch_pd_mixed(t, c.trajf.as<float>(), obs_soa, nrm_soa, (int)ns, directional, chunk,

@@ -22,7 +22,8 @@ constexpr int SUB = kSub;
 
 
 
-// trajf layout (float): [0] bx, [1] by, [2] 1 - bz, and with_a: [3] ax, [4] ay, [5] az
+// trajf layout (float): [0] bx, [1] by, [2] 1 - bz, and with_a: [3] ax, [4] ay, [5] az,
+// [6] 1 - |beta|^2 (from FP64: (1 - bz)(1 + bz) - bx^2 - by^2)
 // (the flux and spectrum paths stage no accelerations: t.ax..az are NULL there).
 __global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f) {
   int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -34,6 +35,8 @@ __global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f
     f[3 * t.n + i] = (float)t.ax[i];
     f[4 * t.n + i] = (float)t.ay[i];
     f[5 * t.n + i] = (float)t.az[i];
+    double const omb = 1.0 - t.bz[i];
+    f[6 * t.n + i] = (float)(omb * (1.0 + t.bz[i]) - t.bx[i] * t.bx[i] - t.by[i] * t.by[i]);
   }
 }
 
@@ -226,21 +229,55 @@ __global__ __launch_bounds__(64 * SPW) void spectrum_mixed_kernel(TrajView t, do
   }
 }
 
-// Mixed power density: D and n in FP64 -> FP32 with 1 - n_z and 1 - beta_z
-// carried explicitly; the denominator 1 - beta.n (~1e-7) is assembled from small
-// well-conditioned pieces: 1 - b.n = omb + omnz - omb*omnz - bx nx - by ny.
-// Power density has no phase, so its mixed form is FP32 throughout the term: positions
-// rounded to FP32 keep R = O - X to ~1e-7 relative (r_z = O_z - z ~ 30 m is exact to
-// ~3e-9), 1/D is one v_rsq_f32, 1 - n_z = (n_x^2 + n_y^2)/(1 + n_z) and
-// 1 - beta.n = (1 - beta_z) + (1 - n_z) - (1 - beta_z)(1 - n_z) - beta_x n_x - beta_y n_y
-// avoid the cancellations, the regions are selected without branches, and
-// 1/(1 - beta.n)^5 is a power of a reciprocal of the 2^20-scaled value.  32-term FP32
-// sub-sums, Kahan-compensated.
+// Mixed power density, FP32 throughout the term.  With u = n - beta and |n| = 1
+// (n is the unit vector (n_x, n_y, 1 - (1 - n_z)) whatever the rounding of 1 - n_z):
+//   1 - beta.n     = (|u|^2 + 1 - |beta|^2) / 2      (a sum of non-negative terms)
+//   n x (u x A)    = u (n.A) - A (1 - beta.n)
+// The first form has no cancellation: 1 - beta.n (down to 1/(2 gamma^2) ~ 1.5e-8 at
+// 3 GeV) is not a difference of terms ~1e-6, and an error in 1 - n_z moves |n|^2 - 1
+// and u_z^2 by amounts that cancel to first order.  1 - |beta|^2 is taken per point in
+// FP64 (traj_to_f32_kernel).  1 - n_z = (n_x^2 + n_y^2)/(1 + n_z) for n_z > 0, else
+// (1 - n_z)^2/(1 - n_z).  Positions rounded to FP32 keep R = O - X to
+// ~1e-7 relative, and 1/D is one v_rsq_f32.
+//
+// Scaling: n and beta are carried as S n, S beta (S = 2^10, staged scaled in LDS), so
+// d2 = S^2 |u|^2 + S^2 (1 - |beta|^2) = 2^21 (1 - beta.n) stays far from the FP32
+// range limits when raised to the -5th power, and the term comes out as 2^-55 times
+// its value; 2^55 is restored in FP64 when the sum is written.  32-term FP32 sub-sums,
+// Kahan-compensated.
+constexpr float kPdS = 0x1p10f;
+
+template <bool DIR>
+__device__ __forceinline__ float pd_mixed_term(float ox, float oy, float oz, float qx, float qy, float qz, float4 P,
+                                               float4 B, float4 A) {
+  float const rx = ox - P.x, ry = oy - P.y, rz = oz - P.z;
+  float const inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)));
+  float const is = inv * kPdS;
+  float const nx = rx * is, ny = ry * is, nz = rz * is;  // S n
+  float const v = __builtin_fmaf(nx, nx, ny * ny);
+  // S (1 - n_z) = w / a with a = S + |S n_z|: w = S^2 (n_x^2 + n_y^2) for n_z > 0, a^2 else;
+  // the select picks between two plain values, so no branch is generated
+  float const a = kPdS + __builtin_fabsf(nz);
+  float const omnz = (nz > 0.f ? v : a * a) * __builtin_amdgcn_rcpf(a);
+  float const ux = nx - B.x, uy = ny - B.y, uz = B.z - omnz;                       // S u
+  float const d2 = __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, __builtin_fmaf(uz, uz, P.w)));
+  float const na = __builtin_fmaf(nx, A.x, __builtin_fmaf(ny, A.y, nz * A.z));     // S n.A
+  float const hd = 0.5f * d2;                                                      // S^2 (1 - beta.n)
+  float const mx = __builtin_fmaf(ux, na, -(A.x * hd)), my = __builtin_fmaf(uy, na, -(A.y * hd));
+  float const mz = __builtin_fmaf(uz, na, -(A.z * hd));                            // S^2 n x (u x A)
+  float const num2 = __builtin_fmaf(mx, mx, __builtin_fmaf(my, my, mz * mz));
+  float const ndot = __builtin_fmaf(nx, qx, __builtin_fmaf(ny, qy, nz * qz));      // S n.normal
+  float const rd = __builtin_amdgcn_rcpf(d2);
+  float const rd2 = rd * rd;
+  float const term = num2 * ndot * (inv * inv) * (rd2 * rd2 * rd);
+  return (DIR && ndot <= 0.f) ? 0.f : term;
+}
+
+template <bool DIR>
 __global__ __launch_bounds__(kBlock) void pd_mixed_kernel(TrajView t, const float* __restrict__ tf,
                                                           const double* __restrict__ obs, const double* __restrict__ nrm,
-                                                          int ns, int directional, int chunk, double* __restrict__ part) {
-  __shared__ float sx[TILE], sy[TILE], sz[TILE];
-  __shared__ float sbx[TILE], sby[TILE], somb[TILE], sax[TILE], say[TILE], saz[TILE];
+                                                          int ns, int chunk, double* __restrict__ part) {
+  __shared__ float4 sP[TILE], sB[TILE], sA[TILE];  // (x, y, z, S^2 (1 - |beta|^2)), S (bx, by, 1 - bz), A
   int const tid = threadIdx.x;
   int const j = blockIdx.x * kBlock + tid;
   int const jj = j < ns ? j : ns - 1;
@@ -248,59 +285,30 @@ __global__ __launch_bounds__(kBlock) void pd_mixed_kernel(TrajView t, const floa
   float const qx = (float)nrm[jj], qy = (float)nrm[ns + jj], qz = (float)nrm[2 * ns + jj];
   int64_t const i0 = (int64_t)blockIdx.y * chunk;
   int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  int64_t const n = t.n;
   KahanF acc{0.f, 0.f};
   for (int64_t base = i0; base < i1; base += TILE) {
     int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
     __syncthreads();
     if (tid < cnt) {
       int64_t const i = base + tid;
-      sx[tid] = (float)t.x[i];
-      sy[tid] = (float)t.y[i];
-      sz[tid] = (float)t.z[i];
-      sbx[tid] = tf[i];
-      sby[tid] = tf[t.n + i];
-      somb[tid] = tf[2 * t.n + i];
-      sax[tid] = tf[3 * t.n + i];
-      say[tid] = tf[4 * t.n + i];
-      saz[tid] = tf[5 * t.n + i];
+      sP[tid] = make_float4((float)t.x[i], (float)t.y[i], (float)t.z[i], tf[6 * n + i] * (kPdS * kPdS));
+      sB[tid] = make_float4(tf[i] * kPdS, tf[n + i] * kPdS, tf[2 * n + i] * kPdS, 0.f);
+      sA[tid] = make_float4(tf[3 * n + i], tf[4 * n + i], tf[5 * n + i], 0.f);
     }
     __syncthreads();
     for (int k0 = 0; k0 < cnt; k0 += SUB) {
-      int const k1 = (k0 + SUB < cnt) ? k0 + SUB : cnt;
       float sub = 0.f;
-      for (int k = k0; k < k1; ++k) {
-        float const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
-        float const inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)));
-        float const nx = rx * inv, ny = ry * inv, nz = rz * inv;
-        float const ndot = __builtin_fmaf(nx, qx, __builtin_fmaf(ny, qy, nz * qz));
-        float const v = __builtin_fmaf(nx, nx, ny * ny);
-        // both forms of 1 - n_z, blended arithmetically (no branch); the first is finite
-        // for every n_z and weighted 0 unless n_z > 0
-        float const oa = v * __builtin_amdgcn_rcpf(__builtin_fmaxf(1.f + nz, 1e-30f));
-        float const ob = 1.f - nz;
-        float const omnz = __builtin_fmaf((float)(nz > 0.f), oa - ob, ob);
-        float const bx = sbx[k], by = sby[k], omb = somb[k];
-        float const den = __builtin_fmaf(-omb, omnz, omb + omnz) - __builtin_fmaf(bx, nx, by * ny);
-        // u = n - beta with u_z = omb - omnz
-        float const ux = nx - bx, uy = ny - by, uz = omb - omnz;
-        float const Ax = sax[k], Ay = say[k], Az = saz[k];
-        float const vx = __builtin_fmaf(uy, Az, -(Ay * uz)), vy = __builtin_fmaf(uz, Ax, -(Az * ux));
-        float const vz = __builtin_fmaf(ux, Ay, -(Ax * uy));
-        float const mx = __builtin_fmaf(ny, vz, -(vy * nz)), my = __builtin_fmaf(nz, vx, -(vz * nx));
-        float const mz = __builtin_fmaf(nx, vy, -(vx * ny));
-        float const num2 = __builtin_fmaf(mx, mx, __builtin_fmaf(my, my, mz * mz));
-        // 1 - beta.n reaches 1/(2 gamma^2) ~ 1.5e-8, whose fifth power is below the FP32
-        // normal range: work with (1 - beta.n) 2^20 (an exact scaling), whose reciprocal
-        // stays below ~70, and restore 2^100 in FP64 when the sum is written
-        float const rd = __builtin_amdgcn_rcpf(den * 0x1p20f);
-        float const rd2 = rd * rd;
-        float const term = num2 * ndot * (inv * inv) * (rd2 * rd2 * rd);
-        sub += (directional && ndot <= 0.f) ? 0.f : term;
+      if (cnt - k0 >= SUB) {
+#pragma unroll 4
+        for (int k = k0; k < k0 + SUB; ++k) sub += pd_mixed_term<DIR>(ox, oy, oz, qx, qy, qz, sP[k], sB[k], sA[k]);
+      } else {
+        for (int k = k0; k < cnt; ++k) sub += pd_mixed_term<DIR>(ox, oy, oz, qx, qy, qz, sP[k], sB[k], sA[k]);
       }
       acc.add(sub);
     }
   }
-  if (j < ns) part[(size_t)blockIdx.y * ns + j] = ((double)acc.s - (double)acc.c) * 0x1p100;
+  if (j < ns) part[(size_t)blockIdx.y * ns + j] = ((double)acc.s - (double)acc.c) * 0x1p55;
 }
 
 inline int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk); }
@@ -338,7 +346,10 @@ hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const d
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
                            int directional, int chunk, double* part, hipStream_t s) {
   dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
-  pd_mixed_kernel<<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, nrm_soa, ns, directional, chunk, part);
+  if (directional)
+    pd_mixed_kernel<true><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, nrm_soa, ns, chunk, part);
+  else
+    pd_mixed_kernel<false><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, nrm_soa, ns, chunk, part);
   return hipGetLastError();
 }
 

@@ -172,6 +172,42 @@ def test_mixed_power_density_c2_within_1e5(abi, oracle):
     assert abs(tp_got - tp_ref) / tp_ref <= TOL_MIXED
 
 
+def test_mixed_power_density_wide_angles_backward_and_directional(abi, oracle):
+    """Mixed power density away from the axis: screens at large angles, observers
+    upstream of the trajectory (n_z < 0), random normals, directional on and off;
+    each group is held to 1e-5 of its own largest value against the FP64 kernel,
+    which is pinned to the reference at 1e-10.  The reference's screen inside the
+    undulator (directional; part of the trajectory lies behind each point) is checked
+    directly."""
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_20001")
+    rng = np.random.default_rng(11)
+    groups = {
+        "near axis": np.stack([rng.uniform(-0.02, 0.02, 300), rng.uniform(-0.02, 0.02, 300), np.full(300, 30.0)], 1),
+        "wide": np.stack([rng.uniform(0.5, 3, 300) * rng.choice([-1, 1], 300),
+                          rng.uniform(-3, 3, 300), rng.uniform(-6, 6, 300)], 1),
+        "upstream": np.stack([rng.uniform(-0.5, 0.5, 300), rng.uniform(-0.5, 0.5, 300), rng.uniform(-30, -5, 300)], 1),
+    }
+    xyz = np.concatenate(list(groups.values()))
+    nrm = rng.normal(size=xyz.shape)
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    nrm[:300] = (0, 0, 1)
+    for directional in (False, True):
+        ref = abi.power_density(t, dt, xyz, nrm, directional, p.q, p.current, abi.FP64)
+        got = abi.power_density(t, dt, xyz, nrm, directional, p.q, p.current, abi.MIXED)
+        for k, name in enumerate(groups):
+            sl = slice(300 * k, 300 * (k + 1))
+            err = normwise(got[sl], ref[sl])
+            say("  %-9s directional=%d: mixed vs fp64 normwise %.3e" % (name, directional, err))
+            assert err <= TOL_MIXED
+    name = "pd_directional_inside"
+    F, p, t, dt = cases.oracle_setup(oracle, name)
+    xyz, nrm, x12 = _rect(oracle, name)
+    ref = golden(name)["container"][:len(xyz), 3]
+    got = abi.power_density(t, dt, xyz, nrm, True, p.q, p.current, abi.MIXED)
+    say("  %s: mixed vs reference normwise %.3e" % (name, normwise(got, ref)))
+    assert normwise(got, ref) <= TOL_MIXED
+
+
 def test_abi_rejects_bad_arguments(abi, oracle):
     F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_2001")
     with pytest.raises(abi.SrsError, match="precision"):
