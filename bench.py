@@ -31,13 +31,17 @@ ENERGY_EV = 143.52176088044024   # first-harmonic argmax of the reference's C0 s
 F_ALG_FLUX = 53                  # algorithmic FLOP per flux term
 PEAK_FP32_VALU = 157.3e12        # MI355X FP32 vector peak (published)
 PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
-# Non-packed FP32 issues at the FP64 rate (64 lanes per CU per cycle); the packed FP32
-# instructions behind the 157.3 TF/s figure are not used by this build.
-PEAK_FP32_NONPACKED = 78.6e12
+# Issue model of the dominant loop (flux_mixed_kernel<1,256>, paraxial terms), from the
+# measured cycles per wave64 instruction on gfx950 (profiles/valu_issue_rates_gfx950.md):
+# FP32 with VGPR operands 2, FP64 (add, fma, fract, cvt) 4, sin/cos 8.  Per term: 8 FP64
+# (phase 5, fract 1, two conversions) + 9 FP32 (8 accumulations, 1 amplitude) + sin + cos
+# = 66 cycles, plus ~3 for the 32-term sub-tile epilogue.
+ISSUE_CYCLES_PER_WAVE_TERM = 69
+SIMDS, CLOCK_HZ = 1024, 2.4e9
 # HBM bytes per launch of flux_mixed_kernel<1,256> on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
 # what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
-TRAFFIC_MIXED_BYTES = 32754.9 * 1024
+TRAFFIC_MIXED_BYTES = 46193.1 * 1024
 
 
 def log(msg):
@@ -457,7 +461,10 @@ def main():
                           "rms_rel_err": rms_rel},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VALU / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VALU, "traffic": TRAFFIC_MIXED_BYTES,
-                     "frac_of_nonpacked_fp32": achieved / PEAK_FP32_NONPACKED,
+                     "issue_model": {"cycles_per_wave_term": ISSUE_CYCLES_PER_WAVE_TERM,
+                                     "bound_ms_at_2p4GHz": terms / 64 * ISSUE_CYCLES_PER_WAVE_TERM / (SIMDS * CLOCK_HZ)
+                                     * 1e3,
+                                     "frac": terms / 64 * ISSUE_CYCLES_PER_WAVE_TERM / (SIMDS * CLOCK_HZ) / kernel_s},
                      "kernel": "flux_mixed_kernel<1,256>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
                      "flop_per_term": F_ALG_FLUX},
     }

@@ -177,15 +177,31 @@ int check_traj(const srs_traj* t, bool need_a) {
   return SRS_OK;
 }
 
-// Chunking of the trajectory so that a launch has ~2048 blocks (8 per CU on 256 CUs).
-int pick_chunk(int64_t n, int64_t lanes, int align) {
+// Chunking of the trajectory for a launch of bx x nch equal blocks (bx = point
+// blocks of 256 lanes) of which the GPU holds `resident` at a time: the launch runs
+// in ceil(bx*nch / resident) rounds of one chunk each, so the chunk (a multiple of
+// align) that minimises rounds x chunk -- plus a small cost per chunk for its partial
+// sums -- keeps every CU busy to the end.  (A C1 flux map, 158 x 12 blocks of 1792
+// points with 1024 resident, runs 2 rounds where 1.85 would do; 19 chunks of 1056
+// points fill 3 rounds to 97 %.)
+int pick_chunk(int64_t n, int64_t lanes, int align, int64_t resident) {
   int64_t const bx = (lanes + srs::kBlock - 1) / srs::kBlock;
-  int64_t nch = (2048 + bx - 1) / bx;
   int64_t const maxch = (n + align - 1) / align;
-  nch = std::max<int64_t>(1, std::min(nch, maxch));
-  int64_t chunk = (n + nch - 1) / nch;
-  chunk = ((chunk + align - 1) / align) * align;
-  return (int)chunk;
+  resident = std::max<int64_t>(1, resident);
+  int64_t const limit = std::max<int64_t>(1, std::min<int64_t>(maxch, std::min<int64_t>(4096, 4 * resident / bx + 2)));
+  int64_t best = n, best_cost = -1;
+  for (int64_t nch = 1; nch <= limit; ++nch) {
+    int64_t chunk = (n + nch - 1) / nch;
+    chunk = (chunk + align - 1) / align * align;
+    int64_t const nc = (n + chunk - 1) / chunk;
+    int64_t const rounds = (bx * nc + resident - 1) / resident;
+    int64_t const cost = rounds * chunk + nc;
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = chunk;
+    }
+  }
+  return (int)best;
 }
 int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk); }
 
@@ -213,9 +229,13 @@ TrajView view_of(const srs_traj* t) {
 int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, const double* energies_ev, int64_t ne,
              double q, double cur, int precision, double* out, hipStream_t s) {
   double const c0 = c0_of(q), c2 = c2_of(q, cur);
-  int const chunk = pick_chunk(t.n, ns, 256);
-  int const nch = nchunks(t.n, chunk);
   int const maxe = srs::kMaxFluxEnergies;
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  int const ne0 = (int)std::min<int64_t>(maxe, ne);
+  int const per_cu = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_blocks_per_cu(ne0)
+                                                      : srs::flux_fp64_blocks_per_cu(ne0);
+  int const chunk = pick_chunk(t.n, ns, 32, (int64_t)c.cus * per_cu);
+  int const nch = nchunks(t.n, chunk);
   HIPCHK(c.part.ensure((size_t)nch * maxe * 6 * ns * sizeof(double)));
   if (precision == SRS_PRECISION_MIXED) {
     HIPCHK(c.trajf.ensure((size_t)6 * t.n * sizeof(float)));
@@ -271,7 +291,8 @@ int run_spectrum(Context& c, const TrajView& t, const double obs[3], const doubl
     }
     HIPCHK(srs::launch_field_reduce(c.part.as<double>(), nch, (int)ne, d + 2 * ne, c2, out, s));
   } else {
-    int const chunk = pick_chunk(t.n, ne, 256);
+    if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+    int const chunk = pick_chunk(t.n, ne, 256, (int64_t)c.cus * srs::spectrum_fp64_blocks_per_cu());
     int const nch = nchunks(t.n, chunk);
     HIPCHK(c.part.ensure((size_t)nch * 12 * ne * sizeof(double)));
     {
@@ -287,7 +308,10 @@ int run_spectrum(Context& c, const TrajView& t, const double obs[3], const doubl
 
 int run_pd(Context& c, const TrajView& t, const double* obs_soa, const double* nrm_soa, int64_t ns, int directional,
            double q, double cur, int precision, double* out, hipStream_t s) {
-  int const chunk = pick_chunk(t.n, ns, 256);
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  int const per_cu = precision == SRS_PRECISION_MIXED ? srs::pd_mixed_blocks_per_cu(directional)
+                                                      : srs::pd_fp64_blocks_per_cu();
+  int const chunk = pick_chunk(t.n, ns, 32, (int64_t)c.cus * per_cu);
   int const nch = nchunks(t.n, chunk);
   HIPCHK(c.part.ensure((size_t)nch * ns * sizeof(double)));
   if (precision == SRS_PRECISION_MIXED) {

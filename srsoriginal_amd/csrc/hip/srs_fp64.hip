@@ -282,6 +282,39 @@ inline int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk)
 
 }  // namespace
 
+template <class K>
+static int blocks_per_cu(K kernel, int threads, int& cache) {
+  if (cache == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kernel), threads, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    cache = nb;
+  }
+  return cache;
+}
+
+int flux_fp64_blocks_per_cu(int ne) {
+  static int c[5];
+  switch (ne) {
+    case 1: return blocks_per_cu(flux_fp64_kernel<1, TILE>, kBlock, c[1]);
+    case 2: return blocks_per_cu(flux_fp64_kernel<2, TILE>, kBlock, c[2]);
+    case 3: return blocks_per_cu(flux_fp64_kernel<3, TILE>, kBlock, c[3]);
+    default: return blocks_per_cu(flux_fp64_kernel<4, TILE>, kBlock, c[4]);
+  }
+}
+
+int spectrum_fp64_blocks_per_cu() {
+  static int c;
+  return blocks_per_cu(spectrum_fp64_kernel, kBlock, c);
+}
+
+int pd_fp64_blocks_per_cu() {
+  static int c;
+  return blocks_per_cu(pd_fp64_kernel, kBlock, c);
+}
+
 hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,
                             double* part, hipStream_t s) {
   dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));

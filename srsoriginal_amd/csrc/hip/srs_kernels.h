@@ -22,6 +22,13 @@ struct EnergySet {
   double fw[kMaxFluxEnergies];
 };
 
+// ---- resident blocks per CU of the integral kernels (for the chunk choice) ----
+int flux_fp64_blocks_per_cu(int ne);
+int spectrum_fp64_blocks_per_cu();
+int pd_fp64_blocks_per_cu();
+int flux_mixed_blocks_per_cu(int ne);
+int pd_mixed_blocks_per_cu(int directional);
+
 // ---- plain FP64 form ------------------------------------------------------
 // Flux partial sums: part[((c*ne + e)*6 + k)*ns + j], c = trajectory chunk.
 hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,

@@ -315,6 +315,36 @@ inline int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk)
 
 }  // namespace
 
+// Resident blocks per CU of a kernel (its VGPR, LDS and wave limits), cached.
+template <class K>
+static int blocks_per_cu(K kernel, int threads, int& cache) {
+  if (cache == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kernel), threads, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    cache = nb;
+  }
+  return cache;
+}
+
+int flux_mixed_blocks_per_cu(int ne) {
+  static int c[5];
+  switch (ne) {
+    case 1: return blocks_per_cu(flux_mixed_kernel<1, TILE>, kBlock, c[1]);
+    case 2: return blocks_per_cu(flux_mixed_kernel<2, TILE>, kBlock, c[2]);
+    case 3: return blocks_per_cu(flux_mixed_kernel<3, TILE>, kBlock, c[3]);
+    default: return blocks_per_cu(flux_mixed_kernel<4, TILE>, kBlock, c[4]);
+  }
+}
+
+int pd_mixed_blocks_per_cu(int directional) {
+  static int c[2];
+  return directional ? blocks_per_cu(pd_mixed_kernel<true>, kBlock, c[1])
+                     : blocks_per_cu(pd_mixed_kernel<false>, kBlock, c[0]);
+}
+
 hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s) {
   int const with_a = (t.ax && t.ay && t.az) ? 1 : 0;
   traj_to_f32_kernel<<<(unsigned)((t.n + 255) / 256), 256, 0, s>>>(t, with_a, trajf);

@@ -355,11 +355,8 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
         double p;
         if constexpr (NE == 1) p = __builtin_fma(s, hs, r.phi[0]);
         else p = __builtin_fma(es.fw[e], u, r.phi[e]);
-        // v_fract_f64 is the faster reduction with several energies, p - rint(p) with one
-        // (measured); either is exact
-        float ph;
-        if constexpr (NE == 1) ph = (float)(p - __builtin_rint(p));
-        else ph = (float)__builtin_amdgcn_fract(p);
+        // one exact FP64 instruction reduces to [0, 1) (p - rint(p) takes two)
+        float const ph = (float)__builtin_amdgcn_fract(p);
         float const sn = __builtin_amdgcn_sinf(ph), cn = __builtin_amdgcn_cosf(ph);
         float* const b = sub[e];
         b[0] = __builtin_fmaf(r.ax, cn, b[0]);
