@@ -112,12 +112,18 @@ int srs_hip_power_density_device(const srs_traj* traj, const double* obs_soa, co
  *     width_k > 0 (TField3D_Gaussian.cc:88-113);
  *   kind SRS_FIELD_UNIFORM_BOX: field inside |r_k| <= |width_k|/2 for the axes with
  *     width_k > 0, 0 outside (TField3D_UniformBox.cc:94-117);
+ *   kind SRS_FIELD_GRID: a field map (TField3D_Grid.cc:124-250): grid holds
+ *     3*grid_n[0]*grid_n[1]*grid_n[2] doubles, the field vectors at (ix, iy, iz) in the
+ *     order ((ix*n1 + iy)*n2 + iz), already rotated; on the axes with grid_n_k > 1 the
+ *     field is 0 unless grid_start_k < r_k < grid_stop_k and is interpolated linearly
+ *     (x, then y, then z) from cell index trunc((r_k - start_k)/step_k);
  * where r = Rot(X) - center, Rot rotating the position by rotations[0], [1], [2] about
  * X, then Y, then Z (TVector3D::RotateSelfXYZ); field is given already rotated, as the
  * reference's constructors rotate it. */
 #define SRS_FIELD_UNDULATOR 0
 #define SRS_FIELD_GAUSSIAN 1
 #define SRS_FIELD_UNIFORM_BOX 2
+#define SRS_FIELD_GRID 3
 typedef struct srs_field {
   double field[3];
   double period[3];
@@ -129,6 +135,11 @@ typedef struct srs_field {
   int32_t kind;
   double width[3];
   double rotations[3];
+  const double* grid; /* SRS_FIELD_GRID only */
+  int32_t grid_n[3];
+  double grid_start[3];
+  double grid_step[3];
+  double grid_stop[3];
 } srs_field;
 
 /* One particle: initial position, beta, q/(m gamma), T0 [m], its ct range and point

@@ -131,6 +131,35 @@ static void setup(OSCARSSR& osr) {
     TVector3D rot = b.size() >= 12 ? TVector3D(b[9], b[10], b[11]) : TVector3D(0, 0, 0);
     osr.AddMagneticField((TField*)new TField3D_UniformBox(TVector3D(b[0], b[1], b[2]), w, ctr, rot));
   }
+  // Field maps read by the reference's TField3D_Grid (OSCARSSR::AddMagneticField,
+  // OSCARSSR.cc:68-90): gridfile1=path gridfmt1=format [gridrot1=rx,ry,rz]
+  // [gridtrans1=tx,ty,tz] [gridscale1=s0,s1,...] (then gridfile2 ...), and one map
+  // interpolated in a parameter (OSCARSSR::AddMagneticFieldInterpolated, :94-114):
+  // gridmap=p0:path0;p1:path1;... gridmapfmt=format gridparam=p [gridmaprot=...]
+  // [gridmaptrans=...] [gridmapscale=...]; added after the boxes, in this order.
+  for (int k = 1; k <= 2; ++k) {
+    std::string const key = "gridfile" + std::to_string(k);
+    if (!gArgs.count(key)) continue;
+    std::string const sk = std::to_string(k);
+    osr.AddMagneticField(str(key, ""), str("gridfmt" + sk, "OSCARS"), v3("gridrot" + sk, TVector3D(0, 0, 0)),
+                         v3("gridtrans" + sk, TVector3D(0, 0, 0)), vec("gridscale" + sk));
+  }
+  if (gArgs.count("gridmap")) {
+    std::vector<std::pair<double, std::string> > mapping;
+    std::string m = str("gridmap", "");
+    size_t p0 = 0;
+    while (p0 < m.size()) {
+      size_t const p1 = m.find(';', p0);
+      std::string const e = m.substr(p0, p1 == std::string::npos ? std::string::npos : p1 - p0);
+      size_t const c = e.find(':');
+      mapping.push_back(std::make_pair(atof(e.substr(0, c).c_str()), e.substr(c + 1)));
+      if (p1 == std::string::npos) break;
+      p0 = p1 + 1;
+    }
+    osr.AddMagneticFieldInterpolated(mapping, str("gridmapfmt", "OSCARS"), num("gridparam", 0),
+                                     v3("gridmaprot", TVector3D(0, 0, 0)), v3("gridmaptrans", TVector3D(0, 0, 0)),
+                                     vec("gridmapscale"));
+  }
   // Pole-scaled undulator: poles=Bx,By,Bz,Px,Py,Pz,N  scales=d0,d1,...
   std::vector<double> pu = vec("poles");
   if (pu.size() == 7) {
@@ -290,6 +319,23 @@ int main(int argc, char** argv) {
       for (size_t i = 0; i < S.GetNPoints(); ++i) {
         put(S.GetEnergy(i));
         put(S.GetFlux(i));
+      }
+    } else if (mode == "bfield") {
+      // total field at the points of the raw float64 file pts=path (x, y, z per point):
+      // OSCARSSR::GetB (TFieldContainer::GetF); layout: N, then N x (Bx, By, Bz)
+      FILE* fp = fopen(str("pts", "").c_str(), "rb");
+      if (!fp) return 4;
+      std::vector<double> xyz;
+      double v;
+      while (fread(&v, sizeof(double), 1, fp) == 1) xyz.push_back(v);
+      fclose(fp);
+      size_t const n = xyz.size() / 3;
+      put((double)n);
+      for (size_t i = 0; i != n; ++i) {
+        TVector3D const B = osr.GetB(TVector3D(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+        put(B.GetX());
+        put(B.GetY());
+        put(B.GetZ());
       }
     } else if (mode == "beam") {
       // layout: count, then count x (x0[3], beta0[3], E0, gamma, q/m/gamma)

@@ -163,13 +163,14 @@ BATCH_GRAPH = 1
 BATCH_AUTOTUNE = 2
 
 
-FIELD_UNDULATOR, FIELD_GAUSSIAN, FIELD_UNIFORM_BOX = 0, 1, 2
+FIELD_UNDULATOR, FIELD_GAUSSIAN, FIELD_UNIFORM_BOX, FIELD_GRID = 0, 1, 2, 3
 
 
 class SrsField(C.Structure):
     _fields_ = [("field", _D * 3), ("period", _D * 3), ("nperiods", C.c_int32), ("phase", _D), ("taper", _D),
                 ("center", _D * 3), ("pole_delta", C.c_void_p), ("kind", C.c_int32), ("width", _D * 3),
-                ("rotations", _D * 3)]
+                ("rotations", _D * 3), ("grid", C.c_void_p), ("grid_n", C.c_int32 * 3), ("grid_start", _D * 3),
+                ("grid_step", _D * 3), ("grid_stop", _D * 3)]
 
 
 class SrsParticle(C.Structure):
@@ -207,7 +208,8 @@ def _bind_batch():
 def undulators(specs, keep):
     """srs_field array from dicts: undulators (field, period, nperiods[, phase, taper,
     center, pole_delta]) or, with kind FIELD_GAUSSIAN / FIELD_UNIFORM_BOX, (field,
-    width[, center, rotations])."""
+    width[, center, rotations]), or with kind FIELD_GRID (grid [n0*n1*n2*3], grid_n,
+    grid_start, grid_step, grid_stop[, center, rotations])."""
     arr = (SrsField * len(specs))()
     for i, u in enumerate(specs):
         pd = None
@@ -215,10 +217,17 @@ def undulators(specs, keep):
             d = _f64(u["pole_delta"])
             keep.append(d)
             pd = d.ctypes.data
-        arr[i] = SrsField((_D * 3)(*u["field"]), (_D * 3)(*u.get("period", (0, 0, 0))), int(u.get("nperiods", 0)),
-                          float(u.get("phase", 0)), float(u.get("taper", 0)), (_D * 3)(*u.get("center", (0, 0, 0))),
-                          pd, int(u.get("kind", FIELD_UNDULATOR)), (_D * 3)(*u.get("width", (0, 0, 0))),
-                          (_D * 3)(*u.get("rotations", (0, 0, 0))))
+        grid = None
+        if u.get("grid") is not None:
+            g = _f64(u["grid"])
+            keep.append(g)
+            grid = g.ctypes.data
+        arr[i] = SrsField((_D * 3)(*u.get("field", (0, 0, 0))), (_D * 3)(*u.get("period", (0, 0, 0))),
+                          int(u.get("nperiods", 0)), float(u.get("phase", 0)), float(u.get("taper", 0)),
+                          (_D * 3)(*u.get("center", (0, 0, 0))), pd, int(u.get("kind", FIELD_UNDULATOR)),
+                          (_D * 3)(*u.get("width", (0, 0, 0))), (_D * 3)(*u.get("rotations", (0, 0, 0))), grid,
+                          (C.c_int32 * 3)(*u.get("grid_n", (1, 1, 1))), (_D * 3)(*u.get("grid_start", (0, 0, 0))),
+                          (_D * 3)(*u.get("grid_step", (0, 0, 0))), (_D * 3)(*u.get("grid_stop", (0, 0, 0))))
     return arr
 
 

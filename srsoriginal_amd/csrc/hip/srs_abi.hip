@@ -82,8 +82,9 @@ struct Context {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   size_t ev_used = 0;
   // batched path
-  Buf b_part, b_und, b_poles, b_traj, b_trajf, b_energies, b_flux_pp, b_out, b_counter, b_obs;
+  Buf b_part, b_und, b_poles, b_grid, b_traj, b_trajf, b_energies, b_flux_pp, b_out, b_counter, b_obs;
   Buf b_state, b_field;  // RK4 state between segments; field sums accumulated over segments
+  bool und_local = false;  // the staged field list holds elements other than undulators
   hipGraph_t b_graph = nullptr;
   hipGraphExec_t b_exec = nullptr;
   std::string b_key;
@@ -391,12 +392,49 @@ int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) 
   if (nund < 1 || !und) return fail(SRS_ERR_INVALID, "no magnetic field (undulator) given");
   std::vector<srs::DevUndulator> du(nund);
   std::vector<double> poles;
+  // field maps: their values go to one device buffer, one after the other
+  size_t grid_doubles = 0;
+  std::vector<size_t> grid_off(nund, 0);
+  for (int u = 0; u < nund; ++u) {
+    srs_field const& h = und[u];
+    if (h.kind != SRS_FIELD_GRID) continue;
+    if (!h.grid) return fail(SRS_ERR_INVALID, "field map values are NULL");
+    int64_t np = 1;
+    int ndim = 0;
+    for (int k = 0; k < 3; ++k) {
+      if (h.grid_n[k] < 1) return fail(SRS_ERR_INVALID, "invalid number of points in at least one dimension");
+      np *= h.grid_n[k];
+      ndim += h.grid_n[k] > 1;
+    }
+    if (ndim == 0) return fail(SRS_ERR_INVALID, "invalid dimensions");
+    if (np > ((int64_t)1 << 31)) return fail(SRS_ERR_INVALID, "field map larger than 2^31 points");
+    grid_off[u] = grid_doubles;
+    grid_doubles += 3 * (size_t)np;
+  }
+  if (grid_doubles) HIPCHK(c.b_grid.ensure(grid_doubles * sizeof(double)));
   for (int u = 0; u < nund; ++u) {
     srs::DevUndulator& d = du[u];
     srs_field const& h = und[u];
     d = srs::DevUndulator{};
     d.kind = h.kind;
     d.pole_off = -1;
+    if (h.kind == SRS_FIELD_GRID) {
+      if (h.pole_delta) return fail(SRS_ERR_INVALID, "pole errors apply to undulators only");
+      size_t const np = (size_t)h.grid_n[0] * h.grid_n[1] * h.grid_n[2];
+      double* const dst = c.b_grid.as<double>() + grid_off[u];
+      HIPCHK(hipMemcpyAsync(dst, h.grid, 3 * np * sizeof(double), hipMemcpyHostToDevice, s));
+      d.gdata = dst;
+      for (int k = 0; k < 3; ++k) {
+        d.center[k] = h.center[k];
+        d.rc[k] = std::cos(h.rotations[k]);
+        d.rs[k] = std::sin(h.rotations[k]);
+        d.gn[k] = h.grid_n[k];
+        d.gstart[k] = h.grid_start[k];
+        d.gstep[k] = h.grid_step[k];
+        d.gstop[k] = h.grid_stop[k];
+      }
+      continue;
+    }
     if (h.kind == SRS_FIELD_GAUSSIAN || h.kind == SRS_FIELD_UNIFORM_BOX) {
       if (h.pole_delta) return fail(SRS_ERR_INVALID, "pole errors apply to undulators only");
       for (int k = 0; k < 3; ++k) {
@@ -438,6 +476,8 @@ int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) 
     }
   }
   if (poles.empty()) poles.push_back(0.0);
+  c.und_local = false;
+  for (int u = 0; u < nund; ++u) c.und_local = c.und_local || und[u].kind != SRS_FIELD_UNDULATOR;
   HIPCHK(c.b_und.ensure(du.size() * sizeof(srs::DevUndulator)));
   HIPCHK(c.b_poles.ensure(poles.size() * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c.b_und.p, du.data(), du.size() * sizeof(srs::DevUndulator), hipMemcpyHostToDevice, s));
@@ -502,7 +542,7 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
   }
   srs::RK4Seg rs{0, L, nseg > 1 ? c.b_state.as<double>() : nullptr};
   if (nseg == 1) {
-    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt,
+    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.b_poles.as<double>(), P, np_run, r.bt,
                                  r.ck, rs, s));
     HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
     srs::FluxSeg const fs{0, L, 1, 1, field != nullptr, field};
@@ -514,7 +554,7 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
     for (int k = 0; k < nseg; ++k) {
       rs.lo = k * L;
       rs.hi = (k + 1) * L;
-      HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.b_poles.as<double>(), P, np_run, r.bt,
+      HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.b_poles.as<double>(), P, np_run, r.bt,
                                    r.ck, rs, c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
     }
@@ -534,9 +574,12 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
 
 std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context const& c) {
   char buf[512];
-  snprintf(buf, sizeof buf, "%d/%d/%d/%d/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%lld/%.17g", r.np, r.ns, r.ne, (int)r.mixed,
-           cfg.ne_per_item, cfg.tile, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
-           c.b_state.p, c.b_field.p, (long long)r.max_leg, r.weight);
+  // every value a captured launch holds: shapes, buffers, the field list's length and
+  // whether it needs the general field evaluation (the RK4 instantiation depends on it)
+  snprintf(buf, sizeof buf, "%d/%d/%d/%d/%d/%d/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%p/%lld/%.17g", r.np, r.ns, r.ne,
+           (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p,
+           c.b_out.p, c.b_obs.p, c.b_energies.p, c.b_state.p, c.b_field.p, c.b_poles.p, (long long)r.max_leg,
+           r.weight);
   return buf;
 }
 
@@ -556,14 +599,14 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
   auto* U = c.b_und.as<srs::DevUndulator>();
   double* const poles = c.b_poles.as<double>();
   if (nseg == 1) {
-    HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
+    HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, poles, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
     HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, 0, INT64_MAX, nch, 0,
                                 c.g_part.as<double>(), s));
   } else {
     HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
     HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
     for (int k = 0; k < nseg; ++k) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, poles, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, poles, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
                                    c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
     }
@@ -876,7 +919,7 @@ int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, 
   HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
   srs::BatchTraj bt;
   TRY(stage_batch_traj(c, tot, true, &bt));
-  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.b_poles.as<double>(),
+  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.und_local, c.b_poles.as<double>(),
                                c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0},
                                srs::RK4Seg{0, INT64_MAX, nullptr}, s));
   double* src[9] = {bt.x, bt.y, bt.z, bt.bx, bt.by, bt.bz, bt.ax, bt.ay, bt.az};
@@ -1123,7 +1166,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   {
     KernelTimer timer(c, s);
     if (nseg == 1) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
                                    srs::RK4Seg{0, INT64_MAX, nullptr}, s));
       HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, 0, INT64_MAX, nch, 0,
                                         c.g_part.as<double>(), s));
@@ -1131,7 +1174,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
       HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
       HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
       for (int k = 0; k < nseg; ++k) {
-        HIPCHK(srs::launch_rk4_batch(U, nund, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+        HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
                                      srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()}, c.stream2));
         HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
       }

@@ -29,6 +29,10 @@ struct DevUndulator {
   int kind;
   double w[3];
   double rc[3], rs[3];
+  // kind 3, field map: 3 doubles per grid point, ((ix*gn[1] + iy)*gn[2] + iz)
+  const double* gdata;
+  int gn[3];
+  double gstart[3], gstep[3], gstop[3];
 };
 
 // Per-particle state and trajectory grid (dt, nb, nf computed on the host with the
@@ -84,8 +88,9 @@ struct RK4Seg {
   int64_t lo, hi;
   double* state;
 };
-hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
-                            BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s);
+// local: the list holds elements other than undulators (Gaussian, box, field map)
+hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, const double* poles, const BatchParticle* P,
+                            int np, BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s);
 hipError_t launch_zero_counter(unsigned* c, hipStream_t s);
 // One time segment of the flux: steps [lo, hi) of both RK4 legs of every particle.
 // The six field sums of each (particle, energy, point) accumulate across segments in

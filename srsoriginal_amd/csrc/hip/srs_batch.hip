@@ -508,11 +508,12 @@ int batch_max_blocks(BatchConfig const& c, bool mixed) {
   return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
-hipError_t launch_rk4_batch(const DevUndulator* U, int nu, const double* poles, const BatchParticle* P, int np,
+hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, const double* poles, const BatchParticle* P,
+                            int np,
                             BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s) {
   dim3 const grid((np + 63) / 64);
-  if (nu == 1) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
-  else if (nu == 2) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  if (nu == 1 && !local) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  else if (nu == 2 && !local) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
   else rk4_batch_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
   return hipGetLastError();
 }

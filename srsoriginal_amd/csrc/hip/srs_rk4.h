@@ -17,9 +17,72 @@ namespace srs {
 
 constexpr double kInvSix = 0x1.5555555555555p-3;  // RN(1/6)
 
-// Gaussian (kind 1) or uniform box (kind 2) at (X, Y, Z), added to B:
-// TField3D_Gaussian::GetF (.cc:88-113), TField3D_UniformBox::GetF (.cc:94-117), the
-// position rotated about X, then Y, then Z as TVector3D::RotateSelfX/Y/Z.
+struct GridV {
+  double v[3];
+};
+
+// Field map at the rotated, translated position r, TField3D_Grid::GetF (.cc:124-250):
+// zero outside the open interval of every axis with more than one point; linear steps
+// a + (d (b - a)) / step, x first, then y, then z.  Not inlined: the field maps are the
+// rare element, and inlining their code into the RK4 step would cost every step
+// registers.
+__device__ __noinline__ GridV grid_eval(const double* __restrict__ gd, int n0, int n1, int n2, double st0, double st1,
+                                        double st2, double sp0, double sp1, double sp2, double so0, double so1,
+                                        double so2, double r0, double r1, double r2) {
+  GridV out{{0, 0, 0}};
+  int const n[3] = {n0, n1, n2};
+  double const start[3] = {st0, st1, st2}, step[3] = {sp0, sp1, sp2}, stop[3] = {so0, so1, so2};
+  double const r[3] = {r0, r1, r2};
+  int64_t i[3] = {0, 0, 0};
+  double d[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (n[k] <= 1) continue;
+    if (!(r[k] > start[k] && r[k] < stop[k])) return out;  // (also for a NaN position)
+    int64_t ik = (int64_t)((r[k] - start[k]) / step[k]);
+    ik = ik > n[k] - 2 ? n[k] - 2 : ik;  // the upper edge within rounding: last cell
+    i[k] = ik;
+    d[k] = (r[k] - start[k]) - (double)ik * step[k];
+  }
+  bool const hx = n0 > 1, hy = n1 > 1, hz = n2 > 1;
+  // the data index of (ix, iy, iz) is (ix n1 + iy) n2 + iz; axes with one point index 0
+  int64_t const s1 = n2, s0 = (int64_t)n1 * n2;
+  auto at = [&](int64_t ix, int64_t iy, int64_t iz, int c) { return gd[3 * (ix * s0 + iy * s1 + iz) + c]; };
+  auto lerp = [](double a, double b, double dd, double st) { return a + (dd * (b - a)) / st; };
+  int64_t const x0 = i[0], y0 = i[1], z0 = i[2];
+  for (int c = 0; c < 3; ++c) {
+    double v;
+    if (hx && hy && hz) {
+      double const v00 = lerp(at(x0, y0, z0, c), at(x0 + 1, y0, z0, c), d[0], step[0]);
+      double const v10 = lerp(at(x0, y0 + 1, z0, c), at(x0 + 1, y0 + 1, z0, c), d[0], step[0]);
+      double const v01 = lerp(at(x0, y0, z0 + 1, c), at(x0 + 1, y0, z0 + 1, c), d[0], step[0]);
+      double const v11 = lerp(at(x0, y0 + 1, z0 + 1, c), at(x0 + 1, y0 + 1, z0 + 1, c), d[0], step[0]);
+      v = lerp(lerp(v00, v10, d[1], step[1]), lerp(v01, v11, d[1], step[1]), d[2], step[2]);
+    } else if (hx && hy) {
+      v = lerp(lerp(at(x0, y0, 0, c), at(x0 + 1, y0, 0, c), d[0], step[0]),
+               lerp(at(x0, y0 + 1, 0, c), at(x0 + 1, y0 + 1, 0, c), d[0], step[0]), d[1], step[1]);
+    } else if (hx && hz) {
+      v = lerp(lerp(at(x0, 0, z0, c), at(x0 + 1, 0, z0, c), d[0], step[0]),
+               lerp(at(x0, 0, z0 + 1, c), at(x0 + 1, 0, z0 + 1, c), d[0], step[0]), d[2], step[2]);
+    } else if (hy && hz) {
+      v = lerp(lerp(at(0, y0, z0, c), at(0, y0 + 1, z0, c), d[1], step[1]),
+               lerp(at(0, y0, z0 + 1, c), at(0, y0 + 1, z0 + 1, c), d[1], step[1]), d[2], step[2]);
+    } else if (hx) {
+      v = lerp(at(x0, 0, 0, c), at(x0 + 1, 0, 0, c), d[0], step[0]);
+    } else if (hy) {
+      v = lerp(at(0, y0, 0, c), at(0, y0 + 1, 0, c), d[1], step[1]);
+    } else {
+      v = lerp(at(0, 0, z0, c), at(0, 0, z0 + 1, c), d[2], step[2]);
+    }
+    out.v[c] = v;
+  }
+  return out;
+}
+
+// Gaussian (kind 1), uniform box (kind 2) or field map (kind 3) at (X, Y, Z), added to
+// B: TField3D_Gaussian::GetF (.cc:88-113), TField3D_UniformBox::GetF (.cc:94-117),
+// TField3D_Grid::GetF (.cc:124-250), the position rotated about X, then Y, then Z as
+// TVector3D::RotateSelfX/Y/Z.
 __device__ __forceinline__ void field_add_local(DevUndulator const& f, double X, double Y, double Z, double* B) {
   double x = X, y = Y, z = Z;
   {
@@ -38,6 +101,14 @@ __device__ __forceinline__ void field_add_local(DevUndulator const& f, double X,
     y = f.rs[2] * xx + f.rc[2] * y;
   }
   double const r[3] = {x - f.center[0], y - f.center[1], z - f.center[2]};
+  if (f.kind == 3) {
+    GridV const g = grid_eval(f.gdata, f.gn[0], f.gn[1], f.gn[2], f.gstart[0], f.gstart[1], f.gstart[2], f.gstep[0],
+                              f.gstep[1], f.gstep[2], f.gstop[0], f.gstop[1], f.gstop[2], r[0], r[1], r[2]);
+    B[0] += g.v[0];
+    B[1] += g.v[1];
+    B[2] += g.v[2];
+    return;
+  }
   if (f.kind == 1) {
     double frac = 1;
 #pragma unroll
@@ -57,14 +128,10 @@ __device__ __forceinline__ void field_add_local(DevUndulator const& f, double X,
   }
 }
 
-// Field of one element at (X, Y, Z), added to B.  poles = this particle's pole
+// Field of one undulator at (X, Y, Z), added to B.  poles = this particle's pole
 // deltas (used when f.pole_off >= 0).
-__device__ __forceinline__ void field_add(DevUndulator const& f, const double* __restrict__ poles, double X, double Y,
-                                          double Z, double* B) {
-  if (f.kind != 0) {
-    field_add_local(f, X, Y, Z, B);
-    return;
-  }
+__device__ __forceinline__ void field_add_und(DevUndulator const& f, const double* __restrict__ poles, double X,
+                                              double Y, double Z, double* B) {
   double const D = (X - f.center[0]) * f.u[0] + (Y - f.center[1]) * f.u[1] + (Z - f.center[2]) * f.u[2];
   double const T = 1 + D * f.taper;
   bool const inside = !(D > f.hi0 || D < f.lo0);
@@ -83,6 +150,15 @@ __device__ __forceinline__ void field_add(DevUndulator const& f, const double* _
     double const b = term ? ((f.field[k] * fct) * s) * T : (f.field[k] * s) * g;
     B[k] += inside ? b : 0.0;
   }
+}
+
+// Field of any element at (X, Y, Z), added to B.
+__device__ __forceinline__ void field_add(DevUndulator const& f, const double* __restrict__ poles, double X, double Y,
+                                          double Z, double* B) {
+  if (f.kind != 0)
+    field_add_local(f, X, Y, Z, B);
+  else
+    field_add_und(f, poles, X, Y, Z, B);
 }
 
 // Field, its derivative along the period direction and its derivative with
@@ -128,7 +204,8 @@ __device__ __forceinline__ void field_add_jet(DevUndulator const& f, const doubl
   }
 }
 
-// Undulator records: NU > 0 held in registers, NU == 0 read from memory (any count).
+// Field records: NU > 0 undulators held in registers (lists of undulators only: the
+// step then carries no code of the other kinds), NU == 0 any elements read from memory.
 template <int NU>
 struct Fields {
   DevUndulator u[NU];
@@ -142,7 +219,7 @@ struct Fields {
   __device__ __forceinline__ void eval(double X, double Y, double Z, double* B) const {
     B[0] = B[1] = B[2] = 0.0;
 #pragma unroll
-    for (int k = 0; k < NU; ++k) field_add(u[k], poles, X, Y, Z, B);
+    for (int k = 0; k < NU; ++k) field_add_und(u[k], poles, X, Y, Z, B);
   }
   __device__ __forceinline__ void jet(double X, double Y, double Z, FieldJet& J) const {
     J = FieldJet{{0, 0, 0}, {0, 0, 0}, 0.0, -1};

@@ -130,6 +130,29 @@ PYBIND11_MODULE(_sr, m) {
              u.center = v3(center);
              e.AddMagneticField(u);
            })
+      .def("add_bfield_file",
+           [](srs::OSCARSSR& e, std::string const& file, std::string const& format, std::vector<double> rot,
+              std::vector<double> trans, std::vector<double> scale) {
+             // OSCARSSR::AddMagneticField(file, format, ...), OSCARSSR.cc:68-90
+             srs::IdealUndulator u;
+             u.kind = srs::FieldKind::Grid;
+             u.rot = v3(rot);
+             u.center = v3(trans);
+             u.grid = std::make_shared<srs::FieldGrid>(srs::read_grid_file(file, format, u.rot, scale));
+             e.AddMagneticField(u);
+           })
+      .def("add_bfield_interpolated",
+           [](srs::OSCARSSR& e, std::vector<std::pair<double, std::string>> mapping, std::string const& format,
+              double parameter, std::vector<double> rot, std::vector<double> trans, std::vector<double> scale) {
+             // OSCARSSR::AddMagneticFieldInterpolated, OSCARSSR.cc:94-114
+             srs::IdealUndulator u;
+             u.kind = srs::FieldKind::Grid;
+             u.rot = v3(rot);
+             u.center = v3(trans);
+             u.grid = std::make_shared<srs::FieldGrid>(
+                 srs::read_grid_interpolated(mapping, format, parameter, u.rot, scale));
+             e.AddMagneticField(u);
+           })
       .def("clear_bfields", &srs::OSCARSSR::ClearMagneticFields)
       .def("n_bfields", [](srs::OSCARSSR const& e) { return e.Fields().size(); })
       .def("get_bfield",
@@ -188,6 +211,13 @@ PYBIND11_MODULE(_sr, m) {
                d["kind"] = (int)u.kind;
                d["width"] = std::vector<double>{u.width.x, u.width.y, u.width.z};
                d["rotations"] = std::vector<double>{u.rot.x, u.rot.y, u.rot.z};
+               if (u.grid) {
+                 d["grid"] = u.grid->data;
+                 d["grid_n"] = std::vector<int>{u.grid->n[0], u.grid->n[1], u.grid->n[2]};
+                 d["grid_start"] = std::vector<double>(u.grid->start, u.grid->start + 3);
+                 d["grid_step"] = std::vector<double>(u.grid->step, u.grid->step + 3);
+                 d["grid_stop"] = std::vector<double>(u.grid->stop, u.grid->stop + 3);
+               }
                out.append(d);
              }
              return out;

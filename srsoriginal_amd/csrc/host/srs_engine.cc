@@ -5,8 +5,11 @@
 #include "srs_engine.h"
 
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
+#include <fstream>
+#include <sstream>
 
 #include "srs_hip.h"
 
@@ -92,10 +95,13 @@ srs_traj view(Trajectory const& t) {
 
 Vec3 IdealUndulator::eval(Vec3 const& X) const {
   if (kind != FieldKind::Undulator) {
-    // TField3D_Gaussian::GetF (.cc:88-113) and TField3D_UniformBox::GetF (.cc:94-117)
+    // TField3D_Gaussian::GetF (.cc:88-113), TField3D_UniformBox::GetF (.cc:94-117) and
+    // TField3D_Grid::GetF (.cc:124-131): rotate the position, then subtract the centre
+    // (the translation)
     Vec3 Xr = X;
     rotate_xyz(Xr, rot);
     Vec3 const RX = sub(Xr, center);
+    if (kind == FieldKind::Grid) return grid->eval(RX);
     if (kind == FieldKind::Gaussian) {
       double frac = 1;
       if (width.x > 0) frac *= std::exp(-std::pow(RX.x / width.x, 2) / 2.);
@@ -139,6 +145,372 @@ Vec3 IdealUndulator::eval(Vec3 const& X) const {
     return Vec3(((field.x * f) * s) * T, ((field.y * f) * s) * T, ((field.z * f) * s) * T);
   }
   return Vec3((field.x * s) * T, (field.y * s) * T, (field.z * s) * T);
+}
+
+// ---- field maps (TField3D_Grid) ---------------------------------------------------
+
+Vec3 FieldGrid::eval(Vec3 const& X) const {
+  // TField3D_Grid::GetF, TField3D_Grid.cc:124-250: zero outside (open interval) on every
+  // axis with more than one point; cell index by truncation, offset d = (x - start) - i*step;
+  // each linear step is a + (d*(b - a))/step, x first, then y, then z.
+  size_t i[3] = {0, 0, 0};
+  double d[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a) {
+    if (n[a] <= 1) continue;
+    if (!(X[a] > start[a] && X[a] < stop[a])) return Vec3(0, 0, 0);  // (also for a NaN position)
+    i[a] = (size_t)((X[a] - start[a]) / step[a]);
+    // a point within rounding of the upper edge would index one past the grid (the
+    // reference reads beyond its array there); stay in the last cell
+    if (i[a] > (size_t)(n[a] - 2)) i[a] = (size_t)(n[a] - 2);
+    d[a] = (X[a] - start[a]) - i[a] * step[a];
+  }
+  auto at = [&](size_t ix, size_t iy, size_t iz) {
+    size_t const k = 3 * ((ix * (size_t)n[1] + iy) * (size_t)n[2] + iz);
+    return Vec3(data[k], data[k + 1], data[k + 2]);
+  };
+  auto lerp = [](Vec3 const& a, Vec3 const& b, double dd, double st) {
+    return Vec3(a.x + (dd * (b.x - a.x)) / st, a.y + (dd * (b.y - a.y)) / st, a.z + (dd * (b.z - a.z)) / st);
+  };
+  bool const hx = n[0] > 1, hy = n[1] > 1, hz = n[2] > 1;
+  size_t const x0 = i[0], y0 = i[1], z0 = i[2];
+  if (hx && hy && hz) {
+    Vec3 const v00 = lerp(at(x0, y0, z0), at(x0 + 1, y0, z0), d[0], step[0]);
+    Vec3 const v10 = lerp(at(x0, y0 + 1, z0), at(x0 + 1, y0 + 1, z0), d[0], step[0]);
+    Vec3 const v01 = lerp(at(x0, y0, z0 + 1), at(x0 + 1, y0, z0 + 1), d[0], step[0]);
+    Vec3 const v11 = lerp(at(x0, y0 + 1, z0 + 1), at(x0 + 1, y0 + 1, z0 + 1), d[0], step[0]);
+    return lerp(lerp(v00, v10, d[1], step[1]), lerp(v01, v11, d[1], step[1]), d[2], step[2]);
+  }
+  if (hx && hy) return lerp(lerp(at(x0, y0, 0), at(x0 + 1, y0, 0), d[0], step[0]),
+                            lerp(at(x0, y0 + 1, 0), at(x0 + 1, y0 + 1, 0), d[0], step[0]), d[1], step[1]);
+  if (hx && hz) return lerp(lerp(at(x0, 0, z0), at(x0 + 1, 0, z0), d[0], step[0]),
+                            lerp(at(x0, 0, z0 + 1), at(x0 + 1, 0, z0 + 1), d[0], step[0]), d[2], step[2]);
+  if (hy && hz) return lerp(lerp(at(0, y0, z0), at(0, y0 + 1, z0), d[1], step[1]),
+                            lerp(at(0, y0, z0 + 1), at(0, y0 + 1, z0 + 1), d[1], step[1]), d[2], step[2]);
+  // one axis: the data are the points along it (index = the axis index)
+  if (hx) return lerp(Vec3(data[3 * x0], data[3 * x0 + 1], data[3 * x0 + 2]),
+                      Vec3(data[3 * x0 + 3], data[3 * x0 + 4], data[3 * x0 + 5]), d[0], step[0]);
+  if (hy) return lerp(Vec3(data[3 * y0], data[3 * y0 + 1], data[3 * y0 + 2]),
+                      Vec3(data[3 * y0 + 3], data[3 * y0 + 4], data[3 * y0 + 5]), d[1], step[1]);
+  return lerp(Vec3(data[3 * z0], data[3 * z0 + 1], data[3 * z0 + 2]),
+              Vec3(data[3 * z0 + 3], data[3 * z0 + 4], data[3 * z0 + 5]), d[2], step[2]);
+}
+
+namespace {
+
+// First number on a header line (istream extraction, as TField3D_Grid::GetHeaderValue).
+double header_value(std::string const& L) {
+  std::istringstream S(L);
+  double v = 0;
+  if (!(S >> v)) throw std::invalid_argument("cannot read header value");
+  return v;
+}
+// A header line that starts with the comment character, then the number (SRW format,
+// TField3D_Grid::GetHeaderValueSRW).
+double header_value_srw(std::string const& L, char comment) {
+  std::istringstream S(L);
+  char c = 0;
+  S.get(c);
+  if (c != comment) throw std::invalid_argument("something is wrong with the comment character, it was not seen");
+  double v = 0;
+  if (!(S >> v)) throw std::invalid_argument("cannot read header value SRW format");
+  return v;
+}
+std::string next_line(std::istream& f) {
+  std::string L;
+  std::getline(f, L);
+  return L;
+}
+void check_points(int nx, int ny, int nz) {
+  // the reference tests NX, NY and NY (not NZ); an NZ < 1 grid has no data either
+  if (nx < 1 || ny < 1 || nz < 1) throw std::out_of_range("invalid number of points in at least one dimension");
+  if (nx == 1 && ny == 1 && nz == 1) throw std::out_of_range("invalid dimensions");
+}
+void finish_axes(FieldGrid& g) {
+  for (int a = 0; a < 3; ++a) g.stop[a] = g.start[a] + (g.n[a] - 1) * g.step[a];
+}
+// Header of the OSCARS format scaled as TField3D_Grid::ReadFile does (.cc:318-400): the
+// grid keeps its middle point while the steps scale.
+void oscars_axes(FieldGrid& g, double const hv[10], std::vector<double> const& scale) {
+  for (int a = 0; a < 3; ++a) {
+    double const start_in = hv[1 + 3 * a], step_in = hv[2 + 3 * a];
+    int const n = (int)hv[3 + 3 * a];
+    double const step = scale.size() > (size_t)a ? step_in * scale[a] : step_in;
+    double const middle = start_in + step_in * (n - 1) / 2.;
+    g.n[a] = n;
+    g.step[a] = step;
+    g.start[a] = middle - step * (n - 1) / 2.;
+  }
+  check_points(g.n[0], g.n[1], g.n[2]);
+  finish_axes(g);
+}
+void push_rotated(std::vector<double>& out, double fx, double fy, double fz, Vec3 const& rot) {
+  Vec3 F(fx, fy, fz);
+  rotate_xyz(F, rot);
+  out.push_back(F.x);
+  out.push_back(F.y);
+  out.push_back(F.z);
+}
+
+FieldGrid read_oscars(std::string const& file, Vec3 const& rot, std::vector<double> const& scale) {
+  std::ifstream f(file);
+  if (!f) throw std::invalid_argument("cannot open file for reading");
+  next_line(f);  // comment
+  double hv[10] = {0};
+  for (int k = 1; k < 10; ++k) hv[k] = header_value(next_line(f));
+  FieldGrid g;
+  oscars_axes(g, hv, scale);
+  double const sx = scale.size() > 3 ? scale[3] : 1, sy = scale.size() > 4 ? scale[4] : 1,
+               sz = scale.size() > 5 ? scale[5] : 1;
+  size_t const np = (size_t)g.n[0] * g.n[1] * g.n[2];
+  g.data.reserve(3 * np);
+  std::string L;
+  for (size_t k = 0; k < np; ++k) {
+    std::getline(f, L);
+    if (f.eof()) throw std::invalid_argument("error reading file.  Check format");
+    std::istringstream S(L);
+    double fx, fy, fz;
+    S >> fx >> fy >> fz;
+    if (sx != 1) fx *= sx;
+    if (sy != 1) fy *= sy;
+    if (sz != 1) fz *= sz;
+    if (S.fail()) throw std::invalid_argument("error reading file.  Check format");
+    push_rotated(g.data, fx, fy, fz, rot);
+  }
+  return g;
+}
+
+// OSCARS1D: columns named by the format string after its first 8 characters (one of
+// X/Y/Z and any of Bx/By/Bz, Ex..., Fx...), sorted by position and resampled to 10000
+// points per metre by linear interpolation (TField3D_Grid.cc:533-831).
+FieldGrid read_oscars1d(std::string const& file, std::string const& format, Vec3 const& rot,
+                        std::vector<double> const& scale, char comment) {
+  std::ifstream f(file);
+  if (!f) throw std::invalid_argument("cannot open file for reading");
+  int order[4] = {-1, -1, -1, -1};
+  std::istringstream fs(format.size() > 8 ? format.substr(8) : std::string());
+  std::string c, axis;
+  int index = 0, xdim = 0, fdim = 0;
+  bool has[4] = {false, false, false, false};
+  while (fs >> c) {
+    if (index > 3) throw std::out_of_range("spatial or B-field dimensions are too large(index>3)");
+    int slot;
+    if (c == "X" || c == "Y" || c == "Z") {
+      axis = c;
+      ++xdim;
+      slot = 0;
+    } else if (c == "Bx" || c == "Ex" || c == "Fx") {
+      slot = 1;
+    } else if (c == "By" || c == "Ey" || c == "Fy") {
+      slot = 2;
+    } else if (c == "Bz" || c == "Ez" || c == "Fz") {
+      slot = 3;
+    } else {
+      throw std::invalid_argument("only excepts X Y Z Bx By Bz");
+    }
+    if (slot > 0) {
+      has[slot] = true;
+      ++fdim;
+    }
+    order[index++] = slot;
+  }
+  if (xdim != 1) throw std::out_of_range("spatial or B-field dimensions are too large");
+  int const ncols = xdim + fdim;
+  std::vector<std::array<double, 4>> in;
+  std::string L;
+  std::getline(f, L);  // comment
+  while (std::getline(f, L)) {
+    size_t const first = L.find_first_not_of(" \t");
+    if (first == std::string::npos || L[first] == comment) continue;
+    std::istringstream S(L);
+    double v[4] = {0, 0, 0, 0};
+    for (int i = 0; i < ncols; ++i) {
+      S >> v[order[i]];
+      if (S.fail()) throw std::length_error("something is incorrect with data format or iformat string");
+    }
+    for (size_t k = 0; k < scale.size() && k < 4; ++k) v[order[k]] *= scale[k];
+    in.push_back({{v[0], v[1], v[2], v[3]}});
+  }
+  std::sort(in.begin(), in.end(),
+            [](std::array<double, 4> const& a, std::array<double, 4> const& b) { return a[0] < b[0]; });
+  if (in.size() < 2) throw std::length_error("not enough data points");
+  double const first = in.front()[0], last = in.back()[0];
+  size_t const np = (size_t)((last - first) * 10000);
+  double const step = (last - first) / (double)(np - 1);
+  FieldGrid g;
+  int const a = axis == "X" ? 0 : (axis == "Y" ? 1 : 2);
+  g.n[a] = (int)np;
+  g.start[a] = first;
+  g.stop[a] = last;
+  g.step[a] = step;
+  if (np < 2) throw std::out_of_range("invalid dimensions");
+  g.data.reserve(3 * np);
+  size_t minbin = 0, after = 0, before = 0;
+  for (size_t i = 0; i != np; ++i) {
+    double const z = i * step + first;
+    for (size_t j = minbin + 1; j != in.size(); ++j)
+      if (in[j][0] > z) {
+        after = j;
+        before = j - 1;
+        minbin = j - 1;
+        break;
+      }
+    double const dz = in[after][0] - in[before][0];
+    double const sx = !has[1] ? 0 : (in[after][1] - in[before][1]) / dz;
+    double const sy = !has[2] ? 0 : (in[after][2] - in[before][2]) / dz;
+    double const sz = !has[3] ? 0 : (in[after][3] - in[before][3]) / dz;
+    double const bx = !has[1] ? 0 : in[before][1] + (z - in[before][0]) * sx;
+    double const by = !has[2] ? 0 : in[before][2] + (z - in[before][0]) * sy;
+    double const bz = !has[3] ? 0 : in[before][3] + (z - in[before][0]) * sz;
+    push_rotated(g.data, bx, by, bz, rot);
+  }
+  return g;
+}
+
+FieldGrid read_srw(std::string const& file, Vec3 const& rot, char comment) {
+  std::ifstream f(file);
+  if (!f) throw std::invalid_argument("cannot open file for reading SRW format");
+  next_line(f);
+  FieldGrid g;
+  for (int a = 0; a < 3; ++a) {
+    g.start[a] = header_value_srw(next_line(f), comment);
+    g.step[a] = header_value_srw(next_line(f), comment);
+    g.n[a] = (int)header_value_srw(next_line(f), comment);
+  }
+  check_points(g.n[0], g.n[1], g.n[2]);
+  finish_axes(g);
+  size_t const np = (size_t)g.n[0] * g.n[1] * g.n[2];
+  g.data.reserve(3 * np);
+  std::string L;
+  for (size_t k = 0; k < np; ++k) {
+    std::getline(f, L);
+    std::istringstream S(L);
+    double fx, fy, fz;
+    S >> fx >> fy >> fz;
+    if (S.fail() || f.fail()) throw std::invalid_argument("input file stream failure");
+    push_rotated(g.data, fx, fy, fz, rot);
+  }
+  return g;
+}
+
+// SPECTRA: "dx dy dz nx ny nz" with the steps in mm, the grid centred on the origin
+// (TField3D_Grid.cc:1002-1146).
+FieldGrid read_spectra(std::string const& file, Vec3 const& rot) {
+  std::ifstream f(file);
+  if (!f) throw std::invalid_argument("cannot open file");
+  next_line(f);
+  std::istringstream S(next_line(f));
+  FieldGrid g;
+  size_t n[3] = {0, 0, 0};
+  S >> g.step[0] >> g.step[1] >> g.step[2] >> n[0] >> n[1] >> n[2];
+  if (S.fail()) throw std::invalid_argument("file stream failure");
+  for (int a = 0; a < 3; ++a) {
+    g.step[a] /= 1000.;
+    if (n[a] > (size_t)1 << 30) throw std::out_of_range("invalid number of points in at least one dimension");
+    g.n[a] = (int)n[a];
+  }
+  check_points(g.n[0], g.n[1], g.n[2]);
+  for (int a = 0; a < 3; ++a) g.start[a] = -(g.step[a] * (n[a] - 1)) / 2;
+  for (int a = 0; a < 3; ++a) g.stop[a] = g.start[a] + (n[a] - 1) * g.step[a];
+  size_t const np = n[0] * n[1] * n[2];
+  g.data.reserve(3 * np);
+  std::string L;
+  for (size_t k = 0; k < np; ++k) {
+    std::getline(f, L);
+    if (f.eof()) throw std::invalid_argument("file stream failure");
+    std::istringstream D(L);
+    double fx, fy, fz;
+    D >> fx >> fy >> fz;
+    if (D.fail()) throw std::invalid_argument("file stream failure");
+    push_rotated(g.data, fx, fy, fz, rot);
+  }
+  return g;
+}
+
+}  // namespace
+
+FieldGrid read_grid_file(std::string const& file, std::string const& format, Vec3 const& rot,
+                         std::vector<double> const& scale, char comment) {
+  // OSCARSSR::AddMagneticField(file, ...), OSCARSSR.cc:68-90 and the TField3D_Grid
+  // constructor (TField3D_Grid.cc:25-45): SRW and SPECTRA take no scaling
+  std::string const F = upper(format);
+  if (F == "OSCARS") return read_oscars(file, rot, scale);
+  if (F == "SRW") return read_srw(file, rot, comment);
+  if (F == "SPECTRA") return read_spectra(file, rot);
+  if (F.size() > 8 && F.compare(0, 8, "OSCARS1D") == 0) return read_oscars1d(file, format, rot, scale, comment);
+  throw std::invalid_argument("Incorrect format in format string");
+}
+
+FieldGrid read_grid_interpolated(std::vector<std::pair<double, std::string>> const& mapping,
+                                 std::string const& format, double parameter, Vec3 const& rot,
+                                 std::vector<double> const& scale) {
+  // OSCARSSR::AddMagneticFieldInterpolated (OSCARSSR.cc:94-114) reads the OSCARS format
+  // only (TField3D_Grid.cc:50-70: its other formats leave the grid unset); the maps are
+  // sorted by parameter, must share one header, and each grid value is the linear
+  // interpolation between the two maps that bracket the parameter
+  // (InterpolateFromFiles / InterpolateFields, TField3D_Grid.cc:1150-1445).
+  if (upper(format) != "OSCARS") throw std::invalid_argument("Incorrect format in format string");
+  std::vector<std::pair<double, std::string>> m = mapping;
+  std::sort(m.begin(), m.end(), [](std::pair<double, std::string> const& a,
+                                          std::pair<double, std::string> const& b) { return a.first < b.first; });
+  if (m.size() < 2) throw std::invalid_argument("need at least two maps to interpolate");
+  int before = -1, after = -1;
+  for (size_t i = 0; i != m.size(); ++i) {
+    if (i == 0 && parameter < m[i].first) throw std::out_of_range("parameter below the mapping range");
+    if (m[i].first < parameter) {
+      before = (int)i;
+    } else {
+      after = (int)i;
+      break;
+    }
+  }
+  if (after < 0) throw std::out_of_range("parameter above the mapping range");
+  if (before < 0) before = after;  // parameter equal to the first map's: that map (the reference leaves the index unset)
+  std::vector<std::unique_ptr<std::ifstream>> files;
+  for (auto const& e : m) {
+    files.emplace_back(new std::ifstream(e.second));
+    if (!*files.back()) throw std::invalid_argument("cannot open file for reading");
+  }
+  double hv[10] = {0};
+  for (int k = 0; k < 10; ++k) {
+    std::string const L0 = next_line(*files[0]);
+    hv[k] = k == 0 ? 0.0 : header_value(L0);
+    for (size_t i = 1; i < files.size(); ++i) {
+      std::string const Li = next_line(*files[i]);
+      if (k > 0 && header_value(Li) != hv[k]) throw std::invalid_argument("maps differ in their headers");
+    }
+  }
+  FieldGrid g;
+  oscars_axes(g, hv, scale);
+  double const sx = scale.size() > 3 ? scale[3] : 1, sy = scale.size() > 4 ? scale[4] : 1,
+               sz = scale.size() > 5 ? scale[5] : 1;
+  size_t const np = (size_t)g.n[0] * g.n[1] * g.n[2];
+  g.data.reserve(3 * np);
+  std::vector<Vec3> F(files.size());
+  std::string L;
+  for (size_t k = 0; k < np; ++k) {
+    for (size_t i = 0; i < files.size(); ++i) {
+      std::getline(*files[i], L);
+      if (files[i]->eof()) throw std::invalid_argument("error reading file.  Check format");
+      std::istringstream S(L);
+      double fx, fy, fz;
+      S >> fx >> fy >> fz;
+      if (sx != 1) fx *= sx;
+      if (sy != 1) fy *= sy;
+      if (sz != 1) fz *= sz;
+      if (S.fail()) throw std::invalid_argument("error reading file.  Check format");
+      F[i] = Vec3(fx, fy, fz);
+    }
+    double const diff = parameter - m[before].first;
+    double const dp = m[after].first - m[before].first;
+    Vec3 v = F[before];
+    if (after != before) {
+      Vec3 const slope((F[after].x - F[before].x) / dp, (F[after].y - F[before].y) / dp,
+                       (F[after].z - F[before].z) / dp);
+      v = Vec3(F[before].x + slope.x * diff, F[before].y + slope.y * diff, F[before].z + slope.z * diff);
+    }
+    push_rotated(g.data, v.x, v.y, v.z, rot);
+  }
+  return g;
 }
 
 Vec3 FieldContainer::eval(Vec3 const& X) const {
@@ -622,6 +994,15 @@ void OSCARSSR::device_fields(std::vector<srs_field>& out) const {
     d.rotations[0] = u.rot.x;
     d.rotations[1] = u.rot.y;
     d.rotations[2] = u.rot.z;
+    if (u.grid) {
+      d.grid = u.grid->data.data();
+      for (int k = 0; k < 3; ++k) {
+        d.grid_n[k] = u.grid->n[k];
+        d.grid_start[k] = u.grid->start[k];
+        d.grid_step[k] = u.grid->step[k];
+        d.grid_stop[k] = u.grid->stop[k];
+      }
+    }
     out.push_back(d);
   }
 }

@@ -33,7 +33,26 @@ struct Vec3 {
 // (1 + delta_p) for the 2N interior half-period poles.
 // A magnetic field element: ideal undulator (TField3D_IdealUndulator), 3-D Gaussian
 // (TField3D_Gaussian) or uniform box (TField3D_UniformBox).
-enum class FieldKind { Undulator = 0, Gaussian = 1, UniformBox = 2 };
+enum class FieldKind { Undulator = 0, Gaussian = 1, UniformBox = 2, Grid = 3 };
+
+// A field map on a regular grid (TField3D_Grid): field vectors x-major (index
+// (ix*ny + iy)*nz + iz), stored already rotated; linear interpolation along the axes
+// that have more than one point, zero outside (TField3D_Grid.cc:124-250).
+struct FieldGrid {
+  int n[3] = {1, 1, 1};
+  double start[3] = {0, 0, 0}, step[3] = {0, 0, 0}, stop[3] = {0, 0, 0};
+  std::vector<double> data;  // 3 per point
+  Vec3 eval(Vec3 const& X) const;  // X already rotated and translated
+};
+
+// Readers of the reference's field-map formats (TField3D_Grid.cc:318-1146) and the
+// parameter interpolation between maps (:1150-1445).  rot is applied to every field
+// vector as it is read (TVector3D::RotateSelfXYZ); scale as the reference applies it.
+FieldGrid read_grid_file(std::string const& file, std::string const& format, Vec3 const& rot,
+                         std::vector<double> const& scale, char comment = '#');
+FieldGrid read_grid_interpolated(std::vector<std::pair<double, std::string>> const& mapping,
+                                 std::string const& format, double parameter, Vec3 const& rot,
+                                 std::vector<double> const& scale);
 
 struct IdealUndulator {
   FieldKind kind = FieldKind::Undulator;
@@ -42,7 +61,8 @@ struct IdealUndulator {
   double phase = 0, taper = 0;
   std::vector<double> pole_delta;  // empty = ideal
   Vec3 width;                      // Gaussian sigma / box full width (<= 0: unbounded axis)
-  Vec3 rot;                        // Gaussian / box: rotation of the position, X then Y then Z
+  Vec3 rot;                        // Gaussian / box / grid: rotation of the position, X then Y then Z
+  std::shared_ptr<const FieldGrid> grid;  // kind Grid (center = translation)
   Vec3 eval(Vec3 const& X) const;
 };
 

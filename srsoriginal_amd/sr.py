@@ -167,6 +167,42 @@ class sr:
         trans = _opt3(translation, "translation")
         self._e.add_field(2, _rotate_xyz(field, rot), w, rot, trans)
 
+    def add_bfield_file(self, ifile="", iformat="", rotations=None, translation=None, scale=None):
+        """OSCARSSR_AddMagneticField (OSCARSSR_Python.cc:337-414): a field map read from
+        ifile in iformat 'OSCARS', 'OSCARS1D <columns>', 'SRW' or 'SPECTRA'
+        (TField3D_Grid.cc:318-1146); scale multiplies the steps (OSCARS: x, y, z then the
+        three field components) or the columns in order (OSCARS1D)."""
+        if not ifile or not iformat:
+            raise ValueError("'ifile' or 'iformat' is blank")
+        rot = _opt3(rotations, "rotations")
+        trans = _opt3(translation, "translation")
+        sc = [float(v) for v in (scale or [])]
+        try:
+            self._e.add_bfield_file(str(ifile), str(iformat), rot, trans, sc)
+        except Exception:
+            raise ValueError("Could not import magnetic field.  Check 'ifile' and 'iformat' are correct")
+
+    def add_bfield_interpolated(self, mapping, iformat, parameter, rotations=None, translation=None, scale=None):
+        """OSCARSSR_AddMagneticFieldInterpolated (OSCARSSR_Python.cc:419-521): mapping is a
+        list of [parameter value, file] pairs of OSCARS-format maps with one header; every
+        grid value is interpolated linearly in the parameter between the two maps that
+        bracket it (TField3D_Grid.cc:1150-1445)."""
+        pairs = []
+        for m in mapping or []:
+            if len(m) != 2:
+                raise ValueError("Incorrect format in 'mapping'")
+            f = m[1].decode() if isinstance(m[1], bytes) else str(m[1])
+            pairs.append((float(m[0]), f))
+        if not iformat:
+            raise ValueError("'iformat' is blank")
+        rot = _opt3(rotations, "rotations")
+        trans = _opt3(translation, "translation")
+        sc = [float(v) for v in (scale or [])]
+        try:
+            self._e.add_bfield_interpolated(pairs, str(iformat), float(parameter), rot, trans, sc)
+        except Exception:
+            raise ValueError("Could not import magnetic field.  Check filenames and 'iformat' are correct")
+
     def get_bfield(self, x):
         return self._e.get_bfield(_vec3(x, "x"))
 

@@ -67,6 +67,36 @@ def undulators(a):
     return out
 
 
+def field_maps(a):
+    """Field maps of a case in the harness's order (gridfile1, gridfile2, then gridmap), as
+    keyword arguments of sr.add_bfield_file / sr.add_bfield_interpolated, with the file
+    paths resolved under tests/golden."""
+    out = []
+    for k in ("1", "2"):
+        if "gridfile" + k not in a:
+            continue
+        out.append(("file", dict(ifile=os.path.join(GOLDEN, a["gridfile" + k]), iformat=a.get("gridfmt" + k, "OSCARS"),
+                                 rotations=vec(a, "gridrot" + k), translation=vec(a, "gridtrans" + k),
+                                 scale=vec(a, "gridscale" + k))))
+    if "gridmap" in a:
+        mapping = []
+        for e in a["gridmap"].split(";"):
+            p, f = e.split(":", 1)
+            mapping.append([float(p), os.path.join(GOLDEN, f)])
+        out.append(("interpolated", dict(mapping=mapping, iformat=a.get("gridmapfmt", "OSCARS"),
+                                         parameter=float(a["gridparam"]), rotations=vec(a, "gridmaprot"),
+                                         translation=vec(a, "gridmaptrans"), scale=vec(a, "gridmapscale"))))
+    return out
+
+
+def add_field_maps(osr, a):
+    for kind, kw in field_maps(a):
+        if kind == "file":
+            osr.add_bfield_file(**kw)
+        else:
+            osr.add_bfield_interpolated(**kw)
+
+
 def pole(a):
     u = vec(a, "poles")
     if not u:

@@ -43,7 +43,7 @@ def run(mode, args):
         path = f.name
     cmd = [HARNESS, mode] + list(args) + ["out=" + path]
     t0 = time.time()
-    subprocess.run(cmd, check=True)
+    subprocess.run(cmd, check=True, cwd=HERE)  # field-map paths are relative to tests/golden
     dt = time.time() - t0
     a = np.fromfile(path, dtype="<f8")
     os.unlink(path)
@@ -393,6 +393,125 @@ def poles_pd_and_fd_gradient():
         dm = d.copy(); dm[p] -= h
         grad[p] = (np.sum((pd(dp) - p0) ** 2) - np.sum((pd(dm) - p0) ** 2)) / (2 * h)
     return base, {"deltas": d, "pd_ideal": p0, "pd_scaled": pdd, "loss": np.array(np.sum((pdd - p0) ** 2)), "fd_grad": grad}, time.time() - t0
+
+# ---- field maps (TField3D_Grid readers, TField3D_Grid.cc:318-1146) ---------------------
+# Small synthetic maps in each of the reference's formats, written here with fixed content
+# (tests/golden/fields/), then read by the reference engine.  An undulator-like vertical
+# field By = B0 sin(2 pi z / lam) with a transverse roll-off and a small Bx, Bz.
+FIELDS = os.path.join(HERE, "fields")
+
+
+def _map_value(x, y, z, b0=0.8, lam=0.04):
+    k = 2 * np.pi / lam
+    by = b0 * np.cos(30.0 * x) * np.cosh(20.0 * y) * np.sin(k * z) * np.exp(-(z / 0.22) ** 8)
+    bx = 0.02 * b0 * np.sin(30.0 * x) * np.sin(k * z + 0.3) * np.exp(-(z / 0.22) ** 8)
+    bz = 0.01 * b0 * y * 20.0 * np.cos(k * z) * np.exp(-(z / 0.22) ** 8)
+    return bx, by, bz
+
+
+def write_field_maps():
+    os.makedirs(FIELDS, exist_ok=True)
+    # OSCARS: comment, then x0, dx, nx, y0, dy, ny, z0, dz, nz, then Bx By Bz x-major
+    def oscars(path, b0):
+        with open(path, "w") as f:
+            f.write("# synthetic undulator-like map, x-major\n")
+            for v, c in ((-0.002, "X start"), (0.002, "X step"), (3, "X points"), (-0.001, "Y start"),
+                         (0.001, "Y step"), (3, "Y points"), (-0.3, "Z start"), (0.002, "Z step"), (301, "Z points")):
+                f.write("%r  # %s\n" % (v, c))
+            for ix in range(3):
+                for iy in range(3):
+                    for iz in range(301):
+                        bx, by, bz = _map_value(-0.002 + 0.002 * ix, -0.001 + 0.001 * iy, -0.3 + 0.002 * iz, b0)
+                        f.write("%.12e %.12e %.12e\n" % (bx, by, bz))
+    oscars(os.path.join(FIELDS, "map_oscars.dat"), 0.8)
+    oscars(os.path.join(FIELDS, "map_oscars_gap10.dat"), 1.0)
+    oscars(os.path.join(FIELDS, "map_oscars_gap20.dat"), 0.5)
+    # OSCARS1D "Z Bx By": irregular z, written out of order, with a comment and a blank line
+    rng = np.random.default_rng(5)
+    z = np.sort(rng.uniform(-0.25, 0.25, 700))
+    z[0], z[-1] = -0.25, 0.25
+    perm = rng.permutation(len(z))
+    with open(os.path.join(FIELDS, "map_oscars1d.dat"), "w") as f:
+        f.write("# z [m]  Bx [T]  By [T], irregular spacing, unsorted\n")
+        for n, i in enumerate(perm):
+            if n == 100:
+                f.write("\n# a comment inside the data\n")
+            bx, by, bz = _map_value(0.0, 0.0, z[i], 0.6, 0.05)
+            f.write("%.12e %.12e %.12e\n" % (z[i], bx, by))
+    # SPECTRA: comment, "dx dy dz nx ny nz" (mm), then Bx By Bz; centred on the origin
+    with open(os.path.join(FIELDS, "map_spectra.dat"), "w") as f:
+        f.write("# SPECTRA-style map\n")
+        f.write("1 1 2 1 1 251\n")
+        for iz in range(251):
+            bx, by, bz = _map_value(0.0, 0.0, -0.25 + 0.002 * iz, 0.3, 0.05)
+            f.write("%.12e %.12e %.12e\n" % (bx, by, bz))
+    # SRW: comment, then the nine header values each after the comment character
+    with open(os.path.join(FIELDS, "map_srw.dat"), "w") as f:
+        f.write("#Bx [T], By [T], Bz [T] on 3D mesh: inmost loop vs Z, outmost loop vs X\n")
+        for v in (-0.001, 0.001, 3, 0.0, 0.0, 1, -0.2, 0.002, 201):
+            f.write("#%r\n" % v)
+        for ix in range(3):
+            for iz in range(201):
+                bx, by, bz = _map_value(-0.001 + 0.001 * ix, 0.0, -0.2 + 0.002 * iz, 0.4, 0.05)
+                f.write("%.12e %.12e %.12e\n" % (bx, by, bz))
+
+
+def _field_points(n=240, seed=9):
+    """Sample points inside, outside and on the edges of the maps (x, y, z), as raw float64."""
+    rng = np.random.default_rng(seed)
+    p = np.stack([rng.uniform(-0.0025, 0.0025, n), rng.uniform(-0.0012, 0.0012, n), rng.uniform(-0.32, 0.32, n)], 1)
+    p[:6] = [[0, 0, -0.3], [0, 0, 0.3], [-0.002, 0, 0], [0.002, 0.001, 0.1], [0, 0, 0], [1e-4, -2e-4, 0.2999]]
+    path = os.path.join(HERE, "fields", "points.bin")
+    p.astype("<f8").tofile(path)
+    return p
+
+
+MAP_CASES = {
+    "oscars": ["gridfile1=fields/map_oscars.dat", "gridfmt1=OSCARS"],
+    "oscars_rot_scaled": ["gridfile1=fields/map_oscars.dat", "gridfmt1=oscars", "gridrot1=0.01,-0.02,0.3",
+                          "gridtrans1=0.0005,-0.0002,0.01", "gridscale1=1.1,0.9,1.05,1.5,0.75,2.0"],
+    "oscars1d": ["gridfile1=fields/map_oscars1d.dat", "gridfmt1=OSCARS1D Z Bx By", "gridscale1=1,1,0.8"],
+    "spectra": ["gridfile1=fields/map_spectra.dat", "gridfmt1=SPECTRA", "gridrot1=0,0,0.5"],
+    "srw": ["gridfile1=fields/map_srw.dat", "gridfmt1=SRW", "gridtrans1=0,0,0.02"],
+    "interpolated": ["gridmap=20:fields/map_oscars_gap20.dat;10:fields/map_oscars_gap10.dat", "gridmapfmt=OSCARS",
+                     "gridparam=13.5"],
+}
+
+
+@case
+def bfield_maps():
+    """Fields of each map format at the sample points (OSCARSSR::GetB)."""
+    write_field_maps()
+    pts = _field_points()
+    out, t = {"points": pts}, 0.0
+    for name, args in MAP_CASES.items():
+        a, dt = run("bfield", args + ["pts=fields/points.bin"])
+        out[name] = a[1:].reshape(-1, 3)
+        t += dt
+    return ["maps=" + ",".join(MAP_CASES)], out, t
+
+
+FIELDS_MAPS = ["gridfile1=fields/map_oscars.dat", "gridfmt1=OSCARS", "gridfile2=fields/map_oscars1d.dat",
+               "gridfmt2=OSCARS1D Z Bx By", "gridtrans2=0,0,0.6", "gridscale2=1,1,0.8"]
+
+
+@case
+def traj_fieldmaps():
+    write_field_maps()
+    args = FIELDS_MAPS + BEAM + ["x0=0,0,-0.5", "ct=0,1.3", "npoints=6001"]
+    a, dt = run("traj", args)
+    n = int(a[0])
+    return args, {"n": np.array(n), "dt": np.array(a[1]), "traj": a[2:].reshape(9, n)}, dt
+
+
+@case
+def flux_fieldmaps():
+    write_field_maps()
+    args = FIELDS_MAPS + BEAM + ["x0=0,0,-0.5", "ct=0,1.3", "npoints=6001", "plane=XY", "width=0.006,0.006",
+                                 "nx=9,9", "trans=0,0,30", "energy_ev=400", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
 
 def main():
     ap = argparse.ArgumentParser()

@@ -153,3 +153,23 @@ def test_batch_spectrum_matches_per_particle_calls():
     err = normwise(got, ref)
     say("  normwise %.3e" % err)
     assert err <= 1e-12
+
+
+def test_batch_graph_follows_a_changed_field_list(abi, oracle):
+    """A captured graph is keyed by the field list's length and kinds: the same shapes
+    with an undulator, then a uniform box of the same count (general field evaluation),
+    then two undulators, each replay equal to the eager run of its own list."""
+    name = "multiflux_c3_small"
+    say("batched flux graph: undulator -> box -> two undulators, each vs eager")
+    a = cases.args(name)
+    rows, p = _c3_particles(oracle, name, 4)
+    parts = abi.particles(rows)
+    xyz, nrm, x12 = oracle.rect_surface("XY", 9, 9, 0.01, 0.008, (0, 0, 0), (0, 0, 30), 0)
+    lists = [[dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)],
+             [dict(kind=abi.FIELD_UNIFORM_BOX, field=(0, 0.05, 0), width=(0, 0, 2.0))],
+             [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41),
+              dict(field=(0.1, 0, 0), period=(0, 0, 0.05), nperiods=41, phase=1.0)]]
+    for und in lists:
+        eager = abi.batch_flux(und, parts, p.q, p.current, xyz, [431.2], 0.25, abi.FP64, 0)
+        graph = abi.batch_flux(und, parts, p.q, p.current, xyz, [431.2], 0.25, abi.FP64, abi.BATCH_GRAPH)
+        assert np.array_equal(graph, eager)

@@ -33,6 +33,7 @@ def _engine(name):
                                    translation=u["center"])
         else:
             osr.add_bfield_undulator(bfield=u["field"], period=u["period"], nperiods=u["nperiods"], phase=u["phase"])
+    cases.add_field_maps(osr, a)
     b = cases.beam(a)
     osr.set_particle_beam(type="electron", name="beam", energy_GeV=b["energy"], x0=b["x0"], d0=b["d0"],
                           current=b["current"])
