@@ -84,6 +84,12 @@ int srs_hip_power_density(const srs_traj* traj, const double* obs_xyz, const dou
 int srs_hip_efield_time(const srs_traj* traj, const double obs[3], double charge, double* efield_out,
                         double* time_out);
 
+/* Total radiated power [W] of one trajectory: sum_i (|A_i|^2 - |beta_i x A_i|^2) dt times
+ * |charge*current| gamma^6 / (6 pi eps0 c).  Computes OSCARSSR::CalculateTotalPower
+ * (OSCARSSR.cc:1939-1978), a sequential CPU sum in the reference; here a fixed-order
+ * device reduction (the same value to rounding, ~1e-15 relative). */
+int srs_hip_total_power(const srs_traj* traj, double charge, double current, double gamma, double* power_out);
+
 /* Device-resident variants: every array pointer is device memory, observation
  * points and normals are SoA ([3][ns]); work is queued on `stream` (a hipStream_t,
  * NULL = the library's stream) and not synchronised.  No counterpart in the
@@ -154,6 +160,30 @@ typedef struct srs_particle {
   double ct_stop;
   int64_t npoints;
 } srs_particle;
+
+/* A Gaussian beam for the device sampler (TParticleBeam, TParticleBeam.cc:137-265):
+ * energy e0 [GeV] with rms sigma_e [GeV], the particle's mass [kg] and charge [C],
+ * start x0 [m], unit direction u0, unit horizontal / vertical directions, the lattice
+ * reference point sigma_at, rms positions sigma_u [m] and angles sigma_up [rad]
+ * (horizontal, vertical), t0, and the ct range and point count every sampled particle's
+ * trajectory gets. */
+typedef struct srs_beam {
+  double e0, sigma_e;
+  double mass, charge;
+  double x0[3], u0[3], hdir[3], vdir[3], sigma_at[3];
+  double sigma_u[2], sigma_up[2];
+  double t0, ct_start, ct_stop;
+  int64_t npoints;
+} srs_beam;
+
+/* np particles of the beam drawn on the device, particle i from counter first + i
+ * under key seed: Philox4x32-10 (Salmon et al., SC'11) -> 53-bit uniforms -> Box-Muller
+ * normals n0..n4, then TParticleBeam::GetNewParticle's transform (TParticleBeam.cc:
+ * 228-265) with n0..n4 in the reference's draw order (energy, x, y, x', y').  Draws are
+ * reproducible and independent of np and of the order of calls (any particle can be
+ * redrawn from its counter); the reference's host stream (std::mt19937) is the other
+ * sampler, kept bit for bit on the host.  No counterpart on the reference's GPU. */
+int srs_hip_sample_beam(const srs_beam* beam, int64_t np, uint64_t seed, uint64_t first, srs_particle* out);
 
 /* Points the trajectories of np particles will have (total over particles). */
 int srs_hip_trajectory_points(const srs_particle* p, int64_t np, int64_t* total);

@@ -560,6 +560,42 @@ void oracle_rng_seed(int seed, int reset_normal) {
 double oracle_rng_normal(void) { return gNormal(*gMT); }
 double oracle_rng_uniform(void) { return gUniform(*gMT); }
 
+// TParticleBeam::GetNewParticle (TParticleBeam.cc:228-265) for given draws n[0..4] in
+// the reference's order: energy, horizontal, vertical position, horizontal, vertical
+// angle.  next() supplies them one at a time (the reference draws as it goes).
+}  // extern "C"
+template <class Next>
+static void beam_particle(double energy_gev, double sigma_e_gev, V3 const& x0, V3 const& u0, V3 const& h, V3 const& v,
+                          double su0, double su1, double sp0, double sp1, V3 const& lattice_ref, Next next,
+                          oracle_particle* out) {
+  double const enew = energy_gev + sigma_e_gev * next();
+  double const gamma = enew / kg_to_gev(out->m);
+  double const b = std::sqrt(1.0 - 1.0 / (gamma * gamma));
+  double const dmid = dot(sub(lattice_ref, x0), u0);
+  V3 X = x0;
+  double const n1 = next();
+  V3 const dx = muls(muls(muls(h, su0), 1 + dmid), n1);
+  X.x += dx.x;
+  X.y += dx.y;
+  X.z += dx.z;
+  double const n2 = next();
+  V3 const dy = muls(muls(muls(v, su1), 1 + dmid), n2);
+  X.x += dy.x;
+  X.y += dy.y;
+  X.z += dy.z;
+  V3 B = muls(u0, b);
+  rot_y(B, sp0 * next());
+  rot_x(B, -sp1 * next());
+  out->x0[0] = X.x;
+  out->x0[1] = X.y;
+  out->x0[2] = X.z;
+  out->beta0[0] = B.x;
+  out->beta0[1] = B.y;
+  out->beta0[2] = B.z;
+  finish_particle(out);
+}
+extern "C" {
+
 void oracle_random_electron(double energy_gev, double sigma_e_gev, const double x0[3], const double d0[3],
                             const double hdir[3], const double beta[2], const double emit[2],
                             const double lattice_ref[3], double current, oracle_particle* out) {
@@ -575,31 +611,60 @@ void oracle_random_electron(double energy_gev, double sigma_e_gev, const double 
   V3 const v = unit(cross(u0, h));
   double const su0 = std::sqrt(emit[0] * beta[0]), su1 = std::sqrt(emit[1] * beta[1]);
   double const sp0 = std::sqrt(emit[0] / beta[0]), sp1 = std::sqrt(emit[1] / beta[1]);
-  double const enew = energy_gev + sigma_e_gev * gNormal(*gMT);
-  double const gamma = enew / kg_to_gev(out->m);
-  double const b = std::sqrt(1.0 - 1.0 / (gamma * gamma));
-  double const dmid = dot(sub(v3(lattice_ref), v3(x0)), u0);
-  V3 X = v3(x0);
-  double const n1 = gNormal(*gMT);
-  V3 const dx = muls(muls(muls(h, su0), 1 + dmid), n1);
-  X.x += dx.x;
-  X.y += dx.y;
-  X.z += dx.z;
-  double const n2 = gNormal(*gMT);
-  V3 const dy = muls(muls(muls(v, su1), 1 + dmid), n2);
-  X.x += dy.x;
-  X.y += dy.y;
-  X.z += dy.z;
-  V3 B = muls(u0, b);
-  rot_y(B, sp0 * gNormal(*gMT));
-  rot_x(B, -sp1 * gNormal(*gMT));
-  out->x0[0] = X.x;
-  out->x0[1] = X.y;
-  out->x0[2] = X.z;
-  out->beta0[0] = B.x;
-  out->beta0[1] = B.y;
-  out->beta0[2] = B.z;
-  finish_particle(out);
+  beam_particle(energy_gev, sigma_e_gev, v3(x0), u0, h, v, su0, su1, sp0, sp1, v3(lattice_ref),
+                [] { return gNormal(*gMT); }, out);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2,
+// 3", SC'11): 10 rounds of two 32x32->64 multiplications with the published multipliers
+// and Weyl key increments.
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]}, k[2] = {key[0], key[1]};
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k[0] += 0x9E3779B9u;
+      k[1] += 0xBB67AE85u;
+    }
+    uint64_t const p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t const n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k[0], n1 = (uint32_t)p1;
+    uint32_t const n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k[1], n3 = (uint32_t)p0;
+    c[0] = n0;
+    c[1] = n1;
+    c[2] = n2;
+    c[3] = n3;
+  }
+  for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+// Six standard normals of counter idx under key seed: three Philox blocks (counter
+// (lo idx, hi idx, j, 0), j = 0, 1, 2), each two 53-bit uniforms u1 in (0, 1], u2 in
+// [0, 1) and the Box-Muller pair sqrt(-2 ln u1) (cos, sin)(2 pi u2).
+void oracle_counter_normals(uint64_t seed, uint64_t idx, double out[6]) {
+  uint32_t const key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int j = 0; j < 3; ++j) {
+    uint32_t const ctr[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)j, 0u};
+    uint32_t w[4];
+    oracle_philox4x32_10(ctr, key, w);
+    uint64_t const a = (((uint64_t)w[0] << 32) | w[1]) >> 11, b = (((uint64_t)w[2] << 32) | w[3]) >> 11;
+    double const u1 = std::ldexp((double)(a + 1), -53), u2 = std::ldexp((double)b, -53);
+    double const r = std::sqrt(-2.0 * std::log(u1));
+    out[2 * j] = r * std::cos(2.0 * kPi * u2);
+    out[2 * j + 1] = r * std::sin(2.0 * kPi * u2);
+  }
+}
+
+void oracle_sample_beam(const oracle_beam* bm, uint64_t seed, uint64_t idx, oracle_particle* out) {
+  // the reference's particle transform on the counter-based draws n[0..4]
+  std::memset(out, 0, sizeof(*out));
+  out->q = bm->charge;
+  out->m = bm->mass;
+  out->current = bm->current;
+  double n[6];
+  oracle_counter_normals(seed, idx, n);
+  int k = 0;
+  beam_particle(bm->e0, bm->sigma_e, v3(bm->x0), v3(bm->u0), v3(bm->hdir), v3(bm->vdir), bm->sigma_u[0],
+                bm->sigma_u[1], bm->sigma_up[0], bm->sigma_up[1], v3(bm->sigma_at), [&] { return n[k++]; }, out);
+  out->t0 = bm->t0;
 }
 
 int64_t oracle_trajectory_npoints(double ct_start, double ct_stop, int64_t npoints, double t0) {

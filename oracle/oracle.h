@@ -65,6 +65,18 @@ void oracle_ideal_electron(double energy_gev, const double x0[3], const double d
 void oracle_rng_seed(int seed, int reset_normal);
 double oracle_rng_normal(void);
 double oracle_rng_uniform(void);
+/* A Gaussian beam (the device sampler's srs_beam): e0 [GeV], sigma_e, particle mass
+ * [kg] and charge [C], current, start, unit direction, horizontal / vertical unit
+ * directions, lattice reference point, rms positions and angles, t0. */
+typedef struct oracle_beam {
+  double e0, sigma_e, mass, charge, current;
+  double x0[3], u0[3], hdir[3], vdir[3], sigma_at[3];
+  double sigma_u[2], sigma_up[2];
+  double t0;
+} oracle_beam;
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void oracle_counter_normals(uint64_t seed, uint64_t idx, double out[6]);
+void oracle_sample_beam(const oracle_beam* bm, uint64_t seed, uint64_t idx, oracle_particle* out);
 void oracle_random_electron(double energy_gev, double sigma_e_gev, const double x0[3], const double d0[3],
                             const double hdir[3], const double beta[2], const double emit[2],
                             const double lattice_ref[3], double current, oracle_particle* out);

@@ -102,6 +102,34 @@ def random_electron(energy_gev, sigma_e_gev, x0, d0, hdir, beta, emit, lattice_r
     return p
 
 
+class Beam(C.Structure):
+    _fields_ = [("e0", D), ("sigma_e", D), ("mass", D), ("charge", D), ("current", D), ("x0", D * 3), ("u0", D * 3),
+                ("hdir", D * 3), ("vdir", D * 3), ("sigma_at", D * 3), ("sigma_u", D * 2), ("sigma_up", D * 2),
+                ("t0", D)]
+
+
+def philox4x32_10(ctr, key):
+    out = (C.c_uint32 * 4)()
+    lib().oracle_philox4x32_10((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out)
+    return list(out)
+
+
+def counter_normals(seed, idx):
+    out = (D * 6)()
+    lib().oracle_counter_normals(C.c_uint64(seed), C.c_uint64(idx), out)
+    return list(out)
+
+
+def sample_beam(b, seed, idx):
+    """The device sampler's particle idx (oracle_sample_beam); b: dict as beam_params."""
+    bm = Beam(b["e0"], b["sigma_e"], b["mass"], b["charge"], b.get("current", 0.0), (D * 3)(*b["x0"]),
+              (D * 3)(*b["u0"]), (D * 3)(*b["hdir"]), (D * 3)(*b["vdir"]), (D * 3)(*b["sigma_at"]),
+              (D * 2)(*b["sigma_u"]), (D * 2)(*b["sigma_up"]), b.get("t0", 0.0))
+    p = Particle()
+    lib().oracle_sample_beam(C.byref(bm), C.c_uint64(seed), C.c_uint64(idx), C.byref(p))
+    return p
+
+
 def trajectory(fields, particle, ct_start, ct_stop, npoints):
     n = lib().oracle_trajectory_npoints(ct_start, ct_stop, npoints, particle.t0)
     t = np.zeros((9, n))

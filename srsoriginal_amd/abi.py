@@ -231,6 +231,29 @@ def undulators(specs, keep):
     return arr
 
 
+class SrsBeam(C.Structure):
+    _fields_ = [("e0", _D), ("sigma_e", _D), ("mass", _D), ("charge", _D), ("x0", _D * 3), ("u0", _D * 3),
+                ("hdir", _D * 3), ("vdir", _D * 3), ("sigma_at", _D * 3), ("sigma_u", _D * 2), ("sigma_up", _D * 2),
+                ("t0", _D), ("ct_start", _D), ("ct_stop", _D), ("npoints", _I64)]
+
+
+def sample_beam(beam, np_, seed, first=0, ct_start=0.0, ct_stop=1.0, npoints=2):
+    """srs_hip_sample_beam: np_ particles of beam (a dict as sr._e.beam_params returns)
+    drawn on the device from counters first .. first + np_ - 1 under key seed; returns
+    particle rows (dicts of x0, beta0, qmg, t0, ct_start, ct_stop, npoints)."""
+    L = lib()
+    L.srs_hip_sample_beam.argtypes = [C.POINTER(SrsBeam), _I64, C.c_uint64, C.c_uint64, C.POINTER(SrsParticle)]
+    L.srs_hip_sample_beam.restype = C.c_int
+    b = SrsBeam(float(beam["e0"]), float(beam["sigma_e"]), float(beam["mass"]), float(beam["charge"]),
+                (_D * 3)(*beam["x0"]), (_D * 3)(*beam["u0"]), (_D * 3)(*beam["hdir"]), (_D * 3)(*beam["vdir"]),
+                (_D * 3)(*beam["sigma_at"]), (_D * 2)(*beam["sigma_u"]), (_D * 2)(*beam["sigma_up"]),
+                float(beam.get("t0", 0.0)), float(ct_start), float(ct_stop), int(npoints))
+    out = (SrsParticle * int(np_))()
+    _check(L.srs_hip_sample_beam(C.byref(b), int(np_), C.c_uint64(int(seed)), C.c_uint64(int(first)), out))
+    return [dict(x0=list(r.x0), beta0=list(r.beta0), qmg=r.qmg, t0=r.t0, ct_start=r.ct_start, ct_stop=r.ct_stop,
+                 npoints=r.npoints) for r in out]
+
+
 def particles(rows):
     """srs_particle array from dicts (x0, beta0, qmg, t0, ct_start, ct_stop, npoints)."""
     arr = (SrsParticle * len(rows))()

@@ -15,6 +15,19 @@ import numpy as np
 from . import abi
 
 
+def sample_particles_device(osr, n, ct_start, ct_stop, npoints, seed, first=0, beam=None):
+    """Draw n random particles of one beam on the device (srs_hip_sample_beam): counter-
+    based (Philox4x32-10), so particle i is the same whatever n and call order; the
+    transform is the reference's (TParticleBeam.cc:228-265).  beam = the beam's name
+    (default: the first beam added).  Returns (rows, charge, current)."""
+    names = osr._e.beam_names()
+    if not names:
+        raise ValueError("No particle beam defined")
+    bp = osr._e.beam_params(beam if beam is not None else names[0])
+    rows = abi.sample_beam(bp, n, seed, first, ct_start, ct_stop, npoints)
+    return rows, bp["charge"], bp["current"]
+
+
 def sample_particles(osr, n, ct_start, ct_stop, npoints, ct_jitter=0.0, points_per_meter=None, rng_seed=0):
     """Draw n random particles from the engine's beam (reference order).
 

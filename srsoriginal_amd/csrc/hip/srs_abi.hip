@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -822,6 +823,72 @@ int srs_hip_efield_time(const srs_traj* traj, const double obs[3], double charge
   HIPCHK(srs::launch_efield_time(t, obs, charge / (kFourPi * kEps0), d, d + 3 * n, c.stream));  // OSCARSSR.cc:2661
   HIPCHK(hipMemcpyAsync(efield_out, d, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipMemcpyAsync(time_out, d + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return SRS_OK;
+}
+
+static_assert(sizeof(srs::SampledParticle) == sizeof(srs_particle) &&
+                  offsetof(srs::SampledParticle, qmg) == offsetof(srs_particle, qmg) &&
+                  offsetof(srs::SampledParticle, npoints) == offsetof(srs_particle, npoints),
+              "sampled rows are srs_particle rows");
+
+int srs_hip_sample_beam(const srs_beam* beam, int64_t np, uint64_t seed, uint64_t first, srs_particle* out) {
+  g_err.clear();
+  if (!beam || !out) return fail(SRS_ERR_INVALID, "NULL argument");
+  if (np < 1) return fail(SRS_ERR_INVALID, "no particles requested");
+  if (!(beam->mass > 0)) return fail(SRS_ERR_INVALID, "beam particle mass must be > 0");
+  double const mgev = 1e-9 * beam->mass * kC * kC / kQe;  // TSRS::kgToGeV, TOSCARSSR.h:68
+  if (!(beam->e0 > mgev)) return fail(SRS_ERR_INVALID, "beam energy must exceed the particle's rest energy");
+  if (!(beam->sigma_e >= 0)) return fail(SRS_ERR_INVALID, "'sigma_energy_GeV' cannot be less than zero");
+  if (beam->npoints < 2) return fail(SRS_ERR_INVALID, "npoints must be >= 2");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  srs::DevBeam b{};
+  b.e0 = beam->e0;
+  b.sigma_e = beam->sigma_e;
+  b.mgev = mgev;
+  b.q = beam->charge;
+  b.m = beam->mass;
+  for (int k = 0; k < 3; ++k) {
+    b.x0[k] = beam->x0[k];
+    b.u0[k] = beam->u0[k];
+    b.h[k] = beam->hdir[k];
+    b.v[k] = beam->vdir[k];
+    b.at[k] = beam->sigma_at[k];
+  }
+  for (int k = 0; k < 2; ++k) {
+    b.su[k] = beam->sigma_u[k];
+    b.sp[k] = beam->sigma_up[k];
+  }
+  b.t0 = beam->t0;
+  b.ct_start = beam->ct_start;
+  b.ct_stop = beam->ct_stop;
+  b.npoints = beam->npoints;
+  HIPCHK(c.out.ensure((size_t)np * sizeof(srs::SampledParticle)));
+  HIPCHK(srs::launch_sample_beam(b, seed, first, np, c.out.as<srs::SampledParticle>(), c.stream));
+  HIPCHK(hipMemcpyAsync(out, c.out.p, (size_t)np * sizeof(srs_particle), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return SRS_OK;
+}
+
+int srs_hip_total_power(const srs_traj* traj, double charge, double current, double gamma, double* power_out) {
+  g_err.clear();
+  TRY(check_traj(traj, true));
+  if (!power_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  TrajView t;
+  TRY(stage_traj(c, traj, true, &t));
+  HIPCHK(c.out.ensure(sizeof(double)));
+  // OSCARSSR.cc:1975: |q I| gamma^6 / (6 pi eps0 c)
+  double const scale = std::fabs(charge * current) * std::pow(gamma, 6) / (6 * kPi * kEps0 * kC);
+  {
+    KernelTimer timer(c, c.stream);
+    HIPCHK(srs::launch_total_power(t, scale, c.out.as<double>(), c.stream));
+  }
+  HIPCHK(hipMemcpyAsync(power_out, c.out.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   return SRS_OK;
 }

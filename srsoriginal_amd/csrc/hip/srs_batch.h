@@ -88,6 +88,22 @@ struct RK4Seg {
   int64_t lo, hi;
   double* state;
 };
+// Beam of the device sampler (mgev = particle mass in GeV) and its output rows, laid
+// out as srs_particle.
+struct DevBeam {
+  double e0, sigma_e, mgev, q, m;
+  double x0[3], u0[3], h[3], v[3], at[3];
+  double su[2], sp[2];
+  double t0, ct_start, ct_stop;
+  int64_t npoints;
+};
+struct SampledParticle {
+  double x0[3], beta0[3], qmg, t0, ct_start, ct_stop;
+  int64_t npoints;
+};
+hipError_t launch_sample_beam(DevBeam const& b, uint64_t seed, uint64_t first, int64_t np, SampledParticle* out,
+                              hipStream_t s);
+
 // local: the list holds elements other than undulators (Gaussian, box, field map)
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, const double* poles, const BatchParticle* P,
                             int np, BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s);

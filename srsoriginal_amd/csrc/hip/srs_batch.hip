@@ -508,6 +508,92 @@ int batch_max_blocks(BatchConfig const& c, bool mixed) {
   return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of two 32x32 -> 64-bit
+// products with the published multipliers and Weyl key increments.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t const h0 = __umulhi(0xD2511F53u, c[0]), l0 = 0xD2511F53u * c[0];
+    uint32_t const h1 = __umulhi(0xCD9E8D57u, c[2]), l1 = 0xCD9E8D57u * c[2];
+    uint32_t const n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = l1;
+    c[2] = n2;
+    c[3] = l0;
+  }
+}
+
+// One lane per particle: counter first + p, three Philox blocks (counter (lo, hi, j, 0))
+// -> 53-bit uniforms u1 in (0, 1], u2 in [0, 1) -> Box-Muller normals; then
+// TParticleBeam::GetNewParticle (TParticleBeam.cc:228-265) in the reference's operation
+// order with the draws n0..n4 (energy, x, y, x', y'), and TParticleA's gamma and
+// q/(m gamma) from the drawn beta (TParticleA.cc:353-382).
+__global__ __launch_bounds__(256) void sample_beam_kernel(DevBeam b, uint64_t seed, uint64_t first, int64_t np,
+                                                          SampledParticle* __restrict__ out) {
+  int64_t const p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  uint64_t const idx = first + (uint64_t)p;
+  double n[6];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)j, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t const a = (((uint64_t)c[0] << 32) | c[1]) >> 11, bb = (((uint64_t)c[2] << 32) | c[3]) >> 11;
+    double const u1 = (double)(a + 1) * 1.1102230246251565404e-16, u2 = (double)bb * 1.1102230246251565404e-16;
+    double const r = sqrt(-2.0 * log(u1));
+    n[2 * j] = r * cos(2.0 * kPi * u2);
+    n[2 * j + 1] = r * sin(2.0 * kPi * u2);
+  }
+  double const enew = b.e0 + b.sigma_e * n[0];
+  double const gamma = enew / b.mgev;
+  double const beta = sqrt(1.0 - 1.0 / (gamma * gamma));
+  double const dmid = (b.at[0] - b.x0[0]) * b.u0[0] + (b.at[1] - b.x0[1]) * b.u0[1] + (b.at[2] - b.x0[2]) * b.u0[2];
+  double X[3], B[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) X[k] = b.x0[k] + ((b.h[k] * b.su[0]) * (1 + dmid)) * n[1];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) X[k] = X[k] + ((b.v[k] * b.su[1]) * (1 + dmid)) * n[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) B[k] = b.u0[k] * beta;
+  {  // TVector3D::RotateSelfY
+    double const a = b.sp[0] * n[3], sa = sin(a), ca = cos(a), zz = B[2];
+    B[2] = ca * zz - sa * B[0];
+    B[0] = sa * zz + ca * B[0];
+  }
+  {  // TVector3D::RotateSelfX
+    double const a = -b.sp[1] * n[4], sa = sin(a), ca = cos(a), yy = B[1];
+    B[1] = ca * yy - sa * B[2];
+    B[2] = sa * yy + ca * B[2];
+  }
+  double const b2 = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+  double qmg = 0.0;
+  if (b2 != 1) {
+    double const g = 1.0 / sqrt(1.0 - b2);
+    if (b.m != 0 && g != 0) qmg = b.q / b.m / g;
+  }
+  SampledParticle& o = out[p];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o.x0[k] = X[k];
+    o.beta0[k] = B[k];
+  }
+  o.qmg = qmg;
+  o.t0 = b.t0;
+  o.ct_start = b.ct_start;
+  o.ct_stop = b.ct_stop;
+  o.npoints = b.npoints;
+}
+
+hipError_t launch_sample_beam(DevBeam const& b, uint64_t seed, uint64_t first, int64_t np, SampledParticle* out,
+                              hipStream_t s) {
+  sample_beam_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(b, seed, first, np, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, const double* poles, const BatchParticle* P,
                             int np,
                             BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s) {

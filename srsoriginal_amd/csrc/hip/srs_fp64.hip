@@ -282,6 +282,30 @@ inline int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk)
 
 }  // namespace
 
+// Total power sum over the trajectory (OSCARSSR.cc:1966-1971): one block of 1024 lanes,
+// each lane a strided share in order, then the lanes' sums in a fixed tree order.
+__global__ __launch_bounds__(1024) void total_power_kernel(TrajView t, double scale, double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0, comp = 0.0;  // compensated lane sum
+  for (int64_t i = threadIdx.x; i < t.n; i += 1024) {
+    double const ax = t.ax[i], ay = t.ay[i], az = t.az[i];
+    double const bx = t.bx[i], by = t.by[i], bz = t.bz[i];
+    double const cx = by * az - bz * ay, cy = bz * ax - bx * az, cz = bx * ay - by * ax;
+    double const v = ((ax * ax + ay * ay + az * az) - (cx * cx + cy * cy + cz * cz)) * t.dt;
+    double const y = v - comp;
+    double const s = acc + y;
+    comp = (s - acc) - y;
+    acc = s;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
 template <class K>
 static int blocks_per_cu(K kernel, int threads, int& cache) {
   if (cache == 0) {
@@ -344,6 +368,11 @@ hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double
 
 hipError_t launch_efield_time(const TrajView& t, const double obs[3], double c0, double* e, double* tm, hipStream_t s) {
   efield_time_kernel<<<(unsigned)((t.n + kBlock - 1) / kBlock), kBlock, 0, s>>>(t, obs[0], obs[1], obs[2], c0, e, tm);
+  return hipGetLastError();
+}
+
+hipError_t launch_total_power(const TrajView& t, double scale, double* out, hipStream_t s) {
+  total_power_kernel<<<1, 1024, 0, s>>>(t, scale, out);
   return hipGetLastError();
 }
 

@@ -22,6 +22,9 @@ struct EnergySet {
   double fw[kMaxFluxEnergies];
 };
 
+// Total power of one trajectory (with accelerations): out[0] = scale * sum of the terms.
+hipError_t launch_total_power(const TrajView& t, double scale, double* out, hipStream_t s);
+
 // ---- resident blocks per CU of the integral kernels (for the chunk choice) ----
 int flux_fp64_blocks_per_cu(int ne);
 int spectrum_fp64_blocks_per_cu();

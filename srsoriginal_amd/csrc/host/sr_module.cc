@@ -171,6 +171,28 @@ PYBIND11_MODULE(_sr, m) {
              double a[2] = {su.at(0), su.at(1)}, b[2] = {sup.at(0), sup.at(1)};
              e.GetParticleBeam(name).set_sigma(v3(h), a, b, v3(at), se);
            })
+      .def("beam_params",
+           [](srs::OSCARSSR& e, std::string const& name) {
+             // the beam's definition, for the device sampler (srs_beam)
+             auto const& b = e.GetParticleBeam(name);
+             py::dict d;
+             d["e0"] = b.e0;
+             d["sigma_e"] = b.sigma_e;
+             d["mass"] = b.proto.m;
+             d["charge"] = b.proto.q;
+             d["current"] = b.proto.current;
+             d["x0"] = std::vector<double>{b.x0.x, b.x0.y, b.x0.z};
+             d["u0"] = std::vector<double>{b.u0.x, b.u0.y, b.u0.z};
+             d["hdir"] = std::vector<double>{b.hdir.x, b.hdir.y, b.hdir.z};
+             d["vdir"] = std::vector<double>{b.vdir.x, b.vdir.y, b.vdir.z};
+             d["sigma_at"] = std::vector<double>{b.sigma_at.x, b.sigma_at.y, b.sigma_at.z};
+             d["sigma_u"] = std::vector<double>{b.sigma_u[0], b.sigma_u[1]};
+             d["sigma_up"] = std::vector<double>{b.sigma_up[0], b.sigma_up[1]};
+             d["t0"] = b.t0;
+             return d;
+           })
+      .def("beam_names",
+           [](srs::OSCARSSR const& e) { return e.BeamNames(); })
       .def("clear_particle_beams", &srs::OSCARSSR::ClearParticleBeams)
       .def("n_beams", &srs::OSCARSSR::GetNParticleBeams)
       .def("set_new_particle", [](srs::OSCARSSR& e) { e.SetNewParticle(); })
@@ -272,6 +294,7 @@ PYBIND11_MODULE(_sr, m) {
            py::arg("surface"), py::arg("dim"), py::arg("directional"), py::arg("nparticles"), py::arg("nthreads"),
            py::arg("gpu"), py::arg("precision"), py::arg("batched") = false)
       .def("calculate_total_power", &srs::OSCARSSR::CalculateTotalPower)
+      .def("calculate_total_power_device", &srs::OSCARSSR::CalculateTotalPowerDevice)
       .def("calculate_efield_vs_time", [](srs::OSCARSSR& e, std::vector<double> obs) {
         std::vector<double> ef, t;
         e.CalculateElectricFieldTimeDomain(v3(obs), ef, t);

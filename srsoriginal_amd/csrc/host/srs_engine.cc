@@ -1200,4 +1200,14 @@ double OSCARSSR::CalculateTotalPower() {
   return tp;
 }
 
+double OSCARSSR::CalculateTotalPowerDevice() {
+  // the same quantity, the trajectory from the host RK4 and the sum on the device
+  ensure_particle();
+  CalculateTrajectory(particle_);
+  srs_traj const v = view(particle_.traj);
+  double p = 0;
+  check_status(srs_hip_total_power(&v, particle_.q, particle_.current, particle_.gamma, &p));
+  return p;
+}
+
 }  // namespace srs

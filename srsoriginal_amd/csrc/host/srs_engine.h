@@ -171,6 +171,11 @@ class OSCARSSR {
                        double t0, double current, double weight);
   ParticleBeam& GetParticleBeam(std::string const& name);
   size_t GetNParticleBeams() const { return beams_.size(); }
+  std::vector<std::string> BeamNames() const {  // in the order the beams were added
+    std::vector<std::string> out(beams_.size());
+    for (auto const& kv : beam_index_) out[kv.second] = kv.first;
+    return out;
+  }
   void ClearParticleBeams();
   void SetNewParticle();                                               // random, by weight
   void SetNewParticle(std::string const& beam, std::string const& ideal_or_random);
@@ -213,6 +218,7 @@ class OSCARSSR {
   void CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer& C, int dim, bool directional, int nparticles,
                              int nthreads, int gpu, Precision prec = Precision::FP64, bool batched = false);
   double CalculateTotalPower();
+  double CalculateTotalPowerDevice();  // the sum on the device (srs_hip_total_power)
   // OSCARSSR::CalculateElectricFieldTimeDomain (OSCARSSR.cc:2627-2690): host RK4, the
   // field on the device; e [3 n], t [n].
   void CalculateElectricFieldTimeDomain(Vec3 const& obs, std::vector<double>& e, std::vector<double>& t);

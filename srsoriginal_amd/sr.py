@@ -397,11 +397,14 @@ class sr:
             containers.write_scalar_text(ofile, E, t, 3)
         return [[float(a), [float(x), float(y), float(z)]] for a, (x, y, z) in zip(t.tolist(), E.tolist())]
 
-    def calculate_total_power(self):
-        """OSCARSSR_Python.cc:2369-2398 (host FP64, O(NT))."""
+    def calculate_total_power(self, gpu=0):
+        """OSCARSSR_Python.cc:2369-2398: the host sum in the reference's order, or with
+        gpu=1 (an extension) the device sum (srs_hip_total_power)."""
         if self._e.n_beams() < 1:
             raise ValueError("No particle beam defined")
-        return self._e.calculate_total_power()
+        if gpu not in (0, 1):
+            raise ValueError("'gpu' must be 0 or 1")
+        return self._e.calculate_total_power_device() if gpu else self._e.calculate_total_power()
 
     # ---- container files (OSCARSSR_Python.cc:3252-3318, :3475-3563) -------------
     @staticmethod

@@ -258,3 +258,39 @@ def test_efield_vs_time_matches_reference(tmp_path):
     assert et <= 1e-15 and ee <= 1e-12
     rows = np.loadtxt(p)
     assert rows.shape == (len(ref), 4) and np.allclose(rows, ref, rtol=1e-6, atol=0)
+
+
+def test_total_power_on_the_device_matches_reference():
+    """srs_hip_total_power (calculate_total_power(gpu=1)) against the correctly rounded
+    sum of the same per-point terms (math.fsum over the host trajectory): within 1e-14.
+    The reference's sequential sum carries up to n*eps of rounding (2.4e-13 measured on
+    the 40001-point C2 EPU, within the 4.4e-12 bound); the C0 reference value includes
+    the unset A/c of the first trajectory point and is compared through the host sum
+    (the reference's order) instead.  The ABI rejects a trajectory without accelerations."""
+    import math
+    from test_host_engine import _engine
+    from srsoriginal_amd import abi
+    for name in ("totalpower_c0", "totalpower_c2"):
+        osr = _engine(name)
+        host = osr.calculate_total_power()
+        got = osr.calculate_total_power(gpu=1)
+        t, dt = osr.trajectory_array()
+        b, a = t[3:6], t[6:9]
+        c = np.cross(b.T, a.T).T
+        terms = ((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2] - ((c[0] * c[0] + c[1] * c[1]) + c[2] * c[2])) * dt
+        q, cur = osr._e.particle_charge_current()
+        gamma = osr._e.particle_gamma()
+        exact = math.fsum(terms.tolist()) * abs(q * cur) * gamma ** 6 / (6 * math.pi * 8.854187817E-12 * 2.99792458e8)
+        n = t.shape[1]
+        say("  %s: device %.15g W, fsum %.15g W (rel %.2e), host sequential %.15g W (rel %.2e)" % (
+            name, got, exact, abs(got - exact) / exact, host, abs(host - exact) / exact))
+        assert abs(got - exact) <= 1e-14 * exact
+        assert abs(host - exact) <= n * 1.1e-16 * exact
+        if name == "totalpower_c2":
+            assert host == float(golden(name)["total_power"])
+    import ctypes as C
+    t = abi.SrsTraj(10, 1e-12, *([None] * 9))
+    out = C.c_double()
+    rc = abi.lib().srs_hip_total_power(C.byref(t), C.c_double(-1.6e-19), C.c_double(0.5), C.c_double(5871.0),
+                                       C.byref(out))
+    assert rc != 0
