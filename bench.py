@@ -198,8 +198,8 @@ def spectrum_and_power_density(reps=5):
 
 def batched_power_density(nparticles=256, reps=2):
     """Multi-particle power density (the C3 Gaussian beam on the C0 undulator, ragged
-    18k-22k point trajectories) on 101x101 points over 40 mm at 30 m, FP64, one device
-    pipeline for all particles; timed with HIP events."""
+    18k-22k point trajectories) on 101x101 points over 40 mm at 30 m, FP64 and mixed, one
+    device pipeline for all particles; timed with HIP events."""
     from srsoriginal_amd import abi, batch
     from srsoriginal_amd.lib import _sr
     from srsoriginal_amd.sr import sr
@@ -214,18 +214,24 @@ def batched_power_density(nparticles=256, reps=2):
     surf = _sr.Surface.rectangle("XY", 101, 101, 0.04, 0.04, [0, 0, 0], [0, 0, 30], 0)
     npts = sum(abi.trajectory_points(abi.particles([r])) for r in rows)
     terms = float(npts) * len(surf.xyz)
-    out = batch.power_density(osr, rows, q, cur, surf.xyz, surf.normals)  # warm-up
-    ms = []
-    for _ in range(reps):
-        abi.timing_enable(True)
-        out = batch.power_density(osr, rows, q, cur, surf.xyz, surf.normals)
-        kms, _ = abi.timing_read()
-        abi.timing_enable(False)
-        ms.append(kms)
-    t = min(ms)
-    log("batched power density: %.1f ms (%d particles)" % (t, nparticles))
+    res = {}
+    for prec in ("double", "mixed"):
+        out = batch.power_density(osr, rows, q, cur, surf.xyz, surf.normals, precision=prec)  # warm-up
+        ms = []
+        for _ in range(reps):
+            abi.timing_enable(True)
+            out = batch.power_density(osr, rows, q, cur, surf.xyz, surf.normals, precision=prec)
+            kms, _ = abi.timing_read()
+            abi.timing_enable(False)
+            ms.append(kms)
+        res[prec] = (min(ms), out)
+    t, out = res["double"]
+    tm, om = res["mixed"]
+    err = float(np.max(np.abs(om - out)) / np.max(np.abs(out)))
+    log("batched power density: fp64 %.1f ms, mixed %.1f ms (%d particles)" % (t, tm, nparticles))
     return {"workload": "%d particles x %d traj points (ragged 18k-22k) x 101x101 over 40 mm at 30 m" % (nparticles, npts),
             "terms": terms, "fp64_ms": t, "fp64_terms_per_s": terms / (t * 1e-3),
+            "mixed_ms": tm, "mixed_terms_per_s": terms / (tm * 1e-3), "mixed_vs_fp64_max_rel_err": err,
             "peak_w_per_mm2": float(np.max(out)),
             "timed": "device RK4 in time segments overlapped with pd_batch_kernel, reduction (HIP events)"}
 

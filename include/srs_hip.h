@@ -207,7 +207,9 @@ int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, in
  * pd_out[j] = sum over particles (in order, Kahan) of weight * P_p(j), each P_p as
  * srs_hip_power_density with `directional`.  Replaces the nparticles loop of
  * OSCARSSR::CalculatePowerDensity with the GPU switch (OSCARSSR.cc:1655-1669: new
- * particle, host RK4, one GPU call per particle with Weight = 1/NP).  FP64. */
+ * particle, host RK4, one GPU call per particle with Weight = 1/NP).  precision
+ * SRS_PRECISION_FP64 (the reference's terms) or SRS_PRECISION_MIXED (the FP32 terms of
+ * srs_hip_power_density's mixed form). */
 int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                                 double current, const double* obs_xyz, const double* normals_xyz, int64_t ns,
                                 int directional, double weight, int precision, double* pd_out);

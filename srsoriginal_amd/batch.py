@@ -64,7 +64,8 @@ def spectrum(osr, rows, charge, current, obs, energies_ev):
                               1.0 / len(rows))
 
 
-def power_density(osr, rows, charge, current, obs_xyz, normals_xyz, directional=False):
+def power_density(osr, rows, charge, current, obs_xyz, normals_xyz, directional=False, precision="double"):
     """Sum over particles of power density/NP [W/mm^2] on obs_xyz (N x 3) -> array (N,)."""
+    prec = {"double": abi.FP64, "fp64": abi.FP64, "mixed": abi.MIXED}[precision]
     return abi.batch_power_density(osr._e.undulators(), abi.particles(rows), charge, current, obs_xyz, normals_xyz,
-                                   directional, 1.0 / len(rows))
+                                   directional, 1.0 / len(rows), prec)

@@ -590,7 +590,8 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
 // with pd_batch_kernel per segment on s (adding in segment order), then the ordered reduction into
 // raw (np * ns) and b_flux_pp (np * ns, the per-particle power density).
 int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int64_t max_leg, int64_t max_pts,
-                     srs::Checkpoints const& ck, int ns, int dir, double qfac, double* raw, hipStream_t s) {
+                     srs::Checkpoints const& ck, int ns, int dir, double qfac, double* raw, hipStream_t s,
+                     bool mixed = false) {
   int nseg = 1;
   int64_t const L = segment_steps(max_leg, &nseg);
   int const nch = srs::pd_batch_chunks(nseg > 1 ? 2 * L : max_pts, ns, np);
@@ -602,7 +603,7 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
   if (nseg == 1) {
     HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, poles, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
     HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, 0, INT64_MAX, nch, 0,
-                                c.g_part.as<double>(), s));
+                                c.g_part.as<double>(), mixed, s));
   } else {
     HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
     HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
@@ -614,7 +615,7 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
     for (int k = 0; k < nseg; ++k) {
       HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
       HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, k * L, (k + 1) * L,
-                                  nch, k > 0, c.g_part.as<double>(), s));
+                                  nch, k > 0, c.g_part.as<double>(), mixed, s));
     }
   }
   HIPCHK(srs::launch_pd_batch_reduce(c.g_part.as<double>(), nch, ns, np, Pd, qfac, dir, raw,
@@ -1138,8 +1139,6 @@ int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_partic
                                 int directional, double weight, int precision, double* pd_out) {
   g_err.clear();
   TRY(check_common(ns, precision));
-  if (precision != SRS_PRECISION_FP64)
-    return fail(SRS_ERR_INVALID, "the batched power density is computed in FP64 (SRS_PRECISION_FP64)");
   if (!p || np < 1) return fail(SRS_ERR_INVALID, "no particles");
   if (np > (1 << 24)) return fail(SRS_ERR_INVALID, "too many particles");
   if (!obs_xyz || !normals_xyz || !pd_out) return fail(SRS_ERR_INVALID, "NULL argument");
@@ -1169,7 +1168,8 @@ int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_partic
   {
     KernelTimer timer(c, s);
     TRY(enqueue_pd_batch(c, bt, (int)np, nund, max_leg, max_pts, srs::Checkpoints{nullptr, 1, 0}, (int)ns,
-                         directional ? 1 : 0, pd_qfac_of(charge, current), c.g_field.as<double>(), s));
+                         directional ? 1 : 0, pd_qfac_of(charge, current), c.g_field.as<double>(), s,
+                         precision == SRS_PRECISION_MIXED));
     HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), (int)np, (int)ns, weight, c.b_out.as<double>(), s));
   }
   HIPCHK(hipMemcpyAsync(pd_out, c.b_out.p, (size_t)ns * sizeof(double), hipMemcpyDeviceToHost, s));

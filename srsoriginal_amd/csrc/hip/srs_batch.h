@@ -157,7 +157,7 @@ hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const do
 int pd_batch_chunks(int64_t npoints, int ns, int np);
 hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
                            int ns, int directional, int64_t lo, int64_t hi, int nch, int add, double* part,
-                           hipStream_t s);
+                           bool mixed, hipStream_t s);
 hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
                                   int directional, double* raw, double* pd, hipStream_t s);
 // Spectrum of np particles at one observation point for ne energies (omega [ne]):

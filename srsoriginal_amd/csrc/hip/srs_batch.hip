@@ -303,13 +303,51 @@ hipError_t launch_batch_m(BatchConfig const& cfg, BatchTraj const& bt, const Bat
 // points staged in LDS; part[(r * nch + ch) * ns + j] = the chunk's sum of pd_term,
 // or += it when add != 0 (time segments after the first, which run in order on one
 // stream, so every slot adds its segments in segment order).
+// MIXED: the FP32 term (pd_mixed_term), its records staged from the FP64 trajectory per
+// tile (1 - |beta|^2 in FP64), 32-term FP32 sub-sums in a Kahan FP32 accumulator; the
+// chunk's sum enters the FP64 slot as in the FP64 form.
+template <bool MIXED, bool DIR>
+__device__ __forceinline__ void pd_batch_mixed_chunk(BatchTraj const& bt, int64_t i0, int64_t i1, int64_t nbs,
+                                                     int64_t first_b, int64_t first_f, double ox, double oy, double oz,
+                                                     double qx, double qy, double qz, float4* sP, float4* sB,
+                                                     float4* sA, double& sum) {
+  int const tid = threadIdx.x;
+  float const fx = (float)ox, fy = (float)oy, fz = (float)oz, gx = (float)qx, gy = (float)qy, gz = (float)qz;
+  KahanF acc{0.f, 0.f};
+  for (int64_t base = i0; base < i1; base += kBlock) {
+    int const cnt = (i1 - base) < kBlock ? (int)(i1 - base) : kBlock;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const v = base + tid;
+      int64_t const i = v < nbs ? first_b + v : first_f + v;
+      double const bx = bt.bx[i], by = bt.by[i], bz = bt.bz[i];
+      double const omb = 1.0 - bz;
+      float const g = (float)(omb * (1.0 + bz) - bx * bx - by * by);
+      sP[tid] = make_float4((float)bt.x[i], (float)bt.y[i], (float)bt.z[i], g * (kPdS * kPdS));
+      sB[tid] = make_float4((float)bx * kPdS, (float)by * kPdS, (float)omb * kPdS, 0.f);
+      sA[tid] = make_float4((float)bt.ax[i], (float)bt.ay[i], (float)bt.az[i], 0.f);
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < cnt; k0 += kSub) {
+      int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
+      float sub = 0.f;
+      for (int k = k0; k < k1; ++k) sub += pd_mixed_term<DIR>(fx, fy, fz, gx, gy, gz, sP[k], sB[k], sA[k]);
+      acc.add(sub);
+    }
+  }
+  sum = ((double)acc.s - (double)acc.c) * 0x1p55;
+}
+
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                           const double* __restrict__ obs,
                                                           const double* __restrict__ nrm, int ns, int directional,
                                                           int64_t lo, int64_t hi, int nch, int ntiles, int add,
                                                           double* __restrict__ part) {
-  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sbx[kBlock], sby[kBlock], sbz[kBlock];
-  __shared__ double sax[kBlock], say[kBlock], saz[kBlock];
+  __shared__ double sx[MIXED ? 1 : kBlock], sy[MIXED ? 1 : kBlock], sz[MIXED ? 1 : kBlock];
+  __shared__ double sbx[MIXED ? 1 : kBlock], sby[MIXED ? 1 : kBlock], sbz[MIXED ? 1 : kBlock];
+  __shared__ double sax[MIXED ? 1 : kBlock], say[MIXED ? 1 : kBlock], saz[MIXED ? 1 : kBlock];
+  __shared__ float4 fP[MIXED ? kBlock : 1], fB[MIXED ? kBlock : 1], fA[MIXED ? kBlock : 1];
   int const tid = threadIdx.x;
   int const tile = (int)(blockIdx.x % (unsigned)ntiles);
   int const rc = (int)(blockIdx.x / (unsigned)ntiles);
@@ -327,6 +365,12 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
   double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
   double const qx = nrm[jj], qy = nrm[ns + jj], qz = nrm[2 * ns + jj];
   double sum = 0.0;
+  if constexpr (MIXED) {
+    if (directional)
+      pd_batch_mixed_chunk<MIXED, true>(bt, i0, i1, nbs, first_b, first_f, ox, oy, oz, qx, qy, qz, fP, fB, fA, sum);
+    else
+      pd_batch_mixed_chunk<MIXED, false>(bt, i0, i1, nbs, first_b, first_f, ox, oy, oz, qx, qy, qz, fP, fB, fA, sum);
+  } else
   for (int64_t base = i0; base < i1; base += kBlock) {
     int const cnt = (i1 - base) < kBlock ? (int)(i1 - base) : kBlock;
     __syncthreads();
@@ -632,12 +676,16 @@ int pd_batch_chunks(int64_t npoints, int ns, int np) {
 
 hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
                            int ns, int directional, int64_t lo, int64_t hi, int nch, int add, double* part,
-                           hipStream_t s) {
+                           bool mixed, hipStream_t s) {
   int const ntiles = (ns + kBlock - 1) / kBlock;
   int64_t const nblocks = (int64_t)ntiles * np * nch;
   if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-  pd_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch, ntiles, add,
-                                                       part);
+  if (mixed)
+    pd_batch_kernel<true><<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch, ntiles,
+                                                               add, part);
+  else
+    pd_batch_kernel<false><<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs, nrm, ns, directional, lo, hi, nch,
+                                                                ntiles, add, part);
   return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 // accumulators with Kahan compensation; each trajectory chunk writes FP64 partial
 // sums and the cross-chunk reduction is FP64 (shared with the plain form).
 #include "srs_kernels.h"
+#include "srs_pd.h"
 #include "srs_terms.h"
 
 namespace srs {
@@ -245,34 +246,6 @@ __global__ __launch_bounds__(64 * SPW) void spectrum_mixed_kernel(TrajView t, do
 // range limits when raised to the -5th power, and the term comes out as 2^-55 times
 // its value; 2^55 is restored in FP64 when the sum is written.  32-term FP32 sub-sums,
 // Kahan-compensated.
-constexpr float kPdS = 0x1p10f;
-
-template <bool DIR>
-__device__ __forceinline__ float pd_mixed_term(float ox, float oy, float oz, float qx, float qy, float qz, float4 P,
-                                               float4 B, float4 A) {
-  float const rx = ox - P.x, ry = oy - P.y, rz = oz - P.z;
-  float const inv = __builtin_amdgcn_rsqf(__builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)));
-  float const is = inv * kPdS;
-  float const nx = rx * is, ny = ry * is, nz = rz * is;  // S n
-  float const v = __builtin_fmaf(nx, nx, ny * ny);
-  // S (1 - n_z) = w / a with a = S + |S n_z|: w = S^2 (n_x^2 + n_y^2) for n_z > 0, a^2 else;
-  // the select picks between two plain values, so no branch is generated
-  float const a = kPdS + __builtin_fabsf(nz);
-  float const omnz = (nz > 0.f ? v : a * a) * __builtin_amdgcn_rcpf(a);
-  float const ux = nx - B.x, uy = ny - B.y, uz = B.z - omnz;                       // S u
-  float const d2 = __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, __builtin_fmaf(uz, uz, P.w)));
-  float const na = __builtin_fmaf(nx, A.x, __builtin_fmaf(ny, A.y, nz * A.z));     // S n.A
-  float const hd = 0.5f * d2;                                                      // S^2 (1 - beta.n)
-  float const mx = __builtin_fmaf(ux, na, -(A.x * hd)), my = __builtin_fmaf(uy, na, -(A.y * hd));
-  float const mz = __builtin_fmaf(uz, na, -(A.z * hd));                            // S^2 n x (u x A)
-  float const num2 = __builtin_fmaf(mx, mx, __builtin_fmaf(my, my, mz * mz));
-  float const ndot = __builtin_fmaf(nx, qx, __builtin_fmaf(ny, qy, nz * qz));      // S n.normal
-  float const rd = __builtin_amdgcn_rcpf(d2);
-  float const rd2 = rd * rd;
-  float const term = num2 * ndot * (inv * inv) * (rd2 * rd2 * rd);
-  return (DIR && ndot <= 0.f) ? 0.f : term;
-}
-
 template <bool DIR>
 __global__ __launch_bounds__(kBlock) void pd_mixed_kernel(TrajView t, const float* __restrict__ tf,
                                                           const double* __restrict__ obs, const double* __restrict__ nrm,

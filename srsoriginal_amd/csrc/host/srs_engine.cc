@@ -1133,7 +1133,7 @@ void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer&
   add_points(surf, C, dim);
   int const np = nparticles == 0 ? 1 : nparticles;
   double const w = nparticles == 0 ? 1.0 : 1.0 / (double)nparticles;
-  if (batched && gpu == 1 && nparticles > 1 && prec == Precision::FP64) {
+  if (batched && gpu == 1 && nparticles > 1) {
     // the gpu=1 route's draws and container points, then one device pipeline
     std::vector<srs_particle> parts;
     parts.reserve(np);

@@ -173,3 +173,27 @@ def test_batch_graph_follows_a_changed_field_list(abi, oracle):
         eager = abi.batch_flux(und, parts, p.q, p.current, xyz, [431.2], 0.25, abi.FP64, 0)
         graph = abi.batch_flux(und, parts, p.q, p.current, xyz, [431.2], 0.25, abi.FP64, abi.BATCH_GRAPH)
         assert np.array_equal(graph, eager)
+
+
+def test_batch_power_density_mixed_within_1e5():
+    """Mixed batched power density (pd_batch_kernel's FP32 terms) against the FP64 batch,
+    16 ragged particles, one segmented, directional and not, normal and tilted screen."""
+    from srsoriginal_amd import abi, batch
+    from srsoriginal_amd.lib import _sr
+    from srsoriginal_amd.sr import sr
+    say("batched power density mixed vs fp64: 16 particles, 101x51 over 40 mm and a tilted screen")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                          sigma_energy_GeV=0.00267, beta=[1.5, 0.8], emittance=[0.9e-9, 8e-12],
+                          horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
+    osr.set_seed(17)
+    rows, q, cur = batch.sample_particles(osr, 16, -0.2, 3.0, 3001, ct_jitter=0.3, points_per_meter=3000, rng_seed=8)
+    rows[3]["npoints"] = 12001  # segmented pipeline
+    for rot, directional in (((0, 0, 0), False), ((0.4, 0.3, 0), True)):
+        surf = _sr.Surface.rectangle("XY", 101, 51, 0.04, 0.02, list(rot), [0, 0, 30], 0)
+        f64 = batch.power_density(osr, rows, q, cur, surf.xyz, surf.normals, directional)
+        mix = batch.power_density(osr, rows, q, cur, surf.xyz, surf.normals, directional, precision="mixed")
+        err = normwise(mix, f64)
+        say("  rotations %s directional=%d: normwise %.3e" % (rot, directional, err))
+        assert err <= 1e-5
