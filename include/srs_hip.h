@@ -185,6 +185,14 @@ typedef struct srs_beam {
  * sampler, kept bit for bit on the host.  No counterpart on the reference's GPU. */
 int srs_hip_sample_beam(const srs_beam* beam, int64_t np, uint64_t seed, uint64_t first, srs_particle* out);
 
+/* The same from nbeams beams with weights (TParticleBeamContainer::GetRandomBeam,
+ * TParticleBeamContainer.cc:100-142): particle i takes the first beam k with
+ * u * W < w_0 + ... + w_k, u a fourth Philox block's 53-bit uniform and W the weights'
+ * sum (no draw for one beam), then that beam's transform.  Weights must be >= 0 with a
+ * positive sum. */
+int srs_hip_sample_beams(const srs_beam* beams, const double* weights, int nbeams, int64_t np, uint64_t seed,
+                         uint64_t first, srs_particle* out);
+
 /* Points the trajectories of np particles will have (total over particles). */
 int srs_hip_trajectory_points(const srs_particle* p, int64_t np, int64_t* total);
 

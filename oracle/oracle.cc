@@ -653,6 +653,27 @@ void oracle_counter_normals(uint64_t seed, uint64_t idx, double out[6]) {
   }
 }
 
+// Beam index for counter idx by weight (TParticleBeamContainer::GetRandomBeamIndexByWeight,
+// TParticleBeamContainer.cc:109-142): the first k with u * W < w_0 + ... + w_k, u the
+// 53-bit uniform of a fourth Philox block (counter (lo, hi, 3, 0)); 0 for one beam.
+int oracle_beam_index(const double* weights, int nb, uint64_t seed, uint64_t idx) {
+  if (nb <= 1) return 0;
+  double sum = 0;
+  for (int k = 0; k < nb; ++k) sum += weights[k];
+  uint32_t const key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t const ctr[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 3u, 0u};
+  uint32_t w[4];
+  oracle_philox4x32_10(ctr, key, w);
+  uint64_t const a = (((uint64_t)w[0] << 32) | w[1]) >> 11;
+  double const r = std::ldexp((double)a, -53) * sum;
+  double acc = 0;
+  for (int k = 0; k < nb; ++k) {
+    acc += weights[k];
+    if (r < acc) return k;
+  }
+  return nb - 1;
+}
+
 void oracle_sample_beam(const oracle_beam* bm, uint64_t seed, uint64_t idx, oracle_particle* out) {
   // the reference's particle transform on the counter-based draws n[0..4]
   std::memset(out, 0, sizeof(*out));

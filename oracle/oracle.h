@@ -77,6 +77,7 @@ typedef struct oracle_beam {
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void oracle_counter_normals(uint64_t seed, uint64_t idx, double out[6]);
 void oracle_sample_beam(const oracle_beam* bm, uint64_t seed, uint64_t idx, oracle_particle* out);
+int oracle_beam_index(const double* weights, int nb, uint64_t seed, uint64_t idx);
 void oracle_random_electron(double energy_gev, double sigma_e_gev, const double x0[3], const double d0[3],
                             const double hdir[3], const double beta[2], const double emit[2],
                             const double lattice_ref[3], double current, oracle_particle* out);

@@ -130,6 +130,11 @@ def sample_beam(b, seed, idx):
     return p
 
 
+def beam_index(weights, seed, idx):
+    w = (D * len(weights))(*weights)
+    return lib().oracle_beam_index(w, C.c_int(len(weights)), C.c_uint64(seed), C.c_uint64(idx))
+
+
 def trajectory(fields, particle, ct_start, ct_stop, npoints):
     n = lib().oracle_trajectory_npoints(ct_start, ct_stop, npoints, particle.t0)
     t = np.zeros((9, n))

@@ -237,6 +237,18 @@ class SrsBeam(C.Structure):
                 ("t0", _D), ("ct_start", _D), ("ct_stop", _D), ("npoints", _I64)]
 
 
+def _srs_beam(beam, ct_start, ct_stop, npoints):
+    return SrsBeam(float(beam["e0"]), float(beam["sigma_e"]), float(beam["mass"]), float(beam["charge"]),
+                   (_D * 3)(*beam["x0"]), (_D * 3)(*beam["u0"]), (_D * 3)(*beam["hdir"]), (_D * 3)(*beam["vdir"]),
+                   (_D * 3)(*beam["sigma_at"]), (_D * 2)(*beam["sigma_u"]), (_D * 2)(*beam["sigma_up"]),
+                   float(beam.get("t0", 0.0)), float(ct_start), float(ct_stop), int(npoints))
+
+
+def _rows(out):
+    return [dict(x0=list(r.x0), beta0=list(r.beta0), qmg=r.qmg, t0=r.t0, ct_start=r.ct_start, ct_stop=r.ct_stop,
+                 npoints=r.npoints) for r in out]
+
+
 def sample_beam(beam, np_, seed, first=0, ct_start=0.0, ct_stop=1.0, npoints=2):
     """srs_hip_sample_beam: np_ particles of beam (a dict as sr._e.beam_params returns)
     drawn on the device from counters first .. first + np_ - 1 under key seed; returns
@@ -244,14 +256,24 @@ def sample_beam(beam, np_, seed, first=0, ct_start=0.0, ct_stop=1.0, npoints=2):
     L = lib()
     L.srs_hip_sample_beam.argtypes = [C.POINTER(SrsBeam), _I64, C.c_uint64, C.c_uint64, C.POINTER(SrsParticle)]
     L.srs_hip_sample_beam.restype = C.c_int
-    b = SrsBeam(float(beam["e0"]), float(beam["sigma_e"]), float(beam["mass"]), float(beam["charge"]),
-                (_D * 3)(*beam["x0"]), (_D * 3)(*beam["u0"]), (_D * 3)(*beam["hdir"]), (_D * 3)(*beam["vdir"]),
-                (_D * 3)(*beam["sigma_at"]), (_D * 2)(*beam["sigma_u"]), (_D * 2)(*beam["sigma_up"]),
-                float(beam.get("t0", 0.0)), float(ct_start), float(ct_stop), int(npoints))
+    b = _srs_beam(beam, ct_start, ct_stop, npoints)
     out = (SrsParticle * int(np_))()
     _check(L.srs_hip_sample_beam(C.byref(b), int(np_), C.c_uint64(int(seed)), C.c_uint64(int(first)), out))
-    return [dict(x0=list(r.x0), beta0=list(r.beta0), qmg=r.qmg, t0=r.t0, ct_start=r.ct_start, ct_stop=r.ct_stop,
-                 npoints=r.npoints) for r in out]
+    return _rows(out)
+
+
+def sample_beams(beams, weights, np_, seed, first=0, ct_start=0.0, ct_stop=1.0, npoints=2):
+    """srs_hip_sample_beams: as sample_beam, each particle from a beam chosen by weight."""
+    L = lib()
+    L.srs_hip_sample_beams.argtypes = [C.POINTER(SrsBeam), _P, C.c_int, _I64, C.c_uint64, C.c_uint64,
+                                       C.POINTER(SrsParticle)]
+    L.srs_hip_sample_beams.restype = C.c_int
+    arr = (SrsBeam * len(beams))(*[_srs_beam(b, ct_start, ct_stop, npoints) for b in beams])
+    w = _f64(weights)
+    out = (SrsParticle * int(np_))()
+    _check(L.srs_hip_sample_beams(arr, w.ctypes.data_as(_P), len(beams), int(np_), C.c_uint64(int(seed)),
+                                  C.c_uint64(int(first)), out))
+    return _rows(out)
 
 
 def particles(rows):

@@ -16,16 +16,20 @@ from . import abi
 
 
 def sample_particles_device(osr, n, ct_start, ct_stop, npoints, seed, first=0, beam=None):
-    """Draw n random particles of one beam on the device (srs_hip_sample_beam): counter-
-    based (Philox4x32-10), so particle i is the same whatever n and call order; the
-    transform is the reference's (TParticleBeam.cc:228-265).  beam = the beam's name
-    (default: the first beam added).  Returns (rows, charge, current)."""
+    """Draw n random particles on the device (srs_hip_sample_beams): counter-based
+    (Philox4x32-10), so particle i is the same whatever n and call order; each particle
+    from a beam chosen by the beams' weights (TParticleBeamContainer.cc:100-142), then the
+    reference's transform (TParticleBeam.cc:228-265).  beam = one beam's name to draw
+    from it alone.  Returns (rows, charge, current); the beams must share charge and
+    current (as the batched pipelines require)."""
     names = osr._e.beam_names()
     if not names:
         raise ValueError("No particle beam defined")
-    bp = osr._e.beam_params(beam if beam is not None else names[0])
-    rows = abi.sample_beam(bp, n, seed, first, ct_start, ct_stop, npoints)
-    return rows, bp["charge"], bp["current"]
+    bps = [osr._e.beam_params(beam)] if beam is not None else [osr._e.beam_params(nm) for nm in names]
+    if any(b["charge"] != bps[0]["charge"] or b["current"] != bps[0]["current"] for b in bps):
+        raise ValueError("the beams must share charge and current")
+    rows = abi.sample_beams(bps, [b["weight"] for b in bps], n, seed, first, ct_start, ct_stop, npoints)
+    return rows, bps[0]["charge"], bps[0]["current"]
 
 
 def sample_particles(osr, n, ct_start, ct_stop, npoints, ct_jitter=0.0, points_per_meter=None, rng_seed=0):

@@ -833,44 +833,71 @@ static_assert(sizeof(srs::SampledParticle) == sizeof(srs_particle) &&
                   offsetof(srs::SampledParticle, npoints) == offsetof(srs_particle, npoints),
               "sampled rows are srs_particle rows");
 
-int srs_hip_sample_beam(const srs_beam* beam, int64_t np, uint64_t seed, uint64_t first, srs_particle* out) {
+int srs_hip_sample_beams(const srs_beam* beams, const double* weights, int nbeams, int64_t np, uint64_t seed,
+                         uint64_t first, srs_particle* out) {
   g_err.clear();
-  if (!beam || !out) return fail(SRS_ERR_INVALID, "NULL argument");
+  if (!beams || !out || (nbeams > 1 && !weights)) return fail(SRS_ERR_INVALID, "NULL argument");
+  if (nbeams < 1) return fail(SRS_ERR_INVALID, "no beam defined");
   if (np < 1) return fail(SRS_ERR_INVALID, "no particles requested");
-  if (!(beam->mass > 0)) return fail(SRS_ERR_INVALID, "beam particle mass must be > 0");
-  double const mgev = 1e-9 * beam->mass * kC * kC / kQe;  // TSRS::kgToGeV, TOSCARSSR.h:68
-  if (!(beam->e0 > mgev)) return fail(SRS_ERR_INVALID, "beam energy must exceed the particle's rest energy");
-  if (!(beam->sigma_e >= 0)) return fail(SRS_ERR_INVALID, "'sigma_energy_GeV' cannot be less than zero");
-  if (beam->npoints < 2) return fail(SRS_ERR_INVALID, "npoints must be >= 2");
+  std::vector<srs::DevBeam> db(nbeams);
+  std::vector<double> wsum(nbeams);
+  double acc = 0;
+  for (int k = 0; k < nbeams; ++k) {
+    srs_beam const& beam = beams[k];
+    if (!(beam.mass > 0)) return fail(SRS_ERR_INVALID, "beam particle mass must be > 0");
+    double const mgev = 1e-9 * beam.mass * kC * kC / kQe;  // TSRS::kgToGeV, TOSCARSSR.h:68
+    if (!(beam.e0 > mgev)) return fail(SRS_ERR_INVALID, "beam energy must exceed the particle's rest energy");
+    if (!(beam.sigma_e >= 0)) return fail(SRS_ERR_INVALID, "'sigma_energy_GeV' cannot be less than zero");
+    if (beam.npoints < 2) return fail(SRS_ERR_INVALID, "npoints must be >= 2");
+    double const w = nbeams > 1 ? weights[k] : 1.0;
+    if (!(w >= 0)) return fail(SRS_ERR_INVALID, "beam weights must be >= 0");
+    acc += w;  // the running sums of TParticleBeamContainer::AddNewParticleBeam
+    wsum[k] = acc;
+    srs::DevBeam& b = db[k];
+    b = srs::DevBeam{};
+    b.e0 = beam.e0;
+    b.sigma_e = beam.sigma_e;
+    b.mgev = mgev;
+    b.q = beam.charge;
+    b.m = beam.mass;
+    for (int i = 0; i < 3; ++i) {
+      b.x0[i] = beam.x0[i];
+      b.u0[i] = beam.u0[i];
+      b.h[i] = beam.hdir[i];
+      b.v[i] = beam.vdir[i];
+      b.at[i] = beam.sigma_at[i];
+    }
+    for (int i = 0; i < 2; ++i) {
+      b.su[i] = beam.sigma_u[i];
+      b.sp[i] = beam.sigma_up[i];
+    }
+    b.t0 = beam.t0;
+    b.ct_start = beam.ct_start;
+    b.ct_stop = beam.ct_stop;
+    b.npoints = beam.npoints;
+  }
+  if (!(acc > 0)) return fail(SRS_ERR_INVALID, "beam weights must have a positive sum");
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
   TRY(init(c));
-  srs::DevBeam b{};
-  b.e0 = beam->e0;
-  b.sigma_e = beam->sigma_e;
-  b.mgev = mgev;
-  b.q = beam->charge;
-  b.m = beam->mass;
-  for (int k = 0; k < 3; ++k) {
-    b.x0[k] = beam->x0[k];
-    b.u0[k] = beam->u0[k];
-    b.h[k] = beam->hdir[k];
-    b.v[k] = beam->vdir[k];
-    b.at[k] = beam->sigma_at[k];
-  }
-  for (int k = 0; k < 2; ++k) {
-    b.su[k] = beam->sigma_u[k];
-    b.sp[k] = beam->sigma_up[k];
-  }
-  b.t0 = beam->t0;
-  b.ct_start = beam->ct_start;
-  b.ct_stop = beam->ct_stop;
-  b.npoints = beam->npoints;
-  HIPCHK(c.out.ensure((size_t)np * sizeof(srs::SampledParticle)));
-  HIPCHK(srs::launch_sample_beam(b, seed, first, np, c.out.as<srs::SampledParticle>(), c.stream));
-  HIPCHK(hipMemcpyAsync(out, c.out.p, (size_t)np * sizeof(srs_particle), hipMemcpyDeviceToHost, c.stream));
+  // one buffer: beam records | cumulative weights | rows, each part 256-byte aligned
+  size_t const beam_bytes = ((nbeams * sizeof(srs::DevBeam) + 255) / 256) * 256;
+  size_t const w_bytes = ((nbeams * sizeof(double) + 255) / 256) * 256;
+  HIPCHK(c.out.ensure(beam_bytes + w_bytes + (size_t)np * sizeof(srs::SampledParticle)));
+  char* const base = c.out.as<char>();
+  auto* const dbeams = reinterpret_cast<srs::DevBeam*>(base);
+  auto* const dw = reinterpret_cast<double*>(base + beam_bytes);
+  auto* const drows = reinterpret_cast<srs::SampledParticle*>(base + beam_bytes + w_bytes);
+  HIPCHK(hipMemcpyAsync(dbeams, db.data(), nbeams * sizeof(srs::DevBeam), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(dw, wsum.data(), nbeams * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(srs::launch_sample_beam(dbeams, dw, nbeams, seed, first, np, drows, c.stream));
+  HIPCHK(hipMemcpyAsync(out, drows, (size_t)np * sizeof(srs_particle), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   return SRS_OK;
+}
+
+int srs_hip_sample_beam(const srs_beam* beam, int64_t np, uint64_t seed, uint64_t first, srs_particle* out) {
+  return srs_hip_sample_beams(beam, nullptr, 1, np, seed, first, out);
 }
 
 int srs_hip_total_power(const srs_traj* traj, double charge, double current, double gamma, double* power_out) {

@@ -101,8 +101,9 @@ struct SampledParticle {
   double x0[3], beta0[3], qmg, t0, ct_start, ct_stop;
   int64_t npoints;
 };
-hipError_t launch_sample_beam(DevBeam const& b, uint64_t seed, uint64_t first, int64_t np, SampledParticle* out,
-                              hipStream_t s);
+// beams: nb device records; wsum: their cumulative weights (device)
+hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
+                              int64_t np, SampledParticle* out, hipStream_t s);
 
 // local: the list holds elements other than undulators (Gaussian, box, field map)
 hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, const double* poles, const BatchParticle* P,

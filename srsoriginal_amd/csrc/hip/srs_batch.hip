@@ -576,11 +576,27 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 // TParticleBeam::GetNewParticle (TParticleBeam.cc:228-265) in the reference's operation
 // order with the draws n0..n4 (energy, x, y, x', y'), and TParticleA's gamma and
 // q/(m gamma) from the drawn beta (TParticleA.cc:353-382).
-__global__ __launch_bounds__(256) void sample_beam_kernel(DevBeam b, uint64_t seed, uint64_t first, int64_t np,
+__global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restrict__ beams,
+                                                          const double* __restrict__ wsum, int nb, uint64_t seed,
+                                                          uint64_t first, int64_t np,
                                                           SampledParticle* __restrict__ out) {
   int64_t const p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   uint64_t const idx = first + (uint64_t)p;
+  int sel = 0;
+  if (nb > 1) {  // TParticleBeamContainer::GetRandomBeamIndexByWeight
+    uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 3u, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t const a = (((uint64_t)c[0] << 32) | c[1]) >> 11;
+    double const r = ((double)a * 1.1102230246251565404e-16) * wsum[nb - 1];
+    sel = nb - 1;
+    for (int k = 0; k < nb; ++k)
+      if (r < wsum[k]) {
+        sel = k;
+        break;
+      }
+  }
+  DevBeam const b = beams[sel];
   double n[6];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
@@ -632,9 +648,9 @@ __global__ __launch_bounds__(256) void sample_beam_kernel(DevBeam b, uint64_t se
   o.npoints = b.npoints;
 }
 
-hipError_t launch_sample_beam(DevBeam const& b, uint64_t seed, uint64_t first, int64_t np, SampledParticle* out,
-                              hipStream_t s) {
-  sample_beam_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(b, seed, first, np, out);
+hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
+                              int64_t np, SampledParticle* out, hipStream_t s) {
+  sample_beam_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(beams, wsum, nb, seed, first, np, out);
   return hipGetLastError();
 }
 

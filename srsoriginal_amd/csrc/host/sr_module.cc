@@ -189,6 +189,7 @@ PYBIND11_MODULE(_sr, m) {
              d["sigma_u"] = std::vector<double>{b.sigma_u[0], b.sigma_u[1]};
              d["sigma_up"] = std::vector<double>{b.sigma_up[0], b.sigma_up[1]};
              d["t0"] = b.t0;
+             d["weight"] = b.weight;
              return d;
            })
       .def("beam_names",

@@ -46,3 +46,37 @@ def test_device_sampler_matches_oracle_and_is_counter_based():
     sx, sxp = bp["sigma_u"][0] * 2.5, bp["sigma_up"][0]  # 1 + (sigma_at - x0).u0 = 2.5
     say("  std x / expected %.4f, std x' / expected %.4f" % (x.std() / sx, xp.std() / sxp))
     assert abs(x.std() / sx - 1) < 0.01 and abs(xp.std() / sxp - 1) < 0.01
+
+
+def test_device_sampler_weighted_beams_match_oracle():
+    """srs_hip_sample_beams with three beams (distinct starts and energies) and weights
+    1 : 3 : 2: each particle's beam and values against the CPU restatement; the engine's
+    beams and weights through batch.sample_particles_device."""
+    from oracle import pyoracle as O
+    from srsoriginal_amd import abi, batch
+    from srsoriginal_amd.sr import sr
+    say("device sampler, three weighted beams vs oracle_beam_index + oracle_sample_beam")
+    osr = sr()
+    for k, (e, w) in enumerate(((3.0, 1.0), (2.5, 3.0), (3.2, 2.0))):
+        osr.add_particle_beam(type="electron", name="b%d" % k, energy_GeV=e, x0=[0.001 * k, 0, -1.5], d0=[0, 0, 1],
+                              current=0.5, weight=w, sigma_energy_GeV=0.001, beta=[1.5, 0.8],
+                              emittance=[0.9e-9, 8e-12], horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
+    bps = [osr._e.beam_params("b%d" % k) for k in range(3)]
+    ws = [b["weight"] for b in bps]
+    assert ws == [1.0, 3.0, 2.0]
+    rows, q, cur = batch.sample_particles_device(osr, 3000, 0.0, 3.0, 2001, seed=5)
+    counts = [0, 0, 0]
+    worst = 0.0
+    for i, r in enumerate(rows):
+        k = O.beam_index(ws, 5, i)
+        counts[k] += 1
+        o = O.sample_beam(bps[k], 5, i)
+        for a, b in ((r["x0"], o.x0), (r["beta0"], o.beta0)):
+            a, b = np.array(a), np.array(list(b))
+            worst = max(worst, float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-12))))
+        worst = max(worst, abs(r["qmg"] - o.qmg) / abs(o.qmg))
+    say("  beams drawn %s, max relative deviation %.3e" % (counts, worst))
+    assert worst <= 1e-14
+    assert all(abs(c / 3000 - w / 6.0) < 0.03 for c, w in zip(counts, ws))
+    with pytest.raises(abi.SrsError, match="weights"):
+        abi.sample_beams(bps, [0.0, 0.0, 0.0], 10, 1)

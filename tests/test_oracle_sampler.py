@@ -61,3 +61,15 @@ def test_sample_beam_closed_forms():
                  (by, b["sigma_up"][1]), (e - 3.0, b["sigma_e"])):
         assert abs(v.std() / s - 1) < 0.03 and abs(v.mean()) < 0.03 * s
     assert math.isclose(e.mean(), 3.0, rel_tol=1e-4)
+
+
+def test_beam_index_by_weight():
+    """Weighted beam choice (TParticleBeamContainer::GetRandomBeamIndexByWeight): the
+    frequencies follow the weights, a zero-weight beam is never chosen, one beam needs no
+    draw."""
+    from oracle import pyoracle as O
+    w = [1.0, 3.0, 0.0, 2.0]
+    idx = np.array([O.beam_index(w, 31, i) for i in range(30000)])
+    f = np.bincount(idx, minlength=4) / len(idx)
+    assert np.all(np.abs(f - np.array(w) / sum(w)) < 0.01) and f[2] == 0
+    assert O.beam_index([5.0], 31, 7) == 0
