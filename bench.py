@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 
 ENERGY_EV = 143.52176088044024   # first-harmonic argmax of the reference's C0 spectrum grid
 F_ALG_FLUX = 53                  # algorithmic FLOP per flux term
+F_ALG_SPEC = 30                  # per spectrum term
+F_ALG_PD = 57                    # per power-density term
 PEAK_FP32_VALU = 157.3e12        # MI355X FP32 vector peak (published)
 PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
 # Issue model of the dominant loop (flux_mixed_kernel<1,256>, paraxial terms), from the
@@ -38,6 +40,15 @@ PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
 # = 66 cycles, plus ~3 for the 32-term sub-tile epilogue.
 ISSUE_CYCLES_PER_WAVE_TERM = 69
 SIMDS, CLOCK_HZ = 1024, 2.4e9
+
+
+def _frac(flop_per_term, terms, fp64_ms, mixed_ms):
+    """Algorithmic rate of a kernel pair against the published vector peaks (FP64 form
+    against 78.6 TF/s, mixed form against 157.3 TF/s FP32)."""
+    a64, amx = flop_per_term * terms / (fp64_ms * 1e-3), flop_per_term * terms / (mixed_ms * 1e-3)
+    return {"flop_per_term": flop_per_term, "unit": "TFLOP/s",
+            "fp64": {"achieved": a64 / 1e12, "peak": PEAK_FP64_VALU / 1e12, "frac": a64 / PEAK_FP64_VALU},
+            "mixed": {"achieved": amx / 1e12, "peak": PEAK_FP32_VALU / 1e12, "frac": amx / PEAK_FP32_VALU}}
 # HBM bytes per launch of flux_mixed_kernel<1,256> on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
 # what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
@@ -149,7 +160,8 @@ def spectrum_and_power_density(reps=5):
                           "fp64_terms_per_s": terms0 / (spec[abi.FP64][0] * 1e-3),
                           "mixed_terms_per_s": terms0 / (spec[abi.MIXED][0] * 1e-3),
                           "fp64_kernel_ms": spec[abi.FP64][0], "mixed_kernel_ms": spec[abi.MIXED][0],
-                          "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(smx - s64)) / np.max(np.abs(s64)))}
+                          "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(smx - s64)) / np.max(np.abs(s64))),
+                          "roofline": _frac(F_ALG_SPEC, terms0, spec[abi.FP64][0], spec[abi.MIXED][0])}
     # config 2
     osr = sr()
     osr.add_bfield_undulator(bfield=[0, 0.6, 0], period=[0, 0, 0.057], nperiods=30)
@@ -190,7 +202,8 @@ def spectrum_and_power_density(reps=5):
                                "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(pmx - p64)) / np.max(np.abs(p64))),
                                "mixed_vs_fp64_rms_rel_err": float(np.sqrt(np.mean((pmx - p64) ** 2))
                                                                   / np.sqrt(np.mean(p64 ** 2))),
-                               "integrated_power_W": {"fp64": float(p64.sum() * area), "mixed": float(pmx.sum() * area)}}
+                               "integrated_power_W": {"fp64": float(p64.sum() * area), "mixed": float(pmx.sum() * area)},
+                               "roofline": _frac(F_ALG_PD, terms2, pdr[abi.FP64][0], pdr[abi.MIXED][0])}
     log("spectrum C0: fp64 %.3f ms, mixed %.3f ms; power density C2: fp64 %.3f ms, mixed %.3f ms"
         % (spec[abi.FP64][0], spec[abi.MIXED][0], pdr[abi.FP64][0], pdr[abi.MIXED][0]))
     return res
@@ -238,8 +251,8 @@ def batched_power_density(nparticles=256, reps=2):
 
 def batched_spectrum(nparticles=256, reps=2):
     """Multi-particle spectrum (the C3 Gaussian beam on the C0 undulator, ragged 18k-22k
-    point trajectories) at (0, 0, 30 m) over the C0 grid of 2000 energies, FP64, one
-    device pipeline; timed with HIP events."""
+    point trajectories) at (0, 0, 30 m) over the C0 grid of 2000 energies, FP64 and mixed,
+    one device pipeline; timed with HIP events."""
     from srsoriginal_amd import abi, batch
     from srsoriginal_amd.sr import sr
     osr = sr()
@@ -253,19 +266,25 @@ def batched_spectrum(nparticles=256, reps=2):
     energies = 100.0 + (3000.0 - 100.0) / 1999.0 * (np.arange(2000) + 1)  # TSpectrumContainer grid
     npts = sum(abi.trajectory_points(abi.particles([r])) for r in rows)
     terms = float(npts) * len(energies)
-    out = batch.spectrum(osr, rows, q, cur, [0.0, 0.0, 30.0], energies)  # warm-up
-    ms = []
-    for _ in range(reps):
-        abi.timing_enable(True)
-        out = batch.spectrum(osr, rows, q, cur, [0.0, 0.0, 30.0], energies)
-        kms, _ = abi.timing_read()
-        abi.timing_enable(False)
-        ms.append(kms)
-    t = min(ms)
-    log("batched spectrum: %.1f ms (%d particles)" % (t, nparticles))
+    res = {}
+    for prec in ("double", "mixed"):
+        out = batch.spectrum(osr, rows, q, cur, [0.0, 0.0, 30.0], energies, precision=prec)  # warm-up
+        ms = []
+        for _ in range(reps):
+            abi.timing_enable(True)
+            out = batch.spectrum(osr, rows, q, cur, [0.0, 0.0, 30.0], energies, precision=prec)
+            kms, _ = abi.timing_read()
+            abi.timing_enable(False)
+            ms.append(kms)
+        res[prec] = (min(ms), out)
+    t, out = res["double"]
+    tm, om = res["mixed"]
+    log("batched spectrum: fp64 %.1f ms, mixed %.1f ms (%d particles)" % (t, tm, nparticles))
     return {"workload": "%d particles x %d traj points (ragged 18k-22k) x 2000 energies at (0,0,30 m)"
                         % (nparticles, npts),
             "terms": terms, "fp64_ms": t, "fp64_terms_per_s": terms / (t * 1e-3),
+            "mixed_ms": tm, "mixed_terms_per_s": terms / (tm * 1e-3),
+            "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(om - out)) / np.max(np.abs(out))),
             "peak_flux": float(np.max(out)),
             "timed": "device RK4 in time segments overlapped with spectrum_batch_kernel, reductions (HIP events)"}
 

@@ -225,7 +225,9 @@ int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_partic
 /* Multi-particle spectrum at one observation point: flux_out[e] = sum over particles
  * (in order, Kahan) of weight * spectrum_p(e).  Replaces the nparticles loop of
  * OSCARSSR::CalculateSpectrum (OSCARSSR.cc:963-982: new particle, host RK4, one
- * spectrum per particle with Weight = 1/NP).  FP64. */
+ * spectrum per particle with Weight = 1/NP).  precision SRS_PRECISION_FP64 (the
+ * reference's FP64 terms) or SRS_PRECISION_MIXED (FP64 phase, FP32 sin/cos, amplitudes
+ * and sub-sums). */
 int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                            double current, const double obs[3], const double* energies_ev, int64_t ne, double weight,
                            int precision, double* flux_out);

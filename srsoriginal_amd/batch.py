@@ -62,10 +62,11 @@ def flux(osr, rows, charge, current, obs_xyz, energies_ev, precision="double", g
                           1.0 / len(rows), prec, flags)
 
 
-def spectrum(osr, rows, charge, current, obs, energies_ev):
+def spectrum(osr, rows, charge, current, obs, energies_ev, precision="double"):
     """Sum over particles of spectrum/NP at obs (3,) for each energy -> array (ne,)."""
+    prec = {"double": abi.FP64, "fp64": abi.FP64, "mixed": abi.MIXED}[precision]
     return abi.batch_spectrum(osr._e.undulators(), abi.particles(rows), charge, current, obs, energies_ev,
-                              1.0 / len(rows))
+                              1.0 / len(rows), prec)
 
 
 def power_density(osr, rows, charge, current, obs_xyz, normals_xyz, directional=False, precision="double"):

@@ -1209,10 +1209,9 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
                            int precision, double* flux_out) {
   g_err.clear();
   TRY(check_common(ne, precision));
-  if (precision != SRS_PRECISION_FP64)
-    return fail(SRS_ERR_INVALID, "the batched spectrum is computed in FP64 (SRS_PRECISION_FP64)");
   if (!p || np < 1) return fail(SRS_ERR_INVALID, "no particles");
   if (np > (1 << 24)) return fail(SRS_ERR_INVALID, "too many particles");
+  bool const mixed = precision == SRS_PRECISION_MIXED;
   if (!obs || !energies_ev || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
   std::vector<srs::BatchParticle> P;
   int64_t total = 0;
@@ -1263,7 +1262,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
       HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
                                    srs::RK4Seg{0, INT64_MAX, nullptr}, s));
       HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, 0, INT64_MAX, nch, 0,
-                                        c.g_part.as<double>(), s));
+                                        c.g_part.as<double>(), mixed, s));
     } else {
       HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
       HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
@@ -1275,7 +1274,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
       for (int k = 0; k < nseg; ++k) {
         HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
         HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, k * L, (k + 1) * L, nch, k > 0,
-                                          c.g_part.as<double>(), s));
+                                          c.g_part.as<double>(), mixed, s));
       }
     }
     HIPCHK(srs::launch_spectrum_batch_reduce(c.g_part.as<double>(), nch, (int)ne, (int)np, Pd, ed + ne, ed + 2 * ne,

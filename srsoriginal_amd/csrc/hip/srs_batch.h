@@ -168,7 +168,7 @@ hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np,
 // i (C0 omega) dt_r and C2: out[r*ne + e].
 hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double obs[3],
                                  const double* omega, int ne, int64_t lo, int64_t hi, int nch, int add, double* part,
-                                 hipStream_t s);
+                                 bool mixed, hipStream_t s);
 hipError_t launch_spectrum_batch_reduce(const double* part, int nch, int ne, int np, const BatchParticle* P,
                                         const double* cw, const double* c0w, double c2, double* out, hipStream_t s);
 // dL/dy (gy [6][total]) and dL/dA (ga [3][total]) of L_r = sum_j (pd_rj - target_j)^2

@@ -424,12 +424,17 @@ __global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch,
 // reference's operation order as spectrum_fp64_kernel) is staged once in LDS and
 // shared by the block's energies.  part[((r * nch + ch) * 12 + k) * ne + e] = the
 // chunk's sums of A e^{i phi} and G e^{i phi} (12 reals), written or added (add).
+// MIXED: the amplitudes staged as FP32, the reference's FP64 phase omega tau reduced
+// to one turn in FP64, sin / cos in FP32 on revolutions, 32-term FP32
+// sub-sums in Kahan FP32 accumulators; the FP64 form keeps the reference's FP64 terms.
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void spectrum_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                                 double ox, double oy, double oz,
                                                                 const double* __restrict__ omega, int ne, int64_t lo,
                                                                 int64_t hi, int nch, int ntiles, int add,
                                                                 double* __restrict__ part) {
-  __shared__ double stau[kBlock], sa[3][kBlock], sg[3][kBlock];
+  __shared__ double stau[kBlock], sa[3][MIXED ? 1 : kBlock], sg[3][MIXED ? 1 : kBlock];
+  __shared__ float fa[3][MIXED ? kBlock : 1], fg[3][MIXED ? kBlock : 1];
   int const tid = threadIdx.x;
   int const tile = (int)(blockIdx.x % (unsigned)ntiles);
   int const rc = (int)(blockIdx.x / (unsigned)ntiles);
@@ -445,8 +450,11 @@ __global__ __launch_bounds__(kBlock) void spectrum_batch_kernel(BatchTraj bt, co
   int const e = tile * kBlock + tid;
   double const w = omega[e < ne ? e : ne - 1];
   double acc[12];
+  KahanF facc[MIXED ? 12 : 1];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < (MIXED ? 12 : 1); ++k) facc[k] = KahanF{0.f, 0.f};
   for (int64_t base = i0; base < i1; base += kBlock) {
     int const cnt = (i1 - base) < kBlock ? (int)(i1 - base) : kBlock;
     __syncthreads();
@@ -460,26 +468,69 @@ __global__ __launch_bounds__(kBlock) void spectrum_batch_kernel(BatchTraj bt, co
       stau[tid] = q.dt * (double)it + div_rn(D, kC, kInvC);
       double const inv = 1.0 / D;
       double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
-      sa[0][tid] = (bt.bx[i] - nx) * inv;
-      sa[1][tid] = (bt.by[i] - ny) * inv;
-      sa[2][tid] = (bt.bz[i] - nz) * inv;
+      double const ax = (bt.bx[i] - nx) * inv, ay = (bt.by[i] - ny) * inv, az = (bt.bz[i] - nz) * inv;
       double const inv2 = inv * inv;
-      sg[0][tid] = nx * inv2;
-      sg[1][tid] = ny * inv2;
-      sg[2][tid] = nz * inv2;
-    }
-    __syncthreads();
-    for (int k = 0; k < cnt; ++k) {
-      double s, c;
-      sincos_fp64(w * stau[k], &s, &c);
-#pragma unroll
-      for (int qq = 0; qq < 3; ++qq) {
-        acc[4 * qq + 0] = __builtin_fma(sa[qq][k], c, acc[4 * qq + 0]);
-        acc[4 * qq + 1] = __builtin_fma(sa[qq][k], s, acc[4 * qq + 1]);
-        acc[4 * qq + 2] = __builtin_fma(sg[qq][k], c, acc[4 * qq + 2]);
-        acc[4 * qq + 3] = __builtin_fma(sg[qq][k], s, acc[4 * qq + 3]);
+      double const gx = nx * inv2, gy = ny * inv2, gz = nz * inv2;
+      if constexpr (MIXED) {
+        fa[0][tid] = (float)ax;
+        fa[1][tid] = (float)ay;
+        fa[2][tid] = (float)az;
+        fg[0][tid] = (float)gx;
+        fg[1][tid] = (float)gy;
+        fg[2][tid] = (float)gz;
+      } else {
+        sa[0][tid] = ax;
+        sa[1][tid] = ay;
+        sa[2][tid] = az;
+        sg[0][tid] = gx;
+        sg[1][tid] = gy;
+        sg[2][tid] = gz;
       }
     }
+    __syncthreads();
+    if constexpr (MIXED) {
+      for (int k0 = 0; k0 < cnt; k0 += kSub) {
+        int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
+        float sub[12];
+#pragma unroll
+        for (int qq = 0; qq < 12; ++qq) sub[qq] = 0.f;
+        for (int k = k0; k < k1; ++k) {
+          // the reference's phase p = RN(omega tau), reduced by one exact FMA
+          // (p - k RN(2 pi) is a multiple of 2^-47 below 4 in magnitude); leaving
+          // out 2 pi's low part turns p into p (1 + 3.9e-17), a constant relative
+          // frequency scale whose phase offset cancels in |.|^2
+          double const p = w * stau[k];
+          double const kk = __builtin_rint(p * kInvTwoPi);
+          float const ph = (float)__builtin_fma(-kk, kTwoPiHi, p) * 0.159154943f;
+          float const sn = __builtin_amdgcn_sinf(ph), cn = __builtin_amdgcn_cosf(ph);
+#pragma unroll
+          for (int qq = 0; qq < 3; ++qq) {
+            sub[4 * qq + 0] = __builtin_fmaf(fa[qq][k], cn, sub[4 * qq + 0]);
+            sub[4 * qq + 1] = __builtin_fmaf(fa[qq][k], sn, sub[4 * qq + 1]);
+            sub[4 * qq + 2] = __builtin_fmaf(fg[qq][k], cn, sub[4 * qq + 2]);
+            sub[4 * qq + 3] = __builtin_fmaf(fg[qq][k], sn, sub[4 * qq + 3]);
+          }
+        }
+#pragma unroll
+        for (int qq = 0; qq < 12; ++qq) facc[qq].add(sub[qq]);
+      }
+    } else {
+      for (int k = 0; k < cnt; ++k) {
+        double s, c;
+        sincos_fp64(w * stau[k], &s, &c);
+#pragma unroll
+        for (int qq = 0; qq < 3; ++qq) {
+          acc[4 * qq + 0] = __builtin_fma(sa[qq][k], c, acc[4 * qq + 0]);
+          acc[4 * qq + 1] = __builtin_fma(sa[qq][k], s, acc[4 * qq + 1]);
+          acc[4 * qq + 2] = __builtin_fma(sg[qq][k], c, acc[4 * qq + 2]);
+          acc[4 * qq + 3] = __builtin_fma(sg[qq][k], s, acc[4 * qq + 3]);
+        }
+      }
+    }
+  }
+  if constexpr (MIXED) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = (double)facc[k].s - (double)facc[k].c;
   }
   if (e < ne) {
 #pragma unroll
@@ -715,12 +766,16 @@ hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np,
 
 hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double obs[3],
                                  const double* omega, int ne, int64_t lo, int64_t hi, int nch, int add, double* part,
-                                 hipStream_t s) {
+                                 bool mixed, hipStream_t s) {
   int const ntiles = (ne + kBlock - 1) / kBlock;
   int64_t const nblocks = (int64_t)ntiles * np * nch;
   if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-  spectrum_batch_kernel<<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs[0], obs[1], obs[2], omega, ne, lo, hi, nch,
-                                                             ntiles, add, part);
+  if (mixed)
+    spectrum_batch_kernel<true><<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs[0], obs[1], obs[2], omega, ne, lo, hi,
+                                                                     nch, ntiles, add, part);
+  else
+    spectrum_batch_kernel<false><<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs[0], obs[1], obs[2], omega, ne, lo,
+                                                                      hi, nch, ntiles, add, part);
   return hipGetLastError();
 }
 

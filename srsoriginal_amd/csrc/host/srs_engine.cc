@@ -923,7 +923,7 @@ void OSCARSSR::CalculateSpectrum(Vec3 const& obs, SpectrumContainer& S, int npar
     return;
   }
   double const w = 1.0 / (double)nparticles;
-  if (batched && gpu == 1 && nparticles > 1 && prec == Precision::FP64) {
+  if (batched && gpu == 1 && nparticles > 1) {
     std::vector<srs_particle> parts;
     parts.reserve(nparticles);
     double q = 0, cur = 0;

@@ -197,3 +197,26 @@ def test_batch_power_density_mixed_within_1e5():
         err = normwise(mix, f64)
         say("  rotations %s directional=%d: normwise %.3e" % (rot, directional, err))
         assert err <= 1e-5
+
+
+def test_batch_spectrum_mixed_within_1e5():
+    """Mixed batched spectrum (spectrum_batch_kernel's FP32 terms, FP64 phase) against the
+    FP64 batch: 12 ragged particles, one segmented, 2000 energies on and off axis."""
+    from srsoriginal_amd import batch
+    from srsoriginal_amd.sr import sr
+    say("batched spectrum mixed vs fp64: 12 particles, 2000 energies")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5,
+                          sigma_energy_GeV=0.00267, beta=[1.5, 0.8], emittance=[0.9e-9, 8e-12],
+                          horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
+    osr.set_seed(23)
+    rows, q, cur = batch.sample_particles(osr, 12, -0.2, 3.0, 3001, ct_jitter=0.3, points_per_meter=3000, rng_seed=9)
+    rows[5]["npoints"] = 12001
+    energies = 100.0 + (3000.0 - 100.0) / 1999 * (np.arange(2000) + 1)
+    for obs in ([0.0, 0.0, 30.0], [0.0012, -0.0004, 30.0]):
+        f64 = batch.spectrum(osr, rows, q, cur, obs, energies)
+        mix = batch.spectrum(osr, rows, q, cur, obs, energies, precision="mixed")
+        err = normwise(mix, f64)
+        say("  obs %s: normwise %.3e" % (obs, err))
+        assert err <= 1e-5
