@@ -31,6 +31,7 @@ ENERGY_EV = 143.52176088044024   # first-harmonic argmax of the reference's C0 s
 F_ALG_FLUX = 53                  # algorithmic FLOP per flux term
 F_ALG_SPEC = 30                  # per spectrum term
 F_ALG_PD = 57                    # per power-density term
+F_ALG_FLUX1 = 77                 # per CalculateFlux1 (Lienard-Wiechert) term
 PEAK_FP32_VALU = 157.3e12        # MI355X FP32 vector peak (published)
 PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
 # Issue model of the dominant loop (flux_mixed_kernel<1,256>, paraxial terms), from the
@@ -207,6 +208,33 @@ def spectrum_and_power_density(reps=5):
     log("spectrum C0: fp64 %.3f ms, mixed %.3f ms; power density C2: fp64 %.3f ms, mixed %.3f ms"
         % (spec[abi.FP64][0], spec[abi.MIXED][0], pdr[abi.FP64][0], pdr[abi.MIXED][0]))
     return res
+
+
+def flux1_c1(reps=5):
+    """CalculateFlux1's near- plus far-field flux (srs_hip_flux1, FP64) on the flux
+    map's geometry: 201x201 over 20 mm at 30 m, 20001 trajectory points, 143.52 eV;
+    kernel time from HIP events around flux1_fp64_kernel."""
+    from srsoriginal_amd import abi
+    traj, dt, xyz, q, cur = setup_problem()
+    e = 143.52
+    for _ in range(2):
+        out = abi.flux1(traj, dt, xyz, e, q, cur)
+    abi.timing_enable(True)
+    for _ in range(reps):
+        out = abi.flux1(traj, dt, xyz, e, q, cur)
+    kms, nl = abi.timing_read()
+    abi.timing_enable(False)
+    kms /= max(nl, 1)
+    terms = float(traj.shape[1]) * len(xyz)
+    ach = F_ALG_FLUX1 * terms / (kms * 1e-3)
+    log("flux1 C1 geometry: kernel %.3f ms" % kms)
+    return {"flux1_c1": {"workload": "CalculateFlux1 (Lienard-Wiechert with A/c) 201x201, 20 mm at 30 m, "
+                                     "143.52 eV, NT=20001, FP64",
+                         "kernel": "flux1_fp64_kernel", "kernel_ms": kms, "terms_per_s": terms / (kms * 1e-3),
+                         "peak_flux": float(out.max()),
+                         "roofline": {"bound": "valu", "flop_per_term": F_ALG_FLUX1, "unit": "TFLOP/s",
+                                      "achieved": ach / 1e12, "peak": PEAK_FP64_VALU / 1e12,
+                                      "frac": ach / PEAK_FP64_VALU}}}
 
 
 def batched_power_density(nparticles=256, reps=2):
@@ -404,7 +432,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
-    ap.add_argument("--no-other", action="store_true", help="skip the config-0 spectrum and config-2 power density")
+    ap.add_argument("--no-other", action="store_true", help="skip the config-0 spectrum, config-2 power density and CalculateFlux1 lines")
     ap.add_argument("--profile", action="store_true",
                     help="one launch of each form plus a 256 MiB device copy (the HBM counter scale), for rocprofv3")
     a = ap.parse_args()
@@ -502,6 +530,7 @@ def main():
         out["gradient_power_density"] = gradient_pd()
     if not a.no_other:
         out.update(spectrum_and_power_density())
+        out.update(flux1_c1())
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(traj, dt, xyz, q, cur)

@@ -69,6 +69,14 @@ int srs_hip_spectrum(const srs_traj* traj, const double obs[3], const double* en
 int srs_hip_flux(const srs_traj* traj, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
                  double charge, double current, int precision, double* flux_out);
 
+/* Near- plus far-field flux on ns observation points (AoS xyz) at one photon energy,
+ * from the Lienard-Wiechert field with the stored A/c (the trajectory's ax..az are
+ * required): the sum of OSCARSSR::CalculateFlux1 (OSCARSSR.cc:2209-2346, its CPU
+ * loop :2266-2300; the reference has no GPU form of it), Omega = E 2 pi / 4.1357e-15
+ * as written there.  FP64; flux_out[j] in photons/s/0.1%bw/mm^2, unweighted. */
+int srs_hip_flux1(const srs_traj* traj, const double* obs_xyz, int64_t ns, double energy_ev, double charge,
+                  double current, double* flux_out);
+
 /* Power density [W/mm^2] on ns points with surface normals (AoS xyz).
  * directional != 0 skips trajectory points seen from behind the surface and keeps
  * the sign; otherwise |sum| (the reference CPU semantics, OSCARSSR.cc:1549-1573).

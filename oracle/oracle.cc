@@ -17,6 +17,8 @@
 //   oracle_power_density . bitwise vs reference (C2 full 301x301, directional tilt)
 //                          + integral vs total power
 //   oracle_total_power ... bitwise vs reference (C0, C2)
+//   oracle_flux1 ......... bitwise vs the reference's CalculateFlux1 on the trajectory it
+//                          used; far-field limit vs oracle_flux (integration by parts)
 //   random particles ..... bitwise vs reference (seeded mt19937/normal stream)
 //   oracle_pole_loss_gradient  dual-number forward mode; vs the reference's own central
 //                          differences (h = 1e-6) of the pole-error flux loss
@@ -851,6 +853,42 @@ void oracle_efield_time(const double* traj, int64_t n, double dt, const double o
     e_out[3 * i + 1] = E.y;
     e_out[3 * i + 2] = E.z;
     t_out[i] = (double)i * dt + D / kC;
+  }
+}
+
+void oracle_flux1(const double* traj, int64_t n, double dt, const double* obs, int64_t ns, double energy_ev, double q,
+                  double current, double* out) {
+  // OSCARSSR::CalculateFlux1, OSCARSSR.cc:2209-2346: the Lienard-Wiechert field with the
+  // stored A/c (velocity term + acceleration term, no integration by parts), summed over
+  // the trajectory in order; TVector3D / TVector3DC / std::complex operation order.
+  const double *X = traj, *Y = traj + n, *Z = traj + 2 * n, *BX = traj + 3 * n, *BY = traj + 4 * n, *BZ = traj + 5 * n;
+  const double *AX = traj + 6 * n, *AY = traj + 7 * n, *AZ = traj + 8 * n;
+  double const C0 = q / (4 * kPi * kC * kEps0 * kSqrt2Pi);                                         // :2245
+  double const C2 = kFourPi * current / (kH * std::fabs(q) * kMu0 * kC) * 1e-6 * 0.001;             // :2249
+  double const Omega = energy_ev * kTwoPi / 4.1357e-15;                                            // :2255
+  for (int64_t ip = 0; ip < ns; ++ip) {
+    V3 const O = v3(obs + 3 * ip);
+    std::complex<double> Sx(0, 0), Sy(0, 0), Sz(0, 0);
+    for (int64_t iT = 0; iT < n; ++iT) {
+      V3 const B{BX[iT], BY[iT], BZ[iT]};
+      V3 const A{AX[iT], AY[iT], AZ[iT]};
+      V3 const R = sub(O, V3{X[iT], Y[iT], Z[iT]});
+      V3 const N = unit(R);
+      double const D = std::sqrt(mag2(R));
+      std::complex<double> const Exponent(0, -Omega * (dt * (double)(int)iT + D / kC));
+      double const p2 = std::pow(1 - dot(N, B), 2);
+      V3 const NB = sub(N, B);
+      V3 const Near = divs(muls(NB, 1 - mag2(B)), D * D * p2);
+      V3 const Far = divs(cross(N, cross(NB, A)), D * p2);
+      std::complex<double> const e = std::exp(Exponent);
+      Sx += std::complex<double>(Near.x + Far.x, 0) * e * dt;
+      Sy += std::complex<double>(Near.y + Far.y, 0) * e * dt;
+      Sz += std::complex<double>(Near.z + Far.z, 0) * e * dt;
+    }
+    Sx *= C0;
+    Sy *= C0;
+    Sz *= C0;
+    out[ip] = C2 * (Sx * std::conj(Sx) + Sy * std::conj(Sy) + Sz * std::conj(Sz)).real();
   }
 }
 

@@ -128,6 +128,11 @@ int oracle_pole_pd_loss_gradient(const oracle_fields* f, const oracle_particle* 
 void oracle_efield_time(const double* traj, int64_t n, double dt, const double obs[3], double q, double* e_out,
                         double* t_out);
 
+/* Near- plus far-field flux of OSCARSSR::CalculateFlux1 (OSCARSSR.cc:2209-2346) on ns
+ * points (AoS xyz) at one photon energy; out[i] = its ThisFlux (no container). */
+void oracle_flux1(const double* traj, int64_t n, double dt, const double* obs, int64_t ns, double energy_ev, double q,
+                  double current, double* out);
+
 /* Kahan add of TSpectrumContainer::AddToFlux / T3DScalarContainer::AddToPoint. */
 void oracle_kahan_add(double* sum, double* comp, const double* v, int64_t n);
 

@@ -181,6 +181,15 @@ def flux(traj, dt, obs, energy_ev, q, current, weight=1.0, nthreads=1):
     return out
 
 
+def flux1(traj, dt, obs, energy_ev, q, current):
+    """CalculateFlux1's near- plus far-field flux on the points obs [n, 3] (traj needs A/c)."""
+    traj = np.ascontiguousarray(traj); obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(obs))
+    lib().oracle_flux1(_p(traj), I64(traj.shape[1]), D(dt), _p(obs), I64(len(obs)), D(energy_ev), D(q), D(current),
+                       _p(out))
+    return out
+
+
 def power_density(traj, dt, obs, normals, directional, q, current, nthreads=1):
     traj = np.ascontiguousarray(traj); obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
     nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)

@@ -100,6 +100,15 @@ static TVector3D v3(std::string const& k, TVector3D def) {
   return v.size() == 3 ? TVector3D(v[0], v[1], v[2]) : def;
 }
 
+// Zero a stretch of the stack below the caller's frame.  CalculateTrajectory stores its
+// uninitialised dxdt as the first point's A/c (OSCARSSR.cc:604-630); called right after
+// this, that array sits in zeroed memory, so the value is 0 (the restatement's convention)
+// instead of leftover bytes.
+__attribute__((noinline)) static void scrub_stack() {
+  volatile char buf[1 << 16];
+  for (size_t i = 0; i < sizeof(buf); ++i) buf[i] = 0;
+}
+
 static FILE* gOut = 0;
 static void put(double v) { fwrite(&v, sizeof(double), 1, gOut); }
 
@@ -284,6 +293,27 @@ int main(int argc, char** argv) {
         osr.CalculatePowerDensity(S, C, (int)num("dim", 2), directional, nparticles, str("ofile", ""), nthreads, 0);
       }
       write_container(C);
+    } else if (mode == "flux1") {
+      // OSCARSSR::CalculateFlux1 (no Python binding; the caller adds the container's
+      // points, 2-D as its hard-coded Dimension): the container, then the trajectory
+      // it computed and used (layout of "traj"), whose first A/c is the reference's
+      // uninitialised value
+      TSurfacePoints_Rectangle S = rect();
+      T3DScalarContainer C;
+      for (size_t i = 0; i != S.GetNPoints(); ++i) C.AddPoint(TVector3D(S.GetX1(i), S.GetX2(i), 0), 0);
+      TParticleA P = osr.GetCurrentParticle();
+      if (num("scrub", 1) != 0) scrub_stack();
+      osr.CalculateFlux1(P, S, num("energy_ev", 100), C, str("ofile", ""));
+      write_container(C);
+      TParticleTrajectoryPoints& T = P.GetTrajectory();
+      size_t const n = T.GetNPoints();
+      put((double)n);
+      put(T.GetDeltaT());
+      for (int c = 0; c < 9; ++c)
+        for (size_t i = 0; i < n; ++i) {
+          TVector3D const& V = c < 3 ? T.GetX(i) : (c < 6 ? T.GetB(i) : T.GetAoverC(i));
+          put(V[c % 3]);
+        }
     } else if (mode == "totalpower") {
       put(osr.CalculateTotalPower());
     } else if (mode == "efield") {

@@ -46,6 +46,7 @@ def lib():
             "srs_hip_spectrum": [C.POINTER(SrsTraj), _P, _P, _I64, _D, _D, C.c_int, _P],
             "srs_hip_flux": [C.POINTER(SrsTraj), _P, _I64, _P, _I64, _D, _D, C.c_int, _P],
             "srs_hip_power_density": [C.POINTER(SrsTraj), _P, _P, _I64, C.c_int, _D, _D, C.c_int, _P],
+            "srs_hip_flux1": [C.POINTER(SrsTraj), _P, _I64, _D, _D, _D, _P],
             "srs_hip_flux_device": [C.POINTER(SrsTraj), C.c_void_p, _I64, _P, _I64, _D, _D, C.c_int, C.c_void_p, C.c_void_p],
             "srs_hip_spectrum_device": [C.POINTER(SrsTraj), _P, _P, _I64, _D, _D, C.c_int, C.c_void_p, C.c_void_p],
             "srs_hip_power_density_device": [C.POINTER(SrsTraj), C.c_void_p, C.c_void_p, _I64, C.c_int, _D, _D, C.c_int,
@@ -109,6 +110,17 @@ def flux(traj, dt, obs_xyz, energies_ev, charge, current, precision=FP64):
     _check(lib().srs_hip_flux(C.byref(t), o.ctypes.data_as(_P), len(o), e.ctypes.data_as(_P), len(e), charge, current,
                               precision, out.ctypes.data_as(_P)))
     return out.reshape(len(e), len(o))
+
+
+def flux1(traj, dt, obs_xyz, energy_ev, charge, current):
+    """CalculateFlux1's near- plus far-field flux (srs_hip_flux1); traj needs A/c."""
+    keep = []
+    t = traj_struct(traj, dt, keep)
+    o = _f64(obs_xyz).reshape(-1, 3)
+    out = np.zeros(len(o))
+    _check(lib().srs_hip_flux1(C.byref(t), o.ctypes.data_as(_P), len(o), energy_ev, charge, current,
+                               out.ctypes.data_as(_P)))
+    return out
 
 
 def power_density(traj, dt, obs_xyz, normals_xyz, directional, charge, current, precision=FP64):

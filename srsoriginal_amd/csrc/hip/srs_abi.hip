@@ -267,6 +267,26 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
   return SRS_OK;
 }
 
+// CalculateFlux1's sum (OSCARSSR.cc:2209-2346): Omega = (E 2 pi) / 4.1357e-15 as the
+// reference writes it (not omega_of), C0 with sqrt(2 pi), the C2 of the other flux routes.
+int run_flux1(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, double energy_ev, double q, double cur,
+              double* out, hipStream_t s) {
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  int const chunk = pick_chunk(t.n, ns, 32, (int64_t)c.cus * srs::flux1_fp64_blocks_per_cu());
+  int const nch = nchunks(t.n, chunk);
+  HIPCHK(c.part.ensure((size_t)nch * 6 * ns * sizeof(double)));
+  EnergySet es{};
+  es.ne = 1;
+  es.omega[0] = energy_ev * kTwoPi / 4.1357e-15;
+  es.kw[0] = c0_of(q) * t.dt;
+  {
+    KernelTimer timer(c, s);
+    HIPCHK(srs::launch_flux1_fp64(t, obs_soa, (int)ns, es.omega[0], chunk, c.part.as<double>(), s));
+  }
+  HIPCHK(srs::launch_flux_reduce(c.part.as<double>(), nch, (int)ns, es, c2_of(q, cur), out, s));
+  return SRS_OK;
+}
+
 int run_spectrum(Context& c, const TrajView& t, const double obs[3], const double* energies_ev, int64_t ne, double q,
                  double cur, int precision, double* out, hipStream_t s) {
   double const c0 = c0_of(q), c2 = c2_of(q, cur);
@@ -783,6 +803,26 @@ int srs_hip_flux(const srs_traj* traj, const double* obs_xyz, int64_t ns, const 
   HIPCHK(c.out.ensure(ne * ns * sizeof(double)));
   TRY(run_flux(c, t, c.obs.as<double>(), ns, energies_ev, ne, charge, current, precision, c.out.as<double>(), c.stream));
   HIPCHK(hipMemcpyAsync(flux_out, c.out.p, ne * ns * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return SRS_OK;
+}
+
+int srs_hip_flux1(const srs_traj* traj, const double* obs_xyz, int64_t ns, double energy_ev, double charge,
+                  double current, double* flux_out) {
+  g_err.clear();
+  TRY(check_traj(traj, true));
+  TRY(check_common(ns, SRS_PRECISION_FP64));
+  if (!obs_xyz || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  if (!(energy_ev > 0)) return fail(SRS_ERR_INVALID, "photon energy must be > 0");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  TrajView t;
+  TRY(stage_traj(c, traj, true, &t));
+  TRY(stage_points(c, c.obs, obs_xyz, ns));
+  HIPCHK(c.out.ensure(ns * sizeof(double)));
+  TRY(run_flux1(c, t, c.obs.as<double>(), ns, energy_ev, charge, current, c.out.as<double>(), c.stream));
+  HIPCHK(hipMemcpyAsync(flux_out, c.out.p, ns * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   return SRS_OK;
 }

@@ -184,6 +184,75 @@ __global__ __launch_bounds__(kBlock) void efield_time_kernel(TrajView t, double 
   tm[i] = (double)i * t.dt + D / kC;
 }
 
+// Near- plus far-field flux of OSCARSSR::CalculateFlux1 (OSCARSSR.cc:2209-2346): the
+// Lienard-Wiechert field with the stored A/c, no integration by parts,
+//   E_k = [ (1 - |b|^2)(n - b) / (D D (1 - n.b)^2) + n x ((n - b) x A/c) / (D (1 - n.b)^2) ]
+//         e^{-i Omega (k dt + D/c)},
+// one lane per observation point, the trajectory staged in LDS tiles.  1 - n.b and
+// 1 - |b|^2 lose ~8 digits to cancellation at GeV energies, so n = R/|R| (IEEE
+// quotients), the dot products and the phase keep the reference's TVector3D operation
+// order; the well-conditioned amplitude quotients use reciprocals.
+// part[(c * 6 + 2 q + {0: re, 1: im}) * ns + j]: the chunk's sum for axis q (times dt
+// later, with C0, in flux_reduce_kernel).
+__global__ __launch_bounds__(kBlock) void flux1_fp64_kernel(TrajView t, const double* __restrict__ obs, int ns,
+                                                            double omega, int chunk, double* __restrict__ part) {
+  __shared__ double sx[TILE], sy[TILE], sz[TILE], sbx[TILE], sby[TILE], sbz[TILE], sax[TILE], say[TILE], saz[TILE];
+  int const tid = threadIdx.x;
+  int const j = blockIdx.x * kBlock + tid;
+  int const jj = j < ns ? j : ns - 1;
+  double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+  int64_t const i0 = (int64_t)blockIdx.y * chunk;
+  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = base + tid;
+      sx[tid] = t.x[i];
+      sy[tid] = t.y[i];
+      sz[tid] = t.z[i];
+      sbx[tid] = t.bx[i];
+      sby[tid] = t.by[i];
+      sbz[tid] = t.bz[i];
+      sax[tid] = t.ax[i];
+      say[tid] = t.ay[i];
+      saz[tid] = t.az[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+      double const bx = sbx[k], by = sby[k], bz = sbz[k];
+      double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
+      double const D = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+      double const inv = 1.0 / D;
+      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
+      double const omb = 1.0 - (nx * bx + ny * by + nz * bz);
+      double const sb = 1.0 - (bx * bx + by * by + bz * bz);
+      double const ux = nx - bx, uy = ny - by, uz = nz - bz;
+      double const ax = sax[k], ay = say[k], az = saz[k];
+      double const vx = uy * az - ay * uz, vy = uz * ax - az * ux, vz = ux * ay - ax * uy;
+      double const fx = ny * vz - vy * nz, fy = nz * vx - vz * nx, fz = nx * vy - vx * ny;
+      double const ip2 = 1.0 / (omb * omb);
+      double const k2 = ip2 * inv;        // 1 / (D (1 - n.b)^2)
+      double const k1 = (sb * k2) * inv;  // (1 - |b|^2) / (D D (1 - n.b)^2)
+      double const Vx = ux * k1 + fx * k2, Vy = uy * k1 + fy * k2, Vz = uz * k1 + fz * k2;
+      double const tau = t.dt * (double)(base + k) + div_rn(D, kC, kInvC);
+      double s, c;
+      sincos_fp64(omega * tau, &s, &c);  // e^{-i phi} = (c, -s)
+      acc[0] = __builtin_fma(Vx, c, acc[0]);
+      acc[1] = __builtin_fma(-Vx, s, acc[1]);
+      acc[2] = __builtin_fma(Vy, c, acc[2]);
+      acc[3] = __builtin_fma(-Vy, s, acc[3]);
+      acc[4] = __builtin_fma(Vz, c, acc[4]);
+      acc[5] = __builtin_fma(-Vz, s, acc[5]);
+    }
+  }
+  if (j < ns) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) part[((size_t)blockIdx.y * 6 + q) * ns + j] = acc[q];
+  }
+}
+
 // Chunk reductions (fixed order) and the reference's final prefactors:
 //   flux:  SumE *= C1*dt (OSCARSSR.cc:2080) -> (re, im) * (0 + i kw) = (-im kw, re kw);
 //          F = C2 * Re(SumE . conj(SumE)) summed x, y then z (OSCARSSR.cc:2092).
@@ -337,6 +406,18 @@ int spectrum_fp64_blocks_per_cu() {
 int pd_fp64_blocks_per_cu() {
   static int c;
   return blocks_per_cu(pd_fp64_kernel, kBlock, c);
+}
+
+int flux1_fp64_blocks_per_cu() {
+  static int c;
+  return blocks_per_cu(flux1_fp64_kernel, kBlock, c);
+}
+
+hipError_t launch_flux1_fp64(const TrajView& t, const double* obs_soa, int ns, double omega, int chunk, double* part,
+                             hipStream_t s) {
+  dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  flux1_fp64_kernel<<<grid, kBlock, 0, s>>>(t, obs_soa, ns, omega, chunk, part);
+  return hipGetLastError();
 }
 
 hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,

@@ -29,6 +29,7 @@ hipError_t launch_total_power(const TrajView& t, double scale, double* out, hipS
 int flux_fp64_blocks_per_cu(int ne);
 int spectrum_fp64_blocks_per_cu();
 int pd_fp64_blocks_per_cu();
+int flux1_fp64_blocks_per_cu();
 int flux_mixed_blocks_per_cu(int ne);
 int pd_mixed_blocks_per_cu(int directional);
 
@@ -39,6 +40,10 @@ hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, co
 // Spectrum partial sums: part[(c*12 + k)*ne + e] (12 = A and G sums, re/im, xyz).
 hipError_t launch_spectrum_fp64(const TrajView& t, const double obs[3], const double* omega, const double* cw, int ne,
                                 int chunk, double* part, hipStream_t s);
+// Near- plus far-field flux (CalculateFlux1) partial sums: part[(c*6 + k)*ns + j], the
+// layout of launch_flux_fp64 with one energy (flux_reduce_kernel applies C0 dt and C2).
+hipError_t launch_flux1_fp64(const TrajView& t, const double* obs_soa, int ns, double omega, int chunk, double* part,
+                             hipStream_t s);
 // Power density partial sums: part[c*ns + j].
 hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double* nrm_soa, int ns, int directional,
                           int chunk, double* part, hipStream_t s);

@@ -294,6 +294,16 @@ PYBIND11_MODULE(_sr, m) {
            },
            py::arg("surface"), py::arg("dim"), py::arg("directional"), py::arg("nparticles"), py::arg("nthreads"),
            py::arg("gpu"), py::arg("precision"), py::arg("batched") = false)
+      .def("calculate_flux1",
+           [](srs::OSCARSSR& e, srs::SurfacePoints const& S, double energy, std::string const& ofile) {
+             // the container's 2-D points, as the flux routes add them for dim = 2
+             srs::ScalarContainer C;
+             for (size_t i = 0; i < S.size(); ++i)
+               C.add_point(S.has_x12 ? srs::Vec3(S.x1[i], S.x2[i], 0) : srs::Vec3(0, 0, 0), 0);
+             e.CalculateFlux1(S, energy, C, ofile);
+             return container_arrays(C);
+           },
+           py::arg("surface"), py::arg("energy"), py::arg("ofile") = "")
       .def("calculate_total_power", &srs::OSCARSSR::CalculateTotalPower)
       .def("calculate_total_power_device", &srs::OSCARSSR::CalculateTotalPowerDevice)
       .def("calculate_efield_vs_time", [](srs::OSCARSSR& e, std::vector<double> obs) {

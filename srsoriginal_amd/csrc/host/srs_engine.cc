@@ -1184,6 +1184,38 @@ void OSCARSSR::CalculateElectricFieldTimeDomain(Vec3 const& obs, std::vector<dou
   check_status(srs_hip_efield_time(&tr, o, particle_.q, e.data(), t.data()));
 }
 
+void OSCARSSR::CalculateFlux1(Particle& P, SurfacePoints const& surf, double energy_ev, ScalarContainer& C,
+                              std::string const& ofile) {
+  // OSCARSSR.cc:2209-2346: particle check, trajectory, the sum per surface point (on the
+  // device), then AddToPoint(ip, flux) or one "X1 X2 flux" line per point (Dimension is
+  // hard-coded to 2 there; point lists have X1 = X2 = 0).
+  if (P.type.empty()) throw std::out_of_range("no particle defined");
+  std::ofstream of;
+  if (!ofile.empty()) {
+    of.open(ofile.c_str());
+    if (!of.is_open()) throw std::ofstream::failure("cannot open output file");
+    of << std::scientific;
+  }
+  CalculateTrajectory(P);
+  srs_traj const t = view(P.traj);
+  std::vector<double> out(surf.size());
+  if (!out.empty())
+    check_status(srs_hip_flux1(&t, surf.xyz.data(), (int64_t)surf.size(), energy_ev, P.q, P.current, out.data()));
+  for (size_t i = 0; i < out.size(); ++i) {
+    if (of.is_open()) {
+      of << (surf.has_x12 ? surf.x1[i] : 0.0) << " " << (surf.has_x12 ? surf.x2[i] : 0.0) << " " << out[i] << "\n";
+    } else {
+      C.add_to(i, out[i]);
+    }
+  }
+}
+
+void OSCARSSR::CalculateFlux1(SurfacePoints const& surf, double energy_ev, ScalarContainer& C,
+                              std::string const& ofile) {
+  ensure_particle();
+  CalculateFlux1(particle_, surf, energy_ev, C, ofile);
+}
+
 double OSCARSSR::CalculateTotalPower() {
   // OSCARSSR::CalculateTotalPower(TParticleA&), OSCARSSR.cc:1939-1978 (host, O(NT))
   ensure_particle();

@@ -213,8 +213,8 @@ class OSCARSSR {
   // host trajectory + one GPU call per particle.
   void CalculateFlux(SurfacePoints const& surf, double energy_ev, ScalarContainer& C, int nparticles, int nthreads,
                      int gpu, int dim, Precision prec = Precision::FP64, bool batched = false);
-  // batched: the same for the power density (srs_hip_batch_power_density, FP64; the
-  // mixed form keeps the per-particle route).
+  // batched: the same for the power density (srs_hip_batch_power_density, FP64 or
+  // mixed).
   void CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer& C, int dim, bool directional, int nparticles,
                              int nthreads, int gpu, Precision prec = Precision::FP64, bool batched = false);
   double CalculateTotalPower();
@@ -222,6 +222,14 @@ class OSCARSSR {
   // OSCARSSR::CalculateElectricFieldTimeDomain (OSCARSSR.cc:2627-2690): host RK4, the
   // field on the device; e [3 n], t [n].
   void CalculateElectricFieldTimeDomain(Vec3 const& obs, std::vector<double>& e, std::vector<double>& t);
+  // OSCARSSR::CalculateFlux1 (OSCARSSR.cc:2209-2346), the Lienard-Wiechert form of the
+  // flux with the stored A/c: P's trajectory recomputed (host RK4), the sum on the
+  // device (srs_hip_flux1, FP64), each point's value added to point i of C (the
+  // caller adds the points, as the reference's callers of the flux sums do), or, with
+  // an ofile, "X1 X2 flux" lines written in std::scientific instead.
+  void CalculateFlux1(Particle& P, SurfacePoints const& surf, double energy_ev, ScalarContainer& C,
+                      std::string const& ofile = "");
+  void CalculateFlux1(SurfacePoints const& surf, double energy_ev, ScalarContainer& C, std::string const& ofile = "");
 
  private:
   void ensure_particle();

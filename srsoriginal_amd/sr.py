@@ -308,9 +308,9 @@ class sr:
     # ---- spectrum (OSCARSSR_Python.cc:2215-2351) --------------------------------
     def calculate_spectrum(self, obs=None, npoints=0, energy_range_eV=None, points_eV=None, nparticles=0, nthreads=0,
                            gpu=0, ofile="", bofile="", precision="double", batched=1):
-        """Extension: with gpu=1, nparticles > 1 and FP64, batched=1 (default) draws all
-        particles first and integrates them in one device pipeline; batched=0 keeps the
-        reference's per-particle loop."""
+        """Extension: with gpu=1 and nparticles > 1, batched=1 (default) draws all
+        particles first and integrates them in one device pipeline (FP64 or mixed);
+        batched=0 keeps the reference's per-particle loop."""
         if nparticles < 0:
             raise ValueError("'nparticles' must be >= 1 (sort of)")
         pts = [float(v) for v in (points_eV or [])]
@@ -358,13 +358,27 @@ class sr:
                                         _prec(precision), bool(batched))
         return self._out(xyz, v, ofile, 3)
 
+    def calculate_flux1_rectangle(self, energy_eV, npoints, plane="", normal=0, width=None, rotations=None,
+                                  translation=None, x0x1x2=None, ofile=""):
+        """OSCARSSR::CalculateFlux1 (OSCARSSR.cc:2209-2346), which the reference keeps in
+        C++ only: the flux of the current particle from the Lienard-Wiechert field with
+        the stored A/c (velocity and acceleration terms, no integration by parts;
+        Omega = 2 pi E / 4.1357e-15 as written there), summed on the device in FP64.
+        Returns [[[x1, x2, 0], flux], ...]; with ofile, writes its "X1 X2 flux" lines
+        (std::scientific) there instead and the values stay 0, as the reference does."""
+        if self._e.n_beams() < 1:
+            raise ValueError("No particle beam defined")
+        surf = self._rectangle(npoints, plane, width, rotations, translation, x0x1x2, normal)
+        xyz, v = self._e.calculate_flux1(surf, float(energy_eV), str(ofile))
+        return self._out(xyz, v, "", 2)
+
     # ---- power density (OSCARSSR_Python.cc:2403-2577, :2587-2807) ----------------
     def calculate_power_density_rectangle(self, npoints, plane="", width=None, x0x1x2=None, rotations=None,
                                           translation=None, ofile="", normal=0, nparticles=0, gpu=0, nthreads=0,
                                           dim=2, precision="double", batched=1):
-        """Extension: with gpu=1, nparticles > 1 and FP64, batched=1 (default) draws all
-        particles first and integrates them in one device pipeline; batched=0 keeps the
-        reference's per-particle loop."""
+        """Extension: with gpu=1 and nparticles > 1, batched=1 (default) draws all
+        particles first and integrates them in one device pipeline (FP64 or mixed);
+        batched=0 keeps the reference's per-particle loop."""
         if self._e.n_beams() < 1:
             raise ValueError("No particle beam defined")
         if dim != 2 and dim != 3:

@@ -313,6 +313,41 @@ def efield_time_c0():
     a, dt = run("efield", args)
     return args, {"container": container(a)}, dt
 
+def _flux1(args):
+    """OSCARSSR::CalculateFlux1 through the harness: the container, then the trajectory
+    it used (its first A/c reads 0: the harness zeroes the stack under the call)."""
+    a, dt = run("flux1", args)
+    c = container(a)
+    r = a[1 + 4 * c.shape[0]:]
+    n = int(r[0])
+    t = r[2:].reshape(9, n)
+    out = {"container": c, "dt": np.array(r[1]), "a_first": t[6:, 0].copy(), "digest_traj": np.array(digest(t))}
+    return out, dt
+
+
+@case
+def flux1_c1_small():
+    # the near- plus far-field form on the C1 geometry at the first harmonic
+    e = first_harmonic_energy()
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=20001", "plane=XY", "width=0.02,0.02", "nx=21,21",
+                                        "trans=0,0,30", "energy_ev=%.17g" % e]
+    out, dt = _flux1(args)
+    out["energy_ev"] = np.array(e)
+    return args, out, dt
+
+
+@case
+def flux1_epu_offresonance():
+    # EPU (two crossed undulators), off resonance where the trajectory ends weigh more,
+    # on a tilted XZ screen
+    args = ["und1=" + UND_EPU_Y, "und2=" + UND_EPU_X] + BEAM + ["ct=0,3", "npoints=40001", "plane=XZ",
+                                                                "width=0.01,0.02", "nx=9,7", "rot=0.2,0,0",
+                                                                "trans=0,0.002,30", "energy_ev=300"]
+    out, dt = _flux1(args)
+    out["energy_ev"] = np.array(300.0)
+    return args, out, dt
+
+
 @case
 def files_and_averages():
     """The reference's text / binary container files and its averaging of them:

@@ -1,0 +1,116 @@
+"""GPU parity of srs_hip_flux1 (OSCARSSR::CalculateFlux1's Lienard-Wiechert flux with the
+stored A/c), through the C ABI and through sr.calculate_flux1_rectangle, against the
+reference engine's own output (golden fixtures) and the CPU oracle.  FP64: <= 1e-10
+normwise relative (max|d| / max|ref|)."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from conftest import golden
+from gpu_util import normwise, say
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+CASES = ["flux1_c1_small", "flux1_epu_offresonance"]
+
+
+@pytest.fixture(scope="module")
+def abi():
+    from srsoriginal_amd import abi as A
+    assert A.device_count() >= 1, "no gfx950 device visible"
+    return A
+
+
+def _rect(oracle, name):
+    r = cases.rect(cases.args(name))
+    return oracle.rect_surface(r["plane"], r["nx1"], r["nx2"], r["w1"], r["w2"], r["rot"], r["trans"], r["normal"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_flux1_abi_matches_reference(abi, oracle, name):
+    say("flux1 %s: srs_hip_flux1 vs the reference's CalculateFlux1 and the oracle" % name)
+    F, p, t, dt = cases.oracle_setup(oracle, name)
+    g = golden(name)
+    xyz, nrm, x12 = _rect(oracle, name)
+    got = abi.flux1(t, dt, xyz, float(g["energy_ev"]), p.q, p.current)
+    err = normwise(got, g["container"][:, 3])
+    ero = normwise(got, oracle.flux1(t, dt, xyz, float(g["energy_ev"]), p.q, p.current))
+    say("  normwise vs reference %.3e, vs oracle %.3e" % (err, ero))
+    assert err <= TOL_FP64 and ero <= TOL_FP64
+
+
+def test_flux1_full_map_mirror_and_sampled_oracle(abi, oracle):
+    """201 x 201 over 20 mm at 30 m, 20001 points (the C1 flux map's size): the map on
+    the same points with y negated is equal bit for bit (motion in the x-z plane, every
+    operation sign-symmetric; the grid's own +-y rows differ by the rounding of -w/2 + i s,
+    which (1 - n.b)^2 amplifies to ~1e-10), and a strided sample of 61 points matches
+    the oracle."""
+    say("flux1 201x201, NT=20001: exact y mirror (bitwise) + 61 sampled points vs oracle")
+    F, p, t, dt = cases.oracle_setup(oracle, "flux1_c1_small")
+    e = float(golden("flux1_c1_small")["energy_ev"])
+    xyz, nrm, x12 = oracle.rect_surface("XY", 201, 201, 0.02, 0.02, (0, 0, 0), (0, 0, 30), 0)
+    got = abi.flux1(t, dt, xyz, e, p.q, p.current).reshape(201, 201)
+    flip = xyz * np.array([1.0, -1.0, 1.0])
+    assert np.array_equal(abi.flux1(t, dt, flip, e, p.q, p.current).reshape(201, 201), got)
+    idx = np.arange(0, 201 * 201, 671)
+    ref = oracle.flux1(t, dt, xyz[idx], e, p.q, p.current)
+    err = normwise(got.reshape(-1)[idx], ref)
+    say("  sampled normwise %.3e" % err)
+    assert err <= TOL_FP64
+
+
+def _engine(name):
+    from srsoriginal_amd.sr import sr
+    a = cases.args(name)
+    osr = sr()
+    for u in cases.undulators(a):
+        osr.add_bfield_undulator(bfield=u["field"], period=u["period"], nperiods=u["nperiods"], phase=u["phase"])
+    b = cases.beam(a)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=b["energy"], x0=b["x0"], d0=b["d0"],
+                          current=b["current"])
+    c0, c1, n = cases.ct(a)
+    osr.set_ctstartstop(c0, c1)
+    osr.set_npoints_trajectory(n)
+    return osr, a
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_sr_calculate_flux1_matches_reference(name, tmp_path):
+    """The front end: trajectory on the host, the sum on the device, container points
+    (x1, x2, 0); with ofile the "X1 X2 flux" lines in std::scientific and zero values."""
+    say("sr.calculate_flux1_rectangle %s vs the reference container" % name)
+    osr, a = _engine(name)
+    r = cases.rect(a)
+    kw = dict(npoints=[r["nx1"], r["nx2"]], plane=r["plane"], width=[r["w1"], r["w2"]], rotations=list(r["rot"]),
+              translation=list(r["trans"]))
+    g = golden(name)
+    c = g["container"]
+    res = osr.calculate_flux1_rectangle(float(g["energy_ev"]), **kw)
+    pts = np.array([p for p, v in res])
+    vals = np.array([v for p, v in res])
+    assert np.array_equal(pts, c[:, :3])
+    err = normwise(vals, c[:, 3])
+    say("  normwise %.3e" % err)
+    assert err <= TOL_FP64
+    path = os.path.join(str(tmp_path), "flux1.txt")
+    res = osr.calculate_flux1_rectangle(float(g["energy_ev"]), ofile=path, **kw)
+    assert all(v == 0.0 for p, v in res)
+    rows = np.loadtxt(path)
+    assert rows.shape == (len(c), 3)
+    assert np.allclose(rows[:, :2], c[:, :2], rtol=1e-6, atol=1e-12)
+    assert np.allclose(rows[:, 2], c[:, 3], rtol=1e-5)
+    line = open(path).readline().split()
+    assert all("e" in f for f in line) and len(line[2].split("e")[0]) == 8  # d.dddddd
+
+
+def test_flux1_errors(abi):
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    with pytest.raises(ValueError, match="No particle beam defined"):
+        osr.calculate_flux1_rectangle(100, npoints=[3, 3], plane="XY", width=[0.01, 0.01])
+    t = np.zeros((6, 8))
+    with pytest.raises(Exception, match="acceleration"):
+        abi.flux1(t, 1e-12, [[0, 0, 30]], 100.0, -1.602176462e-19, 0.5)
