@@ -87,6 +87,28 @@ class sr:
         self._spectrum = None
         self._flux = None
         self._power_density = None
+        self._numpy = False
+
+    def set_output_format(self, fmt="list"):
+        """Extension: 'list' (default) returns the reference's nested lists, built point by
+        point as OSCARSSR_GetSpectrumAsList / OSCARSSR_GetT3DScalarAsList do
+        (OSCARSSR_Python.cc:2046-2106); 'numpy' returns float64 arrays instead:
+        (points [N, 3], values [N]) for flux and power-density maps, (energies [N],
+        flux [N]) for spectra and (t [N], E [N, 3]) for calculate_efield_vs_time.  The
+        add_to_* accumulators take either form."""
+        if fmt not in ("list", "numpy"):
+            raise ValueError("'fmt' must be 'list' or 'numpy'")
+        self._numpy = fmt == "numpy"
+
+    def _spec_out(self, e, f):
+        if self._numpy:
+            return np.array(e, dtype=np.float64), np.array(f, dtype=np.float64)
+        return [[a, b] for a, b in zip(e.tolist(), f.tolist())]
+
+    def _map_out(self, xyz, v):
+        if self._numpy:
+            return np.array(xyz, dtype=np.float64).reshape(-1, 3), np.array(v, dtype=np.float64)
+        return _container_list(xyz, v)
 
     # ---- constants and globals (OSCARSSR_Python.cc:140-270) -------------------
     def pi(self):
@@ -329,7 +351,7 @@ class sr:
             containers.write_spectrum_text(ofile, e, f)
         if bofile:
             containers.write_spectrum_binary(bofile, e, f)
-        return [[a, b] for a, b in zip(e.tolist(), f.tolist())]
+        return self._spec_out(e, f)
 
     # ---- flux (OSCARSSR_Python.cc:2829-2996, :3009-3233) --------------------------
     def calculate_flux_rectangle(self, energy_eV, npoints, plane="", normal=0, dim=2, width=None, rotations=None,
@@ -409,6 +431,8 @@ class sr:
         E, t = self._e.calculate_efield_vs_time(o)
         if ofile:
             containers.write_scalar_text(ofile, E, t, 3)
+        if self._numpy:
+            return np.array(t, dtype=np.float64), np.array(E, dtype=np.float64).reshape(-1, 3)
         return [[float(a), [float(x), float(y), float(z)]] for a, (x, y, z) in zip(t.tolist(), E.tolist())]
 
     def calculate_total_power(self, gpu=0):
@@ -443,7 +467,7 @@ class sr:
             containers.write_spectrum_text(ofile, e, f)
         if bofile:
             containers.write_spectrum_binary(bofile, e, f)
-        return [[a, b] for a, b in zip(e.tolist(), f.tolist())]
+        return self._spec_out(e, f)
 
     def average_flux(self, ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
         """OSCARSSR_AverageT3DScalars: average of flux (or power-density) files, text or
@@ -454,7 +478,7 @@ class sr:
             containers.write_scalar_text(ofile, xyz, v, int(dim))
         if bofile:
             containers.write_scalar_binary(bofile, xyz, v, int(dim))
-        return _container_list(xyz, v)
+        return self._map_out(xyz, v)
 
     def average_power_density(self, ifiles=None, bifiles=None, ofile="", bofile="", dim=2):
         """OSCARSSR_AverageT3DScalars, as average_flux."""
@@ -462,9 +486,14 @@ class sr:
 
     # ---- engine-level accumulators (OSCARSSR_Python.cc:3330-3470) ----------------
     def add_to_spectrum(self, spectrum, weight=1.0):
-        s = np.array([p[1] for p in spectrum], dtype=np.float64) * weight
+        if isinstance(spectrum, tuple):  # (energies, flux) arrays of the numpy output format
+            en = np.array(spectrum[0], dtype=np.float64)
+            s = np.array(spectrum[1], dtype=np.float64) * weight
+        else:
+            en = np.array([p[0] for p in spectrum], dtype=np.float64)
+            s = np.array([p[1] for p in spectrum], dtype=np.float64) * weight
         if self._spectrum is None:
-            self._spectrum = [np.array([p[0] for p in spectrum], dtype=np.float64), s, np.zeros_like(s)]
+            self._spectrum = [en, s, np.zeros_like(s)]
         elif len(s) == len(self._spectrum[1]):
             _kahan(self._spectrum[1], self._spectrum[2], s)
         else:
@@ -472,19 +501,21 @@ class sr:
 
     def get_spectrum(self):
         e, f, _ = self._spectrum if self._spectrum is not None else (np.zeros(0), np.zeros(0), None)
-        return [[a, b] for a, b in zip(e.tolist(), f.tolist())]
+        return self._spec_out(e, f)
 
     def add_to_flux(self, flux, weight=1.0):
         self._flux = _add_container(self._flux, flux, weight)
 
     def get_flux(self):
-        return _container_list(*self._flux[:2]) if self._flux is not None else []
+        return self._map_out(*self._flux[:2]) if self._flux is not None else self._map_out(np.zeros((0, 3)), np.zeros(0))
 
     def add_to_power_density(self, power_density, weight=1.0):
         self._power_density = _add_container(self._power_density, power_density, weight)
 
     def get_power_density(self):
-        return _container_list(*self._power_density[:2]) if self._power_density is not None else []
+        if self._power_density is None:
+            return self._map_out(np.zeros((0, 3)), np.zeros(0))
+        return self._map_out(*self._power_density[:2])
 
     # ---- helpers -------------------------------------------------------------------
     def _check_common(self, nparticles, gpu, nthreads):
@@ -542,7 +573,7 @@ class sr:
     def _out(self, xyz, v, ofile, dim):
         if ofile:
             containers.write_scalar_text(ofile, xyz, v, dim)
-        return _container_list(xyz, v)
+        return self._map_out(xyz, v)
 
 
 def _kahan(s, c, v):
@@ -554,8 +585,12 @@ def _kahan(s, c, v):
 
 
 def _add_container(state, data, weight):
-    xyz = np.array([p[0] for p in data], dtype=np.float64).reshape(-1, 3)
-    v = np.array([p[1] for p in data], dtype=np.float64) * weight
+    if isinstance(data, tuple):  # (points, values) arrays of the numpy output format
+        xyz = np.array(data[0], dtype=np.float64).reshape(-1, 3)
+        v = np.array(data[1], dtype=np.float64) * weight
+    else:
+        xyz = np.array([p[0] for p in data], dtype=np.float64).reshape(-1, 3)
+        v = np.array([p[1] for p in data], dtype=np.float64) * weight
     if state is None:
         return [xyz, v, np.zeros_like(v)]
     if len(v) != len(state[1]):

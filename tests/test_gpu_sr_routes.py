@@ -294,3 +294,26 @@ def test_total_power_on_the_device_matches_reference():
     rc = abi.lib().srs_hip_total_power(C.byref(t), C.c_double(-1.6e-19), C.c_double(0.5), C.c_double(5871.0),
                                        C.byref(out))
     assert rc != 0
+
+
+def test_numpy_output_format_equals_lists():
+    """set_output_format('numpy') returns the same values as the reference's nested lists
+    for the flux map, the power-density map and the spectrum."""
+    from srsoriginal_amd.sr import sr
+    say("sr numpy output format vs lists: flux, power density, spectrum")
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+    osr.set_particle_beam(type="electron", name="b", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(4001)
+    kw = dict(npoints=[21, 11], plane="XY", width=[0.01, 0.005], translation=[0, 0, 30])
+    calls = [lambda: osr.calculate_flux_rectangle(energy_eV=143.52, gpu=1, **kw),
+             lambda: osr.calculate_power_density_rectangle(gpu=1, **kw)]
+    lists = [c() for c in calls]
+    sl = osr.calculate_spectrum(obs=[0, 0, 30], energy_range_eV=[100, 500], npoints=50, gpu=1)
+    osr.set_output_format("numpy")
+    for c, lst in zip(calls, lists):
+        P, V = c()
+        assert np.array_equal(P, np.array([p for p, _ in lst])) and np.array_equal(V, np.array([v for _, v in lst]))
+    E, F = osr.calculate_spectrum(obs=[0, 0, 30], energy_range_eV=[100, 500], npoints=50, gpu=1)
+    assert np.array_equal(E, np.array([a for a, _ in sl])) and np.array_equal(F, np.array([b for _, b in sl]))
