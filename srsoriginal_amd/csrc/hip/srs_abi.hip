@@ -813,7 +813,6 @@ int srs_hip_flux1(const srs_traj* traj, const double* obs_xyz, int64_t ns, doubl
   TRY(check_traj(traj, true));
   TRY(check_common(ns, SRS_PRECISION_FP64));
   if (!obs_xyz || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
-  if (!(energy_ev > 0)) return fail(SRS_ERR_INVALID, "photon energy must be > 0");
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
   TRY(init(c));

@@ -67,6 +67,38 @@ SRS_FN void sincos_fp64(double x, double* s_out, double* c_out) {
   *c_out = ((q + 1) & 2) ? -c1 : c1;
 }
 
+// The same sine and cosine with the polynomials in Estrin's form (powers r^2, r^4, r^8
+// shared by both): 4 dependent levels after r^2 instead of Horner's 8, for callers whose
+// time is one long dependency chain (the RK4 field evaluation); rounding differs from
+// sincos_fp64 by about an ulp.
+SRS_FN void sincos_fp64_short(double x, double* s_out, double* c_out) {
+  double const k = __builtin_rint(x * kTwoOverPi);
+  double r = __builtin_fma(-k, kPio2Hi, x);
+  r = __builtin_fma(-k, kPio2Lo, r);
+  double const kb = k + 0x1.8p52;
+  uint64_t bits;
+  __builtin_memcpy(&bits, &kb, sizeof bits);
+  int const q = (int)(bits & 3u);
+  double const r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+  // P(r^2) = p0 + p1 r^2 + ... + p7 r^14 (the Taylor terms of sincos_fp64)
+  double const s01 = __builtin_fma(-0.008333333333333333, r2, 0.16666666666666666);
+  double const s23 = __builtin_fma(-2.7557319223985893e-06, r2, 0.0001984126984126984);
+  double const s45 = __builtin_fma(-1.6059043836821613e-10, r2, 2.505210838544172e-08);
+  double const s67 = __builtin_fma(-2.8114572543455206e-15, r2, 7.647163731819816e-13);
+  double const ps = __builtin_fma(r8, __builtin_fma(r4, s67, s45), __builtin_fma(r4, s23, s01));
+  double const sn = __builtin_fma(-(r * r2), ps, r);
+  double const c01 = __builtin_fma(0.041666666666666664, r2, -0.5);
+  double const c23 = __builtin_fma(2.48015873015873e-05, r2, -0.001388888888888889);
+  double const c45 = __builtin_fma(2.08767569878681e-09, r2, -2.755731922398589e-07);
+  double const c67 = __builtin_fma(4.779477332387385e-14, r2, -1.1470745597729725e-11);
+  double const pc = __builtin_fma(r8, __builtin_fma(r4, c67, c45), __builtin_fma(r4, c23, c01));
+  double const cs = __builtin_fma(r2, pc, 1.0);
+  double const s1 = (q & 1) ? cs : sn;
+  double const c1 = (q & 1) ? sn : cs;
+  *s_out = (q & 2) ? -s1 : s1;
+  *c_out = ((q + 1) & 2) ? -c1 : c1;
+}
+
 // Correctly rounded quotient a/b from the correctly rounded reciprocal y = RN(1/b):
 // q = RN(a*y), r = a - b*q (exact with an FMA), RN(q + r*y) = RN(a/b) (Markstein's
 // correction step).  Three operations instead of a full IEEE division; the

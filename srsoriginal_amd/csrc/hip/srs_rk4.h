@@ -136,7 +136,7 @@ __device__ __forceinline__ void field_add_und(DevUndulator const& f, const doubl
   double const T = 1 + D * f.taper;
   bool const inside = !(D > f.hi0 || D < f.lo0);
   double s, c;
-  sincos_fp64(div_rn(2.0 * kPi * (D - f.ps), f.lam, f.rlam), &s, &c);
+  sincos_fp64_short(div_rn(2.0 * kPi * (D - f.ps), f.lam, f.rlam), &s, &c);
   bool const term = D < f.lo1 || D > f.hi1;
   double const fct = (D < f.lo2 || D > f.hi2) ? 0.25 : 0.75;
   double g = T;
