@@ -18,7 +18,8 @@
 //                          + integral vs total power
 //   oracle_total_power ... bitwise vs reference (C0, C2)
 //   oracle_flux1 ......... bitwise vs the reference's CalculateFlux1 on the trajectory it
-//                          used; far-field limit vs oracle_flux (integration by parts)
+//                          used; converges to the integration-by-parts identity (integral
+//                          + endpoint terms) as the step shrinks
 //   random particles ..... bitwise vs reference (seeded mt19937/normal stream)
 //   oracle_pole_loss_gradient  dual-number forward mode; vs the reference's own central
 //                          differences (h = 1e-6) of the pole-error flux loss
