@@ -296,10 +296,13 @@ PYBIND11_MODULE(_sr, m) {
            py::arg("gpu"), py::arg("precision"), py::arg("batched") = false)
       .def("calculate_flux1",
            [](srs::OSCARSSR& e, srs::SurfacePoints const& S, double energy, std::string const& ofile) {
-             // the container's 2-D points, as the flux routes add them for dim = 2
+             // container points: (x1, x2, 0) of a rectangle, as the flux routes add them for
+             // dim = 2; the points themselves for a point list (whose x1 = x2 = 0)
              srs::ScalarContainer C;
              for (size_t i = 0; i < S.size(); ++i)
-               C.add_point(S.has_x12 ? srs::Vec3(S.x1[i], S.x2[i], 0) : srs::Vec3(0, 0, 0), 0);
+               C.add_point(S.has_x12 ? srs::Vec3(S.x1[i], S.x2[i], 0)
+                                     : srs::Vec3(S.xyz[3 * i], S.xyz[3 * i + 1], S.xyz[3 * i + 2]),
+                           0);
              e.CalculateFlux1(S, energy, C, ofile);
              return container_arrays(C);
            },

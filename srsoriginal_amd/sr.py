@@ -394,6 +394,16 @@ class sr:
         xyz, v = self._e.calculate_flux1(surf, float(energy_eV), str(ofile))
         return self._out(xyz, v, "", 2)
 
+    def calculate_flux1(self, energy_eV, points, normal=0, rotations=None, translation=None, ofile=""):
+        """calculate_flux1_rectangle on a list of points ([[x, y, z], [nx, ny, nz]] as
+        calculate_flux takes them); returns [[[x, y, z], flux], ...]; ofile lines carry
+        X1 = X2 = 0, as TSurfacePoints_3D::GetX1/GetX2 return for point lists."""
+        if self._e.n_beams() < 1:
+            raise ValueError("No particle beam defined")
+        surf = self._points(points, rotations, translation, normal, flip=False)
+        xyz, v = self._e.calculate_flux1(surf, float(energy_eV), str(ofile))
+        return self._map_out(xyz, v)
+
     # ---- power density (OSCARSSR_Python.cc:2403-2577, :2587-2807) ----------------
     def calculate_power_density_rectangle(self, npoints, plane="", width=None, x0x1x2=None, rotations=None,
                                           translation=None, ofile="", normal=0, nparticles=0, gpu=0, nthreads=0,
