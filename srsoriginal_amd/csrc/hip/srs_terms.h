@@ -31,7 +31,9 @@ struct AccFP64 {
 };
 
 // Plain FP64 form: phase chain in the reference's operation order (bitwise),
-// N = R/D correctly rounded (beta_z - n_z cancels to ~1e-8).
+// n_z = R_z/D correctly rounded (beta_z - n_z cancels to ~1e-8, so its rounding is
+// bit-relevant); n_x, n_y by the reciprocal (beta_x - n_x, beta_y - n_y are ~K/gamma,
+// no cancellation: an ulp of n_x is 1e-16 of a typical term).
 template <int NE, int TILE>
 __device__ __forceinline__ void flux_terms_fp64(FluxTile<TILE> const& T, int cnt, int64_t base, double dt, double ox,
                                                 double oy, double oz, EnergySet const& es, AccFP64<NE>& a) {
@@ -41,7 +43,7 @@ __device__ __forceinline__ void flux_terms_fp64(FluxTile<TILE> const& T, int cnt
     double const D = __builtin_sqrt(d2);
     double const tau = dt * (double)(base + k) + div_rn(D, kC, kInvC);
     double const inv = 1.0 / D;
-    double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
+    double const nx = rx * inv, ny = ry * inv, nz = div_rn(rz, D, inv);
     double const ax = (T.bx[k] - nx) * inv, ay = (T.by[k] - ny) * inv, az = (T.bz[k] - nz) * inv;
     double const inv2 = inv * inv;
 #pragma unroll
