@@ -20,7 +20,10 @@ __device__ __forceinline__ double pd_term(double ox, double oy, double oz, doubl
   double const d2 = rx * rx + ry * ry + rz * rz;
   double const D = __builtin_sqrt(d2);
   double const inv = 1.0 / D;
-  double const n1x = div_rn(rx, D, inv), n1y = div_rn(ry, D, inv), n1z = div_rn(rz, D, inv);
+  // n1z correctly rounded (1 - beta.N1 and n1z - bz cancel to ~1e-8 through it); n1x,
+  // n1y by the reciprocal: beta_x n1x is ~1e-7 of 1, so their last bit is ~1e-15 of
+  // 1 - beta.N1
+  double const n1x = rx * inv, n1y = ry * inv, n1z = div_rn(rz, D, inv);
   double const ndot = n1x * qx + n1y * qy + n1z * qz;
   if (directional && ndot <= 0) return 0.0;
   double const den = 1 - (bx * n1x + by * n1y + bz * n1z);
@@ -50,7 +53,7 @@ __device__ __forceinline__ void pd_term_adjoint(double G, double ox, double oy, 
   double const inv = 1.0 / D;
   // N and b rounded exactly as in pd_term: b = 1 - beta.N cancels to ~1e-8, and its
   // rounding enters the derivative through 1/b^6
-  double const N[3] = {div_rn(r[0], D, inv), div_rn(r[1], D, inv), div_rn(r[2], D, inv)};
+  double const N[3] = {r[0] * inv, r[1] * inv, div_rn(r[2], D, inv)};
   double const nd = N[0] * qx + N[1] * qy + N[2] * qz;
   if (directional && nd <= 0) return;
   double const b = 1 - (B[0] * N[0] + B[1] * N[1] + B[2] * N[2]);
