@@ -189,9 +189,9 @@ __global__ __launch_bounds__(kBlock) void efield_time_kernel(TrajView t, double 
 //   E_k = [ (1 - |b|^2)(n - b) / (D D (1 - n.b)^2) + n x ((n - b) x A/c) / (D (1 - n.b)^2) ]
 //         e^{-i Omega (k dt + D/c)},
 // one lane per observation point, the trajectory staged in LDS tiles.  1 - n.b and
-// 1 - |b|^2 lose ~8 digits to cancellation at GeV energies, so n = R/|R| (IEEE
-// quotients), the dot products and the phase keep the reference's TVector3D operation
-// order; the well-conditioned amplitude quotients use reciprocals.
+// 1 - |b|^2 lose ~8 digits to cancellation at GeV energies, so n_z = R_z/|R| (IEEE
+// quotient), the dot products and the phase keep the reference's TVector3D operation
+// order; n_x, n_y and the well-conditioned amplitude quotients use reciprocals.
 // part[(c * 6 + 2 q + {0: re, 1: im}) * ns + j]: the chunk's sum for axis q (times dt
 // later, with C0, in flux_reduce_kernel).
 __global__ __launch_bounds__(kBlock) void flux1_fp64_kernel(TrajView t, const double* __restrict__ obs, int ns,
@@ -225,7 +225,9 @@ __global__ __launch_bounds__(kBlock) void flux1_fp64_kernel(TrajView t, const do
       double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
       double const D = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
       double const inv = 1.0 / D;
-      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
+      // nz correctly rounded (1 - n.b and nz - bz cancel through it); nx, ny by the
+      // reciprocal (their last bit is ~1e-15 of 1 - n.b)
+      double const nx = rx * inv, ny = ry * inv, nz = div_rn(rz, D, inv);
       double const omb = 1.0 - (nx * bx + ny * by + nz * bz);
       double const sb = 1.0 - (bx * bx + by * by + bz * bz);
       double const ux = nx - bx, uy = ny - by, uz = nz - bz;
