@@ -240,8 +240,8 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
   int const nch = nchunks(t.n, chunk);
   HIPCHK(c.part.ensure((size_t)nch * maxe * 6 * ns * sizeof(double)));
   if (precision == SRS_PRECISION_MIXED) {
-    HIPCHK(c.trajf.ensure((size_t)6 * t.n * sizeof(float)));
-    HIPCHK(srs::launch_traj_to_f32(t, c.trajf.as<float>(), s));
+    HIPCHK(c.trajf.ensure((size_t)3 * t.n * sizeof(float)));
+    HIPCHK(srs::launch_traj_to_f32(t, 0, c.trajf.as<float>(), s));
   }
   for (int64_t e0 = 0; e0 < ne; e0 += maxe) {
     EnergySet es{};
@@ -338,7 +338,7 @@ int run_pd(Context& c, const TrajView& t, const double* obs_soa, const double* n
   HIPCHK(c.part.ensure((size_t)nch * ns * sizeof(double)));
   if (precision == SRS_PRECISION_MIXED) {
     HIPCHK(c.trajf.ensure((size_t)7 * t.n * sizeof(float)));
-    HIPCHK(srs::launch_traj_to_f32(t, c.trajf.as<float>(), s));
+    HIPCHK(srs::launch_traj_to_f32(t, 1, c.trajf.as<float>(), s));
     KernelTimer timer(c, s);
     HIPCHK(srs::launch_pd_mixed(t, c.trajf.as<float>(), obs_soa, nrm_soa, (int)ns, directional, chunk,
                                 c.part.as<double>(), s));
@@ -594,13 +594,16 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
 }
 
 std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context const& c) {
-  char buf[512];
-  // every value a captured launch holds: shapes, buffers, the field list's length and
-  // whether it needs the general field evaluation (the RK4 instantiation depends on it)
-  snprintf(buf, sizeof buf, "%d/%d/%d/%d/%d/%d/%d/%d/%p/%p/%p/%p/%p/%p/%p/%p/%p/%lld/%.17g", r.np, r.ns, r.ne,
-           (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p,
-           c.b_out.p, c.b_obs.p, c.b_energies.p, c.b_state.p, c.b_field.p, c.b_poles.p, (long long)r.max_leg,
-           r.weight);
+  char buf[768];
+  // every value a captured launch holds by value: shapes, tile shape, the field list's
+  // length and whether it needs the general field evaluation (the RK4 instantiation
+  // depends on it), the CU count behind the persistent grid size, the flux prefactor c2
+  // and the weight, and the address of every buffer a launch reads or writes
+  snprintf(buf, sizeof buf,
+           "%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns, r.ne,
+           (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local, c.cus, (long long)r.max_leg,
+           r.weight, r.be.c2, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
+           c.b_state.p, c.b_field.p, c.b_poles.p, c.b_part.p, c.b_counter.p, c.b_und.p);
   return buf;
 }
 
@@ -1169,8 +1172,7 @@ int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, in
   c.b_last = cfg;
 
   if (flags & SRS_BATCH_GRAPH) {
-    std::string const key = batch_key(r, cfg, c) + "/" + std::to_string(c.b_und.p ? (long long)(intptr_t)c.b_und.p : 0) +
-                            "/" + std::to_string(nund);
+    std::string const key = batch_key(r, cfg, c);
     if (key != c.b_key || !c.b_exec) {
       if (c.b_exec) (void)hipGraphExecDestroy(c.b_exec);
       if (c.b_graph) (void)hipGraphDestroy(c.b_graph);

@@ -61,7 +61,8 @@ hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* 
                            int directional, int chunk, double* part, hipStream_t s);
 // Single-precision copies used by the mixed kernels: trajf[k*n + i] for
 // k = 0..5: bx, by, (1 - bz), ax, ay, az (the last three only when t.ax..az are set).
-hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s);
+// writes 3 n floats (bx, by, 1 - bz), or 7 n when with_a (then t.ax..az must be set)
+hipError_t launch_traj_to_f32(const TrajView& t, int with_a, float* trajf, hipStream_t s);
 
 // ---- reductions over trajectory chunks (fixed order, deterministic) --------
 hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,

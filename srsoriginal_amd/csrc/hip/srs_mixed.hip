@@ -24,8 +24,9 @@ constexpr int SUB = kSub;
 
 
 // trajf layout (float): [0] bx, [1] by, [2] 1 - bz, and with_a: [3] ax, [4] ay, [5] az,
-// [6] 1 - |beta|^2 (from FP64: (1 - bz)(1 + bz) - bx^2 - by^2)
-// (the flux and spectrum paths stage no accelerations: t.ax..az are NULL there).
+// [6] 1 - |beta|^2 (from FP64: (1 - bz)(1 + bz) - bx^2 - by^2).  The caller states
+// with_a and sizes trajf to match: 3 n floats without, 7 n with (the flux path needs no
+// accelerations even when its trajectory view carries them).
 __global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f) {
   int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= t.n) return;
@@ -318,8 +319,8 @@ int pd_mixed_blocks_per_cu(int directional) {
                      : blocks_per_cu(pd_mixed_kernel<false>, kBlock, c[0]);
 }
 
-hipError_t launch_traj_to_f32(const TrajView& t, float* trajf, hipStream_t s) {
-  int const with_a = (t.ax && t.ay && t.az) ? 1 : 0;
+hipError_t launch_traj_to_f32(const TrajView& t, int with_a, float* trajf, hipStream_t s) {
+  if (with_a && !(t.ax && t.ay && t.az)) return hipErrorInvalidValue;
   traj_to_f32_kernel<<<(unsigned)((t.n + 255) / 256), 256, 0, s>>>(t, with_a, trajf);
   return hipGetLastError();
 }

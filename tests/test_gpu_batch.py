@@ -175,6 +175,33 @@ def test_batch_graph_follows_a_changed_field_list(abi, oracle):
         assert np.array_equal(graph, eager)
 
 
+def test_batch_graph_follows_current_charge_and_particle_count(abi, oracle):
+    """The captured graph holds the flux prefactor c2 (from charge and current) and the
+    particle buffer's address by value: replays with another current, another charge, or
+    enough particles to move that buffer must each equal the eager run of their own
+    inputs, never an earlier capture."""
+    name = "multiflux_c3_small"
+    say("batched flux graph: current 0.5 -> 0.25 A, charge x2, buffers regrown by a 700-particle call, each vs eager")
+    a = cases.args(name)
+    rows, p = _c3_particles(oracle, name, 4, npoints=2001)
+    xyz, nrm, x12 = oracle.rect_surface("XY", 9, 9, 0.01, 0.008, (0, 0, 0), (0, 0, 30), 0)
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    pp = abi.particles(rows)
+    w = 1.0 / len(rows)
+    prev = None
+    for q, cur, regrow in ((p.q, 0.5, False), (p.q, 0.25, False), (2 * p.q, 0.25, False), (2 * p.q, 0.25, True)):
+        if regrow:  # an eager call with many particles reallocates the batch buffers under the cached graph
+            many = abi.particles((rows * 175)[:700])
+            abi.batch_flux(und, many, q, cur, xyz, [431.2], 1.0 / 700, abi.FP64, 0)
+        graph = abi.batch_flux(und, pp, q, cur, xyz, [431.2], w, abi.FP64, abi.BATCH_GRAPH)
+        eager = abi.batch_flux(und, pp, q, cur, xyz, [431.2], w, abi.FP64, 0)
+        say("  q=%+.3e I=%.2f regrown=%d: max %.6e" % (q, cur, regrow, float(np.max(eager))))
+        assert np.array_equal(graph, eager)
+        if prev is not None and not regrow:
+            assert not np.array_equal(eager, prev)  # c2 changed, so must the flux
+        prev = eager
+
+
 def test_batch_power_density_mixed_within_1e5():
     """Mixed batched power density (pd_batch_kernel's FP32 terms) against the FP64 batch,
     16 ragged particles, one segmented, directional and not, normal and tilted screen."""
