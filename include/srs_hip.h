@@ -158,7 +158,11 @@ typedef struct srs_field {
 
 /* One particle: initial position, beta, q/(m gamma), T0 [m], its ct range and point
  * count (set_ctstartstop / set_npoints_trajectory; dt and the forward/backward point
- * counts follow OSCARSSR.cc:592-598). */
+ * counts follow OSCARSSR.cc:592-598), and its charge [C] and beam current [A] as
+ * TParticleA holds them (TParticleA.cc:254-266).  The batch entry points form each
+ * particle's prefactors from its own charge and current, as the reference's
+ * per-particle loop does (OSCARSSR.cc:2465-2479); charge 0 means the call's charge
+ * and current. */
 typedef struct srs_particle {
   double x0[3];
   double beta0[3];
@@ -167,14 +171,17 @@ typedef struct srs_particle {
   double ct_start;
   double ct_stop;
   int64_t npoints;
+  double charge;
+  double current;
 } srs_particle;
 
 /* A Gaussian beam for the device sampler (TParticleBeam, TParticleBeam.cc:137-265):
  * energy e0 [GeV] with rms sigma_e [GeV], the particle's mass [kg] and charge [C],
  * start x0 [m], unit direction u0, unit horizontal / vertical directions, the lattice
  * reference point sigma_at, rms positions sigma_u [m] and angles sigma_up [rad]
- * (horizontal, vertical), t0, and the ct range and point count every sampled particle's
- * trajectory gets. */
+ * (horizontal, vertical), t0, the ct range and point count every sampled particle's
+ * trajectory gets, and the beam current [A] (0: the charge, as TParticleA.cc:258-262
+ * sets it).  Sampled particles carry the beam's charge and current. */
 typedef struct srs_beam {
   double e0, sigma_e;
   double mass, charge;
@@ -182,6 +189,7 @@ typedef struct srs_beam {
   double sigma_u[2], sigma_up[2];
   double t0, ct_start, ct_stop;
   int64_t npoints;
+  double current;
 } srs_beam;
 
 /* np particles of the beam drawn on the device, particle i from counter first + i
@@ -214,7 +222,8 @@ int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, 
 #define SRS_BATCH_AUTOTUNE 2 /* time the candidate tile shapes on this problem once; keep the fastest */
 
 /* Multi-particle flux: flux_out[e*ns + j] = sum over particles (in order, Kahan) of
- * weight * flux_p(e, j).  All particles share charge and current. */
+ * weight * flux_p(e, j), flux_p with the particle's own charge and current (the call's
+ * charge and current for particles with charge 0). */
 int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,
                        double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
                        double weight, int precision, int flags, double* flux_out);

@@ -763,6 +763,36 @@ void oracle_energy_grid(int64_t n, double e0, double e1, double* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = e0 + (e1 - e0) / (size_t)(n - 1) * (double)(i + 1);
 }
 
+}  // extern "C"
+// Points, normal and frame coordinates of a rectangle from its start corner, step
+// vectors and step sizes (TSurfacePoints_Rectangle::GetXYZ :222-230: start + i1 a1 + i2 a2;
+// normal a1 x a2 / |a1 x a2|, reversed for Normal = -1; GetX1/X2 :244-259).
+static int fill_rect(V3 st, V3 a1, V3 a2, double s1, double s2, int nx1, int nx2, int normal, double* xyz,
+                     double* nrm, double* x1x2) {
+  V3 nv = unit(cross(a1, a2));
+  if (normal == -1) nv = muls(nv, -1);
+  else if (normal != 0 && normal != 1) return -2;
+  int64_t const np = (int64_t)nx1 * nx2;
+  for (int64_t i = 0; i < np; ++i) {
+    int const i1 = (int)(i / nx2), i2 = (int)(i % nx2);
+    V3 p = st;
+    V3 t1 = muls(a1, (double)i1);
+    p = V3{p.x + t1.x, p.y + t1.y, p.z + t1.z};
+    V3 t2 = muls(a2, (double)i2);
+    p = V3{p.x + t2.x, p.y + t2.y, p.z + t2.z};
+    xyz[3 * i] = p.x;
+    xyz[3 * i + 1] = p.y;
+    xyz[3 * i + 2] = p.z;
+    nrm[3 * i] = nv.x;
+    nrm[3 * i + 1] = nv.y;
+    nrm[3 * i + 2] = nv.z;
+    x1x2[2 * i] = s1 * i1 - std::sqrt(mag2(a1)) * (nx1 - 1) / 2.;
+    x1x2[2 * i + 1] = s2 * i2 - std::sqrt(mag2(a2)) * (nx2 - 1) / 2.;
+  }
+  return 0;
+}
+extern "C" {
+
 int oracle_rect_surface(const char* plane, int nx1, int nx2, double w1, double w2, const double rot[3],
                         const double trans[3], int normal, double* xyz, double* nrm, double* x1x2) {
   // TSurfacePoints_Rectangle::Init(plane,...) :130-210, GetXYZ :222-230, GetX1/X2 :244-259
@@ -792,27 +822,16 @@ int oracle_rect_surface(const char* plane, int nx1, int nx2, double w1, double w
   st.x += trans[0];
   st.y += trans[1];
   st.z += trans[2];
-  V3 nv = unit(cross(a1, a2));
-  if (normal == -1) nv = muls(nv, -1);
-  else if (normal != 0 && normal != 1) return -2;
-  int64_t const np = (int64_t)nx1 * nx2;
-  for (int64_t i = 0; i < np; ++i) {
-    int const i1 = (int)(i / nx2), i2 = (int)(i % nx2);
-    V3 p = st;
-    V3 t1 = muls(a1, (double)i1);
-    p = V3{p.x + t1.x, p.y + t1.y, p.z + t1.z};
-    V3 t2 = muls(a2, (double)i2);
-    p = V3{p.x + t2.x, p.y + t2.y, p.z + t2.z};
-    xyz[3 * i] = p.x;
-    xyz[3 * i + 1] = p.y;
-    xyz[3 * i + 2] = p.z;
-    nrm[3 * i] = nv.x;
-    nrm[3 * i + 1] = nv.y;
-    nrm[3 * i + 2] = nv.z;
-    x1x2[2 * i] = s1 * i1 - std::sqrt(mag2(a1)) * (nx1 - 1) / 2.;
-    x1x2[2 * i + 1] = s2 * i2 - std::sqrt(mag2(a2)) * (nx2 - 1) / 2.;
-  }
-  return 0;
+  return fill_rect(st, a1, a2, s1, s2, nx1, nx2, normal, xyz, nrm, x1x2);
+}
+
+int oracle_rect3_surface(int nx1, int nx2, const double x0[3], const double x1[3], const double x2[3], int normal,
+                         double* xyz, double* nrm, double* x1x2) {
+  // TSurfacePoints_Rectangle::Init(NX1, NX2, X0, X1, X2, Normal) :76-123 (corners as given,
+  // already rotated and translated by the caller), GetXYZ :222-230, GetX1/X2 :244-259
+  V3 const c0 = v3(x0), d1 = sub(v3(x1), c0), d2 = sub(v3(x2), c0);
+  double const s1 = std::sqrt(mag2(d1)) / (nx1 - 1), s2 = std::sqrt(mag2(d2)) / (nx2 - 1);
+  return fill_rect(c0, divs(d1, nx1 - 1), divs(d2, nx2 - 1), s1, s2, nx1, nx2, normal, xyz, nrm, x1x2);
 }
 
 void oracle_spectrum(const double* traj, int64_t n, double dt, const double obs[3], const double* energies_ev,

@@ -96,6 +96,10 @@ void oracle_energy_grid(int64_t n, double e0, double e1, double* out);
 /* Rectangle surface (TSurfacePoints_Rectangle::Init(plane, ...)).  Returns 0 or -1 on bad plane. */
 int oracle_rect_surface(const char* plane, int nx1, int nx2, double w1, double w2, const double rot[3],
                         const double trans[3], int normal, double* xyz, double* nrm, double* x1x2);
+/* Rectangle from three corners (TSurfacePoints_Rectangle::Init(NX1, NX2, X0, X1, X2, Normal)).
+ * Returns 0, or -2 for a normal other than -1, 0, 1. */
+int oracle_rect3_surface(int nx1, int nx2, const double x0[3], const double x1[3], const double x2[3], int normal,
+                         double* xyz, double* nrm, double* x1x2);
 
 /* Single-particle integrals; out[i] = value * weight (no container semantics). */
 void oracle_spectrum(const double* traj, int64_t n, double dt, const double obs[3], const double* energies_ev,

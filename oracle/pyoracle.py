@@ -43,6 +43,7 @@ def lib():
         L.oracle_rng_normal.restype = D
         L.oracle_rng_uniform.restype = D
         L.oracle_rect_surface.restype = C.c_int
+        L.oracle_rect3_surface.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -93,6 +94,11 @@ def rng_seed(seed, reset_normal=True):
 
 def rng_normal():
     return lib().oracle_rng_normal()
+
+
+def rng_uniform():
+    """TRandomA::Uniform from the oracle's global generator."""
+    return lib().oracle_rng_uniform()
 
 
 def random_electron(energy_gev, sigma_e_gev, x0, d0, hdir, beta, emit, lattice_ref, current):
@@ -162,6 +168,17 @@ def rect_surface(plane, nx1, nx2, w1, w2, rot=(0, 0, 0), trans=(0, 0, 0), normal
                                    C.c_int(normal), _p(xyz), _p(nrm), _p(x12))
     if rc != 0:
         raise ValueError("bad plane or normal")
+    return xyz, nrm, x12
+
+
+def rect3_surface(nx1, nx2, x0, x1, x2, normal=0):
+    """Rectangle from three corners, already rotated and translated."""
+    n = nx1 * nx2
+    xyz = np.zeros((n, 3)); nrm = np.zeros((n, 3)); x12 = np.zeros((n, 2))
+    rc = lib().oracle_rect3_surface(C.c_int(nx1), C.c_int(nx2), _v3(x0), _v3(x1), _v3(x2), C.c_int(normal), _p(xyz),
+                                    _p(nrm), _p(x12))
+    if rc != 0:
+        raise ValueError("bad normal")
     return xyz, nrm, x12
 
 

@@ -190,7 +190,22 @@ static void setup(OSCARSSR& osr) {
   TVector3D hdir = gArgs.count("hdir") ? v3("hdir", TVector3D(1, 0, 0)) : d0.Orthogonal().UnitVector();
   hdir = hdir.UnitVector();
   osr.GetParticleBeam("beam").SetSigma(hdir, su, sup, v3("latref", TVector3D(0, 0, 0)), num("sigE", 0));
-  osr.SetNewParticle("", "ideal");
+  // A second beam for the weighted pick (TParticleBeamContainer::GetRandomBeamIndexByWeight):
+  // beam2=x0x,x0y,x0z,d0x,d0y,d0z,energy,current weight1=w weight2=w; the same sigma
+  // settings as the first beam (add_particle_beam, OSCARSSR_Python.cc:1661-1870)
+  std::vector<double> b2 = vec("beam2");
+  if (b2.size() == 8) {
+    osr.ClearParticleBeams();
+    osr.AddParticleBeam(str("type", "electron"), "beam", x0, d0, num("energy", 3), num("t0", 0), num("current", 0.5),
+                        num("weight1", 1));
+    osr.GetParticleBeam("beam").SetSigma(hdir, su, sup, v3("latref", TVector3D(0, 0, 0)), num("sigE", 0));
+    TVector3D const d2(b2[3], b2[4], b2[5]);
+    osr.AddParticleBeam(str("type", "electron"), "beam2", TVector3D(b2[0], b2[1], b2[2]), d2, b2[6], num("t0", 0),
+                        b2[7], num("weight2", 1));
+    TVector3D const h2 = gArgs.count("hdir") ? hdir : d2.Orthogonal().UnitVector();
+    osr.GetParticleBeam("beam2").SetSigma(h2, su, sup, v3("latref", TVector3D(0, 0, 0)), num("sigE", 0));
+  }
+  osr.SetNewParticle("beam", "ideal");
   std::vector<double> ct = vec("ct", std::vector<double>{0, 3});
   osr.SetCTStartStop(ct[0], ct[1]);
   if (gArgs.count("npoints")) osr.SetNPointsTrajectory((size_t)num("npoints", 0));
@@ -202,7 +217,41 @@ static TSurfacePoints_Rectangle rect() {
   TSurfacePoints_Rectangle S;
   std::vector<double> n = vec("nx", std::vector<double>{11, 11});
   std::vector<double> w = vec("width", std::vector<double>{0.01, 0.01});
+  std::vector<double> c = vec("x0x1x2");
+  if (c.size() == 9) {
+    // three corners, rotated then translated as calculate_*_rectangle does with x0x1x2
+    // (OSCARSSR_Python.cc:3128-3151), then TSurfacePoints_Rectangle::Init(NX1, NX2, X0, X1, X2, Normal)
+    TVector3D const rot = v3("rot", TVector3D(0, 0, 0)), tr = v3("trans", TVector3D(0, 0, 0));
+    std::vector<TVector3D> X;
+    for (int k = 0; k < 3; ++k) {
+      X.push_back(TVector3D(c[3 * k], c[3 * k + 1], c[3 * k + 2]));
+      X.back().RotateSelfXYZ(rot);
+      X.back() += tr;
+    }
+    S.Init((int)n[0], (int)n[1], X[0], X[1], X[2], (int)num("normal", 0));
+    return S;
+  }
   S.Init(str("plane", "XY"), (int)n[0], (int)n[1], w[0], w[1], v3("rot", TVector3D(0, 0, 0)), v3("trans", TVector3D(0, 0, 30)), (int)num("normal", 0));
+  return S;
+}
+
+// Point list as calculate_flux / calculate_power_density build it (OSCARSSR_Python.cc:
+// 2472-2506, 2911-2940): points=x,y,z,nx,ny,nz,...; each point and normal rotated by
+// prot (RotateSelfXYZ), the normal reversed when flip is set and pnormal == -1, the
+// point translated by ptrans.
+static TSurfacePoints_3D points3d(bool flip) {
+  std::vector<double> p = vec("points");
+  TVector3D const rot = v3("prot", TVector3D(0, 0, 0)), tr = v3("ptrans", TVector3D(0, 0, 0));
+  int const nd = (int)num("pnormal", 0);
+  TSurfacePoints_3D S;
+  for (size_t i = 0; i + 5 < p.size(); i += 6) {
+    TVector3D X(p[i], p[i + 1], p[i + 2]), N(p[i + 3], p[i + 4], p[i + 5]);
+    X.RotateSelfXYZ(rot);
+    N.RotateSelfXYZ(rot);
+    if (flip && nd == -1) N *= -1;
+    X += tr;
+    S.AddPoint(X, N);
+  }
   return S;
 }
 
@@ -271,9 +320,7 @@ int main(int argc, char** argv) {
       T3DScalarContainer C;
       if (gArgs.count("points")) {
         // points=x,y,z,nx,ny,nz,x,y,z,... (calculate_flux with a point list, dim=3)
-        std::vector<double> p = vec("points");
-        TSurfacePoints_3D S;
-        for (size_t i = 0; i + 5 < p.size(); i += 6) S.AddPoint(TVector3D(p[i], p[i + 1], p[i + 2]), TVector3D(p[i + 3], p[i + 4], p[i + 5]));
+        TSurfacePoints_3D S = points3d(false);
         osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, 3, "");
       } else {
         TSurfacePoints_Rectangle S = rect();
@@ -284,10 +331,8 @@ int main(int argc, char** argv) {
       T3DScalarContainer C;
       bool const directional = num("normal", 0) != 0;
       if (gArgs.count("points")) {
-        std::vector<double> p = vec("points");
-        TSurfacePoints_3D S;
-        for (size_t i = 0; i + 5 < p.size(); i += 6) S.AddPoint(TVector3D(p[i], p[i + 1], p[i + 2]), TVector3D(p[i + 3], p[i + 4], p[i + 5]));
-        osr.CalculatePowerDensity(S, C, 3, directional, nparticles, "", nthreads, 0);
+        TSurfacePoints_3D S = points3d(true);
+        osr.CalculatePowerDensity(S, C, 3, num("pnormal", 0) != 0, nparticles, "", nthreads, 0);
       } else {
         TSurfacePoints_Rectangle S = rect();
         osr.CalculatePowerDensity(S, C, (int)num("dim", 2), directional, nparticles, str("ofile", ""), nthreads, 0);
@@ -384,6 +429,12 @@ int main(int argc, char** argv) {
         put(P.GetGamma());
         put(P.GetQoverMGamma());
       }
+    } else if (mode == "uniforms") {
+      // layout: count, then count uniforms [0, 1) from the global generator after set_seed
+      // (TRandomA::Uniform, TRandomA.cc:61-64)
+      int count = (int)num("count", 8);
+      put((double)count);
+      for (int i = 0; i < count; ++i) put(gRandomA->Uniform());
     } else if (mode == "normals") {
       // layout: count, then count standard normals straight from the global generator after set_seed
       int count = (int)num("count", 8);

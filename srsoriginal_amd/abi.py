@@ -187,7 +187,7 @@ class SrsField(C.Structure):
 
 class SrsParticle(C.Structure):
     _fields_ = [("x0", _D * 3), ("beta0", _D * 3), ("qmg", _D), ("t0", _D), ("ct_start", _D), ("ct_stop", _D),
-                ("npoints", _I64)]
+                ("npoints", _I64), ("charge", _D), ("current", _D)]
 
 
 def _bind_batch():
@@ -246,19 +246,20 @@ def undulators(specs, keep):
 class SrsBeam(C.Structure):
     _fields_ = [("e0", _D), ("sigma_e", _D), ("mass", _D), ("charge", _D), ("x0", _D * 3), ("u0", _D * 3),
                 ("hdir", _D * 3), ("vdir", _D * 3), ("sigma_at", _D * 3), ("sigma_u", _D * 2), ("sigma_up", _D * 2),
-                ("t0", _D), ("ct_start", _D), ("ct_stop", _D), ("npoints", _I64)]
+                ("t0", _D), ("ct_start", _D), ("ct_stop", _D), ("npoints", _I64), ("current", _D)]
 
 
 def _srs_beam(beam, ct_start, ct_stop, npoints):
     return SrsBeam(float(beam["e0"]), float(beam["sigma_e"]), float(beam["mass"]), float(beam["charge"]),
                    (_D * 3)(*beam["x0"]), (_D * 3)(*beam["u0"]), (_D * 3)(*beam["hdir"]), (_D * 3)(*beam["vdir"]),
                    (_D * 3)(*beam["sigma_at"]), (_D * 2)(*beam["sigma_u"]), (_D * 2)(*beam["sigma_up"]),
-                   float(beam.get("t0", 0.0)), float(ct_start), float(ct_stop), int(npoints))
+                   float(beam.get("t0", 0.0)), float(ct_start), float(ct_stop), int(npoints),
+                   float(beam.get("current", 0.0)))
 
 
 def _rows(out):
     return [dict(x0=list(r.x0), beta0=list(r.beta0), qmg=r.qmg, t0=r.t0, ct_start=r.ct_start, ct_stop=r.ct_stop,
-                 npoints=r.npoints) for r in out]
+                 npoints=r.npoints, charge=r.charge, current=r.current) for r in out]
 
 
 def sample_beam(beam, np_, seed, first=0, ct_start=0.0, ct_stop=1.0, npoints=2):
@@ -289,11 +290,13 @@ def sample_beams(beams, weights, np_, seed, first=0, ct_start=0.0, ct_stop=1.0, 
 
 
 def particles(rows):
-    """srs_particle array from dicts (x0, beta0, qmg, t0, ct_start, ct_stop, npoints)."""
+    """srs_particle array from dicts (x0, beta0, qmg, t0, ct_start, ct_stop, npoints[, charge,
+    current]); without a charge (or with 0) a particle takes the batch call's charge and current."""
     arr = (SrsParticle * len(rows))()
     for i, p in enumerate(rows):
         arr[i] = SrsParticle((_D * 3)(*p["x0"]), (_D * 3)(*p["beta0"]), float(p["qmg"]), float(p.get("t0", 0.0)),
-                             float(p["ct_start"]), float(p["ct_stop"]), int(p["npoints"]))
+                             float(p["ct_start"]), float(p["ct_stop"]), int(p["npoints"]), float(p.get("charge", 0.0)),
+                             float(p.get("current", 0.0)))
     return arr
 
 

@@ -20,14 +20,12 @@ def sample_particles_device(osr, n, ct_start, ct_stop, npoints, seed, first=0, b
     (Philox4x32-10), so particle i is the same whatever n and call order; each particle
     from a beam chosen by the beams' weights (TParticleBeamContainer.cc:100-142), then the
     reference's transform (TParticleBeam.cc:228-265).  beam = one beam's name to draw
-    from it alone.  Returns (rows, charge, current); the beams must share charge and
-    current (as the batched pipelines require)."""
+    from it alone.  Returns (rows, charge, current): every row carries its beam's charge
+    and current; the pair returned is the first beam's."""
     names = osr._e.beam_names()
     if not names:
         raise ValueError("No particle beam defined")
     bps = [osr._e.beam_params(beam)] if beam is not None else [osr._e.beam_params(nm) for nm in names]
-    if any(b["charge"] != bps[0]["charge"] or b["current"] != bps[0]["current"] for b in bps):
-        raise ValueError("the beams must share charge and current")
     rows = abi.sample_beams(bps, [b["weight"] for b in bps], n, seed, first, ct_start, ct_stop, npoints)
     return rows, bps[0]["charge"], bps[0]["current"]
 
@@ -50,7 +48,7 @@ def sample_particles(osr, n, ct_start, ct_stop, npoints, ct_jitter=0.0, points_p
             stop = ct_start + (ct_stop - ct_start) * (1 + rng.uniform(-ct_jitter, ct_jitter))
             npts = int((points_per_meter or (npoints / (ct_stop - ct_start))) * (stop - ct_start))
         rows.append(dict(x0=st["x0"], beta0=st["beta0"], qmg=st["qmg"], t0=st["t0"], ct_start=ct_start, ct_stop=stop,
-                         npoints=npts))
+                         npoints=npts, charge=q, current=cur))
     return rows, q, cur
 
 

@@ -240,7 +240,7 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
   int const nch = nchunks(t.n, chunk);
   HIPCHK(c.part.ensure((size_t)nch * maxe * 6 * ns * sizeof(double)));
   if (precision == SRS_PRECISION_MIXED) {
-    HIPCHK(c.trajf.ensure((size_t)3 * t.n * sizeof(float)));
+    HIPCHK(c.trajf.ensure(srs::traj_f32_floats(t.n, 0) * sizeof(float)));
     HIPCHK(srs::launch_traj_to_f32(t, 0, c.trajf.as<float>(), s));
   }
   for (int64_t e0 = 0; e0 < ne; e0 += maxe) {
@@ -337,7 +337,7 @@ int run_pd(Context& c, const TrajView& t, const double* obs_soa, const double* n
   int const nch = nchunks(t.n, chunk);
   HIPCHK(c.part.ensure((size_t)nch * ns * sizeof(double)));
   if (precision == SRS_PRECISION_MIXED) {
-    HIPCHK(c.trajf.ensure((size_t)7 * t.n * sizeof(float)));
+    HIPCHK(c.trajf.ensure(srs::traj_f32_floats(t.n, 1) * sizeof(float)));
     HIPCHK(srs::launch_traj_to_f32(t, 1, c.trajf.as<float>(), s));
     KernelTimer timer(c, s);
     HIPCHK(srs::launch_pd_mixed(t, c.trajf.as<float>(), obs_soa, nrm_soa, (int)ns, directional, chunk,
@@ -389,11 +389,23 @@ int particle_grid(const srs_particle& p, double* dt, int64_t* nb, int64_t* nf) {
   return SRS_OK;
 }
 
-int build_particles(const srs_particle* p, int64_t np, std::vector<srs::BatchParticle>& out, int64_t* total) {
+// A particle's charge and current: its own, or the call's for a particle with charge 0.
+void particle_charge(const srs_particle& p, double charge, double current, double* q, double* cur) {
+  *q = p.charge != 0 ? p.charge : charge;
+  *cur = p.charge != 0 ? p.current : current;
+}
+
+int build_particles(const srs_particle* p, int64_t np, double charge, double current,
+                    std::vector<srs::BatchParticle>& out, int64_t* total) {
   out.resize(np);
   int64_t off = 0;
   for (int64_t i = 0; i < np; ++i) {
     srs::BatchParticle& q = out[i];
+    double qq, ii;
+    particle_charge(p[i], charge, current, &qq, &ii);
+    q.c0 = c0_of(qq);
+    q.c2 = c2_of(qq, ii);
+    q.pdq = pd_qfac_of(qq, ii);
     for (int k = 0; k < 3; ++k) {
       q.x0[k] = p[i].x0[k];
       q.beta0[k] = p[i].beta0[k];
@@ -507,14 +519,18 @@ int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) 
   return SRS_OK;
 }
 
-int stage_batch_traj(Context& c, int64_t total, bool with_a, srs::BatchTraj* bt) {
+// Packed trajectory buffers of np particles with `total` points: FP64 rows (A/c when
+// with_a), the FP32 rows, then np ints of per-particle "not along z" flags (the device
+// RK4 writes them).
+int stage_batch_traj(Context& c, int64_t total, int64_t np, bool with_a, srs::BatchTraj* bt) {
   HIPCHK(c.b_traj.ensure((size_t)(with_a ? 9 : 6) * total * sizeof(double)));
-  HIPCHK(c.b_trajf.ensure((size_t)3 * total * sizeof(float)));
+  HIPCHK(c.b_trajf.ensure(((size_t)3 * total + (size_t)np) * sizeof(float)));
   double* d = c.b_traj.as<double>();
   float* f = c.b_trajf.as<float>();
   size_t const n = (size_t)total;
-  *bt = srs::BatchTraj{d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, with_a ? d + 6 * n : nullptr,
-                       with_a ? d + 7 * n : nullptr, with_a ? d + 8 * n : nullptr, f, f + n, f + 2 * n};
+  *bt = srs::BatchTraj{d,          d + n,      d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, with_a ? d + 6 * n : nullptr,
+                       with_a ? d + 7 * n : nullptr, with_a ? d + 8 * n : nullptr, f, f + n, f + 2 * n,
+                       reinterpret_cast<int*>(f + 3 * n)};
   return SRS_OK;
 }
 
@@ -600,10 +616,11 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
   // depends on it), the CU count behind the persistent grid size, the flux prefactor c2
   // and the weight, and the address of every buffer a launch reads or writes
   snprintf(buf, sizeof buf,
-           "%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns, r.ne,
-           (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local, c.cus, (long long)r.max_leg,
+           "%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns,
+           r.ne, (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local, c.cus, (long long)r.max_leg,
            r.weight, r.be.c2, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
-           c.b_state.p, c.b_field.p, c.b_poles.p, c.b_part.p, c.b_counter.p, c.b_und.p);
+           c.b_state.p, c.b_field.p, c.b_poles.p, c.b_part.p, c.b_counter.p, c.b_und.p,
+           (void*)r.bt.bz, (void*)r.bt.tflag);  // row offsets within the packed buffers follow the total point count
   return buf;
 }
 
@@ -613,7 +630,7 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
 // with pd_batch_kernel per segment on s (adding in segment order), then the ordered reduction into
 // raw (np * ns) and b_flux_pp (np * ns, the per-particle power density).
 int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int64_t max_leg, int64_t max_pts,
-                     srs::Checkpoints const& ck, int ns, int dir, double qfac, double* raw, hipStream_t s,
+                     srs::Checkpoints const& ck, int ns, int dir, double* raw, hipStream_t s,
                      bool mixed = false) {
   int nseg = 1;
   int64_t const L = segment_steps(max_leg, &nseg);
@@ -641,7 +658,7 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
                                   nch, k > 0, c.g_part.as<double>(), mixed, s));
     }
   }
-  HIPCHK(srs::launch_pd_batch_reduce(c.g_part.as<double>(), nch, ns, np, Pd, qfac, dir, raw,
+  HIPCHK(srs::launch_pd_batch_reduce(c.g_part.as<double>(), nch, ns, np, Pd, dir, raw,
                                      c.b_flux_pp.as<double>(), s));
   return SRS_OK;
 }
@@ -691,13 +708,13 @@ struct PoleRun {
 // observation points, target, trajectory buffers (with accelerations when with_a)
 // and checkpoints, all queued on s.  Invalidates the tape.
 int pole_stage(Context& c, const srs_field* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
-               const double* deltas, const double* obs_xyz, int64_t ns, const double* target, int every, bool with_a,
-               PoleRun& pr, hipStream_t s) {
+               const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
+               const double* target, int every, bool with_a, PoleRun& pr, hipStream_t s) {
   pr.npoles = 2 * und[poled].nperiods;
   pr.every = every;
   {
     std::vector<srs_particle> rows((size_t)nreal, *particle);
-    TRY(build_particles(rows.data(), nreal, pr.P, &pr.total));
+    TRY(build_particles(rows.data(), nreal, charge, current, pr.P, &pr.total));
     for (int64_t r = 0; r < nreal; ++r) pr.P[r].pole_base = r * pr.npoles;
   }
   pr.nmax = pr.P[0].nb + pr.P[0].nf;
@@ -717,7 +734,7 @@ int pole_stage(Context& c, const srs_field* und, int nund, int poled, const srs_
   TRY(stage_points(c, c.b_obs, obs_xyz, ns));
   HIPCHK(c.g_target.ensure(ns * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c.g_target.p, target, ns * sizeof(double), hipMemcpyHostToDevice, s));
-  TRY(stage_batch_traj(c, pr.total, with_a, &pr.bt));
+  TRY(stage_batch_traj(c, pr.total, nreal, with_a, &pr.bt));
   HIPCHK(c.g_ckpt.ensure((size_t)nreal * pr.stride * 6 * sizeof(double)));
   pr.ck = srs::Checkpoints{c.g_ckpt.as<double>(), every, pr.stride};
   HIPCHK(c.b_flux_pp.ensure((size_t)nreal * ns * sizeof(double)));
@@ -872,7 +889,9 @@ int srs_hip_efield_time(const srs_traj* traj, const double obs[3], double charge
 
 static_assert(sizeof(srs::SampledParticle) == sizeof(srs_particle) &&
                   offsetof(srs::SampledParticle, qmg) == offsetof(srs_particle, qmg) &&
-                  offsetof(srs::SampledParticle, npoints) == offsetof(srs_particle, npoints),
+                  offsetof(srs::SampledParticle, npoints) == offsetof(srs_particle, npoints) &&
+                  offsetof(srs::SampledParticle, charge) == offsetof(srs_particle, charge) &&
+                  offsetof(srs::SampledParticle, current) == offsetof(srs_particle, current),
               "sampled rows are srs_particle rows");
 
 int srs_hip_sample_beams(const srs_beam* beams, const double* weights, int nbeams, int64_t np, uint64_t seed,
@@ -902,6 +921,7 @@ int srs_hip_sample_beams(const srs_beam* beams, const double* weights, int nbeam
     b.mgev = mgev;
     b.q = beam.charge;
     b.m = beam.mass;
+    b.cur = beam.current != 0 ? beam.current : beam.charge;  // TParticleA.cc:258-262
     for (int i = 0; i < 3; ++i) {
       b.x0[i] = beam.x0[i];
       b.u0[i] = beam.u0[i];
@@ -1035,7 +1055,7 @@ int srs_hip_trajectory_points(const srs_particle* p, int64_t np, int64_t* total)
   g_err.clear();
   if (!p || np < 1 || !total) return fail(SRS_ERR_INVALID, "no particles");
   std::vector<srs::BatchParticle> P;
-  return build_particles(p, np, P, total);
+  return build_particles(p, np, 0.0, 0.0, P, total);
 }
 
 int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, int64_t np, double* traj_out,
@@ -1044,7 +1064,7 @@ int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, 
   if (!p || np < 1 || !traj_out || !offsets_out || !dt_out) return fail(SRS_ERR_INVALID, "NULL argument");
   std::vector<srs::BatchParticle> P;
   int64_t tot = 0;
-  TRY(build_particles(p, np, P, &tot));
+  TRY(build_particles(p, np, 0.0, 0.0, P, &tot));
   if (tot != total) return fail(SRS_ERR_INVALID, "traj_out size does not match the trajectory points");
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
@@ -1055,7 +1075,7 @@ int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, 
   HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
   HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
   srs::BatchTraj bt;
-  TRY(stage_batch_traj(c, tot, true, &bt));
+  TRY(stage_batch_traj(c, tot, np, true, &bt));
   HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.und_local, c.b_poles.as<double>(),
                                c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0},
                                srs::RK4Seg{0, INT64_MAX, nullptr}, s));
@@ -1082,7 +1102,7 @@ int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, in
   if (!obs_xyz || !energies_ev || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
   std::vector<srs::BatchParticle> P;
   int64_t total = 0;
-  TRY(build_particles(p, np, P, &total));
+  TRY(build_particles(p, np, charge, current, P, &total));
   int64_t const ntiles = (ns + srs::kBlock - 1) / srs::kBlock;
   if ((double)np * ntiles * ne > 4.0e9) return fail(SRS_ERR_INVALID, "too many work items");
   Context& c = ctx();
@@ -1112,7 +1132,7 @@ int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, in
   HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
   BatchRun r;
   r.nund = nund;
-  TRY(stage_batch_traj(c, total, false, &r.bt));
+  TRY(stage_batch_traj(c, total, np, false, &r.bt));
   double* ed = c.b_energies.as<double>();
   r.be = srs::BatchEnergies{ed, ed + ne, ed + 2 * ne, ed + 3 * ne, ed + 4 * ne, (int)ne, c2_of(charge, current)};
   r.np = (int)np;
@@ -1212,7 +1232,7 @@ int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_partic
   if (!obs_xyz || !normals_xyz || !pd_out) return fail(SRS_ERR_INVALID, "NULL argument");
   std::vector<srs::BatchParticle> P;
   int64_t total = 0;
-  TRY(build_particles(p, np, P, &total));
+  TRY(build_particles(p, np, charge, current, P, &total));
   int64_t max_leg = 0, max_pts = 0;
   for (auto const& q : P) {
     max_leg = std::max(max_leg, std::max(q.nf, q.nb));
@@ -1229,14 +1249,14 @@ int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_partic
   TRY(stage_points(c, c.b_obs, obs_xyz, ns));
   TRY(stage_points(c, c.g_nrm, normals_xyz, ns));
   srs::BatchTraj bt;
-  TRY(stage_batch_traj(c, total, true, &bt));
+  TRY(stage_batch_traj(c, total, np, true, &bt));
   HIPCHK(c.b_flux_pp.ensure((size_t)np * ns * sizeof(double)));
   HIPCHK(c.g_field.ensure((size_t)np * ns * sizeof(double)));
   HIPCHK(c.b_out.ensure((size_t)ns * sizeof(double)));
   {
     KernelTimer timer(c, s);
     TRY(enqueue_pd_batch(c, bt, (int)np, nund, max_leg, max_pts, srs::Checkpoints{nullptr, 1, 0}, (int)ns,
-                         directional ? 1 : 0, pd_qfac_of(charge, current), c.g_field.as<double>(), s,
+                         directional ? 1 : 0, c.g_field.as<double>(), s,
                          precision == SRS_PRECISION_MIXED));
     HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), (int)np, (int)ns, weight, c.b_out.as<double>(), s));
   }
@@ -1256,7 +1276,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   if (!obs || !energies_ev || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
   std::vector<srs::BatchParticle> P;
   int64_t total = 0;
-  TRY(build_particles(p, np, P, &total));
+  TRY(build_particles(p, np, charge, current, P, &total));
   int64_t max_leg = 0, max_pts = 0;
   for (auto const& q : P) {
     max_leg = std::max(max_leg, std::max(q.nf, q.nb));
@@ -1282,7 +1302,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 3 * ne * sizeof(double), hipMemcpyHostToDevice, s));
   double* const ed = c.b_energies.as<double>();
   srs::BatchTraj bt;
-  TRY(stage_batch_traj(c, total, false, &bt));
+  TRY(stage_batch_traj(c, total, np, false, &bt));
   int nseg = 1;
   int64_t const L = segment_steps(max_leg, &nseg);
   // chunks per segment: ~2048 blocks of 256 energies
@@ -1318,8 +1338,8 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
                                           c.g_part.as<double>(), mixed, s));
       }
     }
-    HIPCHK(srs::launch_spectrum_batch_reduce(c.g_part.as<double>(), nch, (int)ne, (int)np, Pd, ed + ne, ed + 2 * ne,
-                                             c2_of(charge, current), c.b_flux_pp.as<double>(), s));
+    HIPCHK(srs::launch_spectrum_batch_reduce(c.g_part.as<double>(), nch, (int)ne, (int)np, Pd, ed + ne, ed,
+                                             c.b_flux_pp.as<double>(), s));
     HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), (int)np, (int)ne, weight, c.b_out.as<double>(), s));
   }
   HIPCHK(hipMemcpyAsync(flux_out, c.b_out.p, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1354,12 +1374,14 @@ int srs_hip_pole_forward(const srs_field* und, int nund, int poled, const srs_pa
   TRY(pole_validate(und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, loss_out,
                     tape_out));
   if (!(energy_ev > 0)) return fail(SRS_ERR_INVALID, "photon energy must be > 0");
+  particle_charge(*particle, charge, current, &charge, &current);
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
   TRY(init(c));
   hipStream_t s = c.stream;
   PoleRun pr;
-  TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, false, pr, s));
+  TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, charge, current, obs_xyz, ns, target, checkpoint_every,
+                 false, pr, s));
   double const w = omega_of(energy_ev), c0 = c0_of(charge);
   double const eh[5] = {w, kC / w, c0 * w, w / kTwoPi, (w / kTwoPi) / kC};
   HIPCHK(c.b_energies.ensure(5 * sizeof(double)));
@@ -1404,19 +1426,21 @@ int srs_hip_pole_pd_forward(const srs_field* und, int nund, int poled, const srs
   TRY(pole_validate(und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, loss_out,
                     tape_out));
   if (!normals_xyz) return fail(SRS_ERR_INVALID, "NULL argument");
+  particle_charge(*particle, charge, current, &charge, &current);
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
   TRY(init(c));
   hipStream_t s = c.stream;
   PoleRun pr;
-  TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, true, pr, s));
+  TRY(pole_stage(c, und, nund, poled, particle, nreal, deltas, charge, current, obs_xyz, ns, target, checkpoint_every,
+                 true, pr, s));
   TRY(stage_points(c, c.g_nrm, normals_xyz, ns));
   HIPCHK(c.g_field.ensure((size_t)nreal * ns * sizeof(double)));
   double const scale = pd_scale_of(charge, current, pr.P[0].dt);
   {
     KernelTimer timer(c, s);
     TRY(enqueue_pd_batch(c, pr.bt, (int)nreal, nund, std::max(pr.P[0].nf, pr.P[0].nb), pr.P[0].nb + pr.P[0].nf, pr.ck,
-                         (int)ns, directional ? 1 : 0, pd_qfac_of(charge, current), c.g_field.as<double>(), s));
+                         (int)ns, directional ? 1 : 0, c.g_field.as<double>(), s));
   }
   TRY(pole_finish(c, pr, (int)nreal, (int)ns, pd_out, loss_out, s));
   Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);

@@ -197,9 +197,15 @@ __global__ __launch_bounds__(kBlock) void pd_adjoint_kernel(BatchTraj bt, const 
       sg[t] = gsc * sgn * (pd[rj] - target[j]);
     }
     __syncthreads();
-    for (int t = 0; t < cnt; ++t)
-      pd_term_adjoint(sg[t], so[0][t], so[1][t], so[2][t], so[3][t], so[4][t], so[5][t], X, B, A, directional, gX,
-                      gB, gA);
+    if (transverse_beam(B[0], B[1])) {  // this lane's point does not travel along z
+      for (int t = 0; t < cnt; ++t)
+        pd_term_adjoint<true>(sg[t], so[0][t], so[1][t], so[2][t], so[3][t], so[4][t], so[5][t], X, B, A, directional,
+                              gX, gB, gA);
+    } else {
+      for (int t = 0; t < cnt; ++t)
+        pd_term_adjoint(sg[t], so[0][t], so[1][t], so[2][t], so[3][t], so[4][t], so[5][t], X, B, A, directional, gX,
+                        gB, gA);
+    }
   }
   if (valid) {
 #pragma unroll

@@ -44,6 +44,10 @@ struct BatchParticle {
   double qmg, dt;
   int64_t nb, nf, off;
   int64_t pole_base;
+  // the particle's prefactors, from its charge q and current I with the reference's
+  // expressions: c0 = q/(4 pi c eps0 sqrt(2 pi)) (OSCARSSR.cc:1191), c2 = 4 pi I/(h |q| mu0 c)
+  // 1e-6 1e-3 (:1194), pdq = |q I|/(16 pi^2 eps0 c) (:1563)
+  double c0, c2, pdq;
 };
 
 // RK4 checkpoints: state [x, vx, y, vy, z, vz] every `every` steps of each leg;
@@ -56,10 +60,13 @@ struct Checkpoints {
 };
 
 // Packed trajectories of all particles (FP64 for the plain form and A/c, FP32
-// beta_x, beta_y, 1 - beta_z for the mixed form).
+// beta_x, beta_y, 1 - beta_z for the mixed form) and, per particle, a flag the device
+// RK4 sets when a point integrated so far does not travel along z (transverse_beam):
+// the mixed kernels take the FP64 terms for such a particle.
 struct BatchTraj {
   double *x, *y, *z, *bx, *by, *bz, *ax, *ay, *az;  // ax..az may be NULL
   float *fbx, *fby, *fomb;
+  int* tflag;  // [np]; may be NULL (then no particle is flagged)
 };
 
 struct BatchEnergies {
@@ -91,7 +98,7 @@ struct RK4Seg {
 // Beam of the device sampler (mgev = particle mass in GeV) and its output rows, laid
 // out as srs_particle.
 struct DevBeam {
-  double e0, sigma_e, mgev, q, m;
+  double e0, sigma_e, mgev, q, m, cur;
   double x0[3], u0[3], h[3], v[3], at[3];
   double su[2], sp[2];
   double t0, ct_start, ct_stop;
@@ -100,6 +107,7 @@ struct DevBeam {
 struct SampledParticle {
   double x0[3], beta0[3], qmg, t0, ct_start, ct_stop;
   int64_t npoints;
+  double charge, current;
 };
 // beams: nb device records; wsum: their cumulative weights (device)
 hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
@@ -153,24 +161,24 @@ hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const do
 // chunks into part (np * nch * ns doubles), writing (add = 0) or adding (add != 0:
 // later time segments, queued in order on one stream); launch_pd_batch_reduce adds
 // the nch chunks in order: raw[r*ns + j] = summed integrand, pd[r*ns + j] =
-// qfac dt_r raw / 1e6 (|.| unless directional), qfac = |qI|/(16 pi^2 eps0 c).
+// pdq_r dt_r raw / 1e6 (|.| unless directional), pdq_r = |q I|/(16 pi^2 eps0 c) of particle r.
 // pd_batch_chunks gives nch for npoints points.
 int pd_batch_chunks(int64_t npoints, int ns, int np);
 hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double* obs, const double* nrm,
                            int ns, int directional, int64_t lo, int64_t hi, int nch, int add, double* part,
                            bool mixed, hipStream_t s);
-hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
+hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P,
                                   int directional, double* raw, double* pd, hipStream_t s);
 // Spectrum of np particles at one observation point for ne energies (omega [ne]):
 // launch_spectrum_batch sums the points of RK4 steps [lo, hi) of both legs in nch
 // chunks into part (np * nch * 12 * ne doubles; write or add as launch_pd_batch);
 // launch_spectrum_batch_reduce adds the chunks in order and applies cw = c/omega,
-// i (C0 omega) dt_r and C2: out[r*ne + e].
+// i (C0_r omega) dt_r and C2_r (particle r's prefactors): out[r*ne + e].
 hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double obs[3],
                                  const double* omega, int ne, int64_t lo, int64_t hi, int nch, int add, double* part,
                                  bool mixed, hipStream_t s);
 hipError_t launch_spectrum_batch_reduce(const double* part, int nch, int ne, int np, const BatchParticle* P,
-                                        const double* cw, const double* c0w, double c2, double* out, hipStream_t s);
+                                        const double* cw, const double* omega, double* out, hipStream_t s);
 // dL/dy (gy [6][total]) and dL/dA (ga [3][total]) of L_r = sum_j (pd_rj - target_j)^2
 // scaled by dloss[r].
 hipError_t launch_pd_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,

@@ -62,6 +62,7 @@ __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __res
     for (int c = 0; c < 6; ++c) ck.state[(p * ck.stride + slot) * 6 + c] = x[c];
   };
   int64_t const nf_slots = ck.state ? (q.nf + ck.every - 1) / ck.every : 0;
+  bool tr = false;  // a point of this segment does not travel along z
   if (sg.lo < q.nf) {
     if (sg.lo == 0) init();
     else load(0);
@@ -69,7 +70,7 @@ __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __res
     int64_t const i1 = sg.hi < q.nf ? sg.hi : q.nf;
     for (int64_t i = sg.lo; i < i1; ++i) {
       if (ck.state && i % ck.every == 0) save(i / ck.every);
-      store_point(bt, q.off + q.nb + i, x, d);
+      tr |= store_point(bt, q.off + q.nb + i, x, d);
       R.derivs(x, d);
       R.step(x, d, h, hh, h6);
     }
@@ -84,10 +85,13 @@ __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __res
       if (ck.state && i % ck.every == 0) save(nf_slots + i / ck.every);
       R.derivs(x, d);
       R.step(x, d, h, hh, h6);
-      store_point(bt, q.off + q.nb - 1 - i, x, d);
+      tr |= store_point(bt, q.off + q.nb - 1 - i, x, d);
     }
     keep(1);
   }
+  // segments run in order, each after the previous one's flux: the flag covers every
+  // point a later kernel of this segment reads
+  if (bt.tflag) bt.tflag[p] = (sg.lo == 0 ? 0 : bt.tflag[p]) | (tr ? 1 : 0);
 }
 
 __global__ void zero_counter_kernel(unsigned* c) {
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
       int const ee = (g * NE + e < be.ne) ? g * NE + e : be.ne - 1;
       es.omega[e] = be.omega[ee];
       es.cw[e] = be.cw[ee];
-      es.kw[e] = be.c0w[ee] * q.dt;  // (C0*omega)*dt, OSCARSSR.cc:2080
+      es.kw[e] = (q.c0 * be.omega[ee]) * q.dt;  // (C0*omega)*dt with the particle's C0, OSCARSSR.cc:2080
       es.fr[e] = be.fr[ee];
       es.fw[e] = be.fw[ee];
       cwf[e] = (float)be.cw[ee];
@@ -166,6 +170,55 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
     int const j = j0 + tid;
     int const jj = j < ns ? j : j0;  // idle lanes repeat the tile's first point
     double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
+    // this segment's six field sums S(e, q6) (+ the earlier segments', in segment order)
+    auto finish = [&](auto sum_of) {
+      if (j >= ns) return;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        int const ee = g * NE + e;
+        if (ee >= be.ne) continue;
+        double S[6];
+#pragma unroll
+        for (int q6 = 0; q6 < 6; ++q6) S[q6] = sum_of(e, q6);
+        size_t const b = ((size_t)p * be.ne + ee) * 6 * ns + j;
+        if (!sg.first) {
+#pragma unroll
+          for (int q6 = 0; q6 < 6; ++q6) S[q6] = sg.field_acc[b + (size_t)q6 * ns] + S[q6];
+        }
+        if (sg.field_acc && !(sg.first && sg.final && !sg.keep_field)) {
+#pragma unroll
+          for (int q6 = 0; q6 < 6; ++q6) sg.field_acc[b + (size_t)q6 * ns] = S[q6];
+        }
+        if (sg.final) flux_pp[((size_t)p * be.ne + ee) * ns + j] = flux_from_sums(S[0], S[1], S[2], S[3], S[4], S[5],
+                                                                                    es.kw[e], q.c2);
+      }
+    };
+    if (MIXED && bt.tflag && bt.tflag[p]) {
+      // a point of this particle does not travel along z, which the mixed terms assume
+      // (they stage 1 - beta_z): the item sums the FP64 terms instead
+      AccFP64<NE> d;
+      d.zero();
+      for (int part = 0; part < 2; ++part) {
+        int64_t const lo = rg.r0[part], hi = rg.r1[part];
+        for (int64_t base = lo; base < hi; base += TILE_) {
+          int const cnt = (hi - base) < TILE_ ? (int)(hi - base) : TILE_;
+          __syncthreads();
+          for (int k = tid; k < cnt; k += kBlock) {
+            int64_t const i = q.off + base + k;
+            T.raw.x[k] = bt.x[i];
+            T.raw.y[k] = bt.y[i];
+            T.raw.z[k] = bt.z[i];
+            T.raw.bx[k] = bt.bx[i];
+            T.raw.by[k] = bt.by[i];
+            T.raw.bz[k] = bt.bz[i];
+          }
+          __syncthreads();
+          flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, d);
+        }
+      }
+      finish([&](int e, int q6) { return (q6 & 1) ? d.im[e][q6 >> 1] : d.re[e][q6 >> 1]; });
+      continue;
+    }
     AccFP64<NE> a64;
     AccMixed<NE> a32;
     bool para = false;
@@ -224,31 +277,8 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
         }
       }
     }
-    if (j < ns) {
-#pragma unroll
-      for (int e = 0; e < NE; ++e) {
-        int const ee = g * NE + e;
-        if (ee >= be.ne) continue;
-        // this segment's six field sums (+ the earlier segments', in segment order)
-        double S[6];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          S[2 * c] = MIXED ? a32.value(e, 2 * c) : a64.re[e][c];
-          S[2 * c + 1] = MIXED ? a32.value(e, 2 * c + 1) : a64.im[e][c];
-        }
-        size_t const b = ((size_t)p * be.ne + ee) * 6 * ns + j;
-        if (!sg.first) {
-#pragma unroll
-          for (int q6 = 0; q6 < 6; ++q6) S[q6] = sg.field_acc[b + (size_t)q6 * ns] + S[q6];
-        }
-        if (sg.field_acc && !(sg.first && sg.final && !sg.keep_field)) {
-#pragma unroll
-          for (int q6 = 0; q6 < 6; ++q6) sg.field_acc[b + (size_t)q6 * ns] = S[q6];
-        }
-        if (sg.final) flux_pp[((size_t)p * be.ne + ee) * ns + j] = flux_from_sums(S[0], S[1], S[2], S[3], S[4], S[5],
-                                                                                    es.kw[e], be.c2);
-      }
-    }
+    if constexpr (MIXED) finish([&](int e, int q6) { return a32.value(e, q6); });
+    else finish([&](int e, int q6) { return (q6 & 1) ? a64.im[e][q6 >> 1] : a64.re[e][q6 >> 1]; });
   }
 }
 
@@ -344,10 +374,23 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
                                                           const double* __restrict__ nrm, int ns, int directional,
                                                           int64_t lo, int64_t hi, int nch, int ntiles, int add,
                                                           double* __restrict__ part) {
-  __shared__ double sx[MIXED ? 1 : kBlock], sy[MIXED ? 1 : kBlock], sz[MIXED ? 1 : kBlock];
-  __shared__ double sbx[MIXED ? 1 : kBlock], sby[MIXED ? 1 : kBlock], sbz[MIXED ? 1 : kBlock];
-  __shared__ double sax[MIXED ? 1 : kBlock], say[MIXED ? 1 : kBlock], saz[MIXED ? 1 : kBlock];
-  __shared__ float4 fP[MIXED ? kBlock : 1], fB[MIXED ? kBlock : 1], fA[MIXED ? kBlock : 1];
+  // the FP64 tile (x, y, z, beta, A/c), or the mixed form's FP32 records
+  __shared__ union {
+    double d[9][kBlock];
+    float4 f[3][kBlock];
+  } sm;
+  double* const sx = sm.d[0];
+  double* const sy = sm.d[1];
+  double* const sz = sm.d[2];
+  double* const sbx = sm.d[3];
+  double* const sby = sm.d[4];
+  double* const sbz = sm.d[5];
+  double* const sax = sm.d[6];
+  double* const say = sm.d[7];
+  double* const saz = sm.d[8];
+  float4* const fP = sm.f[0];
+  float4* const fB = sm.f[1];
+  float4* const fA = sm.f[2];
   int const tid = threadIdx.x;
   int const tile = (int)(blockIdx.x % (unsigned)ntiles);
   int const rc = (int)(blockIdx.x / (unsigned)ntiles);
@@ -365,7 +408,10 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
   double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
   double const qx = nrm[jj], qy = nrm[ns + jj], qz = nrm[2 * ns + jj];
   double sum = 0.0;
-  if constexpr (MIXED) {
+  // a point of this particle does not travel along z, which the mixed terms assume (they
+  // stage 1 - beta_z): the FP64 terms below instead (block-uniform: one particle per block)
+  bool const mixed = MIXED && !(bt.tflag && bt.tflag[r]);
+  if (mixed) {
     if (directional)
       pd_batch_mixed_chunk<MIXED, true>(bt, i0, i1, nbs, first_b, first_f, ox, oy, oz, qx, qy, qz, fP, fB, fA, sum);
     else
@@ -374,6 +420,7 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
   for (int64_t base = i0; base < i1; base += kBlock) {
     int const cnt = (i1 - base) < kBlock ? (int)(i1 - base) : kBlock;
     __syncthreads();
+    bool tr = false;
     if (tid < cnt) {
       int64_t const v = base + tid;
       int64_t const i = v < nbs ? first_b + v : first_f + v;
@@ -386,11 +433,17 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
       sax[tid] = bt.ax[i];
       say[tid] = bt.ay[i];
       saz[tid] = bt.az[i];
+      tr = transverse_beam(sbx[tid], sby[tid]);
     }
-    __syncthreads();
-    for (int k = 0; k < cnt; ++k)
-      sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
-                     directional);
+    if (__syncthreads_or(tr)) {  // block-uniform: a point of this tile does not travel along z
+      for (int k = 0; k < cnt; ++k)
+        sum += pd_term<true>(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k],
+                             saz[k], directional);
+    } else {
+      for (int k = 0; k < cnt; ++k)
+        sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
+                       directional);
+    }
   }
   if (j < ns) {
     double* const dst = part + ((size_t)r * nch + ch) * ns + j;
@@ -400,10 +453,10 @@ __global__ __launch_bounds__(kBlock) void pd_batch_kernel(BatchTraj bt, const Ba
 
 // Chunks in fixed order, then the reference's prefactor and |.| (OSCARSSR.cc:1563-1573):
 // raw[r*ns + j] = the summed integrand, pd[r*ns + j] = the power density.  The
-// prefactor is qfac * dt_r (qfac = |qI|/(16 pi^2 eps0 c), formed on the host), which
-// rounds as the reference's |qI|/(16 pi^2 eps0 c) * dt.
+// prefactor is pdq_r * dt_r (pdq_r = |q I|/(16 pi^2 eps0 c) of particle r, formed on the
+// host), which rounds as the reference's |qI|/(16 pi^2 eps0 c) * dt.
 __global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch, int ns, int np,
-                                       const BatchParticle* __restrict__ P, double qfac, int directional,
+                                       const BatchParticle* __restrict__ P, int directional,
                                        double* __restrict__ raw, double* __restrict__ pd) {
   int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)np * ns) return;
@@ -411,7 +464,7 @@ __global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch,
   double s = 0.0;
   for (int c = 0; c < nch; ++c) s += part[((size_t)r * nch + c) * ns + j];
   raw[t] = s;
-  s *= qfac * P[r].dt;
+  s *= P[r].pdq * P[r].dt;
   s /= 1e6;
   if (!directional && s < 0) s = -s;
   pd[t] = s;
@@ -546,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void spectrum_batch_kernel(BatchTraj bt, co
 // out[r * ne + e].
 __global__ void spectrum_batch_reduce_kernel(const double* __restrict__ part, int nch, int ne, int np,
                                              const BatchParticle* __restrict__ P, const double* __restrict__ cw,
-                                             const double* __restrict__ c0w, double c2, double* __restrict__ out) {
+                                             const double* __restrict__ omega, double* __restrict__ out) {
   int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)np * ne) return;
   int64_t const r = t / ne, e = t % ne;
@@ -556,7 +609,7 @@ __global__ void spectrum_batch_reduce_kernel(const double* __restrict__ part, in
   for (int c = 0; c < nch; ++c)
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] += part[(((size_t)r * nch + c) * 12 + k) * ne + e];
-  double const w = cw[e], k2 = c0w[e] * P[r].dt;
+  double const w = cw[e], k2 = (P[r].c0 * omega[e]) * P[r].dt;
   double p[3];
 #pragma unroll
   for (int qq = 0; qq < 3; ++qq) {
@@ -566,7 +619,7 @@ __global__ void spectrum_batch_reduce_kernel(const double* __restrict__ part, in
     double const im = sre * k2;
     p[qq] = re * re + im * im;
   }
-  out[t] = c2 * (p[0] + p[1] + p[2]);
+  out[t] = P[r].c2 * (p[0] + p[1] + p[2]);
 }
 
 }  // namespace
@@ -697,6 +750,8 @@ __global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restr
   o.ct_start = b.ct_start;
   o.ct_stop = b.ct_stop;
   o.npoints = b.npoints;
+  o.charge = b.q;
+  o.current = b.cur;
 }
 
 hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
@@ -756,11 +811,10 @@ hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, 
   return hipGetLastError();
 }
 
-hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P, double qfac,
+hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P,
                                   int directional, double* raw, double* pd, hipStream_t s) {
   int64_t const n = (int64_t)np * ns;
-  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nslot, ns, np, P, qfac, directional, raw,
-                                                                      pd);
+  pd_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nslot, ns, np, P, directional, raw, pd);
   return hipGetLastError();
 }
 
@@ -780,9 +834,9 @@ hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, in
 }
 
 hipError_t launch_spectrum_batch_reduce(const double* part, int nch, int ne, int np, const BatchParticle* P,
-                                        const double* cw, const double* c0w, double c2, double* out, hipStream_t s) {
+                                        const double* cw, const double* omega, double* out, hipStream_t s) {
   int64_t const n = (int64_t)np * ne;
-  spectrum_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nch, ne, np, P, cw, c0w, c2, out);
+  spectrum_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nch, ne, np, P, cw, omega, out);
   return hipGetLastError();
 }
 

@@ -136,6 +136,7 @@ __global__ __launch_bounds__(kBlock) void pd_fp64_kernel(TrajView t, const doubl
   for (int64_t base = i0; base < i1; base += TILE) {
     int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
     __syncthreads();
+    bool tr = false;
     if (tid < cnt) {
       int64_t const i = base + tid;
       sx[tid] = t.x[i];
@@ -147,11 +148,17 @@ __global__ __launch_bounds__(kBlock) void pd_fp64_kernel(TrajView t, const doubl
       sax[tid] = t.ax[i];
       say[tid] = t.ay[i];
       saz[tid] = t.az[i];
+      tr = transverse_beam(sbx[tid], sby[tid]);
     }
-    __syncthreads();
-    for (int k = 0; k < cnt; ++k)
-      sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
-                     directional);
+    if (__syncthreads_or(tr)) {  // block-uniform: a point of this tile does not travel along z
+      for (int k = 0; k < cnt; ++k)
+        sum += pd_term<true>(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k],
+                             saz[k], directional);
+    } else {
+      for (int k = 0; k < cnt; ++k)
+        sum += pd_term(ox, oy, oz, qx, qy, qz, sx[k], sy[k], sz[k], sbx[k], sby[k], sbz[k], sax[k], say[k], saz[k],
+                       directional);
+    }
   }
   if (j < ns) part[(size_t)blockIdx.y * ns + j] = sum;
 }
@@ -207,6 +214,7 @@ __global__ __launch_bounds__(kBlock) void flux1_fp64_kernel(TrajView t, const do
   for (int64_t base = i0; base < i1; base += TILE) {
     int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
     __syncthreads();
+    bool tr = false;
     if (tid < cnt) {
       int64_t const i = base + tid;
       sx[tid] = t.x[i];
@@ -218,16 +226,19 @@ __global__ __launch_bounds__(kBlock) void flux1_fp64_kernel(TrajView t, const do
       sax[tid] = t.ax[i];
       say[tid] = t.ay[i];
       saz[tid] = t.az[i];
+      tr = transverse_beam(sbx[tid], sby[tid]);
     }
-    __syncthreads();
+    // block-uniform: a point of this tile does not travel along z -> n_x, n_y as quotients
+    bool const exact = __syncthreads_or(tr);
     for (int k = 0; k < cnt; ++k) {
       double const bx = sbx[k], by = sby[k], bz = sbz[k];
       double const rx = ox - sx[k], ry = oy - sy[k], rz = oz - sz[k];
       double const D = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
       double const inv = 1.0 / D;
       // nz correctly rounded (1 - n.b and nz - bz cancel through it); nx, ny by the
-      // reciprocal (their last bit is ~1e-15 of 1 - n.b)
-      double const nx = rx * inv, ny = ry * inv, nz = div_rn(rz, D, inv);
+      // reciprocal (their last bit is ~1e-15 of 1 - n.b) unless the beam is not along z
+      double const nx = exact ? div_rn(rx, D, inv) : rx * inv, ny = exact ? div_rn(ry, D, inv) : ry * inv;
+      double const nz = div_rn(rz, D, inv);
       double const omb = 1.0 - (nx * bx + ny * by + nz * bz);
       double const sb = 1.0 - (bx * bx + by * by + bz * bz);
       double const ux = nx - bx, uy = ny - by, uz = nz - bz;

@@ -61,7 +61,11 @@ hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* 
                            int directional, int chunk, double* part, hipStream_t s);
 // Single-precision copies used by the mixed kernels: trajf[k*n + i] for
 // k = 0..5: bx, by, (1 - bz), ax, ay, az (the last three only when t.ax..az are set).
-// writes 3 n floats (bx, by, 1 - bz), or 7 n when with_a (then t.ax..az must be set)
+// FP32 staging of the mixed kernels: 3 n floats (bx, by, 1 - bz), or 7 n when with_a
+// (then t.ax..az must be set), then one int per kFlagSpan points, nonzero when a point
+// of the span does not travel along z; traj_f32_floats(n, with_a) floats in all.
+constexpr int kFlagSpan = 256;
+size_t traj_f32_floats(int64_t n, int with_a);
 hipError_t launch_traj_to_f32(const TrajView& t, int with_a, float* trajf, hipStream_t s);
 
 // ---- reductions over trajectory chunks (fixed order, deterministic) --------

@@ -24,22 +24,35 @@ constexpr int SUB = kSub;
 
 
 // trajf layout (float): [0] bx, [1] by, [2] 1 - bz, and with_a: [3] ax, [4] ay, [5] az,
-// [6] 1 - |beta|^2 (from FP64: (1 - bz)(1 + bz) - bx^2 - by^2).  The caller states
-// with_a and sizes trajf to match: 3 n floats without, 7 n with (the flux path needs no
-// accelerations even when its trajectory view carries them).
-__global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f) {
+// [6] 1 - |beta|^2 (from FP64: (1 - bz)(1 + bz) - bx^2 - by^2); after the rows, one int
+// per 256 points (kFlagSpan), nonzero when a point of that span does not travel along z
+// (transverse_beam).  The caller states with_a and sizes trajf with traj_f32_floats.
+// Launched with 256-lane blocks: block b writes flag b.
+__global__ __launch_bounds__(kFlagSpan) void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f) {
   int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= t.n) return;
-  f[0 * t.n + i] = (float)t.bx[i];
-  f[1 * t.n + i] = (float)t.by[i];
-  f[2 * t.n + i] = (float)(1.0 - t.bz[i]);
-  if (with_a) {
-    f[3 * t.n + i] = (float)t.ax[i];
-    f[4 * t.n + i] = (float)t.ay[i];
-    f[5 * t.n + i] = (float)t.az[i];
-    double const omb = 1.0 - t.bz[i];
-    f[6 * t.n + i] = (float)(omb * (1.0 + t.bz[i]) - t.bx[i] * t.bx[i] - t.by[i] * t.by[i]);
+  bool tr = false;
+  if (i < t.n) {
+    f[0 * t.n + i] = (float)t.bx[i];
+    f[1 * t.n + i] = (float)t.by[i];
+    f[2 * t.n + i] = (float)(1.0 - t.bz[i]);
+    if (with_a) {
+      f[3 * t.n + i] = (float)t.ax[i];
+      f[4 * t.n + i] = (float)t.ay[i];
+      f[5 * t.n + i] = (float)t.az[i];
+      double const omb = 1.0 - t.bz[i];
+      f[6 * t.n + i] = (float)(omb * (1.0 + t.bz[i]) - t.bx[i] * t.bx[i] - t.by[i] * t.by[i]);
+    }
+    tr = transverse_beam(t.bx[i], t.by[i]);
   }
+  int const any = __syncthreads_or(tr);
+  if (threadIdx.x == 0) reinterpret_cast<int*>(f + (with_a ? 7 : 3) * t.n)[blockIdx.x] = any;
+}
+
+// Whether any point of [i0, i1) does not travel along z, from the span flags (block-uniform).
+__device__ __forceinline__ bool chunk_transverse(const int* __restrict__ flags, int64_t i0, int64_t i1) {
+  int f = 0;
+  for (int64_t b = i0 / kFlagSpan + threadIdx.x; b <= (i1 - 1) / kFlagSpan; b += kBlock) f |= flags[b];
+  return __syncthreads_or(f) != 0;
 }
 
 // A block owns 256 observation points and one trajectory chunk.  It takes the
@@ -48,6 +61,7 @@ __global__ void traj_to_f32_kernel(TrajView t, int with_a, float* __restrict__ f
 // decision), and the plain mixed terms otherwise.
 template <int NE, int TILE_>
 __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const float* __restrict__ tf,
+                                                             const int* __restrict__ flags,
                                                              const double* __restrict__ obs, int ns, EnergySet es,
                                                              int chunk, double* __restrict__ part) {
   __shared__ MixedTile<TILE_, NE> T;
@@ -62,6 +76,39 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
   for (int e = 0; e < NE; ++e) cwf[e] = (float)es.cw[e];
   int64_t const i0 = (int64_t)blockIdx.y * chunk;
   int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  if (chunk_transverse(flags, i0, i1)) {
+    // the beam of this chunk is not along z, which the mixed terms assume (they stage
+    // 1 - beta_z): the block sums the FP64 terms instead
+    AccFP64<NE> d;
+    d.zero();
+    for (int64_t base = i0; base < i1; base += TILE_) {
+      int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
+      __syncthreads();
+      for (int k = tid; k < cnt; k += kBlock) {
+        int64_t const i = base + k;
+        T.raw.x[k] = t.x[i];
+        T.raw.y[k] = t.y[i];
+        T.raw.z[k] = t.z[i];
+        T.raw.bx[k] = t.bx[i];
+        T.raw.by[k] = t.by[i];
+        T.raw.bz[k] = t.bz[i];
+      }
+      __syncthreads();
+      flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, t.dt, ox, oy, oz, es, d);
+    }
+    if (j < ns) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          part[b + (2 * k) * (size_t)ns] = d.re[e][k];
+          part[b + (2 * k + 1) * (size_t)ns] = d.im[e][k];
+        }
+      }
+    }
+    return;
+  }
   ParaBlock const pb = para_block(ox, oy, oz, obs[2 * ns + j0], red);
   bool const para = para_check(pb, t.x, t.y, t.z, i0, i1, es);
   AccMixed<NE> a;
@@ -247,18 +294,60 @@ __global__ __launch_bounds__(64 * SPW) void spectrum_mixed_kernel(TrajView t, do
 // range limits when raised to the -5th power, and the term comes out as 2^-55 times
 // its value; 2^55 is restored in FP64 when the sum is written.  32-term FP32 sub-sums,
 // Kahan-compensated.
+// The FP64 power density of one chunk (the reference's terms, every component of N1 as
+// its quotient) for chunks whose beam is not along z, where the mixed form's staged
+// 1 - beta_z does not apply.  sd: 9 x TILE doubles of LDS.
+template <bool DIR>
+__device__ __forceinline__ double pd_fp64_chunk(TrajView const& t, double* sd, int64_t i0, int64_t i1, double ox,
+                                                double oy, double oz, double qx, double qy, double qz) {
+  int const tid = threadIdx.x;
+  double sum = 0.0;
+  for (int64_t base = i0; base < i1; base += TILE) {
+    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
+    __syncthreads();
+    if (tid < cnt) {
+      int64_t const i = base + tid;
+      double const v[9] = {t.x[i], t.y[i], t.z[i], t.bx[i], t.by[i], t.bz[i], t.ax[i], t.ay[i], t.az[i]};
+#pragma unroll
+      for (int c = 0; c < 9; ++c) sd[c * TILE + tid] = v[c];
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k)
+      sum += pd_term<true>(ox, oy, oz, qx, qy, qz, sd[k], sd[TILE + k], sd[2 * TILE + k], sd[3 * TILE + k],
+                           sd[4 * TILE + k], sd[5 * TILE + k], sd[6 * TILE + k], sd[7 * TILE + k], sd[8 * TILE + k],
+                           DIR ? 1 : 0);
+  }
+  return sum;
+}
+
 template <bool DIR>
 __global__ __launch_bounds__(kBlock) void pd_mixed_kernel(TrajView t, const float* __restrict__ tf,
+                                                          const int* __restrict__ flags,
                                                           const double* __restrict__ obs, const double* __restrict__ nrm,
                                                           int ns, int chunk, double* __restrict__ part) {
-  __shared__ float4 sP[TILE], sB[TILE], sA[TILE];  // (x, y, z, S^2 (1 - |beta|^2)), S (bx, by, 1 - bz), A
+  // (x, y, z, S^2 (1 - |beta|^2)), S (bx, by, 1 - bz), A; or the FP64 tile of pd_fp64_chunk
+  __shared__ union {
+    struct {
+      float4 P[TILE], B[TILE], A[TILE];
+    } f;
+    double d[9 * TILE];
+  } sm;
+  float4* const sP = sm.f.P;
+  float4* const sB = sm.f.B;
+  float4* const sA = sm.f.A;
   int const tid = threadIdx.x;
   int const j = blockIdx.x * kBlock + tid;
   int const jj = j < ns ? j : ns - 1;
-  float const ox = (float)obs[jj], oy = (float)obs[ns + jj], oz = (float)obs[2 * ns + jj];
-  float const qx = (float)nrm[jj], qy = (float)nrm[ns + jj], qz = (float)nrm[2 * ns + jj];
   int64_t const i0 = (int64_t)blockIdx.y * chunk;
   int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  if (chunk_transverse(flags, i0, i1)) {
+    double const sum = pd_fp64_chunk<DIR>(t, sm.d, i0, i1, obs[jj], obs[ns + jj], obs[2 * ns + jj], nrm[jj],
+                                          nrm[ns + jj], nrm[2 * ns + jj]);
+    if (j < ns) part[(size_t)blockIdx.y * ns + j] = sum;
+    return;
+  }
+  float const ox = (float)obs[jj], oy = (float)obs[ns + jj], oz = (float)obs[2 * ns + jj];
+  float const qx = (float)nrm[jj], qy = (float)nrm[ns + jj], qz = (float)nrm[2 * ns + jj];
   int64_t const n = t.n;
   KahanF acc{0.f, 0.f};
   for (int64_t base = i0; base < i1; base += TILE) {
@@ -319,20 +408,23 @@ int pd_mixed_blocks_per_cu(int directional) {
                      : blocks_per_cu(pd_mixed_kernel<false>, kBlock, c[0]);
 }
 
+size_t traj_f32_floats(int64_t n, int with_a) { return (size_t)(with_a ? 7 : 3) * n + (n + kFlagSpan - 1) / kFlagSpan; }
+
 hipError_t launch_traj_to_f32(const TrajView& t, int with_a, float* trajf, hipStream_t s) {
   if (with_a && !(t.ax && t.ay && t.az)) return hipErrorInvalidValue;
-  traj_to_f32_kernel<<<(unsigned)((t.n + 255) / 256), 256, 0, s>>>(t, with_a, trajf);
+  traj_to_f32_kernel<<<(unsigned)((t.n + kFlagSpan - 1) / kFlagSpan), kFlagSpan, 0, s>>>(t, with_a, trajf);
   return hipGetLastError();
 }
 
 hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
                              int chunk, double* part, hipStream_t s) {
   dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  const int* const fl = reinterpret_cast<const int*>(trajf + 3 * t.n);  // staged without accelerations
   switch (es.ne) {
-    case 1: flux_mixed_kernel<1, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
-    case 2: flux_mixed_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
-    case 3: flux_mixed_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
-    case 4: flux_mixed_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, ns, es, chunk, part); break;
+    case 1: flux_mixed_kernel<1, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
+    case 2: flux_mixed_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
+    case 3: flux_mixed_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
+    case 4: flux_mixed_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -350,10 +442,11 @@ hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const d
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
                            int directional, int chunk, double* part, hipStream_t s) {
   dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+  const int* const fl = reinterpret_cast<const int*>(trajf + 7 * t.n);  // staged with accelerations
   if (directional)
-    pd_mixed_kernel<true><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, nrm_soa, ns, chunk, part);
+    pd_mixed_kernel<true><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, nrm_soa, ns, chunk, part);
   else
-    pd_mixed_kernel<false><<<grid, kBlock, 0, s>>>(t, trajf, obs_soa, nrm_soa, ns, chunk, part);
+    pd_mixed_kernel<false><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, nrm_soa, ns, chunk, part);
   return hipGetLastError();
 }
 

@@ -13,6 +13,11 @@
 
 namespace srs {
 
+// EXACT: every component of N1 as the reference's quotient (for points whose beam is not
+// along z, transverse_beam); else n1z correctly rounded (1 - beta.N1 and n1z - bz cancel
+// to ~1e-8 through it) and n1x, n1y by the reciprocal: beta_x n1x is ~1e-7 of 1, so
+// their last bit is ~1e-15 of 1 - beta.N1.
+template <bool EXACT = false>
 __device__ __forceinline__ double pd_term(double ox, double oy, double oz, double qx, double qy, double qz, double X,
                                           double Y, double Z, double bx, double by, double bz, double Ax, double Ay,
                                           double Az, int directional) {
@@ -20,10 +25,8 @@ __device__ __forceinline__ double pd_term(double ox, double oy, double oz, doubl
   double const d2 = rx * rx + ry * ry + rz * rz;
   double const D = __builtin_sqrt(d2);
   double const inv = 1.0 / D;
-  // n1z correctly rounded (1 - beta.N1 and n1z - bz cancel to ~1e-8 through it); n1x,
-  // n1y by the reciprocal: beta_x n1x is ~1e-7 of 1, so their last bit is ~1e-15 of
-  // 1 - beta.N1
-  double const n1x = rx * inv, n1y = ry * inv, n1z = div_rn(rz, D, inv);
+  double const n1x = EXACT ? div_rn(rx, D, inv) : rx * inv, n1y = EXACT ? div_rn(ry, D, inv) : ry * inv;
+  double const n1z = div_rn(rz, D, inv);
   double const ndot = n1x * qx + n1y * qy + n1z * qz;
   if (directional && ndot <= 0) return 0.0;
   double const den = 1 - (bx * n1x + by * n1y + bz * n1z);
@@ -44,6 +47,7 @@ __device__ __forceinline__ double pd_term(double ox, double oy, double oz, doubl
 //   dL/dbeta = -NA m + N (A.m) + 5 G val/b N
 //   dL/dN    = NA m + A (u.m) + beta (A.m) + G s/(b^5 d2) n + 5 G val/b beta
 //   dL/dR    = (I - N N^T) dL/dN / D - 2 G val/D N,   dL/dX = -dL/dR.
+template <bool EXACT = false>
 __device__ __forceinline__ void pd_term_adjoint(double G, double ox, double oy, double oz, double qx, double qy,
                                                 double qz, const double* X, const double* B, const double* A,
                                                 int directional, double* gX, double* gB, double* gA) {
@@ -51,9 +55,10 @@ __device__ __forceinline__ void pd_term_adjoint(double G, double ox, double oy, 
   double const d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
   double const D = __builtin_sqrt(d2);
   double const inv = 1.0 / D;
-  // N and b rounded exactly as in pd_term: b = 1 - beta.N cancels to ~1e-8, and its
-  // rounding enters the derivative through 1/b^6
-  double const N[3] = {r[0] * inv, r[1] * inv, div_rn(r[2], D, inv)};
+  // N and b rounded exactly as in pd_term<EXACT>: b = 1 - beta.N cancels to ~1e-8, and
+  // its rounding enters the derivative through 1/b^6
+  double const N[3] = {EXACT ? div_rn(r[0], D, inv) : r[0] * inv, EXACT ? div_rn(r[1], D, inv) : r[1] * inv,
+                       div_rn(r[2], D, inv)};
   double const nd = N[0] * qx + N[1] * qy + N[2] * qz;
   if (directional && nd <= 0) return;
   double const b = 1 - (B[0] * N[0] + B[1] * N[1] + B[2] * N[2]);

@@ -284,7 +284,8 @@ struct RK4Ctx {
   }
 };
 
-__device__ __forceinline__ void store_point(BatchTraj const& bt, int64_t k, const double* x, const double* d) {
+// Stores point k; returns whether it does not travel along z (transverse_beam).
+__device__ __forceinline__ bool store_point(BatchTraj const& bt, int64_t k, const double* x, const double* d) {
   bt.x[k] = x[0];
   bt.y[k] = x[2];
   bt.z[k] = x[4];
@@ -300,6 +301,7 @@ __device__ __forceinline__ void store_point(BatchTraj const& bt, int64_t k, cons
     bt.ay[k] = div_rn(d[3], kC, kInvC);
     bt.az[k] = div_rn(d[5], kC, kInvC);
   }
+  return transverse_beam(bx, by);
 }
 
 }  // namespace srs

@@ -927,18 +927,14 @@ void OSCARSSR::CalculateSpectrum(Vec3 const& obs, SpectrumContainer& S, int npar
     std::vector<srs_particle> parts;
     parts.reserve(nparticles);
     double q = 0, cur = 0;
-    bool same = true;
     for (int i = 0; i != nparticles; ++i) {
       SetNewParticle();
       parts.push_back(device_particle(particle_));
       if (i == 0) {
-        q = particle_.q;
+        q = particle_.q;  // the call's defaults; every particle carries its own
         cur = particle_.current;
-      } else if (particle_.q != q || particle_.current != cur) {
-        same = false;
       }
     }
-    if (!same) throw std::invalid_argument("batched spectrum needs all beams to share charge and current (use batched=0)");
     if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
     std::vector<srs_field> und;
     device_fields(und);
@@ -1019,6 +1015,8 @@ srs_particle OSCARSSR::device_particle(Particle const& p) const {
   d.t0 = p.t0;
   d.ct_start = ct_start_;
   d.ct_stop = ct_stop_;
+  d.charge = p.q;  // each particle's own prefactors, as the reference's per-particle calls have them
+  d.current = p.current;
   d.npoints = (int64_t)npoints_;
   return d;
 }
@@ -1078,19 +1076,15 @@ void OSCARSSR::CalculateFlux(SurfacePoints const& surf, double energy_ev, Scalar
     std::vector<srs_particle> parts;
     parts.reserve(np);
     double q = 0, cur = 0;
-    bool same = true;
     for (int i = 0; i != np; ++i) {
       SetNewParticle();
       add_points(surf, C, 3);
       parts.push_back(device_particle(particle_));
       if (i == 0) {
-        q = particle_.q;
+        q = particle_.q;  // the call's defaults; every particle carries its own
         cur = particle_.current;
-      } else if (particle_.q != q || particle_.current != cur) {
-        same = false;
       }
     }
-    if (!same) throw std::invalid_argument("batched flux needs all beams to share charge and current (use batched=0)");
     if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
     std::vector<srs_field> und;
     device_fields(und);
@@ -1138,20 +1132,15 @@ void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer&
     std::vector<srs_particle> parts;
     parts.reserve(np);
     double q = 0, cur = 0;
-    bool same = true;
     for (int i = 0; i != np; ++i) {
       SetNewParticle();
       add_points(surf, C, 3);
       parts.push_back(device_particle(particle_));
       if (i == 0) {
-        q = particle_.q;
+        q = particle_.q;  // the call's defaults; every particle carries its own
         cur = particle_.current;
-      } else if (particle_.q != q || particle_.current != cur) {
-        same = false;
       }
     }
-    if (!same)
-      throw std::invalid_argument("batched power density needs all beams to share charge and current (use batched=0)");
     if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
     std::vector<srs_field> und;
     device_fields(und);

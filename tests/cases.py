@@ -134,3 +134,104 @@ def oracle_setup(O, name, npoints=None):
     c0, c1, n = ct(a)
     t, dt = O.trajectory(F, p, c0, c1, npoints or n)
     return F, p, t, dt
+
+
+def c3_beam(a, energy=None, x0=None, current=None):
+    """Keyword arguments of oracle.random_electron for a case's (first) beam."""
+    b = beam(a)
+    return dict(energy_gev=energy if energy is not None else b["energy"], sigma_e_gev=float(a.get("sigE", 0)),
+                x0=x0 if x0 is not None else b["x0"], d0=b["d0"], hdir=vec(a, "hdir", [1, 0, 0]),
+                beta=vec(a, "beta", [0, 0]), emit=vec(a, "emit", [0, 0]), lattice_ref=vec(a, "latref", [0, 0, 0]),
+                current=current if current is not None else b["current"])
+
+
+def beams(a):
+    """(random_electron kwargs, weight) of every beam of a case: the first, and beam2 when
+    present (same sigma settings; ref_harness.cc's beam2 key)."""
+    out = [(c3_beam(a), float(a.get("weight1", 1)))]
+    b2 = vec(a, "beam2")
+    if b2:
+        kw = c3_beam(a, energy=b2[6], x0=b2[0:3], current=b2[7])
+        kw["d0"] = b2[3:6]
+        out.append((kw, float(a.get("weight2", 1))))
+    return out
+
+
+def pick_beam(weights, u):
+    """TParticleBeamContainer::GetRandomBeamIndexByWeight (TParticleBeamContainer.cc:109-142):
+    the first beam whose running weight sum exceeds u * total."""
+    sums = np.cumsum(weights)
+    r = u * sums[-1]
+    for i, s in enumerate(sums):
+        if r < s:
+            return i
+    raise ValueError("random weight out of range")
+
+
+def rotate_xyz(v, r):
+    return _rotate_xyz(v, r)
+
+
+def corners(a):
+    """x0x1x2 corners of a case rotated (RotateSelfXYZ) then translated, as calculate_*_rectangle does."""
+    c = vec(a, "x0x1x2")
+    rot, tr = vec(a, "rot", [0, 0, 0]), vec(a, "trans", [0, 0, 0])
+    out = []
+    for k in range(3):
+        q = _rotate_xyz(c[3 * k:3 * k + 3], rot)
+        out.append([q[i] + tr[i] for i in range(3)])
+    return out
+
+
+def points_transformed(a, flip):
+    """Point list of a case as calculate_flux / calculate_power_density build it: points and
+    normals rotated by prot, normals reversed for pnormal -1 (power density), points translated."""
+    xyz, nrm = points(a)
+    rot, tr = vec(a, "prot", [0, 0, 0]), vec(a, "ptrans", [0, 0, 0])
+    nd = int(float(a.get("pnormal", 0)))
+    X, N = [], []
+    for x, n in zip(xyz, nrm):
+        x = _rotate_xyz(list(x), rot)
+        n = _rotate_xyz(list(n), rot)
+        if flip and nd == -1:
+            n = [-v for v in n]
+        X.append([x[i] + tr[i] for i in range(3)])
+        N.append(n)
+    return np.array(X), np.array(N)
+
+
+def make_sr(name, beams_too=True):
+    """An sr engine set up as the harness set up the reference for a case: field elements
+    in the harness's order, field maps, the beam (or both weighted beams), the ct range
+    and the trajectory point count."""
+    from srsoriginal_amd.sr import sr
+    a = args(name)
+    osr = sr()
+    for u in undulators(a):
+        if u.get("kind") == 1:
+            osr.add_bfield_gaussian(bfield=u["raw_field"], sigma=u["width"], rotations=u["rotations"],
+                                    translation=u["center"])
+        elif u.get("kind") == 2:
+            osr.add_bfield_uniform(bfield=u["raw_field"], width=u["width"], rotations=u["rotations"],
+                                   translation=u["center"])
+        else:
+            osr.add_bfield_undulator(bfield=u["field"], period=u["period"], nperiods=u["nperiods"], phase=u["phase"],
+                                     taper=u["taper"], translation=u["center"])
+    add_field_maps(osr, a)
+    bs = beams(a)
+    if len(bs) == 1 or not beams_too:
+        b = beam(a)
+        osr.set_particle_beam(type="electron", name="beam", energy_GeV=b["energy"], x0=b["x0"], d0=b["d0"],
+                              current=b["current"])
+    else:
+        for k, (b, w) in enumerate(bs):
+            osr.add_particle_beam(type="electron", name="b%d" % k, energy_GeV=b["energy_gev"], x0=b["x0"], d0=b["d0"],
+                                  current=b["current"], weight=w, sigma_energy_GeV=b["sigma_e_gev"], beta=b["beta"],
+                                  emittance=b["emit"], horizontal_direction=b["hdir"],
+                                  lattice_reference=b["lattice_ref"])
+        # the harness sets the first beam's ideal particle (no random draw) before any calculation
+        osr.set_new_particle(beam="b0", particle="ideal")
+    c0, c1, n = ct(a)
+    osr.set_ctstartstop(c0, c1)
+    osr.set_npoints_trajectory(n)
+    return osr

@@ -301,6 +301,168 @@ def multipd_c3_small():
     a, dt = run("pd", args)
     return args, {"container": container(a)}, dt
 
+# ---- fixtures for paths the first set left unpinned ------------------------------------
+# Beam along x (undulator period along x, screen in the YZ plane at x = 30 m): the
+# cancellation beta - n moves to the x component (every other case has it in z).
+UND_X = "0,1,0,0.05,0,0,41"
+BEAM_X = ["x0=-1.5,0,0", "d0=1,0,0", "energy=3", "current=0.5"]
+# Beam at 36.87 deg in the x-z plane (d0 = (0.6, 0, 0.8)), undulator period along d0.
+UND_DIAG = "0,1,0,0.03,0,0.04,41"
+BEAM_DIAG = ["x0=-0.9,0,-1.2", "d0=0.6,0,0.8", "energy=3", "current=0.5"]
+
+
+def _diag_points(n=7, w=0.01):
+    """Observation points on a square perpendicular to d0 = (0.6, 0, 0.8) at 30 m, normal d0."""
+    u = np.array([0.6, 0.0, 0.8])
+    e1 = np.array([0.8, 0.0, -0.6])
+    e2 = np.array([0.0, 1.0, 0.0])
+    pts = []
+    for a in np.linspace(-w / 2, w / 2, n):
+        for b in np.linspace(-w / 2, w / 2, n):
+            x = 30.0 * u + a * e1 + b * e2
+            pts += list(x) + list(u)
+    return ",".join("%.17g" % v for v in pts)
+
+
+@case
+def flux_beam_x():
+    e = first_harmonic_energy()
+    args = ["und1=" + UND_X] + BEAM_X + ["ct=0,3", "npoints=4001", "plane=YZ", "width=0.02,0.02", "nx=15,13",
+                                         "trans=30,0,0", "energy_ev=%.17g" % e, "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a), "energy_ev": np.array(e)}, dt
+
+
+@case
+def pd_beam_x():
+    args = ["und1=" + UND_X] + BEAM_X + ["ct=0,3", "npoints=4001", "plane=YZ", "width=0.04,0.04", "nx=13,11",
+                                         "trans=30,0,0", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def spectrum_beam_x():
+    args = ["und1=" + UND_X] + BEAM_X + ["ct=0,3", "npoints=4001", "ne=300", "erange=100,1500", "obs=30,0.0004,-0.0002",
+                                         "nthreads=1"]
+    a, dt = run("spectrum", args)
+    return args, {"spectrum": a[1:].reshape(-1, 2)}, dt
+
+
+@case
+def flux_beam_diagonal():
+    args = ["und1=" + UND_DIAG] + BEAM_DIAG + ["ct=0,3", "npoints=4001", "points=" + _diag_points(),
+                                               "energy_ev=143.5", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_beam_diagonal():
+    args = ["und1=" + UND_DIAG] + BEAM_DIAG + ["ct=0,3", "npoints=4001", "points=" + _diag_points(9, 0.04),
+                                               "pnormal=0", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def uniforms_seed4321():
+    args = ["und1=" + UND_C0] + BEAM + ["seed=4321", "count=64"]
+    a, dt = run("uniforms", args)
+    return args, {"uniforms": a[1:]}, dt
+
+
+# Two weighted beams (weights 1 : 3), the second at another energy and offset, both with the
+# C3 beam's sigma settings: SetNewParticle() draws the beam by weight, then the particle.
+BEAMS_TWO = BEAM_C3 + ["beam2=0.0002,0,-1.5,0,0,1,2.5,0.4", "weight1=1", "weight2=3"]
+
+
+@case
+def beams_two_weighted():
+    args = ["und1=" + UND_C0] + BEAMS_TWO + ["seed=99", "count=48"]
+    a, dt = run("beam", args)
+    return args, {"particles": a[1:].reshape(-1, 9)}, dt
+
+
+@case
+def multiflux_two_beams():
+    args = ["und1=" + UND_C0] + BEAMS_TWO + ["seed=31", "ct=0,3", "npoints=3001", "plane=XY", "width=0.01,0.01",
+                                              "nx=9,9", "trans=0,0,30", "energy_ev=143.5", "nparticles=4", "nthreads=1"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+_PTS = "0,0,0,0,0,1, 0.002,-0.001,0,0,0,1, -0.003,0.002,0.5,0,0,-1, 0.001,0.003,-0.4,1,0,0, 0,-0.002,0.2,0,0.6,0.8"
+
+
+@case
+def flux_points_rotated():
+    # calculate_flux(points=..., rotations=..., translation=...): points and normals rotated, points translated
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=4001", "points=" + _PTS.replace(" ", ""),
+                                        "prot=0.02,-0.03,0.4", "ptrans=0.001,-0.0005,30", "energy_ev=143.5",
+                                        "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_points_rotated_flip():
+    # calculate_power_density(points=..., normal=-1): normals rotated then reversed, directional
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=4001", "points=" + _PTS.replace(" ", ""),
+                                        "prot=0.02,-0.03,0.4", "ptrans=0.001,-0.0005,30", "pnormal=-1", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_points_rotated_undirected():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=4001", "points=" + _PTS.replace(" ", ""),
+                                        "prot=0.02,-0.03,0.4", "ptrans=0.001,-0.0005,30", "pnormal=0", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+# x0x1x2 rectangle: corners rotated then translated (TSurfacePoints_Rectangle::Init(NX1, NX2, X0, X1, X2, Normal))
+_CORNERS = "-0.004,-0.003,0,0.005,-0.003,0.001,-0.004,0.004,-0.0005"
+
+
+@case
+def flux_x0x1x2():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=4001", "x0x1x2=" + _CORNERS, "rot=0.05,0.02,0.3",
+                                        "trans=0.0005,0,30", "nx=7,5", "energy_ev=143.5", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_x0x1x2_normal():
+    args = ["und1=" + UND_C0] + BEAM + ["ct=0,3", "npoints=4001", "x0x1x2=" + _CORNERS, "rot=0.05,0.02,0.3",
+                                        "trans=0.0005,0,30", "nx=7,5", "normal=-1", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+# Tapered, off-centre undulator (TField3D_IdealUndulator.cc:50-139): taper 0.03 per metre,
+# centre (0.0001, 0, 0.12), phase 0.4
+UND_TAPER = "0,1,0,0,0,0.05,41,0.4,0.03,0.0001,0,0.12"
+
+
+@case
+def traj_taper_center():
+    args = ["und1=" + UND_TAPER] + BEAM + ["ct=0,3", "npoints=4001"]
+    a, dt = run("traj", args)
+    n = int(a[0])
+    return args, {"n": np.array(n), "dt": np.array(a[1]), "traj": a[2:].reshape(9, n)}, dt
+
+
+@case
+def flux_taper_center():
+    args = ["und1=" + UND_TAPER] + BEAM + ["ct=0,3", "npoints=4001", "plane=XY", "width=0.01,0.01", "nx=11,9",
+                                           "trans=0,0,30", "energy_ev=143.5", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
 def _bytes(path):
     with open(path, "rb") as f:
         return np.frombuffer(f.read(), dtype=np.uint8).copy()

@@ -50,7 +50,7 @@ def test_flux_through_field_maps_routes_match_reference():
     row = dict(x0=st["x0"], beta0=st["beta0"], qmg=st["qmg"], t0=st["t0"], ct_start=c0, ct_stop=c1, npoints=n)
     q, cur = osr._e.particle_charge_current()
     xyz = _sr.Surface.rectangle("XY", 9, 9, 0.006, 0.006, [0, 0, 0], [0, 0, 30], 0).xyz
-    for prec, tol in ((abi.FP64, 1e-9), (abi.MIXED, 1e-5)):
+    for prec, tol in ((abi.FP64, 1e-10), (abi.MIXED, 1e-5)):
         got = abi.batch_flux(osr._e.undulators(), abi.particles([row]), q, cur, xyz, [400.0], 1.0, prec, 0)[0]
         err = normwise(got, ref)
         say("  device RK4 batch (%s) normwise %.3e" % ("fp64" if prec == abi.FP64 else "mixed", err))

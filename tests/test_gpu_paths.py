@@ -1,0 +1,185 @@
+"""GPU parity of the sr entry points on paths the reference fixtures now cover:
+point lists with rotation / translation / reversed normals, rectangles from three
+corners, a tapered off-centre undulator, beams that do not travel along z, and the
+multi-particle loop over two weighted beams.  FP64 against the reference at 1e-10
+(normwise, max|d| / max|ref|); mixed precision against the FP64 GPU result at 1e-5."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from conftest import ROOT, golden
+from gpu_util import normwise, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _vals(out, n):
+    return np.array([v for _, v in out[:n]])
+
+
+def _points_arg(a):
+    xyz, nrm = cases.points(a)
+    return [[list(x), list(nn)] for x, nn in zip(xyz, nrm)]
+
+
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+def test_flux_point_list_rotated_translated(prec):
+    name = "flux_points_rotated"
+    a = cases.args(name)
+    osr = cases.make_sr(name)
+    ref = golden(name)["container"]
+    n = len(cases.points(a)[0])
+    out = osr.calculate_flux(energy_eV=float(a["energy_ev"]), points=_points_arg(a), rotations=cases.vec(a, "prot"),
+                             translation=cases.vec(a, "ptrans"), gpu=1, precision=prec)
+    assert np.array_equal(np.array([x for x, _ in out[:n]]), ref[:n, :3])
+    err = normwise(_vals(out, n), ref[:n, 3])
+    say("calculate_flux(points, rotations, translation, gpu=1, %s): normwise %.3e" % (prec, err))
+    assert err <= (TOL if prec == "double" else 1e-5)
+
+
+@pytest.mark.parametrize("name,normal", [("pd_points_rotated_flip", -1), ("pd_points_rotated_undirected", 0)])
+def test_power_density_point_list_normals(name, normal):
+    a = cases.args(name)
+    osr = cases.make_sr(name)
+    ref = golden(name)["container"]
+    n = len(cases.points(a)[0])
+    for prec in ("double", "mixed"):
+        out = osr.calculate_power_density(points=_points_arg(a), normal=normal, rotations=cases.vec(a, "prot"),
+                                          translation=cases.vec(a, "ptrans"), gpu=1, precision=prec)
+        err = normwise(_vals(out, n), ref[:n, 3])
+        say("calculate_power_density(points, normal=%d, gpu=1, %s): normwise %.3e" % (normal, prec, err))
+        assert err <= (TOL if prec == "double" else 1e-5)
+
+
+def test_three_corner_rectangles():
+    for name in ("flux_x0x1x2", "pd_x0x1x2_normal"):
+        a = cases.args(name)
+        osr = cases.make_sr(name)
+        ref = golden(name)["container"]
+        nx = [int(v) for v in cases.vec(a, "nx")]
+        n = nx[0] * nx[1]
+        c = cases.vec(a, "x0x1x2")
+        kw = dict(npoints=nx, x0x1x2=[c[0:3], c[3:6], c[6:9]], rotations=cases.vec(a, "rot"),
+                  translation=cases.vec(a, "trans"), normal=int(float(a.get("normal", 0))), gpu=1)
+        if name.startswith("flux"):
+            out = osr.calculate_flux_rectangle(energy_eV=float(a["energy_ev"]), **kw)
+        else:
+            out = osr.calculate_power_density_rectangle(**kw)
+        assert np.array_equal(np.array([x for x, _ in out[:n]])[:, :2], ref[:n, :2])
+        err = normwise(_vals(out, n), ref[:n, 3])
+        say("%s x0x1x2 gpu=1: normwise %.3e" % (name, err))
+        assert err <= TOL
+
+
+def test_tapered_offcentre_undulator():
+    name = "flux_taper_center"
+    a = cases.args(name)
+    osr = cases.make_sr(name)
+    ref = golden(name)["container"]
+    r = cases.rect(a)
+    n = r["nx1"] * r["nx2"]
+    kw = dict(energy_eV=float(a["energy_ev"]), npoints=[r["nx1"], r["nx2"]], plane=r["plane"],
+              width=[r["w1"], r["w2"]], translation=r["trans"], gpu=1)
+    err = normwise(_vals(osr.calculate_flux_rectangle(**kw), n), ref[:n, 3])
+    say("tapered off-centre undulator flux gpu=1: normwise %.3e" % err)
+    assert err <= TOL
+    # the device RK4 integrates the same tapered field
+    host = np.array(cases.make_sr("traj_taper_center").calculate_trajectory())
+    osr = cases.make_sr("traj_taper_center")
+    osr.calculate_trajectory(gpu=1)
+    td, dtd = osr.trajectory_array()
+    g = golden("traj_taper_center")["traj"]
+    for k, scale in ((0, 1e-5), (2, 1.0), (3, 1e-3), (5, 1.0)):
+        dev = np.max(np.abs(td[k] - g[k])) / scale
+        say("  device RK4 component %d deviation / scale %.3e" % (k, dev))
+        assert dev < 1e-12
+    assert len(host) == g.shape[1]
+
+
+@pytest.mark.parametrize("name", ["flux_beam_x", "pd_beam_x", "spectrum_beam_x"])
+def test_beam_along_x(name):
+    """beta_x - n_x cancels to ~1e-8 here: the kernels must round n_x as the reference does
+    (FP64), and the mixed form must not stage beta_x in FP32 for the cancelling difference."""
+    a = cases.args(name)
+    osr = cases.make_sr(name)
+    res = {}
+    for prec in ("double", "mixed"):
+        if name.startswith("spectrum"):
+            ref = golden(name)["spectrum"]
+            er = cases.vec(a, "erange")
+            out = np.array(osr.calculate_spectrum(obs=cases.vec(a, "obs"), npoints=int(a["ne"]),
+                                                  energy_range_eV=er, gpu=1, precision=prec))
+            assert np.array_equal(out[:, 0], ref[:, 0])
+            got, want = out[:, 1], ref[:, 1]
+        else:
+            ref = golden(name)["container"]
+            r = cases.rect(a)
+            n = r["nx1"] * r["nx2"]
+            kw = dict(npoints=[r["nx1"], r["nx2"]], plane=r["plane"], width=[r["w1"], r["w2"]],
+                      translation=r["trans"], gpu=1, precision=prec)
+            if name.startswith("flux"):
+                out = osr.calculate_flux_rectangle(energy_eV=float(a["energy_ev"]), **kw)
+            else:
+                out = osr.calculate_power_density_rectangle(**kw)
+            got, want = _vals(out, n), ref[:n, 3]
+        res[prec] = got
+        err = normwise(got, want)
+        say("%s %s: normwise vs reference %.3e" % (name, prec, err))
+        assert err <= (TOL if prec == "double" else 1e-5)
+    assert normwise(res["mixed"], res["double"]) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["flux_beam_diagonal", "pd_beam_diagonal"])
+def test_beam_diagonal(name):
+    a = cases.args(name)
+    osr = cases.make_sr(name)
+    ref = golden(name)["container"]
+    n = len(cases.points(a)[0])
+    for prec in ("double", "mixed"):
+        if name.startswith("flux"):
+            out = osr.calculate_flux(energy_eV=float(a["energy_ev"]), points=_points_arg(a), gpu=1, precision=prec)
+        else:
+            out = osr.calculate_power_density(points=_points_arg(a), gpu=1, precision=prec)
+        err = normwise(_vals(out, n), ref[:n, 3])
+        say("%s %s: normwise vs reference %.3e" % (name, prec, err))
+        assert err <= (TOL if prec == "double" else 1e-5)
+
+
+_TWO_BEAMS = r"""
+import json, sys
+sys.path.insert(0, %(root)r); sys.path.insert(0, %(tests)r)
+import cases
+name = "multiflux_two_beams"
+a = cases.args(name)
+osr = cases.make_sr(name)
+osr.set_seed(int(a["seed"]))
+r = cases.rect(a)
+out = osr.calculate_flux_rectangle(energy_eV=float(a["energy_ev"]), npoints=[r["nx1"], r["nx2"]], plane=r["plane"],
+                                   width=[r["w1"], r["w2"]], translation=r["trans"], nparticles=int(a["nparticles"]),
+                                   gpu=1, batched=int(sys.argv[1]))
+print(json.dumps([v for _, v in out]))
+"""
+
+
+@pytest.mark.parametrize("batched", [0, 1])
+def test_multi_particle_two_weighted_beams(batched):
+    """nparticles=4 drawn from two weighted beams after set_seed (fresh interpreter: the
+    normal distribution's cached value survives set_seed, TRandomA.cc:42-48)."""
+    say("calculate_flux_rectangle(nparticles=4, gpu=1, batched=%d) over two weighted beams" % batched)
+    code = _TWO_BEAMS % dict(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code, str(batched)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    vals = np.array(json.loads(r.stdout.strip().splitlines()[-1]))
+    ref = golden("multiflux_two_beams")["container"][:, 3]
+    n = 81
+    assert len(vals) == n + 4 * n and np.all(vals[n:] == 0)
+    err = normwise(vals[:n], ref[:n])
+    say("  normwise vs reference %.3e" % err)
+    assert err <= TOL

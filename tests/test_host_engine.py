@@ -21,26 +21,7 @@ def _digest(a):
 
 
 def _engine(name):
-    from srsoriginal_amd.sr import sr
-    a = cases.args(name)
-    osr = sr()
-    for u in cases.undulators(a):
-        if u.get("kind") == 1:
-            osr.add_bfield_gaussian(bfield=u["raw_field"], sigma=u["width"], rotations=u["rotations"],
-                                    translation=u["center"])
-        elif u.get("kind") == 2:
-            osr.add_bfield_uniform(bfield=u["raw_field"], width=u["width"], rotations=u["rotations"],
-                                   translation=u["center"])
-        else:
-            osr.add_bfield_undulator(bfield=u["field"], period=u["period"], nperiods=u["nperiods"], phase=u["phase"])
-    cases.add_field_maps(osr, a)
-    b = cases.beam(a)
-    osr.set_particle_beam(type="electron", name="beam", energy_GeV=b["energy"], x0=b["x0"], d0=b["d0"],
-                          current=b["current"])
-    c0, c1, n = cases.ct(a)
-    osr.set_ctstartstop(c0, c1)
-    osr.set_npoints_trajectory(n)
-    return osr
+    return cases.make_sr(name)
 
 
 @pytest.mark.parametrize("name", ["traj_c0_20001", "traj_epu_40001"])
@@ -85,7 +66,7 @@ def test_surfaces_match_oracle(oracle):
         assert np.array_equal(S.xyz, xyz) and np.array_equal(S.normals, nrm)
 
 
-def test_random_particles_match_reference():
+def test_random_particles_match_reference(tmp_path):
     # fresh interpreter: the global normal distribution caches its second value
     code = r"""
 import sys, numpy as np
@@ -105,11 +86,50 @@ for _ in range(32):
     rows.append(osr.get_particle_x0() + osr.get_particle_beta0())
 np.save(sys.argv[1], np.array(rows))
 """ % (ROOT, os.path.join(ROOT, "tests"))
-    out = os.path.join(ROOT, "_work", "beam_rows.npy") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp/beam_rows.npy"
+    out = str(tmp_path / "beam_rows.npy")
     subprocess.run([sys.executable, "-c", code, out], check=True)
     rows = np.load(out)
     ref = golden("beam_c3_seed1234")["particles"]
     assert np.array_equal(rows, ref[:, :6])
+
+
+def test_two_weighted_beams_match_reference(tmp_path):
+    """sr.add_particle_beam twice (weights 1 and 3), set_seed, set_new_particle(): the beam is
+    drawn by weight (TParticleBeamContainer.cc:109-142) and then the particle, as the
+    reference does after set_seed (fresh interpreter, as above); rand() continues the same
+    uniform stream (TRandomA.cc:61-64)."""
+    code = r"""
+import sys, numpy as np
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import cases
+from srsoriginal_amd.sr import sr
+a = cases.args("beams_two_weighted")
+(b1, w1), (b2, w2) = cases.beams(a)
+osr = sr()
+osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.05], nperiods=41)
+for name, b, w in (("b1", b1, w1), ("b2", b2, w2)):
+    osr.add_particle_beam(type="electron", name=name, energy_GeV=b["energy_gev"], x0=b["x0"], d0=b["d0"],
+                          current=b["current"], weight=w, sigma_energy_GeV=b["sigma_e_gev"], beta=b["beta"],
+                          emittance=b["emit"], horizontal_direction=b["hdir"], lattice_reference=b["lattice_ref"])
+osr.set_seed(int(a["seed"]))
+rows = []
+for _ in range(48):
+    osr.set_new_particle()
+    rows.append(osr.get_particle_x0() + osr.get_particle_beta0() + [osr.get_particle_e0()])
+osr.set_seed(int(cases.args("uniforms_seed4321")["seed"]))
+u = [osr.rand() for _ in range(64)]
+np.save(sys.argv[1], np.array(rows))
+np.save(sys.argv[2], np.array(u))
+""" % (ROOT, os.path.join(ROOT, "tests"))
+    out, uout = str(tmp_path / "rows.npy"), str(tmp_path / "u.npy")
+    subprocess.run([sys.executable, "-c", code, out, uout], check=True)
+    rows = np.load(out)
+    ref = golden("beams_two_weighted")["particles"]
+    assert np.array_equal(rows, ref[:, :7])
+    e0 = ref[:, 6]  # both beams drawn: energies near 2.5 and near 3 GeV
+    assert np.any(np.abs(e0 - 2.5) < 0.05) and np.any(np.abs(e0 - 3.0) < 0.05)
+    # the uniform stream after another set_seed, without the normals' cache mattering
+    assert np.array_equal(np.load(uout), golden("uniforms_seed4321")["uniforms"])
 
 
 def test_total_power_matches_reference_and_oracle(oracle):
@@ -205,3 +225,25 @@ def test_field_validation():
         osr.add_bfield_gaussian(bfield=[0, 1, 0], sigma=[1, 2])
     with pytest.raises(ValueError, match="Incorrect format in 'bfield'"):
         osr.add_bfield_uniform(bfield=[0, 1])
+
+
+def test_tapered_offcentre_undulator_trajectory_bitwise():
+    # taper, centre and phase of TField3D_IdealUndulator (.cc:50-139) through add_bfield_undulator
+    osr = _engine("traj_taper_center")
+    osr.calculate_trajectory()
+    t, dt = osr.trajectory_array()
+    g = golden("traj_taper_center")
+    assert dt == float(g["dt"])
+    assert np.array_equal(t[:6], g["traj"][:6]) and np.array_equal(t[6:, 1:], g["traj"][6:, 1:])
+
+
+def test_three_corner_rectangle_matches_oracle(oracle):
+    from srsoriginal_amd.lib import _sr
+    for name in ("flux_x0x1x2", "pd_x0x1x2_normal"):
+        a = cases.args(name)
+        n = cases.vec(a, "nx")
+        x0, x1, x2 = cases.corners(a)
+        normal = int(float(a.get("normal", 0)))
+        S = _sr.Surface.rectangle3(int(n[0]), int(n[1]), x0, x1, x2, normal)
+        xyz, nrm, x12 = oracle.rect3_surface(int(n[0]), int(n[1]), x0, x1, x2, normal)
+        assert np.array_equal(S.xyz, xyz) and np.array_equal(S.normals, nrm)

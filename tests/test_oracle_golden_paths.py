@@ -1,0 +1,179 @@
+"""Pins the CPU oracle against golden vectors of the compiled reference engine for
+paths the first fixture set left open: the uniform random stream and the weighted
+pick between two beams, beams that do not travel along z, point lists with
+rotation / translation / reversed normals, rectangles given by three corners, and
+a tapered, off-centre undulator.  Bit-for-bit equality throughout."""
+import numpy as np
+
+import cases
+from conftest import golden
+
+
+def test_uniform_stream_bitwise(oracle):
+    # TRandomA::Uniform (TRandomA.cc:61-64) after set_seed
+    a = cases.args("uniforms_seed4321")
+    oracle.rng_seed(int(a["seed"]), reset_normal=True)
+    got = np.array([oracle.rng_uniform() for _ in range(int(a["count"]))])
+    ref = golden("uniforms_seed4321")["uniforms"]
+    assert np.array_equal(got, ref)
+    assert np.all((ref >= 0) & (ref < 1)) and len(np.unique(ref)) == len(ref)
+
+
+def _two_beam_particles(oracle, a, count):
+    bs = cases.beams(a)
+    w = [wt for _, wt in bs]
+    out = []
+    for _ in range(count):
+        k = cases.pick_beam(w, oracle.rng_uniform())
+        out.append((k, oracle.random_electron(**bs[k][0])))
+    return out
+
+
+def test_two_beam_weighted_pick_bitwise(oracle):
+    # SetNewParticle(): GetRandomBeamIndexByWeight (TParticleBeamContainer.cc:109-142), one
+    # uniform per particle, then that beam's normal draws (TParticleBeam.cc:228-265)
+    name = "beams_two_weighted"
+    a = cases.args(name)
+    ref = golden(name)["particles"]
+    oracle.rng_seed(int(a["seed"]), reset_normal=True)
+    parts = _two_beam_particles(oracle, a, len(ref))
+    picks = np.array([k for k, _ in parts])
+    for (k, p), row in zip(parts, ref):
+        got = list(p.x0) + list(p.beta0) + [p.gamma, p.qmg]
+        assert np.array_equal(np.array(got), row[[0, 1, 2, 3, 4, 5, 7, 8]])
+    # both beams drawn, in about the 1 : 3 proportion of the weights
+    assert 0.1 < np.mean(picks == 0) < 0.45
+    # a plausible slip (weights taken in reverse order) draws another beam sequence
+    oracle.rng_seed(int(a["seed"]), reset_normal=True)
+    w = [wt for _, wt in cases.beams(a)][::-1]
+    flipped = np.array([cases.pick_beam(w, oracle.rng_uniform()) for _ in range(len(ref))])
+    assert not np.array_equal(flipped, picks)
+
+
+def test_multi_particle_two_beams_flux_bitwise(oracle):
+    # nparticles loop over two weighted beams (OSCARSSR.cc:2465-2479), Kahan sum x 1/NP
+    name = "multiflux_two_beams"
+    a = cases.args(name)
+    F = oracle.FieldSet(cases.undulators(a))
+    r = cases.rect(a)
+    xyz, nrm, x12 = oracle.rect_surface(r["plane"], r["nx1"], r["nx2"], r["w1"], r["w2"], r["rot"], r["trans"],
+                                        r["normal"])
+    oracle.rng_seed(int(a["seed"]), reset_normal=True)
+    npart = int(a["nparticles"])
+    s = np.zeros(len(xyz)); comp = np.zeros(len(xyz))
+    c0, c1, n = cases.ct(a)
+    for k, p in _two_beam_particles(oracle, a, npart):
+        t, dt = oracle.trajectory(F, p, c0, c1, n)
+        oracle.kahan_add(s, comp, oracle.flux(t, dt, xyz, float(a["energy_ev"]), p.q, p.current, weight=1.0 / npart))
+    c = golden(name)["container"]
+    assert np.array_equal(s, c[:len(xyz), 3])
+
+
+def _rect(oracle, a):
+    r = cases.rect(a)
+    return oracle.rect_surface(r["plane"], r["nx1"], r["nx2"], r["w1"], r["w2"], r["rot"], r["trans"], r["normal"])
+
+
+def test_beam_along_x_bitwise(oracle):
+    # beta_x - n_x cancels here (1 - beta ~ 1e-8): the unit vector's x component must round
+    # as the reference's R.UnitVector() does
+    for name in ("flux_beam_x", "pd_beam_x", "spectrum_beam_x"):
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        a = cases.args(name)
+        assert abs(t[3, 0] - p.beta0[0]) < 1e-15 and p.beta0[0] > 0.9999999  # beta = (beta0 c)/c
+        if name.startswith("spectrum"):
+            s = golden(name)["spectrum"]
+            f = oracle.spectrum(t, dt, cases.vec(a, "obs"), s[:, 0], p.q, p.current)
+            assert np.array_equal(f, s[:, 1])
+            continue
+        xyz, nrm, x12 = _rect(oracle, a)
+        c = golden(name)["container"]
+        if name.startswith("flux"):
+            v = oracle.flux(t, dt, xyz, float(a["energy_ev"]), p.q, p.current, nthreads=2)
+        else:
+            v = oracle.power_density(t, dt, xyz, nrm, False, p.q, p.current, nthreads=2)
+        assert np.array_equal(v, c[:len(xyz), 3])
+
+
+def test_beam_diagonal_bitwise(oracle):
+    for name in ("flux_beam_diagonal", "pd_beam_diagonal"):
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        a = cases.args(name)
+        xyz, nrm = cases.points_transformed(a, flip=name.startswith("pd"))
+        c = golden(name)["container"]
+        assert np.array_equal(c[:len(xyz), :3], xyz)
+        if name.startswith("flux"):
+            v = oracle.flux(t, dt, xyz, float(a["energy_ev"]), p.q, p.current)
+        else:
+            v = oracle.power_density(t, dt, xyz, nrm, False, p.q, p.current)
+        assert np.array_equal(v, c[:len(xyz), 3])
+
+
+def test_point_lists_rotated_translated_bitwise(oracle):
+    # calculate_flux / calculate_power_density point lists (OSCARSSR_Python.cc:2472-2506,
+    # 2911-2940): rotated, normals reversed for normal=-1 (power density only), translated
+    for name, flip, directional in (("flux_points_rotated", False, None), ("pd_points_rotated_flip", True, True),
+                                    ("pd_points_rotated_undirected", True, False)):
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        a = cases.args(name)
+        xyz, nrm = cases.points_transformed(a, flip)
+        c = golden(name)["container"]
+        assert np.array_equal(c[:len(xyz), :3], xyz)
+        if directional is None:
+            v = oracle.flux(t, dt, xyz, float(a["energy_ev"]), p.q, p.current)
+        else:
+            v = oracle.power_density(t, dt, xyz, nrm, directional, p.q, p.current)
+        assert np.array_equal(v, c[:len(xyz), 3])
+    # the reversed normals switch the directional screen's skipped terms: the two differ
+    flip = golden("pd_points_rotated_flip")["container"][:5, 3]
+    free = golden("pd_points_rotated_undirected")["container"][:5, 3]
+    assert not np.array_equal(flip, free)
+
+
+def test_rectangle_from_three_corners_bitwise(oracle):
+    # TSurfacePoints_Rectangle::Init(NX1, NX2, X0, X1, X2, Normal) (:76-123), dim=2
+    # coordinates from GetX1/GetX2 (:244-259)
+    for name in ("flux_x0x1x2", "pd_x0x1x2_normal"):
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        a = cases.args(name)
+        n = cases.vec(a, "nx")
+        x0, x1, x2 = cases.corners(a)
+        normal = int(float(a.get("normal", 0)))
+        xyz, nrm, x12 = oracle.rect3_surface(int(n[0]), int(n[1]), x0, x1, x2, normal)
+        c = golden(name)["container"]
+        m = len(xyz)
+        assert np.array_equal(c[:m, 0:2], x12) and np.all(c[:m, 2] == 0)
+        # corners land where they were put: first point X0, last along X1 and X2
+        assert np.allclose(xyz[0], x0, rtol=0, atol=1e-15)
+        assert np.allclose(xyz[(int(n[0]) - 1) * int(n[1])], x1, rtol=0, atol=1e-15)
+        assert np.allclose(xyz[int(n[1]) - 1], x2, rtol=0, atol=1e-15)
+        if name.startswith("flux"):
+            v = oracle.flux(t, dt, xyz, float(a["energy_ev"]), p.q, p.current, nthreads=2)
+        else:
+            # normal = -1 reverses (X1 - X0) x (X2 - X0)
+            cr = np.cross(np.subtract(x1, x0), np.subtract(x2, x0))
+            assert np.allclose(nrm[0], -cr / np.linalg.norm(cr), rtol=0, atol=1e-15)
+            v = oracle.power_density(t, dt, xyz, nrm, normal != 0, p.q, p.current, nthreads=2)
+        assert np.array_equal(v, c[:m, 3])
+
+
+def test_tapered_offcentre_undulator_bitwise(oracle):
+    # TField3D_IdealUndulator with taper, centre and phase (TField3D_IdealUndulator.cc:50-139)
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_taper_center")
+    g = golden("traj_taper_center")
+    assert dt == float(g["dt"])
+    assert np.array_equal(t[:6], g["traj"][:6]) and np.array_equal(t[6:, 1:], g["traj"][6:, 1:])
+    name = "flux_taper_center"
+    F, p, t, dt = cases.oracle_setup(oracle, name)
+    a = cases.args(name)
+    xyz, nrm, x12 = _rect(oracle, a)
+    v = oracle.flux(t, dt, xyz, float(a["energy_ev"]), p.q, p.current, nthreads=2)
+    assert np.array_equal(v, golden(name)["container"][:len(xyz), 3])
+    # closed form of the tapered field inside the periodic part: B0 sin(2 pi (D - PS)/lam) (1 + D taper),
+    # D measured from the centre along the period, PS = phase lam / 2 pi
+    u = cases.undulators(a)[0]
+    taper, ctr, lam = u["taper"], u["center"], 0.05
+    ps = u["phase"] * lam / (2 * np.pi)
+    for D in (-0.8, -0.3, 0.05, 0.41, 0.9):  # the period is along z: D = z - centre_z
+        b = oracle.bfield(F, (0.0, 0.0, ctr[2] + D))[1]
+        assert abs(b - np.sin(2 * np.pi * (D - ps) / lam) * (1 + D * taper)) < 1e-12
