@@ -105,6 +105,24 @@ int srs_hip_total_power(const srs_traj* traj, double charge, double current, dou
  * keep data in HBM and timing of the kernels alone. */
 int srs_hip_flux_device(const srs_traj* traj, const double* obs_soa, int64_t ns, const double* energies_ev,
                         int64_t ne, double charge, double current, int precision, double* flux_out, void* stream);
+
+/* srs_hip_flux_device without allocation: the same flux (the same kernels, chunking and
+ * results) with its scratch -- the FP32 trajectory staging of the mixed form and the
+ * per-chunk partial sums -- in SRS_NOALLOC_SCRATCH_BYTES of the library's own device
+ * memory (a static device array, one copy per GPU, held until the process ends)
+ * instead of buffers the library allocates.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  A call needing more scratch -- (trajectory chunks
+ * x energies per launch (<= 4) x 48 B) per observation point, + 12 B per trajectory
+ * point in mixed precision -- is rejected with SRS_ERR_INVALID.  Every call on a GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (on different streams) race.
+ * Nothing in it has to start at zero (each call writes all it reads).  The library's
+ * first call on a device, whichever entry point, also sets it up (streams, the scratch
+ * address): make one call before capturing a graph. */
+#define SRS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int srs_hip_flux_device_noalloc(const srs_traj* traj, const double* obs_soa, int64_t ns, const double* energies_ev,
+                                int64_t ne, double charge, double current, int precision, double* flux_out,
+                                void* stream);
 int srs_hip_spectrum_device(const srs_traj* traj, const double obs[3], const double* energies_ev, int64_t ne,
                             double charge, double current, int precision, double* flux_out, void* stream);
 int srs_hip_power_density_device(const srs_traj* traj, const double* obs_soa, const double* normals_soa, int64_t ns,

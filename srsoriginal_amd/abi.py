@@ -48,6 +48,8 @@ def lib():
             "srs_hip_power_density": [C.POINTER(SrsTraj), _P, _P, _I64, C.c_int, _D, _D, C.c_int, _P],
             "srs_hip_flux1": [C.POINTER(SrsTraj), _P, _I64, _D, _D, _D, _P],
             "srs_hip_flux_device": [C.POINTER(SrsTraj), C.c_void_p, _I64, _P, _I64, _D, _D, C.c_int, C.c_void_p, C.c_void_p],
+            "srs_hip_flux_device_noalloc": [C.POINTER(SrsTraj), C.c_void_p, _I64, _P, _I64, _D, _D, C.c_int,
+                                            C.c_void_p, C.c_void_p],
             "srs_hip_spectrum_device": [C.POINTER(SrsTraj), _P, _P, _I64, _D, _D, C.c_int, C.c_void_p, C.c_void_p],
             "srs_hip_power_density_device": [C.POINTER(SrsTraj), C.c_void_p, C.c_void_p, _I64, C.c_int, _D, _D, C.c_int,
                                              C.c_void_p, C.c_void_p],
@@ -152,6 +154,15 @@ def flux_device(traj_ptrs, n, dt, obs_soa_ptr, ns, energies_ev, charge, current,
     e = _f64(np.atleast_1d(energies_ev))
     _check(lib().srs_hip_flux_device(C.byref(t), obs_soa_ptr, ns, e.ctypes.data_as(_P), len(e), charge, current,
                                      precision, out_ptr, stream))
+
+
+def flux_device_noalloc(traj_ptrs, n, dt, obs_soa_ptr, ns, energies_ev, charge, current, precision, out_ptr,
+                        stream):
+    """flux_device with the library's static device scratch (srs_hip_flux_device_noalloc)."""
+    t = SrsTraj(n, dt, *traj_ptrs)
+    e = _f64(np.atleast_1d(energies_ev))
+    _check(lib().srs_hip_flux_device_noalloc(C.byref(t), obs_soa_ptr, ns, e.ctypes.data_as(_P), len(e), charge,
+                                             current, precision, out_ptr, stream))
 
 
 def power_density_device(traj_ptrs, n, dt, obs_soa_ptr, nrm_soa_ptr, ns, directional, charge, current, precision,

@@ -116,10 +116,36 @@ Context& ctx() {
   return c;
 }
 
+// Scratch of srs_hip_flux_device_noalloc: device memory of the library itself, so that
+// entry point allocates nothing.  Every GPU the library runs on holds its own copy until
+// the process ends.  Host code never passes the array's name (a host-side stand-in) to
+// the device: it uses the address hipGetSymbolAddress returns for the current device.
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[SRS_NOALLOC_SCRATCH_BYTES];
+void* g_scratch_dev[kScratchDevices] = {};  // per device, filled once with that device current
+
+int scratch_address(int dev, void** p) {
+  if (dev < 0 || dev >= kScratchDevices) return fail(SRS_ERR_INVALID, "device index out of range for the scratch");
+  if (!g_scratch_dev[dev]) {
+    void* a = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(SRS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = a;
+  }
+  *p = g_scratch_dev[dev];
+  return SRS_OK;
+}
+
 #define HIPCHK(expr)                                                                 \
   do {                                                                               \
     hipError_t _e = (expr);                                                          \
     if (_e != hipSuccess) return fail(SRS_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+#define TRY(expr)              \
+  do {                         \
+    int _r = (expr);           \
+    if (_r != SRS_OK) return _r; \
   } while (0)
 
 int device_count_raw() {
@@ -134,6 +160,8 @@ int init(Context& c) {
   HIPCHK(hipSetDevice(0));
   HIPCHK(hipStreamCreate(&c.stream));
   HIPCHK(hipStreamCreate(&c.stream2));
+  void* sp = nullptr;
+  TRY(scratch_address(0, &sp));  // set-up of the allocation-free entry point on the library's device
   for (auto& e : c.seg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   c.ready = true;
   return SRS_OK;
@@ -228,21 +256,37 @@ TrajView view_of(const srs_traj* t) {
 
 // ---- core launches on device data ------------------------------------------
 
+// Device scratch of one flux evaluation, in one buffer: the FP32 staging of the mixed form
+// (traj_f32_floats) at offset 0, then the chunk partial sums part[((c*ne0 + e)*6 + k)*ns + j]
+// (ne0 = energies per launch, at most kMaxFluxEnergies; launches of later energy groups
+// reuse them in stream order).
+struct FluxPlan {
+  int chunk = 0, nch = 0, ne0 = 0;
+  size_t part_off = 0, bytes = 0;
+};
+
+int plan_flux(Context& c, const TrajView& t, int64_t ns, int64_t ne, int precision, FluxPlan* pl) {
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  pl->ne0 = (int)std::min<int64_t>(srs::kMaxFluxEnergies, ne);
+  int const per_cu = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_blocks_per_cu(pl->ne0)
+                                                      : srs::flux_fp64_blocks_per_cu(pl->ne0);
+  pl->chunk = pick_chunk(t.n, ns, 32, (int64_t)c.cus * per_cu);
+  pl->nch = nchunks(t.n, pl->chunk);
+  size_t const staged = precision == SRS_PRECISION_MIXED ? srs::traj_f32_floats(t.n, 0) * sizeof(float) : 0;
+  pl->part_off = (staged + 255) / 256 * 256;
+  pl->bytes = pl->part_off + (size_t)pl->nch * pl->ne0 * 6 * ns * sizeof(double);
+  return SRS_OK;
+}
+
+// The flux kernels on device data, with the scratch of plan_flux at `scratch` (device
+// memory of at least pl.bytes).  Queues work on s only: no allocation, no synchronisation.
 int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, const double* energies_ev, int64_t ne,
-             double q, double cur, int precision, double* out, hipStream_t s) {
+             double q, double cur, int precision, double* out, hipStream_t s, FluxPlan const& pl, void* scratch) {
   double const c0 = c0_of(q), c2 = c2_of(q, cur);
   int const maxe = srs::kMaxFluxEnergies;
-  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
-  int const ne0 = (int)std::min<int64_t>(maxe, ne);
-  int const per_cu = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_blocks_per_cu(ne0)
-                                                      : srs::flux_fp64_blocks_per_cu(ne0);
-  int const chunk = pick_chunk(t.n, ns, 32, (int64_t)c.cus * per_cu);
-  int const nch = nchunks(t.n, chunk);
-  HIPCHK(c.part.ensure((size_t)nch * maxe * 6 * ns * sizeof(double)));
-  if (precision == SRS_PRECISION_MIXED) {
-    HIPCHK(c.trajf.ensure(srs::traj_f32_floats(t.n, 0) * sizeof(float)));
-    HIPCHK(srs::launch_traj_to_f32(t, 0, c.trajf.as<float>(), s));
-  }
+  float* const trajf = static_cast<float*>(scratch);
+  double* const part = reinterpret_cast<double*>(static_cast<char*>(scratch) + pl.part_off);
+  if (precision == SRS_PRECISION_MIXED) HIPCHK(srs::launch_traj_to_f32(t, 0, trajf, s));
   for (int64_t e0 = 0; e0 < ne; e0 += maxe) {
     EnergySet es{};
     es.ne = (int)std::min<int64_t>(maxe, ne - e0);
@@ -257,14 +301,23 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
     {
       KernelTimer timer(c, s);
       if (precision == SRS_PRECISION_MIXED) {
-        HIPCHK(srs::launch_flux_mixed(t, c.trajf.as<float>(), obs_soa, (int)ns, es, chunk, c.part.as<double>(), s));
+        HIPCHK(srs::launch_flux_mixed(t, trajf, obs_soa, (int)ns, es, pl.chunk, part, s));
       } else {
-        HIPCHK(srs::launch_flux_fp64(t, obs_soa, (int)ns, es, chunk, c.part.as<double>(), s));
+        HIPCHK(srs::launch_flux_fp64(t, obs_soa, (int)ns, es, pl.chunk, part, s));
       }
     }
-    HIPCHK(srs::launch_flux_reduce(c.part.as<double>(), nch, (int)ns, es, c2, out + e0 * ns, s));
+    HIPCHK(srs::launch_flux_reduce(part, pl.nch, (int)ns, es, c2, out + e0 * ns, s));
   }
   return SRS_OK;
+}
+
+// run_flux with its scratch in the context's grow-only buffer.
+int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, const double* energies_ev, int64_t ne,
+             double q, double cur, int precision, double* out, hipStream_t s) {
+  FluxPlan pl;
+  TRY(plan_flux(c, t, ns, ne, precision, &pl));
+  HIPCHK(c.part.ensure(pl.bytes));
+  return run_flux(c, t, obs_soa, ns, energies_ev, ne, q, cur, precision, out, s, pl, c.part.p);
 }
 
 // CalculateFlux1's sum (OSCARSSR.cc:2209-2346): Omega = (E 2 pi) / 4.1357e-15 as the
@@ -366,12 +419,6 @@ int check_common(int64_t nout, int precision) {
     return fail(SRS_ERR_INVALID, "precision must be SRS_PRECISION_FP64 or SRS_PRECISION_MIXED");
   return SRS_OK;
 }
-
-#define TRY(expr)              \
-  do {                         \
-    int _r = (expr);           \
-    if (_r != SRS_OK) return _r; \
-  } while (0)
 
 // ---- batched multi-particle path -------------------------------------------------
 
@@ -995,6 +1042,40 @@ int srs_hip_flux_device(const srs_traj* traj, const double* obs_soa, int64_t ns,
   TRY(init(c));
   hipStream_t s = stream ? (hipStream_t)stream : c.stream;
   return run_flux(c, view_of(traj), obs_soa, ns, energies_ev, ne, charge, current, precision, flux_out, s);
+}
+
+int srs_hip_flux_device_noalloc(const srs_traj* traj, const double* obs_soa, int64_t ns, const double* energies_ev,
+                                int64_t ne, double charge, double current, int precision, double* flux_out,
+                                void* stream) {
+  g_err.clear();
+  TRY(check_traj(traj, false));
+  TRY(check_common(ns, precision));
+  if (ne < 1) return fail(SRS_ERR_INVALID, "no photon energies");
+  if (!obs_soa || !energies_ev || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  void* sp = nullptr;
+  TRY(scratch_address(dev, &sp));
+  TrajView const t = view_of(traj);
+  FluxPlan pl;
+  TRY(plan_flux(c, t, ns, ne, precision, &pl));
+  if (pl.bytes > (size_t)SRS_NOALLOC_SCRATCH_BYTES) {
+    char msg[256];
+    snprintf(msg, sizeof msg,
+             "this flux needs %zu bytes of scratch, more than the %zu of srs_hip_flux_device_noalloc "
+             "(use srs_hip_flux_device)",
+             pl.bytes, (size_t)SRS_NOALLOC_SCRATCH_BYTES);
+    return fail(SRS_ERR_INVALID, msg);
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : c.stream;
+  bool const tsave = c.timing;
+  c.timing = false;  // the timer creates its events on demand: not here
+  int const rc = run_flux(c, t, obs_soa, ns, energies_ev, ne, charge, current, precision, flux_out, s, pl, sp);
+  c.timing = tsave;
+  return rc;
 }
 
 int srs_hip_spectrum_device(const srs_traj* traj, const double obs[3], const double* energies_ev, int64_t ne,
