@@ -35,7 +35,7 @@ F_ALG_FLUX1 = 77                 # per CalculateFlux1 (Lienard-Wiechert) term
 PEAK_FP32_VALU = 157.3e12        # MI355X FP32 vector peak (published)
 PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
 # Issue model of the dominant loop (flux_mixed_kernel<1,256>, paraxial terms), from the
-# measured cycles per wave64 instruction on gfx950 (profiles/valu_issue_rates_gfx950.md):
+# measured cycles per wave64 instruction on gfx950 (a one-off rocprofv3 measurement per class):
 # FP32 with VGPR operands 2, FP64 (add, fma, fract, cvt) 4, sin/cos 8.  Per term: 8 FP64
 # (phase 5, fract 1, two conversions) + 9 FP32 (8 accumulations, 1 amplitude) + sin + cos
 # = 66 cycles, plus ~3 for the 32-term sub-tile epilogue.

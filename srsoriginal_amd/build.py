@@ -6,7 +6,6 @@ Produces, in-tree:
   srsoriginal_amd/lib/libsrs_hip.so   gfx950 HIP kernels + the C ABI of include/srs_hip.h
   srsoriginal_amd/lib/_sr*.so         the host engine (OSCARSSR-compatible C++ class) and
                                       its Python binding; links libsrs_hip.so
-  srsoriginal_amd/lib/valu_rates      the VALU issue-rate probe (not part of the library)
 Every call recompiles.  hipcc cross-compiles for gfx950 without a GPU.
 """
 import os
@@ -91,19 +90,9 @@ def build_host():
     return out
 
 
-def build_probe():
-    """The VALU issue-rate probe (csrc/probe/valu_rates.hip): a stand-alone program that
-    times each VALU instruction class; it measures the roofline's issue limits."""
-    out = os.path.join(LIB_DIR, "valu_rates")
-    _run_all([[HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] +
-              [os.path.join(PKG, "csrc", "probe", "valu_rates.hip"), "-o", out]])
-    return out
-
-
 def build():
     t0 = time.time()
     lib = build_hip()
-    build_probe()
     ext = build_host()
     print("built %s%s in %.1fs" % (os.path.relpath(lib, ROOT), (", " + os.path.relpath(ext, ROOT)) if ext else "",
                                     time.time() - t0))
