@@ -75,10 +75,10 @@ def setup_problem():
     return traj, dt, surf.xyz, q, cur
 
 
-def batched_c3_share(nparticles=512, reps=2):
-    """Config-3 shape on one GPU's share: 512 of the 4096 Gaussian-beam particles (the
-    8-GPU sharding gives each GPU 512), ragged 18k-22k point trajectories, 101x101
-    points over 10 mm at 30 m, 3 energies around the 3rd harmonic; FP64 and mixed."""
+def batched_c3(nparticles=4096, reps=1):
+    """Config 3 at its stated size on one GPU: 4096 Gaussian-beam particles, ragged 18k-22k
+    point trajectories, 101x101 points over 10 mm at 30 m, 3 energies around the 3rd
+    harmonic; FP64 and mixed, each the whole captured pipeline."""
     from srsoriginal_amd import abi, batch
     from srsoriginal_amd.lib import _sr
     from srsoriginal_amd.sr import sr
@@ -92,7 +92,8 @@ def batched_c3_share(nparticles=512, reps=2):
                                           rng_seed=4096)
     xyz = _sr.Surface.rectangle("XY", 101, 101, 0.01, 0.01, [0, 0, 0], [0, 0, 30], 0).xyz
     es = [430.0, 431.2, 432.4]
-    npts = sum(abi.trajectory_points(abi.particles([r])) for r in rows)
+    npts = abi.trajectory_points(abi.particles(rows))
+    lens = [r["npoints"] for r in rows]
     terms = float(npts) * len(xyz) * len(es)
     res = {}
     for prec in ("double", "mixed"):
@@ -105,17 +106,64 @@ def batched_c3_share(nparticles=512, reps=2):
         kms, nl = abi.timing_read()
         abi.timing_enable(False)
         res[prec] = dict(out=out, wall_s=wall, graph_ms=kms / max(nl, 1), cfg=abi.batch_config())
-        log("batched %s: %.3f s/eval (graph %.1f ms), config %s" % (prec, wall, res[prec]["graph_ms"], res[prec]["cfg"]))
+        log("batched C3 %s: %.3f s/eval (graph %.1f ms), config %s" % (prec, wall, res[prec]["graph_ms"],
+                                                                        res[prec]["cfg"]))
     f64, mix = res["double"]["out"], res["mixed"]["out"]
-    return {"workload": "%d particles x %d traj points (ragged 18k-22k) x 101x101 x 3 energies" % (nparticles, npts),
+    return {"workload": "%d particles x %d traj points (ragged %d-%d) x 101x101 x 3 energies" %
+                        (nparticles, npts, min(lens), max(lens)),
             "terms": terms,
+            "fp64_ms": res["double"]["graph_ms"], "mixed_ms": res["mixed"]["graph_ms"],
             "fp64_terms_per_s": terms / (res["double"]["graph_ms"] * 1e-3),
             "mixed_terms_per_s": terms / (res["mixed"]["graph_ms"] * 1e-3),
             "speedup": res["double"]["graph_ms"] / res["mixed"]["graph_ms"],
             "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(mix - f64)) / np.max(np.abs(f64))),
             "mixed_vs_fp64_rms_rel_err": float(np.sqrt(np.mean((mix - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2))),
             "tile_config_fp64": res["double"]["cfg"], "tile_config_mixed": res["mixed"]["cfg"],
-            "timed": "whole hipGraph (device RK4 + persistent flux + particle reduction), HIP events"}
+            "timed": "whole hipGraph (device RK4 + persistent flux + particle reduction), HIP events; "
+                     "%d timed evaluation(s) per form after one warm-up" % reps}
+
+
+def flux_general_c1(traj, dt, q, cur, reps=5):
+    """C1's map on a tilted screen (201x201 over 20 mm at 30 m, rotated 0.3 rad about y): the
+    points do not lie on one plane z = Z, so the mixed kernel takes its general terms (the
+    reference's D and tau in FP64, FP32 amplitudes) instead of the paraxial ones."""
+    import torch
+    from srsoriginal_amd import abi
+    from srsoriginal_amd.lib import _sr
+    xyz = _sr.Surface.rectangle("XY", 201, 201, 0.02, 0.02, [0, 0.3, 0], [0, 0, 30], 0).xyz
+    n, ns = traj.shape[1], len(xyz)
+    dev = torch.device("cuda:0")
+    d_traj = torch.from_numpy(np.ascontiguousarray(traj)).to(dev)
+    d_obs = torch.from_numpy(np.ascontiguousarray(xyz.T)).to(dev)
+    ptrs = [d_traj[k].data_ptr() for k in range(9)]
+    stream = torch.cuda.current_stream()
+    res = {}
+    for prec in (abi.FP64, abi.MIXED):
+        out = torch.zeros(ns, dtype=torch.float64, device=dev)
+        for _ in range(2):
+            abi.flux_device(ptrs, n, dt, d_obs.data_ptr(), ns, [ENERGY_EV], q, cur, prec, out.data_ptr(),
+                            stream.cuda_stream)
+        torch.cuda.synchronize()
+        abi.timing_enable(True)
+        for _ in range(reps):
+            abi.flux_device(ptrs, n, dt, d_obs.data_ptr(), ns, [ENERGY_EV], q, cur, prec, out.data_ptr(),
+                            stream.cuda_stream)
+        torch.cuda.synchronize()
+        kms, nl = abi.timing_read()
+        abi.timing_enable(False)
+        res[prec] = dict(ms=kms / max(nl, 1), out=out.cpu().numpy())
+    terms = float(n) * ns
+    f64, mix = res[abi.FP64]["out"], res[abi.MIXED]["out"]
+    log("general-form C1 (tilted screen): fp64 %.3f ms, mixed %.3f ms" % (res[abi.FP64]["ms"], res[abi.MIXED]["ms"]))
+    return {"mixed_general": {
+        "workload": "flux map 201x201, 20 mm at 30 m tilted 0.3 rad about y, E=143.52 eV, NT=20001 (general mixed terms)",
+        "terms": terms, "fp64_kernel_ms": res[abi.FP64]["ms"], "mixed_kernel_ms": res[abi.MIXED]["ms"],
+        "fp64_terms_per_s": terms / (res[abi.FP64]["ms"] * 1e-3),
+        "mixed_terms_per_s": terms / (res[abi.MIXED]["ms"] * 1e-3),
+        "speedup": res[abi.FP64]["ms"] / res[abi.MIXED]["ms"],
+        "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(mix - f64)) / np.max(np.abs(f64))),
+        "mixed_vs_fp64_rms_rel_err": float(np.sqrt(np.mean((mix - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2))),
+        "roofline": _frac(F_ALG_FLUX, terms, res[abi.FP64]["ms"], res[abi.MIXED]["ms"])}}
 
 
 def spectrum_and_power_density(reps=5):
@@ -404,7 +452,97 @@ def gradient_pd(nreal=256, reps=2):
                      "adjoint + RK4 adjoint"}
 
 
-def cpu_baseline(traj, dt, xyz, q, cur, budget_s=12.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+UND_C0 = "und1=0,1,0,0,0,0.05,41"
+BEAM_C0 = ["x0=0,0,-1.5", "d0=0,0,1", "energy=3", "current=0.5", "ct=0,3"]
+BEAM_C3 = BEAM_C0 + ["sigE=0.00267", "beta=1.5,0.8", "emit=0.9e-9,8e-12", "hdir=1,0,0", "latref=0,0,0", "seed=4096"]
+
+
+def _ref_time(mode, args):
+    """Wall time of one reference-engine run (oracle/_ref/ref_harness: the reference's own
+    src/*.cc, its -O3 flags, one std::thread per output point as it threads), CPU only."""
+    import subprocess
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix=".bin") as f:
+        t0 = time.perf_counter()
+        subprocess.run([REF_HARNESS, mode] + args + ["out=" + f.name], check=True, stdout=subprocess.DEVNULL)
+        return time.perf_counter() - t0
+
+
+def cpu_baselines():
+    """The reference CPU path timed on this host for every config, with the host's thread
+    share (16 on a one-GPU box) and single-threaded, each on a bounded sample whose terms
+    are stated; C3 and C4 are extrapolated from their samples by the terms count.  Uses
+    the reference engine itself when oracle/_ref/ref_harness was built (kind "reference"),
+    else the oracle's restatement (kind "port", measured 2-2.6x faster than the reference
+    engine on the same samples)."""
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = {"cpu_model": _cpu_model(), "threads": threads}
+    if not os.path.exists(REF_HARNESS):
+        out["kind"] = "port"
+        return out
+    out["kind"] = "reference"
+    # (name, mode, args at `threads`, terms, args single-threaded, terms, full-config terms)
+    E1 = "energy_ev=143.52176088044024"
+    pole_scales = ",".join("%.17g" % x for x in np.random.default_rng(7).normal(0, 0.005, 82))
+    plan = [
+        ("c0_spectrum", "spectrum", [UND_C0] + BEAM_C0 + ["npoints=20001", "ne=2000", "erange=100,3000", "obs=0,0,30"],
+         2000 * 20001.0, [UND_C0] + BEAM_C0 + ["npoints=20001", "ne=200", "erange=100,3000", "obs=0,0,30"],
+         200 * 20001.0, 2000 * 20001.0),
+        ("c1_flux", "flux", [UND_C0] + BEAM_C0 + ["npoints=20001", "plane=XY", "width=0.02,0.02", "nx=101,51",
+                                                   "trans=0,0,30", E1], 101 * 51 * 20001.0,
+         [UND_C0] + BEAM_C0 + ["npoints=20001", "plane=XY", "width=0.02,0.02", "nx=21,11", "trans=0,0,30", E1],
+         21 * 11 * 20001.0, 201 * 201 * 20001.0),
+        ("c2_power_density", "pd", ["und1=0,0.6,0,0,0,0.057,30", "und2=0.6,0,0,0,0,0.057,30,-1.5707963267948966"] +
+         BEAM_C0 + ["npoints=40001", "plane=XY", "width=0.06,0.06", "nx=61,61", "trans=0,0,30"], 61 * 61 * 40001.0,
+         ["und1=0,0.6,0,0,0,0.057,30", "und2=0.6,0,0,0,0,0.057,30,-1.5707963267948966"] + BEAM_C0 +
+         ["npoints=40001", "plane=XY", "width=0.06,0.06", "nx=15,11", "trans=0,0,30"], 15 * 11 * 40001.0,
+         301 * 301 * 40001.0),
+        ("c3_batched_flux", "flux", [UND_C0] + BEAM_C3 + ["npoints=20001", "plane=XY", "width=0.01,0.01", "nx=11,11",
+                                                           "trans=0,0,30", "energy_ev=431.2", "nparticles=16"],
+         16 * 121 * 20001.0, [UND_C0] + BEAM_C3 + ["npoints=20001", "plane=XY", "width=0.01,0.01", "nx=5,5",
+                                                   "trans=0,0,30", "energy_ev=431.2", "nparticles=4"],
+         4 * 25 * 20001.0, 2.51e12),
+        ("c4_gradient_forward", "flux", ["poles=0,1,0,0,0,0.05,41", "scales=" + pole_scales] + BEAM_C0 +
+         ["npoints=20001", "plane=XY", "width=0.005,0.005", "nx=51,51", "trans=0,0,30", E1], 51 * 51 * 20001.0,
+         ["poles=0,1,0,0,0,0.05,41", "scales=" + pole_scales] + BEAM_C0 +
+         ["npoints=20001", "plane=XY", "width=0.005,0.005", "nx=13,13", "trans=0,0,30", E1], 13 * 13 * 20001.0,
+         256 * 51 * 51 * 20001.0),
+    ]
+    for name, mode, a_mt, t_mt, a_1, t_1, t_full in plan:
+        row = {}
+        for label, args, terms, nt in (("threads", a_mt, t_mt, threads), ("single", a_1, t_1, 1)):
+            el = _ref_time(mode, args + ["nthreads=%d" % nt])
+            row[label] = {"terms_per_s": terms / el, "sample_terms": terms, "seconds": el, "threads": nt}
+        row["full_config_terms"] = t_full
+        row["full_config_s_at_threads"] = t_full / row["threads"]["terms_per_s"]
+        row["extrapolated"] = t_full != row["threads"]["sample_terms"]
+        out[name] = row
+        log("cpu %s: %.3e terms/s on %d threads, %.3e single" % (name, row["threads"]["terms_per_s"], threads,
+                                                                  row["single"]["terms_per_s"]))
+    return out
+
+
+def cpu_baseline(traj, dt, xyz, q, cur, budget_s=12.0, table=None):
+    """The headline cpu_baseline object: the C1 flux rate of the reference engine (table from
+    cpu_baselines) or, without it, of the oracle's restatement on a strided sample."""
+    if table and table.get("kind") == "reference":
+        r = table["c1_flux"]["threads"]
+        return {"value": r["terms_per_s"], "unit": "terms/s", "cores": r["threads"], "kind": "reference",
+                "sample": "101x51 of the 201x201 map points (same 20 mm screen) x 20001 trajectory steps, the "
+                          "reference engine (oracle/_ref, its own sources and flags), %d threads, %.1f s; %s"
+                          % (r["threads"], r["seconds"], table["cpu_model"])}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -522,7 +660,7 @@ def main():
                      "flop_per_term": F_ALG_FLUX},
     }
     if not a.no_batch:
-        out["batched_c3_share"] = batched_c3_share()
+        out["batched_c3"] = batched_c3()
         out["batched_power_density"] = batched_power_density()
         out["batched_spectrum"] = batched_spectrum()
     if not a.no_grad:
@@ -531,9 +669,12 @@ def main():
     if not a.no_other:
         out.update(spectrum_and_power_density())
         out.update(flux1_c1())
+        out.update(flux_general_c1(traj, dt, q, cur))
     if not a.no_cpu:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(traj, dt, xyz, q, cur)
+        log("cpu baselines ...")
+        table = cpu_baselines()
+        out["cpu_baseline"] = cpu_baseline(traj, dt, xyz, q, cur, table=table)
+        out["cpu_baselines"] = table
     print(json.dumps(out), flush=True)
 
 
