@@ -284,6 +284,9 @@ int main(int argc, char** argv) {
   setup(osr);
   int const nthreads = (int)num("nthreads", 1);
   int const nparticles = (int)num("nparticles", 0);
+  // gpu=1: the reference's GPU routes (a harness built with -DCUDA and an OSCARSSR_Cuda_*
+  // implementation; the plain build throws "GPU functionality not compiled ...")
+  int const gpu = (int)num("gpu", 0);
 
   try {
     if (mode == "traj") {
@@ -308,7 +311,7 @@ int main(int argc, char** argv) {
         std::vector<double> er = vec("erange", std::vector<double>{100, 3000});
         S.Init((size_t)num("ne", 100), er[0], er[1]);
       }
-      osr.CalculateSpectrum(v3("obs", TVector3D(0, 0, 30)), S, nparticles, nthreads, 0);
+      osr.CalculateSpectrum(v3("obs", TVector3D(0, 0, 30)), S, nparticles, nthreads, gpu);
       // file output as calculate_spectrum(ofile=...) writes it (OSCARSSR_Python.cc:2340-2343)
       if (gArgs.count("ofile")) S.WriteToFileText(str("ofile", ""));
       put((double)S.GetNPoints());
@@ -321,10 +324,10 @@ int main(int argc, char** argv) {
       if (gArgs.count("points")) {
         // points=x,y,z,nx,ny,nz,x,y,z,... (calculate_flux with a point list, dim=3)
         TSurfacePoints_3D S = points3d(false);
-        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, 3, "");
+        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, gpu, 3, "");
       } else {
         TSurfacePoints_Rectangle S = rect();
-        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, 0, (int)num("dim", 2), str("ofile", ""));
+        osr.CalculateFlux(S, num("energy_ev", 100), C, nparticles, nthreads, gpu, (int)num("dim", 2), str("ofile", ""));
       }
       write_container(C);
     } else if (mode == "pd") {
@@ -332,10 +335,10 @@ int main(int argc, char** argv) {
       bool const directional = num("normal", 0) != 0;
       if (gArgs.count("points")) {
         TSurfacePoints_3D S = points3d(true);
-        osr.CalculatePowerDensity(S, C, 3, num("pnormal", 0) != 0, nparticles, "", nthreads, 0);
+        osr.CalculatePowerDensity(S, C, 3, num("pnormal", 0) != 0, nparticles, "", nthreads, gpu);
       } else {
         TSurfacePoints_Rectangle S = rect();
-        osr.CalculatePowerDensity(S, C, (int)num("dim", 2), directional, nparticles, str("ofile", ""), nthreads, 0);
+        osr.CalculatePowerDensity(S, C, (int)num("dim", 2), directional, nparticles, str("ofile", ""), nthreads, gpu);
       }
       write_container(C);
     } else if (mode == "flux1") {
