@@ -134,8 +134,12 @@ int srs_hip_power_density_device(const srs_traj* traj, const double* obs_soa, co
  * call per particle) with one device pipeline.  Particles are sampled by the caller
  * (host RNG, reference order) and passed as initial conditions. */
 
-/* One magnetic field element; the elements of a list are summed in order
- * (TFieldContainer::GetF, TFieldContainer.cc:107-117).
+/* One field element, magnetic [T] (target SRS_FIELD_MAGNETIC, 0) or electric [V/m]
+ * (SRS_FIELD_ELECTRIC, OSCARSSR::AddElectricField); the elements of each kind of field
+ * are summed in order (TFieldContainer::GetF, TFieldContainer.cc:107-117), and the RK4
+ * force is (q/m gamma)(E + v x B) in the operation order of OSCARSSR::DerivativesEB
+ * (OSCARSSR.cc:769-794), which equals DerivativesB / DerivativesE (:741-765, :713-737)
+ * bit for bit when the other field is absent.
  *   kind SRS_FIELD_UNDULATOR (0, so zero-initialised records are undulators): ideal
  *     undulator (TField3D_IdealUndulator.cc:50-139) of field, period, nperiods, phase,
  *     taper, center; pole_delta != NULL scales its 2*nperiods interior half-period poles
@@ -156,6 +160,8 @@ int srs_hip_power_density_device(const srs_traj* traj, const double* obs_soa, co
 #define SRS_FIELD_GAUSSIAN 1
 #define SRS_FIELD_UNIFORM_BOX 2
 #define SRS_FIELD_GRID 3
+#define SRS_FIELD_MAGNETIC 0
+#define SRS_FIELD_ELECTRIC 1
 typedef struct srs_field {
   double field[3];
   double period[3];
@@ -172,6 +178,7 @@ typedef struct srs_field {
   double grid_start[3];
   double grid_step[3];
   double grid_stop[3];
+  int32_t target; /* SRS_FIELD_MAGNETIC or SRS_FIELD_ELECTRIC */
 } srs_field;
 
 /* One particle: initial position, beta, q/(m gamma), T0 [m], its ct range and point

@@ -185,15 +185,43 @@ V3 field_sum(const oracle_fields* f, V3 X) {
   return S;
 }
 
-// OSCARSSR::DerivativesB, OSCARSSR.cc:741-765.
+// The electric container's sum (TFieldContainer::GetF over the electric elements).
+V3 efield_sum(const oracle_fields* f, V3 X) {
+  V3 S{0, 0, 0};
+  for (int i = 0; i < f->neund; ++i) {
+    V3 e = undulator_field(f->eund[i], 0, X);
+    S.x += e.x;
+    S.y += e.y;
+    S.z += e.z;
+  }
+  return S;
+}
+
+// OSCARSSR::SetDerivativesFunction (OSCARSSR.cc:690-703): DerivativesE when only electric
+// fields exist, DerivativesB when only magnetic ones, else DerivativesEB.
 inline void derivs(const oracle_fields* f, double qmg, const double* x, double* dxdt) {
-  V3 B = field_sum(f, V3{x[0], x[2], x[4]});
+  bool const has_b = f->nund > 0 || f->pole_und != 0;
+  bool const has_e = f->neund > 0;
   dxdt[0] = x[1];
-  dxdt[1] = qmg * (-x[5] * B.y + x[3] * B.z);
   dxdt[2] = x[3];
-  dxdt[3] = qmg * (x[5] * B.x - x[1] * B.z);
   dxdt[4] = x[5];
-  dxdt[5] = qmg * (x[1] * B.y - x[3] * B.x);
+  if (!has_b && has_e) {  // DerivativesE, :713-737
+    V3 E = efield_sum(f, V3{x[0], x[2], x[4]});
+    dxdt[1] = qmg * E.x;
+    dxdt[3] = qmg * E.y;
+    dxdt[5] = qmg * E.z;
+  } else if (has_b && !has_e) {  // DerivativesB, :741-765
+    V3 B = field_sum(f, V3{x[0], x[2], x[4]});
+    dxdt[1] = qmg * (-x[5] * B.y + x[3] * B.z);
+    dxdt[3] = qmg * (x[5] * B.x - x[1] * B.z);
+    dxdt[5] = qmg * (x[1] * B.y - x[3] * B.x);
+  } else {  // DerivativesEB, :769-794
+    V3 B = field_sum(f, V3{x[0], x[2], x[4]});
+    V3 E = efield_sum(f, V3{x[0], x[2], x[4]});
+    dxdt[1] = qmg * (E.x - x[5] * B.y + x[3] * B.z);
+    dxdt[3] = qmg * (E.y + x[5] * B.x - x[1] * B.z);
+    dxdt[5] = qmg * (E.z + x[1] * B.y - x[3] * B.x);
+  }
 }
 
 // OSCARSSR::RK4, OSCARSSR.cc:826-874 (in place: y == yout).
@@ -696,6 +724,13 @@ int64_t oracle_trajectory_npoints(double ct_start, double ct_stop, int64_t npoin
   size_t const nf = 1 + (ct_stop - t0) / kC / dt;
   size_t const nb = (t0 - ct_start) / kC / dt;
   return (int64_t)(nf + nb);
+}
+
+void oracle_efield(const oracle_fields* f, const double x[3], double out[3]) {
+  V3 e = efield_sum(f, v3(x));
+  out[0] = e.x;
+  out[1] = e.y;
+  out[2] = e.z;
 }
 
 void oracle_bfield(const oracle_fields* f, const double x[3], double out[3]) {

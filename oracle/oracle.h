@@ -38,13 +38,16 @@ typedef struct oracle_undulator {
   double rotations[3];
 } oracle_undulator;
 
-/* A field set: ideal undulators, plus optionally one pole-scaled undulator
- * (pole_und != NULL, pole_scales[2*nperiods] = delta_p). */
+/* A field set: magnetic elements, optionally one pole-scaled undulator
+ * (pole_und != NULL, pole_scales[2*nperiods] = delta_p), and electric elements (eund,
+ * the same kinds, in V/m). */
 typedef struct oracle_fields {
   const oracle_undulator* und;
   int nund;
   const oracle_undulator* pole_und;
   const double* pole_scales;
+  const oracle_undulator* eund;
+  int neund;
 } oracle_fields;
 
 /* Particle state needed by the trajectory (TParticleA). */
@@ -89,6 +92,8 @@ int64_t oracle_trajectory(const oracle_fields* f, const oracle_particle* p, doub
                           int64_t npoints, double* traj, double* dt);
 /* Summed magnetic field at a point. */
 void oracle_bfield(const oracle_fields* f, const double x[3], double out[3]);
+/* Summed electric field at a point. */
+void oracle_efield(const oracle_fields* f, const double x[3], double out[3]);
 
 /* Energy grid of TSpectrumContainer::Init(N, E0, E1). */
 void oracle_energy_grid(int64_t n, double e0, double e1, double* out);

@@ -22,7 +22,8 @@ class Undulator(C.Structure):
 
 
 class Fields(C.Structure):
-    _fields_ = [("und", C.POINTER(Undulator)), ("nund", C.c_int), ("pole_und", C.POINTER(Undulator)), ("pole_scales", P)]
+    _fields_ = [("und", C.POINTER(Undulator)), ("nund", C.c_int), ("pole_und", C.POINTER(Undulator)), ("pole_scales", P),
+                ("eund", C.POINTER(Undulator)), ("neund", C.c_int)]
 
 
 class Particle(C.Structure):
@@ -63,17 +64,23 @@ def undulator(field, period, nperiods, phase=0.0, taper=0.0, center=(0, 0, 0)):
 class FieldSet:
     """Keeps the ctypes structures alive for the duration of the calls."""
 
-    def __init__(self, undulators=(), pole=None, pole_scales=None):
-        self._und = (Undulator * max(1, len(undulators)))()
-        for i, u in enumerate(undulators):
-            # kind 1 (Gaussian) / 2 (uniform box): field (already rotated), width, center, rotations
-            self._und[i] = Undulator(_v3(u["field"]), _v3(u.get("period", (0, 0, 0))), int(u.get("nperiods", 0)),
-                                     float(u.get("phase", 0)), float(u.get("taper", 0)),
-                                     _v3(u.get("center", (0, 0, 0))), int(u.get("kind", 0)),
-                                     _v3(u.get("width", (0, 0, 0))), _v3(u.get("rotations", (0, 0, 0))))
+    def __init__(self, undulators=(), pole=None, pole_scales=None, efields=()):
+        def records(lst):
+            arr = (Undulator * max(1, len(lst)))()
+            for i, u in enumerate(lst):
+                # kind 1 (Gaussian) / 2 (uniform box): field (already rotated), width, center, rotations
+                arr[i] = Undulator(_v3(u["field"]), _v3(u.get("period", (0, 0, 0))), int(u.get("nperiods", 0)),
+                                   float(u.get("phase", 0)), float(u.get("taper", 0)),
+                                   _v3(u.get("center", (0, 0, 0))), int(u.get("kind", 0)),
+                                   _v3(u.get("width", (0, 0, 0))), _v3(u.get("rotations", (0, 0, 0))))
+            return arr
+        self._und = records(undulators)
+        self._eund = records(efields)
         self.s = Fields()
         self.s.und = self._und
         self.s.nund = len(undulators)
+        self.s.eund = self._eund
+        self.s.neund = len(efields)
         if pole is not None:
             self._pole = Undulator(_v3(pole["field"]), _v3(pole["period"]), int(pole["nperiods"]), 0.0, 0.0, _v3((0, 0, 0)),
                                    0, _v3((0, 0, 0)), _v3((0, 0, 0)))
@@ -147,6 +154,12 @@ def trajectory(fields, particle, ct_start, ct_stop, npoints):
     dt = D()
     lib().oracle_trajectory(C.byref(fields.s), C.byref(particle), D(ct_start), D(ct_stop), I64(npoints), _p(t), C.byref(dt))
     return t, dt.value
+
+
+def efield(fields, x):
+    out = (D * 3)()
+    lib().oracle_efield(C.byref(fields.s), _v3(x), out)
+    return np.array(out[:])
 
 
 def bfield(fields, x):
@@ -269,3 +282,57 @@ def efield_time(traj, dt, obs, q):
     lib().oracle_efield_time(_p(traj), I64(n), D(dt), _p(np.ascontiguousarray(obs, dtype=np.float64)), D(q), _p(e),
                              _p(t))
     return e, t
+
+
+# ---- the integrals in extended precision (x86 80-bit long double, 64-bit mantissa) ----
+# The reference's FP64 phase omega * (dt i + D/c) reaches ~5e11 rad, where one FP64 ulp is
+# ~6e-5 rad: its rounding is a pseudo-random phase error per term, and where the integral
+# cancels (between harmonics) it shows in the result at ~1e-5.  These evaluate the same
+# integrand (OSCARSSR.cc:2054-2092, on the same FP64 trajectory, t_i = i dt exactly) with
+# every step in long double (phase spacing ~3e-8 rad): the yardstick for how close the
+# FP64 and mixed results are to the integral they both approximate.
+_LD = np.longdouble
+
+
+def _ld_consts(q, current):
+    pi = _LD("3.14159265358979323846264338327950288")
+    c = _LD("2.99792458e8")
+    h, hbar, qe = _LD("6.62606876e-34"), _LD("1.054571596e-34"), _LD("1.602176462e-19")
+    eps0, mu0 = _LD("8.854187817e-12"), _LD("1.2566370614e-6")
+    q = _LD(q)
+    c0 = q / (4 * pi * c * eps0 * np.sqrt(2 * pi))
+    c2 = 4 * pi * _LD(current) / (h * abs(q) * mu0 * c) * _LD("1e-6") * _LD("1e-3")
+    return c, qe, hbar, c0, c2
+
+
+def _ld_sum(traj, dt, obs, omega, c):
+    n = traj.shape[1]
+    X, B = traj[:3].astype(_LD), traj[3:6].astype(_LD)
+    t = np.arange(n, dtype=_LD) * _LD(dt)
+    R = np.asarray(obs, dtype=_LD).reshape(3, 1) - X
+    D = np.sqrt(np.sum(R * R, axis=0))
+    N = R / D
+    ph = omega * (t + D / c)
+    cs, sn = np.cos(ph), np.sin(ph)
+    g = c / (omega * D)  # the i c/(omega D) near-field factor
+    are = (B - N) / D    # real part of the amplitude
+    aim = -(N * g) / D   # imaginary part
+    # (are + i aim)(cos + i sin)
+    return np.sum(are * cs - aim * sn, axis=1), np.sum(are * sn + aim * cs, axis=1)
+
+
+def flux_extended(traj, dt, obs, energy_ev, q, current):
+    """Flux on obs [n, 3] at one energy, the integral in long double (see above)."""
+    c, qe, hbar, c0, c2 = _ld_consts(q, current)
+    omega = _LD(energy_ev) * qe / hbar
+    k = c0 * omega * _LD(dt)
+    out = np.zeros(len(obs))
+    for j, o in enumerate(np.asarray(obs, dtype=np.float64).reshape(-1, 3)):
+        re, im = _ld_sum(traj, dt, o, omega, c)
+        out[j] = float(c2 * k * k * np.sum(re * re + im * im))
+    return out
+
+
+def spectrum_extended(traj, dt, obs, energies_ev, q, current):
+    """Spectrum at obs over energies, the integral in long double (see above)."""
+    return np.array([flux_extended(traj, dt, [obs], e, q, current)[0] for e in energies_ev])

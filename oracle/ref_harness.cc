@@ -140,6 +140,31 @@ static void setup(OSCARSSR& osr) {
     TVector3D rot = b.size() >= 12 ? TVector3D(b[9], b[10], b[11]) : TVector3D(0, 0, 0);
     osr.AddMagneticField((TField*)new TField3D_UniformBox(TVector3D(b[0], b[1], b[2]), w, ctr, rot));
   }
+  // Electric fields (OSCARSSR::AddElectricField, as add_efield_undulator / _gaussian /
+  // _uniform add them, OSCARSSR_Python.cc:994-1240): eund1=Ex,Ey,Ez,Px,Py,Pz,N[,phase[,cx,cy,cz]]
+  // (no taper), egauss1= and ebox1= as gauss1= and box1=; added in this order.
+  for (int k = 1; k <= 2; ++k) {
+    std::vector<double> u = vec("eund" + std::to_string(k));
+    if (u.size() < 7) continue;
+    TVector3D ctr = u.size() > 10 ? TVector3D(u[8], u[9], u[10]) : TVector3D(0, 0, 0);
+    osr.AddElectricField((TField*)new TField3D_IdealUndulator(TVector3D(u[0], u[1], u[2]), TVector3D(u[3], u[4], u[5]),
+                                                              (int)u[6], ctr, u.size() > 7 ? u[7] : 0));
+  }
+  for (int k = 1; k <= 2; ++k) {
+    std::vector<double> g = vec("egauss" + std::to_string(k));
+    if (g.size() < 6) continue;
+    TVector3D ctr = g.size() >= 9 ? TVector3D(g[6], g[7], g[8]) : TVector3D(0, 0, 0);
+    TVector3D rot = g.size() >= 12 ? TVector3D(g[9], g[10], g[11]) : TVector3D(0, 0, 0);
+    osr.AddElectricField((TField*)new TField3D_Gaussian(TVector3D(g[0], g[1], g[2]), ctr, TVector3D(g[3], g[4], g[5]), rot));
+  }
+  for (int k = 1; k <= 2; ++k) {
+    std::vector<double> b = vec("ebox" + std::to_string(k));
+    if (b.size() < 3) continue;
+    TVector3D w = b.size() >= 6 ? TVector3D(b[3], b[4], b[5]) : TVector3D(0, 0, 0);
+    TVector3D ctr = b.size() >= 9 ? TVector3D(b[6], b[7], b[8]) : TVector3D(0, 0, 0);
+    TVector3D rot = b.size() >= 12 ? TVector3D(b[9], b[10], b[11]) : TVector3D(0, 0, 0);
+    osr.AddElectricField((TField*)new TField3D_UniformBox(TVector3D(b[0], b[1], b[2]), w, ctr, rot));
+  }
   // Field maps read by the reference's TField3D_Grid (OSCARSSR::AddMagneticField,
   // OSCARSSR.cc:68-90): gridfile1=path gridfmt1=format [gridrot1=rx,ry,rz]
   // [gridtrans1=tx,ty,tz] [gridscale1=s0,s1,...] (then gridfile2 ...), and one map
@@ -397,6 +422,22 @@ int main(int argc, char** argv) {
       for (size_t i = 0; i < S.GetNPoints(); ++i) {
         put(S.GetEnergy(i));
         put(S.GetFlux(i));
+      }
+    } else if (mode == "efield_at") {
+      // summed electric field at the points of pts=path (OSCARSSR::GetE); layout as "bfield"
+      FILE* fp = fopen(str("pts", "").c_str(), "rb");
+      if (!fp) return 4;
+      std::vector<double> xyz;
+      double v;
+      while (fread(&v, sizeof(double), 1, fp) == 1) xyz.push_back(v);
+      fclose(fp);
+      size_t const n = xyz.size() / 3;
+      put((double)n);
+      for (size_t i = 0; i != n; ++i) {
+        TVector3D const E = osr.GetE(TVector3D(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+        put(E.GetX());
+        put(E.GetY());
+        put(E.GetZ());
       }
     } else if (mode == "bfield") {
       // total field at the points of the raw float64 file pts=path (x, y, z per point):

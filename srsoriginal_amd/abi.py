@@ -187,13 +187,14 @@ BATCH_AUTOTUNE = 2
 
 
 FIELD_UNDULATOR, FIELD_GAUSSIAN, FIELD_UNIFORM_BOX, FIELD_GRID = 0, 1, 2, 3
+FIELD_MAGNETIC, FIELD_ELECTRIC = 0, 1
 
 
 class SrsField(C.Structure):
     _fields_ = [("field", _D * 3), ("period", _D * 3), ("nperiods", C.c_int32), ("phase", _D), ("taper", _D),
                 ("center", _D * 3), ("pole_delta", C.c_void_p), ("kind", C.c_int32), ("width", _D * 3),
                 ("rotations", _D * 3), ("grid", C.c_void_p), ("grid_n", C.c_int32 * 3), ("grid_start", _D * 3),
-                ("grid_step", _D * 3), ("grid_stop", _D * 3)]
+                ("grid_step", _D * 3), ("grid_stop", _D * 3), ("target", C.c_int32)]
 
 
 class SrsParticle(C.Structure):
@@ -232,7 +233,8 @@ def undulators(specs, keep):
     """srs_field array from dicts: undulators (field, period, nperiods[, phase, taper,
     center, pole_delta]) or, with kind FIELD_GAUSSIAN / FIELD_UNIFORM_BOX, (field,
     width[, center, rotations]), or with kind FIELD_GRID (grid [n0*n1*n2*3], grid_n,
-    grid_start, grid_step, grid_stop[, center, rotations])."""
+    grid_start, grid_step, grid_stop[, center, rotations]); target FIELD_ELECTRIC for an
+    electric field (default magnetic)."""
     arr = (SrsField * len(specs))()
     for i, u in enumerate(specs):
         pd = None
@@ -250,7 +252,8 @@ def undulators(specs, keep):
                           (_D * 3)(*u.get("center", (0, 0, 0))), pd, int(u.get("kind", FIELD_UNDULATOR)),
                           (_D * 3)(*u.get("width", (0, 0, 0))), (_D * 3)(*u.get("rotations", (0, 0, 0))), grid,
                           (C.c_int32 * 3)(*u.get("grid_n", (1, 1, 1))), (_D * 3)(*u.get("grid_start", (0, 0, 0))),
-                          (_D * 3)(*u.get("grid_step", (0, 0, 0))), (_D * 3)(*u.get("grid_stop", (0, 0, 0))))
+                          (_D * 3)(*u.get("grid_step", (0, 0, 0))), (_D * 3)(*u.get("grid_stop", (0, 0, 0))),
+                          int(u.get("target", FIELD_MAGNETIC)))
     return arr
 
 

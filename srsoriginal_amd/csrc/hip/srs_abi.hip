@@ -85,7 +85,8 @@ struct Context {
   // batched path
   Buf b_part, b_und, b_poles, b_grid, b_traj, b_trajf, b_energies, b_flux_pp, b_out, b_counter, b_obs;
   Buf b_state, b_field;  // RK4 state between segments; field sums accumulated over segments
-  bool und_local = false;  // the staged field list holds elements other than undulators
+  bool und_local = false;  // the staged field list holds elements other than magnetic undulators
+  int und_force = srs::kForceB;  // the derivative form: magnetic, electric or both kinds of elements
   hipGraph_t b_graph = nullptr;
   hipGraphExec_t b_exec = nullptr;
   std::string b_key;
@@ -469,7 +470,7 @@ int build_particles(const srs_particle* p, int64_t np, double charge, double cur
 }
 
 int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) {
-  if (nund < 1 || !und) return fail(SRS_ERR_INVALID, "no magnetic field (undulator) given");
+  if (nund < 1 || !und) return fail(SRS_ERR_INVALID, "no magnetic or electric field given");
   std::vector<srs::DevUndulator> du(nund);
   std::vector<double> poles;
   // field maps: their values go to one device buffer, one after the other
@@ -497,6 +498,11 @@ int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) 
     srs_field const& h = und[u];
     d = srs::DevUndulator{};
     d.kind = h.kind;
+    if (h.target != SRS_FIELD_MAGNETIC && h.target != SRS_FIELD_ELECTRIC)
+      return fail(SRS_ERR_INVALID, "field target must be SRS_FIELD_MAGNETIC or SRS_FIELD_ELECTRIC");
+    if (h.target == SRS_FIELD_ELECTRIC && h.pole_delta)
+      return fail(SRS_ERR_INVALID, "pole errors apply to magnetic undulators only");
+    d.target = h.target;
     d.pole_off = -1;
     if (h.kind == SRS_FIELD_GRID) {
       if (h.pole_delta) return fail(SRS_ERR_INVALID, "pole errors apply to undulators only");
@@ -557,7 +563,13 @@ int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) 
   }
   if (poles.empty()) poles.push_back(0.0);
   c.und_local = false;
-  for (int u = 0; u < nund; ++u) c.und_local = c.und_local || und[u].kind != SRS_FIELD_UNDULATOR;
+  bool has_b = false, has_e = false;
+  for (int u = 0; u < nund; ++u) {
+    has_e = has_e || und[u].target == SRS_FIELD_ELECTRIC;
+    has_b = has_b || und[u].target != SRS_FIELD_ELECTRIC;
+    c.und_local = c.und_local || und[u].kind != SRS_FIELD_UNDULATOR || und[u].target == SRS_FIELD_ELECTRIC;
+  }
+  c.und_force = has_e ? (has_b ? srs::kForceEB : srs::kForceE) : srs::kForceB;  // OSCARSSR.cc:690-703
   HIPCHK(c.b_und.ensure(du.size() * sizeof(srs::DevUndulator)));
   HIPCHK(c.b_poles.ensure(poles.size() * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c.b_und.p, du.data(), du.size() * sizeof(srs::DevUndulator), hipMemcpyHostToDevice, s));
@@ -626,7 +638,7 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
   }
   srs::RK4Seg rs{0, L, nseg > 1 ? c.b_state.as<double>() : nullptr};
   if (nseg == 1) {
-    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.b_poles.as<double>(), P, np_run, r.bt,
+    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), P, np_run, r.bt,
                                  r.ck, rs, s));
     HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
     srs::FluxSeg const fs{0, L, 1, 1, field != nullptr, field};
@@ -638,7 +650,7 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
     for (int k = 0; k < nseg; ++k) {
       rs.lo = k * L;
       rs.hi = (k + 1) * L;
-      HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.b_poles.as<double>(), P, np_run, r.bt,
+      HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), P, np_run, r.bt,
                                    r.ck, rs, c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
     }
@@ -664,7 +676,7 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
   // and the weight, and the address of every buffer a launch reads or writes
   snprintf(buf, sizeof buf,
            "%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns,
-           r.ne, (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local, c.cus, (long long)r.max_leg,
+           r.ne, (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local + 2 * c.und_force, c.cus, (long long)r.max_leg,
            r.weight, r.be.c2, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
            c.b_state.p, c.b_field.p, c.b_poles.p, c.b_part.p, c.b_counter.p, c.b_und.p,
            (void*)r.bt.bz, (void*)r.bt.tflag);  // row offsets within the packed buffers follow the total point count
@@ -688,14 +700,14 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
   auto* U = c.b_und.as<srs::DevUndulator>();
   double* const poles = c.b_poles.as<double>();
   if (nseg == 1) {
-    HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, poles, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
+    HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
     HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, 0, INT64_MAX, nch, 0,
                                 c.g_part.as<double>(), mixed, s));
   } else {
     HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
     HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
     for (int k = 0; k < nseg; ++k) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, poles, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
                                    c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
     }
@@ -722,8 +734,8 @@ int pole_validate(const srs_field* und, int nund, int poled, const srs_particle*
     if (u != poled && und[u].pole_delta)
       return fail(SRS_ERR_INVALID, "only the differentiated undulator may carry pole errors");
   for (int u = 0; u < nund; ++u)
-    if (und[u].kind != SRS_FIELD_UNDULATOR)
-      return fail(SRS_ERR_INVALID, "the pole-error gradient supports undulator fields only");
+    if (und[u].kind != SRS_FIELD_UNDULATOR || und[u].target != SRS_FIELD_MAGNETIC)
+      return fail(SRS_ERR_INVALID, "the pole-error gradient supports magnetic undulator fields only");
   if (!particle) return fail(SRS_ERR_INVALID, "no particle");
   if (nreal < 1 || nreal > 65536) return fail(SRS_ERR_INVALID, "number of realisations must be in 1..65536");
   if (!deltas || !target || !loss_out || !tape_out || !obs_xyz) return fail(SRS_ERR_INVALID, "NULL argument");
@@ -1157,7 +1169,7 @@ int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, 
   HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
   srs::BatchTraj bt;
   TRY(stage_batch_traj(c, tot, np, true, &bt));
-  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.und_local, c.b_poles.as<double>(),
+  HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.und_local, c.und_force, c.b_poles.as<double>(),
                                c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0},
                                srs::RK4Seg{0, INT64_MAX, nullptr}, s));
   double* src[9] = {bt.x, bt.y, bt.z, bt.bx, bt.by, bt.bz, bt.ax, bt.ay, bt.az};
@@ -1401,7 +1413,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   {
     KernelTimer timer(c, s);
     if (nseg == 1) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
                                    srs::RK4Seg{0, INT64_MAX, nullptr}, s));
       HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, 0, INT64_MAX, nch, 0,
                                         c.g_part.as<double>(), mixed, s));
@@ -1409,7 +1421,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
       HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
       HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
       for (int k = 0; k < nseg; ++k) {
-        HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+        HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
                                      srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()}, c.stream2));
         HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
       }

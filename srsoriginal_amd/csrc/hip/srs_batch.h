@@ -27,6 +27,7 @@ struct DevUndulator {
   int nperiods;
   int pole_off;
   int kind;
+  int target;  // 0 magnetic, 1 electric
   double w[3];
   double rc[3], rs[3];
   // kind 3, field map: 3 doubles per grid point, ((ix*gn[1] + iy)*gn[2] + iz)
@@ -113,8 +114,12 @@ struct SampledParticle {
 hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
                               int64_t np, SampledParticle* out, hipStream_t s);
 
-// local: the list holds elements other than undulators (Gaussian, box, field map)
-hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, const double* poles, const BatchParticle* P,
+// local: the list holds elements other than magnetic undulators (Gaussian, box, field
+// map, or any electric element); force: kForceB (magnetic elements only), kForceE
+// (electric only) or kForceEB (both), OSCARSSR::SetDerivativesFunction's choice
+constexpr int kForceB = 0, kForceE = 1, kForceEB = 2;
+hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, int force, const double* poles,
+                            const BatchParticle* P,
                             int np, BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s);
 hipError_t launch_zero_counter(unsigned* c, hipStream_t s);
 // One time segment of the flux: steps [lo, hi) of both RK4 legs of every particle.

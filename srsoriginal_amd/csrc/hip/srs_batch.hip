@@ -22,7 +22,7 @@ namespace {
 // Same point layout as the host trajectory (OSCARSSR.cc:562-673): forward pass from
 // T0 stores the state before each step with the previous derivative (A/c lags one
 // point; 0 at the first point), the backward pass fills the nb points before T0.
-template <int NU>
+template <int NU, Force FORCE>
 __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu,
                                                         const double* __restrict__ poles,
                                                         const BatchParticle* __restrict__ Ps, int np, BatchTraj bt,
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __res
   int const p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   BatchParticle const q = Ps[p];
-  RK4Ctx<NU> const R{Fields<NU>(U, nu, poles + q.pole_base), q.qmg};
+  RK4Ctx<NU, FORCE> const R{Fields<NU>(U, nu, poles + q.pole_base), q.qmg};
   double x[6], d[6] = {0, 0, 0, 0, 0, 0};
   auto init = [&]() {
     x[0] = q.x0[0];
@@ -760,13 +760,15 @@ hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, 
   return hipGetLastError();
 }
 
-hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, const double* poles, const BatchParticle* P,
-                            int np,
-                            BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s) {
+hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, int force, const double* poles,
+                            const BatchParticle* P, int np, BatchTraj const& bt, Checkpoints const& ck,
+                            RK4Seg const& sg, hipStream_t s) {
   dim3 const grid((np + 63) / 64);
-  if (nu == 1 && !local) rk4_batch_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
-  else if (nu == 2 && !local) rk4_batch_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
-  else rk4_batch_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  if (force == kForceE) rk4_batch_kernel<0, Force::E><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  else if (force == kForceEB) rk4_batch_kernel<0, Force::EB><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  else if (nu == 1 && !local) rk4_batch_kernel<1, Force::B><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  else if (nu == 2 && !local) rk4_batch_kernel<2, Force::B><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  else rk4_batch_kernel<0, Force::B><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
   return hipGetLastError();
 }
 

@@ -239,13 +239,24 @@ struct Fields<0> {
     B[0] = B[1] = B[2] = 0.0;
     for (int k = 0; k < nu; ++k) field_add(U[k], poles, X, Y, Z, B);
   }
+  // magnetic and electric sums, each element into the field of its target
+  __device__ __forceinline__ void eval_eb(double X, double Y, double Z, double* B, double* E) const {
+    B[0] = B[1] = B[2] = 0.0;
+    E[0] = E[1] = E[2] = 0.0;
+    for (int k = 0; k < nu; ++k) field_add(U[k], poles, X, Y, Z, U[k].target ? E : B);
+  }
   __device__ __forceinline__ void jet(double X, double Y, double Z, FieldJet& J) const {
     J = FieldJet{{0, 0, 0}, {0, 0, 0}, 0.0, -1};
     for (int k = 0; k < nu; ++k) field_add_jet(U[k], poles, X, Y, Z, J);
   }
 };
 
-template <int NU>
+// Derivative form, as OSCARSSR::SetDerivativesFunction picks it (OSCARSSR.cc:690-703):
+// magnetic elements only (DerivativesB), electric only (DerivativesE) or both
+// (DerivativesEB); the electric forms take the general records (NU == 0).
+enum class Force { B, E, EB };
+
+template <int NU, Force FORCE = Force::B>
 struct RK4Ctx {
   Fields<NU> F;
   double qmg;
@@ -259,10 +270,36 @@ struct RK4Ctx {
     d[4] = x[5];
     d[5] = qmg * (x[1] * B[1] - x[3] * B[0]);
   }
+  // dv/dt = (q/m gamma)(E + v x B), OSCARSSR::DerivativesEB (OSCARSSR.cc:769-794), and
+  // (q/m gamma) E, DerivativesE (:713-737)
+  __device__ __forceinline__ void derivs_from_eb(const double* x, const double* B, const double* E,
+                                                 double* d) const {
+    d[0] = x[1];
+    d[1] = qmg * (E[0] - x[5] * B[1] + x[3] * B[2]);
+    d[2] = x[3];
+    d[3] = qmg * (E[1] + x[5] * B[0] - x[1] * B[2]);
+    d[4] = x[5];
+    d[5] = qmg * (E[2] + x[1] * B[1] - x[3] * B[0]);
+  }
+  __device__ __forceinline__ void derivs_from_e(const double* x, const double* E, double* d) const {
+    d[0] = x[1];
+    d[1] = qmg * E[0];
+    d[2] = x[3];
+    d[3] = qmg * E[1];
+    d[4] = x[5];
+    d[5] = qmg * E[2];
+  }
   __device__ __forceinline__ void derivs(const double* x, double* d) const {
     double B[3];
-    F.eval(x[0], x[2], x[4], B);
-    derivs_from(x, B, d);
+    if constexpr (FORCE == Force::B) {
+      F.eval(x[0], x[2], x[4], B);
+      derivs_from(x, B, d);
+    } else {
+      double E[3];
+      F.eval_eb(x[0], x[2], x[4], B, E);
+      if constexpr (FORCE == Force::E) derivs_from_e(x, E, d);
+      else derivs_from_eb(x, B, E, d);
+    }
   }
   // OSCARSSR::RK4 (OSCARSSR.cc:826-874); hh = h/2 and h6 = h/6 are formed by the caller
   __device__ __forceinline__ void step(double* y, const double* dydx, double h, double hh, double h6) const {

@@ -53,6 +53,13 @@ srs::Precision prec_of(int p) {
 
 }  // namespace
 
+// A field element into the magnetic (target 0) or electric (target 1) container.
+static void add_to(srs::OSCARSSR& e, srs::IdealUndulator const& u, int target) {
+  if (target == 0) e.AddMagneticField(u);
+  else if (target == 1) e.AddElectricField(u);
+  else throw std::invalid_argument("field target must be 0 (magnetic) or 1 (electric)");
+}
+
 PYBIND11_MODULE(_sr, m) {
   m.doc() = "srsoriginal_amd host engine (OSCARSSR-compatible) over the gfx950 kernels";
   py::register_exception_translator([](std::exception_ptr p) {
@@ -102,7 +109,7 @@ PYBIND11_MODULE(_sr, m) {
       .def(py::init<>())
       .def("add_undulator",
            [](srs::OSCARSSR& e, std::vector<double> field, std::vector<double> period, int nperiods, double phase,
-              double taper, std::vector<double> center, std::vector<double> pole_deltas) {
+              double taper, std::vector<double> center, std::vector<double> pole_deltas, int target) {
              srs::IdealUndulator u;
              u.field = v3(field);
              u.period = v3(period);
@@ -115,11 +122,11 @@ PYBIND11_MODULE(_sr, m) {
                  throw std::invalid_argument("'pole_scales' must have 2*nperiods entries");
                u.pole_delta = pole_deltas;
              }
-             e.AddMagneticField(u);
+             add_to(e, u, target);
            })
       .def("add_field",
            [](srs::OSCARSSR& e, int kind, std::vector<double> field, std::vector<double> width,
-              std::vector<double> rot, std::vector<double> center) {
+              std::vector<double> rot, std::vector<double> center, int target) {
              // TField3D_Gaussian / TField3D_UniformBox (field given already rotated)
              srs::IdealUndulator u;
              if (kind != 1 && kind != 2) throw std::invalid_argument("unknown field kind");
@@ -128,18 +135,18 @@ PYBIND11_MODULE(_sr, m) {
              u.width = v3(width);
              u.rot = v3(rot);
              u.center = v3(center);
-             e.AddMagneticField(u);
+             add_to(e, u, target);
            })
       .def("add_bfield_file",
            [](srs::OSCARSSR& e, std::string const& file, std::string const& format, std::vector<double> rot,
-              std::vector<double> trans, std::vector<double> scale) {
+              std::vector<double> trans, std::vector<double> scale, int target) {
              // OSCARSSR::AddMagneticField(file, format, ...), OSCARSSR.cc:68-90
              srs::IdealUndulator u;
              u.kind = srs::FieldKind::Grid;
              u.rot = v3(rot);
              u.center = v3(trans);
              u.grid = std::make_shared<srs::FieldGrid>(srs::read_grid_file(file, format, u.rot, scale));
-             e.AddMagneticField(u);
+             add_to(e, u, target);
            })
       .def("add_bfield_interpolated",
            [](srs::OSCARSSR& e, std::vector<std::pair<double, std::string>> mapping, std::string const& format,
@@ -159,6 +166,13 @@ PYBIND11_MODULE(_sr, m) {
            [](srs::OSCARSSR const& e, std::vector<double> x) {
              Vec3 const b = e.GetB(v3(x));
              return std::vector<double>{b.x, b.y, b.z};
+           })
+      .def("clear_efields", &srs::OSCARSSR::ClearElectricFields)
+      .def("n_efields", [](srs::OSCARSSR const& e) { return e.EFields().size(); })
+      .def("get_efield",
+           [](srs::OSCARSSR const& e, std::vector<double> x) {
+             Vec3 const f = e.GetE(v3(x));
+             return std::vector<double>{f.x, f.y, f.z};
            })
       .def("add_particle_beam",
            [](srs::OSCARSSR& e, std::string const& type, std::string const& name, std::vector<double> x0,
@@ -221,9 +235,12 @@ PYBIND11_MODULE(_sr, m) {
            })
       .def("undulators",
            [](srs::OSCARSSR const& e) {
+             // every field element, the magnetic ones first (each list in insertion order)
              py::list out;
-             for (auto const& u : e.Fields().fields()) {
+             for (int target = 0; target < 2; ++target)
+             for (auto const& u : (target == 0 ? e.Fields() : e.EFields()).fields()) {
                py::dict d;
+               d["target"] = target;
                d["field"] = std::vector<double>{u.field.x, u.field.y, u.field.z};
                d["period"] = std::vector<double>{u.period.x, u.period.y, u.period.z};
                d["nperiods"] = u.nperiods;

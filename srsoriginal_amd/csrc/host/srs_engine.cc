@@ -843,14 +843,34 @@ void OSCARSSR::CalculateTrajectory(Particle& P) const {
   for (auto* v : {&T.x, &T.y, &T.z, &T.bx, &T.by, &T.bz, &T.ax, &T.ay, &T.az}) v->assign(n, 0.0);
   double const qmg = P.qmg;
   FieldContainer const& F = bfield_;
+  FieldContainer const& G = efield_;
+  // OSCARSSR::SetDerivativesFunction (OSCARSSR.cc:690-703): DerivativesE (:713-737) when
+  // only electric fields exist, DerivativesB (:741-765) when only magnetic ones, else
+  // DerivativesEB (:769-794).  The three agree in value where a field is absent but not
+  // in the sign of zero results, which the stored A/c keeps.
+  bool const has_b = !F.fields().empty(), has_e = !G.fields().empty();
   auto derivs = [&](const double* x, double* d) {
-    Vec3 const B = F.eval(Vec3(x[0], x[2], x[4]));
+    Vec3 const X(x[0], x[2], x[4]);
     d[0] = x[1];
-    d[1] = qmg * (-x[5] * B.y + x[3] * B.z);
     d[2] = x[3];
-    d[3] = qmg * (x[5] * B.x - x[1] * B.z);
     d[4] = x[5];
-    d[5] = qmg * (x[1] * B.y - x[3] * B.x);
+    if (has_e && !has_b) {
+      Vec3 const E = G.eval(X);
+      d[1] = qmg * E.x;
+      d[3] = qmg * E.y;
+      d[5] = qmg * E.z;
+    } else if (has_b && !has_e) {
+      Vec3 const B = F.eval(X);
+      d[1] = qmg * (-x[5] * B.y + x[3] * B.z);
+      d[3] = qmg * (x[5] * B.x - x[1] * B.z);
+      d[5] = qmg * (x[1] * B.y - x[3] * B.x);
+    } else {
+      Vec3 const B = F.eval(X);
+      Vec3 const E = G.eval(X);
+      d[1] = qmg * (E.x - x[5] * B.y + x[3] * B.z);
+      d[3] = qmg * (E.y + x[5] * B.x - x[1] * B.z);
+      d[5] = qmg * (E.z + x[1] * B.y - x[3] * B.x);
+    }
   };
   auto rk4 = [&](double* y, const double* dydx, double h) {
     double dym[6], dyt[6], yt[6];
@@ -935,7 +955,7 @@ void OSCARSSR::CalculateSpectrum(Vec3 const& obs, SpectrumContainer& S, int npar
         cur = particle_.current;
       }
     }
-    if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+    if (no_fields()) throw std::invalid_argument("no magnetic or electric field given");
     std::vector<srs_field> und;
     device_fields(und);
     double const o[3] = {obs.x, obs.y, obs.z};
@@ -968,8 +988,10 @@ void add_points(SurfacePoints const& surf, ScalarContainer& C, int dim) {
 
 void OSCARSSR::device_fields(std::vector<srs_field>& out) const {
   out.clear();
-  for (auto const& u : bfield_.fields()) {
+  for (int target = 0; target < 2; ++target)
+  for (auto const& u : (target == 0 ? bfield_ : efield_).fields()) {
     srs_field d{};
+    d.target = target == 0 ? SRS_FIELD_MAGNETIC : SRS_FIELD_ELECTRIC;
     d.field[0] = u.field.x;
     d.field[1] = u.field.y;
     d.field[2] = u.field.z;
@@ -1032,7 +1054,7 @@ void OSCARSSR::CalculateTrajectoryGPU(Particle& P) const {
   if (ct_start_ >= ct_stop_) throw std::out_of_range("start time is greater than stop time.  check that set_ctstartstop is set");
   if (P.type.empty()) throw std::out_of_range("particle not initialized.  make sure you have a particle or beam defined");
   if (npoints_ < 2) throw std::out_of_range("number of trajectory points must be >= 2");
-  if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+  if (no_fields()) throw std::invalid_argument("no magnetic or electric field given");
   std::vector<srs_field> und;
   device_fields(und);
   srs_particle const dp = device_particle(P);
@@ -1085,7 +1107,7 @@ void OSCARSSR::CalculateFlux(SurfacePoints const& surf, double energy_ev, Scalar
         cur = particle_.current;
       }
     }
-    if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+    if (no_fields()) throw std::invalid_argument("no magnetic or electric field given");
     std::vector<srs_field> und;
     device_fields(und);
     std::vector<double> out(surf.size());
@@ -1141,7 +1163,7 @@ void OSCARSSR::CalculatePowerDensity(SurfacePoints const& surf, ScalarContainer&
         cur = particle_.current;
       }
     }
-    if (bfield_.fields().empty()) throw std::invalid_argument("no magnetic field (undulator) given");
+    if (no_fields()) throw std::invalid_argument("no magnetic or electric field given");
     std::vector<srs_field> und;
     device_fields(und);
     std::vector<double> out(surf.size());

@@ -160,11 +160,16 @@ class OSCARSSR {
  public:
   OSCARSSR();
 
-  // fields (OSCARSSR.cc:68-299 as used by add_bfield_undulator)
+  // fields (OSCARSSR.cc:68-299): magnetic [T] and electric [V/m] elements of the same
+  // kinds, each summed in insertion order (TFieldContainer::GetF)
   void AddMagneticField(IdealUndulator const& u) { bfield_.add(u); }
   void ClearMagneticFields() { bfield_.clear(); }
   Vec3 GetB(Vec3 const& X) const { return bfield_.eval(X); }
   FieldContainer const& Fields() const { return bfield_; }
+  void AddElectricField(IdealUndulator const& u) { efield_.add(u); }
+  void ClearElectricFields() { efield_.clear(); }
+  Vec3 GetE(Vec3 const& X) const { return efield_.eval(X); }
+  FieldContainer const& EFields() const { return efield_; }
 
   // beams and particles (OSCARSSR.cc:324-404)
   void AddParticleBeam(std::string const& type, std::string const& name, Vec3 const& x0, Vec3 const& d0, double e_gev,
@@ -241,7 +246,9 @@ class OSCARSSR {
               Precision prec);
   void spectrum_one(Particle& p, Vec3 const& obs, SpectrumContainer& S, double weight, Precision prec);
 
-  FieldContainer bfield_;
+  bool no_fields() const { return bfield_.fields().empty() && efield_.fields().empty(); }
+
+  FieldContainer bfield_, efield_;
   std::vector<ParticleBeam> beams_;
   std::vector<double> beam_weight_sums_;
   std::map<std::string, size_t> beam_index_;

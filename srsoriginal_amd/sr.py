@@ -43,6 +43,9 @@ def _opt3(v, name, default=(0.0, 0.0, 0.0)):
     return _vec3(v, name)
 
 
+_MAGNETIC, _ELECTRIC = 0, 1  # field container of an element (srs_field.target)
+
+
 def _rotate_xyz(v, r):
     """TVector3D::RotateSelfXYZ (TVector3D.cc:183-207) for point/normal lists."""
     x, y, z = v
@@ -169,7 +172,7 @@ class sr:
         field = _rotate_xyz(field, rot)
         per = _rotate_xyz(per, rot)
         self._e.add_undulator(field, per, int(nperiods), float(phase), float(taper), trans,
-                              [float(x) for x in (pole_scales if pole_scales is not None else [])])
+                              [float(x) for x in (pole_scales if pole_scales is not None else [])], _MAGNETIC)
 
     def add_bfield_gaussian(self, bfield, sigma, rotations=None, translation=None):
         """OSCARSSR_AddMagneticFieldGaussian (OSCARSSR_Python.cc:561-636): bfield rotated by
@@ -178,7 +181,7 @@ class sr:
         sig = _vec3(sigma, "sigma")
         rot = _opt3(rotations, "rotations")
         trans = _opt3(translation, "translation")
-        self._e.add_field(1, _rotate_xyz(field, rot), sig, rot, trans)
+        self._e.add_field(1, _rotate_xyz(field, rot), sig, rot, trans, _MAGNETIC)
 
     def add_bfield_uniform(self, bfield, width=None, rotations=None, translation=None):
         """OSCARSSR_AddMagneticFieldUniform (OSCARSSR_Python.cc:640-718): bfield inside the
@@ -187,7 +190,7 @@ class sr:
         w = _opt3(width, "width")
         rot = _opt3(rotations, "rotations")
         trans = _opt3(translation, "translation")
-        self._e.add_field(2, _rotate_xyz(field, rot), w, rot, trans)
+        self._e.add_field(2, _rotate_xyz(field, rot), w, rot, trans, _MAGNETIC)
 
     def add_bfield_file(self, ifile="", iformat="", rotations=None, translation=None, scale=None):
         """OSCARSSR_AddMagneticField (OSCARSSR_Python.cc:337-414): a field map read from
@@ -200,7 +203,7 @@ class sr:
         trans = _opt3(translation, "translation")
         sc = [float(v) for v in (scale or [])]
         try:
-            self._e.add_bfield_file(str(ifile), str(iformat), rot, trans, sc)
+            self._e.add_bfield_file(str(ifile), str(iformat), rot, trans, sc, _MAGNETIC)
         except Exception:
             raise ValueError("Could not import magnetic field.  Check 'ifile' and 'iformat' are correct")
 
@@ -230,6 +233,66 @@ class sr:
 
     def clear_bfields(self):
         self._e.clear_bfields()
+
+    # ---- electric fields (OSCARSSR_Python.cc:876-1290, 1306-1400) -----------------
+    # The same field elements in the electric container (OSCARSSR::AddElectricField); the
+    # trajectory's force is then q/(m gamma) (E + v x B) (OSCARSSR::DerivativesE / EB,
+    # OSCARSSR.cc:690-794), on the host and in the device RK4.
+    def add_efield_undulator(self, efield, period, nperiods, phase=0.0, rotations=None, translation=None):
+        """OSCARSSR_AddElectricFieldIdealUndulator (OSCARSSR_Python.cc:1156-1240): field and
+        period rotated, centred at translation, no taper."""
+        field = _rotate_xyz(_vec3(efield, "efield"), _opt3(rotations, "rotations"))
+        per = _rotate_xyz(_vec3(period, "period"), _opt3(rotations, "rotations"))
+        self._e.add_undulator(field, per, int(nperiods), float(phase), 0.0, _opt3(translation, "translation"), [],
+                              _ELECTRIC)
+
+    def add_efield_gaussian(self, efield, sigma, rotations=None, translation=None):
+        """OSCARSSR_AddElectricFieldGaussian (OSCARSSR_Python.cc:994-1069)."""
+        field = _vec3(efield, "efield")
+        sig = _vec3(sigma, "sigma")
+        rot = _opt3(rotations, "rotations")
+        self._e.add_field(1, _rotate_xyz(field, rot), sig, rot, _opt3(translation, "translation"), _ELECTRIC)
+
+    def add_efield_uniform(self, efield, width=None, rotations=None, translation=None):
+        """OSCARSSR_AddElectricFieldUniform (OSCARSSR_Python.cc:1073-1150)."""
+        field = _vec3(efield, "efield")
+        rot = _opt3(rotations, "rotations")
+        self._e.add_field(2, _rotate_xyz(field, rot), _opt3(width, "width"), rot, _opt3(translation, "translation"),
+                          _ELECTRIC)
+
+    def add_efield_file(self, ifile="", iformat="", rotations=None, translation=None, scale=None):
+        """OSCARSSR_AddElectricField (OSCARSSR_Python.cc:876-954): a field map, as add_bfield_file."""
+        if not ifile or not iformat:
+            raise ValueError("'ifile' or 'iformat' is blank")
+        rot = _opt3(rotations, "rotations")
+        trans = _opt3(translation, "translation")
+        sc = [float(v) for v in (scale or [])]
+        try:
+            self._e.add_bfield_file(str(ifile), str(iformat), rot, trans, sc, _ELECTRIC)
+        except Exception:
+            raise ValueError("Could not import electric field.  Check 'ifile' and 'iformat' are correct")
+
+    def add_field_gaussian(self, sigma, bfield=None, efield=None, rotations=None, translation=None):
+        """OSCARSSR_AddFieldGaussian (OSCARSSR_Python.cc:1306-1400): at most one of bfield and
+        efield.  As in the reference, a bfield adds nothing (that branch is commented out,
+        :1392-1396) and otherwise an electric Gaussian is added (field 0 without efield)."""
+        has_b = bfield is not None and len(bfield) > 0
+        has_e = efield is not None and len(efield) > 0
+        if has_b and has_e:
+            raise ValueError("You cannot define both 'bfield' and 'efield'")
+        field = _vec3(bfield, "bfield") if has_b else (_vec3(efield, "efield") if has_e else [0.0, 0.0, 0.0])
+        sig = _vec3(sigma, "sigma")
+        rot = _opt3(rotations, "rotations")
+        trans = _opt3(translation, "translation")
+        if not has_b:
+            self._e.add_field(1, _rotate_xyz(field, rot), sig, rot, trans, _ELECTRIC)
+
+    def get_efield(self, x):
+        """OSCARSSR_GetEField (OSCARSSR_Python.cc:1244-1278): the summed electric field."""
+        return self._e.get_efield(_vec3(x, "x"))
+
+    def clear_efields(self):
+        self._e.clear_efields()
 
     # ---- beams (OSCARSSR_Python.cc:1641-1938) -----------------------------------
     def add_particle_beam(self, type="", name="", energy_GeV=0.0, d0=None, x0=None, sigma_energy_GeV=0.0, t0=0.0,

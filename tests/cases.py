@@ -43,6 +43,19 @@ def _rotate_xyz(v, r):
     return [x, y, z]
 
 
+def efields(a):
+    """Electric field elements of a case in the harness's order: eund1..2 (no taper;
+    centre after the phase), egauss1..2, ebox1..2, as undulators() describes them."""
+    out = []
+    for k in ("eund1", "eund2"):
+        u = vec(a, k)
+        if not u:
+            continue
+        out.append(dict(field=u[0:3], period=u[3:6], nperiods=int(u[6]), phase=u[7] if len(u) > 7 else 0.0,
+                        taper=0.0, center=u[8:11] if len(u) > 10 else [0.0, 0.0, 0.0]))
+    return out + _local_fields(a, "e")
+
+
 def undulators(a):
     """Field elements of a case in the harness's order: und1..4, gauss1..2, box1..2.
     Gaussian / box entries carry kind 1 / 2, the raw field (rot_field: rotated as the
@@ -54,7 +67,12 @@ def undulators(a):
             continue
         out.append(dict(field=u[0:3], period=u[3:6], nperiods=int(u[6]), phase=u[7] if len(u) > 7 else 0.0,
                         taper=u[8] if len(u) > 8 else 0.0, center=u[9:12] if len(u) > 11 else [0.0, 0.0, 0.0]))
-    for kind, keys, lo in ((1, ("gauss1", "gauss2"), 6), (2, ("box1", "box2"), 3)):
+    return out + _local_fields(a, "")
+
+
+def _local_fields(a, prefix):
+    out = []
+    for kind, keys, lo in ((1, (prefix + "gauss1", prefix + "gauss2"), 6), (2, (prefix + "box1", prefix + "box2"), 3)):
         for k in keys:
             g = vec(a, k)
             if not g or len(g) < lo:
@@ -128,7 +146,7 @@ def points(a):
 def oracle_setup(O, name, npoints=None):
     """Oracle fields, ideal particle, trajectory and dt for a case."""
     a = args(name)
-    F = O.FieldSet(undulators(a), pole(a), None)
+    F = O.FieldSet(undulators(a), pole(a), None, efields(a))
     b = beam(a)
     p = O.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
     c0, c1, n = ct(a)
@@ -217,6 +235,16 @@ def make_sr(name, beams_too=True):
         else:
             osr.add_bfield_undulator(bfield=u["field"], period=u["period"], nperiods=u["nperiods"], phase=u["phase"],
                                      taper=u["taper"], translation=u["center"])
+    for u in efields(a):
+        if u.get("kind") == 1:
+            osr.add_efield_gaussian(efield=u["raw_field"], sigma=u["width"], rotations=u["rotations"],
+                                    translation=u["center"])
+        elif u.get("kind") == 2:
+            osr.add_efield_uniform(efield=u["raw_field"], width=u["width"], rotations=u["rotations"],
+                                   translation=u["center"])
+        else:
+            osr.add_efield_undulator(efield=u["field"], period=u["period"], nperiods=u["nperiods"], phase=u["phase"],
+                                     translation=u["center"])
     add_field_maps(osr, a)
     bs = beams(a)
     if len(bs) == 1 or not beams_too:

@@ -463,6 +463,70 @@ def flux_taper_center():
     return args, {"container": container(a)}, dt
 
 
+# ---- electric fields (OSCARSSR::DerivativesE / DerivativesEB, OSCARSSR.cc:690-794) ----------
+# E only: an electric "undulator" Ex = 1.5e8 V/m sin(2 pi z / 50 mm) (the force of a 0.5 T
+# magnetic one at 3 GeV) and a vertical Gaussian kick; E + B: the C0 undulator, a vertical
+# electric undulator of 30 mm period and phase 0.3, and a longitudinal uniform box.
+EFIELDS_ONLY = ["eund1=1.5e8,0,0,0,0,0.05,41", "egauss1=0,2e6,0,0,0,0.05,0,0,1.0"]
+EFIELDS_MIXED = ["und1=" + UND_C0, "eund1=0,6e7,0,0,0,0.03,50,0.3", "ebox1=0,0,5e5,0,0,0.4,0,0,-1.0"]
+
+
+@case
+def traj_efield_only():
+    args = EFIELDS_ONLY + BEAM + ["ct=0,3", "npoints=4001"]
+    a, dt = run("traj", args)
+    n = int(a[0])
+    return args, {"n": np.array(n), "dt": np.array(a[1]), "traj": a[2:].reshape(9, n)}, dt
+
+
+@case
+def traj_efield_bfield():
+    args = EFIELDS_MIXED + BEAM + ["ct=0,3", "npoints=4001"]
+    a, dt = run("traj", args)
+    n = int(a[0])
+    return args, {"n": np.array(n), "dt": np.array(a[1]), "traj": a[2:].reshape(9, n)}, dt
+
+
+@case
+def flux_efield_only():
+    args = EFIELDS_ONLY + BEAM + ["ct=0,3", "npoints=4001", "plane=XY", "width=0.01,0.01", "nx=9,9",
+                                  "trans=0,0,30", "energy_ev=300", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def flux_efield_bfield():
+    args = EFIELDS_MIXED + BEAM + ["ct=0,3", "npoints=4001", "plane=XY", "width=0.01,0.01", "nx=9,9",
+                                   "trans=0,0,30", "energy_ev=143.5", "nthreads=2"]
+    a, dt = run("flux", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def pd_efield_bfield():
+    args = EFIELDS_MIXED + BEAM + ["ct=0,3", "npoints=4001", "plane=XY", "width=0.03,0.03", "nx=9,7",
+                                   "trans=0,0,30", "nthreads=2"]
+    a, dt = run("pd", args)
+    return args, {"container": container(a)}, dt
+
+
+@case
+def efield_values():
+    """Summed electric fields of both sets at sample points (OSCARSSR::GetE)."""
+    rng = np.random.default_rng(11)
+    p = np.stack([rng.uniform(-0.002, 0.002, 64), rng.uniform(-0.002, 0.002, 64), rng.uniform(-1.4, 1.4, 64)], 1)
+    path = os.path.join(HERE, "fields", "epoints.bin")
+    p.astype("<f8").tofile(path)
+    out, t = {"points": p}, 0.0
+    for name, fl in (("only", EFIELDS_ONLY), ("mixed", EFIELDS_MIXED)):
+        a, dt = run("efield_at", fl + ["pts=fields/epoints.bin"])
+        out[name] = a[1:].reshape(-1, 3)
+        t += dt
+    os.unlink(path)
+    return ["sets=only,mixed"], out, t
+
+
 def _bytes(path):
     with open(path, "rb") as f:
         return np.frombuffer(f.read(), dtype=np.uint8).copy()

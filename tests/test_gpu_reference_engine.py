@@ -53,7 +53,8 @@ def test_reference_engine_spectrum_gpu_route(name):
 
 @needs_harness
 @pytest.mark.parametrize("name", ["flux_c1_small", "flux_points_list", "flux_fields_mixed", "multiflux_c3_small",
-                                  "multiflux_two_beams", "flux_beam_x", "flux_taper_center", "flux_x0x1x2"])
+                                  "multiflux_two_beams", "flux_beam_x", "flux_taper_center", "flux_x0x1x2",
+                                  "flux_efield_only", "flux_efield_bfield"])
 def test_reference_engine_flux_gpu_route(name):
     got = _container(_run("flux", name))
     ref = golden(name)["container"]
@@ -73,7 +74,7 @@ def test_reference_engine_flux_gpu_route(name):
 
 @needs_harness
 @pytest.mark.parametrize("name", ["pd_c2_small", "pd_directional_tilted", "pd_directional_inside", "pd_beam_x",
-                                  "pd_points_rotated_flip", "pd_x0x1x2_normal"])
+                                  "pd_points_rotated_flip", "pd_x0x1x2_normal", "pd_efield_bfield"])
 def test_reference_engine_power_density_gpu_route(name):
     got = _container(_run("pd", name))
     ref = golden(name)["container"]

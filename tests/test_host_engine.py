@@ -247,3 +247,32 @@ def test_three_corner_rectangle_matches_oracle(oracle):
         S = _sr.Surface.rectangle3(int(n[0]), int(n[1]), x0, x1, x2, normal)
         xyz, nrm, x12 = oracle.rect3_surface(int(n[0]), int(n[1]), x0, x1, x2, normal)
         assert np.array_equal(S.xyz, xyz) and np.array_equal(S.normals, nrm)
+
+
+@pytest.mark.parametrize("name", ["traj_efield_only", "traj_efield_bfield"])
+def test_electric_field_trajectories_bitwise(name):
+    # add_efield_undulator / _gaussian / _uniform (OSCARSSR_Python.cc:994-1240) and the
+    # E / E+B derivatives (OSCARSSR.cc:690-794)
+    osr = _engine(name)
+    osr.calculate_trajectory()
+    t, dt = osr.trajectory_array()
+    g = golden(name)
+    assert dt == float(g["dt"])
+    assert np.array_equal(t[:6], g["traj"][:6]) and np.array_equal(t[6:, 1:], g["traj"][6:, 1:])
+
+
+def test_electric_field_values_and_clearing():
+    g = golden("efield_values")
+    for key, name in (("only", "traj_efield_only"), ("mixed", "traj_efield_bfield")):
+        osr = _engine(name)
+        got = np.array([osr.get_efield(list(x)) for x in g["points"]])
+        assert np.array_equal(got, g[key])
+    osr.clear_efields()
+    assert osr.get_efield([0, 0, 0.1]) == [0.0, 0.0, 0.0]
+    # add_field_gaussian: an efield is added; a bfield adds nothing (the reference's branch is commented out)
+    osr.add_field_gaussian(sigma=[0, 0, 0.1], efield=[0, 1e6, 0])
+    assert osr.get_efield([0, 0, 0]) == [0.0, 1e6, 0.0]
+    osr.add_field_gaussian(sigma=[0, 0, 0.1], bfield=[0, 1, 0])
+    assert osr.get_efield([0, 0, 0]) == [0.0, 1e6, 0.0]
+    with pytest.raises(ValueError, match="both"):
+        osr.add_field_gaussian(sigma=[0, 0, 0.1], bfield=[0, 1, 0], efield=[0, 1, 0])

@@ -177,3 +177,42 @@ def test_tapered_offcentre_undulator_bitwise(oracle):
     for D in (-0.8, -0.3, 0.05, 0.41, 0.9):  # the period is along z: D = z - centre_z
         b = oracle.bfield(F, (0.0, 0.0, ctr[2] + D))[1]
         assert abs(b - np.sin(2 * np.pi * (D - ps) / lam) * (1 + D * taper)) < 1e-12
+
+
+def test_electric_field_trajectories_bitwise(oracle):
+    # OSCARSSR::DerivativesE (E only, OSCARSSR.cc:713-737) and DerivativesEB (E and B,
+    # :769-794), picked by SetDerivativesFunction (:690-703)
+    for name in ("traj_efield_only", "traj_efield_bfield"):
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        g = golden(name)
+        assert dt == float(g["dt"])
+        assert np.array_equal(t[:6], g["traj"][:6]) and np.array_equal(t[6:, 1:], g["traj"][6:, 1:])
+    # a plausible slip -- the E-only force taken as E + v x 0 through the magnetic form
+    # -- would still agree; dropping the field does not
+    F, p, t, dt = cases.oracle_setup(oracle, "traj_efield_only")
+    t0, _ = oracle.trajectory(oracle.FieldSet([]), p, 0, 3, t.shape[1])
+    assert np.max(np.abs(t0[3] - t[3])) > 1e-4  # the electric undulator swings beta_x by ~4e-4
+
+
+def test_electric_field_values_bitwise(oracle):
+    g = golden("efield_values")
+    for key, name in (("only", "traj_efield_only"), ("mixed", "traj_efield_bfield")):
+        a = cases.args(name)
+        F = oracle.FieldSet(cases.undulators(a), None, None, cases.efields(a))
+        got = np.array([oracle.efield(F, x) for x in g["points"]])
+        assert np.array_equal(got, g[key])
+
+
+def test_electric_field_flux_and_power_density_bitwise(oracle):
+    for name in ("flux_efield_only", "flux_efield_bfield", "pd_efield_bfield"):
+        F, p, t, dt = cases.oracle_setup(oracle, name)
+        a = cases.args(name)
+        r = cases.rect(a)
+        xyz, nrm, x12 = oracle.rect_surface(r["plane"], r["nx1"], r["nx2"], r["w1"], r["w2"], r["rot"], r["trans"],
+                                            r["normal"])
+        c = golden(name)["container"]
+        if name.startswith("flux"):
+            v = oracle.flux(t, dt, xyz, float(a["energy_ev"]), p.q, p.current, nthreads=2)
+        else:
+            v = oracle.power_density(t, dt, xyz, nrm, False, p.q, p.current, nthreads=2)
+        assert np.array_equal(v, c[:len(xyz), 3])
