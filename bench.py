@@ -166,6 +166,22 @@ def flux_general_c1(traj, dt, q, cur, reps=5):
         "roofline": _frac(F_ALG_FLUX, terms, res[abi.FP64]["ms"], res[abi.MIXED]["ms"])}}
 
 
+def spectrum_c0_setup():
+    """Config 0: the 41 x 50 mm, 1 T undulator, 3 GeV 0.5 A, NT=20001, and the reference's
+    2000-point spectrum grid 100-3000 eV (TSpectrumContainer.cc:82)."""
+    from srsoriginal_amd.sr import sr
+    osr = sr()
+    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.050], nperiods=41)
+    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
+    osr.set_ctstartstop(0, 3)
+    osr.set_npoints_trajectory(20001)
+    osr.calculate_trajectory()
+    traj, dt = osr.trajectory_array()
+    q, cur = osr._e.particle_charge_current()
+    energies = 100.0 + (3000.0 - 100.0) / 1999 * (np.arange(2000) + 1)
+    return traj, dt, q, cur, energies
+
+
 def spectrum_and_power_density(reps=5):
     """Config 0 (FP64 spectrum, 2000 energies at (0,0,30 m)) and config 2 (EPU power
     density on 301x301 over 60 mm at 30 m, NT=40001; mixed, checked against FP64 and
@@ -178,15 +194,7 @@ def spectrum_and_power_density(reps=5):
     stream = torch.cuda.current_stream()
     res = {}
     # config 0
-    osr = sr()
-    osr.add_bfield_undulator(bfield=[0, 1, 0], period=[0, 0, 0.050], nperiods=41)
-    osr.set_particle_beam(type="electron", name="beam", energy_GeV=3, x0=[0, 0, -1.5], d0=[0, 0, 1], current=0.5)
-    osr.set_ctstartstop(0, 3)
-    osr.set_npoints_trajectory(20001)
-    osr.calculate_trajectory()
-    traj, dt = osr.trajectory_array()
-    q, cur = osr._e.particle_charge_current()
-    energies = 100.0 + (3000.0 - 100.0) / 1999 * (np.arange(2000) + 1)  # TSpectrumContainer.cc:82
+    traj, dt, q, cur, energies = spectrum_c0_setup()
     d_traj = torch.from_numpy(np.ascontiguousarray(traj)).to(dev)
     ptrs = [d_traj[k].data_ptr() for k in range(9)]
     terms0 = float(traj.shape[1]) * len(energies)
@@ -325,7 +333,7 @@ def batched_power_density(nparticles=256, reps=2):
             "timed": "device RK4 in time segments overlapped with pd_batch_kernel, reduction (HIP events)"}
 
 
-def batched_spectrum(nparticles=256, reps=2):
+def batched_spectrum(nparticles=4096, reps=1):
     """Multi-particle spectrum (the C3 Gaussian beam on the C0 undulator, ragged 18k-22k
     point trajectories) at (0, 0, 30 m) over the C0 grid of 2000 energies, FP64 and mixed,
     one device pipeline; timed with HIP events."""
@@ -573,12 +581,26 @@ def main():
     ap.add_argument("--no-other", action="store_true", help="skip the config-0 spectrum, config-2 power density and CalculateFlux1 lines")
     ap.add_argument("--profile", action="store_true",
                     help="one launch of each form plus a 256 MiB device copy (the HBM counter scale), for rocprofv3")
+    ap.add_argument("--profile-spectrum", action="store_true",
+                    help="one launch of each form of the config-0 spectrum, for rocprofv3")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.exit("bench.py measures one GPU: this build runs on a single MI355X (multi-GPU sharding is design-only)")
 
     import torch
     from srsoriginal_amd import abi
+
+    if a.profile_spectrum:
+        traj, dt, q, cur, energies = spectrum_c0_setup()
+        d_traj = torch.from_numpy(np.ascontiguousarray(traj)).to("cuda:0")
+        ptrs = [d_traj[k].data_ptr() for k in range(9)]
+        out = torch.zeros(len(energies), dtype=torch.float64, device="cuda:0")
+        s = torch.cuda.current_stream().cuda_stream
+        for prec in (abi.FP64, abi.MIXED):
+            abi.spectrum_device(ptrs, traj.shape[1], dt, (0, 0, 30), energies, q, cur, prec, out.data_ptr(), s)
+        torch.cuda.synchronize()
+        log("profile run: one fp64 and one mixed config-0 spectrum launch")
+        return
 
     traj, dt, xyz, q, cur = setup_problem()
     n, ns = traj.shape[1], len(xyz)

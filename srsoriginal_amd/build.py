@@ -32,6 +32,7 @@ PER_FILE = {
     "srs_mixed.hip": ["-ffp-contract=off"],
     "srs_batch.hip": ["-ffp-contract=off"],
     "srs_adjoint.hip": ["-ffp-contract=off"],
+    "srs_spectrum.hip": ["-ffp-contract=off"],
     "srs_abi.hip": [],
 }
 

@@ -93,6 +93,9 @@ struct Context {
   std::map<std::string, srs::BatchConfig> b_tuned;
   srs::BatchConfig b_last{1, 128};
   int cus = 0;
+  // pinned host flag of the spectrum kernels: the trajectory left the range of the
+  // energy-run phase correction (the call is then evaluated one energy per lane)
+  int* spec_range = nullptr;
   // pole-error gradient: the forward call leaves its tape here for the backward call
   Buf g_ckpt, g_field, g_target, g_loss, g_dloss, g_gy, g_rec, g_grad, g_lam, g_sweep, g_nrm, g_part;
   struct Tape {
@@ -164,6 +167,7 @@ int init(Context& c) {
   void* sp = nullptr;
   TRY(scratch_address(0, &sp));  // set-up of the allocation-free entry point on the library's device
   for (auto& e : c.seg_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIPCHK(hipHostMalloc((void**)&c.spec_range, sizeof(int), hipHostMallocDefault));
   c.ready = true;
   return SRS_OK;
 }
@@ -344,41 +348,38 @@ int run_flux1(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, 
 int run_spectrum(Context& c, const TrajView& t, const double obs[3], const double* energies_ev, int64_t ne, double q,
                  double cur, int precision, double* out, hipStream_t s) {
   double const c0 = c0_of(q), c2 = c2_of(q, cur);
-  std::vector<double> h(3 * ne);
+  int const mixed = precision == SRS_PRECISION_MIXED ? 1 : 0;
+  // per energy omega = (E Qe)/Hbar, c/omega and kw = (C0 omega) dt; per run the
+  // step and |tau| bound of the phase correction
+  int64_t const nr = (ne + srs::kSpecRun - 1) / srs::kSpecRun;
+  std::vector<double> h(3 * ne + 2 * nr);
   for (int64_t e = 0; e < ne; ++e) {
     double const w = omega_of(energies_ev[e]);
     h[e] = w;
     h[ne + e] = kC / w;
     h[2 * ne + e] = (c0 * w) * t.dt;
   }
-  HIPCHK(c.energies.ensure(3 * ne * sizeof(double)));
+  bool uniform = srs::spectrum_runs_uniform(h.data(), ne);
+  if (uniform) srs::spectrum_run_tables(h.data(), ne, mixed, h.data() + 3 * ne, h.data() + 3 * ne + nr);
+  HIPCHK(c.energies.ensure(h.size() * sizeof(double)));
   double* d = c.energies.as<double>();
-  HIPCHK(hipMemcpyAsync(d, h.data(), 3 * ne * sizeof(double), hipMemcpyHostToDevice, s));
-  if (precision == SRS_PRECISION_MIXED) {
-    // 8-wave blocks, energy groups x trajectory chunks ~ 512 (two per CU)
-    int64_t const groups = srs::spectrum_mixed_groups((int)ne);
-    int64_t const nch_want = std::max<int64_t>(1, std::min<int64_t>((512 + groups - 1) / groups, (t.n + 63) / 64));
-    int const chunk = (int)((t.n + nch_want - 1) / nch_want);
-    int const nch = nchunks(t.n, chunk);
-    HIPCHK(c.part.ensure((size_t)nch * 6 * ne * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  srs::SpecTables const tab{d, d + ne, d + 3 * ne, d + 3 * ne + nr};
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  for (;;) {
+    srs::SpectrumPlan const pl = srs::plan_spectrum(t.n, ne, uniform, mixed, c.cus);
+    HIPCHK(c.part.ensure((size_t)pl.nchunk * 6 * ne * sizeof(double)));
+    *c.spec_range = 0;
     {
       KernelTimer timer(c, s);
-      HIPCHK(srs::launch_spectrum_mixed(t, obs, d, d + ne, (int)ne, chunk, 0, c.part.as<double>(), s));
+      HIPCHK(srs::launch_spectrum(t, obs, tab, (int)ne, pl, mixed, c.part.as<double>(), c.spec_range, s));
     }
-    HIPCHK(srs::launch_field_reduce(c.part.as<double>(), nch, (int)ne, d + 2 * ne, c2, out, s));
-  } else {
-    if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
-    int const chunk = pick_chunk(t.n, ne, 256, (int64_t)c.cus * srs::spectrum_fp64_blocks_per_cu());
-    int const nch = nchunks(t.n, chunk);
-    HIPCHK(c.part.ensure((size_t)nch * 12 * ne * sizeof(double)));
-    {
-      KernelTimer timer(c, s);
-      HIPCHK(srs::launch_spectrum_fp64(t, obs, d, d + ne, (int)ne, chunk, c.part.as<double>(), s));
-    }
-    HIPCHK(srs::launch_spectrum_reduce(c.part.as<double>(), nch, (int)ne, d + ne, d + 2 * ne, c2, out, s));
+    HIPCHK(srs::launch_spectrum_fold(c.part.as<double>(), pl.nchunk, (int)ne, d + 2 * ne, c2, out, s));
+    // the host-side energy table must outlive the async copy; the flag is read after
+    HIPCHK(hipStreamSynchronize(s));
+    if (!uniform || *(volatile int*)c.spec_range == 0) break;
+    uniform = false;  // beyond the correction's range: one energy per lane
   }
-  // the host-side energy table must outlive the async copy
-  HIPCHK(hipStreamSynchronize(s));
   return SRS_OK;
 }
 
@@ -1383,26 +1384,32 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   TRY(stage_undulators(c, und, nund, s));
   HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
   HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
-  std::vector<double> eh(3 * ne);
-  double const c0 = c0_of(charge);
+  // per energy omega and c/omega; per run the step and |tau| bound of the phase
+  // correction (srs_spectrum.h)
+  int64_t const nr = (ne + srs::kSpecRun - 1) / srs::kSpecRun;
+  std::vector<double> eh(2 * ne + 2 * nr);
   for (int64_t e = 0; e < ne; ++e) {
     double const w = omega_of(energies_ev[e]);
     eh[e] = w;
     eh[ne + e] = kC / w;
-    eh[2 * ne + e] = c0 * w;
   }
-  HIPCHK(c.b_energies.ensure(3 * ne * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 3 * ne * sizeof(double), hipMemcpyHostToDevice, s));
+  bool const uniform = srs::spectrum_runs_uniform(eh.data(), ne);
+  if (uniform) srs::spectrum_run_tables(eh.data(), ne, mixed ? 1 : 0, eh.data() + 2 * ne, eh.data() + 2 * ne + nr);
+  HIPCHK(c.b_energies.ensure(eh.size() * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), eh.size() * sizeof(double), hipMemcpyHostToDevice, s));
   double* const ed = c.b_energies.as<double>();
+  srs::SpecTables const tab{ed, ed + ne, ed + 2 * ne, ed + 2 * ne + nr};
+  int const run = uniform ? srs::kSpecRun : 1;
   srs::BatchTraj bt;
   TRY(stage_batch_traj(c, total, np, false, &bt));
   int nseg = 1;
   int64_t const L = segment_steps(max_leg, &nseg);
-  // chunks per segment: ~2048 blocks of 256 energies
-  int64_t const tiles = ((ne + srs::kBlock - 1) / srs::kBlock) * np;
+  // chunks per segment: at least ~4 blocks (energy group x particle x chunk) per CU
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  int64_t const tiles = (int64_t)srs::spectrum_batch_groups((int)ne, run) * np;
   int64_t const pts = nseg > 1 ? 2 * L : max_pts;
-  int const nch = (int)std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, (pts + 1023) / 1024));
-  HIPCHK(c.g_part.ensure((size_t)np * nch * 12 * ne * sizeof(double)));
+  int const nch = (int)std::max<int64_t>(1, std::min<int64_t>((4 * c.cus + tiles - 1) / tiles, (pts + 255) / 256));
+  HIPCHK(c.g_part.ensure((size_t)np * nch * 6 * ne * sizeof(double)));
   HIPCHK(c.b_flux_pp.ensure((size_t)np * ne * sizeof(double)));
   HIPCHK(c.b_out.ensure((size_t)ne * sizeof(double)));
   if (nseg > 1) HIPCHK(c.b_state.ensure((size_t)np * 24 * sizeof(double)));
@@ -1410,13 +1417,14 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   auto* Pd = c.b_part.as<srs::BatchParticle>();
   auto* U = c.b_und.as<srs::DevUndulator>();
   srs::Checkpoints const ck{nullptr, 1, 0};
+  *c.spec_range = 0;
   {
     KernelTimer timer(c, s);
     if (nseg == 1) {
       HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
                                    srs::RK4Seg{0, INT64_MAX, nullptr}, s));
-      HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, 0, INT64_MAX, nch, 0,
-                                        c.g_part.as<double>(), mixed, s));
+      HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, tab, (int)ne, run, 0, INT64_MAX, nch, 0,
+                                        c.g_part.as<double>(), c.spec_range, mixed, s));
     } else {
       HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
       HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
@@ -1427,11 +1435,21 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
       }
       for (int k = 0; k < nseg; ++k) {
         HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
-        HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, ed, (int)ne, k * L, (k + 1) * L, nch, k > 0,
-                                          c.g_part.as<double>(), mixed, s));
+        HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, tab, (int)ne, run, k * L, (k + 1) * L, nch, k > 0,
+                                          c.g_part.as<double>(), c.spec_range, mixed, s));
       }
     }
-    HIPCHK(srs::launch_spectrum_batch_reduce(c.g_part.as<double>(), nch, (int)ne, (int)np, Pd, ed + ne, ed,
+    HIPCHK(srs::launch_spectrum_batch_reduce(c.g_part.as<double>(), nch, (int)ne, (int)np, Pd, ed,
+                                             c.b_flux_pp.as<double>(), s));
+    HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), (int)np, (int)ne, weight, c.b_out.as<double>(), s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  if (run > 1 && *(volatile int*)c.spec_range != 0) {
+    // a trajectory left the range of the runs' phase correction: the whole spectrum
+    // again, one energy per lane, over the trajectories the RK4 stored
+    HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, tab, (int)ne, 1, 0, INT64_MAX, nch, 0,
+                                      c.g_part.as<double>(), c.spec_range, mixed, s));
+    HIPCHK(srs::launch_spectrum_batch_reduce(c.g_part.as<double>(), nch, (int)ne, (int)np, Pd, ed,
                                              c.b_flux_pp.as<double>(), s));
     HIPCHK(srs::launch_particle_reduce(c.b_flux_pp.as<double>(), (int)np, (int)ne, weight, c.b_out.as<double>(), s));
   }

@@ -174,16 +174,18 @@ hipError_t launch_pd_batch(BatchTraj const& bt, const BatchParticle* P, int np, 
                            bool mixed, hipStream_t s);
 hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np, const BatchParticle* P,
                                   int directional, double* raw, double* pd, hipStream_t s);
-// Spectrum of np particles at one observation point for ne energies (omega [ne]):
-// launch_spectrum_batch sums the points of RK4 steps [lo, hi) of both legs in nch
-// chunks into part (np * nch * 12 * ne doubles; write or add as launch_pd_batch);
-// launch_spectrum_batch_reduce adds the chunks in order and applies cw = c/omega,
-// i (C0_r omega) dt_r and C2_r (particle r's prefactors): out[r*ne + e].
+// Spectrum of np particles at one observation point for ne energies: launch_spectrum_batch
+// sums the points of RK4 steps [lo, hi) of both legs in nch chunks into part
+// (np * nch * 6 * ne doubles; write or add as launch_pd_batch), with runs of `run`
+// energies per lane (kSpecRun on a uniform grid with the tables of spectrum_run_tables,
+// else 1; *range as launch_spectrum); launch_spectrum_batch_reduce adds the chunks in
+// order and applies i (C0_r omega) dt_r and C2_r (particle r's prefactors): out[r*ne + e].
+int spectrum_batch_groups(int ne, int run);
 hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double obs[3],
-                                 const double* omega, int ne, int64_t lo, int64_t hi, int nch, int add, double* part,
-                                 bool mixed, hipStream_t s);
+                                 SpecTables const& tab, int ne, int run, int64_t lo, int64_t hi, int nch, int add,
+                                 double* part, int* range, bool mixed, hipStream_t s);
 hipError_t launch_spectrum_batch_reduce(const double* part, int nch, int ne, int np, const BatchParticle* P,
-                                        const double* cw, const double* omega, double* out, hipStream_t s);
+                                        const double* omega, double* out, hipStream_t s);
 // dL/dy (gy [6][total]) and dL/dA (ga [3][total]) of L_r = sum_j (pd_rj - target_j)^2
 // scaled by dloss[r].
 hipError_t launch_pd_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,

@@ -14,6 +14,7 @@
 #include "srs_batch.h"
 #include "srs_pd.h"
 #include "srs_rk4.h"
+#include "srs_spectrum.h"
 #include "srs_terms.h"
 
 namespace srs {
@@ -471,26 +472,36 @@ __global__ void pd_batch_reduce_kernel(const double* __restrict__ part, int nch,
 }
 
 // Spectrum of every particle at one observation point over the points of RK4 steps
-// [lo, hi) of both legs (the same index range and chunking as pd_batch_kernel): block
-// (energy tile, particle r, chunk ch) flattened into blockIdx.x, one lane per photon
-// energy; the geometry of each trajectory point (tau, (beta - n)/D, n/D^2, in the
-// reference's operation order as spectrum_fp64_kernel) is staged once in LDS and
-// shared by the block's energies.  part[((r * nch + ch) * 12 + k) * ne + e] = the
-// chunk's sums of A e^{i phi} and G e^{i phi} (12 reals), written or added (add).
-// MIXED: the amplitudes staged as FP32, the reference's FP64 phase omega tau reduced
-// to one turn in FP64, sin / cos in FP32 on revolutions, 32-term FP32
-// sub-sums in Kahan FP32 accumulators; the FP64 form keeps the reference's FP64 terms.
-template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void spectrum_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
-                                                                double ox, double oy, double oz,
-                                                                const double* __restrict__ omega, int ne, int64_t lo,
-                                                                int64_t hi, int nch, int ntiles, int add,
-                                                                double* __restrict__ part) {
-  __shared__ double stau[kBlock], sa[3][MIXED ? 1 : kBlock], sg[3][MIXED ? 1 : kBlock];
-  __shared__ float fa[3][MIXED ? kBlock : 1], fg[3][MIXED ? kBlock : 1];
-  int const tid = threadIdx.x;
-  int const tile = (int)(blockIdx.x % (unsigned)ntiles);
-  int const rc = (int)(blockIdx.x / (unsigned)ntiles);
+// [lo, hi) of both legs (the same index range and chunking as pd_batch_kernel), on the
+// energy-run recurrence of srs_spectrum.h: block (energy group, particle r, chunk ch)
+// flattened into blockIdx.x; a group is 64 lanes x RUN energies; the block's 4 waves
+// split the chunk in a fixed order, each staging the geometry of 64 points at a time
+// in LDS (tau = dt_r it + D/c, (beta - n)/D, n/D^2 in the reference's operation order)
+// and adding their FP64 sums in LDS in wave order at the end:
+// part[((r * nch + ch) * 6 + q) * ne + e], written or added (add).  *range as in
+// spectrum_kernel (srs_spectrum.hip).
+constexpr int kSbWaves = 4;
+constexpr int kSbFold = 4;
+
+template <bool MIXED, int RUN>
+__global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                                       double ox, double oy, double oz, SpecTables tab,
+                                                                       int ne, int64_t lo, int64_t hi, int nch,
+                                                                       int ngroups, int add, double* __restrict__ part,
+                                                                       int* __restrict__ range) {
+  using T = typename SpecLane<MIXED, RUN>::T;
+  constexpr int FR = RUN < kSbFold ? RUN : kSbFold;
+  struct Tile {
+    double tau[64];
+    T a[3][64], g[3][64];
+  };
+  __shared__ union {
+    Tile tile[kSbWaves];
+    double fold[kSbWaves][FR][6][64];
+  } sm;
+  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int const grp = (int)(blockIdx.x % (unsigned)ngroups);
+  int const rc = (int)(blockIdx.x / (unsigned)ngroups);
   int const r = rc / nch, ch = rc % nch;
   BatchParticle const q = P[r];
   int64_t const bl = lo < q.nb ? lo : q.nb, bh = hi < q.nb ? hi : q.nb;
@@ -498,125 +509,88 @@ __global__ __launch_bounds__(kBlock) void spectrum_batch_kernel(BatchTraj bt, co
   int64_t const nbs = bh - bl, n = nbs + (fh - fl);
   int64_t const first_b = q.nb - bh, first_f = q.nb + fl - nbs;  // point index within the particle
   int64_t const chunk = (n + nch - 1) / nch;
-  int64_t const i0 = ch * chunk;
-  int64_t const i1 = (i0 + chunk < n) ? i0 + chunk : n;
-  int const e = tile * kBlock + tid;
-  double const w = omega[e < ne ? e : ne - 1];
-  double acc[12];
-  KahanF facc[MIXED ? 12 : 1];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-#pragma unroll
-  for (int k = 0; k < (MIXED ? 12 : 1); ++k) facc[k] = KahanF{0.f, 0.f};
-  for (int64_t base = i0; base < i1; base += kBlock) {
-    int const cnt = (i1 - base) < kBlock ? (int)(i1 - base) : kBlock;
-    __syncthreads();
-    if (tid < cnt) {
-      int64_t const v = base + tid;
+  int64_t const c0 = ch * chunk;
+  int64_t const c1 = (c0 + chunk < n) ? c0 + chunk : n;
+  int64_t const sub = (chunk + kSbWaves - 1) / kSbWaves;
+  int64_t const i0 = c0 + wave * sub;
+  int64_t const i1 = (i0 + sub < c1) ? i0 + sub : c1;
+  int const ntiles = (int)((sub + 63) / 64);  // the same for every wave of the block
+  SpecLane<MIXED, RUN> L;
+  L.init(tab.omega, tab.cw, tab.rdw, tab.rtl, ne, (grp * 64 + lane) * RUN);
+  Tile& tl = sm.tile[wave];
+  double tmax = 0.0;
+  for (int tt = 0; tt < ntiles; ++tt) {
+    int64_t const base = i0 + (int64_t)tt * 64;
+    int const cnt = base < i1 ? ((i1 - base) < 64 ? (int)(i1 - base) : 64) : 0;
+    if (tt > 0) __syncthreads();
+    if (lane < cnt) {
+      int64_t const v = base + lane;
       int64_t const it = v < nbs ? first_b + v : first_f + v;  // time index of the point
       int64_t const i = q.off + it;
-      double const rx = ox - bt.x[i], ry = oy - bt.y[i], rz = oz - bt.z[i];
-      double const d2 = rx * rx + ry * ry + rz * rz;
-      double const D = __builtin_sqrt(d2);
-      stau[tid] = q.dt * (double)it + div_rn(D, kC, kInvC);
-      double const inv = 1.0 / D;
-      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
-      double const ax = (bt.bx[i] - nx) * inv, ay = (bt.by[i] - ny) * inv, az = (bt.bz[i] - nz) * inv;
-      double const inv2 = inv * inv;
-      double const gx = nx * inv2, gy = ny * inv2, gz = nz * inv2;
-      if constexpr (MIXED) {
-        fa[0][tid] = (float)ax;
-        fa[1][tid] = (float)ay;
-        fa[2][tid] = (float)az;
-        fg[0][tid] = (float)gx;
-        fg[1][tid] = (float)gy;
-        fg[2][tid] = (float)gz;
-      } else {
-        sa[0][tid] = ax;
-        sa[1][tid] = ay;
-        sa[2][tid] = az;
-        sg[0][tid] = gx;
-        sg[1][tid] = gy;
-        sg[2][tid] = gz;
+      SpecPoint G;
+      spec_geometry(ox, oy, oz, bt.x[i], bt.y[i], bt.z[i], bt.bx[i], bt.by[i], bt.bz[i], q.dt * (double)it, G);
+      tl.tau[lane] = G.tau;
+      tmax = __builtin_fmax(tmax, __builtin_fabs(G.tau));
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        tl.a[k][lane] = (T)G.a[k];
+        tl.g[k][lane] = (T)G.g[k];
       }
     }
     __syncthreads();
-    if constexpr (MIXED) {
-      for (int k0 = 0; k0 < cnt; k0 += kSub) {
-        int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
-        float sub[12];
-#pragma unroll
-        for (int qq = 0; qq < 12; ++qq) sub[qq] = 0.f;
-        for (int k = k0; k < k1; ++k) {
-          // the reference's phase p = RN(omega tau), reduced by one exact FMA
-          // (p - k RN(2 pi) is a multiple of 2^-47 below 4 in magnitude); leaving
-          // out 2 pi's low part turns p into p (1 + 3.9e-17), a constant relative
-          // frequency scale whose phase offset cancels in |.|^2
-          double const p = w * stau[k];
-          double const kk = __builtin_rint(p * kInvTwoPi);
-          float const ph = (float)__builtin_fma(-kk, kTwoPiHi, p) * 0.159154943f;
-          float const sn = __builtin_amdgcn_sinf(ph), cn = __builtin_amdgcn_cosf(ph);
-#pragma unroll
-          for (int qq = 0; qq < 3; ++qq) {
-            sub[4 * qq + 0] = __builtin_fmaf(fa[qq][k], cn, sub[4 * qq + 0]);
-            sub[4 * qq + 1] = __builtin_fmaf(fa[qq][k], sn, sub[4 * qq + 1]);
-            sub[4 * qq + 2] = __builtin_fmaf(fg[qq][k], cn, sub[4 * qq + 2]);
-            sub[4 * qq + 3] = __builtin_fmaf(fg[qq][k], sn, sub[4 * qq + 3]);
-          }
-        }
-#pragma unroll
-        for (int qq = 0; qq < 12; ++qq) facc[qq].add(sub[qq]);
-      }
-    } else {
-      for (int k = 0; k < cnt; ++k) {
-        double s, c;
-        sincos_fp64(w * stau[k], &s, &c);
-#pragma unroll
-        for (int qq = 0; qq < 3; ++qq) {
-          acc[4 * qq + 0] = __builtin_fma(sa[qq][k], c, acc[4 * qq + 0]);
-          acc[4 * qq + 1] = __builtin_fma(sa[qq][k], s, acc[4 * qq + 1]);
-          acc[4 * qq + 2] = __builtin_fma(sg[qq][k], c, acc[4 * qq + 2]);
-          acc[4 * qq + 3] = __builtin_fma(sg[qq][k], s, acc[4 * qq + 3]);
-        }
-      }
+    for (int k0 = 0; k0 < cnt; k0 += kSub) {
+      int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
+      for (int k = k0; k < k1; ++k)
+        L.add(tl.tau[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k], tl.g[2][k]);
+      L.flush();
     }
   }
-  if constexpr (MIXED) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = (double)facc[k].s - (double)facc[k].c;
-  }
-  if (e < ne) {
+  for (int m = 32; m > 0; m >>= 1) tmax = __builtin_fmax(tmax, __shfl_xor(tmax, m, 64));
+  if (tmax > L.tau_lim) *range = 1;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      double* const dst = part + (((size_t)r * nch + ch) * 12 + k) * ne + e;
-      *dst = add ? *dst + acc[k] : acc[k];
+  for (int r0 = 0; r0 < RUN; r0 += FR) {
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < FR; ++rr)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sm.fold[wave][rr][k][lane] = L.acc[r0 + rr][k];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < FR * 6 * 64; idx += 64 * kSbWaves) {
+      int const rr = idx / 384, k = (idx / 64) % 6, l = idx % 64;
+      double v = sm.fold[0][rr][k][l];
+#pragma unroll
+      for (int w = 1; w < kSbWaves; ++w) v += sm.fold[w][rr][k][l];
+      int const e = (grp * 64 + l) * RUN + r0 + rr;
+      if (e < ne) {
+        double* const dst = part + (((size_t)r * nch + ch) * 6 + k) * ne + e;
+        *dst = add ? *dst + v : v;
+      }
     }
   }
 }
 
-// Chunks in fixed order, then the reference's prefactors (OSCARSSR.cc:1202-1246):
-// sum (A - i cw G) e^{i phi} per axis, times i (C0 omega) dt_r, F = C2 sum |.|^2;
+// Chunks in fixed order, then the reference's prefactors (OSCARSSR.cc:1202-1246): the
+// field sums (c/omega applied per term) times i (C0_r omega) dt_r, F = C2_r sum |.|^2;
 // out[r * ne + e].
 __global__ void spectrum_batch_reduce_kernel(const double* __restrict__ part, int nch, int ne, int np,
-                                             const BatchParticle* __restrict__ P, const double* __restrict__ cw,
-                                             const double* __restrict__ omega, double* __restrict__ out) {
+                                             const BatchParticle* __restrict__ P, const double* __restrict__ omega,
+                                             double* __restrict__ out) {
   int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)np * ne) return;
   int64_t const r = t / ne, e = t % ne;
-  double v[12];
+  double v[6];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = 0.0;
+  for (int k = 0; k < 6; ++k) v[k] = 0.0;
   for (int c = 0; c < nch; ++c)
 #pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] += part[(((size_t)r * nch + c) * 12 + k) * ne + e];
-  double const w = cw[e], k2 = (P[r].c0 * omega[e]) * P[r].dt;
+    for (int k = 0; k < 6; ++k) v[k] += part[(((size_t)r * nch + c) * 6 + k) * ne + e];
+  double const k2 = (P[r].c0 * omega[e]) * P[r].dt;
   double p[3];
 #pragma unroll
   for (int qq = 0; qq < 3; ++qq) {
-    double const sre = v[4 * qq + 0] + w * v[4 * qq + 3];
-    double const sim = v[4 * qq + 1] - w * v[4 * qq + 2];
-    double const re = -(sim * k2);
-    double const im = sre * k2;
+    double const re = -(v[2 * qq + 1] * k2);
+    double const im = v[2 * qq] * k2;
     p[qq] = re * re + im * im;
   }
   out[t] = P[r].c2 * (p[0] + p[1] + p[2]);
@@ -820,25 +794,37 @@ hipError_t launch_pd_batch_reduce(const double* part, int nslot, int ns, int np,
   return hipGetLastError();
 }
 
+int spectrum_batch_groups(int ne, int run) { return (ne + 64 * run - 1) / (64 * run); }
+
 hipError_t launch_spectrum_batch(BatchTraj const& bt, const BatchParticle* P, int np, const double obs[3],
-                                 const double* omega, int ne, int64_t lo, int64_t hi, int nch, int add, double* part,
-                                 bool mixed, hipStream_t s) {
-  int const ntiles = (ne + kBlock - 1) / kBlock;
-  int64_t const nblocks = (int64_t)ntiles * np * nch;
+                                 SpecTables const& tab, int ne, int run, int64_t lo, int64_t hi, int nch, int add,
+                                 double* part, int* range, bool mixed, hipStream_t s) {
+  int const ng = spectrum_batch_groups(ne, run);
+  int64_t const nblocks = (int64_t)ng * np * nch;
   if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
-  if (mixed)
-    spectrum_batch_kernel<true><<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs[0], obs[1], obs[2], omega, ne, lo, hi,
-                                                                     nch, ntiles, add, part);
-  else
-    spectrum_batch_kernel<false><<<(unsigned)nblocks, kBlock, 0, s>>>(bt, P, obs[0], obs[1], obs[2], omega, ne, lo,
-                                                                      hi, nch, ntiles, add, part);
+  dim3 const grid((unsigned)nblocks), block(64 * kSbWaves);
+  if (mixed) {
+    if (run > 1)
+      spectrum_batch_kernel<true, kSpecRun><<<grid, block, 0, s>>>(bt, P, obs[0], obs[1], obs[2], tab, ne, lo, hi, nch,
+                                                                    ng, add, part, range);
+    else
+      spectrum_batch_kernel<true, 1><<<grid, block, 0, s>>>(bt, P, obs[0], obs[1], obs[2], tab, ne, lo, hi, nch, ng, add,
+                                                             part, range);
+  } else {
+    if (run > 1)
+      spectrum_batch_kernel<false, kSpecRun><<<grid, block, 0, s>>>(bt, P, obs[0], obs[1], obs[2], tab, ne, lo, hi,
+                                                                     nch, ng, add, part, range);
+    else
+      spectrum_batch_kernel<false, 1><<<grid, block, 0, s>>>(bt, P, obs[0], obs[1], obs[2], tab, ne, lo, hi, nch, ng,
+                                                              add, part, range);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_spectrum_batch_reduce(const double* part, int nch, int ne, int np, const BatchParticle* P,
-                                        const double* cw, const double* omega, double* out, hipStream_t s) {
+                                        const double* omega, double* out, hipStream_t s) {
   int64_t const n = (int64_t)np * ne;
-  spectrum_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nch, ne, np, P, cw, omega, out);
+  spectrum_batch_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nch, ne, np, P, omega, out);
   return hipGetLastError();
 }
 

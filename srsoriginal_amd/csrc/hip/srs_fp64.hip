@@ -1,7 +1,7 @@
 // Plain FP64 radiation kernels for gfx950.
 //
-// These reproduce the reference CPU integrands (OSCARSSR.cc:1202-1246 spectrum,
-// :2054-2092 flux, :1522-1573 power density) with the phase path in the
+// These reproduce the reference CPU integrands (OSCARSSR.cc:2054-2092 flux,
+// :1522-1573 power density; the spectrum is in srs_spectrum.hip) with the phase path in the
 // reference's exact operation order, so results agree with the reference to
 // rounding of the amplitude terms (well inside 1e-10 relative):
 //   R = O - X;  D = sqrt((Rx*Rx + Ry*Ry) + Rz*Rz);  tau = dt*i + D/C;  N = R/D
@@ -11,7 +11,7 @@
 // reordering of that chain would break parity; amplitudes tolerate ulp changes.
 //
 // Work decomposition (VALU-bound, trig/division heavy): a block owns 256 output
-// points (observation points, or energies for the spectrum) and one chunk of the
+// points (observation points) and one chunk of the
 // trajectory; the chunk is staged through LDS in 256-point tiles that every lane
 // reads by broadcast.  Each chunk writes FP64 partial sums; a second kernel adds
 // the chunks in fixed order (deterministic) and applies the prefactors.
@@ -60,61 +60,6 @@ __global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const dou
         part[b + (2 * k + 1) * (size_t)ns] = a.im[e][k];
       }
     }
-  }
-}
-
-// Spectrum: one lane per photon energy; the geometry of each trajectory point is
-// shared by all energies, so it is computed once per tile into LDS:
-//   tau = dt*i + D/C (exact reference order), A_k = (beta_k - n_k)/D, G_k = n_k/D^2.
-// Per energy: term_k = (A_k - i cw G_k) e^{i omega tau}; the lane keeps
-// sum A e^{i phi} and sum G e^{i phi} (12 reals) and applies cw at the end.
-__global__ __launch_bounds__(kBlock) void spectrum_fp64_kernel(TrajView t, double ox, double oy, double oz,
-                                                               const double* __restrict__ omega, int ne, int chunk,
-                                                               double* __restrict__ part) {
-  __shared__ double stau[TILE], sa[3][TILE], sg[3][TILE];
-  int const tid = threadIdx.x;
-  int const e = blockIdx.x * kBlock + tid;
-  double const w = omega[e < ne ? e : ne - 1];
-  int64_t const i0 = (int64_t)blockIdx.y * chunk;
-  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
-  double acc[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
-  for (int64_t base = i0; base < i1; base += TILE) {
-    int const cnt = (i1 - base) < TILE ? (int)(i1 - base) : TILE;
-    __syncthreads();
-    if (tid < cnt) {
-      int64_t const i = base + tid;
-      double const rx = ox - t.x[i], ry = oy - t.y[i], rz = oz - t.z[i];
-      double const d2 = rx * rx + ry * ry + rz * rz;
-      double const D = __builtin_sqrt(d2);
-      stau[tid] = t.dt * (double)i + div_rn(D, kC, kInvC);
-      double const inv = 1.0 / D;
-      double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
-      sa[0][tid] = (t.bx[i] - nx) * inv;
-      sa[1][tid] = (t.by[i] - ny) * inv;
-      sa[2][tid] = (t.bz[i] - nz) * inv;
-      double const inv2 = inv * inv;
-      sg[0][tid] = nx * inv2;
-      sg[1][tid] = ny * inv2;
-      sg[2][tid] = nz * inv2;
-    }
-    __syncthreads();
-    for (int k = 0; k < cnt; ++k) {
-      double s, c;
-      sincos_fp64(w * stau[k], &s, &c);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        acc[4 * q + 0] = __builtin_fma(sa[q][k], c, acc[4 * q + 0]);
-        acc[4 * q + 1] = __builtin_fma(sa[q][k], s, acc[4 * q + 1]);
-        acc[4 * q + 2] = __builtin_fma(sg[q][k], c, acc[4 * q + 2]);
-        acc[4 * q + 3] = __builtin_fma(sg[q][k], s, acc[4 * q + 3]);
-      }
-    }
-  }
-  if (e < ne) {
-#pragma unroll
-    for (int k = 0; k < 12; ++k) part[((size_t)blockIdx.y * 12 + k) * ne + e] = acc[k];
   }
 }
 
@@ -291,62 +236,6 @@ __global__ void flux_reduce_kernel(const double* __restrict__ part, int nchunk, 
   out[(size_t)e * ns + j] = c2 * (p[0] + p[1] + p[2]);
 }
 
-// Complex-field partials part[(c*6 + q)*npts + j] with a per-point kw (mixed spectrum).
-// One lane per (point, component) sums the chunks in order (loads coalesce across the
-// points of a block); 42 points x 6 components per 256-lane block, combined in LDS.
-constexpr int kFieldReducePts = 42;
-
-__global__ __launch_bounds__(256) void field_reduce_kernel(const double* __restrict__ part, int nchunk, int npts,
-                                                           const double* __restrict__ kw, double c2,
-                                                           double* __restrict__ out) {
-  __shared__ double v[kFieldReducePts][6];
-  int const jl = threadIdx.x / 6, k = threadIdx.x % 6;
-  int const j = blockIdx.x * kFieldReducePts + jl;
-  if (jl < kFieldReducePts && j < npts) {
-    double acc = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < nchunk; ++c) acc += part[((size_t)c * 6 + k) * npts + j];
-    v[jl][k] = acc;
-  }
-  __syncthreads();
-  if (threadIdx.x >= kFieldReducePts) return;
-  int const jj = blockIdx.x * kFieldReducePts + threadIdx.x;
-  if (jj >= npts) return;
-  double const k2 = kw[jj];
-  double p[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    double const re = -(v[threadIdx.x][2 * q + 1] * k2);
-    double const im = v[threadIdx.x][2 * q] * k2;
-    p[q] = re * re + im * im;
-  }
-  out[jj] = c2 * (p[0] + p[1] + p[2]);
-}
-
-__global__ void spectrum_reduce_kernel(const double* __restrict__ part, int nchunk, int ne, const double* __restrict__ cw,
-                                       const double* __restrict__ kw, double c2, double* __restrict__ out) {
-  int const e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  double v[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = 0.0;
-  for (int c = 0; c < nchunk; ++c)
-#pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] += part[((size_t)c * 12 + k) * ne + e];
-  double const w = cw[e], k2 = kw[e];
-  double p[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    // sum (A - i w G)(c + i s):  re = Ac + w Gs,  im = As - w Gc
-    double const sre = v[4 * q + 0] + w * v[4 * q + 3];
-    double const sim = v[4 * q + 1] - w * v[4 * q + 2];
-    double const re = -(sim * k2);
-    double const im = sre * k2;
-    p[q] = re * re + im * im;
-  }
-  out[e] = c2 * (p[0] + p[1] + p[2]);
-}
-
 __global__ void pd_reduce_kernel(const double* __restrict__ part, int nchunk, int ns, double scale, int directional,
                                  double* __restrict__ out) {
   int const j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -411,11 +300,6 @@ int flux_fp64_blocks_per_cu(int ne) {
   }
 }
 
-int spectrum_fp64_blocks_per_cu() {
-  static int c;
-  return blocks_per_cu(spectrum_fp64_kernel, kBlock, c);
-}
-
 int pd_fp64_blocks_per_cu() {
   static int c;
   return blocks_per_cu(pd_fp64_kernel, kBlock, c);
@@ -446,12 +330,6 @@ hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, co
   return hipGetLastError();
 }
 
-hipError_t launch_spectrum_fp64(const TrajView& t, const double obs[3], const double* omega, const double*, int ne,
-                                int chunk, double* part, hipStream_t s) {
-  dim3 grid((ne + kBlock - 1) / kBlock, nchunks(t.n, chunk));
-  spectrum_fp64_kernel<<<grid, kBlock, 0, s>>>(t, obs[0], obs[1], obs[2], omega, ne, chunk, part);
-  return hipGetLastError();
-}
 
 hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double* nrm_soa, int ns, int directional,
                           int chunk, double* part, hipStream_t s) {
@@ -477,17 +355,7 @@ hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const Ener
   return hipGetLastError();
 }
 
-hipError_t launch_spectrum_reduce(const double* part, int nchunk, int ne, const double* cw, const double* kw, double c2,
-                                  double* out, hipStream_t s) {
-  spectrum_reduce_kernel<<<(ne + 255) / 256, 256, 0, s>>>(part, nchunk, ne, cw, kw, c2, out);
-  return hipGetLastError();
-}
 
-hipError_t launch_field_reduce(const double* part, int nchunk, int npts, const double* kw, double c2, double* out,
-                               hipStream_t s) {
-  field_reduce_kernel<<<(npts + kFieldReducePts - 1) / kFieldReducePts, 256, 0, s>>>(part, nchunk, npts, kw, c2, out);
-  return hipGetLastError();
-}
 
 hipError_t launch_pd_reduce(const double* part, int nchunk, int ns, double scale, int directional, double* out,
                             hipStream_t s) {

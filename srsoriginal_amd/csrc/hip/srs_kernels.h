@@ -27,7 +27,6 @@ hipError_t launch_total_power(const TrajView& t, double scale, double* out, hipS
 
 // ---- resident blocks per CU of the integral kernels (for the chunk choice) ----
 int flux_fp64_blocks_per_cu(int ne);
-int spectrum_fp64_blocks_per_cu();
 int pd_fp64_blocks_per_cu();
 int flux1_fp64_blocks_per_cu();
 int flux_mixed_blocks_per_cu(int ne);
@@ -37,9 +36,6 @@ int pd_mixed_blocks_per_cu(int directional);
 // Flux partial sums: part[((c*ne + e)*6 + k)*ns + j], c = trajectory chunk.
 hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,
                             double* part, hipStream_t s);
-// Spectrum partial sums: part[(c*12 + k)*ne + e] (12 = A and G sums, re/im, xyz).
-hipError_t launch_spectrum_fp64(const TrajView& t, const double obs[3], const double* omega, const double* cw, int ne,
-                                int chunk, double* part, hipStream_t s);
 // Near- plus far-field flux (CalculateFlux1) partial sums: part[(c*6 + k)*ns + j], the
 // layout of launch_flux_fp64 with one energy (flux_reduce_kernel applies C0 dt and C2).
 hipError_t launch_flux1_fp64(const TrajView& t, const double* obs_soa, int ns, double omega, int chunk, double* part,
@@ -51,12 +47,6 @@ hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double
 // ---- mixed form (FP64 phase, FP32 trig / amplitudes / tile sums) -----------
 hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
                              int chunk, double* part, hipStream_t s);
-// Mixed spectrum partials: complex field sums part[(c*6 + q)*ne + e] of trajectory
-// chunk c; the energies form spectrum_mixed_groups(ne) groups of 64 lanes x 8
-// consecutive energies, one block per (group, chunk).
-int spectrum_mixed_groups(int ne);
-hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const double* omega, const double* cw, int ne,
-                                 int chunk, int anchor, double* part, hipStream_t s);
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
                            int directional, int chunk, double* part, hipStream_t s);
 // Single-precision copies used by the mixed kernels: trajf[k*n + i] for
@@ -68,15 +58,43 @@ constexpr int kFlagSpan = 256;
 size_t traj_f32_floats(int64_t n, int with_a);
 hipError_t launch_traj_to_f32(const TrajView& t, int with_a, float* trajf, hipStream_t s);
 
+// ---- spectrum at one observer (srs_spectrum.hip), FP64 or mixed -----------
+constexpr int kSpecRun = 8;  // energies per lane on a uniform grid
+// Whether every run of kSpecRun consecutive energies is a uniform grid up to rounding (then
+// a lane steps across its run by recurrence; otherwise one energy per lane).
+bool spectrum_runs_uniform(const double* omega, int64_t ne);
+struct SpectrumPlan {
+  int run;     // energies per lane (8 or 1)
+  int groups;  // blocks along the energies (64 lanes x run each)
+  int nchunk;  // blocks along the trajectory
+  int sub;     // trajectory points per wave (8 waves per block)
+};
+SpectrumPlan plan_spectrum(int64_t n, int64_t ne, bool uniform, int mixed, int cus);
+// Device tables of a spectrum launch: per energy omega and c/omega; per run of
+// kSpecRun energies the step rdw and the |tau| bound rtl of the phase correction, from
+// spectrum_run_tables (host) for the form (mixed or FP64) that is launched.
+struct SpecTables {
+  const double* omega;
+  const double* cw;
+  const double* rdw;
+  const double* rtl;
+};
+void spectrum_run_tables(const double* omega, int64_t ne, int mixed, double* rdw, double* rtl);
+// Complex field sums of chunk c: part[(c*6 + q)*ne + e], nchunk x 6 x ne doubles.  With
+// runs of 8 the kernel writes 1 to *range (host-visible memory, cleared by the caller)
+// when the trajectory leaves the range of its phase correction; the caller then plans
+// again with uniform = false.
+hipError_t launch_spectrum(const TrajView& t, const double obs[3], const SpecTables& tab, int ne, const SpectrumPlan& p,
+                           int mixed, double* part, int* range, hipStream_t s);
+// out[e] = c2 sum |i kw[e] S|^2 over the chunks' sums S, added in chunk order.
+hipError_t launch_spectrum_fold(const double* part, int nchunk, int ne, const double* kw, double c2, double* out,
+                                hipStream_t s);
+
 // ---- reductions over trajectory chunks (fixed order, deterministic) --------
 hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
                               hipStream_t s);
 // Time-domain electric field: e[3*i + k], arrival time tm[i] for every trajectory point.
 hipError_t launch_efield_time(const TrajView& t, const double obs[3], double c0, double* e, double* tm, hipStream_t s);
-hipError_t launch_spectrum_reduce(const double* part, int nchunk, int ne, const double* cw, const double* kw, double c2,
-                                  double* out, hipStream_t s);
-hipError_t launch_field_reduce(const double* part, int nchunk, int npts, const double* kw, double c2, double* out,
-                               hipStream_t s);
 hipError_t launch_pd_reduce(const double* part, int nchunk, int ns, double scale, int directional, double* out,
                             hipStream_t s);
 

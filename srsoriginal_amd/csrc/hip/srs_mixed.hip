@@ -156,128 +156,6 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
   }
 }
 
-// Spectrum with the energy-grid recurrence: a lane owns RUN consecutive energies.
-// For each trajectory point it anchors exp(i omega_0 tau) with an FP64-reduced
-// phase and steps exp(i d_omega tau) across the run (FP64-reduced as well); the
-// run restarts from a fresh FP64 anchor, so rounding drift is bounded by RUN steps.
-// Uniform steps are exact to the grid's own rounding (the reference computes each
-// energy separately, TSpectrumContainer.cc:82); d_omega is the run's mean spacing.
-//
-// The workload is small (C0: 4e7 terms) and a lane's output (6 sums x RUN energies)
-// is large next to its work, so the block splits its trajectory chunk among its
-// SPW waves (all on the same 64 x RUN energies) and adds the waves' sums in LDS in
-// wave order before writing one partial per block: the FP64 partials, and the chunk
-// reduction that reads them, shrink SPW-fold.
-constexpr int RUN = 8;
-constexpr int SPW = 8;  // waves per spectrum block (8 x 64 lanes: 256 VGPRs for the 48 Kahan sums)
-
-__global__ __launch_bounds__(64 * SPW) void spectrum_mixed_kernel(TrajView t, double ox, double oy, double oz,
-                                                                   const double* __restrict__ omega, int ne, int chunk,
-                                                                   double* __restrict__ part) {
-  __shared__ double stau[SPW][64];
-  __shared__ float sa[SPW][3][64], sg[SPW][3][64];
-  __shared__ double red[SPW][6][64];
-  int const tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  int const e0 = (blockIdx.x * 64 + lane) * RUN;
-  int const ea = e0 < ne ? e0 : ne - 1;
-  int const eb = (e0 + RUN - 1 < ne) ? e0 + RUN - 1 : ne - 1;
-  double const w0 = omega[ea];
-  double const dw = eb > ea ? (omega[eb] - omega[ea]) / (double)(eb - ea) : 0.0;
-  int64_t const i0 = (int64_t)blockIdx.y * chunk;
-  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
-  // this wave's part of the chunk; every wave runs the same number of 64-point tiles
-  int64_t const sub = ((int64_t)chunk + SPW - 1) / SPW;
-  int64_t const w0i = i0 + wave * sub;
-  int64_t const w1i = (w0i + sub < i1) ? w0i + sub : i1;
-  int const ntiles = (int)((sub + 63) / 64);
-  float cw[RUN];
-#pragma unroll
-  for (int r = 0; r < RUN; ++r) {
-    int const e = (e0 + r < ne) ? e0 + r : ne - 1;
-    cw[r] = (float)(kC / omega[e]);
-  }
-  KahanF ex[RUN][6];
-#pragma unroll
-  for (int r = 0; r < RUN; ++r)
-#pragma unroll
-    for (int q = 0; q < 6; ++q) ex[r][q] = KahanF{0.f, 0.f};
-
-  for (int tt = 0; tt < ntiles; ++tt) {
-    int64_t const base = w0i + (int64_t)tt * 64;
-    int const cnt = base < w1i ? ((w1i - base) < 64 ? (int)(w1i - base) : 64) : 0;
-    __syncthreads();
-    if (lane < cnt) {
-      int64_t const i = base + lane;
-      double const rx = ox - t.x[i], ry = oy - t.y[i], rz = oz - t.z[i];
-      double const d2 = rx * rx + ry * ry + rz * rz;
-      double const D = __builtin_sqrt(d2);
-      stau[wave][lane] = t.dt * (double)i + D / kC;
-      double const inv = 1.0 / D;
-      double const nx = rx * inv, ny = ry * inv, nz = rz * inv;
-      sa[wave][0][lane] = (float)((t.bx[i] - nx) * inv);
-      sa[wave][1][lane] = (float)((t.by[i] - ny) * inv);
-      sa[wave][2][lane] = (float)((t.bz[i] - nz) * inv);
-      double const inv2 = inv * inv;
-      sg[wave][0][lane] = (float)(nx * inv2);
-      sg[wave][1][lane] = (float)(ny * inv2);
-      sg[wave][2][lane] = (float)(nz * inv2);
-    }
-    __syncthreads();
-    for (int k0 = 0; k0 < cnt; k0 += SUB) {
-      int const k1 = (k0 + SUB < cnt) ? k0 + SUB : cnt;
-      float sub_s[RUN][6];
-#pragma unroll
-      for (int r = 0; r < RUN; ++r)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) sub_s[r][q] = 0.f;
-      for (int k = k0; k < k1; ++k) {
-        double const tau = stau[wave][k];
-        float ps, pc, ws, wc;
-        sincos_reduced_f32(reduce_two_pi(w0 * tau), &ps, &pc);
-        sincos_reduced_f32(reduce_two_pi(dw * tau), &ws, &wc);
-        float const a0 = sa[wave][0][k], a1 = sa[wave][1][k], a2 = sa[wave][2][k];
-        float const g0 = sg[wave][0][k], g1 = sg[wave][1][k], g2 = sg[wave][2][k];
-#pragma unroll
-        for (int r = 0; r < RUN; ++r) {
-          float const cs = cw[r] * ps, cc = cw[r] * pc;
-          sub_s[r][0] = __builtin_fmaf(a0, pc, __builtin_fmaf(g0, cs, sub_s[r][0]));
-          sub_s[r][1] = __builtin_fmaf(a0, ps, __builtin_fmaf(-g0, cc, sub_s[r][1]));
-          sub_s[r][2] = __builtin_fmaf(a1, pc, __builtin_fmaf(g1, cs, sub_s[r][2]));
-          sub_s[r][3] = __builtin_fmaf(a1, ps, __builtin_fmaf(-g1, cc, sub_s[r][3]));
-          sub_s[r][4] = __builtin_fmaf(a2, pc, __builtin_fmaf(g2, cs, sub_s[r][4]));
-          sub_s[r][5] = __builtin_fmaf(a2, ps, __builtin_fmaf(-g2, cc, sub_s[r][5]));
-          // advance the phasor to the next energy of the run
-          float const npc = __builtin_fmaf(pc, wc, -ps * ws);
-          float const nps = __builtin_fmaf(ps, wc, pc * ws);
-          pc = npc;
-          ps = nps;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RUN; ++r)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) ex[r][q].add(sub_s[r][q]);
-    }
-  }
-  // combine the waves in wave order, one energy of the run at a time; partial layout
-  // part[(c*6 + q)*ne + e] holds the complex field sums (re/im per axis)
-#pragma unroll
-  for (int r = 0; r < RUN; ++r) {
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 6; ++q) red[wave][q][lane] = (double)ex[r][q].s - (double)ex[r][q].c;
-    __syncthreads();
-    if (tid < 6 * 64) {
-      int const q = tid / 64, l = tid % 64;
-      double v = 0.0;
-#pragma unroll
-      for (int w = 0; w < SPW; ++w) v += red[w][q][l];
-      int const e = (blockIdx.x * 64 + l) * RUN + r;
-      if (e < ne) part[((size_t)blockIdx.y * 6 + q) * ne + e] = v;
-    }
-  }
-}
-
 // Mixed power density, FP32 throughout the term.  With u = n - beta and |n| = 1
 // (n is the unit vector (n_x, n_y, 1 - (1 - n_z)) whatever the rounding of 1 - n_z):
 //   1 - beta.n     = (|u|^2 + 1 - |beta|^2) / 2      (a sum of non-negative terms)
@@ -430,14 +308,6 @@ hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double
   return hipGetLastError();
 }
 
-int spectrum_mixed_groups(int ne) { return (ne + 64 * RUN - 1) / (64 * RUN); }
-
-hipError_t launch_spectrum_mixed(const TrajView& t, const double obs[3], const double* omega, const double*, int ne,
-                                 int chunk, int, double* part, hipStream_t s) {
-  dim3 grid(spectrum_mixed_groups(ne), nchunks(t.n, chunk));
-  spectrum_mixed_kernel<<<grid, 64 * SPW, 0, s>>>(t, obs[0], obs[1], obs[2], omega, ne, chunk, part);
-  return hipGetLastError();
-}
 
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
                            int directional, int chunk, double* part, hipStream_t s) {

@@ -157,6 +157,53 @@ def test_mixed_spectrum_c0_within_1e5(abi, oracle):
     assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
 
 
+# The mixed spectrum reproduces the reference's FP64 phase RN(omega tau) for every
+# energy (the recurrence over a run of 8 energies plus an exact FP64 correction), so it
+# stays within ~2e-7 of the FP64 result where the FP64 result itself is 1e-5..1e-4 away
+# from the integral evaluated in extended precision (high harmonics at 10 keV).
+TOL_MIXED_SPECTRUM = 2e-6
+SPEC_OBSERVERS = [(0.0, 0.0, 30.0), (1.2e-3, -0.4e-3, 30.0), (-2.0e-3, 1.5e-3, 30.0)]
+SPEC_GRIDS = {"c0 grid 101-3001 eV": None, "100-10000 eV": (100.0, 10000.0)}
+
+
+@pytest.mark.parametrize("grid", list(SPEC_GRIDS))
+@pytest.mark.parametrize("obs", SPEC_OBSERVERS)
+def test_spectrum_observers_and_10kev(abi, oracle, obs, grid):
+    F, p, t, dt = cases.oracle_setup(oracle, "spectrum_c0_full")
+    s = golden("spectrum_c0_full")["spectrum"]
+    lim = SPEC_GRIDS[grid]
+    es = s[:, 0].copy() if lim is None else lim[0] + (lim[1] - lim[0]) / 1999 * np.arange(2000)
+    f64 = abi.spectrum(t, dt, obs, es, p.q, p.current, abi.FP64)
+    mix = abi.spectrum(t, dt, obs, es, p.q, p.current, abi.MIXED)
+    sub = slice(3, None, 50)  # 40 energies through the FP64 oracle (the reference's arithmetic)
+    ref = oracle.spectrum(t, dt, obs, es[sub], p.q, p.current, nthreads=8)
+    e64 = normwise(f64[sub], ref)
+    e1, e2 = normwise(mix, f64), rms_rel(mix, f64)
+    say("spectrum %s at %s: fp64 vs oracle %.3e; mixed vs fp64 normwise %.3e rms %.3e" % (grid, obs, e64, e1, e2))
+    assert e64 <= TOL_FP64
+    assert e1 <= TOL_MIXED_SPECTRUM and e2 <= TOL_MIXED_SPECTRUM
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_spectrum_far_observer_and_nonuniform_grid(abi, oracle, prec):
+    """Runs of 8 energies need |tau| |theta bound| within the correction's series range:
+    an observer 3 km away (tau ~ 1e-5 s) makes the kernel report it and the call is
+    evaluated one energy per lane; a geometric energy grid is not a uniform run and goes
+    that way from the start.  Both against the oracle."""
+    P = abi.FP64 if prec == "fp64" else abi.MIXED
+    tol = TOL_FP64 if prec == "fp64" else TOL_MIXED_SPECTRUM
+    F, p, t, dt = cases.oracle_setup(oracle, "spectrum_c0_full")
+    s = golden("spectrum_c0_full")["spectrum"]
+    for obs, es in (((0.0, 0.0, 3000.0), s[::10, 0].copy()), ((0.05, 0.0, 3000.0), s[::10, 0].copy()),
+                    ((0.0, 0.0, 30.0), np.geomspace(150.0, 2900.0, 301))):
+        ref = oracle.spectrum(t, dt, obs, es, p.q, p.current, nthreads=8)
+        got = abi.spectrum(t, dt, obs, es, p.q, p.current, P)
+        err = normwise(got, ref)
+        say("spectrum %s at %s, %d energies (%s grid): normwise vs oracle %.3e" %
+            (prec, obs, len(es), "geometric" if obs[2] == 30.0 else "uniform", err))
+        assert err <= tol
+
+
 def test_mixed_power_density_c2_within_1e5(abi, oracle):
     say("power density C2 mixed vs reference fp64")
     F, p, t, dt = cases.oracle_setup(oracle, "pd_c2_full")
