@@ -253,6 +253,18 @@ int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, in
                        double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
                        double weight, int precision, int flags, double* flux_out);
 
+/* The multi-particle flux of np particles drawn on the device (the draws of
+ * srs_hip_sample_beams: particle i from counter first + i under key seed) straight into
+ * the batch pipeline's particle records, with no host round trip: each particle takes its
+ * beam's charge, current, ct range and point count; then srs_hip_batch_flux's pipeline
+ * (flags as there).  The device counterpart of the reference's per-particle loop over
+ * GetNewParticle (OSCARSSR.cc:2465-2479), which has no GPU form in the reference.
+ * Beams need a nonzero charge. */
+int srs_hip_batch_flux_beams(const srs_field* und, int nund, const srs_beam* beams, const double* weights,
+                             int nbeams, int64_t np, uint64_t seed, uint64_t first, const double* obs_xyz, int64_t ns,
+                             const double* energies_ev, int64_t ne, double weight, int precision, int flags,
+                             double* flux_out);
+
 /* Multi-particle power density [W/mm^2] on ns points with normals (AoS xyz):
  * pd_out[j] = sum over particles (in order, Kahan) of weight * P_p(j), each P_p as
  * srs_hip_power_density with `directional`.  Replaces the nparticles loop of

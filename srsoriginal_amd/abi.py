@@ -347,6 +347,30 @@ def batch_flux(und_specs, parts, charge, current, obs_xyz, energies_ev, weight, 
     return out.reshape(len(e), len(o))
 
 
+def batch_flux_beams(und_specs, beams, weights, np_, seed, obs_xyz, energies_ev, weight, precision=FP64, flags=0,
+                     first=0, ct_start=0.0, ct_stop=1.0, npoints=2):
+    """srs_hip_batch_flux_beams: the batch flux of np_ particles drawn on the device (the
+    draws of sample_beams, counters first .. first + np_ - 1 under key seed) straight into
+    the pipeline; beams as sample_beams takes them, every particle with the beam's ct range
+    and point count (ct_start, ct_stop, npoints, or per beam from its dict)."""
+    L = lib()
+    L.srs_hip_batch_flux_beams.argtypes = [C.POINTER(SrsField), C.c_int, C.POINTER(SrsBeam), _P, C.c_int, _I64,
+                                           C.c_uint64, C.c_uint64, _P, _I64, _P, _I64, _D, C.c_int, C.c_int, _P]
+    L.srs_hip_batch_flux_beams.restype = C.c_int
+    keep = []
+    U = undulators(und_specs, keep)
+    arr = (SrsBeam * len(beams))(*[_srs_beam(b, b.get("ct_start", ct_start), b.get("ct_stop", ct_stop),
+                                             b.get("npoints", npoints)) for b in beams])
+    w = _f64(weights)
+    o = _f64(obs_xyz).reshape(-1, 3)
+    e = _f64(np.atleast_1d(energies_ev))
+    out = np.zeros(len(e) * len(o))
+    _check(L.srs_hip_batch_flux_beams(U, len(U), arr, w.ctypes.data_as(_P), len(beams), int(np_),
+                                      C.c_uint64(int(seed)), C.c_uint64(int(first)), o.ctypes.data_as(_P), len(o),
+                                      e.ctypes.data_as(_P), len(e), weight, precision, flags, out.ctypes.data_as(_P)))
+    return out.reshape(len(e), len(o))
+
+
 def batch_power_density(und_specs, parts, charge, current, obs_xyz, normals_xyz, directional, weight, precision=FP64):
     L = _bind_batch()
     keep = []

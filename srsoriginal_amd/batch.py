@@ -52,6 +52,22 @@ def sample_particles(osr, n, ct_start, ct_stop, npoints, ct_jitter=0.0, points_p
     return rows, q, cur
 
 
+def flux_device_sampled(osr, n, ct_start, ct_stop, npoints, seed, obs_xyz, energies_ev, precision="double",
+                        first=0, beam=None, graph=True, autotune=True):
+    """Sum over n particles of flux/n on obs_xyz (N x 3) at each energy -> array (ne, N),
+    the particles drawn on the device as sample_particles_device draws them and written
+    straight into the pipeline's particle records (srs_hip_batch_flux_beams): no host
+    round trip between the sampler and the work queue."""
+    names = osr._e.beam_names()
+    if not names:
+        raise ValueError("No particle beam defined")
+    bps = [osr._e.beam_params(beam)] if beam is not None else [osr._e.beam_params(nm) for nm in names]
+    prec = {"double": abi.FP64, "fp64": abi.FP64, "mixed": abi.MIXED}[precision]
+    flags = (abi.BATCH_GRAPH if graph else 0) | (abi.BATCH_AUTOTUNE if autotune else 0)
+    return abi.batch_flux_beams(osr._e.undulators(), bps, [b["weight"] for b in bps], n, seed, obs_xyz, energies_ev,
+                                1.0 / n, prec, flags, first, ct_start, ct_stop, npoints)
+
+
 def flux(osr, rows, charge, current, obs_xyz, energies_ev, precision="double", graph=True, autotune=True):
     """Sum over particles of flux/NP on obs_xyz (N x 3) at each energy -> array (ne, N)."""
     prec = {"double": abi.FP64, "fp64": abi.FP64, "mixed": abi.MIXED}[precision]

@@ -85,6 +85,7 @@ struct Context {
   // batched path
   Buf b_part, b_und, b_poles, b_grid, b_traj, b_trajf, b_energies, b_flux_pp, b_out, b_counter, b_obs;
   Buf b_state, b_field;  // RK4 state between segments; field sums accumulated over segments
+  Buf b_beams;           // beam records of the device-sampled batch
   bool und_local = false;  // the staged field list holds elements other than magnetic undulators
   int und_force = srs::kForceB;  // the derivative form: magnetic, electric or both kinds of elements
   hipGraph_t b_graph = nullptr;
@@ -838,6 +839,182 @@ Context::Tape& pole_tape(Context& c, PoleRun const& pr, int nreal, int ns, int n
   return t;
 }
 
+// Device records of nbeams beams and their cumulative weights (the running sums of
+// TParticleBeamContainer::AddNewParticleBeam); with grid, also each beam's batch
+// prefactors (its charge and current) and time grid (OSCARSSR.cc:574-598).
+int beam_records(const srs_beam* beams, const double* weights, int nbeams, bool grid, std::vector<srs::DevBeam>& db,
+                 std::vector<double>& wsum) {
+  if (!beams || (nbeams > 1 && !weights)) return fail(SRS_ERR_INVALID, "NULL argument");
+  if (nbeams < 1) return fail(SRS_ERR_INVALID, "no beam defined");
+  db.assign(nbeams, srs::DevBeam{});
+  wsum.assign(nbeams, 0.0);
+  double acc = 0;
+  for (int k = 0; k < nbeams; ++k) {
+    srs_beam const& beam = beams[k];
+    if (!(beam.mass > 0)) return fail(SRS_ERR_INVALID, "beam particle mass must be > 0");
+    double const mgev = 1e-9 * beam.mass * kC * kC / kQe;  // TSRS::kgToGeV, TOSCARSSR.h:68
+    if (!(beam.e0 > mgev)) return fail(SRS_ERR_INVALID, "beam energy must exceed the particle's rest energy");
+    if (!(beam.sigma_e >= 0)) return fail(SRS_ERR_INVALID, "'sigma_energy_GeV' cannot be less than zero");
+    if (beam.npoints < 2) return fail(SRS_ERR_INVALID, "npoints must be >= 2");
+    double const w = nbeams > 1 ? weights[k] : 1.0;
+    if (!(w >= 0)) return fail(SRS_ERR_INVALID, "beam weights must be >= 0");
+    acc += w;
+    wsum[k] = acc;
+    srs::DevBeam& b = db[k];
+    b.e0 = beam.e0;
+    b.sigma_e = beam.sigma_e;
+    b.mgev = mgev;
+    b.q = beam.charge;
+    b.m = beam.mass;
+    b.cur = beam.current != 0 ? beam.current : beam.charge;  // TParticleA.cc:258-262
+    for (int i = 0; i < 3; ++i) {
+      b.x0[i] = beam.x0[i];
+      b.u0[i] = beam.u0[i];
+      b.h[i] = beam.hdir[i];
+      b.v[i] = beam.vdir[i];
+      b.at[i] = beam.sigma_at[i];
+    }
+    for (int i = 0; i < 2; ++i) {
+      b.su[i] = beam.sigma_u[i];
+      b.sp[i] = beam.sigma_up[i];
+    }
+    b.t0 = beam.t0;
+    b.ct_start = beam.ct_start;
+    b.ct_stop = beam.ct_stop;
+    b.npoints = beam.npoints;
+    if (grid) {
+      if (!(beam.charge != 0)) return fail(SRS_ERR_INVALID, "beam charge must be nonzero");
+      b.c0 = c0_of(b.q);
+      b.c2 = c2_of(b.q, b.cur);
+      b.pdq = pd_qfac_of(b.q, b.cur);
+      srs_particle g{};
+      g.t0 = beam.t0;
+      g.ct_start = beam.ct_start;
+      g.ct_stop = beam.ct_stop;
+      g.npoints = beam.npoints;
+      TRY(particle_grid(g, &b.dt, &b.gnb, &b.gnf));
+    }
+  }
+  if (!(acc > 0)) return fail(SRS_ERR_INVALID, "beam weights must have a positive sum");
+  return SRS_OK;
+}
+
+// The batch flux pipeline for np particle records already in b_part (total trajectory
+// points, longest leg max_leg) and the field list in b_und: observation points and
+// energies staged, the tile shape (autotuned on request), then the pipeline eagerly or
+// as a replayed hipGraph, and the result copied to flux_out.  Called with c.mu held.
+int batch_flux_staged(Context& c, hipStream_t s, int nund, int64_t np, int64_t total, int64_t max_leg, double charge,
+                      double current, const double* obs_xyz, int64_t ns, const double* energies_ev, int64_t ne,
+                      double weight, int precision, int flags, double* flux_out) {
+  TRY(stage_points(c, c.b_obs, obs_xyz, ns));
+  double const c0 = c0_of(charge);
+  std::vector<double> eh(5 * ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    double const w = omega_of(energies_ev[e]);
+    eh[e] = w;
+    eh[ne + e] = kC / w;
+    eh[2 * ne + e] = c0 * w;
+    eh[3 * ne + e] = w / kTwoPi;
+    eh[4 * ne + e] = eh[3 * ne + e] / kC;
+  }
+  HIPCHK(c.b_energies.ensure(5 * ne * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 5 * ne * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c.b_flux_pp.ensure((size_t)np * ne * ns * sizeof(double)));
+  HIPCHK(c.b_out.ensure((size_t)ne * ns * sizeof(double)));
+  HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
+  BatchRun r;
+  r.nund = nund;
+  TRY(stage_batch_traj(c, total, np, false, &r.bt));
+  double* ed = c.b_energies.as<double>();
+  r.be = srs::BatchEnergies{ed, ed + ne, ed + 2 * ne, ed + 3 * ne, ed + 4 * ne, (int)ne, c2_of(charge, current)};
+  r.np = (int)np;
+  r.ns = (int)ns;
+  r.ne = (int)ne;
+  r.mixed = precision == SRS_PRECISION_MIXED;
+  r.weight = weight;
+  r.max_leg = max_leg;
+  // the segmented pipeline's buffers are sized here: nothing may allocate while a
+  // graph is being captured
+  HIPCHK(c.b_state.ensure((size_t)np * 24 * sizeof(double)));
+  HIPCHK(c.b_field.ensure((size_t)np * ne * 6 * ns * sizeof(double)));
+  HIPCHK(hipStreamSynchronize(s));  // host staging vectors
+
+  // tile shape: default, or the autotuned one for this problem signature
+  srs::BatchConfig cfg{(int)std::min<int64_t>(ne, 4) == 3 ? 3 : (ne >= 4 ? 4 : (int)ne), 128};
+  if (!srs::batch_config_valid(cfg)) cfg = srs::BatchConfig{1, 128};
+  char sig[128];
+  snprintf(sig, sizeof sig, "%d/%lld/%lld/%lld", (int)r.mixed, (long long)((ns + 255) / 256), (long long)ne,
+           (long long)(total / np));
+  if (flags & SRS_BATCH_AUTOTUNE) {
+    auto it = c.b_tuned.find(sig);
+    if (it != c.b_tuned.end()) {
+      cfg = it->second;
+    } else {
+      // time each candidate on a subset of the particles (same RK4 and inputs)
+      static const srs::BatchConfig cands[] = {{1, 64}, {1, 128}, {1, 256}, {2, 128}, {2, 256},
+                                               {3, 64}, {3, 128}, {3, 256}, {4, 128}, {4, 256}};
+      int const nsub = (int)std::min<int64_t>(np, 64);
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      float best = 1e30f;
+      bool const tsave = c.timing;
+      c.timing = false;
+      for (auto const& k : cands) {
+        if (k.ne_per_item > ne || (k.ne_per_item > 1 && ne % k.ne_per_item != 0 && k.ne_per_item != ne)) continue;
+        if (!srs::batch_config_valid(k)) continue;
+        TRY(enqueue_batch(c, r, k, nsub, s));  // warm-up
+        HIPCHK(hipEventRecord(e0, s));
+        TRY(enqueue_batch(c, r, k, nsub, s));
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          cfg = k;
+        }
+      }
+      c.timing = tsave;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      c.b_tuned[sig] = cfg;
+    }
+  }
+  c.b_last = cfg;
+
+  if (flags & SRS_BATCH_GRAPH) {
+    std::string const key = batch_key(r, cfg, c);
+    if (key != c.b_key || !c.b_exec) {
+      if (c.b_exec) (void)hipGraphExecDestroy(c.b_exec);
+      if (c.b_graph) (void)hipGraphDestroy(c.b_graph);
+      c.b_exec = nullptr;
+      c.b_graph = nullptr;
+      bool const tsave = c.timing;
+      c.timing = false;  // no timing events inside the captured graph
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int const rc = enqueue_batch(c, r, cfg, r.np, s);
+      hipGraph_t g = nullptr;
+      hipError_t const ec = hipStreamEndCapture(s, &g);
+      c.timing = tsave;
+      if (rc != SRS_OK) return rc;
+      if (ec != hipSuccess) return fail(SRS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      c.b_graph = g;
+      HIPCHK(hipGraphInstantiate(&c.b_exec, c.b_graph, nullptr, nullptr, 0));
+      c.b_key = key;
+    }
+    KernelTimer timer(c, s);
+    HIPCHK(hipGraphLaunch(c.b_exec, s));
+  } else {
+    KernelTimer timer(c, s);
+    TRY(enqueue_batch(c, r, cfg, r.np, s));
+  }
+  HIPCHK(hipMemcpyAsync(flux_out, c.b_out.p, (size_t)ne * ns * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SRS_OK;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -958,47 +1135,10 @@ int srs_hip_sample_beams(const srs_beam* beams, const double* weights, int nbeam
                          uint64_t first, srs_particle* out) {
   g_err.clear();
   if (!beams || !out || (nbeams > 1 && !weights)) return fail(SRS_ERR_INVALID, "NULL argument");
-  if (nbeams < 1) return fail(SRS_ERR_INVALID, "no beam defined");
   if (np < 1) return fail(SRS_ERR_INVALID, "no particles requested");
-  std::vector<srs::DevBeam> db(nbeams);
-  std::vector<double> wsum(nbeams);
-  double acc = 0;
-  for (int k = 0; k < nbeams; ++k) {
-    srs_beam const& beam = beams[k];
-    if (!(beam.mass > 0)) return fail(SRS_ERR_INVALID, "beam particle mass must be > 0");
-    double const mgev = 1e-9 * beam.mass * kC * kC / kQe;  // TSRS::kgToGeV, TOSCARSSR.h:68
-    if (!(beam.e0 > mgev)) return fail(SRS_ERR_INVALID, "beam energy must exceed the particle's rest energy");
-    if (!(beam.sigma_e >= 0)) return fail(SRS_ERR_INVALID, "'sigma_energy_GeV' cannot be less than zero");
-    if (beam.npoints < 2) return fail(SRS_ERR_INVALID, "npoints must be >= 2");
-    double const w = nbeams > 1 ? weights[k] : 1.0;
-    if (!(w >= 0)) return fail(SRS_ERR_INVALID, "beam weights must be >= 0");
-    acc += w;  // the running sums of TParticleBeamContainer::AddNewParticleBeam
-    wsum[k] = acc;
-    srs::DevBeam& b = db[k];
-    b = srs::DevBeam{};
-    b.e0 = beam.e0;
-    b.sigma_e = beam.sigma_e;
-    b.mgev = mgev;
-    b.q = beam.charge;
-    b.m = beam.mass;
-    b.cur = beam.current != 0 ? beam.current : beam.charge;  // TParticleA.cc:258-262
-    for (int i = 0; i < 3; ++i) {
-      b.x0[i] = beam.x0[i];
-      b.u0[i] = beam.u0[i];
-      b.h[i] = beam.hdir[i];
-      b.v[i] = beam.vdir[i];
-      b.at[i] = beam.sigma_at[i];
-    }
-    for (int i = 0; i < 2; ++i) {
-      b.su[i] = beam.sigma_u[i];
-      b.sp[i] = beam.sigma_up[i];
-    }
-    b.t0 = beam.t0;
-    b.ct_start = beam.ct_start;
-    b.ct_stop = beam.ct_stop;
-    b.npoints = beam.npoints;
-  }
-  if (!(acc > 0)) return fail(SRS_ERR_INVALID, "beam weights must have a positive sum");
+  std::vector<srs::DevBeam> db;
+  std::vector<double> wsum;
+  TRY(beam_records(beams, weights, nbeams, false, db, wsum));
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
   TRY(init(c));
@@ -1208,112 +1348,54 @@ int srs_hip_batch_flux(const srs_field* und, int nund, const srs_particle* p, in
   TRY(stage_undulators(c, und, nund, s));
   HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
   HIPCHK(hipMemcpyAsync(c.b_part.p, P.data(), np * sizeof(srs::BatchParticle), hipMemcpyHostToDevice, s));
-  TRY(stage_points(c, c.b_obs, obs_xyz, ns));
-  double const c0 = c0_of(charge);
-  std::vector<double> eh(5 * ne);
-  for (int64_t e = 0; e < ne; ++e) {
-    double const w = omega_of(energies_ev[e]);
-    eh[e] = w;
-    eh[ne + e] = kC / w;
-    eh[2 * ne + e] = c0 * w;
-    eh[3 * ne + e] = w / kTwoPi;
-    eh[4 * ne + e] = eh[3 * ne + e] / kC;
-  }
-  HIPCHK(c.b_energies.ensure(5 * ne * sizeof(double)));
-  HIPCHK(hipMemcpyAsync(c.b_energies.p, eh.data(), 5 * ne * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(c.b_flux_pp.ensure((size_t)np * ne * ns * sizeof(double)));
-  HIPCHK(c.b_out.ensure((size_t)ne * ns * sizeof(double)));
-  HIPCHK(c.b_counter.ensure(sizeof(unsigned)));
-  BatchRun r;
-  r.nund = nund;
-  TRY(stage_batch_traj(c, total, np, false, &r.bt));
-  double* ed = c.b_energies.as<double>();
-  r.be = srs::BatchEnergies{ed, ed + ne, ed + 2 * ne, ed + 3 * ne, ed + 4 * ne, (int)ne, c2_of(charge, current)};
-  r.np = (int)np;
-  r.ns = (int)ns;
-  r.ne = (int)ne;
-  r.mixed = precision == SRS_PRECISION_MIXED;
-  r.weight = weight;
-  for (auto const& q : P) r.max_leg = std::max(r.max_leg, std::max(q.nf, q.nb));
-  // the segmented pipeline's buffers are sized here: nothing may allocate while a
-  // graph is being captured
-  HIPCHK(c.b_state.ensure((size_t)np * 24 * sizeof(double)));
-  HIPCHK(c.b_field.ensure((size_t)np * ne * 6 * ns * sizeof(double)));
-  HIPCHK(hipStreamSynchronize(s));  // host staging vectors
+  int64_t max_leg = 0;
+  for (auto const& q : P) max_leg = std::max(max_leg, std::max(q.nf, q.nb));
+  return batch_flux_staged(c, s, nund, np, total, max_leg, charge, current, obs_xyz, ns, energies_ev, ne, weight,
+                           precision, flags, flux_out);
+}
 
-  // tile shape: default, or the autotuned one for this problem signature
-  srs::BatchConfig cfg{(int)std::min<int64_t>(ne, 4) == 3 ? 3 : (ne >= 4 ? 4 : (int)ne), 128};
-  if (!srs::batch_config_valid(cfg)) cfg = srs::BatchConfig{1, 128};
-  char sig[128];
-  snprintf(sig, sizeof sig, "%d/%lld/%lld/%lld", (int)r.mixed, (long long)((ns + 255) / 256), (long long)ne,
-           (long long)(total / np));
-  if (flags & SRS_BATCH_AUTOTUNE) {
-    auto it = c.b_tuned.find(sig);
-    if (it != c.b_tuned.end()) {
-      cfg = it->second;
-    } else {
-      // time each candidate on a subset of the particles (same RK4 and inputs)
-      static const srs::BatchConfig cands[] = {{1, 64}, {1, 128}, {1, 256}, {2, 128}, {2, 256},
-                                               {3, 64}, {3, 128}, {3, 256}, {4, 128}, {4, 256}};
-      int const nsub = (int)std::min<int64_t>(np, 64);
-      hipEvent_t e0, e1;
-      HIPCHK(hipEventCreate(&e0));
-      HIPCHK(hipEventCreate(&e1));
-      float best = 1e30f;
-      bool const tsave = c.timing;
-      c.timing = false;
-      for (auto const& k : cands) {
-        if (k.ne_per_item > ne || (k.ne_per_item > 1 && ne % k.ne_per_item != 0 && k.ne_per_item != ne)) continue;
-        if (!srs::batch_config_valid(k)) continue;
-        TRY(enqueue_batch(c, r, k, nsub, s));  // warm-up
-        HIPCHK(hipEventRecord(e0, s));
-        TRY(enqueue_batch(c, r, k, nsub, s));
-        HIPCHK(hipEventRecord(e1, s));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          cfg = k;
-        }
-      }
-      c.timing = tsave;
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      c.b_tuned[sig] = cfg;
-    }
+int srs_hip_batch_flux_beams(const srs_field* und, int nund, const srs_beam* beams, const double* weights,
+                             int nbeams, int64_t np, uint64_t seed, uint64_t first, const double* obs_xyz, int64_t ns,
+                             const double* energies_ev, int64_t ne, double weight, int precision, int flags,
+                             double* flux_out) {
+  g_err.clear();
+  TRY(check_common(ns, precision));
+  if (ne < 1) return fail(SRS_ERR_INVALID, "no photon energies");
+  if (np < 1) return fail(SRS_ERR_INVALID, "no particles");
+  if (np > (1 << 24)) return fail(SRS_ERR_INVALID, "too many particles");
+  if (!obs_xyz || !energies_ev || !flux_out) return fail(SRS_ERR_INVALID, "NULL argument");
+  std::vector<srs::DevBeam> db;
+  std::vector<double> wsum;
+  TRY(beam_records(beams, weights, nbeams, true, db, wsum));
+  // every particle gets its beam's time grid; trajectories are laid out at a fixed
+  // stride (the longest beam's point count)
+  int64_t stride = 0, max_leg = 0;
+  for (auto const& b : db) {
+    stride = std::max(stride, b.gnb + b.gnf);
+    max_leg = std::max(max_leg, std::max(b.gnb, b.gnf));
   }
-  c.b_last = cfg;
-
-  if (flags & SRS_BATCH_GRAPH) {
-    std::string const key = batch_key(r, cfg, c);
-    if (key != c.b_key || !c.b_exec) {
-      if (c.b_exec) (void)hipGraphExecDestroy(c.b_exec);
-      if (c.b_graph) (void)hipGraphDestroy(c.b_graph);
-      c.b_exec = nullptr;
-      c.b_graph = nullptr;
-      bool const tsave = c.timing;
-      c.timing = false;  // no timing events inside the captured graph
-      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int const rc = enqueue_batch(c, r, cfg, r.np, s);
-      hipGraph_t g = nullptr;
-      hipError_t const ec = hipStreamEndCapture(s, &g);
-      c.timing = tsave;
-      if (rc != SRS_OK) return rc;
-      if (ec != hipSuccess) return fail(SRS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-      c.b_graph = g;
-      HIPCHK(hipGraphInstantiate(&c.b_exec, c.b_graph, nullptr, nullptr, 0));
-      c.b_key = key;
-    }
-    KernelTimer timer(c, s);
-    HIPCHK(hipGraphLaunch(c.b_exec, s));
-  } else {
-    KernelTimer timer(c, s);
-    TRY(enqueue_batch(c, r, cfg, r.np, s));
-  }
-  HIPCHK(hipMemcpyAsync(flux_out, c.b_out.p, (size_t)ne * ns * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return SRS_OK;
+  if ((double)stride * np > (double)((int64_t)1 << 40)) return fail(SRS_ERR_INVALID, "too many trajectory points");
+  int64_t const ntiles = (ns + srs::kBlock - 1) / srs::kBlock;
+  if ((double)np * ntiles * ne > 4.0e9) return fail(SRS_ERR_INVALID, "too many work items");
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  TRY(init(c));
+  c.tape.valid = false;  // the batch buffers below are shared with the gradient tape
+  hipStream_t s = c.stream;
+  if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
+  TRY(stage_undulators(c, und, nund, s));
+  // beam records | cumulative weights, 256-byte aligned; the records of the particles
+  // are drawn into b_part on the device
+  size_t const beam_bytes = ((nbeams * sizeof(srs::DevBeam) + 255) / 256) * 256;
+  HIPCHK(c.b_beams.ensure(beam_bytes + nbeams * sizeof(double)));
+  auto* const dbeams = c.b_beams.as<srs::DevBeam>();
+  auto* const dw = reinterpret_cast<double*>(c.b_beams.as<char>() + beam_bytes);
+  HIPCHK(hipMemcpyAsync(dbeams, db.data(), nbeams * sizeof(srs::DevBeam), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(dw, wsum.data(), nbeams * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c.b_part.ensure(np * sizeof(srs::BatchParticle)));
+  HIPCHK(srs::launch_sample_batch(dbeams, dw, nbeams, seed, first, np, stride, c.b_part.as<srs::BatchParticle>(), s));
+  return batch_flux_staged(c, s, nund, np, np * stride, max_leg, db[0].q, db[0].cur, obs_xyz, ns, energies_ev, ne,
+                           weight, precision, flags, flux_out);
 }
 
 int srs_hip_batch_power_density(const srs_field* und, int nund, const srs_particle* p, int64_t np, double charge,

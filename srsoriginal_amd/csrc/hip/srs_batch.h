@@ -104,6 +104,10 @@ struct DevBeam {
   double su[2], sp[2];
   double t0, ct_start, ct_stop;
   int64_t npoints;
+  // the batch records of its particles (launch_sample_batch): prefactors c0, c2, pdq
+  // of (q, cur) and the time grid dt, gnb, gnf of (t0, ct_start, ct_stop, npoints)
+  double c0, c2, pdq, dt;
+  int64_t gnb, gnf;
 };
 struct SampledParticle {
   double x0[3], beta0[3], qmg, t0, ct_start, ct_stop;
@@ -113,6 +117,9 @@ struct SampledParticle {
 // beams: nb device records; wsum: their cumulative weights (device)
 hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
                               int64_t np, SampledParticle* out, hipStream_t s);
+// The same draws as batch records (out[p]: trajectory offset p * stride).
+hipError_t launch_sample_batch(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
+                               int64_t np, int64_t stride, BatchParticle* out, hipStream_t s);
 
 // local: the list holds elements other than magnetic undulators (Gaussian, box, field
 // map, or any electric element); force: kForceB (magnetic elements only), kForceE

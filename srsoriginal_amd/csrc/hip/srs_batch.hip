@@ -649,20 +649,18 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
   }
 }
 
-// One lane per particle: counter first + p, three Philox blocks (counter (lo, hi, j, 0))
-// -> 53-bit uniforms u1 in (0, 1], u2 in [0, 1) -> Box-Muller normals; then
-// TParticleBeam::GetNewParticle (TParticleBeam.cc:228-265) in the reference's operation
-// order with the draws n0..n4 (energy, x, y, x', y'), and TParticleA's gamma and
-// q/(m gamma) from the drawn beta (TParticleA.cc:353-382).
-__global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restrict__ beams,
-                                                          const double* __restrict__ wsum, int nb, uint64_t seed,
-                                                          uint64_t first, int64_t np,
-                                                          SampledParticle* __restrict__ out) {
-  int64_t const p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  uint64_t const idx = first + (uint64_t)p;
+// Particle idx of the beams under key seed: the beam k with u * W < w_0 + ... + w_k
+// (u a fourth Philox block's 53-bit uniform, W the weights' sum; no draw for one beam,
+// TParticleBeamContainer::GetRandomBeamIndexByWeight), then three Philox blocks (counter
+// (lo, hi, j, 0)) -> 53-bit uniforms u1 in (0, 1], u2 in [0, 1) -> Box-Muller normals,
+// and TParticleBeam::GetNewParticle (TParticleBeam.cc:228-265) in the reference's
+// operation order with the draws n0..n4 (energy, x, y, x', y'), and TParticleA's gamma
+// and q/(m gamma) from the drawn beta (TParticleA.cc:353-382).
+__device__ __forceinline__ int draw_particle(const DevBeam* __restrict__ beams, const double* __restrict__ wsum,
+                                             int nb, uint64_t seed, uint64_t idx, double X[3], double B[3],
+                                             double* qmg_out) {
   int sel = 0;
-  if (nb > 1) {  // TParticleBeamContainer::GetRandomBeamIndexByWeight
+  if (nb > 1) {
     uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 3u, 0u};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     uint64_t const a = (((uint64_t)c[0] << 32) | c[1]) >> 11;
@@ -674,7 +672,7 @@ __global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restr
         break;
       }
   }
-  DevBeam const b = beams[sel];
+  DevBeam const& b = beams[sel];
   double n[6];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
@@ -690,7 +688,6 @@ __global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restr
   double const gamma = enew / b.mgev;
   double const beta = sqrt(1.0 - 1.0 / (gamma * gamma));
   double const dmid = (b.at[0] - b.x0[0]) * b.u0[0] + (b.at[1] - b.x0[1]) * b.u0[1] + (b.at[2] - b.x0[2]) * b.u0[2];
-  double X[3], B[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) X[k] = b.x0[k] + ((b.h[k] * b.su[0]) * (1 + dmid)) * n[1];
 #pragma unroll
@@ -713,6 +710,20 @@ __global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restr
     double const g = 1.0 / sqrt(1.0 - b2);
     if (b.m != 0 && g != 0) qmg = b.q / b.m / g;
   }
+  *qmg_out = qmg;
+  return sel;
+}
+
+// One lane per particle: counter first + p, the row of srs_particle.
+__global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restrict__ beams,
+                                                          const double* __restrict__ wsum, int nb, uint64_t seed,
+                                                          uint64_t first, int64_t np,
+                                                          SampledParticle* __restrict__ out) {
+  int64_t const p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double X[3], B[3], qmg;
+  int const sel = draw_particle(beams, wsum, nb, seed, first + (uint64_t)p, X, B, &qmg);
+  DevBeam const& b = beams[sel];
   SampledParticle& o = out[p];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -728,9 +739,43 @@ __global__ __launch_bounds__(256) void sample_beam_kernel(const DevBeam* __restr
   o.current = b.cur;
 }
 
+// The same draws written straight into the batch records: the beam's prefactors and
+// time grid, the particle's trajectory at p * stride.
+__global__ __launch_bounds__(256) void sample_batch_kernel(const DevBeam* __restrict__ beams,
+                                                           const double* __restrict__ wsum, int nb, uint64_t seed,
+                                                           uint64_t first, int64_t np, int64_t stride,
+                                                           BatchParticle* __restrict__ out) {
+  int64_t const p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double X[3], B[3], qmg;
+  int const sel = draw_particle(beams, wsum, nb, seed, first + (uint64_t)p, X, B, &qmg);
+  DevBeam const& b = beams[sel];
+  BatchParticle& o = out[p];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o.x0[k] = X[k];
+    o.beta0[k] = B[k];
+  }
+  o.qmg = qmg;
+  o.dt = b.dt;
+  o.nb = b.gnb;
+  o.nf = b.gnf;
+  o.off = p * stride;
+  o.pole_base = 0;
+  o.c0 = b.c0;
+  o.c2 = b.c2;
+  o.pdq = b.pdq;
+}
+
 hipError_t launch_sample_beam(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
                               int64_t np, SampledParticle* out, hipStream_t s) {
   sample_beam_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(beams, wsum, nb, seed, first, np, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_batch(const DevBeam* beams, const double* wsum, int nb, uint64_t seed, uint64_t first,
+                               int64_t np, int64_t stride, BatchParticle* out, hipStream_t s) {
+  sample_batch_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(beams, wsum, nb, seed, first, np, stride, out);
   return hipGetLastError();
 }
 
