@@ -53,7 +53,7 @@ def _frac(flop_per_term, terms, fp64_ms, mixed_ms):
 # HBM bytes per launch of flux_mixed_kernel<1,256> on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
 # what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
-TRAFFIC_MIXED_BYTES = 46193.1 * 1024
+TRAFFIC_MIXED_BYTES = 46139.4 * 1024
 
 
 def log(msg):
@@ -88,8 +88,10 @@ def batched_c3(nparticles=4096, reps=1):
                           sigma_energy_GeV=8.9e-4 * 3, beta=[1.5, 0.8], emittance=[0.9e-9, 8e-12],
                           horizontal_direction=[1, 0, 0], lattice_reference=[0, 0, 0])
     osr.set_seed(4096)
+    t0 = time.perf_counter()
     rows, q, cur = batch.sample_particles(osr, nparticles, 0.0, 3.0, 20001, ct_jitter=0.1, points_per_meter=6667,
                                           rng_seed=4096)
+    host_sampling_s = time.perf_counter() - t0
     xyz = _sr.Surface.rectangle("XY", 101, 101, 0.01, 0.01, [0, 0, 0], [0, 0, 30], 0).xyz
     es = [430.0, 431.2, 432.4]
     npts = abi.trajectory_points(abi.particles(rows))
@@ -109,6 +111,17 @@ def batched_c3(nparticles=4096, reps=1):
         log("batched C3 %s: %.3f s/eval (graph %.1f ms), config %s" % (prec, wall, res[prec]["graph_ms"],
                                                                         res[prec]["cfg"]))
     f64, mix = res["double"]["out"], res["mixed"]["out"]
+    # the same beam drawn on the device straight into the pipeline (uniform 20001-point
+    # trajectories: the device sampler gives every particle its beam's ct range)
+    dev = batch.flux_device_sampled(osr, nparticles, 0.0, 3.0, 20001, 4096, xyz, es, "mixed")  # warm-up + capture
+    abi.timing_enable(True)
+    t0 = time.perf_counter()
+    dev = batch.flux_device_sampled(osr, nparticles, 0.0, 3.0, 20001, 4096, xyz, es, "mixed")
+    dev_wall = time.perf_counter() - t0
+    kms, nl = abi.timing_read()
+    abi.timing_enable(False)
+    log("batched C3 device-sampled mixed: %.3f s/eval (graph %.1f ms); host sampling of the ragged set %.3f s"
+        % (dev_wall, kms / max(nl, 1), host_sampling_s))
     return {"workload": "%d particles x %d traj points (ragged %d-%d) x 101x101 x 3 energies" %
                         (nparticles, npts, min(lens), max(lens)),
             "terms": terms,
@@ -119,6 +132,11 @@ def batched_c3(nparticles=4096, reps=1):
             "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(mix - f64)) / np.max(np.abs(f64))),
             "mixed_vs_fp64_rms_rel_err": float(np.sqrt(np.mean((mix - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2))),
             "tile_config_fp64": res["double"]["cfg"], "tile_config_mixed": res["mixed"]["cfg"],
+            "host_sampling_s": host_sampling_s,
+            "device_sampled_mixed": {"particles": nparticles, "traj_points": 20001, "wall_s": dev_wall,
+                                     "graph_ms": kms / max(nl, 1), "peak_flux": float(np.max(dev)),
+                                     "path": "srs_hip_batch_flux_beams: device sampler -> particle records -> "
+                                             "RK4 + flux + reduction, no host round trip"},
             "timed": "whole hipGraph (device RK4 + persistent flux + particle reduction), HIP events; "
                      "%d timed evaluation(s) per form after one warm-up" % reps}
 
