@@ -696,7 +696,7 @@ def main():
                                      "bound_ms_at_2p4GHz": terms / 64 * ISSUE_CYCLES_PER_WAVE_TERM / (SIMDS * CLOCK_HZ)
                                      * 1e3,
                                      "frac": terms / 64 * ISSUE_CYCLES_PER_WAVE_TERM / (SIMDS * CLOCK_HZ) / kernel_s},
-                     "kernel": "flux_mixed_kernel<1,256>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
+                     "kernel": "flux_mixed2_kernel<256>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
                      "flop_per_term": F_ALG_FLUX},
     }
     if not a.no_batch:

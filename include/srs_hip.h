@@ -118,8 +118,9 @@ int srs_hip_flux_device(const srs_traj* traj, const double* obs_soa, int64_t ns,
  * shares that GPU's scratch: calls that overlap on one GPU (on different streams) race.
  * Nothing in it has to start at zero (each call writes all it reads).  The library's
  * first call on a device, whichever entry point, also sets it up (streams, the scratch
- * address): make one call before capturing a graph. */
-#define SRS_NOALLOC_SCRATCH_BYTES (64u << 20)
+ * address): make one call before capturing a graph.  128 MiB holds, for example, a
+ * 201 x 201 map at one energy over 20001 trajectory points (50 chunks). */
+#define SRS_NOALLOC_SCRATCH_BYTES (128u << 20)
 int srs_hip_flux_device_noalloc(const srs_traj* traj, const double* obs_soa, int64_t ns, const double* energies_ev,
                                 int64_t ne, double charge, double current, int precision, double* flux_out,
                                 void* stream);

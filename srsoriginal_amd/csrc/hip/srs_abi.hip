@@ -241,6 +241,29 @@ int pick_chunk(int64_t n, int64_t lanes, int align, int64_t resident) {
 }
 int nchunks(int64_t n, int chunk) { return (int)((n + chunk - 1) / chunk); }
 
+// Trajectory chunks of a flux launch: pick_chunk's chunk c1, and, when the launch takes
+// more than one round of resident blocks, its last part re-split into chunks of c1 / 8
+// (32-aligned), about one round of them (resident / point blocks).  Blocks run in chunk
+// order, so the blocks that drain the device at the end are the short ones.
+constexpr int kTailRatio = 8;
+constexpr double kTailRounds = 1.0;
+srs::ChunkMap flux_chunks(int64_t n, int64_t lanes, int align, int64_t resident) {
+  int const c1 = pick_chunk(n, lanes, align, resident);
+  int64_t const bx = (lanes + srs::kBlock - 1) / srs::kBlock;
+  int const nu = nchunks(n, c1);
+  srs::ChunkMap m{c1, nu, c1, nu};
+  int64_t const c2 = (int64_t)(c1 / kTailRatio + align - 1) / align * align;
+  if (bx * nu <= resident || c2 >= c1 || nu < 2) return m;
+  int64_t const want = (int64_t)std::ceil(kTailRounds * (double)resident / (double)bx) * c2;
+  int64_t const n1 = std::max<int64_t>(1, (n - std::min<int64_t>(want, n)) / c1);
+  int64_t const rest = n - n1 * c1;
+  if (rest <= 0) return m;
+  m.n1 = (int)n1;
+  m.c2 = (int)c2;
+  m.nch = (int)(n1 + (rest + c2 - 1) / c2);
+  return m;
+}
+
 // Copy a host trajectory into the context's device buffer (9 SoA arrays).
 int stage_traj(Context& c, const srs_traj* h, bool need_a, TrajView* v) {
   size_t const n = (size_t)h->n;
@@ -267,7 +290,8 @@ TrajView view_of(const srs_traj* t) {
 // (ne0 = energies per launch, at most kMaxFluxEnergies; launches of later energy groups
 // reuse them in stream order).
 struct FluxPlan {
-  int chunk = 0, nch = 0, ne0 = 0;
+  srs::ChunkMap cm{};
+  int nch = 0, ne0 = 0;
   size_t part_off = 0, bytes = 0;
 };
 
@@ -276,8 +300,9 @@ int plan_flux(Context& c, const TrajView& t, int64_t ns, int64_t ne, int precisi
   pl->ne0 = (int)std::min<int64_t>(srs::kMaxFluxEnergies, ne);
   int const per_cu = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_blocks_per_cu(pl->ne0)
                                                       : srs::flux_fp64_blocks_per_cu(pl->ne0);
-  pl->chunk = pick_chunk(t.n, ns, 32, (int64_t)c.cus * per_cu);
-  pl->nch = nchunks(t.n, pl->chunk);
+  int64_t const ppl = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_points_per_lane(pl->ne0) : 1;
+  pl->cm = flux_chunks(t.n, (ns + ppl - 1) / ppl, 32, (int64_t)c.cus * per_cu);
+  pl->nch = pl->cm.nch;
   size_t const staged = precision == SRS_PRECISION_MIXED ? srs::traj_f32_floats(t.n, 0) * sizeof(float) : 0;
   pl->part_off = (staged + 255) / 256 * 256;
   pl->bytes = pl->part_off + (size_t)pl->nch * pl->ne0 * 6 * ns * sizeof(double);
@@ -307,9 +332,9 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
     {
       KernelTimer timer(c, s);
       if (precision == SRS_PRECISION_MIXED) {
-        HIPCHK(srs::launch_flux_mixed(t, trajf, obs_soa, (int)ns, es, pl.chunk, part, s));
+        HIPCHK(srs::launch_flux_mixed(t, trajf, obs_soa, (int)ns, es, pl.cm, part, s));
       } else {
-        HIPCHK(srs::launch_flux_fp64(t, obs_soa, (int)ns, es, pl.chunk, part, s));
+        HIPCHK(srs::launch_flux_fp64(t, obs_soa, (int)ns, es, pl.cm, part, s));
       }
     }
     HIPCHK(srs::launch_flux_reduce(part, pl.nch, (int)ns, es, c2, out + e0 * ns, s));

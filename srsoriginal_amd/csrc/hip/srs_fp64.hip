@@ -26,14 +26,14 @@ constexpr int TILE = kBlock;
 
 template <int NE, int TILE_>
 __global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const double* __restrict__ obs, int ns,
-                                                            EnergySet es, int chunk, double* __restrict__ part) {
+                                                            EnergySet es, ChunkMap cm, double* __restrict__ part) {
   __shared__ FluxTile<TILE_> T;
   int const tid = threadIdx.x;
   int const j = blockIdx.x * kBlock + tid;
   int const jj = j < ns ? j : ns - 1;  // padding lanes compute a valid point, never store
   double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
-  int64_t const i0 = (int64_t)blockIdx.y * chunk;
-  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  int64_t const i0 = cm.begin(blockIdx.y);
+  int64_t const i1 = cm.end(blockIdx.y, t.n);
   AccFP64<NE> a;
   a.zero();
   for (int64_t base = i0; base < i1; base += TILE_) {
@@ -214,17 +214,34 @@ __global__ __launch_bounds__(kBlock) void flux1_fp64_kernel(TrajView t, const do
 // Chunk reductions (fixed order) and the reference's final prefactors:
 //   flux:  SumE *= C1*dt (OSCARSSR.cc:2080) -> (re, im) * (0 + i kw) = (-im kw, re kw);
 //          F = C2 * Re(SumE . conj(SumE)) summed x, y then z (OSCARSSR.cc:2092).
-__global__ void flux_reduce_kernel(const double* __restrict__ part, int nchunk, int ns, EnergySet es, double c2,
-                                   double* __restrict__ out) {
-  int const j = blockIdx.x * blockDim.x + threadIdx.x;
+// A block folds 64 points: wave w adds chunks w, w + 4, w + 8, ... of each point in
+// order, then wave 0 adds the four slice sums in order 0, 1, 2, 3 (a fixed order: the
+// result does not depend on timing).  Four waves per 64 points keep four times the
+// loads in flight of one lane per point.
+__global__ __launch_bounds__(256) void flux_reduce_kernel(const double* __restrict__ part, int nchunk, int ns,
+                                                          EnergySet es, double c2, double* __restrict__ out) {
+  __shared__ double red[3][6][64];
+  int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int const j = blockIdx.x * 64 + lane;
   int const e = blockIdx.y;
-  if (j >= ns) return;
   double v[6] = {0, 0, 0, 0, 0, 0};
-  for (int c = 0; c < nchunk; ++c) {
-    size_t const b = ((size_t)(c * es.ne + e) * 6) * ns + j;
+  if (j < ns) {
+    for (int c = w; c < nchunk; c += 4) {
+      size_t const b = ((size_t)(c * es.ne + e) * 6) * ns + j;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] += part[b + (size_t)k * ns];
+      for (int k = 0; k < 6; ++k) v[k] += part[b + (size_t)k * ns];
+    }
   }
+  if (w > 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[w - 1][k][lane] = v[k];
+  }
+  __syncthreads();
+  if (w > 0 || j >= ns) return;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] += red[r][k][lane];
   double const kw = es.kw[e];
   double p[3];
 #pragma unroll
@@ -317,14 +334,15 @@ hipError_t launch_flux1_fp64(const TrajView& t, const double* obs_soa, int ns, d
   return hipGetLastError();
 }
 
-hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,
-                            double* part, hipStream_t s) {
-  dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es,
+                            ChunkMap const& cm, double* part, hipStream_t s) {
+  if (cm.c1 < 1 || cm.c2 < 1 || cm.nch < 1 || cm.end(cm.nch - 1, t.n) != t.n) return hipErrorInvalidValue;
+  dim3 grid((ns + kBlock - 1) / kBlock, cm.nch);
   switch (es.ne) {
-    case 1: flux_fp64_kernel<1, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
-    case 2: flux_fp64_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
-    case 3: flux_fp64_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
-    case 4: flux_fp64_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, chunk, part); break;
+    case 1: flux_fp64_kernel<1, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, cm, part); break;
+    case 2: flux_fp64_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, cm, part); break;
+    case 3: flux_fp64_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, cm, part); break;
+    case 4: flux_fp64_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, obs_soa, ns, es, cm, part); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -350,7 +368,7 @@ hipError_t launch_total_power(const TrajView& t, double scale, double* out, hipS
 
 hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
                               hipStream_t s) {
-  dim3 grid((ns + 255) / 256, es.ne);
+  dim3 grid((ns + 63) / 64, es.ne);
   flux_reduce_kernel<<<grid, 256, 0, s>>>(part, nchunk, ns, es, c2, out);
   return hipGetLastError();
 }

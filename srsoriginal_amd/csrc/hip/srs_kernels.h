@@ -22,6 +22,21 @@ struct EnergySet {
   double fw[kMaxFluxEnergies];
 };
 
+// Trajectory chunks of a flux launch: chunk c < n1 covers [c c1, (c + 1) c1), chunk c >= n1
+// covers c2 points from n1 c1 + (c - n1) c2, the last one clipped at the trajectory's end;
+// nch chunks in all.  Blocks are dispatched in chunk order, so tail chunks shorter than
+// the rest (c2 < c1) shorten the drain at the end of a launch.
+struct ChunkMap {
+  int c1, n1, c2, nch;
+  __host__ __device__ int64_t begin(int c) const {
+    return c < n1 ? (int64_t)c * c1 : (int64_t)n1 * c1 + (int64_t)(c - n1) * c2;
+  }
+  __host__ __device__ int64_t end(int c, int64_t n) const {
+    int64_t const e = begin(c) + (c < n1 ? c1 : c2);
+    return e < n ? e : n;
+  }
+};
+
 // Total power of one trajectory (with accelerations): out[0] = scale * sum of the terms.
 hipError_t launch_total_power(const TrajView& t, double scale, double* out, hipStream_t s);
 
@@ -30,12 +45,14 @@ int flux_fp64_blocks_per_cu(int ne);
 int pd_fp64_blocks_per_cu();
 int flux1_fp64_blocks_per_cu();
 int flux_mixed_blocks_per_cu(int ne);
+// Observation points per lane of the mixed flux launch for ne energies (one energy: two).
+constexpr int flux_mixed_points_per_lane(int ne) { return ne == 1 ? 2 : 1; }
 int pd_mixed_blocks_per_cu(int directional);
 
 // ---- plain FP64 form ------------------------------------------------------
 // Flux partial sums: part[((c*ne + e)*6 + k)*ns + j], c = trajectory chunk.
-hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es, int chunk,
-                            double* part, hipStream_t s);
+hipError_t launch_flux_fp64(const TrajView& t, const double* obs_soa, int ns, const EnergySet& es,
+                            ChunkMap const& cm, double* part, hipStream_t s);
 // Near- plus far-field flux (CalculateFlux1) partial sums: part[(c*6 + k)*ns + j], the
 // layout of launch_flux_fp64 with one energy (flux_reduce_kernel applies C0 dt and C2).
 hipError_t launch_flux1_fp64(const TrajView& t, const double* obs_soa, int ns, double omega, int chunk, double* part,
@@ -46,7 +63,7 @@ hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double
 
 // ---- mixed form (FP64 phase, FP32 trig / amplitudes / tile sums) -----------
 hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
-                             int chunk, double* part, hipStream_t s);
+                             ChunkMap const& cm, double* part, hipStream_t s);
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
                            int directional, int chunk, double* part, hipStream_t s);
 // Single-precision copies used by the mixed kernels: trajf[k*n + i] for

@@ -63,7 +63,7 @@ template <int NE, int TILE_>
 __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const float* __restrict__ tf,
                                                              const int* __restrict__ flags,
                                                              const double* __restrict__ obs, int ns, EnergySet es,
-                                                             int chunk, double* __restrict__ part) {
+                                                             ChunkMap cm, double* __restrict__ part) {
   __shared__ MixedTile<TILE_, NE> T;
   __shared__ double red[4 * (kBlock / 64)];
   int const tid = threadIdx.x;
@@ -74,8 +74,8 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
   float cwf[NE];
 #pragma unroll
   for (int e = 0; e < NE; ++e) cwf[e] = (float)es.cw[e];
-  int64_t const i0 = (int64_t)blockIdx.y * chunk;
-  int64_t const i1 = (i0 + chunk < t.n) ? i0 + chunk : t.n;
+  int64_t const i0 = cm.begin(blockIdx.y);
+  int64_t const i1 = cm.end(blockIdx.y, t.n);
   if (chunk_transverse(flags, i0, i1)) {
     // the beam of this chunk is not along z, which the mixed terms assume (they stage
     // 1 - beta_z): the block sums the FP64 terms instead
@@ -154,6 +154,119 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
       for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = a.value(e, q);
     }
   }
+}
+
+// One energy, two observation points per lane: a block owns points j0 + tid and
+// j0 + 256 + tid (j0 = 512 blockIdx.x) and one trajectory chunk; the paraxial records
+// are read once for both points (flux_terms_para2).  Chunks whose beam is not along z
+// and blocks that fail the paraxial check sum the FP64 / general mixed terms one point
+// after the other.  part as flux_mixed_kernel<1, TILE>.
+template <int TILE_>
+__global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const float* __restrict__ tf,
+                                                              const int* __restrict__ flags,
+                                                              const double* __restrict__ obs, int ns, EnergySet es,
+                                                              ChunkMap cm, double* __restrict__ part) {
+  __shared__ MixedTile<TILE_, 1> T;
+  __shared__ double red[4 * (kBlock / 64)];
+  int const tid = threadIdx.x;
+  int const j0 = blockIdx.x * (2 * kBlock);
+  int const jp[2] = {j0 + tid, j0 + kBlock + tid};
+  double ox[2], oy[2], oz[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    int const jj = jp[p] < ns ? jp[p] : j0;  // idle lanes repeat the block's first point
+    ox[p] = obs[jj];
+    oy[p] = obs[ns + jj];
+    oz[p] = obs[2 * ns + jj];
+  }
+  float cwf = (float)es.cw[0];
+  int64_t const i0 = cm.begin(blockIdx.y);
+  int64_t const i1 = cm.end(blockIdx.y, t.n);
+  if (chunk_transverse(flags, i0, i1)) {
+    // the beam of this chunk is not along z: the FP64 terms, one point after the other
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+      // selects, not indexing: a runtime index would put the arrays in scratch memory
+      double const px = p ? ox[1] : ox[0], py = p ? oy[1] : oy[0], pz = p ? oz[1] : oz[0];
+      int const j = p ? jp[1] : jp[0];
+      AccFP64<1> d;
+      d.zero();
+      for (int64_t base = i0; base < i1; base += TILE_) {
+        int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
+        __syncthreads();
+        for (int k = tid; k < cnt; k += kBlock) {
+          int64_t const i = base + k;
+          T.raw.x[k] = t.x[i];
+          T.raw.y[k] = t.y[i];
+          T.raw.z[k] = t.z[i];
+          T.raw.bx[k] = t.bx[i];
+          T.raw.by[k] = t.by[i];
+          T.raw.bz[k] = t.bz[i];
+        }
+        __syncthreads();
+        flux_terms_fp64<1, TILE_>(T.raw, cnt, base, t.dt, px, py, pz, es, d);
+      }
+      if (j < ns) {
+        size_t const b = ((size_t)blockIdx.y * 6) * ns + j;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          part[b + (2 * k) * (size_t)ns] = d.re[0][k];
+          part[b + (2 * k + 1) * (size_t)ns] = d.im[0][k];
+        }
+      }
+    }
+    return;
+  }
+  double const z0 = obs[2 * ns + j0];
+  ParaBlock const pb = para_block_range(__builtin_fmin(ox[0], ox[1]), __builtin_fmax(ox[0], ox[1]),
+                                        __builtin_fmin(oy[0], oy[1]), __builtin_fmax(oy[0], oy[1]),
+                                        oz[0] == z0 && oz[1] == z0, z0, red);
+  bool const para = para_check(pb, t.x, t.y, t.z, i0, i1, es);
+  AccMixed<1> a[2];
+  a[0].zero();
+  a[1].zero();
+  if (para) {
+    for (int64_t base = i0; base < i1; base += TILE_) {
+      int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
+      __syncthreads();
+      int const cntp = (cnt + kSub - 1) / kSub * kSub;
+      for (int k = tid; k < cntp; k += kBlock) {
+        int64_t const i = base + k;
+        if (k < cnt)
+          para_stage<TILE_, 1>(T.para, k, t.x[i], t.y[i], t.z[i], tf[i], tf[t.n + i], tf[2 * t.n + i], i, t.dt, pb,
+                               es);
+        else
+          para_pad<TILE_, 1>(T.para, k);
+      }
+      __syncthreads();
+      flux_terms_para2<TILE_>(T.para, cntp, ox, oy, cwf, a);
+    }
+  } else {
+    for (int64_t base = i0; base < i1; base += TILE_) {
+      int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
+      __syncthreads();
+      for (int k = tid; k < cnt; k += kBlock) {
+        int64_t const i = base + k;
+        T.raw.x[k] = t.x[i];
+        T.raw.y[k] = t.y[i];
+        T.raw.z[k] = t.z[i];
+        T.raw.fbx[k] = tf[i];
+        T.raw.fby[k] = tf[t.n + i];
+        T.raw.fomb[k] = tf[2 * t.n + i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+        flux_terms_mixed<1, TILE_>(T.raw, cnt, base, t.dt, ox[p], oy[p], oz[p], es, &cwf, a[p]);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+    if (jp[p] < ns) {
+      size_t const b = ((size_t)blockIdx.y * 6) * ns + jp[p];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = a[p].value(0, q);
+    }
 }
 
 // Mixed power density, FP32 throughout the term.  With u = n - beta and |n| = 1
@@ -273,7 +386,7 @@ static int blocks_per_cu(K kernel, int threads, int& cache) {
 int flux_mixed_blocks_per_cu(int ne) {
   static int c[5];
   switch (ne) {
-    case 1: return blocks_per_cu(flux_mixed_kernel<1, TILE>, kBlock, c[1]);
+    case 1: return blocks_per_cu(flux_mixed2_kernel<TILE>, kBlock, c[1]);
     case 2: return blocks_per_cu(flux_mixed_kernel<2, TILE>, kBlock, c[2]);
     case 3: return blocks_per_cu(flux_mixed_kernel<3, TILE>, kBlock, c[3]);
     default: return blocks_per_cu(flux_mixed_kernel<4, TILE>, kBlock, c[4]);
@@ -295,14 +408,20 @@ hipError_t launch_traj_to_f32(const TrajView& t, int with_a, float* trajf, hipSt
 }
 
 hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
-                             int chunk, double* part, hipStream_t s) {
-  dim3 grid((ns + kBlock - 1) / kBlock, nchunks(t.n, chunk));
+                             ChunkMap const& cm, double* part, hipStream_t s) {
+  if (cm.c1 < 1 || cm.c2 < 1 || cm.nch < 1 || cm.end(cm.nch - 1, t.n) != t.n) return hipErrorInvalidValue;
+  dim3 grid((ns + kBlock - 1) / kBlock, cm.nch);
   const int* const fl = reinterpret_cast<const int*>(trajf + 3 * t.n);  // staged without accelerations
   switch (es.ne) {
-    case 1: flux_mixed_kernel<1, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
-    case 2: flux_mixed_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
-    case 3: flux_mixed_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
-    case 4: flux_mixed_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, chunk, part); break;
+    case 1: {
+      // one energy: two observation points per lane
+      dim3 const grid2((ns + 2 * kBlock - 1) / (2 * kBlock), cm.nch);
+      flux_mixed2_kernel<TILE><<<grid2, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, cm, part);
+      break;
+    }
+    case 2: flux_mixed_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, cm, part); break;
+    case 3: flux_mixed_kernel<3, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, cm, part); break;
+    case 4: flux_mixed_kernel<4, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, cm, part); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

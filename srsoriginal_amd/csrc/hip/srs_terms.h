@@ -216,10 +216,12 @@ struct ParaBlock {
   bool flat;  // every point of the block has oz == z
 };
 
-// Called by every lane of the block; lanes without a point pass a point of the
+// Called by every lane of the block with the range [xlo, xhi] x [ylo, yhi] of its
+// points and whether they all lie on z = z0; lanes without a point pass a point of the
 // block.  red holds 4 * (kBlock / 64) doubles of LDS.
-__device__ __forceinline__ ParaBlock para_block(double ox, double oy, double oz, double z0, double* red) {
-  double v[4] = {ox, -ox, oy, -oy};  // minima of these give xlo, xhi, ylo, yhi
+__device__ __forceinline__ ParaBlock para_block_range(double xlo, double xhi, double ylo, double yhi, bool on_plane,
+                                                      double z0, double* red) {
+  double v[4] = {xlo, -xhi, ylo, -yhi};  // minima of these give xlo, xhi, ylo, yhi
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
@@ -228,7 +230,7 @@ __device__ __forceinline__ ParaBlock para_block(double ox, double oy, double oz,
   if (lane == 0)
 #pragma unroll
     for (int q = 0; q < 4; ++q) red[wave * 4 + q] = v[q];
-  bool const flat = __syncthreads_and(oz == z0) != 0;  // also orders the LDS writes above
+  bool const flat = __syncthreads_and(on_plane) != 0;  // also orders the LDS writes above
   ParaBlock b;
   b.z = z0;
   b.flat = flat;
@@ -241,6 +243,11 @@ __device__ __forceinline__ ParaBlock para_block(double ox, double oy, double oz,
   b.ylo = m[2];
   b.yhi = -m[3];
   return b;
+}
+
+// One point per lane.
+__device__ __forceinline__ ParaBlock para_block(double ox, double oy, double oz, double z0, double* red) {
+  return para_block_range(ox, ox, oy, oy, oz == z0, z0, red);
 }
 
 // True when trajectory point (xk, yk, zk) satisfies the expansion bounds for every
@@ -386,6 +393,65 @@ __device__ __forceinline__ void flux_terms_para(ParaTile<TILE, NE> const& P, int
       a.q[e][3].add(__builtin_fmaf(-oyf, b[5], b[3]) * m);
       a.q[e][4].add(__builtin_fmaf(cw, b[5], b[6]) * m);
       a.q[e][5].add(__builtin_fmaf(-cw, b[4], b[7]) * m);
+    }
+  }
+}
+
+// The paraxial terms at one energy for two observation points per lane: each staged
+// record is read from LDS once and integrated for both points (two independent
+// dependency chains per record; half the LDS reads per term).
+template <int TILE>
+__device__ __forceinline__ void flux_terms_para2(ParaTile<TILE, 1> const& P, int cnt, double const ox[2],
+                                                 double const oy[2], float cw, AccMixed<1> a[2]) {
+  double asq[2], mx[2], my[2];
+  float oxf[2], oyf[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    asq[p] = __builtin_fma(ox[p], ox[p], oy[p] * oy[p]);
+    mx[p] = -2.0 * ox[p];
+    my[p] = -2.0 * oy[p];
+    oxf[p] = (float)ox[p];
+    oyf[p] = (float)oy[p];
+  }
+  for (int k0 = 0; k0 < cnt; k0 += kSub) {
+    float sub[2][8];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sub[p][q] = 0.f;
+#pragma unroll 2
+    for (int kk = 0; kk < kSub; ++kk) {
+      ParaPoint<1> const& r = P.p[k0 + kk];
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        double const s = __builtin_fma(mx[p], r.x, __builtin_fma(my[p], r.y, asq[p] + r.bsq));
+        double const hs = __builtin_fma(-r.t3, s, r.h);
+        float const az = __builtin_fmaf((float)s, r.gh, -r.ombr);
+        float const ph = (float)__builtin_amdgcn_fract(__builtin_fma(s, hs, r.phi[0]));
+        float const sn = __builtin_amdgcn_sinf(ph), cn = __builtin_amdgcn_cosf(ph);
+        float* const b = sub[p];
+        b[0] = __builtin_fmaf(r.ax, cn, b[0]);
+        b[1] = __builtin_fmaf(r.ax, sn, b[1]);
+        b[2] = __builtin_fmaf(r.ay, cn, b[2]);
+        b[3] = __builtin_fmaf(r.ay, sn, b[3]);
+        b[4] = __builtin_fmaf(r.q, cn, b[4]);
+        b[5] = __builtin_fmaf(r.q, sn, b[5]);
+        b[6] = __builtin_fmaf(az, cn, b[6]);
+        b[7] = __builtin_fmaf(az, sn, b[7]);
+      }
+    }
+    ParaPoint<1> const& rl = P.p[k0];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      double const sl = __builtin_fma(mx[p], rl.x, __builtin_fma(my[p], rl.y, asq[p] + rl.bsq));
+      float const m = __builtin_fmaf(-0.5f * (float)sl, rl.q, 1.f);
+      float const* const b = sub[p];
+      a[p].q[0][0].add(__builtin_fmaf(-oxf[p], b[4], b[0]) * m);
+      a[p].q[0][1].add(__builtin_fmaf(-oxf[p], b[5], b[1]) * m);
+      a[p].q[0][2].add(__builtin_fmaf(-oyf[p], b[4], b[2]) * m);
+      a[p].q[0][3].add(__builtin_fmaf(-oyf[p], b[5], b[3]) * m);
+      a[p].q[0][4].add(__builtin_fmaf(cw, b[5], b[6]) * m);
+      a[p].q[0][5].add(__builtin_fmaf(-cw, b[4], b[7]) * m);
     }
   }
 }

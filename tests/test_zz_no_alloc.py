@@ -1,8 +1,9 @@
 """GPU test of srs_hip_flux_device_noalloc, the entry point that allocates nothing (its
 scratch is the library's static device array): through the C ABI it must give what
-srs_hip_flux_device gives on the same inputs -- bit for bit, both precisions -- on two
-calls in a row, the second with a smaller input (fewer points, a shorter trajectory,
-fewer energies), and refuse a call whose scratch would not fit, with a clear error."""
+srs_hip_flux_device gives on the same inputs -- bit for bit, both precisions -- on
+three calls in a row, each with fewer observation points than the one before (the
+201 x 201 map at one energy, then more energies, then a shorter trajectory), and
+refuse a call whose scratch would not fit, with a clear error."""
 import numpy as np
 import pytest
 
@@ -26,7 +27,8 @@ def test_noalloc_flux_equals_flux_device_twice(oracle):
     assert abi.device_count() >= 1
     dev = "cuda:0"
     st = torch.cuda.Stream()
-    cases = [_case(oracle, 20001, 101, 0.02, [143.5, 287.0, 431.0, 574.5, 718.0]),  # two energy launches
+    cases = [_case(oracle, 20001, 201, 0.02, [143.52]),                            # the 201 x 201 flux map
+             _case(oracle, 20001, 101, 0.02, [143.5, 287.0, 431.0, 574.5, 718.0]),  # two energy launches
              _case(oracle, 4001, 37, 0.01, [143.5])]                                  # then a smaller one
     for prec in (abi.FP64, abi.MIXED):
         for k, (p, t, dt, xyz, es) in enumerate(cases):
@@ -55,7 +57,7 @@ def test_noalloc_flux_rejects_scratch_beyond_its_maximum(oracle):
     import torch
     from srsoriginal_amd import abi
     p, t, dt, xyz, es = _case(oracle, 20001, 3, 0.01, [143.5])
-    ns = 400000  # ~20 chunks x 48 B x 4e5 points: well beyond 64 MiB; rejected before any launch
+    ns = 400000  # ~20 chunks x 48 B x 4e5 points: well beyond 128 MiB; rejected before any launch
     d_t = torch.from_numpy(np.ascontiguousarray(t[:6])).to("cuda:0")
     d_obs = torch.zeros(3 * ns, dtype=torch.float64, device="cuda:0")
     d_obs[2 * ns:] = 30.0
