@@ -20,6 +20,33 @@ constexpr int kSpecWaves = 8;
 constexpr int kSpecThreads = 64 * kSpecWaves;
 constexpr int kFoldRun = 4;  // energies per lane folded per LDS round (8 x 4 x 6 x 64 doubles)
 
+// Lanes of one wave only: the wave's own LDS tile is written and read back by its lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Issue balance of the two waves that share a SIMD (waves w and w ^ 4 of a block of 8,
+// the CU placing a block's waves on its 4 SIMDs in turn).  Both have the same number of
+// points; left alone, the older takes most issue slots and the younger finishes its
+// points alone, with nothing to hide its latencies.  After each point a wave publishes
+// its count and takes the lower priority while it is ahead of its partner, so the pair
+// advances together.  Only the order of issue changes, never a result.
+struct IssueBalance {
+  int* prog;
+  int self, partner;
+  __device__ __forceinline__ void step(int done) {
+    int const other = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(prog + partner, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    __hip_atomic_store(prog + self, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (done > other)
+      __builtin_amdgcn_s_setprio(0);
+    else
+      __builtin_amdgcn_s_setprio(2);
+  }
+};
+
 template <bool MIXED, int RUN>
 __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, double ox, double oy, double oz,
                                                                 SpecTables tab, int ne, int sub,
@@ -34,18 +61,23 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
     Tile tile[kSpecWaves];
     double fold[kSpecWaves][FR][6][64];
   } sm;
+  __shared__ int prog[kSpecWaves];
   int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) prog[wave] = 0;
   SpecLane<MIXED, RUN> L;
   L.init(tab.omega, tab.cw, tab.rdw, tab.rtl, ne, (blockIdx.x * 64 + lane) * RUN);
   int64_t const i0 = ((int64_t)blockIdx.y * kSpecWaves + wave) * sub;
   int64_t const i1 = (i0 + sub < t.n) ? i0 + sub : t.n;
-  int const ntiles = (sub + 63) / 64;  // the same for every wave (block barriers below)
+  int const ntiles = (sub + 63) / 64;
   Tile& tl = sm.tile[wave];
+  IssueBalance bal{prog, wave, wave ^ 4};
+  int done = 0;
   double tmax = 0.0;  // largest |tau| this lane staged
+  __syncthreads();  // prog is set
   for (int tt = 0; tt < ntiles; ++tt) {
     int64_t const base = i0 + (int64_t)tt * 64;
     int const cnt = base < i1 ? ((i1 - base) < 64 ? (int)(i1 - base) : 64) : 0;
-    if (tt > 0) __syncthreads();  // the previous tile is consumed
+    if (tt > 0) wave_sync();  // the previous tile is consumed
     if (lane < cnt) {
       int64_t const i = base + lane;
       SpecPoint P;
@@ -58,7 +90,7 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
         tl.g[k][lane] = (T)P.g[k];
       }
     }
-    __syncthreads();
+    wave_sync();
     for (int k0 = 0; k0 < cnt; k0 += kSub) {
       int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
       if constexpr (MIXED) {
@@ -78,10 +110,13 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
           gn1 = tl.g[1][kn];
           gn2 = tl.g[2][kn];
           L.add(tc, a0, a1, a2, g0, g1, g2);
+          bal.step(++done);
         }
       } else {
-        for (int k = k0; k < k1; ++k)
+        for (int k = k0; k < k1; ++k) {
           L.add(tl.tau[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k], tl.g[2][k]);
+          bal.step(++done);
+        }
       }
       L.flush();
     }
@@ -90,6 +125,7 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) tmax = __builtin_fmax(tmax, __shfl_xor(tmax, m, 64));
   if (tmax > L.tau_lim) *range = 1;
+  __builtin_amdgcn_s_setprio(0);
   // the waves' sums in wave order, FR energies of the run per round
 #pragma unroll
   for (int r0 = 0; r0 < RUN; r0 += FR) {
