@@ -206,6 +206,7 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
           __syncthreads();
           for (int k = tid; k < cnt; k += kBlock) {
             int64_t const i = q.off + base + k;
+            T.raw.t[k] = q.dt * (double)(base + k);
             T.raw.x[k] = bt.x[i];
             T.raw.y[k] = bt.y[i];
             T.raw.z[k] = bt.z[i];
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
             T.raw.bz[k] = bt.bz[i];
           }
           __syncthreads();
-          flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, d);
+          flux_terms_fp64<NE, TILE_>(T.raw, cnt, ox, oy, oz, es, d);
         }
       }
       finish([&](int e, int q6) { return (q6 & 1) ? d.im[e][q6 >> 1] : d.re[e][q6 >> 1]; });
@@ -259,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
           __syncthreads();
           for (int k = tid; k < cnt; k += kBlock) {
             int64_t const i = q.off + base + k;
+            T.raw.t[k] = q.dt * (double)(base + k);
             T.raw.x[k] = bt.x[i];
             T.raw.y[k] = bt.y[i];
             T.raw.z[k] = bt.z[i];
@@ -273,8 +275,8 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
             }
           }
           __syncthreads();
-          if (MIXED) flux_terms_mixed<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, cwf, a32);
-          else flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, q.dt, ox, oy, oz, es, a64);
+          if (MIXED) flux_terms_mixed<NE, TILE_>(T.raw, cnt, ox, oy, oz, es, cwf, a32);
+          else flux_terms_fp64<NE, TILE_>(T.raw, cnt, ox, oy, oz, es, a64);
         }
       }
     }

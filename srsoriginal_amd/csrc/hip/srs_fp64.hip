@@ -40,6 +40,7 @@ __global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const dou
     int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
     __syncthreads();
     for (int k = tid; k < cnt; k += kBlock) {
+      T.t[k] = t.dt * (double)(base + k);
       T.x[k] = t.x[base + k];
       T.y[k] = t.y[base + k];
       T.z[k] = t.z[base + k];
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void flux_fp64_kernel(TrajView t, const dou
       T.bz[k] = t.bz[base + k];
     }
     __syncthreads();
-    flux_terms_fp64<NE, TILE_>(T, cnt, base, t.dt, ox, oy, oz, es, a);
+    flux_terms_fp64<NE, TILE_>(T, cnt, ox, oy, oz, es, a);
   }
   if (j < ns) {
 #pragma unroll

@@ -110,6 +110,26 @@ SRS_FN double div_rn(double a, double b, double y) {
 }
 constexpr double kInvC = 0x1.ca726eb25f9dbp-29;  // RN(1/c)
 
+// RN(sqrt(x)) for 2^-900 <= x <= 2^1000 from an estimate y0 of 1/sqrt(x) with relative
+// error below 2^-16 (the hardware reciprocal square root is far closer).  With
+// g = x y0 ~ sqrt(x) and h = y0/2 ~ 1/(2 sqrt(x)), e = 1/2 - h g is, to first order,
+// the relative error of both; g += g e, h += h e squares it (Goldschmidt), and each
+// g += (x - g^2) h (the residual exact with an FMA) is a Newton step: the error falls
+// far below half an ulp before the last one rounds.  No scaling or special cases:
+// in this range no intermediate is subnormal or overflows (the squared distances of
+// the flux terms, from ~1e-30 to ~1e30 m^2).
+SRS_FN double sqrt_rn(double x, double y0) {
+  double g = x * y0;
+  double h = 0.5 * y0;
+  double const e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
 // Reduce an FP64 phase to r in [-pi, pi] (two FMAs, same argument as above with
 // 2*pi) for the single-precision trig of the mixed form.
 constexpr double kInvTwoPi = 0x1.45f306dc9c883p-3;

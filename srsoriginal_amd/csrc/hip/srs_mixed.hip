@@ -86,6 +86,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
       __syncthreads();
       for (int k = tid; k < cnt; k += kBlock) {
         int64_t const i = base + k;
+        T.raw.t[k] = t.dt * (double)(base + k);
         T.raw.x[k] = t.x[i];
         T.raw.y[k] = t.y[i];
         T.raw.z[k] = t.z[i];
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
         T.raw.bz[k] = t.bz[i];
       }
       __syncthreads();
-      flux_terms_fp64<NE, TILE_>(T.raw, cnt, base, t.dt, ox, oy, oz, es, d);
+      flux_terms_fp64<NE, TILE_>(T.raw, cnt, ox, oy, oz, es, d);
     }
     if (j < ns) {
 #pragma unroll
@@ -135,6 +136,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
       __syncthreads();
       for (int k = tid; k < cnt; k += kBlock) {
         int64_t const i = base + k;
+        T.raw.t[k] = t.dt * (double)(base + k);
         T.raw.x[k] = t.x[i];
         T.raw.y[k] = t.y[i];
         T.raw.z[k] = t.z[i];
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
         T.raw.fomb[k] = tf[2 * t.n + i];
       }
       __syncthreads();
-      flux_terms_mixed<NE, TILE_>(T.raw, cnt, base, t.dt, ox, oy, oz, es, cwf, a);
+      flux_terms_mixed<NE, TILE_>(T.raw, cnt, ox, oy, oz, es, cwf, a);
     }
   }
   if (j < ns) {
@@ -196,6 +198,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
         __syncthreads();
         for (int k = tid; k < cnt; k += kBlock) {
           int64_t const i = base + k;
+          T.raw.t[k] = t.dt * (double)(base + k);
           T.raw.x[k] = t.x[i];
           T.raw.y[k] = t.y[i];
           T.raw.z[k] = t.z[i];
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
           T.raw.bz[k] = t.bz[i];
         }
         __syncthreads();
-        flux_terms_fp64<1, TILE_>(T.raw, cnt, base, t.dt, px, py, pz, es, d);
+        flux_terms_fp64<1, TILE_>(T.raw, cnt, px, py, pz, es, d);
       }
       if (j < ns) {
         size_t const b = ((size_t)blockIdx.y * 6) * ns + j;
@@ -247,6 +250,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
       __syncthreads();
       for (int k = tid; k < cnt; k += kBlock) {
         int64_t const i = base + k;
+        T.raw.t[k] = t.dt * (double)(base + k);
         T.raw.x[k] = t.x[i];
         T.raw.y[k] = t.y[i];
         T.raw.z[k] = t.z[i];
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
       __syncthreads();
 #pragma unroll
       for (int p = 0; p < 2; ++p)
-        flux_terms_mixed<1, TILE_>(T.raw, cnt, base, t.dt, ox[p], oy[p], oz[p], es, &cwf, a[p]);
+        flux_terms_mixed<1, TILE_>(T.raw, cnt, ox[p], oy[p], oz[p], es, &cwf, a[p]);
     }
   }
 #pragma unroll

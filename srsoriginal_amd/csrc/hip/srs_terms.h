@@ -14,6 +14,7 @@ namespace srs {
 // LDS tile of trajectory points for the flux kernels.
 template <int TILE>
 struct FluxTile {
+  double t[TILE];  // dt * i, the point's time in the reference's rounding, formed once when staged
   double x[TILE], y[TILE], z[TILE];
   double bx[TILE], by[TILE], bz[TILE];  // FP64 form
   float fbx[TILE], fby[TILE], fomb[TILE];  // mixed form: beta_x, beta_y, 1 - beta_z
@@ -30,18 +31,23 @@ struct AccFP64 {
   }
 };
 
+// |R| = RN(sqrt(d2)) for the squared distance of a term (sqrt_rn from the hardware
+// reciprocal square root: the correctly rounded root without the scaling and special
+// cases of a general square root, which squared distances never need).
+__device__ __forceinline__ double dist_rn(double d2) { return sqrt_rn(d2, __builtin_amdgcn_rsq(d2)); }
+
 // Plain FP64 form: phase chain in the reference's operation order (bitwise),
 // n_z = R_z/D correctly rounded (beta_z - n_z cancels to ~1e-8, so its rounding is
 // bit-relevant); n_x, n_y by the reciprocal (beta_x - n_x, beta_y - n_y are ~K/gamma,
 // no cancellation: an ulp of n_x is 1e-16 of a typical term).
 template <int NE, int TILE>
-__device__ __forceinline__ void flux_terms_fp64(FluxTile<TILE> const& T, int cnt, int64_t base, double dt, double ox,
-                                                double oy, double oz, EnergySet const& es, AccFP64<NE>& a) {
+__device__ __forceinline__ void flux_terms_fp64(FluxTile<TILE> const& T, int cnt, double ox, double oy, double oz,
+                                                EnergySet const& es, AccFP64<NE>& a) {
   for (int k = 0; k < cnt; ++k) {
     double const rx = ox - T.x[k], ry = oy - T.y[k], rz = oz - T.z[k];
     double const d2 = rx * rx + ry * ry + rz * rz;
-    double const D = __builtin_sqrt(d2);
-    double const tau = dt * (double)(base + k) + div_rn(D, kC, kInvC);
+    double const D = dist_rn(d2);
+    double const tau = T.t[k] + div_rn(D, kC, kInvC);
     double const inv = 1.0 / D;
     double const nx = rx * inv, ny = ry * inv, nz = div_rn(rz, D, inv);
     double const ax = (T.bx[k] - nx) * inv, ay = (T.by[k] - ny) * inv, az = (T.bz[k] - nz) * inv;
@@ -97,9 +103,8 @@ __device__ __forceinline__ void sincos_reduced_f32(double r, float* s, float* c)
 // reduction to [-pi, pi]; trig, amplitudes and the 32-term sub-tile sums in FP32;
 // sub-tile sums folded into Kahan-compensated FP32 accumulators.
 template <int NE, int TILE>
-__device__ __forceinline__ void flux_terms_mixed(FluxTile<TILE> const& T, int cnt, int64_t base, double dt, double ox,
-                                                 double oy, double oz, EnergySet const& es, const float* cwf,
-                                                 AccMixed<NE>& a) {
+__device__ __forceinline__ void flux_terms_mixed(FluxTile<TILE> const& T, int cnt, double ox, double oy, double oz,
+                                                 EnergySet const& es, const float* cwf, AccMixed<NE>& a) {
   for (int k0 = 0; k0 < cnt; k0 += kSub) {
     int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
     float sub[NE][6];
@@ -110,8 +115,8 @@ __device__ __forceinline__ void flux_terms_mixed(FluxTile<TILE> const& T, int cn
     for (int k = k0; k < k1; ++k) {
       double const rx = ox - T.x[k], ry = oy - T.y[k], rz = oz - T.z[k];
       double const d2 = rx * rx + ry * ry + rz * rz;
-      double const D = __builtin_sqrt(d2);
-      double const tau = dt * (double)(base + k) + div_rn(D, kC, kInvC);
+      double const D = dist_rn(d2);
+      double const tau = T.t[k] + div_rn(D, kC, kInvC);
       float const inv = __builtin_amdgcn_rcpf((float)D);
       float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
       float const v = __builtin_fmaf(nx, nx, ny * ny);

@@ -14,3 +14,7 @@ void dm_reduce(const double* x, long n, double* r) {
   for (long i = 0; i < n; ++i) r[i] = srs::reduce_two_pi(x[i]);
 }
 }
+
+extern "C" void dm_sqrt(const double* x, const double* y0, long n, double* out) {
+  for (long i = 0; i < n; ++i) out[i] = srs::sqrt_rn(x[i], y0[i]);
+}

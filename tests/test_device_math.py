@@ -57,3 +57,25 @@ def test_reduce_two_pi(dm):
     (r,) = _call(dm.dm_reduce, x)
     assert np.all(np.abs(r) <= np.pi + 1e-9)
     assert np.max(np.abs(np.sin(r) - np.sin(x))) <= 1e-15
+
+
+def test_sqrt_rn_is_correctly_rounded_from_a_rough_estimate(dm):
+    """sqrt_rn(x, y0) equals the correctly rounded square root (numpy's, IEEE) for any
+    estimate y0 of 1/sqrt(x) within 2^-16 relative -- the hardware v_rsq_f64 is far
+    closer -- over the squared distances the flux terms meet and near perfect squares,
+    where a misrounding would show first."""
+    rng = np.random.default_rng(4)
+    x = np.concatenate([
+        10.0 ** rng.uniform(-30, 30, 400000),              # 1e-15 .. 1e15 m distances, squared
+        rng.uniform(899.0, 1001.0, 200000),               # ~30 m
+        np.ldexp(rng.uniform(0.5, 1.0, 100000), rng.integers(-900, 1000, 100000)),
+    ])
+    k = rng.integers(1, 2 ** 26, 100000).astype(np.float64)
+    sq = k * k                                            # exact squares and their neighbours
+    x = np.concatenate([x, sq, np.nextafter(sq, 0), np.nextafter(sq, np.inf)])
+    exact = np.sqrt(x)
+    for rel in (0.0, 2.0 ** -40, -(2.0 ** -30), 2.0 ** -20, -(2.0 ** -16)):
+        y0 = (1.0 / exact) * (1.0 + rel * rng.uniform(0.0, 1.0, len(x)))
+        (g,) = _call(dm.dm_sqrt, x, y0)
+        bad = np.flatnonzero(g != exact)
+        assert bad.size == 0, (rel, x[bad[:5]], g[bad[:5]], exact[bad[:5]])
