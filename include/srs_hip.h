@@ -291,6 +291,19 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
  * points per LDS tile. */
 int srs_hip_batch_config(int* ne_per_item, int* tile);
 
+/* Observation points per lane of the tile shape used by the last batch call: 1 (256
+ * points per work item), or 2 (512 points per item, each staged trajectory record
+ * integrated for two points; the mixed form at one energy per item). */
+int srs_hip_batch_points_per_lane(int* points_per_lane);
+
+/* Fix the tile shape of later batch flux calls that do not autotune: energies per
+ * item, trajectory points per LDS tile, observation points per lane.  ne_per_item = 0
+ * restores the default shape.  A shape with no kernel (see srs_hip_batch_config's
+ * candidates: 1-4 energies, tiles of 64-256, two points per lane only at one energy
+ * and tile 128 or 256) fails; two points per lane fail later calls in FP64.  No
+ * counterpart in the reference (its per-particle loop has no tiles). */
+int srs_hip_batch_set_config(int ne_per_item, int tile, int points_per_lane);
+
 /* Kernel timing with HIP events recorded on the launch stream around each main
  * integral kernel (not the staging or the chunk reduction).  enable resets the
  * record; read waits for the recorded kernels, returns their summed duration and

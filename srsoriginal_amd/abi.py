@@ -216,12 +216,15 @@ def _bind_batch():
     L.srs_hip_batch_spectrum.argtypes = [C.POINTER(SrsField), C.c_int, C.POINTER(SrsParticle), _I64, _D, _D, _P,
                                          _P, _I64, _D, C.c_int, _P]
     L.srs_hip_batch_config.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.srs_hip_batch_points_per_lane.argtypes = [C.POINTER(C.c_int)]
+    L.srs_hip_batch_set_config.argtypes = [C.c_int, C.c_int, C.c_int]
     L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsField), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
                                        _D, _P, _I64, _D, _P, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_pd_forward.argtypes = [C.POINTER(SrsField), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P,
                                           _D, _D, _P, _P, _I64, C.c_int, _P, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_backward.argtypes = [_I64, _P, _P]
     for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config",
+              "srs_hip_batch_points_per_lane", "srs_hip_batch_set_config",
               "srs_hip_batch_power_density", "srs_hip_batch_spectrum",
               "srs_hip_pole_forward", "srs_hip_pole_pd_forward", "srs_hip_pole_backward"):
         getattr(L, n).restype = C.c_int
@@ -399,10 +402,18 @@ def batch_spectrum(und_specs, parts, charge, current, obs, energies_ev, weight, 
 
 
 def batch_config():
+    """(energies per item, trajectory tile, observation points per lane) of the last batch call."""
     L = _bind_batch()
-    a, b = C.c_int(), C.c_int()
+    a, b, p = C.c_int(), C.c_int(), C.c_int()
     _check(L.srs_hip_batch_config(C.byref(a), C.byref(b)))
-    return a.value, b.value
+    _check(L.srs_hip_batch_points_per_lane(C.byref(p)))
+    return a.value, b.value, p.value
+
+
+def batch_set_config(ne_per_item=0, tile=0, points_per_lane=1):
+    """Fix the tile shape of later non-autotuned batch calls (ne_per_item=0: the default)."""
+    L = _bind_batch()
+    _check(L.srs_hip_batch_set_config(int(ne_per_item), int(tile), int(points_per_lane)))
 
 
 def pole_forward(und_specs, poled, particle, deltas, charge, current, obs_xyz, energy_ev, target, checkpoint_every=1000,

@@ -93,6 +93,7 @@ struct Context {
   std::string b_key;
   std::map<std::string, srs::BatchConfig> b_tuned;
   srs::BatchConfig b_last{1, 128};
+  srs::BatchConfig b_fixed{0, 0};  // srs_hip_batch_set_config (ne_per_item 0: none)
   int cus = 0;
   // pinned host flag of the spectrum kernels: the trajectory left the range of the
   // energy-run phase correction (the call is then evaluated one energy per lane)
@@ -645,7 +646,8 @@ int64_t segment_steps(int64_t max_leg, int* nseg) {
 // Queue the whole pipeline: RK4 segments, persistent flux per segment, particle reduction.
 int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, int np_run, hipStream_t s,
                   bool reduce = true) {
-  int const ntiles = (r.ns + srs::kBlock - 1) / srs::kBlock;
+  int const tile_pts = srs::kBlock * cfg.ppl;
+  int const ntiles = (r.ns + tile_pts - 1) / tile_pts;
   int const ngroups = (r.ne + cfg.ne_per_item - 1) / cfg.ne_per_item;
   unsigned const nitems = (unsigned)((int64_t)np_run * ntiles * ngroups);
   int nseg = 1;
@@ -702,8 +704,8 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
   // depends on it), the CU count behind the persistent grid size, the flux prefactor c2
   // and the weight, and the address of every buffer a launch reads or writes
   snprintf(buf, sizeof buf,
-           "%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns,
-           r.ne, (int)r.mixed, cfg.ne_per_item, cfg.tile, r.nund, (int)c.und_local + 2 * c.und_force, c.cus, (long long)r.max_leg,
+           "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns,
+           r.ne, (int)r.mixed, cfg.ne_per_item, cfg.tile, cfg.ppl, r.nund, (int)c.und_local + 2 * c.und_force, c.cus, (long long)r.max_leg,
            r.weight, r.be.c2, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
            c.b_state.p, c.b_field.p, c.b_poles.p, c.b_part.p, c.b_counter.p, c.b_und.p,
            (void*)r.bt.bz, (void*)r.bt.tflag);  // row offsets within the packed buffers follow the total point count
@@ -967,6 +969,11 @@ int batch_flux_staged(Context& c, hipStream_t s, int nund, int64_t np, int64_t t
   // tile shape: default, or the autotuned one for this problem signature
   srs::BatchConfig cfg{(int)std::min<int64_t>(ne, 4) == 3 ? 3 : (ne >= 4 ? 4 : (int)ne), 128};
   if (!srs::batch_config_valid(cfg)) cfg = srs::BatchConfig{1, 128};
+  if (c.b_fixed.ne_per_item > 0) {
+    if (c.b_fixed.ppl == 2 && !r.mixed)
+      return fail(SRS_ERR_INVALID, "two observation points per lane need the mixed precision form");
+    cfg = c.b_fixed;
+  }
   char sig[128];
   snprintf(sig, sizeof sig, "%d/%lld/%lld/%lld", (int)r.mixed, (long long)((ns + 255) / 256), (long long)ne,
            (long long)(total / np));
@@ -975,10 +982,11 @@ int batch_flux_staged(Context& c, hipStream_t s, int nund, int64_t np, int64_t t
     if (it != c.b_tuned.end()) {
       cfg = it->second;
     } else {
-      // time each candidate on a subset of the particles (same RK4 and inputs)
-      static const srs::BatchConfig cands[] = {{1, 64}, {1, 128}, {1, 256}, {2, 128}, {2, 256},
-                                               {3, 64}, {3, 128}, {3, 256}, {4, 128}, {4, 256}};
-      int const nsub = (int)std::min<int64_t>(np, 64);
+      // time each candidate on a subset of the particles (same RK4 and inputs), large
+      // enough that every candidate runs many items per persistent block
+      static const srs::BatchConfig cands[] = {{1, 64},  {1, 128}, {1, 256}, {2, 128},    {2, 256},   {3, 64},
+                                               {3, 128}, {3, 256}, {4, 128}, {4, 256},    {1, 128, 2}, {1, 256, 2}};
+      int const nsub = (int)std::min<int64_t>(np, 256);
       hipEvent_t e0, e1;
       HIPCHK(hipEventCreate(&e0));
       HIPCHK(hipEventCreate(&e1));
@@ -987,7 +995,7 @@ int batch_flux_staged(Context& c, hipStream_t s, int nund, int64_t np, int64_t t
       c.timing = false;
       for (auto const& k : cands) {
         if (k.ne_per_item > ne || (k.ne_per_item > 1 && ne % k.ne_per_item != 0 && k.ne_per_item != ne)) continue;
-        if (!srs::batch_config_valid(k)) continue;
+        if (!srs::batch_config_valid(k) || (k.ppl == 2 && !r.mixed)) continue;
         TRY(enqueue_batch(c, r, k, nsub, s));  // warm-up
         HIPCHK(hipEventRecord(e0, s));
         TRY(enqueue_batch(c, r, k, nsub, s));
@@ -1567,8 +1575,30 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
 
 int srs_hip_batch_config(int* ne_per_item, int* tile) {
   Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
   if (ne_per_item) *ne_per_item = c.b_last.ne_per_item;
   if (tile) *tile = c.b_last.tile;
+  return SRS_OK;
+}
+
+int srs_hip_batch_points_per_lane(int* points_per_lane) {
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (points_per_lane) *points_per_lane = c.b_last.ppl;
+  return SRS_OK;
+}
+
+int srs_hip_batch_set_config(int ne_per_item, int tile, int points_per_lane) {
+  g_err.clear();
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lock(c.mu);
+  if (ne_per_item == 0) {
+    c.b_fixed = srs::BatchConfig{0, 0};
+    return SRS_OK;
+  }
+  srs::BatchConfig const k{ne_per_item, tile, points_per_lane};
+  if (!srs::batch_config_valid(k)) return fail(SRS_ERR_INVALID, "no batch kernel for this tile shape");
+  c.b_fixed = k;
   return SRS_OK;
 }
 

@@ -80,11 +80,13 @@ struct BatchEnergies {
   double c2;
 };
 
-// Tile shape of the persistent kernel: energies per work item and trajectory points
-// per LDS tile (256 observation points per block).  Chosen by the autotuner.
+// Tile shape of the persistent kernel: energies per work item, trajectory points per
+// LDS tile and observation points per lane (1: 256 per item; 2: 512 per item, the mixed
+// form at one energy per item).  Chosen by the autotuner.
 struct BatchConfig {
   int ne_per_item;
   int tile;
+  int ppl = 1;
 };
 
 bool batch_config_valid(BatchConfig const& c);

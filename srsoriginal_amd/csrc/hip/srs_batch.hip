@@ -285,6 +285,168 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
   }
 }
 
+// The mixed persistent kernel at one energy per item with two observation points per
+// lane: item = (particle, observation tile of 512 points, energy), lane tid owning points
+// j0 + tid and j0 + 256 + tid, so each staged paraxial record serves both
+// (flux_terms_para2).  Particles flagged as not travelling along z take the FP64 terms
+// and items that fail the paraxial bounds the general mixed terms, one point after the
+// other.  Same queue, segments and outputs as batch_flux_kernel.
+template <int TILE_>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_flux2_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                             const double* __restrict__ obs, int ns, BatchEnergies be,
+                                                             unsigned nitems, unsigned* __restrict__ counter,
+                                                             double* __restrict__ flux_pp, FluxSeg sg) {
+  __shared__ BatchTile<TILE_, 1, true> T;
+  __shared__ double red[4 * (kBlock / 64)];
+  __shared__ unsigned s_item;
+  int const tid = threadIdx.x;
+  int const ntiles = (ns + 2 * kBlock - 1) / (2 * kBlock);
+  for (;;) {
+    __syncthreads();  // every lane has finished with the previous item and s_item
+    if (tid == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    unsigned const item = s_item;
+    if (item >= nitems) break;
+    int const ee = (int)(item % (unsigned)be.ne);
+    unsigned const rest = item / (unsigned)be.ne;
+    int const tile = (int)(rest % (unsigned)ntiles);
+    int const p = (int)(rest / (unsigned)ntiles);
+    BatchParticle const q = P[p];
+    ItemRanges const rg(q, sg);
+    EnergySet es;
+    es.ne = 1;
+    es.omega[0] = be.omega[ee];
+    es.cw[0] = be.cw[ee];
+    es.kw[0] = (q.c0 * be.omega[ee]) * q.dt;  // (C0*omega)*dt with the particle's C0, OSCARSSR.cc:2080
+    es.fr[0] = be.fr[ee];
+    es.fw[0] = be.fw[ee];
+    float cwf = (float)be.cw[ee];
+    int const j0 = tile * (2 * kBlock);
+    int const jp[2] = {j0 + tid, j0 + kBlock + tid};
+    double ox[2], oy[2], oz[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      int const jj = jp[k] < ns ? jp[k] : j0;  // idle lanes repeat the tile's first point
+      ox[k] = obs[jj];
+      oy[k] = obs[ns + jj];
+      oz[k] = obs[2 * ns + jj];
+    }
+    // this segment's six field sums S(q6) of point j (+ the earlier segments', in segment order)
+    auto finish = [&](int j, auto sum_of) {
+      if (j >= ns) return;
+      double S[6];
+#pragma unroll
+      for (int q6 = 0; q6 < 6; ++q6) S[q6] = sum_of(q6);
+      size_t const b = ((size_t)p * be.ne + ee) * 6 * ns + j;
+      if (!sg.first) {
+#pragma unroll
+        for (int q6 = 0; q6 < 6; ++q6) S[q6] = sg.field_acc[b + (size_t)q6 * ns] + S[q6];
+      }
+      if (sg.field_acc && !(sg.first && sg.final && !sg.keep_field)) {
+#pragma unroll
+        for (int q6 = 0; q6 < 6; ++q6) sg.field_acc[b + (size_t)q6 * ns] = S[q6];
+      }
+      if (sg.final)
+        flux_pp[((size_t)p * be.ne + ee) * ns + j] = flux_from_sums(S[0], S[1], S[2], S[3], S[4], S[5], es.kw[0], q.c2);
+    };
+    // a point of this particle does not travel along z, which the mixed terms assume
+    // (they stage 1 - beta_z): such items take the FP64 terms (block-uniform)
+    bool const flagged = bt.tflag && bt.tflag[p];
+    bool para = false;
+    if (!flagged) {
+      double const z0 = obs[2 * ns + j0];
+      ParaBlock const pb = para_block_range(__builtin_fmin(ox[0], ox[1]), __builtin_fmax(ox[0], ox[1]),
+                                            __builtin_fmin(oy[0], oy[1]), __builtin_fmax(oy[0], oy[1]),
+                                            oz[0] == z0 && oz[1] == z0, z0, red);
+      bool const p0 = para_check(pb, bt.x + q.off, bt.y + q.off, bt.z + q.off, rg.r0[0], rg.r1[0], es);
+      bool const p1 = para_check(pb, bt.x + q.off, bt.y + q.off, bt.z + q.off, rg.r0[1], rg.r1[1], es);
+      para = p0 && p1;
+      if (para) {
+        AccMixed<1> a[2];
+        a[0].zero();
+        a[1].zero();
+        for (int part = 0; part < 2; ++part) {
+          int64_t const lo = rg.r0[part], hi = rg.r1[part];
+          for (int64_t base = lo; base < hi; base += TILE_) {
+            int const cnt = (hi - base) < TILE_ ? (int)(hi - base) : TILE_;
+            int const cntp = (cnt + kSub - 1) / kSub * kSub;
+            __syncthreads();
+            for (int kk = tid; kk < cntp; kk += kBlock) {
+              int64_t const i = q.off + base + kk;
+              if (kk < cnt)
+                para_stage<TILE_, 1>(T.para, kk, bt.x[i], bt.y[i], bt.z[i], bt.fbx[i], bt.fby[i], bt.fomb[i],
+                                     base + kk, q.dt, pb, es);
+              else
+                para_pad<TILE_, 1>(T.para, kk);
+            }
+            __syncthreads();
+            flux_terms_para2<TILE_>(T.para, cntp, ox, oy, cwf, a);
+          }
+        }
+        finish(jp[0], [&](int q6) { return a[0].value(0, q6); });
+        finish(jp[1], [&](int q6) { return a[1].value(0, q6); });
+        continue;
+      }
+    }
+    // otherwise one observation point after the other: the FP64 terms (flagged) or the
+    // general mixed terms (outside the paraxial bounds), the tile staged for each point;
+    // coordinates re-read per point (fewer live registers in the kernel)
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+      int const j = k ? jp[1] : jp[0];
+      int const jj = j < ns ? j : j0;
+      double const px = obs[jj], py = obs[ns + jj], pz = obs[2 * ns + jj];
+      if (flagged) {
+        AccFP64<1> d;
+        d.zero();
+        for (int part = 0; part < 2; ++part) {
+          int64_t const lo = rg.r0[part], hi = rg.r1[part];
+          for (int64_t base = lo; base < hi; base += TILE_) {
+            int const cnt = (hi - base) < TILE_ ? (int)(hi - base) : TILE_;
+            __syncthreads();
+            for (int kk = tid; kk < cnt; kk += kBlock) {
+              int64_t const i = q.off + base + kk;
+              T.raw.t[kk] = q.dt * (double)(base + kk);
+              T.raw.x[kk] = bt.x[i];
+              T.raw.y[kk] = bt.y[i];
+              T.raw.z[kk] = bt.z[i];
+              T.raw.bx[kk] = bt.bx[i];
+              T.raw.by[kk] = bt.by[i];
+              T.raw.bz[kk] = bt.bz[i];
+            }
+            __syncthreads();
+            flux_terms_fp64<1, TILE_>(T.raw, cnt, px, py, pz, es, d);
+          }
+        }
+        finish(j, [&](int q6) { return (q6 & 1) ? d.im[0][q6 >> 1] : d.re[0][q6 >> 1]; });
+      } else {
+        AccMixed<1> g;
+        g.zero();
+        for (int part = 0; part < 2; ++part) {
+          int64_t const lo = rg.r0[part], hi = rg.r1[part];
+          for (int64_t base = lo; base < hi; base += TILE_) {
+            int const cnt = (hi - base) < TILE_ ? (int)(hi - base) : TILE_;
+            __syncthreads();
+            for (int kk = tid; kk < cnt; kk += kBlock) {
+              int64_t const i = q.off + base + kk;
+              T.raw.t[kk] = q.dt * (double)(base + kk);
+              T.raw.x[kk] = bt.x[i];
+              T.raw.y[kk] = bt.y[i];
+              T.raw.z[kk] = bt.z[i];
+              T.raw.fbx[kk] = bt.fbx[i];
+              T.raw.fby[kk] = bt.fby[i];
+              T.raw.fomb[kk] = bt.fomb[i];
+            }
+            __syncthreads();
+            flux_terms_mixed<1, TILE_>(T.raw, cnt, px, py, pz, es, &cwf, g);
+          }
+        }
+        finish(j, [&](int q6) { return g.value(0, q6); });
+      }
+    }
+  }
+}
+
 __global__ void particle_reduce_kernel(const double* __restrict__ flux_pp, int np, int nout, double weight,
                                        double* __restrict__ out) {
   int const o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -311,6 +473,19 @@ template <bool MIXED>
 hipError_t launch_batch_m(BatchConfig const& cfg, BatchTraj const& bt, const BatchParticle* P, const double* obs, int ns,
                           BatchEnergies const& be, unsigned nitems, unsigned* counter, double* flux_pp,
                           FluxSeg const& sg, int nblocks, hipStream_t s) {
+  if (cfg.ppl == 2) {
+    if (!MIXED || cfg.ne_per_item != 1) return hipErrorInvalidValue;
+    if (cfg.tile == 128) {
+      batch_flux2_kernel<128><<<nblocks, kBlock, 0, s>>>(bt, P, obs, ns, be, nitems, counter, flux_pp, sg);
+      return hipGetLastError();
+    }
+    if (cfg.tile == 256) {
+      batch_flux2_kernel<256><<<nblocks, kBlock, 0, s>>>(bt, P, obs, ns, be, nitems, counter, flux_pp, sg);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
+  if (cfg.ppl != 1) return hipErrorInvalidValue;
 #define SRS_BATCH_CASE(NE_, TL_)                 \
   if (cfg.ne_per_item == NE_ && cfg.tile == TL_) \
     return launch_batch_t<NE_, TL_, MIXED>(bt, P, obs, ns, be, nitems, counter, flux_pp, sg, nblocks, s);
@@ -601,6 +776,8 @@ __global__ void spectrum_batch_reduce_kernel(const double* __restrict__ part, in
 }  // namespace
 
 bool batch_config_valid(BatchConfig const& c) {
+  if (c.ppl == 2) return c.ne_per_item == 1 && (c.tile == 128 || c.tile == 256);
+  if (c.ppl != 1) return false;
   switch (c.ne_per_item) {
     case 1: return c.tile == 64 || c.tile == 128 || c.tile == 256;
     case 2: return c.tile == 128 || c.tile == 256;
@@ -614,6 +791,11 @@ int batch_max_blocks(BatchConfig const& c, bool mixed) {
   // resident blocks per CU for this instantiation (sizes the persistent grid)
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
+  if (c.ppl == 2) {
+    if (mixed && c.tile == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_flux2_kernel<128>, kBlock, 0);
+    if (mixed && c.tile == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_flux2_kernel<256>, kBlock, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+  }
 #define SRS_OCC(NE_, TL_)                                                                                          \
   if (c.ne_per_item == NE_ && c.tile == TL_)                                                                      \
     e = mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_flux_kernel<NE_, TL_, true>, kBlock, 0)  \

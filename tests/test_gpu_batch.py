@@ -286,3 +286,45 @@ def test_batch_routes_use_each_particles_charge_and_current(abi, oracle):
     assert all(r["charge"] == p.q and r["current"] == 0.37 for r in s)
     bp["current"] = 0.0  # TParticleA: current 0 means I = q
     assert all(r["current"] == p.q for r in abi.sample_beam(bp, 2, 5, ct_start=0.0, ct_stop=3.0, npoints=301))
+
+
+def test_batch_every_tile_shape_matches_fp64(abi, oracle):
+    """Each tile shape of the persistent kernel, fixed with srs_hip_batch_set_config (two
+    observation points per lane included, on an odd point count so the last 512-point
+    tile is ragged), against the FP64 batch; the two-point shape also through replayed
+    graphs, bit for bit against its eager run."""
+    name = "multiflux_c3_small"
+    say("batched flux, every tile shape: mixed vs fp64 (<=1e-5), fp64 shapes vs each other (<=1e-12)")
+    rows, p = _c3_particles(oracle, name, 6)
+    parts = abi.particles(rows)
+    xyz, nrm, x12 = oracle.rect_surface("XY", 27, 23, 0.01, 0.008, (0, 0, 0), (0, 0, 30), 0)
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    es = [430.0, 431.2, 432.4]
+    w = 1.0 / len(rows)
+    shapes = [(1, 64, 1), (1, 128, 1), (1, 256, 1), (3, 64, 1), (3, 128, 1), (3, 256, 1), (1, 128, 2), (1, 256, 2)]
+    try:
+        abi.batch_set_config(1, 128, 1)
+        f64 = abi.batch_flux(und, parts, p.q, p.current, xyz, es, w, abi.FP64, 0)
+        for sh in shapes:
+            abi.batch_set_config(*sh)
+            if sh[2] == 1:
+                g = abi.batch_flux(und, parts, p.q, p.current, xyz, es, w, abi.FP64, 0)
+                assert abi.batch_config() == sh
+                assert normwise(g, f64) <= 1e-12, sh
+            m = abi.batch_flux(und, parts, p.q, p.current, xyz, es, w, abi.MIXED, 0)
+            assert abi.batch_config() == sh
+            errs = [normwise(m[k], f64[k]) for k in range(3)]
+            say("  shape %s: mixed vs fp64 normwise %s" % (sh, " ".join("%.2e" % e for e in errs)))
+            assert max(errs) <= 1e-5 and max(rms_rel(m[k], f64[k]) for k in range(3)) <= 1e-5
+            if sh[2] == 2:
+                for rep in range(3):
+                    gr = abi.batch_flux(und, parts, p.q, p.current, xyz, es, w, abi.MIXED, abi.BATCH_GRAPH)
+                    assert np.array_equal(gr, m), "graph replay %d of %s differs from eager" % (rep, sh)
+                    _ = abi.flux(np.zeros((6, 8)) + 1e-9, 1e-12, [[0, 0, 30]], [100.0], p.q, p.current)
+        abi.batch_set_config(1, 256, 2)
+        with pytest.raises(abi.SrsError, match="mixed precision"):
+            abi.batch_flux(und, parts, p.q, p.current, xyz, es, w, abi.FP64, 0)
+        with pytest.raises(abi.SrsError, match="no batch kernel"):
+            abi.batch_set_config(3, 256, 2)
+    finally:
+        abi.batch_set_config(0)
