@@ -696,10 +696,12 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
   L.init(tab.omega, tab.cw, tab.rdw, tab.rtl, ne, (grp * 64 + lane) * RUN);
   Tile& tl = sm.tile[wave];
   double tmax = 0.0;
+  double const cwmax = wave_max(L.cw_max());
   for (int tt = 0; tt < ntiles; ++tt) {
     int64_t const base = i0 + (int64_t)tt * 64;
     int const cnt = base < i1 ? ((i1 - base) < 64 ? (int)(i1 - base) : 64) : 0;
-    if (tt > 0) __syncthreads();
+    if (tt > 0) wave_sync();  // the wave's previous tile is consumed
+    double dmin = 1e300;
     if (lane < cnt) {
       int64_t const v = base + lane;
       int64_t const it = v < nbs ? first_b + v : first_f + v;  // time index of the point
@@ -708,17 +710,46 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
       spec_geometry(ox, oy, oz, bt.x[i], bt.y[i], bt.z[i], bt.bx[i], bt.by[i], bt.bz[i], q.dt * (double)it, G);
       tl.tau[lane] = G.tau;
       tmax = __builtin_fmax(tmax, __builtin_fabs(G.tau));
+      dmin = G.d;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         tl.a[k][lane] = (T)G.a[k];
         tl.g[k][lane] = (T)G.g[k];
       }
     }
-    __syncthreads();
+    wave_sync();
+    // mixed: this tile keeps the x, y near field unless (c/omega)/D <= kSpecNear throughout
+    bool const xy_near = !MIXED || cwmax > kSpecNear * wave_min(dmin);
     for (int k0 = 0; k0 < cnt; k0 += kSub) {
       int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
-      for (int k = k0; k < k1; ++k)
-        L.add(tl.tau[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k], tl.g[2][k]);
+      if constexpr (MIXED) {
+        // the next point's values are read from LDS while the current one is integrated
+        auto run = [&](auto near) {
+          double tn = tl.tau[k0];
+          T an0 = tl.a[0][k0], an1 = tl.a[1][k0], an2 = tl.a[2][k0];
+          T gn0 = tl.g[0][k0], gn1 = tl.g[1][k0], gn2 = tl.g[2][k0];
+          for (int k = k0; k < k1; ++k) {
+            double const tc = tn;
+            T const a0 = an0, a1 = an1, a2 = an2, g0 = gn0, g1 = gn1, g2 = gn2;
+            int const kn = k + 1 < k1 ? k + 1 : k;
+            tn = tl.tau[kn];
+            an0 = tl.a[0][kn];
+            an1 = tl.a[1][kn];
+            an2 = tl.a[2][kn];
+            gn0 = tl.g[0][kn];
+            gn1 = tl.g[1][kn];
+            gn2 = tl.g[2][kn];
+            L.template add<decltype(near)::value>(tc, a0, a1, a2, g0, g1, g2);
+          }
+        };
+        if (xy_near)
+          run(std::true_type{});
+        else
+          run(std::false_type{});
+      } else {
+        for (int k = k0; k < k1; ++k)
+          L.add(tl.tau[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k], tl.g[2][k]);
+      }
       L.flush();
     }
   }

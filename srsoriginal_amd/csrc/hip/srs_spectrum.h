@@ -42,13 +42,42 @@ namespace srs {
 struct SpecPoint {
   double tau;
   double a[3], g[3];
+  double d;  // |R|
 };
+
+// x, y near field of the mixed form: the amplitude (beta - n)/D - i (c/omega) n/D^2 of
+// the x and y axes is ((beta - n) - i ((c/omega)/D) n)/D; a tile drops the second part
+// when (c/omega)/D is below kSpecNear = 1e-9 (~2^-30, under FP32 resolution 2^-24) for
+// every energy of the wave and every point of the tile, as the paraxial flux terms do.
+// The z near field -i (c/omega) n_z/D^2 is always kept (the z far field (beta_z - n_z)/D
+// is itself ~1e-8 of the x, y one).  The FP64 form keeps every term.
+constexpr double kSpecNear = 1e-9;
+
+// Orders the LDS accesses of one wave's lanes, for a tile that the wave writes and reads
+// back by itself (no other wave touches it), in place of a block barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = __builtin_fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = __builtin_fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
 
 __device__ __forceinline__ void spec_geometry(double ox, double oy, double oz, double x, double y, double z,
                                               double bx, double by, double bz, double ti, SpecPoint& P) {
   double const rx = ox - x, ry = oy - y, rz = oz - z;
   double const d2 = rx * rx + ry * ry + rz * rz;
   double const D = __builtin_sqrt(d2);
+  P.d = D;
   P.tau = ti + div_rn(D, kC, kInvC);
   double const inv = 1.0 / D;
   double const nx = div_rn(rx, D, inv), ny = div_rn(ry, D, inv), nz = div_rn(rz, D, inv);
@@ -134,8 +163,11 @@ struct SpecLane {
     }
   }
 
-  // One trajectory point: tau (FP64) and the amplitudes A, G.
+  // One trajectory point: tau (FP64) and the amplitudes A, G.  XY_NEAR false (mixed
+  // form, tiles within kSpecNear): the x, y near-field terms are left out.
+  template <bool XY_NEAR = true>
   __device__ __forceinline__ void add(double tau, T a0, T a1, T a2, T g0, T g1, T g2) {
+    static_assert(XY_NEAR || MIXED, "the FP64 form keeps every term");
     double const h0 = w[0] * tau;
     T pc, ps, wc = (T)1, ws = (T)0;
     double psi = 0.0;
@@ -173,8 +205,17 @@ struct SpecLane {
           us = __builtin_fma(pc, st, ps * ct);
         }
       }
-      T const cs = cw[r] * us, cc = cw[r] * uc;
-      if constexpr (MIXED) {
+      if constexpr (MIXED && !XY_NEAR) {
+        float* const x = sub[r];
+        float const gz = g2 * cw[r];
+        x[0] = __builtin_fmaf(a0, uc, x[0]);
+        x[1] = __builtin_fmaf(a0, us, x[1]);
+        x[2] = __builtin_fmaf(a1, uc, x[2]);
+        x[3] = __builtin_fmaf(a1, us, x[3]);
+        x[4] = __builtin_fmaf(a2, uc, __builtin_fmaf(gz, us, x[4]));
+        x[5] = __builtin_fmaf(a2, us, __builtin_fmaf(-gz, uc, x[5]));
+      } else if constexpr (MIXED) {
+        T const cs = cw[r] * us, cc = cw[r] * uc;
         float* const x = sub[r];
         x[0] = __builtin_fmaf(a0, uc, __builtin_fmaf(g0, cs, x[0]));
         x[1] = __builtin_fmaf(a0, us, __builtin_fmaf(-g0, cc, x[1]));
@@ -183,6 +224,7 @@ struct SpecLane {
         x[4] = __builtin_fmaf(a2, uc, __builtin_fmaf(g2, cs, x[4]));
         x[5] = __builtin_fmaf(a2, us, __builtin_fmaf(-g2, cc, x[5]));
       } else {
+        T const cs = cw[r] * us, cc = cw[r] * uc;
         double* const y = acc[r];
         y[0] = __builtin_fma(a0, uc, __builtin_fma(g0, cs, y[0]));
         y[1] = __builtin_fma(a0, us, __builtin_fma(-g0, cc, y[1]));
@@ -204,6 +246,15 @@ struct SpecLane {
         ps = nps;
       }
     }
+  }
+
+  // The largest c/omega of this lane's run (its lowest energy; entries past ne repeat
+  // the last one).
+  __device__ __forceinline__ double cw_max() const {
+    T m = cw[0];
+#pragma unroll
+    for (int r = 1; r < RUN; ++r) m = m > cw[r] ? m : cw[r];
+    return (double)m;
   }
 
   // Mixed: move the FP32 sub-sums into the FP64 accumulators.

@@ -20,13 +20,6 @@ constexpr int kSpecWaves = 8;
 constexpr int kSpecThreads = 64 * kSpecWaves;
 constexpr int kFoldRun = 4;  // energies per lane folded per LDS round (8 x 4 x 6 x 64 doubles)
 
-// Lanes of one wave only: the wave's own LDS tile is written and read back by its lanes.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Issue balance of the two waves that share a SIMD (waves w and w ^ 4 of a block of 8,
 // the CU placing a block's waves on its 4 SIMDs in turn).  Both have the same number of
 // points; left alone, the older takes most issue slots and the younger finishes its
@@ -73,17 +66,20 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
   IssueBalance bal{prog, wave, wave ^ 4};
   int done = 0;
   double tmax = 0.0;  // largest |tau| this lane staged
+  double const cwmax = wave_max(L.cw_max());
   __syncthreads();  // prog is set
   for (int tt = 0; tt < ntiles; ++tt) {
     int64_t const base = i0 + (int64_t)tt * 64;
     int const cnt = base < i1 ? ((i1 - base) < 64 ? (int)(i1 - base) : 64) : 0;
     if (tt > 0) wave_sync();  // the previous tile is consumed
+    double dmin = 1e300;
     if (lane < cnt) {
       int64_t const i = base + lane;
       SpecPoint P;
       spec_geometry(ox, oy, oz, t.x[i], t.y[i], t.z[i], t.bx[i], t.by[i], t.bz[i], t.dt * (double)i, P);
       tl.tau[lane] = P.tau;
       tmax = __builtin_fmax(tmax, __builtin_fabs(P.tau));
+      dmin = P.d;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         tl.a[k][lane] = (T)P.a[k];
@@ -91,27 +87,35 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
       }
     }
     wave_sync();
+    // mixed: this tile keeps the x, y near field unless (c/omega)/D <= kSpecNear throughout
+    bool const xy_near = !MIXED || cwmax > kSpecNear * wave_min(dmin);
     for (int k0 = 0; k0 < cnt; k0 += kSub) {
       int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
       if constexpr (MIXED) {
         // the next point's values are read from LDS while the current one is integrated
-        double tn = tl.tau[k0];
-        T an0 = tl.a[0][k0], an1 = tl.a[1][k0], an2 = tl.a[2][k0];
-        T gn0 = tl.g[0][k0], gn1 = tl.g[1][k0], gn2 = tl.g[2][k0];
-        for (int k = k0; k < k1; ++k) {
-          double const tc = tn;
-          T const a0 = an0, a1 = an1, a2 = an2, g0 = gn0, g1 = gn1, g2 = gn2;
-          int const kn = k + 1 < k1 ? k + 1 : k;
-          tn = tl.tau[kn];
-          an0 = tl.a[0][kn];
-          an1 = tl.a[1][kn];
-          an2 = tl.a[2][kn];
-          gn0 = tl.g[0][kn];
-          gn1 = tl.g[1][kn];
-          gn2 = tl.g[2][kn];
-          L.add(tc, a0, a1, a2, g0, g1, g2);
-          bal.step(++done);
-        }
+        auto run = [&](auto near) {
+          double tn = tl.tau[k0];
+          T an0 = tl.a[0][k0], an1 = tl.a[1][k0], an2 = tl.a[2][k0];
+          T gn0 = tl.g[0][k0], gn1 = tl.g[1][k0], gn2 = tl.g[2][k0];
+          for (int k = k0; k < k1; ++k) {
+            double const tc = tn;
+            T const a0 = an0, a1 = an1, a2 = an2, g0 = gn0, g1 = gn1, g2 = gn2;
+            int const kn = k + 1 < k1 ? k + 1 : k;
+            tn = tl.tau[kn];
+            an0 = tl.a[0][kn];
+            an1 = tl.a[1][kn];
+            an2 = tl.a[2][kn];
+            gn0 = tl.g[0][kn];
+            gn1 = tl.g[1][kn];
+            gn2 = tl.g[2][kn];
+            L.template add<decltype(near)::value>(tc, a0, a1, a2, g0, g1, g2);
+            bal.step(++done);
+          }
+        };
+        if (xy_near)
+          run(std::true_type{});
+        else
+          run(std::false_type{});
       } else {
         for (int k = k0; k < k1; ++k) {
           L.add(tl.tau[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k], tl.g[2][k]);
