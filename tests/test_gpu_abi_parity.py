@@ -285,3 +285,22 @@ def test_mixed_flux_paraxial_and_general_paths(abi, oracle):
             e1, e2 = normwise(mix[k], f64[k]), rms_rel(mix[k], f64[k])
             say("  %-14s %7.2f eV: mixed vs fp64 normwise %.3e rms %.3e" % (name, e, e1, e2))
             assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
+
+
+@pytest.mark.parametrize("obs", [(0.0, 0.0, 2.0), (1.0e-3, 0.0, 5.0)])
+def test_mixed_spectrum_close_observer_keeps_near_field(abi, oracle, obs):
+    """Observers 0.5-6.5 m from the trajectory: at the low energies of the C0 grid
+    (c/omega)/D exceeds 1e-9 on the tiles near the observer, so the mixed kernel keeps
+    the x, y near-field terms there and leaves them out elsewhere; both against FP64,
+    which the oracle pins."""
+    F, p, t, dt = cases.oracle_setup(oracle, "spectrum_c0_full")
+    es = golden("spectrum_c0_full")["spectrum"][:, 0].copy()
+    f64 = abi.spectrum(t, dt, obs, es, p.q, p.current, abi.FP64)
+    mix = abi.spectrum(t, dt, obs, es, p.q, p.current, abi.MIXED)
+    sub = slice(1, None, 100)
+    ref = oracle.spectrum(t, dt, obs, es[sub], p.q, p.current, nthreads=8)
+    e64 = normwise(f64[sub], ref)
+    e1, e2 = normwise(mix, f64), rms_rel(mix, f64)
+    say("spectrum at %s: fp64 vs oracle %.3e; mixed vs fp64 normwise %.3e rms %.3e" % (obs, e64, e1, e2))
+    assert e64 <= TOL_FP64
+    assert e1 <= TOL_MIXED_SPECTRUM and e2 <= TOL_MIXED_SPECTRUM
