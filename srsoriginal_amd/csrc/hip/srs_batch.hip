@@ -290,12 +290,13 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
 // j0 + tid and j0 + 256 + tid, so each staged paraxial record serves both
 // (flux_terms_para2).  Particles flagged as not travelling along z take the FP64 terms
 // and items that fail the paraxial bounds the general mixed terms, one point after the
-// other.  Same queue, segments and outputs as batch_flux_kernel.
+// other.  Same queue, segments and outputs as batch_flux_kernel.  Held to 4 waves per
+// SIMD (128 VGPRs): the fallback paths' extra registers spill, but only outside the
+// integration loops (per tile and per item), which the compiler left without spills.
 template <int TILE_>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_flux2_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
-                                                             const double* __restrict__ obs, int ns, BatchEnergies be,
-                                                             unsigned nitems, unsigned* __restrict__ counter,
-                                                             double* __restrict__ flux_pp, FluxSeg sg) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_flux2_kernel(
+    BatchTraj bt, const BatchParticle* __restrict__ P, const double* __restrict__ obs, int ns, BatchEnergies be,
+    unsigned nitems, unsigned* __restrict__ counter, double* __restrict__ flux_pp, FluxSeg sg) {
   __shared__ BatchTile<TILE_, 1, true> T;
   __shared__ double red[4 * (kBlock / 64)];
   __shared__ unsigned s_item;
