@@ -301,8 +301,9 @@ int plan_flux(Context& c, const TrajView& t, int64_t ns, int64_t ne, int precisi
   pl->ne0 = (int)std::min<int64_t>(srs::kMaxFluxEnergies, ne);
   int const per_cu = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_blocks_per_cu(pl->ne0)
                                                       : srs::flux_fp64_blocks_per_cu(pl->ne0);
-  int64_t const ppl = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_points_per_lane(pl->ne0) : 1;
-  pl->cm = flux_chunks(t.n, (ns + ppl - 1) / ppl, 32, (int64_t)c.cus * per_cu);
+  // point blocks of the launch, as the lanes of that many 256-lane blocks for the chunk choice
+  int64_t const ppb = precision == SRS_PRECISION_MIXED ? srs::flux_mixed_points_per_block(pl->ne0) : srs::kBlock;
+  pl->cm = flux_chunks(t.n, (ns + ppb - 1) / ppb * srs::kBlock, 32, (int64_t)c.cus * per_cu);
   pl->nch = pl->cm.nch;
   size_t const staged = precision == SRS_PRECISION_MIXED ? srs::traj_f32_floats(t.n, 0) * sizeof(float) : 0;
   pl->part_off = (staged + 255) / 256 * 256;

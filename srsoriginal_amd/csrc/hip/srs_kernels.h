@@ -45,8 +45,10 @@ int flux_fp64_blocks_per_cu(int ne);
 int pd_fp64_blocks_per_cu();
 int flux1_fp64_blocks_per_cu();
 int flux_mixed_blocks_per_cu(int ne);
-// Observation points per lane of the mixed flux launch for ne energies (one energy: two).
-constexpr int flux_mixed_points_per_lane(int ne) { return ne == 1 ? 2 : 1; }
+// One energy: blocks of kFlux2Threads lanes, two observation points per lane.
+constexpr int kFlux2Threads = 256;
+// Observation points per block of the mixed flux launch for ne energies.
+constexpr int flux_mixed_points_per_block(int ne) { return ne == 1 ? 2 * kFlux2Threads : kBlock; }
 int pd_mixed_blocks_per_cu(int directional);
 
 // ---- plain FP64 form ------------------------------------------------------

@@ -48,10 +48,12 @@ __global__ __launch_bounds__(kFlagSpan) void traj_to_f32_kernel(TrajView t, int 
   if (threadIdx.x == 0) reinterpret_cast<int*>(f + (with_a ? 7 : 3) * t.n)[blockIdx.x] = any;
 }
 
-// Whether any point of [i0, i1) does not travel along z, from the span flags (block-uniform).
+// Whether any point of [i0, i1) does not travel along z, from the span flags (block-uniform,
+// NT threads).
+template <int NT = kBlock>
 __device__ __forceinline__ bool chunk_transverse(const int* __restrict__ flags, int64_t i0, int64_t i1) {
   int f = 0;
-  for (int64_t b = i0 / kFlagSpan + threadIdx.x; b <= (i1 - 1) / kFlagSpan; b += kBlock) f |= flags[b];
+  for (int64_t b = i0 / kFlagSpan + threadIdx.x; b <= (i1 - 1) / kFlagSpan; b += NT) f |= flags[b];
   return __syncthreads_or(f) != 0;
 }
 
@@ -158,21 +160,21 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
   }
 }
 
-// One energy, two observation points per lane: a block owns points j0 + tid and
-// j0 + 256 + tid (j0 = 512 blockIdx.x) and one trajectory chunk; the paraxial records
+// One energy, two observation points per lane: a block of NT lanes owns points j0 + tid
+// and j0 + NT + tid (j0 = 2 NT blockIdx.x) and one trajectory chunk; the paraxial records
 // are read once for both points (flux_terms_para2).  Chunks whose beam is not along z
 // and blocks that fail the paraxial check sum the FP64 / general mixed terms one point
 // after the other.  part as flux_mixed_kernel<1, TILE>.
-template <int TILE_>
-__global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const float* __restrict__ tf,
+template <int TILE_, int NT>
+__global__ __launch_bounds__(NT) void flux_mixed2_kernel(TrajView t, const float* __restrict__ tf,
                                                               const int* __restrict__ flags,
                                                               const double* __restrict__ obs, int ns, EnergySet es,
                                                               ChunkMap cm, double* __restrict__ part) {
   __shared__ MixedTile<TILE_, 1> T;
-  __shared__ double red[4 * (kBlock / 64)];
+  __shared__ double red[4 * (NT / 64)];
   int const tid = threadIdx.x;
-  int const j0 = blockIdx.x * (2 * kBlock);
-  int const jp[2] = {j0 + tid, j0 + kBlock + tid};
+  int const j0 = blockIdx.x * (2 * NT);
+  int const jp[2] = {j0 + tid, j0 + NT + tid};
   double ox[2], oy[2], oz[2];
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
   float cwf = (float)es.cw[0];
   int64_t const i0 = cm.begin(blockIdx.y);
   int64_t const i1 = cm.end(blockIdx.y, t.n);
-  if (chunk_transverse(flags, i0, i1)) {
+  if (chunk_transverse<NT>(flags, i0, i1)) {
     // the beam of this chunk is not along z: the FP64 terms, one point after the other
 #pragma unroll 1
     for (int p = 0; p < 2; ++p) {
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
       for (int64_t base = i0; base < i1; base += TILE_) {
         int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
         __syncthreads();
-        for (int k = tid; k < cnt; k += kBlock) {
+        for (int k = tid; k < cnt; k += NT) {
           int64_t const i = base + k;
           T.raw.t[k] = t.dt * (double)(base + k);
           T.raw.x[k] = t.x[i];
@@ -221,10 +223,10 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
     return;
   }
   double const z0 = obs[2 * ns + j0];
-  ParaBlock const pb = para_block_range(__builtin_fmin(ox[0], ox[1]), __builtin_fmax(ox[0], ox[1]),
+  ParaBlock const pb = para_block_range<NT>(__builtin_fmin(ox[0], ox[1]), __builtin_fmax(ox[0], ox[1]),
                                         __builtin_fmin(oy[0], oy[1]), __builtin_fmax(oy[0], oy[1]),
                                         oz[0] == z0 && oz[1] == z0, z0, red);
-  bool const para = para_check(pb, t.x, t.y, t.z, i0, i1, es);
+  bool const para = para_check<NT>(pb, t.x, t.y, t.z, i0, i1, es);
   AccMixed<1> a[2];
   a[0].zero();
   a[1].zero();
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
       int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
       __syncthreads();
       int const cntp = (cnt + kSub - 1) / kSub * kSub;
-      for (int k = tid; k < cntp; k += kBlock) {
+      for (int k = tid; k < cntp; k += NT) {
         int64_t const i = base + k;
         if (k < cnt)
           para_stage<TILE_, 1>(T.para, k, t.x[i], t.y[i], t.z[i], tf[i], tf[t.n + i], tf[2 * t.n + i], i, t.dt, pb,
@@ -248,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed2_kernel(TrajView t, const f
     for (int64_t base = i0; base < i1; base += TILE_) {
       int const cnt = (i1 - base) < TILE_ ? (int)(i1 - base) : TILE_;
       __syncthreads();
-      for (int k = tid; k < cnt; k += kBlock) {
+      for (int k = tid; k < cnt; k += NT) {
         int64_t const i = base + k;
         T.raw.t[k] = t.dt * (double)(base + k);
         T.raw.x[k] = t.x[i];
@@ -390,7 +392,7 @@ static int blocks_per_cu(K kernel, int threads, int& cache) {
 int flux_mixed_blocks_per_cu(int ne) {
   static int c[5];
   switch (ne) {
-    case 1: return blocks_per_cu(flux_mixed2_kernel<TILE>, kBlock, c[1]);
+    case 1: return blocks_per_cu(flux_mixed2_kernel<kFlux2Threads, kFlux2Threads>, kFlux2Threads, c[1]);
     case 2: return blocks_per_cu(flux_mixed_kernel<2, TILE>, kBlock, c[2]);
     case 3: return blocks_per_cu(flux_mixed_kernel<3, TILE>, kBlock, c[3]);
     default: return blocks_per_cu(flux_mixed_kernel<4, TILE>, kBlock, c[4]);
@@ -419,8 +421,10 @@ hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double
   switch (es.ne) {
     case 1: {
       // one energy: two observation points per lane
-      dim3 const grid2((ns + 2 * kBlock - 1) / (2 * kBlock), cm.nch);
-      flux_mixed2_kernel<TILE><<<grid2, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, cm, part);
+      constexpr int ppb = flux_mixed_points_per_block(1);
+      dim3 const grid2((ns + ppb - 1) / ppb, cm.nch);
+      flux_mixed2_kernel<kFlux2Threads, kFlux2Threads>
+          <<<grid2, kFlux2Threads, 0, s>>>(t, trajf, fl, obs_soa, ns, es, cm, part);
       break;
     }
     case 2: flux_mixed_kernel<2, TILE><<<grid, kBlock, 0, s>>>(t, trajf, fl, obs_soa, ns, es, cm, part); break;

@@ -221,9 +221,10 @@ struct ParaBlock {
   bool flat;  // every point of the block has oz == z
 };
 
-// Called by every lane of the block with the range [xlo, xhi] x [ylo, yhi] of its
-// points and whether they all lie on z = z0; lanes without a point pass a point of the
-// block.  red holds 4 * (kBlock / 64) doubles of LDS.
+// Called by every lane of the block (NT threads) with the range [xlo, xhi] x [ylo, yhi]
+// of its points and whether they all lie on z = z0; lanes without a point pass a point
+// of the block.  red holds 4 * (NT / 64) doubles of LDS.
+template <int NT = kBlock>
 __device__ __forceinline__ ParaBlock para_block_range(double xlo, double xhi, double ylo, double yhi, bool on_plane,
                                                       double z0, double* red) {
   double v[4] = {xlo, -xhi, ylo, -yhi};  // minima of these give xlo, xhi, ylo, yhi
@@ -240,7 +241,7 @@ __device__ __forceinline__ ParaBlock para_block_range(double xlo, double xhi, do
   b.z = z0;
   b.flat = flat;
   double m[4] = {red[0], red[1], red[2], red[3]};
-  for (int w = 1; w < kBlock / 64; ++w)
+  for (int w = 1; w < NT / 64; ++w)
 #pragma unroll
     for (int q = 0; q < 4; ++q) m[q] = __builtin_fmin(m[q], red[w * 4 + q]);
   b.xlo = m[0];
@@ -268,8 +269,9 @@ __device__ __forceinline__ bool para_point_ok(ParaBlock const& b, double xk, dou
 }
 
 // Block-uniform decision for trajectory points [i0, i1) of arrays x, y, z: every
-// lane checks a strided share, the block takes the paraxial form only if all pass.
-// Called by every lane of the block.
+// lane checks a strided share, the block (NT threads) takes the paraxial form only if
+// all pass.  Called by every lane of the block.
+template <int NT = kBlock>
 __device__ __forceinline__ bool para_check(ParaBlock const& b, const double* __restrict__ x,
                                            const double* __restrict__ y, const double* __restrict__ z, int64_t i0,
                                            int64_t i1, EnergySet const& es) {
@@ -280,7 +282,7 @@ __device__ __forceinline__ bool para_check(ParaBlock const& b, const double* __r
   }
   bool ok = b.flat;
   if (ok)
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBlock)
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += NT)
       ok = ok && para_point_ok(b, x[i], y[i], z[i], fwmax, cwmax);
   return __syncthreads_and(ok) != 0;
 }
