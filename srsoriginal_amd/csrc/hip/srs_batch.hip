@@ -670,7 +670,8 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
   using T = typename SpecLane<MIXED, RUN>::T;
   constexpr int FR = RUN < kSbFold ? RUN : kSbFold;
   struct Tile {
-    double tau[64];
+    double tau[64], psi[64];
+    T wc[64], ws[64];
     T a[3][64], g[3][64];
   };
   __shared__ union {
@@ -698,6 +699,8 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
   Tile& tl = sm.tile[wave];
   double tmax = 0.0;
   double const cwmax = wave_max(L.cw_max());
+  // the step of every run of this block (one value per block, spectrum_run_tables)
+  double const dwb = RUN > 1 ? tab.rdw[(grp * 64 * RUN < ne ? grp * 64 * RUN : 0) / RUN] : 0.0;
   for (int tt = 0; tt < ntiles; ++tt) {
     int64_t const base = i0 + (int64_t)tt * 64;
     int const cnt = base < i1 ? ((i1 - base) < 64 ? (int)(i1 - base) : 64) : 0;
@@ -710,6 +713,12 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
       SpecPoint G;
       spec_geometry(ox, oy, oz, bt.x[i], bt.y[i], bt.z[i], bt.bx[i], bt.by[i], bt.bz[i], q.dt * (double)it, G);
       tl.tau[lane] = G.tau;
+      double psi = 0.0;
+      T wc = (T)1, ws = (T)0;
+      if constexpr (RUN > 1) spec_step<MIXED>(dwb, G.tau, psi, wc, ws);
+      tl.psi[lane] = psi;
+      tl.wc[lane] = wc;
+      tl.ws[lane] = ws;
       tmax = __builtin_fmax(tmax, __builtin_fabs(G.tau));
       dmin = G.d;
 #pragma unroll
@@ -726,21 +735,26 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
       if constexpr (MIXED) {
         // the next point's values are read from LDS while the current one is integrated
         auto run = [&](auto near) {
-          double tn = tl.tau[k0];
+          double tn = tl.tau[k0], pn = tl.psi[k0];
+          T wcn = tl.wc[k0], wsn = tl.ws[k0];
           T an0 = tl.a[0][k0], an1 = tl.a[1][k0], an2 = tl.a[2][k0];
           T gn0 = tl.g[0][k0], gn1 = tl.g[1][k0], gn2 = tl.g[2][k0];
           for (int k = k0; k < k1; ++k) {
-            double const tc = tn;
+            double const tc = tn, psc = pn;
+            T const wcc = wcn, wsc = wsn;
             T const a0 = an0, a1 = an1, a2 = an2, g0 = gn0, g1 = gn1, g2 = gn2;
             int const kn = k + 1 < k1 ? k + 1 : k;
             tn = tl.tau[kn];
+            pn = tl.psi[kn];
+            wcn = tl.wc[kn];
+            wsn = tl.ws[kn];
             an0 = tl.a[0][kn];
             an1 = tl.a[1][kn];
             an2 = tl.a[2][kn];
             gn0 = tl.g[0][kn];
             gn1 = tl.g[1][kn];
             gn2 = tl.g[2][kn];
-            L.template add<decltype(near)::value>(tc, a0, a1, a2, g0, g1, g2);
+            L.template add<decltype(near)::value>(tc, psc, wcc, wsc, a0, a1, a2, g0, g1, g2);
           }
         };
         if (xy_near)
@@ -749,7 +763,8 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
           run(std::false_type{});
       } else {
         for (int k = k0; k < k1; ++k)
-          L.add(tl.tau[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k], tl.g[2][k]);
+          L.add(tl.tau[k], tl.psi[k], tl.wc[k], tl.ws[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k],
+                tl.g[2][k]);
       }
       L.flush();
     }

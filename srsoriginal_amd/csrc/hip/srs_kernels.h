@@ -90,7 +90,8 @@ struct SpectrumPlan {
 };
 SpectrumPlan plan_spectrum(int64_t n, int64_t ne, bool uniform, int mixed, int cus);
 // Device tables of a spectrum launch: per energy omega and c/omega; per run of
-// kSpecRun energies the step rdw and the |tau| bound rtl of the phase correction, from
+// kSpecRun energies the step rdw (the mean spacing of the run's block of 64 runs, the
+// same for all of them) and the |tau| bound rtl of the phase correction, from
 // spectrum_run_tables (host) for the form (mixed or FP64) that is launched.
 struct SpecTables {
   const double* omega;

@@ -10,8 +10,11 @@
 //
 // A lane owns a run of RUN consecutive energies omega_0..omega_{RUN-1} (a uniform grid
 // up to rounding) and, per trajectory point:
-//   h_0 = RN(omega_0 tau), psi = RN(dw tau)      (dw: the run's mean spacing)
-//   p_0 = exp(i h_0),  w = exp(i psi)           (one sin/cos pair each, FP64 reduction)
+//   h_0 = RN(omega_0 tau), psi = RN(dw tau)      (dw: the mean spacing of the block's
+//                                                 64 runs, the same for all its lanes)
+//   p_0 = exp(i h_0),  w = exp(i psi)           (one sin/cos pair each, FP64 reduction;
+//                                                 psi and w once per point per block,
+//                                                 by the lane that stages the point)
 //   p_r = p_{r-1} w                              (the recurrence: angle h_0 + r psi)
 //   theta_r = RN(omega_r tau) - h_0 - r psi      (three FP64 operations, exactly: the
 //             difference of the two products is exact by Sterbenz's lemma while
@@ -90,6 +93,22 @@ __device__ __forceinline__ void spec_geometry(double ox, double oy, double oz, d
   P.g[2] = nz * inv2;
 }
 
+// Forward declaration (defined below).
+__device__ __forceinline__ void sincos_quadrant_f32(double x, float* s_out, float* c_out);
+
+// The step of a run for one trajectory point: psi = RN(dw tau) and exp(i psi) (FP32 to
+// ~1 ulp for the mixed form, sincos_fp64 for the FP64 form).
+template <bool MIXED>
+__device__ __forceinline__ void spec_step(double dw, double tau, double& psi,
+                                          typename std::conditional<MIXED, float, double>::type& wc,
+                                          typename std::conditional<MIXED, float, double>::type& ws) {
+  psi = dw * tau;
+  if constexpr (MIXED)
+    sincos_quadrant_f32(psi, &ws, &wc);
+  else
+    sincos_fp64(psi, &ws, &wc);
+}
+
 // sin and cos of an FP64 angle to FP32 accuracy (~1 ulp): FP64 reduction to
 // [-pi/4, pi/4] and the quadrant (as sincos_fp64), then Taylor polynomials through
 // r^9 / r^8 in FP32 (truncation < 2.5e-8 on the interval).
@@ -134,8 +153,8 @@ struct SpecLane {
   T sub[RUN][6];   // mixed: the current FP32 sub-sums; FP64: unused
 
   // Energies e0 .. e0 + RUN - 1 of the tables omega[ne], cw[ne] (= c/omega) and of the
-  // run's step rdw[e0 / RUN] and bound rtl[e0 / RUN] (spectrum_run_tables); entries
-  // beyond ne continue the run's line (their sums are not written).
+  // run's step rdw[e0 / RUN] (its block's) and bound rtl[e0 / RUN] (spectrum_run_tables);
+  // entries beyond ne continue the run's line (their sums are not written).
   __device__ __forceinline__ void init(const double* __restrict__ omega, const double* __restrict__ cwt,
                                        const double* __restrict__ rdw, const double* __restrict__ rtl, int ne,
                                        int e0) {
@@ -163,14 +182,15 @@ struct SpecLane {
     }
   }
 
-  // One trajectory point: tau (FP64) and the amplitudes A, G.  XY_NEAR false (mixed
-  // form, tiles within kSpecNear): the x, y near-field terms are left out.
+  // One trajectory point: tau (FP64), the step angle psi = RN(dw tau) and its phasor
+  // (wc, ws) = exp(i psi) (spec_step; unused for RUN 1), and the amplitudes A, G.
+  // XY_NEAR false (mixed form, tiles within kSpecNear): the x, y near-field terms are
+  // left out.
   template <bool XY_NEAR = true>
-  __device__ __forceinline__ void add(double tau, T a0, T a1, T a2, T g0, T g1, T g2) {
+  __device__ __forceinline__ void add(double tau, double psi, T wc, T ws, T a0, T a1, T a2, T g0, T g1, T g2) {
     static_assert(XY_NEAR || MIXED, "the FP64 form keeps every term");
     double const h0 = w[0] * tau;
-    T pc, ps, wc = (T)1, ws = (T)0;
-    double psi = 0.0;
+    T pc, ps;
     if constexpr (MIXED) {
       float s, c;
       sincos_reduced_f32(reduce_two_pi(h0), &s, &c);
@@ -178,13 +198,6 @@ struct SpecLane {
       pc = c;
     } else {
       sincos_fp64(h0, &ps, &pc);
-    }
-    if constexpr (RUN > 1) {
-      psi = dw * tau;
-      if constexpr (MIXED)
-        sincos_quadrant_f32(psi, &ws, &wc);
-      else
-        sincos_fp64(psi, &ws, &wc);
     }
 #pragma unroll
     for (int r = 0; r < RUN; ++r) {

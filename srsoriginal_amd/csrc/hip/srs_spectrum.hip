@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
   using T = typename SpecLane<MIXED, RUN>::T;
   constexpr int FR = RUN < kFoldRun ? RUN : kFoldRun;  // energies of the run folded per round
   struct Tile {
-    double tau[64];
+    double tau[64], psi[64];
+    T wc[64], ws[64];
     T a[3][64], g[3][64];
   };
   __shared__ union {
@@ -67,6 +68,8 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
   int done = 0;
   double tmax = 0.0;  // largest |tau| this lane staged
   double const cwmax = wave_max(L.cw_max());
+  // the step of every run of this block (one value per block, spectrum_run_tables)
+  double const dwb = RUN > 1 ? tab.rdw[(blockIdx.x * 64 * RUN < ne ? blockIdx.x * 64 * RUN : 0) / RUN] : 0.0;
   __syncthreads();  // prog is set
   for (int tt = 0; tt < ntiles; ++tt) {
     int64_t const base = i0 + (int64_t)tt * 64;
@@ -78,6 +81,12 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
       SpecPoint P;
       spec_geometry(ox, oy, oz, t.x[i], t.y[i], t.z[i], t.bx[i], t.by[i], t.bz[i], t.dt * (double)i, P);
       tl.tau[lane] = P.tau;
+      double psi = 0.0;
+      T wc = (T)1, ws = (T)0;
+      if constexpr (RUN > 1) spec_step<MIXED>(dwb, P.tau, psi, wc, ws);
+      tl.psi[lane] = psi;
+      tl.wc[lane] = wc;
+      tl.ws[lane] = ws;
       tmax = __builtin_fmax(tmax, __builtin_fabs(P.tau));
       dmin = P.d;
 #pragma unroll
@@ -94,21 +103,26 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
       if constexpr (MIXED) {
         // the next point's values are read from LDS while the current one is integrated
         auto run = [&](auto near) {
-          double tn = tl.tau[k0];
+          double tn = tl.tau[k0], pn = tl.psi[k0];
+          T wcn = tl.wc[k0], wsn = tl.ws[k0];
           T an0 = tl.a[0][k0], an1 = tl.a[1][k0], an2 = tl.a[2][k0];
           T gn0 = tl.g[0][k0], gn1 = tl.g[1][k0], gn2 = tl.g[2][k0];
           for (int k = k0; k < k1; ++k) {
-            double const tc = tn;
+            double const tc = tn, psc = pn;
+            T const wcc = wcn, wsc = wsn;
             T const a0 = an0, a1 = an1, a2 = an2, g0 = gn0, g1 = gn1, g2 = gn2;
             int const kn = k + 1 < k1 ? k + 1 : k;
             tn = tl.tau[kn];
+            pn = tl.psi[kn];
+            wcn = tl.wc[kn];
+            wsn = tl.ws[kn];
             an0 = tl.a[0][kn];
             an1 = tl.a[1][kn];
             an2 = tl.a[2][kn];
             gn0 = tl.g[0][kn];
             gn1 = tl.g[1][kn];
             gn2 = tl.g[2][kn];
-            L.template add<decltype(near)::value>(tc, a0, a1, a2, g0, g1, g2);
+            L.template add<decltype(near)::value>(tc, psc, wcc, wsc, a0, a1, a2, g0, g1, g2);
             bal.step(++done);
           }
         };
@@ -118,7 +132,8 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
           run(std::false_type{});
       } else {
         for (int k = k0; k < k1; ++k) {
-          L.add(tl.tau[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k], tl.g[2][k]);
+          L.add(tl.tau[k], tl.psi[k], tl.wc[k], tl.ws[k], tl.a[0][k], tl.a[1][k], tl.a[2][k], tl.g[0][k], tl.g[1][k],
+                tl.g[2][k]);
           bal.step(++done);
         }
       }
@@ -211,18 +226,31 @@ int spectrum_blocks_per_cu(int mixed, int run) {
 
 }  // namespace
 
+// Energies per block of the run kernels: 64 lanes x kSpecRun, all stepping with one
+// spacing (the block's mean step), so that the step's angle and phasor are formed once
+// per trajectory point and block.
+constexpr int64_t kSpecGroup = 64 * kSpecRun;
+
+// The mean step of the group of energies starting at g0.
+static double group_step(const double* omega, int64_t ne, int64_t g0) {
+  int64_t const len = (ne - g0) < kSpecGroup ? ne - g0 : kSpecGroup;
+  return len > 1 ? (omega[g0 + len - 1] - omega[g0]) / (double)(len - 1) : 0.0;
+}
+
 bool spectrum_runs_uniform(const double* omega, int64_t ne) {
-  for (int64_t e0 = 0; e0 < ne; e0 += kSpecRun) {
-    int64_t const len = (ne - e0) < kSpecRun ? ne - e0 : kSpecRun;
-    double const w0 = omega[e0];
-    if (!(w0 > 0.0) || !(w0 < 1e300)) return false;
-    if (len < 2) continue;
-    double const dw = (omega[e0 + len - 1] - w0) / (double)(len - 1);
-    for (int64_t r = 1; r < len; ++r) {
-      double const wr = omega[e0 + r];
-      if (!(wr >= 0.6 * w0 && wr <= 1.6 * w0)) return false;
-      // a uniform grid up to rounding: the correction angle stays in its series' range
-      if (!(__builtin_fabs(wr - (w0 + (double)r * dw)) <= 64.0 * 0x1p-52 * wr)) return false;
+  for (int64_t g0 = 0; g0 < ne; g0 += kSpecGroup) {
+    double const dw = group_step(omega, ne, g0);
+    int64_t const gend = (ne - g0) < kSpecGroup ? ne : g0 + kSpecGroup;
+    for (int64_t e0 = g0; e0 < gend; e0 += kSpecRun) {
+      int64_t const len = (gend - e0) < kSpecRun ? gend - e0 : kSpecRun;
+      double const w0 = omega[e0];
+      if (!(w0 > 0.0) || !(w0 < 1e300)) return false;
+      for (int64_t r = 1; r < len; ++r) {
+        double const wr = omega[e0 + r];
+        if (!(wr >= 0.6 * w0 && wr <= 1.6 * w0)) return false;
+        // a uniform grid up to rounding: the correction angle stays in its series' range
+        if (!(__builtin_fabs(wr - (w0 + (double)r * dw)) <= 64.0 * 0x1p-52 * wr)) return false;
+      }
     }
   }
   return true;
@@ -232,12 +260,13 @@ void spectrum_run_tables(const double* omega, int64_t ne, int mixed, double* rdw
   // theta_r = (omega_r - omega_0 - r dw) tau + e_r - e_0 - r e_psi with each rounding
   // error e_x at most 2^-53 |x tau|, so |theta_r| <= |tau| K, K the largest over the run
   // of |omega_r - omega_0 - r dw| (plus one ulp of omega_r for its own rounding)
-  // + 2^-53 (omega_r + omega_0 + r |dw|); tau_lim = range / (1.01 K).
+  // + 2^-53 (omega_r + omega_0 + r |dw|); tau_lim = range / (1.01 K).  dw is the mean
+  // step of the run's group of kSpecGroup energies (one per block).
   double const thr = mixed ? (double)kThetaF32 : kThetaF64;
   for (int64_t e0 = 0, g = 0; e0 < ne; e0 += kSpecRun, ++g) {
     int64_t const len = (ne - e0) < kSpecRun ? ne - e0 : kSpecRun;
     double const w0 = omega[e0];
-    double const dw = len > 1 ? (omega[e0 + len - 1] - w0) / (double)(len - 1) : 0.0;
+    double const dw = group_step(omega, ne, e0 / kSpecGroup * kSpecGroup);
     double K = 0.0;
     for (int64_t r = 1; r < len; ++r) {
       double const wr = omega[e0 + r];
