@@ -699,13 +699,16 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
   Tile& tl = sm.tile[wave];
   double tmax = 0.0;
   double const cwmax = wave_max(L.cw_max());
+  // mixed: tiles whose |tau| stays within a tenth of every lane's series range take the
+  // first-order correction
+  double const tlim1 = kFirstOrderFrac * wave_min(L.tau_lim);
   // the step of every run of this block (one value per block, spectrum_run_tables)
   double const dwb = RUN > 1 ? tab.rdw[(grp * 64 * RUN < ne ? grp * 64 * RUN : 0) / RUN] : 0.0;
   for (int tt = 0; tt < ntiles; ++tt) {
     int64_t const base = i0 + (int64_t)tt * 64;
     int const cnt = base < i1 ? ((i1 - base) < 64 ? (int)(i1 - base) : 64) : 0;
     if (tt > 0) wave_sync();  // the wave's previous tile is consumed
-    double dmin = 1e300;
+    double dmin = 1e300, tabs = 0.0;
     if (lane < cnt) {
       int64_t const v = base + lane;
       int64_t const it = v < nbs ? first_b + v : first_f + v;  // time index of the point
@@ -719,7 +722,8 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
       tl.psi[lane] = psi;
       tl.wc[lane] = wc;
       tl.ws[lane] = ws;
-      tmax = __builtin_fmax(tmax, __builtin_fabs(G.tau));
+      tabs = __builtin_fabs(G.tau);
+      tmax = __builtin_fmax(tmax, tabs);
       dmin = G.d;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -730,6 +734,7 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
     wave_sync();
     // mixed: this tile keeps the x, y near field unless (c/omega)/D <= kSpecNear throughout
     bool const xy_near = !MIXED || cwmax > kSpecNear * wave_min(dmin);
+    bool const second = !MIXED || wave_max(tabs) > tlim1;
     for (int k0 = 0; k0 < cnt; k0 += kSub) {
       int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
       if constexpr (MIXED) {
@@ -754,10 +759,12 @@ __global__ __launch_bounds__(64 * kSbWaves) void spectrum_batch_kernel(BatchTraj
             gn0 = tl.g[0][kn];
             gn1 = tl.g[1][kn];
             gn2 = tl.g[2][kn];
-            L.template add<decltype(near)::value>(tc, psc, wcc, wsc, a0, a1, a2, g0, g1, g2);
+            L.template add<decltype(near)::value, decltype(near)::value>(tc, psc, wcc, wsc, a0, a1, a2, g0, g1, g2);
           }
         };
-        if (xy_near)
+        // every term of the tile in full, or the fast form: no x, y near field and the
+        // first-order correction
+        if (xy_near || second)
           run(std::true_type{});
         else
           run(std::false_type{});

@@ -137,9 +137,13 @@ __device__ __forceinline__ void sincos_quadrant_f32(double x, float* s_out, floa
 
 // Series ranges of the correction exp(i theta): FP32 keeps 1 - theta^2/2 and theta
 // (dropped terms < 1e-8 for |theta| <= 4e-3); FP64 keeps 1 - theta^2/2 + theta^4/24 and
-// theta - theta^3/6 (dropped terms < 1e-12 for |theta| <= 1e-2).
+// theta - theta^3/6 (dropped terms < 1e-12 for |theta| <= 1e-2).  Where every |theta|
+// of a tile is within kThetaF32First = kThetaF32 / 10 the mixed form keeps only
+// 1 + i theta (the dropped theta^2/2 < 8e-8, FP32 rounding level); the same bound K
+// gives that range at |tau| <= tau_lim / 10.
 constexpr float kThetaF32 = 4e-3f;
 constexpr double kThetaF64 = 1e-2;
+constexpr double kFirstOrderFrac = 0.1;
 
 // The sums of one lane: RUN energies x 6 reals (re/im of the x, y, z field sums).
 template <bool MIXED, int RUN>
@@ -185,10 +189,11 @@ struct SpecLane {
   // One trajectory point: tau (FP64), the step angle psi = RN(dw tau) and its phasor
   // (wc, ws) = exp(i psi) (spec_step; unused for RUN 1), and the amplitudes A, G.
   // XY_NEAR false (mixed form, tiles within kSpecNear): the x, y near-field terms are
-  // left out.
-  template <bool XY_NEAR = true>
+  // left out; SECOND false (mixed form, tiles within tau_lim / 10): the correction
+  // keeps its first order only.
+  template <bool XY_NEAR = true, bool SECOND = true>
   __device__ __forceinline__ void add(double tau, double psi, T wc, T ws, T a0, T a1, T a2, T g0, T g1, T g2) {
-    static_assert(XY_NEAR || MIXED, "the FP64 form keeps every term");
+    static_assert((XY_NEAR && SECOND) || MIXED, "the FP64 form keeps every term");
     double const h0 = w[0] * tau;
     T pc, ps;
     if constexpr (MIXED) {
@@ -205,11 +210,15 @@ struct SpecLane {
       if (r > 0) {
         // exp(i theta_r) by its series (the caller checks |tau| <= tau_lim)
         double const th = __builtin_fma(-(double)r, psi, w[r] * tau - h0);
-        if constexpr (MIXED) {
+        if constexpr (MIXED && SECOND) {
           float const f = (float)th;
           float const ct = __builtin_fmaf(f * f, -0.5f, 1.0f);
           uc = __builtin_fmaf(-ps, f, pc * ct);
           us = __builtin_fmaf(pc, f, ps * ct);
+        } else if constexpr (MIXED) {
+          float const f = (float)th;
+          uc = __builtin_fmaf(-ps, f, pc);
+          us = __builtin_fmaf(pc, f, ps);
         } else {
           double const t2 = th * th;
           double const ct = __builtin_fma(t2, __builtin_fma(t2, 1.0 / 24.0, -0.5), 1.0);

@@ -68,6 +68,9 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
   int done = 0;
   double tmax = 0.0;  // largest |tau| this lane staged
   double const cwmax = wave_max(L.cw_max());
+  // mixed: tiles whose |tau| stays within a tenth of every lane's series range take the
+  // first-order correction
+  double const tlim1 = kFirstOrderFrac * wave_min(L.tau_lim);
   // the step of every run of this block (one value per block, spectrum_run_tables)
   double const dwb = RUN > 1 ? tab.rdw[(blockIdx.x * 64 * RUN < ne ? blockIdx.x * 64 * RUN : 0) / RUN] : 0.0;
   __syncthreads();  // prog is set
@@ -75,7 +78,7 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
     int64_t const base = i0 + (int64_t)tt * 64;
     int const cnt = base < i1 ? ((i1 - base) < 64 ? (int)(i1 - base) : 64) : 0;
     if (tt > 0) wave_sync();  // the previous tile is consumed
-    double dmin = 1e300;
+    double dmin = 1e300, tabs = 0.0;
     if (lane < cnt) {
       int64_t const i = base + lane;
       SpecPoint P;
@@ -87,7 +90,8 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
       tl.psi[lane] = psi;
       tl.wc[lane] = wc;
       tl.ws[lane] = ws;
-      tmax = __builtin_fmax(tmax, __builtin_fabs(P.tau));
+      tabs = __builtin_fabs(P.tau);
+      tmax = __builtin_fmax(tmax, tabs);
       dmin = P.d;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -98,6 +102,7 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
     wave_sync();
     // mixed: this tile keeps the x, y near field unless (c/omega)/D <= kSpecNear throughout
     bool const xy_near = !MIXED || cwmax > kSpecNear * wave_min(dmin);
+    bool const second = !MIXED || wave_max(tabs) > tlim1;
     for (int k0 = 0; k0 < cnt; k0 += kSub) {
       int const k1 = (k0 + kSub < cnt) ? k0 + kSub : cnt;
       if constexpr (MIXED) {
@@ -122,11 +127,13 @@ __global__ __launch_bounds__(kSpecThreads) void spectrum_kernel(TrajView t, doub
             gn0 = tl.g[0][kn];
             gn1 = tl.g[1][kn];
             gn2 = tl.g[2][kn];
-            L.template add<decltype(near)::value>(tc, psc, wcc, wsc, a0, a1, a2, g0, g1, g2);
+            L.template add<decltype(near)::value, decltype(near)::value>(tc, psc, wcc, wsc, a0, a1, a2, g0, g1, g2);
             bal.step(++done);
           }
         };
-        if (xy_near)
+        // every term of the tile in full, or the fast form: no x, y near field and the
+        // first-order correction
+        if (xy_near || second)
           run(std::true_type{});
         else
           run(std::false_type{});
