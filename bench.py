@@ -34,7 +34,7 @@ F_ALG_PD = 57                    # per power-density term
 F_ALG_FLUX1 = 77                 # per CalculateFlux1 (Lienard-Wiechert) term
 PEAK_FP32_VALU = 157.3e12        # MI355X FP32 vector peak (published)
 PEAK_FP64_VALU = 78.6e12         # MI355X FP64 vector peak (published)
-# Issue model of the dominant loop (flux_mixed2_kernel<256>, paraxial terms), from the
+# Issue model of the dominant loop (flux_mixed2_kernel<256, 256>, paraxial terms), from the
 # measured cycles per wave64 instruction on gfx950 (a one-off rocprofv3 measurement per class):
 # FP32 with VGPR operands 2, FP64 (add, fma, fract, cvt) 4, sin/cos 8.  Per term: 8 FP64
 # (phase 6 with fract, two conversions) + 9 FP32 (8 accumulations, 1 amplitude) + sin + cos
@@ -50,10 +50,10 @@ def _frac(flop_per_term, terms, fp64_ms, mixed_ms):
     return {"flop_per_term": flop_per_term, "unit": "TFLOP/s",
             "fp64": {"achieved": a64 / 1e12, "peak": PEAK_FP64_VALU / 1e12, "frac": a64 / PEAK_FP64_VALU},
             "mixed": {"achieved": amx / 1e12, "peak": PEAK_FP32_VALU / 1e12, "frac": amx / PEAK_FP32_VALU}}
-# HBM bytes per launch of flux_mixed2_kernel<256> on this workload, from rocprofv3
+# HBM bytes per launch of flux_mixed2_kernel<256, 256> on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
 # what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
-TRAFFIC_MIXED_BYTES = 111048.2 * 1024
+TRAFFIC_MIXED_BYTES = 111355.8 * 1024
 
 
 def log(msg):
@@ -696,7 +696,7 @@ def main():
                                      "bound_ms_at_2p4GHz": terms / 64 * ISSUE_CYCLES_PER_WAVE_TERM / (SIMDS * CLOCK_HZ)
                                      * 1e3,
                                      "frac": terms / 64 * ISSUE_CYCLES_PER_WAVE_TERM / (SIMDS * CLOCK_HZ) / kernel_s},
-                     "kernel": "flux_mixed2_kernel<256>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
+                     "kernel": "flux_mixed2_kernel<256, 256>", "kernel_ms": res[abi.MIXED]["kernel_ms"],
                      "flop_per_term": F_ALG_FLUX},
     }
     if not a.no_batch:
