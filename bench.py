@@ -50,6 +50,14 @@ def _frac(flop_per_term, terms, fp64_ms, mixed_ms):
     return {"flop_per_term": flop_per_term, "unit": "TFLOP/s",
             "fp64": {"achieved": a64 / 1e12, "peak": PEAK_FP64_VALU / 1e12, "frac": a64 / PEAK_FP64_VALU},
             "mixed": {"achieved": amx / 1e12, "peak": PEAK_FP32_VALU / 1e12, "frac": amx / PEAK_FP32_VALU}}
+def _pipeline_frac(flop_per_term, terms, fp64_ms, mixed_ms):
+    """_frac over a batched pipeline's whole time (device RK4, term kernel and reductions):
+    a lower bound on its term kernel's own fraction."""
+    r = _frac(flop_per_term, terms, fp64_ms, mixed_ms)
+    r["over"] = "whole pipeline time (RK4, term kernel, reductions)"
+    return r
+
+
 # HBM bytes per launch of flux_mixed2_kernel<256, 256> on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
 # what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
@@ -129,6 +137,7 @@ def batched_c3(nparticles=4096, reps=1):
             "fp64_terms_per_s": terms / (res["double"]["graph_ms"] * 1e-3),
             "mixed_terms_per_s": terms / (res["mixed"]["graph_ms"] * 1e-3),
             "speedup": res["double"]["graph_ms"] / res["mixed"]["graph_ms"],
+            "roofline": _pipeline_frac(F_ALG_FLUX, terms, res["double"]["graph_ms"], res["mixed"]["graph_ms"]),
             "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(mix - f64)) / np.max(np.abs(f64))),
             "mixed_vs_fp64_rms_rel_err": float(np.sqrt(np.mean((mix - f64) ** 2)) / np.sqrt(np.mean(f64 ** 2))),
             "tile_config_fp64": res["double"]["cfg"], "tile_config_mixed": res["mixed"]["cfg"],
@@ -347,6 +356,7 @@ def batched_power_density(nparticles=256, reps=2):
     return {"workload": "%d particles x %d traj points (ragged 18k-22k) x 101x101 over 40 mm at 30 m" % (nparticles, npts),
             "terms": terms, "fp64_ms": t, "fp64_terms_per_s": terms / (t * 1e-3),
             "mixed_ms": tm, "mixed_terms_per_s": terms / (tm * 1e-3), "mixed_vs_fp64_max_rel_err": err,
+            "roofline": _pipeline_frac(F_ALG_PD, terms, t, tm),
             "peak_w_per_mm2": float(np.max(out)),
             "timed": "device RK4 in time segments overlapped with pd_batch_kernel, reduction (HIP events)"}
 
@@ -386,6 +396,7 @@ def batched_spectrum(nparticles=4096, reps=1):
                         % (nparticles, npts),
             "terms": terms, "fp64_ms": t, "fp64_terms_per_s": terms / (t * 1e-3),
             "mixed_ms": tm, "mixed_terms_per_s": terms / (tm * 1e-3),
+            "roofline": _pipeline_frac(F_ALG_SPEC, terms, t, tm),
             "mixed_vs_fp64_max_rel_err": float(np.max(np.abs(om - out)) / np.max(np.abs(out))),
             "peak_flux": float(np.max(out)),
             "timed": "device RK4 in time segments overlapped with spectrum_batch_kernel, reductions (HIP events)"}
