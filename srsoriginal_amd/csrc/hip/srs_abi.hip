@@ -287,8 +287,8 @@ TrajView view_of(const srs_traj* t) {
 // ---- core launches on device data ------------------------------------------
 
 // Device scratch of one flux evaluation, in one buffer: the FP32 staging of the mixed form
-// (traj_f32_floats) at offset 0, then the chunk partial sums part[((c*ne0 + e)*6 + k)*ns + j]
-// (ne0 = energies per launch, at most kMaxFluxEnergies; launches of later energy groups
+// (traj_f32_floats) at offset 0, then the chunk partial sums part[((c*ne0 + e)*6 + k)*ns + j],
+// FP32 for the mixed form and FP64 for the FP64 form (ne0 = energies per launch, at most kMaxFluxEnergies; launches of later energy groups
 // reuse them in stream order).
 struct FluxPlan {
   srs::ChunkMap cm{};
@@ -307,7 +307,8 @@ int plan_flux(Context& c, const TrajView& t, int64_t ns, int64_t ne, int precisi
   pl->nch = pl->cm.nch;
   size_t const staged = precision == SRS_PRECISION_MIXED ? srs::traj_f32_floats(t.n, 0) * sizeof(float) : 0;
   pl->part_off = (staged + 255) / 256 * 256;
-  pl->bytes = pl->part_off + (size_t)pl->nch * pl->ne0 * 6 * ns * sizeof(double);
+  size_t const psize = precision == SRS_PRECISION_MIXED ? sizeof(float) : sizeof(double);
+  pl->bytes = pl->part_off + (size_t)pl->nch * pl->ne0 * 6 * ns * psize;
   return SRS_OK;
 }
 
@@ -318,7 +319,7 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
   double const c0 = c0_of(q), c2 = c2_of(q, cur);
   int const maxe = srs::kMaxFluxEnergies;
   float* const trajf = static_cast<float*>(scratch);
-  double* const part = reinterpret_cast<double*>(static_cast<char*>(scratch) + pl.part_off);
+  void* const part = static_cast<char*>(scratch) + pl.part_off;  // FP32 partials (mixed) or FP64
   if (precision == SRS_PRECISION_MIXED) HIPCHK(srs::launch_traj_to_f32(t, 0, trajf, s));
   for (int64_t e0 = 0; e0 < ne; e0 += maxe) {
     EnergySet es{};
@@ -334,12 +335,15 @@ int run_flux(Context& c, const TrajView& t, const double* obs_soa, int64_t ns, c
     {
       KernelTimer timer(c, s);
       if (precision == SRS_PRECISION_MIXED) {
-        HIPCHK(srs::launch_flux_mixed(t, trajf, obs_soa, (int)ns, es, pl.cm, part, s));
+        HIPCHK(srs::launch_flux_mixed(t, trajf, obs_soa, (int)ns, es, pl.cm, static_cast<float*>(part), s));
       } else {
-        HIPCHK(srs::launch_flux_fp64(t, obs_soa, (int)ns, es, pl.cm, part, s));
+        HIPCHK(srs::launch_flux_fp64(t, obs_soa, (int)ns, es, pl.cm, static_cast<double*>(part), s));
       }
     }
-    HIPCHK(srs::launch_flux_reduce(part, pl.nch, (int)ns, es, c2, out + e0 * ns, s));
+    if (precision == SRS_PRECISION_MIXED)
+      HIPCHK(srs::launch_flux_reduce(static_cast<const float*>(part), pl.nch, (int)ns, es, c2, out + e0 * ns, s));
+    else
+      HIPCHK(srs::launch_flux_reduce(static_cast<const double*>(part), pl.nch, (int)ns, es, c2, out + e0 * ns, s));
   }
   return SRS_OK;
 }

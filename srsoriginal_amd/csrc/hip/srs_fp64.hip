@@ -219,7 +219,8 @@ __global__ __launch_bounds__(kBlock) void flux1_fp64_kernel(TrajView t, const do
 // order, then wave 0 adds the four slice sums in order 0, 1, 2, 3 (a fixed order: the
 // result does not depend on timing).  Four waves per 64 points keep four times the
 // loads in flight of one lane per point.
-__global__ __launch_bounds__(256) void flux_reduce_kernel(const double* __restrict__ part, int nchunk, int ns,
+template <class P>
+__global__ __launch_bounds__(256) void flux_reduce_kernel(const P* __restrict__ part, int nchunk, int ns,
                                                           EnergySet es, double c2, double* __restrict__ out) {
   __shared__ double red[3][6][64];
   int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(256) void flux_reduce_kernel(const double* __restri
     for (int c = w; c < nchunk; c += 4) {
       size_t const b = ((size_t)(c * es.ne + e) * 6) * ns + j;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) v[k] += part[b + (size_t)k * ns];
+      for (int k = 0; k < 6; ++k) v[k] += (double)part[b + (size_t)k * ns];
     }
   }
   if (w > 0) {
@@ -370,7 +371,14 @@ hipError_t launch_total_power(const TrajView& t, double scale, double* out, hipS
 hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
                               hipStream_t s) {
   dim3 grid((ns + 63) / 64, es.ne);
-  flux_reduce_kernel<<<grid, 256, 0, s>>>(part, nchunk, ns, es, c2, out);
+  flux_reduce_kernel<double><<<grid, 256, 0, s>>>(part, nchunk, ns, es, c2, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_flux_reduce(const float* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
+                              hipStream_t s) {
+  dim3 grid((ns + 63) / 64, es.ne);
+  flux_reduce_kernel<float><<<grid, 256, 0, s>>>(part, nchunk, ns, es, c2, out);
   return hipGetLastError();
 }
 

@@ -64,8 +64,11 @@ hipError_t launch_pd_fp64(const TrajView& t, const double* obs_soa, const double
                           int chunk, double* part, hipStream_t s);
 
 // ---- mixed form (FP64 phase, FP32 trig / amplitudes / tile sums) -----------
+// Partial sums as launch_flux_fp64's, rounded to FP32 (each is a Kahan-compensated FP32
+// sum, or an FP64 one for chunks the FP64 terms integrate): half the bytes to write and
+// to fold.
 hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
-                             ChunkMap const& cm, double* part, hipStream_t s);
+                             ChunkMap const& cm, float* part, hipStream_t s);
 hipError_t launch_pd_mixed(const TrajView& t, const float* trajf, const double* obs_soa, const double* nrm_soa, int ns,
                            int directional, int chunk, double* part, hipStream_t s);
 // Single-precision copies used by the mixed kernels: trajf[k*n + i] for
@@ -112,6 +115,9 @@ hipError_t launch_spectrum_fold(const double* part, int nchunk, int ne, const do
 
 // ---- reductions over trajectory chunks (fixed order, deterministic) --------
 hipError_t launch_flux_reduce(const double* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
+                              hipStream_t s);
+// The same over FP32 partials (launch_flux_mixed), added in FP64.
+hipError_t launch_flux_reduce(const float* part, int nchunk, int ns, const EnergySet& es, double c2, double* out,
                               hipStream_t s);
 // Time-domain electric field: e[3*i + k], arrival time tm[i] for every trajectory point.
 hipError_t launch_efield_time(const TrajView& t, const double obs[3], double c0, double* e, double* tm, hipStream_t s);

@@ -65,7 +65,7 @@ template <int NE, int TILE_>
 __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const float* __restrict__ tf,
                                                              const int* __restrict__ flags,
                                                              const double* __restrict__ obs, int ns, EnergySet es,
-                                                             ChunkMap cm, double* __restrict__ part) {
+                                                             ChunkMap cm, float* __restrict__ part) {
   __shared__ MixedTile<TILE_, NE> T;
   __shared__ double red[4 * (kBlock / 64)];
   int const tid = threadIdx.x;
@@ -105,8 +105,8 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
         size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          part[b + (2 * k) * (size_t)ns] = d.re[e][k];
-          part[b + (2 * k + 1) * (size_t)ns] = d.im[e][k];
+          part[b + (2 * k) * (size_t)ns] = (float)d.re[e][k];
+          part[b + (2 * k + 1) * (size_t)ns] = (float)d.im[e][k];
         }
       }
     }
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
     for (int e = 0; e < NE; ++e) {
       size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = a.value(e, q);
+      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = (float)a.value(e, q);
     }
   }
 }
@@ -169,7 +169,7 @@ template <int TILE_, int NT>
 __global__ __launch_bounds__(NT) void flux_mixed2_kernel(TrajView t, const float* __restrict__ tf,
                                                               const int* __restrict__ flags,
                                                               const double* __restrict__ obs, int ns, EnergySet es,
-                                                              ChunkMap cm, double* __restrict__ part) {
+                                                              ChunkMap cm, float* __restrict__ part) {
   __shared__ MixedTile<TILE_, 1> T;
   __shared__ double red[4 * (NT / 64)];
   int const tid = threadIdx.x;
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(NT) void flux_mixed2_kernel(TrajView t, const float
         size_t const b = ((size_t)blockIdx.y * 6) * ns + j;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          part[b + (2 * k) * (size_t)ns] = d.re[0][k];
-          part[b + (2 * k + 1) * (size_t)ns] = d.im[0][k];
+          part[b + (2 * k) * (size_t)ns] = (float)d.re[0][k];
+          part[b + (2 * k + 1) * (size_t)ns] = (float)d.im[0][k];
         }
       }
     }
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(NT) void flux_mixed2_kernel(TrajView t, const float
     if (jp[p] < ns) {
       size_t const b = ((size_t)blockIdx.y * 6) * ns + jp[p];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = a[p].value(0, q);
+      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = (float)a[p].value(0, q);
     }
 }
 
@@ -414,7 +414,7 @@ hipError_t launch_traj_to_f32(const TrajView& t, int with_a, float* trajf, hipSt
 }
 
 hipError_t launch_flux_mixed(const TrajView& t, const float* trajf, const double* obs_soa, int ns, const EnergySet& es,
-                             ChunkMap const& cm, double* part, hipStream_t s) {
+                             ChunkMap const& cm, float* part, hipStream_t s) {
   if (cm.c1 < 1 || cm.c2 < 1 || cm.nch < 1 || cm.end(cm.nch - 1, t.n) != t.n) return hipErrorInvalidValue;
   dim3 grid((ns + kBlock - 1) / kBlock, cm.nch);
   const int* const fl = reinterpret_cast<const int*>(trajf + 3 * t.n);  // staged without accelerations
