@@ -131,8 +131,10 @@ struct ItemRanges {
   }
 };
 
+// Three energies per item hold 3 waves per SIMD (at most 168 VGPRs), as they need.
 template <int NE, int TILE_, bool MIXED>
-__global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NE == 3 ? 3 : 1))) void batch_flux_kernel(
+    BatchTraj bt, const BatchParticle* __restrict__ P,
                                                             const double* __restrict__ obs, int ns, BatchEnergies be,
                                                             unsigned nitems, unsigned* __restrict__ counter,
                                                             double* __restrict__ flux_pp, FluxSeg sg) {
@@ -171,7 +173,10 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
     int const j = j0 + tid;
     int const jj = j < ns ? j : j0;  // idle lanes repeat the tile's first point
     double const ox = obs[jj], oy = obs[ns + jj], oz = obs[2 * ns + jj];
-    // this segment's six field sums S(e, q6) (+ the earlier segments', in segment order)
+    // this segment's six field sums S(e, q6) (+ the earlier segments', in segment order);
+    // turn: paraxial sums, which take the phase omega Z/c of their plane z = turn_z
+    bool turn = false;
+    double turn_z = 0.0;
     auto finish = [&](auto sum_of) {
       if (j >= ns) return;
 #pragma unroll
@@ -181,6 +186,16 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
         double S[6];
 #pragma unroll
         for (int q6 = 0; q6 < 6; ++q6) S[q6] = sum_of(e, q6);
+        if (turn) {
+          double tc, ts;
+          para_plane_turn(es.omega[e], turn_z, &tc, &ts);
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax) {
+            double const re = S[2 * ax], im = S[2 * ax + 1];
+            S[2 * ax] = __builtin_fma(re, tc, -(im * ts));
+            S[2 * ax + 1] = __builtin_fma(re, ts, im * tc);
+          }
+        }
         size_t const b = ((size_t)p * be.ne + ee) * 6 * ns + j;
         if (!sg.first) {
 #pragma unroll
@@ -280,8 +295,13 @@ __global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const 
         }
       }
     }
-    if constexpr (MIXED) finish([&](int e, int q6) { return a32.value(e, q6); });
-    else finish([&](int e, int q6) { return (q6 & 1) ? a64.im[e][q6 >> 1] : a64.re[e][q6 >> 1]; });
+    if constexpr (MIXED) {
+      turn = para;  // paraxial sums take the plane's phase omega Z/c
+      turn_z = pb.z;
+      finish([&](int e, int q6) { return a32.value(e, q6); });
+    } else {
+      finish([&](int e, int q6) { return (q6 & 1) ? a64.im[e][q6 >> 1] : a64.re[e][q6 >> 1]; });
+    }
   }
 }
 
@@ -384,8 +404,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             flux_terms_para2<TILE_>(T.para, cntp, ox, oy, cwf, a);
           }
         }
-        finish(jp[0], [&](int q6) { return a[0].value(0, q6); });
-        finish(jp[1], [&](int q6) { return a[1].value(0, q6); });
+        double tc, ts;  // paraxial sums take the plane's phase omega Z/c
+        para_plane_turn(es.omega[0], pb.z, &tc, &ts);
+        finish(jp[0], [&](int q6) { return turned([&](int k) { return a[0].value(0, k); }, q6, tc, ts); });
+        finish(jp[1], [&](int q6) { return turned([&](int k) { return a[1].value(0, k); }, q6, tc, ts); });
         continue;
       }
     }

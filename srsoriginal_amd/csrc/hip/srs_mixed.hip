@@ -153,9 +153,12 @@ __global__ __launch_bounds__(kBlock) void flux_mixed_kernel(TrajView t, const fl
   if (j < ns) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
+      double tc = 1.0, ts = 0.0;  // paraxial sums take the plane's phase omega Z/c
+      if (para) para_plane_turn(es.omega[e], pb.z, &tc, &ts);
       size_t const b = ((size_t)(blockIdx.y * NE + e) * 6) * ns + j;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = (float)a.value(e, q);
+      for (int q = 0; q < 6; ++q)
+        part[b + (size_t)q * ns] = (float)turned([&](int k) { return a.value(e, k); }, q, tc, ts);
     }
   }
 }
@@ -266,12 +269,15 @@ __global__ __launch_bounds__(NT) void flux_mixed2_kernel(TrajView t, const float
         flux_terms_mixed<1, TILE_>(T.raw, cnt, ox[p], oy[p], oz[p], es, &cwf, a[p]);
     }
   }
+  double tc = 1.0, ts = 0.0;  // paraxial sums take the plane's phase omega Z/c
+  if (para) para_plane_turn(es.omega[0], pb.z, &tc, &ts);
 #pragma unroll
   for (int p = 0; p < 2; ++p)
     if (jp[p] < ns) {
       size_t const b = ((size_t)blockIdx.y * 6) * ns + jp[p];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) part[b + (size_t)q * ns] = (float)a[p].value(0, q);
+      for (int q = 0; q < 6; ++q)
+        part[b + (size_t)q * ns] = (float)turned([&](int k) { return a[p].value(0, k); }, q, tc, ts);
     }
 }
 

@@ -177,6 +177,34 @@ constexpr double kParaPhase = 1e-9;
 // (beta_z - n_z)/D is itself ~1e-8 of the x, y one).
 constexpr double kParaNear = 1e-9;
 
+// The phase the paraxial terms leave out.  Their phase is omega tau - omega Z/c for the
+// plane z = Z: omega Z/c is the same for every term of one observation point, so |E|^2
+// does not see it, but the sums that the other forms give for the same point (other
+// trajectory chunks or time segments, outside the paraxial bounds) carry it.  Every
+// paraxial sum is therefore turned by e^{i omega Z/c} before it is stored or combined:
+// psi = omega Z / (2 pi c) in revolutions, reduced to [-1/2, 1/2] in double-double
+// arithmetic (omega Z as an exact pair, 1/(2 pi c) as a pair); its cosine and sine by the
+// hardware (revolutions, ~1e-7 absolute: the sums' own FP32 level), scaled in FP64 to
+// unit modulus so that |E|^2 is unchanged.  *c, *s = cos, sin of 2 pi psi.
+constexpr double kInv2PiCHi = 0x1.23db3eb8a3fa7p-31, kInv2PiCLo = -0x1.79488f78d104bp-86;  // 1/(2 pi c)
+__device__ __forceinline__ void para_plane_turn(double omega, double Z, double* c, double* s) {
+  double const ah = omega * Z, al = __builtin_fma(omega, Z, -ah);
+  double const yh = ah * kInv2PiCHi;
+  double const yl = __builtin_fma(ah, kInv2PiCHi, -yh) + (ah * kInv2PiCLo + al * kInv2PiCHi);
+  float const psi = (float)((yh - __builtin_rint(yh)) + yl);
+  double const cf = (double)__builtin_amdgcn_cosf(psi), sf = (double)__builtin_amdgcn_sinf(psi);
+  double const r = 1.0 / __builtin_sqrt(__builtin_fma(cf, cf, sf * sf));
+  *c = cf * r;
+  *s = sf * r;
+}
+
+// Component q6 (re_x, im_x, re_y, im_y, re_z, im_z) of the sums v(q6) turned by (c, s).
+template <class V>
+__device__ __forceinline__ double turned(V const& v, int q6, double c, double s) {
+  double const re = v(q6 & ~1), im = v(q6 | 1);
+  return (q6 & 1) ? __builtin_fma(re, s, im * c) : __builtin_fma(re, c, -(im * s));
+}
+
 // One staged trajectory point of the paraxial form (one LDS record, so a term
 // reads it from a single base address).
 //

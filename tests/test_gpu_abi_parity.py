@@ -268,7 +268,10 @@ def test_mixed_flux_paraxial_and_general_paths(abi, oracle):
     expansion bounds hold, and the plain mixed terms otherwise; both stay within 1e-5
     of the FP64 form.  Cases: a tilted screen (no common plane), a close screen
     (eps ~ 1e-4 fails the bound), a 300-point list whose first block mixes two planes
-    and whose second block is flat, and a flat screen just inside the eps bound."""
+    and whose second block is flat, a flat screen just inside the eps bound, and a
+    42 mm screen at 30 m whose corner blocks pass the eps bound for trajectory chunks
+    far from the screen (rz ~ 31.5 m) and fail it for near ones (rz ~ 28.5 m), so their
+    chunk sums come from both forms and must share one phase (the plane's omega Z/c)."""
     F, p, t, dt = cases.oracle_setup(oracle, "traj_c0_20001")
     es = [143.52176088044024, 431.2]
     tilted, _, _ = oracle.rect_surface("XY", 17, 15, 0.012, 0.01, (0.3, 0.1, 0), (0, 0, 30), 0)
@@ -278,7 +281,9 @@ def test_mixed_flux_paraxial_and_general_paths(abi, oracle):
                            np.where(np.arange(300) % 2 == 0, 30.0, 30.5)])
     two[256:, 2] = 30.0
     edge, _, _ = oracle.rect_surface("XY", 23, 23, 0.023, 0.023, (0, 0, 0), (0, 0, 20.0), 0)
-    for name, xyz in (("tilted", tilted), ("close", close), ("two planes", two), ("edge of bound", edge)):
+    cross, _, _ = oracle.rect_surface("XY", 41, 41, 0.042, 0.042, (0, 0, 0), (0, 0, 30.0), 0)
+    for name, xyz in (("tilted", tilted), ("close", close), ("two planes", two), ("edge of bound", edge),
+                      ("bound crossed", cross)):
         f64 = abi.flux(t, dt, xyz, es, p.q, p.current, abi.FP64)
         mix = abi.flux(t, dt, xyz, es, p.q, p.current, abi.MIXED)
         for k, e in enumerate(es):

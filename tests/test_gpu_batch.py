@@ -288,6 +288,33 @@ def test_batch_routes_use_each_particles_charge_and_current(abi, oracle):
     assert all(r["current"] == p.q for r in abi.sample_beam(bp, 2, 5, ct_start=0.0, ct_stop=3.0, npoints=301))
 
 
+def test_batch_mixed_segments_across_the_paraxial_bound(abi, oracle):
+    """20001-point trajectories (integrated in time segments) on a 42 mm screen at 30 m:
+    the corner tiles pass the paraxial eps bound in the segments far from the screen and
+    fail it in the near ones, so one point's field sums come from both forms and must
+    share one phase.  Mixed vs FP64 batch, one and three energies, the one-energy shapes
+    with one and two points per lane."""
+    name = "multiflux_c3_small"
+    say("batched flux across the paraxial bound: mixed vs fp64 (<=1e-5)")
+    rows, p = _c3_particles(oracle, name, 3, npoints=20001)
+    parts = abi.particles(rows)
+    xyz, nrm, x12 = oracle.rect_surface("XY", 41, 41, 0.042, 0.042, (0, 0, 0), (0, 0, 30), 0)
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    w = 1.0 / len(rows)
+    try:
+        for es, shapes in (([143.52176088044024], [(1, 128, 1), (1, 256, 2)]), ([430.0, 431.2, 432.4], [(3, 128, 1)])):
+            abi.batch_set_config(shapes[0][0], 128, 1)
+            f64 = np.atleast_2d(abi.batch_flux(und, parts, p.q, p.current, xyz, es, w, abi.FP64, 0))
+            for sh in shapes:
+                abi.batch_set_config(*sh)
+                m = np.atleast_2d(abi.batch_flux(und, parts, p.q, p.current, xyz, es, w, abi.MIXED, 0))
+                errs = [normwise(m[k], f64[k]) for k in range(len(es))]
+                say("  %s shape %s: mixed vs fp64 normwise %s" % (es, sh, " ".join("%.2e" % e for e in errs)))
+                assert max(errs) <= 1e-5 and max(rms_rel(m[k], f64[k]) for k in range(len(es))) <= 1e-5
+    finally:
+        abi.batch_set_config(0)
+
+
 def test_batch_every_tile_shape_matches_fp64(abi, oracle):
     """Each tile shape of the persistent kernel, fixed with srs_hip_batch_set_config (two
     observation points per lane included, on an odd point count so the last 512-point
