@@ -435,12 +435,34 @@ def gradient_c4(nreal=256, reps=2):
     f_ms, b_ms = min(fwd), min(bwd)
     terms = float(nreal) * len(xyz) * 20001
     log("gradient C4: forward %.1f ms, backward %.1f ms" % (f_ms, b_ms))
+    # the same in the mixed form (mixed flux terms forward, mixed adjoint), against FP64
+    prob.precision = "mixed"
+    lmx, _, tape = prob.forward(deltas)
+    prob.backward(tape, nreal)
+    mfwd, mbwd = [], []
+    for _ in range(reps):
+        abi.timing_enable(True)
+        lmx, _, tape = prob.forward(deltas)
+        mf, _ = abi.timing_read()
+        gmx = prob.backward(tape, nreal)
+        mb, _ = abi.timing_read()
+        abi.timing_enable(False)
+        mfwd.append(mf)
+        mbwd.append(mb)
+    mf, mb = min(mfwd), min(mbwd)
+    gerr = np.max(np.abs(gmx - g), axis=1) / np.max(np.abs(g), axis=1)
+    log("gradient C4 mixed: forward %.1f ms, backward %.1f ms, gradient vs FP64 max normwise %.2e"
+        % (mf, mb, float(np.max(gerr))))
     return {"workload": "%d realisations x 82 poles, 51x51 over 5 mm at 30 m, E=%.2f eV, NT=20001, checkpoints "
                         "every 1000 steps" % (nreal, ENERGY_EV),
             "forward_ms": f_ms, "backward_ms": b_ms, "bwd_over_fwd": b_ms / f_ms,
             "gradient_over_forward": (f_ms + b_ms) / f_ms,
             "forward_terms_per_s": terms / (f_ms * 1e-3), "backward_terms_per_s": terms / (b_ms * 1e-3),
             "loss_mean": float(np.mean(loss)), "grad_rms": float(np.sqrt(np.mean(g ** 2))),
+            "mixed": {"forward_ms": mf, "backward_ms": mb, "bwd_over_fwd": mb / mf,
+                      "gradient_over_forward": (mf + mb) / mf, "gradient_speedup_vs_fp64": (f_ms + b_ms) / (mf + mb),
+                      "gradient_vs_fp64_max_normwise": float(np.max(gerr)),
+                      "loss_vs_fp64_max_rel": float(np.max(np.abs(lmx - loss) / np.abs(loss)))},
             "timed": "sum of kernel durations (HIP events): forward = RK4 + flux, backward = flux adjoint + RK4 adjoint"}
 
 

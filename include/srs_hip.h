@@ -330,6 +330,15 @@ int srs_hip_pole_forward(const srs_field* und, int nund, int poled, const srs_pa
                          const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
                          double energy_ev, const double* target, int checkpoint_every, double* flux_out,
                          double* loss_out, int64_t* tape_out);
+/* srs_hip_pole_forward with a precision: SRS_PRECISION_FP64 (as srs_hip_pole_forward) or
+ * SRS_PRECISION_MIXED, where the forward flux takes the mixed terms (FP64 phase, FP32
+ * trig, amplitudes and tile sums) and srs_hip_pole_backward the mixed adjoint (the same
+ * FP64 phase; the amplitudes, their derivatives and the products in FP32, sums in
+ * FP64).  No counterpart in the reference (its flux has no gradient). */
+int srs_hip_pole_forward_ex(const srs_field* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+                            const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
+                            double energy_ev, const double* target, int checkpoint_every, int precision,
+                            double* flux_out, double* loss_out, int64_t* tape_out);
 /* The same for the power density (OSCARSSR.cc:1477-1598: W/mm^2 on ns points with
  * surface normals, AoS xyz; directional as in srs_hip_power_density):
  *   L_r = sum_j (P_rj - target_j)^2,  P_rj = power density of realisation r at point j;

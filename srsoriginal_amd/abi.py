@@ -220,13 +220,15 @@ def _bind_batch():
     L.srs_hip_batch_set_config.argtypes = [C.c_int, C.c_int, C.c_int]
     L.srs_hip_pole_forward.argtypes = [C.POINTER(SrsField), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P, _D,
                                        _D, _P, _I64, _D, _P, C.c_int, _P, _P, C.POINTER(_I64)]
+    L.srs_hip_pole_forward_ex.argtypes = [C.POINTER(SrsField), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P,
+                                          _D, _D, _P, _I64, _D, _P, C.c_int, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_pd_forward.argtypes = [C.POINTER(SrsField), C.c_int, C.c_int, C.POINTER(SrsParticle), _I64, _P,
                                           _D, _D, _P, _P, _I64, C.c_int, _P, C.c_int, _P, _P, C.POINTER(_I64)]
     L.srs_hip_pole_backward.argtypes = [_I64, _P, _P]
     for n in ("srs_hip_trajectory_points", "srs_hip_trajectories", "srs_hip_batch_flux", "srs_hip_batch_config",
               "srs_hip_batch_points_per_lane", "srs_hip_batch_set_config",
               "srs_hip_batch_power_density", "srs_hip_batch_spectrum",
-              "srs_hip_pole_forward", "srs_hip_pole_pd_forward", "srs_hip_pole_backward"):
+              "srs_hip_pole_forward", "srs_hip_pole_forward_ex", "srs_hip_pole_pd_forward", "srs_hip_pole_backward"):
         getattr(L, n).restype = C.c_int
     L._batch_bound = True
     return L
@@ -417,8 +419,9 @@ def batch_set_config(ne_per_item=0, tile=0, points_per_lane=1):
 
 
 def pole_forward(und_specs, poled, particle, deltas, charge, current, obs_xyz, energy_ev, target, checkpoint_every=1000,
-                 want_flux=True):
-    """Forward of the pole-error flux loss for nreal realisations (deltas [nreal, 2N]).
+                 want_flux=True, precision=FP64):
+    """Forward of the pole-error flux loss for nreal realisations (deltas [nreal, 2N]),
+    in FP64 or in the mixed form (MIXED: mixed flux terms forward and mixed adjoint).
     Returns (loss [nreal], flux [nreal, ns] or None, tape id)."""
     L = _bind_batch()
     keep = []
@@ -432,10 +435,11 @@ def pole_forward(und_specs, poled, particle, deltas, charge, current, obs_xyz, e
     loss = np.zeros(nreal)
     flux = np.zeros((nreal, len(o))) if want_flux else None
     tape = _I64()
-    _check(L.srs_hip_pole_forward(U, len(U), int(poled), C.byref(particle), nreal, d.ctypes.data_as(_P), charge, current,
-                                  o.ctypes.data_as(_P), len(o), float(energy_ev), t.ctypes.data_as(_P),
-                                  int(checkpoint_every), flux.ctypes.data_as(_P) if want_flux else None,
-                                  loss.ctypes.data_as(_P), C.byref(tape)))
+    _check(L.srs_hip_pole_forward_ex(U, len(U), int(poled), C.byref(particle), nreal, d.ctypes.data_as(_P), charge,
+                                     current, o.ctypes.data_as(_P), len(o), float(energy_ev), t.ctypes.data_as(_P),
+                                     int(checkpoint_every), int(precision),
+                                     flux.ctypes.data_as(_P) if want_flux else None, loss.ctypes.data_as(_P),
+                                     C.byref(tape)))
     return loss, flux, tape.value
 
 

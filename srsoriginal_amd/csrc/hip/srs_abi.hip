@@ -88,6 +88,7 @@ struct Context {
   Buf b_beams;           // beam records of the device-sampled batch
   bool und_local = false;  // the staged field list holds elements other than magnetic undulators
   int und_force = srs::kForceB;  // the derivative form: magnetic, electric or both kinds of elements
+  int pole_count = 0;            // pole deltas in each particle's table in b_poles (0: none)
   hipGraph_t b_graph = nullptr;
   hipGraphExec_t b_exec = nullptr;
   std::string b_key;
@@ -111,6 +112,7 @@ struct Context {
     srs::Checkpoints ck{};
     double omega = 0, cw = 0, c0w = 0, c2 = 0;  // flux
     int quantity = 0;                             // 0 flux, 1 power density
+    bool mixed = false;                           // flux: mixed-precision forward and adjoint
     int directional = 0;
     double pd_scale = 0;
   } tape;
@@ -594,6 +596,7 @@ int stage_undulators(Context& c, const srs_field* und, int nund, hipStream_t s) 
       poles.insert(poles.end(), h.pole_delta, h.pole_delta + 2 * h.nperiods);
     }
   }
+  c.pole_count = (int)poles.size();
   if (poles.empty()) poles.push_back(0.0);
   c.und_local = false;
   bool has_b = false, has_e = false;
@@ -672,7 +675,7 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
   }
   srs::RK4Seg rs{0, L, nseg > 1 ? c.b_state.as<double>() : nullptr};
   if (nseg == 1) {
-    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), P, np_run, r.bt,
+    HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), c.pole_count, P, np_run, r.bt,
                                  r.ck, rs, s));
     HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
     srs::FluxSeg const fs{0, L, 1, 1, field != nullptr, field};
@@ -684,7 +687,7 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
     for (int k = 0; k < nseg; ++k) {
       rs.lo = k * L;
       rs.hi = (k + 1) * L;
-      HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), P, np_run, r.bt,
+      HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), c.pole_count, P, np_run, r.bt,
                                    r.ck, rs, c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
     }
@@ -706,11 +709,13 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
   char buf[768];
   // every value a captured launch holds by value: shapes, tile shape, the field list's
   // length and whether it needs the general field evaluation (the RK4 instantiation
-  // depends on it), the CU count behind the persistent grid size, the flux prefactor c2
-  // and the weight, and the address of every buffer a launch reads or writes
+  // depends on it), the pole table length (the RK4's LDS copy of it), the CU count
+  // behind the persistent grid size, the flux prefactor c2 and the weight, and the
+  // address of every buffer a launch reads or writes
   snprintf(buf, sizeof buf,
-           "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns,
-           r.ne, (int)r.mixed, cfg.ne_per_item, cfg.tile, cfg.ppl, r.nund, (int)c.und_local + 2 * c.und_force, c.cus, (long long)r.max_leg,
+           "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%lld/%.17g/%.17g/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p/%p", r.np, r.ns,
+           r.ne, (int)r.mixed, cfg.ne_per_item, cfg.tile, cfg.ppl, r.nund, (int)c.und_local + 2 * c.und_force,
+           c.pole_count, c.cus, (long long)r.max_leg,
            r.weight, r.be.c2, c.b_traj.p, c.b_trajf.p, c.b_flux_pp.p, c.b_out.p, c.b_obs.p, c.b_energies.p,
            c.b_state.p, c.b_field.p, c.b_poles.p, c.b_part.p, c.b_counter.p, c.b_und.p,
            (void*)r.bt.bz, (void*)r.bt.tflag);  // row offsets within the packed buffers follow the total point count
@@ -734,14 +739,14 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
   auto* U = c.b_und.as<srs::DevUndulator>();
   double* const poles = c.b_poles.as<double>();
   if (nseg == 1) {
-    HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
+    HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, c.pole_count, Pd, np, bt, ck, srs::RK4Seg{0, INT64_MAX, nullptr}, s));
     HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, 0, INT64_MAX, nch, 0,
                                 c.g_part.as<double>(), mixed, s));
   } else {
     HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
     HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
     for (int k = 0; k < nseg; ++k) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, c.pole_count, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
                                    c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
     }
@@ -820,6 +825,7 @@ int pole_stage(Context& c, const srs_field* und, int nund, int poled, const srs_
     TRY(stage_undulators(c, uu.data(), nund, s));
   }
   size_t const nd = (size_t)nreal * pr.npoles;
+  c.pole_count = pr.npoles;  // each realisation's own table (pole_base = r * npoles)
   HIPCHK(c.b_poles.ensure(nd * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c.b_poles.p, deltas, nd * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(c.b_part.ensure(nreal * sizeof(srs::BatchParticle)));
@@ -1349,7 +1355,7 @@ int srs_hip_trajectories(const srs_field* und, int nund, const srs_particle* p, 
   srs::BatchTraj bt;
   TRY(stage_batch_traj(c, tot, np, true, &bt));
   HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), nund, c.und_local, c.und_force, c.b_poles.as<double>(),
-                               c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0},
+                               c.pole_count, c.b_part.as<srs::BatchParticle>(), (int)np, bt, srs::Checkpoints{nullptr, 1, 0},
                                srs::RK4Seg{0, INT64_MAX, nullptr}, s));
   double* src[9] = {bt.x, bt.y, bt.z, bt.bx, bt.by, bt.bz, bt.ax, bt.ay, bt.az};
   for (int k = 0; k < 9; ++k)
@@ -1541,7 +1547,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   {
     KernelTimer timer(c, s);
     if (nseg == 1) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), c.pole_count, Pd, (int)np, bt, ck,
                                    srs::RK4Seg{0, INT64_MAX, nullptr}, s));
       HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, tab, (int)ne, run, 0, INT64_MAX, nch, 0,
                                         c.g_part.as<double>(), c.spec_range, mixed, s));
@@ -1549,7 +1555,7 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
       HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
       HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
       for (int k = 0; k < nseg; ++k) {
-        HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), Pd, (int)np, bt, ck,
+        HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), c.pole_count, Pd, (int)np, bt, ck,
                                      srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()}, c.stream2));
         HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
       }
@@ -1623,10 +1629,20 @@ int srs_hip_pole_forward(const srs_field* und, int nund, int poled, const srs_pa
                          const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
                          double energy_ev, const double* target, int checkpoint_every, double* flux_out,
                          double* loss_out, int64_t* tape_out) {
+  return srs_hip_pole_forward_ex(und, nund, poled, particle, nreal, deltas, charge, current, obs_xyz, ns, energy_ev,
+                                 target, checkpoint_every, SRS_PRECISION_FP64, flux_out, loss_out, tape_out);
+}
+
+int srs_hip_pole_forward_ex(const srs_field* und, int nund, int poled, const srs_particle* particle, int64_t nreal,
+                            const double* deltas, double charge, double current, const double* obs_xyz, int64_t ns,
+                            double energy_ev, const double* target, int checkpoint_every, int precision,
+                            double* flux_out, double* loss_out, int64_t* tape_out) {
   g_err.clear();
   TRY(pole_validate(und, nund, poled, particle, nreal, deltas, obs_xyz, ns, target, checkpoint_every, loss_out,
                     tape_out));
   if (!(energy_ev > 0)) return fail(SRS_ERR_INVALID, "photon energy must be > 0");
+  if (precision != SRS_PRECISION_FP64 && precision != SRS_PRECISION_MIXED)
+    return fail(SRS_ERR_INVALID, "precision must be SRS_PRECISION_FP64 or SRS_PRECISION_MIXED");
   particle_charge(*particle, charge, current, &charge, &current);
   Context& c = ctx();
   std::lock_guard<std::mutex> lock(c.mu);
@@ -1651,7 +1667,7 @@ int srs_hip_pole_forward(const srs_field* und, int nund, int poled, const srs_pa
     r.np = (int)nreal;
     r.ns = (int)ns;
     r.ne = 1;
-    r.mixed = false;
+    r.mixed = precision == SRS_PRECISION_MIXED;
     r.weight = 1.0;
     r.max_leg = std::max(pr.P[0].nf, pr.P[0].nb);
     r.ck = pr.ck;
@@ -1662,6 +1678,7 @@ int srs_hip_pole_forward(const srs_field* und, int nund, int poled, const srs_pa
   TRY(pole_finish(c, pr, (int)nreal, (int)ns, flux_out, loss_out, s));
   Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);
   t.quantity = 0;
+  t.mixed = precision == SRS_PRECISION_MIXED;
   t.omega = w;
   t.cw = kC / w;
   t.c0w = c0 * w;
@@ -1698,6 +1715,7 @@ int srs_hip_pole_pd_forward(const srs_field* und, int nund, int poled, const srs
   TRY(pole_finish(c, pr, (int)nreal, (int)ns, pd_out, loss_out, s));
   Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);
   t.quantity = 1;
+  t.mixed = false;
   t.directional = directional ? 1 : 0;
   t.pd_scale = scale;
   *tape_out = t.id;
@@ -1740,7 +1758,7 @@ int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out) {
     else
       HIPCHK(srs::launch_flux_adjoint(t.bt, Pd, t.np, t.nmax, c.b_obs.as<double>(), t.ns, t.omega, t.cw, t.c0w, t.c2,
                                       c.g_field.as<double>(), c.b_flux_pp.as<double>(), c.g_target.as<double>(),
-                                      c.g_dloss.as<double>(), gy, t.total, s));
+                                      c.g_dloss.as<double>(), gy, t.total, t.mixed, s));
   }
   HIPCHK(srs::launch_zero(c.g_grad.as<double>(), (int64_t)t.np * t.npoles, s));
   {

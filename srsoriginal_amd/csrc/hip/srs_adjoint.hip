@@ -75,21 +75,21 @@ __global__ __launch_bounds__(256) void loss_kernel(const double* __restrict__ fl
   if (tid == 0) loss[r] = red[0];
 }
 
-__global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
-                                                              const double* __restrict__ obs, int ns, double omega,
-                                                              double cw, double c0w, double c2,
-                                                              const double* __restrict__ field_pp,
-                                                              const double* __restrict__ flux_pp,
-                                                              const double* __restrict__ target,
-                                                              const double* __restrict__ dloss,
-                                                              double* __restrict__ gy, int64_t total) {
+// The FP64 terms of one block of flux_adjoint_kernel (also the fallback of the mixed
+// kernel for a realisation whose beam is not along z).
+__device__ __forceinline__ void flux_adjoint_fp64_block(BatchTraj const& bt, BatchParticle const& q, int r,
+                                                        const double* __restrict__ obs, int ns, double omega,
+                                                        double cw, double c0w, double c2,
+                                                        const double* __restrict__ field_pp,
+                                                        const double* __restrict__ flux_pp,
+                                                        const double* __restrict__ target,
+                                                        const double* __restrict__ dloss, double* __restrict__ gy,
+                                                        int64_t total) {
   __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile];
   __shared__ double sgr[3][kAdjTile], sgi[3][kAdjTile];
-  int const r = blockIdx.y, tid = threadIdx.x;
-  BatchParticle const q = P[r];
+  int const tid = threadIdx.x;
   int64_t const n = q.nb + q.nf;
   int64_t const k = (int64_t)blockIdx.x * kBlock + tid;
-  if ((int64_t)blockIdx.x * kBlock >= n) return;  // whole block past this realisation's points
   bool const valid = k < n;
   int64_t const kk = valid ? k : n - 1;
   int64_t const i = q.off + kk;
@@ -144,6 +144,147 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, cons
         double const t1 = -inv2 * __builtin_fma(nn[d], S1, hr[d]);
         double const t2 = cw3 * __builtin_fma(-3.0 * nn[d], S2, hi[d]);
         gR[d] += t1 + t2 - woc * nn[d] * S3;
+      }
+    }
+  }
+  if (valid) {
+    gy[0 * total + i] = -gR[0];
+    gy[1 * total + i] = gb[0] * kInvC;
+    gy[2 * total + i] = -gR[1];
+    gy[3 * total + i] = gb[1] * kInvC;
+    gy[4 * total + i] = -gR[2];
+    gy[5 * total + i] = gb[2] * kInvC;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void flux_adjoint_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                              const double* __restrict__ obs, int ns, double omega,
+                                                              double cw, double c0w, double c2,
+                                                              const double* __restrict__ field_pp,
+                                                              const double* __restrict__ flux_pp,
+                                                              const double* __restrict__ target,
+                                                              const double* __restrict__ dloss,
+                                                              double* __restrict__ gy, int64_t total) {
+  int const r = blockIdx.y;
+  BatchParticle const q = P[r];
+  if ((int64_t)blockIdx.x * kBlock >= q.nb + q.nf) return;  // whole block past this realisation's points
+  flux_adjoint_fp64_block(bt, q, r, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target, dloss, gy, total);
+}
+
+// Mixed form of flux_adjoint_kernel: the phase as the mixed flux terms take it (D, tau
+// and omega tau in FP64, bitwise as the reference; the reduction to [-pi, pi] in FP64,
+// sin/cos in FP32), everything after it in FP32: n, the amplitudes a_c with
+// beta_z - n_z = (1 - n_z) - (1 - beta_z) (1 - beta_z staged by the RK4 in FP32 from
+// FP64), H_c, S1..S3 and the term's dL/dR, dL/dbeta, summed over 32 observation points
+// in FP32 and added into FP64 per lane.  The seeds G_c of a tile of 256 points are
+// carried as G_c 2^-e (e: the exponent of the tile's largest |G|) so that the FP32
+// products stay in range; 2^e is restored in FP64 with each sub-sum.  A realisation
+// whose beam is not along z takes the FP64 terms.
+__global__ __launch_bounds__(kBlock) void flux_adjoint_mixed_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
+                                                                    const double* __restrict__ obs, int ns,
+                                                                    double omega, double cw, double c0w, double c2,
+                                                                    const double* __restrict__ field_pp,
+                                                                    const double* __restrict__ flux_pp,
+                                                                    const double* __restrict__ target,
+                                                                    const double* __restrict__ dloss,
+                                                                    double* __restrict__ gy, int64_t total) {
+  __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile];
+  __shared__ float sgr[3][kAdjTile], sgi[3][kAdjTile];
+  __shared__ double smax[kBlock / 64];
+  static_assert(kAdjTile == kBlock, "one observation point of a tile per thread");
+  int const r = blockIdx.y, tid = threadIdx.x;
+  BatchParticle const q = P[r];
+  int64_t const n = q.nb + q.nf;
+  if ((int64_t)blockIdx.x * kBlock >= n) return;  // whole block past this realisation's points
+  if (bt.tflag && bt.tflag[r]) {
+    flux_adjoint_fp64_block(bt, q, r, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target, dloss, gy, total);
+    return;
+  }
+  int64_t const k = (int64_t)blockIdx.x * kBlock + tid;
+  bool const valid = k < n;
+  int64_t const kk = valid ? k : n - 1;
+  int64_t const i = q.off + kk;
+  double const X = bt.x[i], Y = bt.y[i], Z = bt.z[i];
+  float const fbx = bt.fbx[i], fby = bt.fby[i], fomb = bt.fomb[i];  // beta_x, beta_y, 1 - beta_z
+  double const t0 = q.dt * (double)kk;
+  double const kw = c0w * q.dt;
+  double const gsc = dloss[r] * 4.0 * c2 * kw * kw;
+  float const woc = (float)(omega * kInvC), cwf = (float)cw;
+  double gR[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+  for (int j0 = 0; j0 < ns; j0 += kAdjTile) {
+    int const cnt = (ns - j0) < kAdjTile ? ns - j0 : kAdjTile;
+    __syncthreads();
+    // this thread's seeds (one point of the tile), their largest magnitude over the block
+    double g[6] = {0, 0, 0, 0, 0, 0};
+    double m = 0.0;
+    if (tid < cnt) {
+      int const j = j0 + tid;
+      sx[tid] = obs[j];
+      sy[tid] = obs[ns + j];
+      sz[tid] = obs[2 * ns + j];
+      double const w = gsc * (flux_pp[(size_t)r * ns + j] - target[j]);
+      size_t const base = (size_t)r * 6 * ns + j;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        g[c] = w * field_pp[base + (size_t)c * ns];
+        m = __builtin_fmax(m, __builtin_fabs(g[c]));
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = __builtin_fmax(m, __shfl_xor(m, o, 64));
+    if ((tid & 63) == 0) smax[tid >> 6] = m;
+    __syncthreads();
+    double tm = smax[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) tm = __builtin_fmax(tm, smax[w]);
+    int const e = tm > 0.0 ? ilogb(tm) : 0;  // tile seeds carried as G 2^-e
+    if (tid < cnt) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sgr[c][tid] = (float)ldexp(g[2 * c], -e);
+        sgi[c][tid] = (float)ldexp(g[2 * c + 1], -e);
+      }
+    }
+    __syncthreads();
+    for (int t0s = 0; t0s < cnt; t0s += kSub) {
+      int const t1s = (t0s + kSub < cnt) ? t0s + kSub : cnt;
+      float fR[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
+      for (int t = t0s; t < t1s; ++t) {
+        double const rx = sx[t] - X, ry = sy[t] - Y, rz = sz[t] - Z;
+        double const D = dist_rn(rx * rx + ry * ry + rz * rz);
+        double const tau = t0 + div_rn(D, kC, kInvC);
+        float sn, cs;
+        sincos_reduced_f32(reduce_two_pi(omega * tau), &sn, &cs);
+        float const inv = __builtin_amdgcn_rcpf((float)D), inv2 = inv * inv;
+        float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
+        float const v = __builtin_fmaf(nx, nx, ny * ny);
+        float const omnz = nz > 0.f ? v * __builtin_amdgcn_rcpf(1.f + nz) : 1.f - nz;  // 1 - n_z
+        float const nn[3] = {nx, ny, nz};
+        float const bmn[3] = {fbx - nx, fby - ny, omnz - fomb};  // beta - n
+        float hr[3], hi[3];
+        float S1 = 0.f, S2 = 0.f, S3 = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          hr[c] = __builtin_fmaf(sgr[c][t], cs, sgi[c][t] * sn);
+          hi[c] = __builtin_fmaf(sgr[c][t], sn, -(sgi[c][t] * cs));
+          float const ar = bmn[c] * inv, ai = -(cwf * nn[c]) * inv2;
+          fb[c] = __builtin_fmaf(hr[c], inv, fb[c]);
+          S1 = __builtin_fmaf(hr[c], bmn[c] - nn[c], S1);  // beta_c - 2 n_c
+          S2 = __builtin_fmaf(hi[c], nn[c], S2);
+          S3 = __builtin_fmaf(hr[c], ai, __builtin_fmaf(hi[c], ar, S3));
+        }
+        float const cw3 = cwf * inv2 * inv;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          float const t1 = -inv2 * __builtin_fmaf(nn[d], S1, hr[d]);
+          float const t2 = cw3 * __builtin_fmaf(-3.f * nn[d], S2, hi[d]);
+          fR[d] += t1 + t2 - woc * nn[d] * S3;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        gR[c] += ldexp((double)fR[c], e);
+        gb[c] += ldexp((double)fb[c], e);
       }
     }
   }
@@ -331,18 +472,21 @@ __device__ __forceinline__ void step_adjoint(const double* __restrict__ st, doub
 // the forward pass (same arithmetic from the same checkpointed state).
 template <int NU>
 __global__ __launch_bounds__(64) void rk4_record_kernel(const DevUndulator* __restrict__ U, int nu,
-                                                         const double* __restrict__ poles,
+                                                         const double* __restrict__ poles, int npl,
                                                          const BatchParticle* __restrict__ Ps, int np, Checkpoints ck,
                                                          int backward_leg, int64_t sa, int64_t sb,
                                                          const double* __restrict__ gy,
                                                          const double* __restrict__ ga, int64_t total,
                                                          double* __restrict__ rec) {
+  extern __shared__ double spole[];  // npl x 64 (launch_rk4_adjoint)
   int64_t const lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= (sb - sa) * np) return;
   int const r = (int)(lane % np);
   int64_t const seg = sa + lane / np;
   BatchParticle const q = Ps[r];
-  RK4Ctx<NU> const R{Fields<NU>(U, nu, poles + q.pole_base), q.qmg};
+  int ps;
+  const double* const pt = stage_poles(poles + q.pole_base, spole, npl, &ps);
+  RK4Ctx<NU> const R{Fields<NU>(U, nu, pt, ps), q.qmg};
   int64_t const C = ck.every;
   int64_t const nleg = backward_leg ? q.nb : q.nf;
   int64_t const slot = backward_leg ? (q.nf + C - 1) / C + seg : seg;
@@ -483,10 +627,14 @@ hipError_t launch_loss(const double* flux_pp, const double* target, int ns, int 
 hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
                                int ns, double omega, double cw, double c0w, double c2, const double* field_pp,
                                const double* flux_pp, const double* target, const double* dloss, double* gy,
-                               int64_t total, hipStream_t s) {
+                               int64_t total, bool mixed, hipStream_t s) {
   dim3 const grid((unsigned)((nmax + kBlock - 1) / kBlock), (unsigned)np);
-  flux_adjoint_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target, dloss,
-                                              gy, total);
+  if (mixed)
+    flux_adjoint_mixed_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target,
+                                                      dloss, gy, total);
+  else
+    flux_adjoint_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target, dloss,
+                                                gy, total);
   return hipGetLastError();
 }
 
@@ -522,12 +670,14 @@ hipError_t launch_rk4_adjoint(const DevUndulator* U, int nu, int poled, const do
       int64_t const sa = sb - seg_per_pass > 0 ? sb - seg_per_pass : 0;
       int64_t const lanes = (sb - sa) * np;
       dim3 const grid((unsigned)((lanes + 63) / 64));
+      int const npl = pole_lds_count(npoles);
+      size_t const lds = (size_t)npl * kPoleLane * sizeof(double);
       if (nu == 1)
-        rk4_record_kernel<1><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, ga, total, rec);
+        rk4_record_kernel<1><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, ck, leg, sa, sb, gy, ga, total, rec);
       else if (nu == 2)
-        rk4_record_kernel<2><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, ga, total, rec);
+        rk4_record_kernel<2><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, ck, leg, sa, sb, gy, ga, total, rec);
       else
-        rk4_record_kernel<0><<<grid, 64, 0, s>>>(U, nu, poles, P, np, ck, leg, sa, sb, gy, ga, total, rec);
+        rk4_record_kernel<0><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, ck, leg, sa, sb, gy, ga, total, rec);
       e = hipGetLastError();
       if (e != hipSuccess) return e;
       e = launch_zero(del_out, (sb - sa) * kSweeps * np * npoles, s);

@@ -125,11 +125,17 @@ hipError_t launch_sample_batch(const DevBeam* beams, const double* wsum, int nb,
 
 // local: the list holds elements other than magnetic undulators (Gaussian, box, field
 // map, or any electric element); force: kForceB (magnetic elements only), kForceE
-// (electric only) or kForceEB (both), OSCARSSR::SetDerivativesFunction's choice
+// (electric only) or kForceEB (both), OSCARSSR::SetDerivativesFunction's choice;
+// npoles: length of each particle's pole-delta table at poles + pole_base (0: none),
+// staged in LDS by the kernel when at most kPoleLdsMax
 constexpr int kForceB = 0, kForceE = 1, kForceEB = 2;
-hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, int force, const double* poles,
+hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, int force, const double* poles, int npoles,
                             const BatchParticle* P,
                             int np, BatchTraj const& bt, Checkpoints const& ck, RK4Seg const& sg, hipStream_t s);
+// Pole tables of up to kPoleLdsMax deltas (64 KiB for a block of 64 particles) are read
+// from LDS by the RK4 kernels; longer ones from global memory.
+constexpr int kPoleLdsMax = 128;
+inline int pole_lds_count(int npoles) { return npoles > 0 && npoles <= kPoleLdsMax ? npoles : 0; }
 hipError_t launch_zero_counter(unsigned* c, hipStream_t s);
 // One time segment of the flux: steps [lo, hi) of both RK4 legs of every particle.
 // The six field sums of each (particle, energy, point) accumulate across segments in
@@ -151,11 +157,12 @@ hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, doubl
 hipError_t launch_zero(double* a, int64_t n, hipStream_t s);
 // loss[r] = sum_j (flux_pp[r*ns + j] - target[j])^2
 hipError_t launch_loss(const double* flux_pp, const double* target, int ns, int np, double* loss, hipStream_t s);
-// gy[c*total + point] = dL/dy_c of every trajectory point (y = x, vx, y, vy, z, vz), one energy
+// gy[c*total + point] = dL/dy_c of every trajectory point (y = x, vx, y, vy, z, vz), one energy;
+// mixed: FP64 phase, FP32 amplitudes and products (bt.fbx, fby, fomb staged by the RK4)
 hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
                                int ns, double omega, double cw, double c0w, double c2, const double* field_pp,
                                const double* flux_pp, const double* target, const double* dloss, double* gy,
-                               int64_t total, hipStream_t s);
+                               int64_t total, bool mixed, hipStream_t s);
 // doubles of record scratch for `steps` recomputed RK4 steps of np particles
 size_t adjoint_record_doubles(int64_t steps, int np);
 // doubles of per-segment sweep results for `segments` segments

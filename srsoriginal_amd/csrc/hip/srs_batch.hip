@@ -25,13 +25,16 @@ namespace {
 // point; 0 at the first point), the backward pass fills the nb points before T0.
 template <int NU, Force FORCE>
 __global__ __launch_bounds__(64) void rk4_batch_kernel(const DevUndulator* __restrict__ U, int nu,
-                                                        const double* __restrict__ poles,
+                                                        const double* __restrict__ poles, int npl,
                                                         const BatchParticle* __restrict__ Ps, int np, BatchTraj bt,
                                                         Checkpoints ck, RK4Seg sg) {
+  extern __shared__ double spole[];  // npl x 64 (launch_rk4_batch)
   int const p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   BatchParticle const q = Ps[p];
-  RK4Ctx<NU, FORCE> const R{Fields<NU>(U, nu, poles + q.pole_base), q.qmg};
+  int ps;
+  const double* const pt = stage_poles(poles + q.pole_base, spole, npl, &ps);
+  RK4Ctx<NU, FORCE> const R{Fields<NU>(U, nu, pt, ps), q.qmg};
   double x[6], d[6] = {0, 0, 0, 0, 0, 0};
   auto init = [&]() {
     x[0] = q.x0[0];
@@ -1039,15 +1042,17 @@ hipError_t launch_sample_batch(const DevBeam* beams, const double* wsum, int nb,
   return hipGetLastError();
 }
 
-hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, int force, const double* poles,
+hipError_t launch_rk4_batch(const DevUndulator* U, int nu, bool local, int force, const double* poles, int npoles,
                             const BatchParticle* P, int np, BatchTraj const& bt, Checkpoints const& ck,
                             RK4Seg const& sg, hipStream_t s) {
   dim3 const grid((np + 63) / 64);
-  if (force == kForceE) rk4_batch_kernel<0, Force::E><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
-  else if (force == kForceEB) rk4_batch_kernel<0, Force::EB><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
-  else if (nu == 1 && !local) rk4_batch_kernel<1, Force::B><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
-  else if (nu == 2 && !local) rk4_batch_kernel<2, Force::B><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
-  else rk4_batch_kernel<0, Force::B><<<grid, 64, 0, s>>>(U, nu, poles, P, np, bt, ck, sg);
+  int const npl = pole_lds_count(npoles);
+  size_t const lds = (size_t)npl * kPoleLane * sizeof(double);
+  if (force == kForceE) rk4_batch_kernel<0, Force::E><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, bt, ck, sg);
+  else if (force == kForceEB) rk4_batch_kernel<0, Force::EB><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, bt, ck, sg);
+  else if (nu == 1 && !local) rk4_batch_kernel<1, Force::B><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, bt, ck, sg);
+  else if (nu == 2 && !local) rk4_batch_kernel<2, Force::B><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, bt, ck, sg);
+  else rk4_batch_kernel<0, Force::B><<<grid, 64, lds, s>>>(U, nu, poles, npl, P, np, bt, ck, sg);
   return hipGetLastError();
 }
 

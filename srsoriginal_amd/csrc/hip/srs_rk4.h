@@ -129,9 +129,11 @@ __device__ __forceinline__ void field_add_local(DevUndulator const& f, double X,
 }
 
 // Field of one undulator at (X, Y, Z), added to B.  poles = this particle's pole
-// deltas (used when f.pole_off >= 0).
-__device__ __forceinline__ void field_add_und(DevUndulator const& f, const double* __restrict__ poles, double X,
-                                              double Y, double Z, double* B) {
+// deltas (used when f.pole_off >= 0), entry k at poles[k * ps] (ps = kPoleLane when the
+// kernel staged them in LDS, a column per lane; 1 in global memory).
+constexpr int kPoleLane = 64;
+__device__ __forceinline__ void field_add_und(DevUndulator const& f, const double* __restrict__ poles, int ps,
+                                              double X, double Y, double Z, double* B) {
   double const D = (X - f.center[0]) * f.u[0] + (Y - f.center[1]) * f.u[1] + (Z - f.center[2]) * f.u[2];
   double const T = 1 + D * f.taper;
   bool const inside = !(D > f.hi0 || D < f.lo0);
@@ -143,7 +145,7 @@ __device__ __forceinline__ void field_add_und(DevUndulator const& f, const doubl
   if (f.pole_off >= 0) {  // uniform across the wave
     int p = (int)__builtin_floor(div_rn(D - f.lo1, f.lam / 2., 2.0 * f.rlam));
     p = p < 0 ? 0 : (p > 2 * f.nperiods - 1 ? 2 * f.nperiods - 1 : p);
-    g = 1.0 + poles[f.pole_off + p];
+    g = 1.0 + poles[(f.pole_off + p) * ps];
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -153,12 +155,12 @@ __device__ __forceinline__ void field_add_und(DevUndulator const& f, const doubl
 }
 
 // Field of any element at (X, Y, Z), added to B.
-__device__ __forceinline__ void field_add(DevUndulator const& f, const double* __restrict__ poles, double X, double Y,
-                                          double Z, double* B) {
+__device__ __forceinline__ void field_add(DevUndulator const& f, const double* __restrict__ poles, int ps, double X,
+                                          double Y, double Z, double* B) {
   if (f.kind != 0)
     field_add_local(f, X, Y, Z, B);
   else
-    field_add_und(f, poles, X, Y, Z, B);
+    field_add_und(f, poles, ps, X, Y, Z, B);
 }
 
 // Field, its derivative along the period direction and its derivative with
@@ -174,8 +176,8 @@ struct FieldJet {
   int p;           // that pole, or -1
 };
 
-__device__ __forceinline__ void field_add_jet(DevUndulator const& f, const double* __restrict__ poles, double X,
-                                              double Y, double Z, FieldJet& J) {
+__device__ __forceinline__ void field_add_jet(DevUndulator const& f, const double* __restrict__ poles, int ps,
+                                              double X, double Y, double Z, FieldJet& J) {
   double const D = (X - f.center[0]) * f.u[0] + (Y - f.center[1]) * f.u[1] + (Z - f.center[2]) * f.u[2];
   double const T = 1 + D * f.taper;
   bool const inside = !(D > f.hi0 || D < f.lo0);
@@ -189,7 +191,7 @@ __device__ __forceinline__ void field_add_jet(DevUndulator const& f, const doubl
   if (f.pole_off >= 0) {
     p = (int)__builtin_floor(div_rn(D - f.lo1, f.lam / 2., 2.0 * f.rlam));
     p = p < 0 ? 0 : (p > 2 * f.nperiods - 1 ? 2 * f.nperiods - 1 : p);
-    g = 1.0 + poles[f.pole_off + p];
+    g = 1.0 + poles[(f.pole_off + p) * ps];
   }
   double const dt = term ? fct * (c * k * T + s * f.taper) : c * k * g + s * f.taper;
 #pragma unroll
@@ -210,7 +212,9 @@ template <int NU>
 struct Fields {
   DevUndulator u[NU];
   const double* poles;
-  __device__ __forceinline__ Fields(const DevUndulator* U, int, const double* p) : poles(p) {
+  int ps;  // stride of the pole deltas (field_add_und)
+  __device__ __forceinline__ Fields(const DevUndulator* U, int, const double* p, int stride = 1)
+      : poles(p), ps(stride) {
 #pragma unroll
     for (int k = 0; k < NU; ++k) u[k] = U[k];
   }
@@ -219,12 +223,12 @@ struct Fields {
   __device__ __forceinline__ void eval(double X, double Y, double Z, double* B) const {
     B[0] = B[1] = B[2] = 0.0;
 #pragma unroll
-    for (int k = 0; k < NU; ++k) field_add_und(u[k], poles, X, Y, Z, B);
+    for (int k = 0; k < NU; ++k) field_add_und(u[k], poles, ps, X, Y, Z, B);
   }
   __device__ __forceinline__ void jet(double X, double Y, double Z, FieldJet& J) const {
     J = FieldJet{{0, 0, 0}, {0, 0, 0}, 0.0, -1};
 #pragma unroll
-    for (int k = 0; k < NU; ++k) field_add_jet(u[k], poles, X, Y, Z, J);
+    for (int k = 0; k < NU; ++k) field_add_jet(u[k], poles, ps, X, Y, Z, J);
   }
 };
 template <>
@@ -232,24 +236,41 @@ struct Fields<0> {
   const DevUndulator* U;
   int nu;
   const double* poles;
-  __device__ __forceinline__ Fields(const DevUndulator* U_, int n, const double* p) : U(U_), nu(n), poles(p) {}
+  int ps;
+  __device__ __forceinline__ Fields(const DevUndulator* U_, int n, const double* p, int stride = 1)
+      : U(U_), nu(n), poles(p), ps(stride) {}
   __device__ __forceinline__ DevUndulator const& at(int k) const { return U[k]; }
   __device__ __forceinline__ int count() const { return nu; }
   __device__ __forceinline__ void eval(double X, double Y, double Z, double* B) const {
     B[0] = B[1] = B[2] = 0.0;
-    for (int k = 0; k < nu; ++k) field_add(U[k], poles, X, Y, Z, B);
+    for (int k = 0; k < nu; ++k) field_add(U[k], poles, ps, X, Y, Z, B);
   }
   // magnetic and electric sums, each element into the field of its target
   __device__ __forceinline__ void eval_eb(double X, double Y, double Z, double* B, double* E) const {
     B[0] = B[1] = B[2] = 0.0;
     E[0] = E[1] = E[2] = 0.0;
-    for (int k = 0; k < nu; ++k) field_add(U[k], poles, X, Y, Z, U[k].target ? E : B);
+    for (int k = 0; k < nu; ++k) field_add(U[k], poles, ps, X, Y, Z, U[k].target ? E : B);
   }
   __device__ __forceinline__ void jet(double X, double Y, double Z, FieldJet& J) const {
     J = FieldJet{{0, 0, 0}, {0, 0, 0}, 0.0, -1};
-    for (int k = 0; k < nu; ++k) field_add_jet(U[k], poles, X, Y, Z, J);
+    for (int k = 0; k < nu; ++k) field_add_jet(U[k], poles, ps, X, Y, Z, J);
   }
 };
+
+// The pole deltas of one lane's particle, npl of them from global memory at g, copied
+// into the lane's column of the block's LDS table (kPoleLane lanes a row) when npl > 0:
+// the field then reads them at LDS latency, not at the global memory latency that
+// every RK4 stage would otherwise wait for.  Returns the table to read and its stride.
+__device__ __forceinline__ const double* stage_poles(const double* g, double* lds, int npl, int* ps) {
+  if (npl <= 0) {
+    *ps = 1;
+    return g;
+  }
+  int const lane = threadIdx.x % kPoleLane;
+  for (int k = 0; k < npl; ++k) lds[k * kPoleLane + lane] = g[k];
+  *ps = kPoleLane;
+  return lds + lane;
+}
 
 // Derivative form, as OSCARSSR::SetDerivativesFunction picks it (OSCARSSR.cc:690-703):
 // magnetic elements only (DerivativesB), electric only (DerivativesE) or both

@@ -33,10 +33,11 @@ class PoleProblem:
     poled: index of the undulator whose poles carry the errors; particle: dict with
     x0, beta0, qmg, t0, ct_start, ct_stop, npoints (the srs_particle fields);
     quantity: "flux" (at energy_ev) or "power_density" (normals [ns, 3], directional
-    as calculate_power_density's normal handling)."""
+    as calculate_power_density's normal handling); precision: "fp64", or "mixed" for the
+    flux (mixed flux terms forward, mixed adjoint: FP64 phase, FP32 amplitudes)."""
 
     def __init__(self, undulators, poled, particle, obs_xyz, energy_ev, charge, current, target=None,
-                 checkpoint_every=1000, quantity="flux", normals=None, directional=False):
+                 checkpoint_every=1000, quantity="flux", normals=None, directional=False, precision="fp64"):
         self.undulators = [dict(u) for u in undulators]
         for u in self.undulators:
             u.pop("pole_delta", None)
@@ -47,6 +48,11 @@ class PoleProblem:
         if quantity not in ("flux", "power_density"):
             raise ValueError("quantity must be 'flux' or 'power_density'")
         self.quantity = quantity
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision == "mixed" and quantity != "flux":
+            raise ValueError("the mixed-precision gradient is for the flux loss")
+        self.precision = precision
         self.energy_ev = float(energy_ev) if energy_ev is not None else None
         if quantity == "flux" and self.energy_ev is None:
             raise ValueError("the flux loss needs energy_ev")
@@ -78,7 +84,8 @@ class PoleProblem:
                                        self.obs_xyz, self.normals, self.target, self.directional,
                                        self.checkpoint_every, want_flux)
         return abi.pole_forward(self.undulators, self.poled, self._particle(), d, self.charge, self.current,
-                                self.obs_xyz, self.energy_ev, self.target, self.checkpoint_every, want_flux)
+                                self.obs_xyz, self.energy_ev, self.target, self.checkpoint_every, want_flux,
+                                abi.MIXED if self.precision == "mixed" else abi.FP64)
 
     def flux(self, deltas):
         """Maps [nreal, ns] of the realisations (flux, or power density for that quantity)."""

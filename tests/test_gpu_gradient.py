@@ -109,6 +109,60 @@ def test_pole_gradient_central_differences_config4_shape(grad, oracle):
         assert np.all(err <= 1e-3)
 
 
+# The mixed gradient (mixed flux terms forward, mixed adjoint: FP64 phase, FP32
+# amplitudes, their derivatives and products) is held to the mixed forward's bound
+# against the FP64 result.
+TOL_MIXED_GRAD = 1e-5
+
+
+def test_mixed_pole_gradient_against_oracle_and_fp64(grad, oracle):
+    say("mixed-precision pole-error gradient (mixed flux forward, mixed adjoint) vs oracle dual numbers "
+        "and the FP64 GPU gradient (5x5, 2001 points, 3 realisations)")
+    prob, p, pole, xyz, (c0, c1, n), g = _fixture_problem(grad, oracle)
+    D = _deltas(g, 3)
+    l64, f64, tape = prob.forward(D, want_flux=True)
+    G64 = prob.backward(tape, len(D))
+    prob.precision = "mixed"
+    lmx, fmx, tape = prob.forward(D, want_flux=True)
+    Gmx = prob.backward(tape, len(D))
+    ef = np.max(np.abs(fmx - f64)) / np.max(np.abs(f64))
+    say("  flux mixed vs FP64 %.3e" % ef)
+    assert ef <= 1e-5
+    for r in range(len(D)):
+        F = oracle.FieldSet([], pole, D[r])
+        lo, go = oracle.pole_loss_gradient(F, p, c0, c1, n, xyz, prob.energy_ev, prob.target, nthreads=8)
+        el = abs(lmx[r] - lo) / lo
+        eg = np.max(np.abs(Gmx[r] - go)) / np.max(np.abs(go))
+        e64 = np.max(np.abs(Gmx[r] - G64[r])) / np.max(np.abs(G64[r]))
+        say("  realisation %d: loss rel %.3e  gradient normwise vs oracle %.3e, vs FP64 GPU %.3e" % (r, el, eg, e64))
+        assert el <= TOL_MIXED_GRAD and eg <= TOL_MIXED_GRAD and e64 <= TOL_MIXED_GRAD
+
+
+def test_mixed_pole_gradient_config4_shape(grad, oracle):
+    say("mixed-precision pole-error gradient vs the FP64 one, 21x21 over 5 mm at 30 m, 20001 points, 2 realisations")
+    name = "poles_flux_and_fd_gradient"
+    a = cases.args(name)
+    b = cases.beam(a)
+    p = oracle.ideal_electron(b["energy"], b["x0"], b["d0"], b["current"])
+    xyz, _, _ = oracle.rect_surface("XY", 21, 21, 0.005, 0.005, (0, 0, 0), (0, 0, 30), 0)
+    row = dict(x0=list(p.x0), beta0=list(p.beta0), qmg=p.qmg, t0=0.0, ct_start=0.0, ct_stop=3.0, npoints=20001)
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    prob = grad.PoleProblem(und, 0, row, xyz, 143.52176088044024, p.q, p.current)
+    prob.target = prob.flux(np.zeros((1, 82)))[0]
+    D = np.random.default_rng(7).normal(0, 0.005, (2, 82))
+    l64, _, tape = prob.forward(D)
+    G64 = prob.backward(tape, 2)
+    prob.precision = "mixed"
+    lmx, _, tape = prob.forward(D)
+    Gmx = prob.backward(tape, 2)
+    for r in range(2):
+        el = abs(lmx[r] - l64[r]) / l64[r]
+        eg = np.max(np.abs(Gmx[r] - G64[r])) / np.max(np.abs(G64[r]))
+        cs = np.dot(Gmx[r], G64[r]) / (np.linalg.norm(Gmx[r]) * np.linalg.norm(G64[r]))
+        say("  realisation %d: loss rel %.3e  gradient normwise %.3e  cosine %.12f" % (r, el, eg, cs))
+        assert el <= TOL_MIXED_GRAD and eg <= TOL_MIXED_GRAD
+
+
 def test_autograd_function_and_tape(grad, oracle):
     import torch
     from srsoriginal_amd import abi
