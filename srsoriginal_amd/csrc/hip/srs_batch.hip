@@ -134,10 +134,8 @@ struct ItemRanges {
   }
 };
 
-// Three energies per item hold 3 waves per SIMD (at most 168 VGPRs), as they need.
 template <int NE, int TILE_, bool MIXED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NE == 3 ? 3 : 1))) void batch_flux_kernel(
-    BatchTraj bt, const BatchParticle* __restrict__ P,
+__global__ __launch_bounds__(kBlock) void batch_flux_kernel(BatchTraj bt, const BatchParticle* __restrict__ P,
                                                             const double* __restrict__ obs, int ns, BatchEnergies be,
                                                             unsigned nitems, unsigned* __restrict__ counter,
                                                             double* __restrict__ flux_pp, FluxSeg sg) {
