@@ -61,7 +61,7 @@ def _pipeline_frac(flop_per_term, terms, fp64_ms, mixed_ms):
 # HBM bytes per launch of flux_mixed2_kernel<256, 256> on this workload, from rocprofv3
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs of `bench.py --profile`, the factors being
 # what the same counters read for a 256 MiB device copy (profiles/flux_c1_201x201_pmc.md).
-TRAFFIC_MIXED_BYTES = 61648.6 * 1024
+TRAFFIC_MIXED_BYTES = 61667.3 * 1024
 
 
 def log(msg):
