@@ -163,6 +163,30 @@ def test_mixed_pole_gradient_config4_shape(grad, oracle):
         assert el <= TOL_MIXED_GRAD and eg <= TOL_MIXED_GRAD
 
 
+def test_mixed_pole_gradient_beam_along_x_takes_the_fp64_terms(grad, oracle):
+    """A realisation whose beam does not travel along z is flagged by the device RK4; the
+    mixed forward then sums its FP64 terms and the mixed adjoint takes its FP64 terms, so
+    the mixed gradient equals the FP64 one (undulator and beam along x, screen at x = 30 m)."""
+    say("mixed pole-error gradient, beam along x: the FP64 terms on both passes")
+    p = oracle.ideal_electron(3.0, (-1.5, 0.0, 0.0), (1.0, 0.0, 0.0), 0.5)
+    xyz, _, _ = oracle.rect_surface("YZ", 7, 7, 0.004, 0.004, (0, 0, 0), (30, 0, 0), 0)
+    row = dict(x0=list(p.x0), beta0=list(p.beta0), qmg=p.qmg, t0=0.0, ct_start=0.0, ct_stop=3.0, npoints=2001)
+    und = [dict(field=(0, 1, 0), period=(0.05, 0, 0), nperiods=41)]
+    prob = grad.PoleProblem(und, 0, row, xyz, 143.52176088044024, p.q, p.current)
+    prob.target = prob.flux(np.zeros((1, 82)))[0]
+    D = np.random.default_rng(5).normal(0, 0.005, (2, 82))
+    l64, f64, tape = prob.forward(D, want_flux=True)
+    G64 = prob.backward(tape, 2)
+    prob.precision = "mixed"
+    lmx, fmx, tape = prob.forward(D, want_flux=True)
+    Gmx = prob.backward(tape, 2)
+    ef = np.max(np.abs(fmx - f64)) / np.max(np.abs(f64))
+    eg = np.max(np.abs(Gmx - G64)) / np.max(np.abs(G64))
+    say("  flux %.3e  loss %.3e  gradient %.3e" % (ef, np.max(np.abs(lmx - l64) / l64), eg))
+    assert np.max(np.abs(G64)) > 0
+    assert ef <= 1e-13 and eg <= 1e-12
+
+
 def test_autograd_function_and_tape(grad, oracle):
     import torch
     from srsoriginal_amd import abi
