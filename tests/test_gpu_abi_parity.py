@@ -2,7 +2,7 @@
 against the reference's own FP64 CPU results (golden fixtures written by the
 compiled reference) and the CPU oracle.
 
-Tolerances (north star): plain FP64 form <= 1e-10 normwise relative
+Tolerances: plain FP64 form <= 1e-10 normwise relative
 (max|d|/max|ref|); mixed form <= 1e-5 normwise and RMS relative vs FP64.
 """
 import numpy as np
