@@ -119,9 +119,15 @@ __device__ __forceinline__ void flux_adjoint_fp64_block(BatchTraj const& bt, Bat
     __syncthreads();
     for (int t = 0; t < cnt; ++t) {
       double const rr[3] = {sx[t] - X, sy[t] - Y, sz[t] - Z};
-      double const D = __builtin_sqrt(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
+      // D and tau bitwise as the forward (correctly rounded root and quotient); 1/D only
+      // scales the derivative's amplitudes, so the hardware reciprocal refined by two
+      // Newton steps (within an ulp) serves in place of a full division
+      double const D = dist_rn(rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2]);
       double const tau = t0 + div_rn(D, kC, kInvC);
-      double const inv = 1.0 / D, inv2 = inv * inv;
+      double inv = __builtin_amdgcn_rcp(D);
+      inv = __builtin_fma(inv, __builtin_fma(-D, inv, 1.0), inv);
+      inv = __builtin_fma(inv, __builtin_fma(-D, inv, 1.0), inv);
+      double const inv2 = inv * inv;
       double sn, cs;
       sincos_fp64(omega * tau, &sn, &cs);
       double nn[3], hr[3], hi[3], ar[3], ai[3];
