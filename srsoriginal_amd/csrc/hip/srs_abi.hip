@@ -113,6 +113,7 @@ struct Context {
     double omega = 0, cw = 0, c0w = 0, c2 = 0;  // flux
     int quantity = 0;                             // 0 flux, 1 power density
     bool mixed = false;                           // flux: mixed-precision forward and adjoint
+    srs::AdjScreen screen{};                      // flux: the points' plane and extent
     int directional = 0;
     double pd_scale = 0;
   } tape;
@@ -1679,6 +1680,18 @@ int srs_hip_pole_forward_ex(const srs_field* und, int nund, int poled, const srs
   Context::Tape& t = pole_tape(c, pr, (int)nreal, (int)ns, nund, poled);
   t.quantity = 0;
   t.mixed = precision == SRS_PRECISION_MIXED;
+  {
+    srs::AdjScreen sc{1, obs_xyz[2], obs_xyz[0], obs_xyz[0], obs_xyz[1], obs_xyz[1]};
+    for (int64_t j = 0; j < ns; ++j) {
+      double const* o = obs_xyz + 3 * j;
+      sc.flat = sc.flat && o[2] == sc.z;
+      sc.xlo = std::min(sc.xlo, o[0]);
+      sc.xhi = std::max(sc.xhi, o[0]);
+      sc.ylo = std::min(sc.ylo, o[1]);
+      sc.yhi = std::max(sc.yhi, o[1]);
+    }
+    t.screen = sc;
+  }
   t.omega = w;
   t.cw = kC / w;
   t.c0w = c0 * w;
@@ -1758,7 +1771,7 @@ int srs_hip_pole_backward(int64_t tape, const double* dloss, double* grad_out) {
     else
       HIPCHK(srs::launch_flux_adjoint(t.bt, Pd, t.np, t.nmax, c.b_obs.as<double>(), t.ns, t.omega, t.cw, t.c0w, t.c2,
                                       c.g_field.as<double>(), c.b_flux_pp.as<double>(), c.g_target.as<double>(),
-                                      c.g_dloss.as<double>(), gy, t.total, t.mixed, s));
+                                      c.g_dloss.as<double>(), gy, t.total, t.mixed, t.screen, s));
   }
   HIPCHK(srs::launch_zero(c.g_grad.as<double>(), (int64_t)t.np * t.npoles, s));
   {

@@ -193,7 +193,8 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_mixed_kernel(BatchTraj bt
                                                                     const double* __restrict__ flux_pp,
                                                                     const double* __restrict__ target,
                                                                     const double* __restrict__ dloss,
-                                                                    double* __restrict__ gy, int64_t total) {
+                                                                    double* __restrict__ gy, int64_t total,
+                                                                    AdjScreen sc) {
   __shared__ double sx[kAdjTile], sy[kAdjTile], sz[kAdjTile];
   __shared__ float sgr[3][kAdjTile], sgi[3][kAdjTile];
   __shared__ double smax[kBlock / 64];
@@ -216,6 +217,32 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_mixed_kernel(BatchTraj bt
   double const kw = c0w * q.dt;
   double const gsc = dloss[r] * 4.0 * c2 * kw * kw;
   float const woc = (float)(omega * kInvC), cwf = (float)cw;
+  // The paraxial phase of the mixed flux terms, on a screen in one plane z = sc.z when
+  // every (point, trajectory point of this block) pair meets their bounds (the same
+  // as para_point_ok's, with the screen's extent): phase = frac(f (t_k - z_k/c) +
+  // omega Z/(2 pi c)) + f u, u = s/(2 r_z) - s^2/(8 r_z^3), in revolutions, without the
+  // root and quotients of the general form; D ~ r_z + s/(2 r_z) for the FP32 amplitudes.
+  double const fr = omega / (2.0 * kPi), fw = fr * kInvC;
+  double const rzd = sc.z - Z;
+  bool ok = false;
+  if (sc.flat) {
+    double const dx = __builtin_fmax(__builtin_fabs(sc.xlo - X), __builtin_fabs(sc.xhi - X));
+    double const dy = __builtin_fmax(__builtin_fabs(sc.ylo - Y), __builtin_fabs(sc.yhi - Y));
+    double const eps = __builtin_fma(dx, dx, dy * dy) / (rzd * rzd);
+    ok = rzd > 0.0 && eps <= kParaEps && fw * rzd * (eps * eps * eps) <= 16.0 * kParaPhase;
+  }
+  bool const para = __syncthreads_and(ok) != 0;
+  double ph_h = 0.0, ph_t3 = 0.0, ph0 = 0.0;
+  float rzf = 1.f, hzf = 0.f;
+  if (para) {
+    double const ir = 1.0 / rzd;
+    ph_h = fw * (0.5 * ir);
+    ph_t3 = fw * (0.125 * ir * ir * ir);
+    double const p0 = para_point_phase(fr, t0, Z) + para_plane_phase(omega, sc.z);
+    ph0 = p0 - __builtin_rint(p0);
+    rzf = (float)rzd;
+    hzf = (float)(0.5 * ir);
+  }
   double gR[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
   for (int j0 = 0; j0 < ns; j0 += kAdjTile) {
     int const cnt = (ns - j0) < kAdjTile ? ns - j0 : kAdjTile;
@@ -252,17 +279,29 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_mixed_kernel(BatchTraj bt
       }
     }
     __syncthreads();
-    for (int t0s = 0; t0s < cnt; t0s += kSub) {
-      int const t1s = (t0s + kSub < cnt) ? t0s + kSub : cnt;
-      float fR[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
+    // the terms of points [t0s, t1s) of the tile: paraxial (PARA) or general phase
+    auto terms = [&](auto PARA, int t0s, int t1s, float* fR, float* fb) {
       for (int t = t0s; t < t1s; ++t) {
-        double const rx = sx[t] - X, ry = sy[t] - Y, rz = sz[t] - Z;
-        double const D = dist_rn(rx * rx + ry * ry + rz * rz);
-        double const tau = t0 + div_rn(D, kC, kInvC);
-        float sn, cs;
-        sincos_reduced_f32(reduce_two_pi(omega * tau), &sn, &cs);
-        float const inv = __builtin_amdgcn_rcpf((float)D), inv2 = inv * inv;
-        float const nx = (float)rx * inv, ny = (float)ry * inv, nz = (float)rz * inv;
+        double const rx = sx[t] - X, ry = sy[t] - Y;
+        float sn, cs, inv, nz;
+        if constexpr (decltype(PARA)::value) {
+          double const sq = __builtin_fma(rx, rx, ry * ry);
+          double const hs = __builtin_fma(-ph_t3, sq, ph_h);
+          float const ph = (float)__builtin_amdgcn_fract(__builtin_fma(sq, hs, ph0));
+          sn = __builtin_amdgcn_sinf(ph);
+          cs = __builtin_amdgcn_cosf(ph);
+          inv = __builtin_amdgcn_rcpf(__builtin_fmaf((float)sq, hzf, rzf));
+          nz = rzf * inv;
+        } else {
+          double const rz = sz[t] - Z;
+          double const D = dist_rn(rx * rx + ry * ry + rz * rz);
+          double const tau = t0 + div_rn(D, kC, kInvC);
+          sincos_reduced_f32(reduce_two_pi(omega * tau), &sn, &cs);
+          inv = __builtin_amdgcn_rcpf((float)D);
+          nz = (float)rz * inv;
+        }
+        float const inv2 = inv * inv;
+        float const nx = (float)rx * inv, ny = (float)ry * inv;
         float const v = __builtin_fmaf(nx, nx, ny * ny);
         float const omnz = nz > 0.f ? v * __builtin_amdgcn_rcpf(1.f + nz) : 1.f - nz;  // 1 - n_z
         float const nn[3] = {nx, ny, nz};
@@ -287,6 +326,12 @@ __global__ __launch_bounds__(kBlock) void flux_adjoint_mixed_kernel(BatchTraj bt
           fR[d] += t1 + t2 - woc * nn[d] * S3;
         }
       }
+    };
+    for (int t0s = 0; t0s < cnt; t0s += kSub) {
+      int const t1s = (t0s + kSub < cnt) ? t0s + kSub : cnt;
+      float fR[3] = {0.f, 0.f, 0.f}, fb[3] = {0.f, 0.f, 0.f};
+      if (para) terms(std::true_type{}, t0s, t1s, fR, fb);
+      else terms(std::false_type{}, t0s, t1s, fR, fb);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         gR[c] += ldexp((double)fR[c], e);
@@ -633,11 +678,11 @@ hipError_t launch_loss(const double* flux_pp, const double* target, int ns, int 
 hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
                                int ns, double omega, double cw, double c0w, double c2, const double* field_pp,
                                const double* flux_pp, const double* target, const double* dloss, double* gy,
-                               int64_t total, bool mixed, hipStream_t s) {
+                               int64_t total, bool mixed, AdjScreen const& sc, hipStream_t s) {
   dim3 const grid((unsigned)((nmax + kBlock - 1) / kBlock), (unsigned)np);
   if (mixed)
     flux_adjoint_mixed_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target,
-                                                      dloss, gy, total);
+                                                      dloss, gy, total, sc);
   else
     flux_adjoint_kernel<<<grid, kBlock, 0, s>>>(bt, P, obs, ns, omega, cw, c0w, c2, field_pp, flux_pp, target, dloss,
                                                 gy, total);

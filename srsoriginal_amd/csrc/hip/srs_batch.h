@@ -157,12 +157,19 @@ hipError_t launch_particle_reduce(const double* flux_pp, int np, int nout, doubl
 hipError_t launch_zero(double* a, int64_t n, hipStream_t s);
 // loss[r] = sum_j (flux_pp[r*ns + j] - target[j])^2
 hipError_t launch_loss(const double* flux_pp, const double* target, int ns, int np, double* loss, hipStream_t s);
+// The observation screen of a flux gradient: flat when every point has z == z, with the
+// points' x and y ranges (the mixed adjoint's paraxial bounds).
+struct AdjScreen {
+  int flat;
+  double z, xlo, xhi, ylo, yhi;
+};
 // gy[c*total + point] = dL/dy_c of every trajectory point (y = x, vx, y, vy, z, vz), one energy;
-// mixed: FP64 phase, FP32 amplitudes and products (bt.fbx, fby, fomb staged by the RK4)
+// mixed: FP64 phase (paraxial on a flat screen sc within the bounds), FP32 amplitudes
+// and products (bt.fbx, fby, fomb staged by the RK4)
 hipError_t launch_flux_adjoint(BatchTraj const& bt, const BatchParticle* P, int np, int64_t nmax, const double* obs,
                                int ns, double omega, double cw, double c0w, double c2, const double* field_pp,
                                const double* flux_pp, const double* target, const double* dloss, double* gy,
-                               int64_t total, bool mixed, hipStream_t s);
+                               int64_t total, bool mixed, AdjScreen const& sc, hipStream_t s);
 // doubles of record scratch for `steps` recomputed RK4 steps of np particles
 size_t adjoint_record_doubles(int64_t steps, int np);
 // doubles of per-segment sweep results for `segments` segments

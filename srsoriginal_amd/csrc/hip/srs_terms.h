@@ -187,11 +187,14 @@ constexpr double kParaNear = 1e-9;
 // hardware (revolutions, ~1e-7 absolute: the sums' own FP32 level), scaled in FP64 to
 // unit modulus so that |E|^2 is unchanged.  *c, *s = cos, sin of 2 pi psi.
 constexpr double kInv2PiCHi = 0x1.23db3eb8a3fa7p-31, kInv2PiCLo = -0x1.79488f78d104bp-86;  // 1/(2 pi c)
-__device__ __forceinline__ void para_plane_turn(double omega, double Z, double* c, double* s) {
+__device__ __forceinline__ double para_plane_phase(double omega, double Z) {
   double const ah = omega * Z, al = __builtin_fma(omega, Z, -ah);
   double const yh = ah * kInv2PiCHi;
   double const yl = __builtin_fma(ah, kInv2PiCHi, -yh) + (ah * kInv2PiCLo + al * kInv2PiCHi);
-  float const psi = (float)((yh - __builtin_rint(yh)) + yl);
+  return (yh - __builtin_rint(yh)) + yl;
+}
+__device__ __forceinline__ void para_plane_turn(double omega, double Z, double* c, double* s) {
+  float const psi = (float)para_plane_phase(omega, Z);
   double const cf = (double)__builtin_amdgcn_cosf(psi), sf = (double)__builtin_amdgcn_sinf(psi);
   double const r = 1.0 / __builtin_sqrt(__builtin_fma(cf, cf, sf * sf));
   *c = cf * r;
@@ -203,6 +206,20 @@ template <class V>
 __device__ __forceinline__ double turned(V const& v, int q6, double c, double s) {
   double const re = v(q6 & ~1), im = v(q6 | 1);
   return (q6 & 1) ? __builtin_fma(re, s, im * c) : __builtin_fma(re, c, -(im * s));
+}
+
+// frac(f (t_k - z_k/c)) in revolutions, t_k = RN(dt*ti) as the reference forms it
+// (OSCARSSR.cc:2072): v = t_k - z_k/c as an unevaluated sum vh + vl (two-sum, minus
+// the error of the quotient z_k/c), then f v with the product's error.
+__device__ __forceinline__ double para_point_phase(double f, double t, double zk) {
+  double const zc = zk * kInvC;
+  double const ez = __builtin_fma(-zc, kC, zk) * kInvC;  // z_k/c - zc
+  double const vh = t - zc;
+  double const tv = vh - t;
+  double const vl = ((t - (vh - tv)) + (-zc - tv)) - ez;
+  double const p = f * vh;
+  double const pl = __builtin_fma(f, vh, -p) + f * vl;
+  return (p - __builtin_rint(p)) + pl;
 }
 
 // One staged trajectory point of the paraxial form (one LDS record, so a term
@@ -336,20 +353,8 @@ __device__ __forceinline__ void para_stage(ParaTile<TILE, NE>& P, int k, double 
   r.ombr = fomb * irf;
   r.q = (float)q;
   r.gh = (float)(0.5 * q * ir);
-  // v = t_k - z_k/c as an unevaluated sum vh + vl, with t_k = RN(dt*ti) as the
-  // reference forms it (OSCARSSR.cc:2072)
-  double const t = dt * (double)ti;
-  double const zc = zk * kInvC;
-  double const ez = __builtin_fma(-zc, kC, zk) * kInvC;  // z_k/c - zc
-  double const vh = t - zc;
-  double const tv = vh - t;
-  double const vl = ((t - (vh - tv)) + (-zc - tv)) - ez;  // two-sum error, minus the quotient's error
 #pragma unroll
-  for (int e = 0; e < NE; ++e) {
-    double const p = es.fr[e] * vh;
-    double const pl = __builtin_fma(es.fr[e], vh, -p) + es.fr[e] * vl;
-    r.phi[e] = (p - __builtin_rint(p)) + pl;
-  }
+  for (int e = 0; e < NE; ++e) r.phi[e] = para_point_phase(es.fr[e], dt * (double)ti, zk);
 }
 
 // A padding record past the tile's last point: zero amplitudes, so its terms add
