@@ -644,12 +644,28 @@ struct BatchRun {
 
 // Time segments of the pipeline: one for short trajectories, else up to kMaxSegments
 // of >= 2048 steps, so the device RK4 (latency-bound, a few waves) of segment s + 1
-// runs on stream2 while the flux integrates segment s on the main stream.
-int64_t segment_steps(int64_t max_leg, int* nseg) {
-  int n = max_leg <= 4096 ? 1 : (int)std::min<int64_t>(Context::kMaxSegments, (max_leg + 2047) / 2048);
+// runs on stream2 while the integral of segment s runs on the main stream.  The first
+// segment is a quarter of the others (at least 512 steps): the integral waits for it
+// alone, while every later RK4 segment runs beside the integral of the one before.
+struct Segments {
+  int n = 1;
+  int64_t first = 0, len = 0;  // steps of segment 0 and of each later one
+  int64_t lo(int k) const { return k == 0 ? 0 : first + (int64_t)(k - 1) * len; }
+  int64_t hi(int k) const { return first + (int64_t)k * len; }
+  int64_t longest() const { return std::max(first, len); }
+};
+Segments segments_of(int64_t max_leg) {
+  Segments g;
+  int const n = max_leg <= 4096 ? 1 : (int)std::min<int64_t>(Context::kMaxSegments, (max_leg + 2047) / 2048);
+  if (n == 1) {
+    g.first = g.len = max_leg;
+    return g;
+  }
   int64_t const L = (max_leg + n - 1) / n;
-  *nseg = (int)((max_leg + L - 1) / L);
-  return L;
+  g.n = n;
+  g.first = std::max<int64_t>(512, L / 4);
+  g.len = (max_leg - g.first + n - 2) / (n - 1);  // hi(n - 1) >= max_leg
+  return g;
 }
 
 // Queue the whole pipeline: RK4 segments, persistent flux per segment, particle reduction.
@@ -659,8 +675,8 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
   int const ntiles = (r.ns + tile_pts - 1) / tile_pts;
   int const ngroups = (r.ne + cfg.ne_per_item - 1) / cfg.ne_per_item;
   unsigned const nitems = (unsigned)((int64_t)np_run * ntiles * ngroups);
-  int nseg = 1;
-  int64_t const L = segment_steps(r.max_leg, &nseg);
+  Segments const g = segments_of(r.max_leg);
+  int const nseg = g.n;
   // leave room for the RK4 blocks next to the persistent grid
   int const reserve = nseg > 1 ? (np_run + 63) / 64 + 1 : 0;
   int nblocks = std::max(1, c.cus * srs::batch_max_blocks(cfg, r.mixed) - reserve);
@@ -674,20 +690,20 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
       field = c.b_field.as<double>();
     }
   }
-  srs::RK4Seg rs{0, L, nseg > 1 ? c.b_state.as<double>() : nullptr};
+  srs::RK4Seg rs{0, g.hi(0), nseg > 1 ? c.b_state.as<double>() : nullptr};
   if (nseg == 1) {
     HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), c.pole_count, P, np_run, r.bt,
                                  r.ck, rs, s));
     HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
-    srs::FluxSeg const fs{0, L, 1, 1, field != nullptr, field};
+    srs::FluxSeg const fs{0, g.hi(0), 1, 1, field != nullptr, field};
     HIPCHK(srs::launch_batch_flux(cfg, r.mixed, r.bt, P, c.b_obs.as<double>(), r.ns, r.be, nitems,
                                   c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), fs, nblocks, s));
   } else {
     HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
     HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
     for (int k = 0; k < nseg; ++k) {
-      rs.lo = k * L;
-      rs.hi = (k + 1) * L;
+      rs.lo = g.lo(k);
+      rs.hi = g.hi(k);
       HIPCHK(srs::launch_rk4_batch(c.b_und.as<srs::DevUndulator>(), r.nund, c.und_local, c.und_force, c.b_poles.as<double>(), c.pole_count, P, np_run, r.bt,
                                    r.ck, rs, c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
@@ -695,7 +711,7 @@ int enqueue_batch(Context& c, BatchRun const& r, srs::BatchConfig const& cfg, in
     for (int k = 0; k < nseg; ++k) {
       HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
       HIPCHK(srs::launch_zero_counter(c.b_counter.as<unsigned>(), s));
-      srs::FluxSeg const fs{k * L, (k + 1) * L, k == 0, k == nseg - 1, 1, field};
+      srs::FluxSeg const fs{g.lo(k), g.hi(k), k == 0, k == nseg - 1, 1, field};
       HIPCHK(srs::launch_batch_flux(cfg, r.mixed, r.bt, P, c.b_obs.as<double>(), r.ns, r.be, nitems,
                                     c.b_counter.as<unsigned>(), c.b_flux_pp.as<double>(), fs, nblocks, s));
     }
@@ -731,9 +747,9 @@ std::string batch_key(BatchRun const& r, srs::BatchConfig const& cfg, Context co
 int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int64_t max_leg, int64_t max_pts,
                      srs::Checkpoints const& ck, int ns, int dir, double* raw, hipStream_t s,
                      bool mixed = false) {
-  int nseg = 1;
-  int64_t const L = segment_steps(max_leg, &nseg);
-  int const nch = srs::pd_batch_chunks(nseg > 1 ? 2 * L : max_pts, ns, np);
+  Segments const g = segments_of(max_leg);
+  int const nseg = g.n;
+  int const nch = srs::pd_batch_chunks(nseg > 1 ? 2 * g.longest() : max_pts, ns, np);
   HIPCHK(c.g_part.ensure((size_t)np * nch * ns * sizeof(double)));
   if (nseg > 1) HIPCHK(c.b_state.ensure((size_t)np * 24 * sizeof(double)));
   auto* Pd = c.b_part.as<srs::BatchParticle>();
@@ -747,13 +763,13 @@ int enqueue_pd_batch(Context& c, srs::BatchTraj const& bt, int np, int nund, int
     HIPCHK(hipEventRecord(c.seg_ev[Context::kMaxSegments], s));  // fork: the RK4 follows the staging on s
     HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
     for (int k = 0; k < nseg; ++k) {
-      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, c.pole_count, Pd, np, bt, ck, srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()},
-                                   c.stream2));
+      HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, poles, c.pole_count, Pd, np, bt, ck,
+                                   srs::RK4Seg{g.lo(k), g.hi(k), c.b_state.as<double>()}, c.stream2));
       HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
     }
     for (int k = 0; k < nseg; ++k) {
       HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
-      HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, k * L, (k + 1) * L,
+      HIPCHK(srs::launch_pd_batch(bt, Pd, np, c.b_obs.as<double>(), c.g_nrm.as<double>(), ns, dir, g.lo(k), g.hi(k),
                                   nch, k > 0, c.g_part.as<double>(), mixed, s));
     }
   }
@@ -1529,12 +1545,12 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
   int const run = uniform ? srs::kSpecRun : 1;
   srs::BatchTraj bt;
   TRY(stage_batch_traj(c, total, np, false, &bt));
-  int nseg = 1;
-  int64_t const L = segment_steps(max_leg, &nseg);
+  Segments const g = segments_of(max_leg);
+  int const nseg = g.n;
   // chunks per segment: at least ~4 blocks (energy group x particle x chunk) per CU
   if (!c.cus) HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, 0));
   int64_t const tiles = (int64_t)srs::spectrum_batch_groups((int)ne, run) * np;
-  int64_t const pts = nseg > 1 ? 2 * L : max_pts;
+  int64_t const pts = nseg > 1 ? 2 * g.longest() : max_pts;
   int const nch = (int)std::max<int64_t>(1, std::min<int64_t>((4 * c.cus + tiles - 1) / tiles, (pts + 255) / 256));
   HIPCHK(c.g_part.ensure((size_t)np * nch * 6 * ne * sizeof(double)));
   HIPCHK(c.b_flux_pp.ensure((size_t)np * ne * sizeof(double)));
@@ -1557,12 +1573,12 @@ int srs_hip_batch_spectrum(const srs_field* und, int nund, const srs_particle* p
       HIPCHK(hipStreamWaitEvent(c.stream2, c.seg_ev[Context::kMaxSegments], 0));
       for (int k = 0; k < nseg; ++k) {
         HIPCHK(srs::launch_rk4_batch(U, nund, c.und_local, c.und_force, c.b_poles.as<double>(), c.pole_count, Pd, (int)np, bt, ck,
-                                     srs::RK4Seg{k * L, (k + 1) * L, c.b_state.as<double>()}, c.stream2));
+                                     srs::RK4Seg{g.lo(k), g.hi(k), c.b_state.as<double>()}, c.stream2));
         HIPCHK(hipEventRecord(c.seg_ev[k], c.stream2));
       }
       for (int k = 0; k < nseg; ++k) {
         HIPCHK(hipStreamWaitEvent(s, c.seg_ev[k], 0));  // the last wait also joins stream2
-        HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, tab, (int)ne, run, k * L, (k + 1) * L, nch, k > 0,
+        HIPCHK(srs::launch_spectrum_batch(bt, Pd, (int)np, obs, tab, (int)ne, run, g.lo(k), g.hi(k), nch, k > 0,
                                           c.g_part.as<double>(), c.spec_range, mixed, s));
       }
     }
