@@ -187,6 +187,30 @@ def test_mixed_pole_gradient_tilted_screen_general_phase(grad, oracle):
         assert el <= TOL_MIXED_GRAD and eg <= TOL_MIXED_GRAD
 
 
+def test_mixed_pole_gradient_screen_across_the_paraxial_bound(grad, oracle):
+    """A 42 mm screen at 30 m: adjoint blocks of trajectory points far from the screen
+    pass the paraxial bounds for the screen's whole extent and take the paraxial phase,
+    near ones fail them and take the general phase; against the FP64 gradient."""
+    say("mixed pole-error gradient, 42 mm screen at 30 m (paraxial and general adjoint blocks) vs FP64")
+    p = oracle.ideal_electron(3.0, (0.0, 0.0, -1.5), (0.0, 0.0, 1.0), 0.5)
+    xyz, _, _ = oracle.rect_surface("XY", 9, 9, 0.042, 0.042, (0, 0, 0), (0, 0, 30), 0)
+    row = dict(x0=list(p.x0), beta0=list(p.beta0), qmg=p.qmg, t0=0.0, ct_start=0.0, ct_stop=3.0, npoints=4001)
+    und = [dict(field=(0, 1, 0), period=(0, 0, 0.05), nperiods=41)]
+    prob = grad.PoleProblem(und, 0, row, xyz, 143.52176088044024, p.q, p.current)
+    prob.target = prob.flux(np.zeros((1, 82)))[0]
+    D = np.random.default_rng(13).normal(0, 0.005, (2, 82))
+    l64, _, tape = prob.forward(D)
+    G64 = prob.backward(tape, 2)
+    prob.precision = "mixed"
+    lmx, _, tape = prob.forward(D)
+    Gmx = prob.backward(tape, 2)
+    for r in range(2):
+        el = abs(lmx[r] - l64[r]) / l64[r]
+        eg = np.max(np.abs(Gmx[r] - G64[r])) / np.max(np.abs(G64[r]))
+        say("  realisation %d: loss rel %.3e  gradient normwise %.3e" % (r, el, eg))
+        assert el <= TOL_MIXED_GRAD and eg <= TOL_MIXED_GRAD
+
+
 def test_mixed_pole_gradient_beam_along_x_takes_the_fp64_terms(grad, oracle):
     """A realisation whose beam does not travel along z is flagged by the device RK4; the
     mixed forward then sums its FP64 terms and the mixed adjoint takes its FP64 terms, so
