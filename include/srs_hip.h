@@ -113,8 +113,9 @@ int srs_hip_flux_device(const srs_traj* traj, const double* obs_soa, int64_t ns,
  * instead of buffers the library allocates.  No counterpart in the reference: it is for
  * callers that cannot allocate (code under hipGraph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  A call needing more scratch -- (trajectory chunks
- * x energies per launch (<= 4) x 48 B) per observation point, + 12 B per trajectory
- * point in mixed precision -- is rejected with SRS_ERR_INVALID.  Every call on a GPU
+ * x energies per launch (<= 4) x 48 B in FP64, 24 B in mixed precision) per observation
+ * point, + 12 B per trajectory point in mixed precision -- is rejected with
+ * SRS_ERR_INVALID.  Every call on a GPU
  * shares that GPU's scratch: calls that overlap on one GPU (on different streams) race.
  * Nothing in it has to start at zero (each call writes all it reads).  The library's
  * first call on a device, whichever entry point, also sets it up (streams, the scratch
